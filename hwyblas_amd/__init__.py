@@ -1,0 +1,239 @@
+"""Python (ctypes) binding of libhwyblas_amd, the MI355X Level-3 BLAS.
+
+The product is the C-ABI shared library built from csrc/ (see
+include/hwyblas_amd.h); this module only loads it and passes pointers, for
+tests and benchmarks. It never computes anything itself: when the library is
+missing, loading fails with an error instead of falling back to any other
+implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhwyblas_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "hwyblas_amd.h")
+
+_lib = None
+
+_c = ctypes.c_char
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_u64 = ctypes.c_uint64
+_d = ctypes.c_double
+_f = ctypes.c_float
+_p = ctypes.c_void_p
+_pi = ctypes.POINTER(ctypes.c_int)
+_sz = ctypes.c_size_t
+
+# CBLAS enum values (include/hwyblas_amd.h)
+RowMajor, ColMajor = 101, 102
+NoTrans, Trans, ConjTrans = 111, 112, 113
+Upper, Lower = 121, 122
+NonUnit, Unit = 131, 132
+Left, Right = 141, 142
+
+ERROR_HANDLER = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_int)
+
+
+class LibraryMissing(ImportError):
+    pass
+
+
+def _declare(L):
+    def fn(name, res, args):
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    for pre, s in (("d", _d), ("s", _f)):
+        fn(f"{pre}gemm_", None, [ctypes.c_char_p, ctypes.c_char_p, _pi, _pi, _pi, _p, _p, _pi,
+                                 _p, _pi, _p, _p, _pi])
+        fn(f"{pre}trsm_", None, [ctypes.c_char_p] * 4 + [_pi, _pi, _p, _p, _pi, _p, _pi])
+        fn(f"cblas_{pre}gemm", None, [_i, _i, _i, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i])
+        fn(f"cblas_{pre}trsm", None, [_i, _i, _i, _i, _i, _i, _i, s, _p, _i, _p, _i])
+        fn(f"hwyblas_{pre}gemm_dev", _i, [_c, _c, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i, _p])
+        fn(f"hwyblas_{pre}trsm_dev", _i, [_c, _c, _c, _c, _i, _i, s, _p, _i, _p, _i, _p])
+    fn("xerbla_", None, [ctypes.c_char_p, _pi, _sz])
+    fn("hwyblas_set_error_handler", _p, [_p])
+    fn("hwyblas_amd_device_count", _i, [])
+    fn("hwyblas_amd_malloc", _p, [_sz])
+    fn("hwyblas_amd_free", None, [_p])
+    fn("hwyblas_amd_memcpy", None, [_p, _p, _sz, _i])
+    fn("hwyblas_amd_memset", None, [_p, _i, _sz])
+    fn("hwyblas_amd_synchronize", None, [])
+    fn("hwyblas_amd_thread_stream", _p, [])
+    fn("hwyblas_amd_stream_create", _p, [])
+    fn("hwyblas_amd_stream_destroy", None, [_p])
+    fn("hwyblas_amd_stream_synchronize", None, [_p])
+    fn("hwyblas_amd_event_create", _p, [])
+    fn("hwyblas_amd_event_destroy", None, [_p])
+    fn("hwyblas_amd_event_record", None, [_p, _p])
+    fn("hwyblas_amd_event_elapsed_ms", _f, [_p, _p])
+    fn("hwyblas_amd_fill_uniform", None, [_p, _i, _i64, _i64, _i64, _u64, _d, _d, _p])
+    fn("hwyblas_amd_version", ctypes.c_char_p, [])
+
+
+def lib():
+    """Loads libhwyblas_amd.so (raises LibraryMissing if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(
+                f"{LIB_PATH} is not built; run `python -m hwyblas_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def exported_symbols_in_header():
+    """Names of every function the public header declares."""
+    import re
+    text = "\n".join(l for l in open(HEADER).read().splitlines()
+                     if not l.lstrip().startswith("#"))
+    return sorted(set(re.findall(r"HWYBLAS_AMD_API\s+[^;(]*?\b(\w+)\s*\(", text)))
+
+
+# ---------------------------------------------------------------------------
+# Helpers for host arrays (numpy, Fortran order) through the Fortran ABI.
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, DeviceArray):
+        return ctypes.c_void_p(x.ptr)
+    return x.ctypes.data_as(ctypes.c_void_p)
+
+
+def _ip(v):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+def _sp(v, dtype):
+    return ctypes.byref(ctypes.c_double(v) if np.dtype(dtype) == np.float64 else ctypes.c_float(v))
+
+
+def _pre(dtype):
+    return "d" if np.dtype(dtype) == np.float64 else "s"
+
+
+def gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, dtype=None):
+    """Fortran ?gemm_ on numpy (host) arrays or DeviceArrays; C in place."""
+    dtype = dtype or c.dtype
+    f = getattr(lib(), f"{_pre(dtype)}gemm_")
+    f(ta.encode(), tb.encode(), _ip(m), _ip(n), _ip(k), _sp(alpha, dtype), _ptr(a), _ip(lda),
+      _ptr(b), _ip(ldb), _sp(beta, dtype), _ptr(c), _ip(ldc))
+
+
+def trsm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, dtype=None):
+    dtype = dtype or b.dtype
+    f = getattr(lib(), f"{_pre(dtype)}trsm_")
+    f(side.encode(), uplo.encode(), ta.encode(), diag.encode(), _ip(m), _ip(n),
+      _sp(alpha, dtype), _ptr(a), _ip(lda), _ptr(b), _ip(ldb))
+
+
+def cblas_gemm(layout, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, dtype=None):
+    dtype = dtype or c.dtype
+    f = getattr(lib(), f"cblas_{_pre(dtype)}gemm")
+    f(layout, ta, tb, m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc)
+
+
+def cblas_trsm(layout, side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, dtype=None):
+    dtype = dtype or b.dtype
+    f = getattr(lib(), f"cblas_{_pre(dtype)}trsm")
+    f(layout, side, uplo, ta, diag, m, n, alpha, _ptr(a), lda, _ptr(b), ldb)
+
+
+def gemm_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None, dtype=None):
+    dtype = dtype or c.dtype
+    f = getattr(lib(), f"hwyblas_{_pre(dtype)}gemm_dev")
+    return f(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta,
+             _ptr(c), ldc, stream)
+
+
+def trsm_dev(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, stream=None, dtype=None):
+    dtype = dtype or b.dtype
+    f = getattr(lib(), f"hwyblas_{_pre(dtype)}trsm_dev")
+    return f(side.encode(), uplo.encode(), ta.encode(), diag.encode(), m, n, alpha,
+             _ptr(a), lda, _ptr(b), ldb, stream)
+
+
+class ErrorCapture:
+    """Installs an error handler that records (name, position) pairs."""
+
+    def __init__(self):
+        self.calls = []
+        self._cb = ERROR_HANDLER(lambda name, pos: self.calls.append((name.decode(), pos)))
+
+    def __enter__(self):
+        self._prev = lib().hwyblas_set_error_handler(ctypes.cast(self._cb, ctypes.c_void_p))
+        return self
+
+    def __exit__(self, *exc):
+        lib().hwyblas_set_error_handler(self._prev)
+        return False
+
+
+# ---------------------------------------------------------------------------
+# Device memory
+
+class DeviceArray:
+    """A column-major device matrix (ld x cols) owned by this object."""
+
+    def __init__(self, ld, cols, dtype=np.float64):
+        self.ld, self.cols, self.dtype = int(ld), int(cols), np.dtype(dtype)
+        self.nbytes = max(self.ld * self.cols * self.dtype.itemsize, 1)
+        self.ptr = lib().hwyblas_amd_malloc(self.nbytes)
+
+    @classmethod
+    def from_host(cls, x):
+        x = np.asfortranarray(x)
+        d = cls(x.shape[0], x.shape[1] if x.ndim > 1 else 1, x.dtype)
+        lib().hwyblas_amd_memcpy(d.ptr, x.ctypes.data, x.nbytes, 1)
+        return d
+
+    def to_host(self):
+        out = np.empty((self.ld, self.cols), dtype=self.dtype, order="F")
+        lib().hwyblas_amd_memcpy(out.ctypes.data, self.ptr, out.nbytes, 2)
+        return out
+
+    def fill_uniform(self, rows, cols, seed, lo=-1.0, hi=1.0, stream=None):
+        lib().hwyblas_amd_fill_uniform(self.ptr, self.dtype.itemsize, rows, cols, self.ld,
+                                       seed, lo, hi, stream)
+
+    def free(self):
+        if self.ptr:
+            lib().hwyblas_amd_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self.ev = lib().hwyblas_amd_event_create()
+
+    def record(self, stream=None):
+        lib().hwyblas_amd_event_record(self.ev, stream)
+
+    def elapsed_ms(self, later):
+        return lib().hwyblas_amd_event_elapsed_ms(self.ev, later.ev)
+
+    def __del__(self):
+        try:
+            lib().hwyblas_amd_event_destroy(self.ev)
+        except Exception:
+            pass
+
+
+def synchronize():
+    lib().hwyblas_amd_synchronize()
+
+
+def device_count():
+    return lib().hwyblas_amd_device_count()

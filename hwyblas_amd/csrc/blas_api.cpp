@@ -1,0 +1,280 @@
+// Fortran-convention entry points (sgemm_, dgemm_, strsm_, dtrsm_) and their
+// device-pointer forms.
+//
+// Validation order, info positions and quick returns follow hwyBLAS
+// (src/gemm.cc:73-132, src/trsm.cc:137-180). Host operands are staged into
+// the calling thread's device buffers: only the referenced rows x cols of
+// each operand travel (at compact leading dimension), C is copied in only
+// when beta != 0, and only rows [0, m) of each column are copied back, so
+// the padding of C between m and ldc is never written. Device operands
+// (hipMalloc or managed memory) are used in place.
+#include <string.h>
+
+#include <algorithm>
+
+#include "common.h"
+
+namespace hwy_amd {
+namespace {
+
+template <typename T>
+void CopyIn(T* dst, const T* src, int64_t rows, int64_t cols, int64_t ld,
+            hipStream_t s) {
+  HWY_HIP_CHECK(hipMemcpy2DAsync(dst, rows * sizeof(T), src, ld * sizeof(T),
+                                 rows * sizeof(T), cols, hipMemcpyHostToDevice, s));
+}
+
+template <typename T>
+void CopyOut(T* dst, int64_t ld, const T* src, int64_t rows, int64_t cols,
+             hipStream_t s) {
+  HWY_HIP_CHECK(hipMemcpy2DAsync(dst, ld * sizeof(T), src, rows * sizeof(T),
+                                 rows * sizeof(T), cols, hipMemcpyDeviceToHost, s));
+}
+
+// Returns the Fortran info of the first invalid GEMM argument, or 0
+// (src/gemm.cc:84-106).
+blasint GemmInfo(char transa, char transb, blasint m, blasint n, blasint k,
+                 blasint lda, blasint ldb, blasint ldc, Op* oa, Op* ob) {
+  const bool va = DecodeOp(transa, oa);
+  const bool vb = DecodeOp(transb, ob);
+  const blasint rows_a = (*oa == Op::kNoTrans) ? m : k;
+  const blasint rows_b = (*ob == Op::kNoTrans) ? k : n;
+  if (!va) return 1;
+  if (!vb) return 2;
+  if (m < 0) return 3;
+  if (n < 0) return 4;
+  if (k < 0) return 5;
+  if (lda < std::max(1, rows_a)) return 8;
+  if (ldb < std::max(1, rows_b)) return 10;
+  if (ldc < std::max(1, m)) return 13;
+  return 0;
+}
+
+// Returns the Fortran info of the first invalid TRSM argument, or 0
+// (src/trsm.cc:145-168).
+blasint TrsmInfo(char side, char uplo, char transa, char diag, blasint m,
+                 blasint n, blasint lda, blasint ldb, bool* left, bool* lower,
+                 Op* op, bool* unit) {
+  const bool vs = DecodeSide(side, left);
+  const bool vu = DecodeUplo(uplo, lower);
+  const bool vt = DecodeOp(transa, op);
+  const bool vd = DecodeDiag(diag, unit);
+  const blasint rows_a = *left ? m : n;
+  if (!vs) return 1;
+  if (!vu) return 2;
+  if (!vt) return 3;
+  if (!vd) return 4;
+  if (m < 0) return 5;
+  if (n < 0) return 6;
+  if (lda < std::max(1, rows_a)) return 9;
+  if (ldb < std::max(1, m)) return 11;
+  return 0;
+}
+
+template <typename T>
+void GemmHost(const char* name, const char* transa, const char* transb,
+              const blasint* mp, const blasint* np, const blasint* kp,
+              const T* alphap, const T* a, const blasint* ldap, const T* b,
+              const blasint* ldbp, const T* betap, T* c, const blasint* ldcp) {
+  const blasint m = *mp, n = *np, k = *kp, lda = *ldap, ldb = *ldbp, ldc = *ldcp;
+  Op oa, ob;
+  const blasint info = GemmInfo(*transa, *transb, m, n, k, lda, ldb, ldc, &oa, &ob);
+  if (info != 0) {
+    ReportError(name, info);
+    return;
+  }
+  const T alpha = *alphap, beta = *betap;
+  if (m == 0 || n == 0) return;
+  if ((k == 0 || alpha == T(0)) && beta == T(1)) return;
+
+  ThreadContext& ctx = Context();
+  hipStream_t s = ctx.stream;
+  const bool c_dev = IsDevicePointer(c);
+  T* dc = c;
+  int64_t ldc_d = ldc;
+  if (!c_dev) {
+    dc = static_cast<T*>(Staging(kSlotC, size_t(m) * n * sizeof(T)));
+    ldc_d = m;
+    if (beta != T(0)) CopyIn(dc, c, m, n, ldc, s);
+  }
+  if (k == 0 || alpha == T(0)) {
+    ScaleDevice<T>(m, n, beta, dc, ldc_d, s);
+  } else {
+    const bool ta = oa == Op::kTrans, tb = ob == Op::kTrans;
+    const int64_t ra = ta ? k : m, ca = ta ? m : k;
+    const int64_t rb = tb ? n : k, cb = tb ? k : n;
+    const T* da = a;
+    const T* db = b;
+    int64_t lda_d = lda, ldb_d = ldb;
+    if (!IsDevicePointer(a)) {
+      T* buf = static_cast<T*>(Staging(kSlotA, size_t(ra) * ca * sizeof(T)));
+      CopyIn(buf, a, ra, ca, lda, s);
+      da = buf;
+      lda_d = ra;
+    }
+    if (!IsDevicePointer(b)) {
+      T* buf = static_cast<T*>(Staging(kSlotB, size_t(rb) * cb * sizeof(T)));
+      CopyIn(buf, b, rb, cb, ldb, s);
+      db = buf;
+      ldb_d = rb;
+    }
+    GemmDevice<T>(ta, tb, m, n, k, alpha, da, lda_d, db, ldb_d, beta, dc, ldc_d, s);
+  }
+  if (!c_dev) CopyOut(c, ldc, dc, m, n, s);
+  HWY_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <typename T>
+int GemmDev(const char* name, char transa, char transb, int m, int n, int k,
+            T alpha, const T* a, int lda, const T* b, int ldb, T beta, T* c,
+            int ldc, hipStream_t s) {
+  Op oa, ob;
+  const blasint info = GemmInfo(transa, transb, m, n, k, lda, ldb, ldc, &oa, &ob);
+  if (info != 0) {
+    ReportError(name, info);
+    return info;
+  }
+  if (m == 0 || n == 0) return 0;
+  if ((k == 0 || alpha == T(0)) && beta == T(1)) return 0;
+  if (k == 0 || alpha == T(0)) {
+    ScaleDevice<T>(m, n, beta, c, ldc, s);
+    return 0;
+  }
+  GemmDevice<T>(oa == Op::kTrans, ob == Op::kTrans, m, n, k, alpha, a, lda, b,
+                ldb, beta, c, ldc, s);
+  return 0;
+}
+
+template <typename T>
+void TrsmHost(const char* name, const char* sidep, const char* uplop,
+              const char* transp, const char* diagp, const blasint* mp,
+              const blasint* np, const T* alphap, const T* a,
+              const blasint* ldap, T* b, const blasint* ldbp) {
+  const blasint m = *mp, n = *np, lda = *ldap, ldb = *ldbp;
+  bool left, lower, unit;
+  Op op;
+  const blasint info = TrsmInfo(*sidep, *uplop, *transp, *diagp, m, n, lda, ldb,
+                                &left, &lower, &op, &unit);
+  if (info != 0) {
+    ReportError(name, info);
+    return;
+  }
+  const T alpha = *alphap;
+  if (m == 0 || n == 0) return;
+
+  ThreadContext& ctx = Context();
+  hipStream_t s = ctx.stream;
+  const bool b_dev = IsDevicePointer(b);
+  T* db = b;
+  int64_t ldb_d = ldb;
+  if (!b_dev) {
+    db = static_cast<T*>(Staging(kSlotB, size_t(m) * n * sizeof(T)));
+    ldb_d = m;
+    if (alpha != T(0)) CopyIn(db, b, m, n, ldb, s);
+  }
+  // B := alpha B first; alpha == 0 leaves zeros and never reads A
+  // (src/trsm.cc:172-175).
+  if (alpha != T(1)) ScaleDevice<T>(m, n, alpha, db, ldb_d, s);
+  if (alpha != T(0)) {
+    const int64_t ka = left ? m : n;
+    const T* da = a;
+    int64_t lda_d = lda;
+    if (!IsDevicePointer(a)) {
+      T* buf = static_cast<T*>(Staging(kSlotA, size_t(ka) * ka * sizeof(T)));
+      CopyIn(buf, a, ka, ka, lda, s);
+      da = buf;
+      lda_d = ka;
+    }
+    TrsmDevice<T>(left, lower, op == Op::kTrans, unit, m, n, da, lda_d, db, ldb_d, s);
+  }
+  if (!b_dev) CopyOut(b, ldb, db, m, n, s);
+  HWY_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <typename T>
+int TrsmDev(const char* name, char side, char uplo, char transa, char diag,
+            int m, int n, T alpha, const T* a, int lda, T* b, int ldb,
+            hipStream_t s) {
+  bool left, lower, unit;
+  Op op;
+  const blasint info =
+      TrsmInfo(side, uplo, transa, diag, m, n, lda, ldb, &left, &lower, &op, &unit);
+  if (info != 0) {
+    ReportError(name, info);
+    return info;
+  }
+  if (m == 0 || n == 0) return 0;
+  if (alpha != T(1)) ScaleDevice<T>(m, n, alpha, b, ldb, s);
+  if (alpha == T(0)) return 0;
+  TrsmDevice<T>(left, lower, op == Op::kTrans, unit, m, n, a, lda, b, ldb, s);
+  return 0;
+}
+
+}  // namespace
+}  // namespace hwy_amd
+
+using hwy_amd::blasint;
+
+extern "C" {
+
+HWYBLAS_AMD_API void sgemm_(const char* transa, const char* transb,
+                            const blasint* m, const blasint* n, const blasint* k,
+                            const float* alpha, const float* a, const blasint* lda,
+                            const float* b, const blasint* ldb, const float* beta,
+                            float* c, const blasint* ldc) {
+  hwy_amd::GemmHost("SGEMM", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+HWYBLAS_AMD_API void dgemm_(const char* transa, const char* transb,
+                            const blasint* m, const blasint* n, const blasint* k,
+                            const double* alpha, const double* a, const blasint* lda,
+                            const double* b, const blasint* ldb, const double* beta,
+                            double* c, const blasint* ldc) {
+  hwy_amd::GemmHost("DGEMM", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+HWYBLAS_AMD_API void strsm_(const char* side, const char* uplo, const char* transa,
+                            const char* diag, const blasint* m, const blasint* n,
+                            const float* alpha, const float* a, const blasint* lda,
+                            float* b, const blasint* ldb) {
+  hwy_amd::TrsmHost("STRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+}
+
+HWYBLAS_AMD_API void dtrsm_(const char* side, const char* uplo, const char* transa,
+                            const char* diag, const blasint* m, const blasint* n,
+                            const double* alpha, const double* a, const blasint* lda,
+                            double* b, const blasint* ldb) {
+  hwy_amd::TrsmHost("DTRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+}
+
+HWYBLAS_AMD_API int hwyblas_dgemm_dev(char transa, char transb, int m, int n, int k,
+                                      double alpha, const double* a, int lda,
+                                      const double* b, int ldb, double beta,
+                                      double* c, int ldc, void* stream) {
+  return hwy_amd::GemmDev("DGEMM", transa, transb, m, n, k, alpha, a, lda, b, ldb,
+                          beta, c, ldc, static_cast<hipStream_t>(stream));
+}
+
+HWYBLAS_AMD_API int hwyblas_sgemm_dev(char transa, char transb, int m, int n, int k,
+                                      float alpha, const float* a, int lda,
+                                      const float* b, int ldb, float beta,
+                                      float* c, int ldc, void* stream) {
+  return hwy_amd::GemmDev("SGEMM", transa, transb, m, n, k, alpha, a, lda, b, ldb,
+                          beta, c, ldc, static_cast<hipStream_t>(stream));
+}
+
+HWYBLAS_AMD_API int hwyblas_dtrsm_dev(char side, char uplo, char transa, char diag,
+                                      int m, int n, double alpha, const double* a,
+                                      int lda, double* b, int ldb, void* stream) {
+  return hwy_amd::TrsmDev("DTRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b,
+                          ldb, static_cast<hipStream_t>(stream));
+}
+
+HWYBLAS_AMD_API int hwyblas_strsm_dev(char side, char uplo, char transa, char diag,
+                                      int m, int n, float alpha, const float* a,
+                                      int lda, float* b, int ldb, void* stream) {
+  return hwy_amd::TrsmDev("STRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b,
+                          ldb, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

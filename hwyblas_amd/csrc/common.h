@@ -1,0 +1,130 @@
+// Internal declarations shared by the host (C++) and device (HIP) parts of
+// libhwyblas_amd: argument decoding, error reporting, the per-thread HIP
+// context and the device-side launchers.
+#ifndef HWYBLAS_AMD_CSRC_COMMON_H_
+#define HWYBLAS_AMD_CSRC_COMMON_H_
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/hwyblas_amd.h"
+
+namespace hwy_amd {
+
+using blasint = hwyblas_int;
+
+// ---------------------------------------------------------------------------
+// CHARACTER*1 decoding: only the first character is read, case-insensitively
+// (hwyBLAS src/common.h:68-92, src/api.h:72-112). Each decoder returns false
+// for an invalid value and still sets its output to the default.
+
+inline char ToUpper(char c) { return (c >= 'a' && c <= 'z') ? char(c - 32) : c; }
+
+enum class Op { kNoTrans, kTrans };  // ConjTrans is Trans for real types
+
+inline bool DecodeOp(char c, Op* op) {
+  switch (ToUpper(c)) {
+    case 'N': *op = Op::kNoTrans; return true;
+    case 'T':
+    case 'C': *op = Op::kTrans; return true;
+    default: *op = Op::kNoTrans; return false;
+  }
+}
+inline bool DecodeUplo(char c, bool* lower) {
+  switch (ToUpper(c)) {
+    case 'L': *lower = true; return true;
+    case 'U': *lower = false; return true;
+    default: *lower = false; return false;
+  }
+}
+inline bool DecodeSide(char c, bool* left) {
+  switch (ToUpper(c)) {
+    case 'L': *left = true; return true;
+    case 'R': *left = false; return true;
+    default: *left = false; return false;
+  }
+}
+inline bool DecodeDiag(char c, bool* unit) {
+  switch (ToUpper(c)) {
+    case 'U': *unit = true; return true;
+    case 'N': *unit = false; return true;
+    default: *unit = false; return false;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Error reporting (errors.cpp)
+
+// Reports invalid argument `info` (1-based) of routine `name`: through the
+// installed handler, else cblas_xerbla with the mapped CBLAS position while a
+// CBLAS call is active on this thread, else xerbla_. Never aborts.
+void ReportError(const char* name, blasint info);
+bool CallErrorHandler(const char* name, blasint position);
+
+// Fortran argument i (1-based) of the routine running under a CBLAS wrapper
+// is CBLAS argument positions[i - 1].
+struct CblasContext {
+  const char* name = nullptr;
+  const int* positions = nullptr;
+};
+void SetCblasContext(const CblasContext* context);
+
+// Prints the HIP error and aborts: device failures have no BLAS error code,
+// and a routine that cannot run must not return as if it had.
+[[noreturn]] void FatalHip(hipError_t e, const char* what, const char* file,
+                           int line);
+
+#define HWY_HIP_CHECK(expr)                                          \
+  do {                                                               \
+    hipError_t hwy_e_ = (expr);                                      \
+    if (hwy_e_ != hipSuccess) ::hwy_amd::FatalHip(hwy_e_, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Per-thread context (runtime.cpp): a stream of default priority and
+// grow-only device staging buffers, reused across calls like hwyBLAS's
+// thread-local scratch slots (src/common.cc:46-64).
+
+enum StagingSlot : int { kSlotA = 0, kSlotB = 1, kSlotC = 2, kNumSlots = 3 };
+
+struct ThreadContext {
+  hipStream_t stream = nullptr;
+  int device = -1;
+  void* buf[kNumSlots] = {nullptr, nullptr, nullptr};
+  size_t cap[kNumSlots] = {0, 0, 0};
+};
+
+ThreadContext& Context();
+void* Staging(int slot, size_t bytes);
+
+// Whether p can be dereferenced by kernels on the current device (device or
+// managed memory); host memory, including pinned memory, is staged.
+bool IsDevicePointer(const void* p);
+
+// ---------------------------------------------------------------------------
+// Device operands and launchers (gemm.hip, trsm.hip). All take device
+// pointers, enqueue work on `stream` and return without synchronising.
+// Leading dimensions and offsets are 64-bit inside the kernels.
+
+// C = alpha op(A) op(B) + beta C for m, n, k > 0 and alpha != 0; beta == 0
+// never reads C. Op is encoded as trans = false (N) / true (T).
+template <typename T>
+void GemmDevice(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha,
+                const T* a, int64_t lda, const T* b, int64_t ldb, T beta,
+                T* c, int64_t ldc, hipStream_t stream);
+
+// C := beta C over m x n; beta == 0 writes zeros without reading C.
+template <typename T>
+void ScaleDevice(int64_t m, int64_t n, T beta, T* c, int64_t ldc,
+                 hipStream_t stream);
+
+// op(A) X = B (left) or X op(A) = B (right) in place, alpha already applied.
+template <typename T>
+void TrsmDevice(bool left, bool lower, bool trans, bool unit, int64_t m,
+                int64_t n, const T* a, int64_t lda, T* b, int64_t ldb,
+                hipStream_t stream);
+
+}  // namespace hwy_amd
+
+#endif  // HWYBLAS_AMD_CSRC_COMMON_H_

@@ -1,0 +1,215 @@
+// Per-thread HIP context, device staging buffers, pointer classification and
+// the exported device utilities (memory, streams, events, the seeded fill).
+//
+// Each calling thread gets, per device, a stream of default priority and
+// three grow-only staging buffers (A, B, C), the device counterpart of
+// hwyBLAS's thread-local scratch slots (src/common.cc:46-64): reused across
+// calls, freed first when they must grow (to keep peak memory low), never
+// shrunk. The Fortran/CBLAS routines are therefore reentrant across threads
+// without locks, as hwyBLAS's are.
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "common.h"
+
+namespace hwy_amd {
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+struct PerThread {
+  ThreadContext ctx[kMaxDevices];
+  ~PerThread() {
+    for (int d = 0; d < kMaxDevices; ++d) {
+      ThreadContext& c = ctx[d];
+      if (c.device < 0) continue;
+      // Best effort at thread exit: the runtime may already be shutting down.
+      if (hipSetDevice(d) != hipSuccess) continue;
+      for (int s = 0; s < kNumSlots; ++s)
+        if (c.buf[s]) (void)hipFree(c.buf[s]);
+      if (c.stream) (void)hipStreamDestroy(c.stream);
+    }
+  }
+};
+
+thread_local PerThread t_state;
+
+}  // namespace
+
+ThreadContext& Context() {
+  int dev = 0;
+  HWY_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    fprintf(stderr, "hwyblas_amd: device %d out of range\n", dev);
+    abort();
+  }
+  ThreadContext& c = t_state.ctx[dev];
+  if (c.device < 0) {
+    HWY_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.device = dev;
+  }
+  return c;
+}
+
+void* Staging(int slot, size_t bytes) {
+  ThreadContext& c = Context();
+  if (bytes == 0) bytes = 1;
+  if (c.cap[slot] < bytes) {
+    if (c.buf[slot]) {
+      HWY_HIP_CHECK(hipStreamSynchronize(c.stream));
+      HWY_HIP_CHECK(hipFree(c.buf[slot]));
+      c.buf[slot] = nullptr;
+      c.cap[slot] = 0;
+    }
+    // Round up to 2 MiB so that slowly growing sizes do not reallocate often.
+    const size_t rounded = (bytes + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+    HWY_HIP_CHECK(hipMalloc(&c.buf[slot], rounded));
+    c.cap[slot] = rounded;
+  }
+  return c.buf[slot];
+}
+
+bool IsDevicePointer(const void* p) {
+  if (p == nullptr) return false;
+  hipPointerAttribute_t attr;
+  const hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // unknown (pageable host) memory: clear the error
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+namespace {
+
+__device__ __forceinline__ uint64_t SplitMix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+uint64_t SplitMix64Host(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+template <typename T>
+__global__ void FillUniformKernel(T* x, int64_t rows, int64_t cols, int64_t ld,
+                                  uint64_t stream_key, double lo, double width) {
+  const int64_t total = rows * cols;
+  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const double u = double(SplitMix64(stream_key + uint64_t(idx)) >> 11) * 0x1.0p-53;
+    const int64_t i = idx % rows, j = idx / rows;
+    x[i + j * ld] = T(__builtin_fma(u, width, lo));
+  }
+}
+
+}  // namespace
+}  // namespace hwy_amd
+
+using namespace hwy_amd;
+
+extern "C" {
+
+HWYBLAS_AMD_API int hwyblas_amd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+HWYBLAS_AMD_API void* hwyblas_amd_malloc(size_t bytes) {
+  void* p = nullptr;
+  HWY_HIP_CHECK(hipMalloc(&p, bytes ? bytes : 1));
+  return p;
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_free(void* p) {
+  if (p) HWY_HIP_CHECK(hipFree(p));
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_memcpy(void* dst, const void* src, size_t bytes,
+                                        int kind) {
+  const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                          : kind == 2 ? hipMemcpyDeviceToHost
+                                      : hipMemcpyDeviceToDevice;
+  HWY_HIP_CHECK(hipMemcpy(dst, src, bytes, k));
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_memset(void* dst, int value, size_t bytes) {
+  HWY_HIP_CHECK(hipMemset(dst, value, bytes));
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_synchronize(void) {
+  HWY_HIP_CHECK(hipDeviceSynchronize());
+}
+
+HWYBLAS_AMD_API void* hwyblas_amd_thread_stream(void) { return Context().stream; }
+
+HWYBLAS_AMD_API void* hwyblas_amd_stream_create(void) {
+  hipStream_t s = nullptr;
+  HWY_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_stream_destroy(void* stream) {
+  if (stream) HWY_HIP_CHECK(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_stream_synchronize(void* stream) {
+  HWY_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+}
+
+HWYBLAS_AMD_API void* hwyblas_amd_event_create(void) {
+  hipEvent_t e = nullptr;
+  HWY_HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_event_destroy(void* event) {
+  if (event) HWY_HIP_CHECK(hipEventDestroy(static_cast<hipEvent_t>(event)));
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_event_record(void* event, void* stream) {
+  HWY_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(event),
+                               static_cast<hipStream_t>(stream)));
+}
+
+HWYBLAS_AMD_API float hwyblas_amd_event_elapsed_ms(void* start, void* stop) {
+  HWY_HIP_CHECK(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+  float ms = 0;
+  HWY_HIP_CHECK(hipEventElapsedTime(&ms, static_cast<hipEvent_t>(start),
+                                    static_cast<hipEvent_t>(stop)));
+  return ms;
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_fill_uniform(void* x, int dtype_bytes,
+                                              int64_t rows, int64_t cols,
+                                              int64_t ld, uint64_t seed,
+                                              double lo, double hi,
+                                              void* stream) {
+  if (rows <= 0 || cols <= 0) return;
+  const uint64_t key = SplitMix64Host(seed ^ 0x5851F42D4C957F2Dull);
+  const int64_t total = rows * cols;
+  const unsigned blocks = unsigned(std::min<int64_t>((total + 255) / 256, 1 << 16));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype_bytes == 8) {
+    hipLaunchKernelGGL(FillUniformKernel<double>, dim3(blocks), dim3(256), 0, s,
+                       static_cast<double*>(x), rows, cols, ld, key, lo, hi - lo);
+  } else {
+    hipLaunchKernelGGL(FillUniformKernel<float>, dim3(blocks), dim3(256), 0, s,
+                       static_cast<float*>(x), rows, cols, ld, key, lo, hi - lo);
+  }
+  HWY_HIP_CHECK(hipGetLastError());
+}
+
+HWYBLAS_AMD_API const char* hwyblas_amd_version(void) { return "hwyblas_amd 0.1.0 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,214 @@
+/*
+ * libhwyblas_amd — the Level-3 hot path of hwyBLAS (?gemm_, ?trsm_) on
+ * AMD Instinct MI355X (gfx950), behind the same Fortran and CBLAS ABI.
+ *
+ * Every declaration cites the hwyBLAS interface it replaces (paths relative
+ * to the hwyBLAS source tree). Conventions are those of hwyBLAS:
+ *  - LP64: dimensions, leading dimensions and positions are `int`
+ *    (include/hwyblas/hwyblas.h:42);
+ *  - Fortran routines take every argument by pointer and read only the first
+ *    character of CHARACTER arguments; hidden length arguments are neither
+ *    declared nor needed (include/hwyblas/hwyblas.h:14-16);
+ *  - invalid arguments are reported through the installed handler, else
+ *    cblas_xerbla / xerbla_, which print and return: the routine then
+ *    returns with its outputs untouched (src/common.cc:33-44);
+ *  - A and B are read only; only rows [0, m) of each column of C (of B for
+ *    TRSM) are written; C is not read when beta == 0, A and B are not read
+ *    when alpha == 0 (README.md:508-525).
+ *
+ * The Fortran and CBLAS routines accept host pointers (operands are staged
+ * through device memory on a per-thread stream; the call returns when the
+ * result is back) or device pointers (hipMalloc or managed memory: the work
+ * runs in place). They are synchronous, reentrant and callable from many
+ * threads. The *_dev entry points take device pointers and a stream and are
+ * asynchronous. A HIP failure (out of device memory, failed launch) prints a
+ * message and aborts, as hwyBLAS does when scratch allocation fails
+ * (src/common.cc:57-60).
+ */
+#ifndef HWYBLAS_AMD_H_
+#define HWYBLAS_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#define HWYBLAS_AMD_API __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/hwyblas/hwyblas.h:42 */
+typedef int hwyblas_int;
+
+/* ------------------------------------------------------------------------ */
+/* Fortran ABI                                                              */
+
+/* C := alpha op(A) op(B) + beta C. Replaces sgemm_/dgemm_
+ * (include/hwyblas/hwyblas.h:564-575, src/gemm.cc:142-158); argument checks
+ * in the order of src/gemm.cc:89-106 (info 1, 2, 3, 4, 5, 8, 10, 13). */
+HWYBLAS_AMD_API void sgemm_(const char* transa, const char* transb,
+                            const hwyblas_int* m, const hwyblas_int* n,
+                            const hwyblas_int* k, const float* alpha,
+                            const float* a, const hwyblas_int* lda,
+                            const float* b, const hwyblas_int* ldb,
+                            const float* beta, float* c,
+                            const hwyblas_int* ldc);
+HWYBLAS_AMD_API void dgemm_(const char* transa, const char* transb,
+                            const hwyblas_int* m, const hwyblas_int* n,
+                            const hwyblas_int* k, const double* alpha,
+                            const double* a, const hwyblas_int* lda,
+                            const double* b, const hwyblas_int* ldb,
+                            const double* beta, double* c,
+                            const hwyblas_int* ldc);
+
+/* Solves op(A) X = alpha B (side L) or X op(A) = alpha B (side R); X
+ * overwrites B. Replaces strsm_/dtrsm_ (include/hwyblas/hwyblas.h:787-796,
+ * src/trsm.cc:188-201); checks in the order of src/trsm.cc:151-168 (info 1,
+ * 2, 3, 4, 5, 6, 9, 11). */
+HWYBLAS_AMD_API void strsm_(const char* side, const char* uplo,
+                            const char* transa, const char* diag,
+                            const hwyblas_int* m, const hwyblas_int* n,
+                            const float* alpha, const float* a,
+                            const hwyblas_int* lda, float* b,
+                            const hwyblas_int* ldb);
+HWYBLAS_AMD_API void dtrsm_(const char* side, const char* uplo,
+                            const char* transa, const char* diag,
+                            const hwyblas_int* m, const hwyblas_int* n,
+                            const double* alpha, const double* a,
+                            const hwyblas_int* lda, double* b,
+                            const hwyblas_int* ldb);
+
+/* ------------------------------------------------------------------------ */
+/* Error handling (include/hwyblas/hwyblas.h:888-902, cblas.h:45)           */
+
+/* Prints " ** On entry to NAME parameter number I had an illegal value" and
+ * returns (src/common.cc:71-79). */
+HWYBLAS_AMD_API void xerbla_(const char* srname, const hwyblas_int* info,
+                             size_t srname_len);
+/* Prints "Parameter P to routine ROUT was incorrect" and then `form`
+ * (src/cblas.cc:714-726). */
+HWYBLAS_AMD_API void cblas_xerbla(hwyblas_int p, const char* rout,
+                                  const char* form, ...);
+typedef void (*hwyblas_error_handler)(const char* name, hwyblas_int position);
+/* Installs a handler called instead of xerbla_/cblas_xerbla; returns the
+ * previous one; NULL restores the default (src/common.cc:81-84). */
+HWYBLAS_AMD_API hwyblas_error_handler
+hwyblas_set_error_handler(hwyblas_error_handler handler);
+
+/* ------------------------------------------------------------------------ */
+/* CBLAS (include/hwyblas/cblas.h:26-38 enums, :607-622, :858-871)          */
+
+typedef enum CBLAS_LAYOUT { CblasRowMajor = 101, CblasColMajor = 102 } CBLAS_LAYOUT;
+typedef enum CBLAS_TRANSPOSE {
+  CblasNoTrans = 111,
+  CblasTrans = 112,
+  CblasConjTrans = 113,
+  CblasConjNoTrans = 114
+} CBLAS_TRANSPOSE;
+typedef enum CBLAS_UPLO { CblasUpper = 121, CblasLower = 122 } CBLAS_UPLO;
+typedef enum CBLAS_DIAG { CblasNonUnit = 131, CblasUnit = 132 } CBLAS_DIAG;
+typedef enum CBLAS_SIDE { CblasLeft = 141, CblasRight = 142 } CBLAS_SIDE;
+
+/* Row-major maps to the column-major call with A and B swapped
+ * (src/cblas.cc:517-537); error positions count the layout as argument 1. */
+HWYBLAS_AMD_API void cblas_sgemm(const CBLAS_LAYOUT layout,
+                                 const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_TRANSPOSE transb,
+                                 const hwyblas_int m, const hwyblas_int n,
+                                 const hwyblas_int k, const float alpha,
+                                 const float* a, const hwyblas_int lda,
+                                 const float* b, const hwyblas_int ldb,
+                                 const float beta, float* c,
+                                 const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_dgemm(const CBLAS_LAYOUT layout,
+                                 const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_TRANSPOSE transb,
+                                 const hwyblas_int m, const hwyblas_int n,
+                                 const hwyblas_int k, const double alpha,
+                                 const double* a, const hwyblas_int lda,
+                                 const double* b, const hwyblas_int ldb,
+                                 const double beta, double* c,
+                                 const hwyblas_int ldc);
+/* Row-major flips side and uplo and swaps m and n (src/cblas.cc:661-691). */
+HWYBLAS_AMD_API void cblas_strsm(const CBLAS_LAYOUT layout,
+                                 const CBLAS_SIDE side, const CBLAS_UPLO uplo,
+                                 const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_DIAG diag, const hwyblas_int m,
+                                 const hwyblas_int n, const float alpha,
+                                 const float* a, const hwyblas_int lda,
+                                 float* b, const hwyblas_int ldb);
+HWYBLAS_AMD_API void cblas_dtrsm(const CBLAS_LAYOUT layout,
+                                 const CBLAS_SIDE side, const CBLAS_UPLO uplo,
+                                 const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_DIAG diag, const hwyblas_int m,
+                                 const hwyblas_int n, const double alpha,
+                                 const double* a, const hwyblas_int lda,
+                                 double* b, const hwyblas_int ldb);
+
+/* ------------------------------------------------------------------------ */
+/* Device-pointer entry points (no hwyBLAS counterpart: the reference has no
+ * device). Same arguments, meaning, checks and quick returns as the Fortran
+ * routines, with values instead of pointers for scalars, device pointers for
+ * the matrices and a HIP stream (hipStream_t; NULL = the null stream). The
+ * work is enqueued and the call returns without waiting. Returns 0, or the
+ * Fortran position of the first invalid argument (also reported as above,
+ * under the Fortran routine name). */
+HWYBLAS_AMD_API int hwyblas_dgemm_dev(char transa, char transb, int m, int n,
+                                      int k, double alpha, const double* a,
+                                      int lda, const double* b, int ldb,
+                                      double beta, double* c, int ldc,
+                                      void* stream);
+HWYBLAS_AMD_API int hwyblas_sgemm_dev(char transa, char transb, int m, int n,
+                                      int k, float alpha, const float* a,
+                                      int lda, const float* b, int ldb,
+                                      float beta, float* c, int ldc,
+                                      void* stream);
+HWYBLAS_AMD_API int hwyblas_dtrsm_dev(char side, char uplo, char transa,
+                                      char diag, int m, int n, double alpha,
+                                      const double* a, int lda, double* b,
+                                      int ldb, void* stream);
+HWYBLAS_AMD_API int hwyblas_strsm_dev(char side, char uplo, char transa,
+                                      char diag, int m, int n, float alpha,
+                                      const float* a, int lda, float* b,
+                                      int ldb, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Device utilities (no hwyBLAS counterpart): memory, streams, timing and the
+ * seeded generator shared with the CPU oracle, for callers and benchmarks
+ * that hold no other HIP binding. Errors abort like any HIP failure.        */
+
+HWYBLAS_AMD_API int hwyblas_amd_device_count(void);
+HWYBLAS_AMD_API void* hwyblas_amd_malloc(size_t bytes);
+HWYBLAS_AMD_API void hwyblas_amd_free(void* p);
+/* kind: 1 host->device, 2 device->host, 3 device->device. */
+HWYBLAS_AMD_API void hwyblas_amd_memcpy(void* dst, const void* src,
+                                        size_t bytes, int kind);
+HWYBLAS_AMD_API void hwyblas_amd_memset(void* dst, int value, size_t bytes);
+HWYBLAS_AMD_API void hwyblas_amd_synchronize(void);
+/* The calling thread's library stream (used by the Fortran/CBLAS routines). */
+HWYBLAS_AMD_API void* hwyblas_amd_thread_stream(void);
+HWYBLAS_AMD_API void* hwyblas_amd_stream_create(void);
+HWYBLAS_AMD_API void hwyblas_amd_stream_destroy(void* stream);
+HWYBLAS_AMD_API void hwyblas_amd_stream_synchronize(void* stream);
+HWYBLAS_AMD_API void* hwyblas_amd_event_create(void);
+HWYBLAS_AMD_API void hwyblas_amd_event_destroy(void* event);
+HWYBLAS_AMD_API void hwyblas_amd_event_record(void* event, void* stream);
+/* Milliseconds between two recorded events (waits for the second). */
+HWYBLAS_AMD_API float hwyblas_amd_event_elapsed_ms(void* start, void* stop);
+/* x(i, j) = fma(u, hi - lo, lo) for i < rows, j < cols of a column-major
+ * device matrix; u in [0,1) from the counter-based splitmix64 stream `seed`
+ * at compact index i + j * rows (the same values as the CPU oracle's
+ * generator). dtype_bytes is 8 (double) or 4 (float, rounded from double). */
+HWYBLAS_AMD_API void hwyblas_amd_fill_uniform(void* x, int dtype_bytes,
+                                              int64_t rows, int64_t cols,
+                                              int64_t ld, uint64_t seed,
+                                              double lo, double hi,
+                                              void* stream);
+/* Library version string. */
+HWYBLAS_AMD_API const char* hwyblas_amd_version(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* HWYBLAS_AMD_H_ */

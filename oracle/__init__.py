@@ -1,0 +1,145 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY. The oracle is a single-threaded restatement of
+hwyBLAS's Level-3 algorithms plus double-double references (see oracle.h).
+Only the tests, the smoke check and the CPU-baseline leg of bench.py import
+this module, as the checker; the product library never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_c_char = ctypes.c_char
+_i = ctypes.c_int
+_d = ctypes.c_double
+_f = ctypes.c_float
+_p = ctypes.c_void_p
+
+
+def build(force=False):
+    """Compile liboracle.so with the Makefile beside this file."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE],
+                       check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_splitmix64.restype = ctypes.c_uint64
+        L.orc_splitmix64.argtypes = [ctypes.c_uint64]
+        L.orc_uniform01.restype = _d
+        L.orc_uniform01.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        for name, real in (("orc_fill_uniform_f64", None), ("orc_fill_uniform_f32", None)):
+            fn = getattr(L, name)
+            fn.restype = None
+            fn.argtypes = [_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                           ctypes.c_uint64, _d, _d]
+        for pre, s in (("d", _d), ("s", _f)):
+            g = getattr(L, f"orc_{pre}gemm")
+            g.restype = None
+            g.argtypes = [_c_char, _c_char, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i]
+            t = getattr(L, f"orc_{pre}trsm")
+            t.restype = None
+            t.argtypes = [_c_char, _c_char, _c_char, _c_char, _i, _i, s, _p, _i, _p, _i]
+            r = getattr(L, f"orc_{pre}gemm_ref")
+            r.restype = None
+            r.argtypes = [_c_char, _c_char, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i, _p, _p]
+            e = getattr(L, f"orc_{pre}gemm_ref_entries")
+            e.restype = None
+            e.argtypes = [_c_char, _c_char, _i, s, _p, _i, _p, _i, s, _p, _i, _p, _p, _i, _p, _p]
+            q = getattr(L, f"orc_{pre}trsm_residual")
+            q.restype = None
+            q.argtypes = [_c_char, _c_char, _c_char, _c_char, _i, _i, s, _p, _i, _p, _i,
+                          _p, _i, _p, _i, _p]
+        L.orc_plant_mistake.restype = None
+        L.orc_plant_mistake.argtypes = [_i]
+        _lib = L
+    return _lib
+
+
+def _ptr(x):
+    return None if x is None else x.ctypes.data_as(ctypes.c_void_p)
+
+
+def _b(c):
+    return c.encode() if isinstance(c, str) else c
+
+
+def _pre(dtype):
+    return "d" if np.dtype(dtype) == np.float64 else "s"
+
+
+# ---- generator ----------------------------------------------------------
+
+def uniform01(seed, index):
+    return lib().orc_uniform01(seed, index)
+
+
+def fill_uniform(rows, cols, seed, lo=-1.0, hi=1.0, dtype=np.float64, ld=None):
+    """Column-major rows x cols matrix (Fortran-ordered array of shape
+    (ld, cols)) filled by the generator shared with the device fill."""
+    ld = rows if ld is None else ld
+    x = np.zeros((max(ld, 1), cols), dtype=dtype, order="F")
+    fn = lib().orc_fill_uniform_f64 if np.dtype(dtype) == np.float64 else lib().orc_fill_uniform_f32
+    fn(_ptr(x), rows, cols, ld, seed, lo, hi)
+    return x
+
+
+# ---- restatement ------------------------------------------------------------
+
+def gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """In-place C update with hwyBLAS's blocked algorithm (Fortran args)."""
+    fn = getattr(lib(), f"orc_{_pre(c.dtype)}gemm")
+    fn(_b(ta), _b(tb), m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc)
+
+
+def trsm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb):
+    fn = getattr(lib(), f"orc_{_pre(b.dtype)}trsm")
+    fn(_b(side), _b(uplo), _b(ta), _b(diag), m, n, alpha, _ptr(a), lda, _ptr(b), ldb)
+
+
+# ---- references ---------------------------------------------------------
+
+def gemm_ref(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """Double-double expected C (compact m x n, float64) and error scale."""
+    exp = np.zeros((max(m, 1), n), dtype=np.float64, order="F")
+    bnd = np.zeros((max(m, 1), n), dtype=np.float64, order="F")
+    fn = getattr(lib(), f"orc_{_pre(a.dtype)}gemm_ref")
+    fn(_b(ta), _b(tb), m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta,
+       _ptr(c) if c is not None else None, ldc, _ptr(exp), _ptr(bnd))
+    return exp[:m], bnd[:m]
+
+
+def gemm_ref_entries(ta, tb, k, alpha, a, lda, b, ldb, beta, c, ldc, rows, cols):
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    exp = np.zeros(len(rows), dtype=np.float64)
+    bnd = np.zeros(len(rows), dtype=np.float64)
+    fn = getattr(lib(), f"orc_{_pre(a.dtype)}gemm_ref_entries")
+    fn(_b(ta), _b(tb), k, alpha, _ptr(a), lda, _ptr(b), ldb, beta,
+       _ptr(c) if c is not None else None, ldc, _ptr(rows), _ptr(cols), len(rows),
+       _ptr(exp), _ptr(bnd))
+    return exp, bnd
+
+
+def trsm_residual(side, uplo, ta, diag, m, n, alpha, a, lda, b0, ldb, x, ldx, which):
+    which = np.ascontiguousarray(which, dtype=np.int32)
+    ratio = np.zeros(len(which), dtype=np.float64)
+    fn = getattr(lib(), f"orc_{_pre(b0.dtype)}trsm_residual")
+    fn(_b(side), _b(uplo), _b(ta), _b(diag), m, n, alpha, _ptr(a), lda, _ptr(b0), ldb,
+       _ptr(x), ldx, _ptr(which), len(which), _ptr(ratio))
+    return ratio
+
+
+def plant_mistake(which):
+    lib().orc_plant_mistake(which)

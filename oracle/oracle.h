@@ -1,0 +1,102 @@
+/*
+ * CPU oracle for the Level-3 hot path (?gemm_, ?trsm_) and its getrf/getrs/
+ * gerfs consumer. TEST INFRASTRUCTURE ONLY: tests, the smoke check and the
+ * CPU-baseline leg of the benchmark may load this library, as the checker;
+ * the product library never does.
+ *
+ * Three groups of functions:
+ *  - orc_fill_*      the counter-based generator shared with the device fill
+ *                    kernel (element (i, j) of a matrix depends only on the
+ *                    seed and the compact index i + j * rows);
+ *  - orc_?gemm/trsm  a single-threaded restatement of hwyBLAS's algorithms
+ *                    (Goto/BLIS blocked GEMM with an MR x NR FMA register tile,
+ *                    recursive TRSM halving at multiples of 32 with a
+ *                    reciprocal-multiply substitution base);
+ *  - orc_*_ref_*     double-double recomputations with per-element error
+ *                    scales, used to check any result against the
+ *                    componentwise bound |C - C_ref| <= c * k * u * scale.
+ *
+ * Each function cites the reference file:line it follows (paths relative to
+ * the hwyBLAS source tree).
+ */
+#ifndef HWYBLAS_AMD_ORACLE_H_
+#define HWYBLAS_AMD_ORACLE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- generator (shared bit-for-bit with the device fill kernel) -------- */
+uint64_t orc_splitmix64(uint64_t x);
+/* u in [0,1) with 53 random bits for element `index` of stream `seed`. */
+double orc_uniform01(uint64_t seed, uint64_t index);
+/* x(i,j) = fma(u, hi - lo, lo) for i < rows, j < cols; u from index i+j*rows. */
+void orc_fill_uniform_f64(double* x, int64_t rows, int64_t cols, int64_t ld,
+                          uint64_t seed, double lo, double hi);
+void orc_fill_uniform_f32(float* x, int64_t rows, int64_t cols, int64_t ld,
+                          uint64_t seed, double lo, double hi);
+
+/* ---- hwyBLAS restatement (column-major, Fortran semantics; arguments are
+ * assumed valid: validation is the ABI layer's job) ------------------------ */
+void orc_dgemm(char transa, char transb, int m, int n, int k, double alpha,
+               const double* a, int lda, const double* b, int ldb,
+               double beta, double* c, int ldc);
+void orc_sgemm(char transa, char transb, int m, int n, int k, float alpha,
+               const float* a, int lda, const float* b, int ldb, float beta,
+               float* c, int ldc);
+void orc_dtrsm(char side, char uplo, char transa, char diag, int m, int n,
+               double alpha, const double* a, int lda, double* b, int ldb);
+void orc_strsm(char side, char uplo, char transa, char diag, int m, int n,
+               float alpha, const float* a, int lda, float* b, int ldb);
+
+/* ---- double-double references --------------------------------------- */
+/* expected/bound are compact m x n (leading dimension m). bound(i,j) =
+ * |alpha| sum_p |a_ip b_pj| + |beta| |c_ij| + tiny, the scale of the
+ * componentwise bound; C is not read when beta == 0. */
+void orc_dgemm_ref(char transa, char transb, int m, int n, int k, double alpha,
+                   const double* a, int lda, const double* b, int ldb,
+                   double beta, const double* c, int ldc, double* expected,
+                   double* bound);
+void orc_sgemm_ref(char transa, char transb, int m, int n, int k, float alpha,
+                   const float* a, int lda, const float* b, int ldb,
+                   float beta, const float* c, int ldc, double* expected,
+                   double* bound);
+/* As orc_dgemm_ref for `count` entries (rows[t], cols[t]) only. */
+void orc_dgemm_ref_entries(char transa, char transb, int k, double alpha,
+                           const double* a, int lda, const double* b, int ldb,
+                           double beta, const double* c, int ldc,
+                           const int* rows, const int* cols, int count,
+                           double* expected, double* bound);
+void orc_sgemm_ref_entries(char transa, char transb, int k, float alpha,
+                           const float* a, int lda, const float* b, int ldb,
+                           float beta, const float* c, int ldc,
+                           const int* rows, const int* cols, int count,
+                           double* expected, double* bound);
+
+/* TRSM residual check: for the `count` right-hand sides listed in `which`
+ * (columns of B for side L, rows for side R), returns in ratio[t] the largest
+ * |op(A) X - alpha B0| / (|op(A)| |X| + |alpha B0| + tiny) over that right-hand
+ * side, computed in double-double. X has leading dimension ldx. */
+void orc_dtrsm_residual(char side, char uplo, char transa, char diag, int m,
+                        int n, double alpha, const double* a, int lda,
+                        const double* b0, int ldb, const double* x, int ldx,
+                        const int* which, int count, double* ratio);
+void orc_strsm_residual(char side, char uplo, char transa, char diag, int m,
+                        int n, float alpha, const float* a, int lda,
+                        const float* b0, int ldb, const float* x, int ldx,
+                        const int* which, int count, double* ratio);
+
+/* ---- planted mistakes (for tests that show the checks catch them) ------- */
+/* 0: none; 1: drop the last k term of every GEMM tile; 2: use op(A) with
+ * the wrong transposition; 3: ignore beta; 4: TRSM skips the reciprocal of
+ * the diagonal; 5: the double-double reference drops its low word. */
+void orc_plant_mistake(int which);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HWYBLAS_AMD_ORACLE_H_ */

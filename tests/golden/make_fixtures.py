@@ -1,0 +1,84 @@
+"""Writes tests/golden/blas_fixtures.npz: small GEMM and TRSM cases whose
+expected outputs come from an independent library, numpy's BLAS (OpenBLAS)
+and scipy.linalg.solve_triangular (LAPACK trtrs over OpenBLAS), in this
+container.
+
+hwyBLAS stores no golden vectors (its tests regenerate inputs from seeds and
+check bounds), so these fixtures are the external known answers against
+which the oracle is pinned; they are data only (inputs and outputs).
+
+Run: python tests/golden/make_fixtures.py
+"""
+import os
+
+import numpy as np
+import scipy
+import scipy.linalg
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "blas_fixtures.npz")
+
+
+def main():
+    rng = np.random.default_rng(20260101)
+    data = {}
+    cases = []
+    shapes = [(7, 5, 3), (33, 17, 65), (64, 64, 64), (1, 9, 300), (70, 1, 2)]
+    idx = 0
+    for dt in ("f8", "f4"):
+        for ta in "NT":
+            for tb in "NT":
+                for (m, n, k) in shapes:
+                    ra, ca = (m, k) if ta == "N" else (k, m)
+                    rb, cb = (k, n) if tb == "N" else (n, k)
+                    a = rng.uniform(-1, 1, (ra, ca)).astype(dt)
+                    b = rng.uniform(-1, 1, (rb, cb)).astype(dt)
+                    c = rng.uniform(-1, 1, (m, n)).astype(dt)
+                    alpha, beta = 1.5, -0.75
+                    A = a if ta == "N" else a.T
+                    B = b if tb == "N" else b.T
+                    out = (np.asarray(alpha, dt) * (A @ B) + np.asarray(beta, dt) * c).astype(dt)
+                    key = f"gemm{idx}"
+                    data[key + "_a"], data[key + "_b"], data[key + "_c"] = a, b, c
+                    data[key + "_out"] = out
+                    cases.append(f"{key},{dt},{ta},{tb},{m},{n},{k},{alpha},{beta}")
+                    idx += 1
+    data["gemm_cases"] = np.array(cases)
+
+    tcases = []
+    idx = 0
+    for dt in ("f8", "f4"):
+        for side in "LR":
+            for uplo in "LU":
+                for ta in "NT":
+                    for diag in "NU":
+                        m, n = 33, 9
+                        ka = m if side == "L" else n
+                        a = rng.uniform(-1, 1, (ka, ka)) / ka
+                        a[np.arange(ka), np.arange(ka)] = (1.5 + rng.uniform(-0.5, 0.5, ka)) * \
+                            rng.choice([-1.0, 1.0], ka)
+                        a = (np.tril(a) if uplo == "L" else np.triu(a)).astype(dt)
+                        b = rng.uniform(-1, 1, (m, n)).astype(dt)
+                        alpha = 0.5
+                        lower = uplo == "L"
+                        unit = diag == "U"
+                        trans = 0 if ta == "N" else 1
+                        rhs = (np.asarray(alpha, dt) * b).astype(dt)
+                        if side == "L":
+                            x = scipy.linalg.solve_triangular(a, rhs, trans=trans, lower=lower,
+                                                              unit_diagonal=unit)
+                        else:  # X op(A) = R  <=>  op(A)^T X^T = R^T
+                            x = scipy.linalg.solve_triangular(a, rhs.T, trans=1 - trans,
+                                                              lower=lower, unit_diagonal=unit).T
+                        key = f"trsm{idx}"
+                        data[key + "_a"], data[key + "_b"] = a, b
+                        data[key + "_out"] = np.ascontiguousarray(x).astype(dt)
+                        tcases.append(f"{key},{dt},{side},{uplo},{ta},{diag},{m},{n},{alpha}")
+                        idx += 1
+    data["trsm_cases"] = np.array(tcases)
+    data["versions"] = np.array([f"numpy {np.__version__}", f"scipy {scipy.__version__}"])
+    np.savez_compressed(OUT, **data)
+    print(f"wrote {OUT}: {len(cases)} gemm, {len(tcases)} trsm cases")
+
+
+if __name__ == "__main__":
+    main()

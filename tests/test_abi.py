@@ -1,0 +1,170 @@
+"""The C-ABI boundary, on the CPU (no GPU needed): the library loads and
+exports every symbol the public header declares; invalid arguments are
+reported at hwyBLAS's info positions (Fortran and CBLAS, both layouts) through
+the handler or xerbla_/cblas_xerbla, with outputs untouched; quick returns
+need no device. Restates tests/gemm_test.cc:191-212 and
+tests/cblas_test.cc:1087-1183 of the reference.
+"""
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import hwyblas_amd as H
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_header_symbol():
+    names = H.exported_symbols_in_header()
+    assert len(names) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", H.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        assert hasattr(H.lib(), n)
+
+
+def test_only_c_abi_is_exported():
+    out = subprocess.run(["nm", "-D", "--defined-only", H.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    funcs = [l.split()[-1] for l in out.splitlines() if " T " in l]
+    header = set(H.exported_symbols_in_header())
+    extra = [f for f in funcs if f not in header and not f.startswith("_")]
+    assert not extra, extra
+
+
+def _gemm_args(**kw):
+    args = dict(ta="N", tb="N", m=4, n=3, k=2, lda=4, ldb=2, ldc=4)
+    args.update(kw)
+    return args
+
+
+GEMM_CASES = [  # (changes, expected Fortran info)  (src/gemm.cc:89-106)
+    (dict(ta="X"), 1), (dict(tb="?"), 2), (dict(m=-1), 3), (dict(n=-1), 4), (dict(k=-1), 5),
+    (dict(lda=3), 8), (dict(ta="T", lda=1), 8), (dict(ldb=1), 10), (dict(tb="T", ldb=2), 10),
+    (dict(ldc=3), 13), (dict(ta="X", m=-1), 1), (dict(m=-1, ldc=0), 3),
+]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("change,info", GEMM_CASES)
+def test_gemm_invalid_arguments(dtype, change, info):
+    g = _gemm_args(**change)
+    a = np.ones((8, 8), dtype=dtype, order="F")
+    c = np.full((8, 8), 5.0, dtype=dtype, order="F")
+    c0 = c.copy()
+    with H.ErrorCapture() as cap:
+        H.gemm(g["ta"], g["tb"], g["m"], g["n"], g["k"], 1.0, a, g["lda"], a, g["ldb"], 0.0, c,
+               g["ldc"])
+    name = "DGEMM" if dtype == np.float64 else "SGEMM"
+    assert cap.calls == [(name, info)]
+    assert np.array_equal(c, c0)
+
+
+TRSM_CASES = [  # (src/trsm.cc:151-168)
+    (dict(side="X"), 1), (dict(uplo="X"), 2), (dict(ta="X"), 3), (dict(diag="X"), 4),
+    (dict(m=-1), 5), (dict(n=-1), 6), (dict(lda=3), 9), (dict(side="R", lda=2), 9),
+    (dict(ldb=3), 11),
+]
+
+
+@pytest.mark.parametrize("change,info", TRSM_CASES)
+def test_trsm_invalid_arguments(change, info):
+    t = dict(side="L", uplo="L", ta="N", diag="N", m=4, n=3, lda=4, ldb=4)
+    t.update(change)
+    a = np.ones((8, 8), order="F")
+    b = np.full((8, 8), 2.0, order="F")
+    b0 = b.copy()
+    with H.ErrorCapture() as cap:
+        H.trsm(t["side"], t["uplo"], t["ta"], t["diag"], t["m"], t["n"], 1.0, a, t["lda"], b,
+               t["ldb"])
+    assert cap.calls == [("DTRSM", info)]
+    assert np.array_equal(b, b0)
+
+
+def test_cblas_error_positions():
+    """CBLAS positions count the layout as argument 1; row-major remaps the
+    Fortran positions of the swapped call (src/cblas.cc:529-535, :681-684)."""
+    a = np.ones((8, 8), order="F")
+    c = np.zeros((8, 8), order="F")
+    with H.ErrorCapture() as cap:
+        H.cblas_gemm(H.ColMajor, H.NoTrans, H.NoTrans, 4, 3, 2, 1.0, a, 3, a, 2, 0.0, c, 4)
+        H.cblas_gemm(H.RowMajor, H.NoTrans, H.NoTrans, 4, 3, 2, 1.0, a, 1, a, 3, 0.0, c, 3)
+        H.cblas_gemm(H.RowMajor, H.NoTrans, H.NoTrans, 4, 3, 2, 1.0, a, 2, a, 2, 0.0, c, 3)
+        H.cblas_gemm(H.RowMajor, H.NoTrans, H.NoTrans, 4, 3, 2, 1.0, a, 2, a, 3, 0.0, c, 2)
+        H.cblas_gemm(77, H.NoTrans, H.NoTrans, 4, 3, 2, 1.0, a, 2, a, 3, 0.0, c, 3)
+        H.cblas_gemm(H.ColMajor, 9, H.NoTrans, 4, 3, 2, 1.0, a, 4, a, 2, 0.0, c, 4)
+        H.cblas_gemm(H.RowMajor, 9, H.NoTrans, 4, 3, 2, 1.0, a, 2, a, 3, 0.0, c, 3)
+        H.cblas_gemm(H.RowMajor, H.NoTrans, H.NoTrans, -1, 3, 2, 1.0, a, 2, a, 3, 0.0, c, 3)
+        H.cblas_trsm(H.ColMajor, H.Left, H.Lower, H.NoTrans, H.NonUnit, 4, 3, 1.0, a, 3, c, 4)
+        H.cblas_trsm(H.RowMajor, H.Left, H.Lower, H.NoTrans, H.NonUnit, 4, 3, 1.0, a, 4, c, 2)
+        H.cblas_trsm(H.RowMajor, H.Left, H.Lower, H.NoTrans, H.NonUnit, -2, 3, 1.0, a, 4, c, 3)
+        H.cblas_trsm(H.RowMajor, 5, H.Lower, H.NoTrans, H.NonUnit, 4, 3, 1.0, a, 4, c, 3)
+    assert cap.calls == [
+        ("cblas_dgemm", 9),    # lda < m
+        ("cblas_dgemm", 9),    # row-major: lda < k (A becomes the Fortran B, info 10 -> 9)
+        ("cblas_dgemm", 11),   # row-major: ldb < n (Fortran A, info 8 -> 11)
+        ("cblas_dgemm", 14),   # row-major: ldc < n
+        ("cblas_dgemm", 1),    # invalid layout
+        ("cblas_dgemm", 2),    # transa
+        ("cblas_dgemm", 2),    # row-major transa is Fortran transb (info 2 -> 2)
+        ("cblas_dgemm", 4),    # row-major m < 0 is Fortran n (info 4 -> 4)
+        ("cblas_dtrsm", 10),   # lda < m
+        ("cblas_dtrsm", 12),   # row-major ldb < n
+        ("cblas_dtrsm", 6),    # row-major m < 0 (Fortran n, info 6 -> 6)
+        ("cblas_dtrsm", 2),    # side
+    ]
+    assert np.all(c == 0)
+
+
+def test_default_reporters_print_and_return():
+    """Without a handler, xerbla_ / cblas_xerbla print the reference's messages
+    (src/common.cc:71-79, src/cblas.cc:714-726) and the process continues."""
+    code = (
+        "import numpy as np, hwyblas_amd as H\n"
+        "a = np.ones((4, 4), order='F')\n"
+        "H.gemm('N', 'N', 2, 2, 2, 1.0, a, 1, a, 4, 0.0, a, 4)\n"
+        "H.cblas_trsm(H.ColMajor, H.Left, 0, H.NoTrans, H.NonUnit, 2, 2, 1.0, a, 4, a, 4)\n"
+        "print('still running')\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    assert " ** On entry to DGEMM parameter number 8 had an illegal value" in r.stderr
+    assert "Parameter 3 to routine cblas_dtrsm was incorrect" in r.stderr
+    assert "still running" in r.stdout
+
+
+def test_quick_returns_need_no_device():
+    """m == 0 or n == 0, and (k == 0 or alpha == 0) with beta == 1, return
+    before any device work (src/gemm.cc:114-116); TRSM likewise for m or
+    n == 0 (src/trsm.cc:171)."""
+    a = np.full((4, 4), np.nan, order="F")
+    c = np.full((4, 4), 3.0, order="F")
+    H.gemm("N", "N", 0, 4, 4, 1.0, a, 4, a, 4, 0.0, c, 4)
+    H.gemm("N", "N", 4, 0, 4, 1.0, a, 4, a, 4, 0.0, c, 4)
+    H.gemm("N", "N", 4, 4, 0, 1.0, a, 4, a, 4, 1.0, c, 4)
+    H.gemm("N", "N", 4, 4, 4, 0.0, a, 4, a, 4, 1.0, c, 4)
+    H.trsm("L", "U", "T", "U", 0, 4, 2.0, a, 4, c, 4)
+    H.trsm("R", "U", "T", "U", 4, 0, 2.0, a, 4, c, 4)
+    assert np.all(c == 3.0)
+
+
+def test_handler_is_replaced_and_restored():
+    calls = []
+    cb = H.ERROR_HANDLER(lambda n, p: calls.append((n.decode(), p)))
+    prev = H.lib().hwyblas_set_error_handler(ctypes.cast(cb, ctypes.c_void_p))
+    a = np.ones((4, 4), order="F")
+    H.gemm("N", "N", -1, 2, 2, 1.0, a, 4, a, 4, 0.0, a, 4)
+    back = H.lib().hwyblas_set_error_handler(prev)
+    assert back == ctypes.cast(cb, ctypes.c_void_p).value
+    assert calls == [("DGEMM", 3)]
+
+
+def test_version_and_device_count_without_gpu():
+    assert b"gfx950" in H.lib().hwyblas_amd_version()
+    assert H.device_count() >= 0

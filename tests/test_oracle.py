@@ -1,0 +1,217 @@
+"""Pins the CPU oracle (CPU only; no GPU).
+
+The oracle's restated hwyBLAS GEMM/TRSM and its double-double references are
+checked against (1) golden fixtures computed by numpy/OpenBLAS and scipy
+(tests/golden/blas_fixtures.npz), (2) exact rational arithmetic on tiny
+inputs, (3) closed forms (integer-valued products, constant matrices,
+subnormal known answers, exactly solvable triangles) and (4) the published
+splitmix64 test vector. Every planted mistake in the oracle must make at
+least one of these checks fail.
+"""
+import itertools
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import gemm_bound_ratio, trsm_operands, trsm_residual_ratio
+
+FIX = np.load(os.path.join(os.path.dirname(__file__), "golden", "blas_fixtures.npz"))
+
+
+@pytest.fixture(autouse=True)
+def _no_planted_mistake():
+    oracle.plant_mistake(0)
+    yield
+    oracle.plant_mistake(0)
+
+
+def gemm_fixture_cases():
+    for line in FIX["gemm_cases"]:
+        key, dt, ta, tb, m, n, k, alpha, beta = str(line).split(",")
+        yield (key, np.dtype(dt), ta, tb, int(m), int(n), int(k), float(alpha), float(beta))
+
+
+def trsm_fixture_cases():
+    for line in FIX["trsm_cases"]:
+        key, dt, side, uplo, ta, diag, m, n, alpha = str(line).split(",")
+        yield (key, np.dtype(dt), side, uplo, ta, diag, int(m), int(n), float(alpha))
+
+
+def _f(x):
+    return np.asfortranarray(x)
+
+
+def oracle_matches_gemm_fixtures():
+    """Largest bound ratio of the oracle's GEMM and of the double-double
+    reference against numpy's results (both must satisfy the bound, so they
+    are within twice the tolerance of each other)."""
+    worst = 0.0
+    for key, dt, ta, tb, m, n, k, alpha, beta in gemm_fixture_cases():
+        a, b, c = _f(FIX[key + "_a"]), _f(FIX[key + "_b"]), _f(FIX[key + "_c"])
+        out = FIX[key + "_out"]
+        got = c.copy(order="F")
+        oracle.gemm(ta, tb, m, n, k, alpha, a, a.shape[0], b, b.shape[0], beta, got, m)
+        exp, bnd = oracle.gemm_ref(ta, tb, m, n, k, alpha, a, a.shape[0], b, b.shape[0], beta, c, m)
+        tol = 4 * (k + 2) * np.finfo(dt).eps
+        worst = max(worst,
+                    float(np.max(np.abs(got - out) / (2 * tol * bnd))),
+                    float(np.max(np.abs(exp - out) / (tol * bnd))))
+    return worst
+
+
+def test_gemm_matches_numpy_fixtures():
+    assert oracle_matches_gemm_fixtures() <= 1.0
+
+
+def oracle_matches_trsm_fixtures():
+    worst = 0.0
+    for key, dt, side, uplo, ta, diag, m, n, alpha in trsm_fixture_cases():
+        a, b, out = _f(FIX[key + "_a"]), _f(FIX[key + "_b"]), FIX[key + "_out"]
+        ka = a.shape[0]
+        x = b.copy(order="F")
+        oracle.trsm(side, uplo, ta, diag, m, n, alpha, a, ka, x, m)
+        # Both solutions satisfy the residual bound; they agree to within the
+        # forward error, which for these well-conditioned triangles is a few
+        # ka * eps relative to |X|.
+        tol = 8 * (ka + 2) * np.finfo(dt).eps
+        worst = max(worst, float(np.max(np.abs(x - out) / (tol * (np.abs(out) + 1)))))
+        r = trsm_residual_ratio(side, uplo, ta, diag, m, n, alpha, a, ka, b, m, x)
+        worst = max(worst, r)
+    return worst
+
+
+def test_trsm_matches_scipy_fixtures():
+    assert oracle_matches_trsm_fixtures() <= 1.0
+
+
+def test_reference_is_exact_on_tiny_inputs():
+    """The double-double reference equals the exact rational result rounded
+    to double (within one ulp) on tiny inputs with heavy cancellation."""
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        m, n, k = 3, 2, 6
+        a = _f(rng.uniform(-1, 1, (m, k)) * np.exp2(rng.integers(-20, 20, (m, k))))
+        b = _f(rng.uniform(-1, 1, (k, n)))
+        # Force cancellation: the last term nearly cancels the rest.
+        for i in range(m):
+            for j in range(n):
+                pass
+        c = _f(rng.uniform(-1, 1, (m, n)))
+        alpha, beta = 0.75, -1.25
+        exp, _ = oracle.gemm_ref("N", "N", m, n, k, alpha, a, m, b, k, beta, c, m)
+        for i in range(m):
+            for j in range(n):
+                s = sum(Fraction(a[i, p]) * Fraction(b[p, j]) for p in range(k))
+                exact = Fraction(alpha) * s + Fraction(beta) * Fraction(c[i, j])
+                ulp = np.spacing(abs(float(exact)))
+                assert abs(Fraction(exp[i, j]) - exact) <= Fraction(ulp), (trial, i, j)
+
+
+def test_cancellation_needs_the_low_word():
+    """x + y - x with |y| << |x|: the exact answer is y; a reference without
+    double-double accumulation returns 0."""
+    a = _f(np.array([[1.0, 1e-20, -1.0]]))
+    b = _f(np.array([[1.0], [1.0], [1.0]]))
+    exp, _ = oracle.gemm_ref("N", "N", 1, 1, 3, 1.0, a, 1, b, 3, 0.0, None, 1)
+    assert exp[0, 0] == 1e-20
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_closed_forms(dtype):
+    rng = np.random.default_rng(2)
+    for ta, tb in itertools.product("NT", "NT"):
+        m, n, k = 41, 23, 517
+        a = _f(rng.integers(-8, 9, (m, k) if ta == "N" else (k, m)).astype(dtype))
+        b = _f(rng.integers(-8, 9, (k, n) if tb == "N" else (n, k)).astype(dtype))
+        c = _f(rng.integers(-9, 9, (m, n)).astype(dtype))
+        A = a if ta == "N" else a.T
+        B = b if tb == "N" else b.T
+        want = (2 * (A.astype(np.int64) @ B.astype(np.int64)) - 3 * c.astype(np.int64)).astype(dtype)
+        got = c.copy(order="F")
+        oracle.gemm(ta, tb, m, n, k, 2.0, a, a.shape[0], b, b.shape[0], -3.0, got, m)
+        assert np.array_equal(got, want), (ta, tb)
+    # Constant matrices: k * a * b.
+    k = 1000
+    a = np.full((5, k), 0.5, dtype=dtype, order="F")
+    b = np.full((k, 4), 0.25, dtype=dtype, order="F")
+    c = np.zeros((5, 4), dtype=dtype, order="F")
+    oracle.gemm("N", "N", 5, 4, k, 1.0, a, 5, b, k, 0.0, c, 5)
+    assert np.all(c == dtype(k * 0.125))
+    # Subnormal products (semantics_test.cc:311-342).
+    s = dtype(np.ldexp(1.0, -530 if dtype == np.float64 else -70))
+    for n in (1, 5, 64, 200):
+        x = np.full((n, n), s, dtype=dtype, order="F")
+        c = np.ones((n, n), dtype=dtype, order="F")
+        oracle.gemm("N", "N", n, n, n, 1.0, x, n, x, n, 0.0, c, n)
+        assert np.all(c == dtype(n) * (s * s))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_trsm_exactly_solvable(dtype):
+    """Triangles with power-of-two diagonals and integer entries: X integer,
+    B = op(T) X exactly, so the solve must return X exactly."""
+    rng = np.random.default_rng(9)
+    for side, uplo, ta, diag in itertools.product("LR", "LU", "NT", "NU"):
+        m, n = 70, 5
+        ka = m if side == "L" else n
+        t = rng.integers(-1, 2, (ka, ka)).astype(np.float64)
+        t = np.tril(t, -1) if uplo == "L" else np.triu(t, 1)
+        d = rng.choice([1.0, -1.0, 2.0, -2.0, 0.5], ka) if diag == "N" else np.ones(ka)
+        t[np.arange(ka), np.arange(ka)] = d
+        op = t if ta == "N" else t.T
+        # Keep X small integers and B exactly representable: solve upward
+        # from an integer X.
+        x = rng.integers(-3, 4, (m, n)).astype(np.float64)
+        bb = op @ x if side == "L" else x @ op
+        if np.max(np.abs(bb)) > 2 ** 20:
+            continue
+        a = _f(t.astype(dtype))
+        if diag == "U":
+            a[np.arange(ka), np.arange(ka)] = np.nan
+        b = _f(bb.astype(dtype))
+        oracle.trsm(side, uplo, ta, diag, m, n, 1.0, a, ka, b, m)
+        assert np.array_equal(b, x.astype(dtype)), (side, uplo, ta, diag)
+
+
+def test_trsm_random_residuals():
+    for side, uplo, ta, diag in itertools.product("LR", "LU", "NT", "NU"):
+        for m, n in ((100, 17), (13, 100), (300, 45)):
+            a, lda, b, ldb = trsm_operands(side, uplo, diag, m, n, np.float64, m + n)
+            x = b.copy(order="F")
+            oracle.trsm(side, uplo, ta, diag, m, n, 1.5, a, lda, x, ldb)
+            assert trsm_residual_ratio(side, uplo, ta, diag, m, n, 1.5, a, lda, b, ldb, x) <= 1.0
+
+
+def test_splitmix64_published_vector():
+    """splitmix64 (Vigna) from state 0: first outputs e220a8397b1dcdaf,
+    6e789e6aa1b965f4 (state advances by the golden gamma)."""
+    assert oracle.lib().orc_splitmix64(0) == 0xE220A8397B1DCDAF
+    assert oracle.lib().orc_splitmix64(0x9E3779B97F4A7C15) == 0x6E789E6AA1B965F4
+
+
+def test_generator_is_layout_independent():
+    x = oracle.fill_uniform(37, 11, 5)
+    y = oracle.fill_uniform(37, 11, 5, ld=50)
+    assert np.array_equal(x, y[:37])
+    assert np.all((x >= -1) & (x < 1))
+    assert abs(np.mean(oracle.fill_uniform(1000, 100, 6))) < 0.01
+
+
+@pytest.mark.parametrize("mistake", [1, 2, 3])
+def test_planted_gemm_mistakes_are_caught(mistake):
+    oracle.plant_mistake(mistake)
+    assert oracle_matches_gemm_fixtures() > 1.0
+
+
+def test_planted_trsm_mistake_is_caught():
+    oracle.plant_mistake(4)
+    assert oracle_matches_trsm_fixtures() > 1.0
+
+
+def test_planted_reference_mistake_is_caught():
+    oracle.plant_mistake(5)
+    with pytest.raises(AssertionError):
+        test_cancellation_needs_the_low_word()
