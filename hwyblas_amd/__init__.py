@@ -54,6 +54,13 @@ def _declare(L):
         fn(f"cblas_{pre}trsm", None, [_i, _i, _i, _i, _i, _i, _i, s, _p, _i, _p, _i])
         fn(f"hwyblas_{pre}gemm_dev", _i, [_c, _c, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i, _p])
         fn(f"hwyblas_{pre}trsm_dev", _i, [_c, _c, _c, _c, _i, _i, s, _p, _i, _p, _i, _p])
+        fn(f"hwyblas_{pre}gemm_dev_cfg", _i, [_i, _c, _c, _i, _i, _i, s, _p, _i, _p, _i, s, _p,
+                                             _i, _p])
+    fn("hwyblas_dgemm_emulated_dev", _i, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i,
+                                          _i, _p, _pi])
+    fn("hwyblas_gemm_config_count", _i, [])
+    fn("hwyblas_gemm_config_info", _i, [_i, _pi, _pi, _pi, _pi])
+    fn("hwyblas_gemm_default_config", _i, [_i, _c, _c, _i, _i, _i])
     fn("xerbla_", None, [ctypes.c_char_p, _pi, _sz])
     fn("hwyblas_set_error_handler", _p, [_p])
     fn("hwyblas_amd_device_count", _i, [])
@@ -72,6 +79,8 @@ def _declare(L):
     fn("hwyblas_amd_event_elapsed_ms", _f, [_p, _p])
     fn("hwyblas_amd_fill_uniform", None, [_p, _i, _i64, _i64, _i64, _u64, _d, _d, _p])
     fn("hwyblas_amd_version", ctypes.c_char_p, [])
+    fn("hwyblas_amd_profile_enable", None, [_i])
+    fn("hwyblas_amd_profile_ms", _f, [_i])
 
 
 def lib():
@@ -150,6 +159,34 @@ def gemm_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None, 
     f = getattr(lib(), f"hwyblas_{_pre(dtype)}gemm_dev")
     return f(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta,
              _ptr(c), ldc, stream)
+
+
+def gemm_dev_cfg(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None,
+                 dtype=None):
+    dtype = dtype or c.dtype
+    f = getattr(lib(), f"hwyblas_{_pre(dtype)}gemm_dev_cfg")
+    return f(cfg, ta.encode(), tb.encode(), m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta,
+             _ptr(c), ldc, stream)
+
+
+def dgemm_emulated_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nmoduli=0,
+                       stream=None):
+    """Emulated DGEMM (INT8 Ozaki II); returns (info, moduli used)."""
+    used = ctypes.c_int(0)
+    info = lib().hwyblas_dgemm_emulated_dev(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(a),
+                                            lda, _ptr(b), ldb, beta, _ptr(c), ldc, nmoduli,
+                                            stream, ctypes.byref(used))
+    return info, used.value
+
+
+def gemm_configs():
+    """[(bm, bn, bk, waves)] of the plain GEMM kernel's configurations."""
+    out = []
+    for i in range(lib().hwyblas_gemm_config_count()):
+        v = [ctypes.c_int() for _ in range(4)]
+        lib().hwyblas_gemm_config_info(i, *[ctypes.byref(x) for x in v])
+        out.append(tuple(x.value for x in v))
+    return out
 
 
 def trsm_dev(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, stream=None, dtype=None):

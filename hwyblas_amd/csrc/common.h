@@ -113,6 +113,18 @@ template <typename T>
 void GemmDevice(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha,
                 const T* a, int64_t lda, const T* b, int64_t ldb, T beta,
                 T* c, int64_t ldc, hipStream_t stream);
+template <>
+void GemmDevice<double>(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                        const double* a, int64_t lda, const double* b, int64_t ldb,
+                        double beta, double* c, int64_t ldc, hipStream_t stream);
+template <>
+void GemmDevice<float>(bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
+                       const float* a, int64_t lda, const float* b, int64_t ldb, float beta,
+                       float* c, int64_t ldc, hipStream_t stream);
+
+// Tile configuration GemmDevice uses for a shape (an index into the table of
+// gemm_configs.h).
+int ChooseGemmConfig(int dtype_bytes, bool ta, bool tb, int64_t m, int64_t n, int64_t k);
 
 // C := beta C over m x n; beta == 0 writes zeros without reading C.
 template <typename T>

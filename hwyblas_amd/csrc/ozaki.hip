@@ -1,0 +1,807 @@
+// Emulated DGEMM: FP64 in, FP64 out, computed on the INT8 matrix cores
+// (v_mfma_i32_32x32x32_i8) by the Ozaki scheme II (Ozaki, Uchino, Imamura,
+// "Ozaki Scheme II: A GEMM-oriented emulation of floating-point matrix
+// multiplication using an integer modular technique", 2025), restated here:
+//
+//  1. Scaling. Row i of op(A) is scaled by 2^sa_i and column j of op(B) by
+//     2^sb_j and rounded to integers A', B' with |A'| <= 2^bA, |B'| <= 2^bB,
+//     bA + bB chosen so that k 2^(bA+bB) < P / 2, P = p_1 ... p_N. The exact
+//     integer product C' = A' B' is then the unique integer of (-P/2, P/2)
+//     with its residues modulo every p_l.
+//  2. Residues. For N pairwise coprime moduli p_l <= 256, A' and B' are
+//     reduced to symmetric residues in [-128, 127] (int8 "slices").
+//  3. Products. C'_l = A'_l B'_l is exact in INT32 (|C'_l| <= k 128^2 < 2^31
+//     for k < 131072) and is reduced mod p_l in the GEMM epilogue to one byte
+//     per element and modulus.
+//  4. Reconstruction (Chinese remainder theorem). With M_l = P / p_l and
+//     y_l = M_l^-1 mod p_l, C'/P = sum_l c_l y_l / p_l (mod 1). The fractions
+//     y_l / p_l are held as 192-bit fixed-point numbers, so the sum mod 1 is
+//     an exact wrap-around integer sum (error < N 256 2^-192 of P); its
+//     centred value times P gives C' to about one rounding. Finally
+//     C = alpha 2^-(sa_i + sb_j) C' + beta C with hwyBLAS's Store rules.
+// The number of moduli is given by the caller, or chosen from the operands:
+// a sampled estimate of how much the per-row fixed-point truncation can
+// perturb C relative to |A||B| sets bA, bB, hence N, so that the error stays
+// at the level of native FP64 accumulation (see ChooseModuli). Inputs with
+// NaN or Inf, which the integer scheme cannot carry, fall back to the native
+// FP64 kernel, as does k >= 131072.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+
+namespace hwy_amd {
+namespace ozaki {
+namespace {
+
+constexpr int kMaxModuli = 24;
+constexpr int kLimbs = 6;  // 192-bit CRT fractions
+// Pairwise coprime, largest first: 2^8, 3*5*17, 11*23, 251, 13*19, primes,
+// 7*31, primes.
+constexpr int kModuli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233,
+                                     229, 227, 223, 217, 211, 199, 197, 193,
+                                     191, 181, 179, 173, 167, 163, 157, 151};
+
+struct Crt {
+  int nmod;
+  int p[kMaxModuli];
+  double inv_p[kMaxModuli];
+  uint32_t frac[kMaxModuli][kLimbs];  // floor(2^192 y_l / p_l), little endian
+  double P;                           // prod p_l, rounded
+};
+
+// ---------------------------------------------------------------------------
+// Host-side constants
+
+int64_t ModInverse(int64_t a, int64_t m) {  // a^-1 mod m, gcd(a, m) = 1
+  int64_t t = 0, nt = 1, r = m, nr = a % m;
+  while (nr != 0) {
+    const int64_t q = r / nr;
+    int64_t tmp = t - q * nt; t = nt; nt = tmp;
+    tmp = r - q * nr; r = nr; nr = tmp;
+  }
+  return t < 0 ? t + m : t;
+}
+
+Crt MakeCrt(int nmod) {
+  Crt c{};
+  c.nmod = nmod;
+  double logp = 0;
+  c.P = 1;
+  for (int l = 0; l < nmod; ++l) {
+    c.p[l] = kModuli[l];
+    c.inv_p[l] = 1.0 / kModuli[l];
+    c.P *= kModuli[l];
+    logp += std::log2(double(kModuli[l]));
+  }
+  (void)logp;
+  for (int l = 0; l < nmod; ++l) {
+    const int64_t p = kModuli[l];
+    // M_l mod p_l, then y_l = (M_l mod p_l)^-1 mod p_l.
+    int64_t ml = 1;
+    for (int q = 0; q < nmod; ++q)
+      if (q != l) ml = (ml * kModuli[q]) % p;
+    const int64_t y = ModInverse(ml, p);
+    // floor(2^192 y / p) by long division in base 2^32, most significant
+    // limb first.
+    uint64_t rem = uint64_t(y);
+    for (int t = kLimbs - 1; t >= 0; --t) {
+      const uint64_t num = rem << 32;
+      c.frac[l][t] = uint32_t(num / uint64_t(p));
+      rem = num % uint64_t(p);
+    }
+  }
+  return c;
+}
+
+double Log2P(int nmod) {
+  double s = 0;
+  for (int l = 0; l < nmod; ++l) s += std::log2(double(kModuli[l]));
+  return s;
+}
+
+// Total fixed-point bits bA + bB that N moduli allow for inner dimension k:
+// k 2^(bA+bB) < P / 2.
+int TotalBits(int nmod, int64_t k) {
+  return int(std::floor(Log2P(nmod) - 1.0 - std::log2(double(k)) - 1e-9));
+}
+
+// ---------------------------------------------------------------------------
+// Device helpers
+
+__device__ __forceinline__ int SymMod(double x, int p, double inv_p) {
+  // x an integer with |x| < 2^52: x - p rint(x / p) is exact, then centred
+  // into [-p/2, p/2).
+  const double q = rint(x * inv_p);
+  int r = int(__builtin_fma(-q, double(p), x));
+  const int half = p >> 1;
+  if (r >= half) r -= p;
+  if (r < -half) r += p;
+  return r;
+}
+
+__device__ __forceinline__ int ModU(int64_t x, int p, double inv_p) {
+  // x mod p in [0, p) for |x| < 2^52.
+  const double q = floor(double(x) * inv_p);
+  int64_t r = x - int64_t(q) * p;
+  if (r < 0) r += p;
+  if (r >= p) r -= p;
+  return int(r);
+}
+
+__device__ __forceinline__ int MulMod(int a, int b, int p, double inv_p) {
+  return ModU(int64_t(a) * b, p, inv_p);
+}
+
+// Symmetric residue of the integer v = x 2^s (x a double; the rounded value
+// when v is not an integer).
+__device__ __forceinline__ int ResidueOf(double v, int p, double inv_p) {
+  if (fabs(v) < 0x1p52) return SymMod(rint(v), p, inv_p);
+  // v = mi 2^E exactly, mi an integer below 2^53, E >= 0.
+  int e;
+  const double m = frexp(v, &e);
+  const double mi = ldexp(m, 53);
+  int E = e - 53;
+  int r = ModU(int64_t(mi), p, inv_p);
+  int base = 2 % p, pw = 1;
+  while (E > 0) {
+    if (E & 1) pw = MulMod(pw, base, p, inv_p);
+    base = MulMod(base, base, p, inv_p);
+    E >>= 1;
+  }
+  r = MulMod(r, pw, p, inv_p);
+  if (r >= (p >> 1)) r -= p;
+  return r;
+}
+
+__global__ void FillU64(uint64_t* x, int64_t n, uint64_t v) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    x[i] = v;
+}
+
+// Max |x(r, p)| per row r of a strided operand (element at x[r*rs + p*cs]),
+// as the bit pattern of a non-negative double (which orders like the
+// value; NaN orders above Inf). Rows contiguous (rs == 1): one thread per
+// row over a chunk of p. p contiguous (cs == 1): one wave per row.
+template <bool ROWS_CONTIG>
+__global__ void RowAbsMaxKernel(const double* x, int64_t ld, int64_t R, int64_t K,
+                                int64_t pchunk, uint64_t* out) {
+  if (ROWS_CONTIG) {
+    const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const int64_t p0 = int64_t(blockIdx.y) * pchunk;
+    const int64_t p1 = min(K, p0 + pchunk);
+    uint64_t mx = 0;
+    for (int64_t p = p0; p < p1; ++p) {
+      const uint64_t b = __double_as_longlong(fabs(x[r + p * ld]));
+      mx = b > mx ? b : mx;
+    }
+    if (p1 > p0) atomicMax(reinterpret_cast<unsigned long long*>(out + r), mx);
+  } else {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t r = int64_t(blockIdx.x) * (blockDim.x >> 6) + wave;
+    if (r >= R) return;
+    const int64_t p0 = int64_t(blockIdx.y) * pchunk;
+    const int64_t p1 = min(K, p0 + pchunk);
+    uint64_t mx = 0;
+    for (int64_t p = p0 + lane; p < p1; p += 64) {
+      const uint64_t b = __double_as_longlong(fabs(x[r * ld + p]));
+      mx = b > mx ? b : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint64_t o = __shfl_xor(mx, off, 64);
+      mx = o > mx ? o : mx;
+    }
+    if (lane == 0 && p1 > p0) atomicMax(reinterpret_cast<unsigned long long*>(out + r), mx);
+  }
+}
+
+// Per-row scale exponents s_r = bits - E_r, rowmax_r = f 2^E_r, f in
+// [0.5, 1) (0 for zero rows), and the global max bit pattern in *gmax.
+__global__ void ScaleExpKernel(const uint64_t* rowmax, int64_t R, int bits, int* s,
+                               uint64_t* gmax) {
+  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint64_t b = 0;
+  if (r < R) {
+    b = rowmax[r];
+    const double v = __longlong_as_double(b);
+    int e = 0;
+    if (v > 0 && b < 0x7FF0000000000000ull) frexp(v, &e);
+    s[r] = (v > 0) ? bits - e : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = __shfl_xor(b, off, 64);
+    b = o > b ? o : b;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned long long*>(gmax), b);
+}
+
+// Slices: Y[l][r][p] (row stride KP, plane stride RP * KP) = symmetric
+// residue mod p_l of round(x(r, p) 2^s_r), zero for r >= R or p >= K. One
+// block per 64 x 64 tile; the tile passes through LDS so that the global
+// reads are coalesced along whichever dimension of x is contiguous and the
+// byte writes along p.
+template <bool ROWS_CONTIG>
+__global__ void __launch_bounds__(256)
+    SliceKernel(const double* x, int64_t ld, int64_t R, int64_t K, int64_t RP, int64_t KP,
+                const int* s, Crt crt, int8_t* Y) {
+  __shared__ double T[64][65];
+  const int tid = threadIdx.x;
+  const int64_t r0 = int64_t(blockIdx.y) * 64, p0 = int64_t(blockIdx.x) * 64;
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    int rr, pp;
+    if (ROWS_CONTIG) { rr = idx & 63; pp = idx >> 6; }
+    else { pp = idx & 63; rr = idx >> 6; }
+    const int64_t r = r0 + rr, p = p0 + pp;
+    double v = 0;
+    if (r < R && p < K) v = ROWS_CONTIG ? x[r + p * ld] : x[r * ld + p];
+    T[rr][pp] = v;
+  }
+  __syncthreads();
+  // Thread -> (row rr, 4 consecutive p): writes one 32-bit word per modulus.
+  const int rr = tid >> 4, pq = (tid & 15) * 4;
+  const int64_t plane = RP * KP;
+  for (int rb = 0; rb < 64; rb += 16) {
+    const int row = rr + rb;
+    const int64_t r = r0 + row;
+    if (r >= RP) continue;
+    const int sc = (r < R) ? s[r] : 0;
+    double v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = ldexp(T[row][pq + q], sc);
+    int8_t* dst = Y + r * KP + p0 + pq;
+    for (int l = 0; l < crt.nmod; ++l) {
+      const int p = crt.p[l];
+      const double ip = crt.inv_p[l];
+      uint32_t word = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int res = (v[q] == 0.0) ? 0 : ResidueOf(v[q], p, ip);
+        word |= uint32_t(uint8_t(int8_t(res))) << (8 * q);
+      }
+      *reinterpret_cast<uint32_t*>(dst + l * plane) = word;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// INT8 GEMM per modulus: R_l = (A_l B_l^T) mod p_l, with A_l (MP x KP) and
+// B_l (NP x KP) row-major int8 planes (k contiguous, zero padded) and R_l a
+// column-major MP x NP byte plane. blockIdx.y selects the modulus.
+//
+// v_mfma_i32_32x32x32_i8: lane l supplies 16 consecutive k bytes of row
+// l % 32 (k = 16 (l / 32) + byte within the 32-k step) for A, likewise of
+// column l % 32 for B; result element r of lane l is column l % 32, row
+// 8 (r / 4) + 4 (l / 32) + r % 4.
+// LDS holds each BM x BK (BN x BK) slice as rows of BK bytes in 16-byte
+// chunks, chunk c of row r stored at c ^ ((r / RPB) % CH) (CH chunks per
+// row, RPB rows per 256-byte bank row) so the MFMA operand reads of 16
+// consecutive rows hit distinct banks.
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+
+struct ModGemmArgs {
+  const int8_t* a;
+  const int8_t* b;
+  uint8_t* r;
+  int64_t a_plane, b_plane, r_plane;
+  int64_t kp, ldr;
+  int tiles_m, tiles_n;
+  int p[kMaxModuli];
+  double inv_p[kMaxModuli];
+};
+
+template <int BM, int BN, int BK, int WAVES_M, int WAVES_N>
+__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmKernel(ModGemmArgs g) {
+  constexpr int NT = 64 * WAVES_M * WAVES_N;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+  constexpr int TM = WM / 32, TN = WN / 32;
+  constexpr int CH = BK / 16;            // 16-byte chunks per row
+  constexpr int RPB = 16 / CH;           // rows per 256-byte bank row
+  constexpr int A_CHUNKS = BM * CH / NT; // chunks per thread
+  constexpr int B_CHUNKS = BN * CH / NT;
+  static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "load shape");
+  static_assert(WM % 32 == 0 && WN % 32 == 0 && BK % 32 == 0, "tile shape");
+  __shared__ i32x4 As[2][BM * CH];
+  __shared__ i32x4 Bs[2][BN * CH];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+  const int l = blockIdx.y;
+  // XCD-aware raster (as in the FP64/FP32 kernel).
+  int tm, tn;
+  {
+    const int nblk = gridDim.x, bid = blockIdx.x;
+    const int xcd = bid % 8, idx = bid / 8, per = nblk / 8, rem = nblk % 8;
+    const int lid = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    const int group = 8 * g.tiles_n;
+    const int first_m = (lid / group) * 8;
+    const int gm = min(g.tiles_m - first_m, 8);
+    tm = first_m + (lid % group) % gm;
+    tn = (lid % group) / gm;
+  }
+  const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
+  const int8_t* A = g.a + l * g.a_plane + m0 * g.kp;
+  const int8_t* B = g.b + l * g.b_plane + n0 * g.kp;
+
+  i32x4 ra[A_CHUNKS], rb[B_CHUNKS];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < A_CHUNKS; ++q) {
+      const int idx = tid + q * NT, row = idx / CH, c = idx % CH;
+      ra[q] = *reinterpret_cast<const i32x4*>(A + row * g.kp + k0 + c * 16);
+    }
+#pragma unroll
+    for (int q = 0; q < B_CHUNKS; ++q) {
+      const int idx = tid + q * NT, row = idx / CH, c = idx % CH;
+      rb[q] = *reinterpret_cast<const i32x4*>(B + row * g.kp + k0 + c * 16);
+    }
+  };
+  auto swz = [](int row, int c) { return row * CH + (c ^ ((row / RPB) % CH)); };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < A_CHUNKS; ++q) {
+      const int idx = tid + q * NT, row = idx / CH, c = idx % CH;
+      As[buf][swz(row, c)] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < B_CHUNKS; ++q) {
+      const int idx = tid + q * NT, row = idx / CH, c = idx % CH;
+      Bs[buf][swz(row, c)] = rb[q];
+    }
+  };
+
+  i32x16 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = i32x16{};
+
+  const int ktiles = int(g.kp / BK);
+  load(0);
+  store(0);
+  __syncthreads();
+  const int lr = lane & 31, lh = lane >> 5;
+  for (int kt = 0; kt < ktiles; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < ktiles;
+    if (more) load(int64_t(kt + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK / 32; ++ks) {
+      const int c = ks * 2 + lh;
+      i32x4 af[TM], bf[TN];
+#pragma unroll
+      for (int ti = 0; ti < TM; ++ti) af[ti] = As[cur][swz(wm0 + ti * 32 + lr, c)];
+#pragma unroll
+      for (int tj = 0; tj < TN; ++tj) bf[tj] = Bs[cur][swz(wn0 + tj * 32 + lr, c)];
+#pragma unroll
+      for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TN; ++tj)
+          acc[ti][tj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
+    }
+    if (more) store(cur ^ 1);
+    __syncthreads();
+  }
+
+  // Epilogue: reduce mod p_l into bytes; four consecutive rows of a column
+  // form one 32-bit store (the plane is column-major, rows contiguous).
+  const int p = g.p[l];
+  const double ip = g.inv_p[l];
+  uint8_t* R = g.r + l * g.r_plane;
+#pragma unroll
+  for (int ti = 0; ti < TM; ++ti) {
+#pragma unroll
+    for (int tj = 0; tj < TN; ++tj) {
+      const int64_t col = n0 + wn0 + tj * 32 + lr;
+#pragma unroll
+      for (int grp = 0; grp < 4; ++grp) {
+        const int64_t row = m0 + wm0 + ti * 32 + 8 * grp + 4 * lh;
+        uint32_t word = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          word |= uint32_t(ModU(acc[ti][tj][grp * 4 + q], p, ip)) << (8 * q);
+        *reinterpret_cast<uint32_t*>(R + row + col * g.ldr) = word;
+      }
+    }
+  }
+}
+
+// C(i, j) = alpha 2^-(sa_i + sb_j) C'(i, j) + beta C(i, j), C' from the
+// residue planes through the 192-bit fixed-point CRT sum.
+__global__ void __launch_bounds__(256)
+    ReconstructKernel(const uint8_t* R, int64_t r_plane, int64_t ldr, int64_t m, int64_t n,
+                      const int* sa, const int* sb, Crt crt, double alpha, double beta,
+                      double* c, int64_t ldc) {
+  const int64_t total = m * n;
+  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = idx % m, j = idx / m;
+    uint64_t acc[kLimbs] = {};
+    const uint8_t* rp = R + i + j * ldr;
+    for (int l = 0; l < crt.nmod; ++l) {
+      const uint64_t cl = rp[l * r_plane];
+#pragma unroll
+      for (int t = 0; t < kLimbs; ++t) acc[t] += cl * crt.frac[l][t];
+    }
+    uint32_t limb[kLimbs];
+    uint64_t carry = 0;
+#pragma unroll
+    for (int t = 0; t < kLimbs; ++t) {
+      const uint64_t v = acc[t] + carry;
+      limb[t] = uint32_t(v);
+      carry = v >> 32;
+    }
+    // Centre: the top bit set means a fraction in [1/2, 1), i.e. negative.
+    const bool neg = (limb[kLimbs - 1] >> 31) != 0;
+    if (neg) {
+      uint64_t borrow = 1;
+#pragma unroll
+      for (int t = 0; t < kLimbs; ++t) {
+        const uint64_t v = uint64_t(uint32_t(~limb[t])) + borrow;
+        limb[t] = uint32_t(v);
+        borrow = v >> 32;
+      }
+    }
+    // Magnitude (a 192-bit fraction) to double from its top three non-zero
+    // limbs.
+    int top = kLimbs - 1;
+    while (top > 2 && limb[top] == 0) --top;
+    double d = (double(limb[top]) * 0x1p32 + double(limb[top - 1])) * 0x1p32 +
+               double(limb[top - 2]);
+    d = ldexp(d, 32 * (top - 2) - 32 * kLimbs);
+    double x = d * crt.P;
+    x = ldexp(neg ? -x : x, -(sa[i] + sb[j]));
+    double* cij = c + i + j * ldc;
+    double v = x * alpha;
+    if (beta != 0.0) v = (beta == 1.0) ? v + *cij : __builtin_fma(*cij, beta, v);
+    *cij = v;
+  }
+}
+
+// Sampled statistics for the choice of moduli: for sample rows i of op(A)
+// (blockIdx.x) and sample columns j of op(B): sum_p |a_ip| |b_pj|,
+// sum_p |a_ip|, sum_p |b_pj|, max_p |a_ip| and max_p |b_pj|.
+constexpr int kSamples = 32;
+struct SampleOut {
+  double sab[kSamples][kSamples];
+  double sa[kSamples], sb[kSamples], ma[kSamples], mb[kSamples];
+  uint64_t amax_bits, bmax_bits;
+};
+
+__global__ void __launch_bounds__(256)
+    SampleKernel(const double* a, int64_t a_rs, int64_t a_cs, const double* b, int64_t b_rs,
+                 int64_t b_cs, int64_t m, int64_t n, int64_t k, SampleOut* out) {
+  const int si = blockIdx.x;
+  const int64_t i = (m - 1) * si / (kSamples - 1 > 0 ? kSamples - 1 : 1);
+  __shared__ double red[256];
+  double s_ab[kSamples] = {};
+  double s_a = 0, m_a = 0;
+  double s_b[kSamples] = {}, m_b[kSamples] = {};
+  for (int64_t p = threadIdx.x; p < k; p += blockDim.x) {
+    const double av = fabs(a[i * a_rs + p * a_cs]);
+    s_a += av;
+    m_a = fmax(m_a, av);
+#pragma unroll
+    for (int sj = 0; sj < kSamples; ++sj) {
+      const int64_t j = (n - 1) * sj / (kSamples - 1);
+      const double bv = fabs(b[p * b_rs + j * b_cs]);
+      s_ab[sj] += av * bv;
+      if (si == 0) {
+        s_b[sj] += bv;
+        m_b[sj] = fmax(m_b[sj], bv);
+      }
+    }
+  }
+  auto reduce = [&](double v, bool is_max) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s)
+        red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + s])
+                                  : red[threadIdx.x] + red[threadIdx.x + s];
+      __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+  };
+  for (int sj = 0; sj < kSamples; ++sj) {
+    const double v = reduce(s_ab[sj], false);
+    if (threadIdx.x == 0) out->sab[si][sj] = v;
+  }
+  const double sa = reduce(s_a, false), ma = reduce(m_a, true);
+  if (threadIdx.x == 0) {
+    out->sa[si] = sa;
+    out->ma[si] = ma;
+  }
+  if (si == 0) {
+    for (int sj = 0; sj < kSamples; ++sj) {
+      const double sb = reduce(s_b[sj], false), mb = reduce(m_b[sj], true);
+      if (threadIdx.x == 0) {
+        out->sb[sj] = sb;
+        out->mb[sj] = mb;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Workspace: one grow-only device buffer per thread and device.
+
+struct Workspace {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  int device = -1;
+};
+thread_local Workspace t_ws[64];
+
+void* GetWorkspace(size_t bytes, hipStream_t stream) {
+  int dev = 0;
+  HWY_HIP_CHECK(hipGetDevice(&dev));
+  Workspace& w = t_ws[dev];
+  if (w.cap < bytes) {
+    if (w.ptr) {
+      HWY_HIP_CHECK(hipStreamSynchronize(stream));
+      HWY_HIP_CHECK(hipFree(w.ptr));
+    }
+    const size_t rounded = (bytes + (size_t(64) << 20) - 1) & ~((size_t(64) << 20) - 1);
+    HWY_HIP_CHECK(hipMalloc(&w.ptr, rounded));
+    w.cap = rounded;
+    w.device = dev;
+  }
+  return w.ptr;
+}
+
+size_t AlignUp(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Optional per-phase timing (hwyblas_amd_profile_*): HIP events recorded on
+// the call's stream around the phases of the last emulated call.
+enum Phase { kPhaseStats = 0, kPhaseSlice = 1, kPhaseModGemm = 2, kPhaseReconstruct = 3,
+             kNumPhases = 4 };
+struct PhaseTimer {
+  bool enabled = false;
+  bool recorded = false;
+  hipEvent_t ev[kNumPhases + 1] = {};
+};
+thread_local PhaseTimer t_timer;
+
+void Mark(int phase_boundary, hipStream_t stream) {
+  PhaseTimer& t = t_timer;
+  if (!t.enabled) return;
+  if (!t.ev[0])
+    for (auto& e : t.ev) HWY_HIP_CHECK(hipEventCreate(&e));
+  HWY_HIP_CHECK(hipEventRecord(t.ev[phase_boundary], stream));
+  if (phase_boundary == kNumPhases) t.recorded = true;
+}
+
+constexpr int kTileM = 256, kTileN = 256, kTileK = 64;
+
+}  // namespace
+
+// Chooses the number of moduli from the sampled statistics: the per-row
+// fixed-point truncation perturbs C(i, j) by at most
+//   2^-(bA+1) maxrow|a_i| sum_p |b_pj| + 2^-(bB+1) maxcol|b_j| sum_p |a_ip|,
+// and native FP64 accumulation's typical error is about sqrt(k) u
+// sum_p |a_ip||b_pj|; the smallest N whose bits keep the first within the
+// second on every sampled (i, j) is used.
+int ChooseModuli(const SampleOut& s, int64_t k) {
+  double worst_a = 0, worst_b = 0;
+  for (int i = 0; i < kSamples; ++i)
+    for (int j = 0; j < kSamples; ++j) {
+      const double denom = s.sab[i][j];
+      if (denom <= 0) continue;
+      worst_a = std::max(worst_a, s.ma[i] * s.sb[j] / denom);
+      worst_b = std::max(worst_b, s.mb[j] * s.sa[i] / denom);
+    }
+  const double u = 0x1p-53;
+  const double target = std::sqrt(double(k)) * u;
+  for (int nmod = 8; nmod <= kMaxModuli; ++nmod) {
+    const int bits = TotalBits(nmod, k);
+    const int ba = bits / 2, bb = bits - ba;
+    const double err = std::ldexp(worst_a, -(ba + 1)) + std::ldexp(worst_b, -(bb + 1));
+    if (err <= target) return nmod;
+  }
+  return kMaxModuli;
+}
+
+int EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                  const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                  double* c, int64_t ldc, int nmod, hipStream_t stream) {
+  t_timer.recorded = false;
+  if (k >= 131072) {
+    GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+    return 0;
+  }
+  // op(A)(i, p) = a[i * a_rs + p * a_cs]; op(B)(p, j) = b[p * b_rs + j * b_cs].
+  const int64_t a_rs = ta ? lda : 1, a_cs = ta ? 1 : lda;
+  const int64_t b_rs = tb ? ldb : 1, b_cs = tb ? 1 : ldb;
+  const int64_t MP = AlignUp(m, kTileM), NP = AlignUp(n, kTileN), KP = AlignUp(k, kTileK);
+
+  // Workspace layout.
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o = off;
+    off = AlignUp(off + bytes, 256);
+    return o;
+  };
+  const size_t o_rowmax = carve(sizeof(uint64_t) * (m + n + 2));
+  const size_t o_sa = carve(sizeof(int) * m);
+  const size_t o_sb = carve(sizeof(int) * n);
+  const size_t o_sample = carve(sizeof(SampleOut));
+  const size_t o_a8 = carve(size_t(kMaxModuli) * MP * KP);
+  const size_t o_b8 = carve(size_t(kMaxModuli) * NP * KP);
+  const size_t o_r = carve(size_t(kMaxModuli) * MP * NP);
+  char* ws = static_cast<char*>(GetWorkspace(off, stream));
+  uint64_t* rowmax = reinterpret_cast<uint64_t*>(ws + o_rowmax);
+  uint64_t* colmax = rowmax + m;
+  uint64_t* gmax = colmax + n;  // [0]: A, [1]: B
+  int* sa = reinterpret_cast<int*>(ws + o_sa);
+  int* sb = reinterpret_cast<int*>(ws + o_sb);
+  SampleOut* sample = reinterpret_cast<SampleOut*>(ws + o_sample);
+  int8_t* a8 = reinterpret_cast<int8_t*>(ws + o_a8);
+  int8_t* b8 = reinterpret_cast<int8_t*>(ws + o_b8);
+  uint8_t* rr = reinterpret_cast<uint8_t*>(ws + o_r);
+
+  Mark(kPhaseStats, stream);
+  // 1. Row maxima of op(A), column maxima of op(B) (= row maxima of op(B)^T).
+  hipLaunchKernelGGL(FillU64, dim3(unsigned(std::min<int64_t>((m + n + 2 + 255) / 256, 4096))),
+                     dim3(256), 0, stream, rowmax, m + n + 2, uint64_t(0));
+  const int64_t pchunk = 512;
+  auto rowmax_launch = [&](const double* x, int64_t rs, int64_t cs, int64_t R, int64_t K,
+                           uint64_t* out) {
+    const unsigned chunks = unsigned((K + pchunk - 1) / pchunk);
+    if (rs == 1) {
+      hipLaunchKernelGGL(RowAbsMaxKernel<true>, dim3(unsigned((R + 255) / 256), chunks),
+                         dim3(256), 0, stream, x, cs, R, K, pchunk, out);
+    } else {
+      hipLaunchKernelGGL(RowAbsMaxKernel<false>, dim3(unsigned((R + 3) / 4), chunks), dim3(256),
+                         0, stream, x, rs, R, K, pchunk, out);
+    }
+    HWY_HIP_CHECK(hipGetLastError());
+  };
+  rowmax_launch(a, a_rs, a_cs, m, k, rowmax);
+  // op(B)^T(j, p) = b[j * b_cs + p * b_rs]
+  rowmax_launch(b, b_cs, b_rs, n, k, colmax);
+
+  // 2. Moduli: given, or from sampled statistics. Either way the host
+  //    learns whether the operands hold NaN or Inf.
+  hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((m + 255) / 256)), dim3(256), 0, stream,
+                     rowmax, m, 0, sa, gmax);
+  hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream,
+                     colmax, n, 0, sb, gmax + 1);
+  if (nmod <= 0) {
+    hipLaunchKernelGGL(SampleKernel, dim3(kSamples), dim3(256), 0, stream, a, a_rs, a_cs, b,
+                       b_rs, b_cs, m, n, k, sample);
+  }
+  HWY_HIP_CHECK(hipGetLastError());
+  SampleOut host{};
+  uint64_t gm[2];
+  HWY_HIP_CHECK(hipMemcpyAsync(gm, gmax, sizeof(gm), hipMemcpyDeviceToHost, stream));
+  if (nmod <= 0)
+    HWY_HIP_CHECK(hipMemcpyAsync(&host, sample, sizeof(host), hipMemcpyDeviceToHost, stream));
+  HWY_HIP_CHECK(hipStreamSynchronize(stream));
+  const uint64_t kInfBits = 0x7FF0000000000000ull;
+  if (gm[0] >= kInfBits || gm[1] >= kInfBits) {
+    GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+    return 0;
+  }
+  if (nmod <= 0) nmod = ChooseModuli(host, k);
+  nmod = std::min(std::max(nmod, 2), kMaxModuli);
+  const Crt crt = MakeCrt(nmod);
+  const int bits = std::max(TotalBits(nmod, k), 2);
+  const int ba = bits / 2, bb = bits - ba;
+
+  // 3. Scale exponents with the chosen bits, then the int8 slices.
+  Mark(kPhaseSlice, stream);
+  hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((m + 255) / 256)), dim3(256), 0, stream,
+                     rowmax, m, ba, sa, gmax);
+  hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream,
+                     colmax, n, bb, sb, gmax + 1);
+  auto slice = [&](const double* x, int64_t rs, int64_t cs, int64_t R, int64_t RP, const int* s,
+                   int8_t* y) {
+    const dim3 grid(unsigned(KP / 64), unsigned(RP / 64));
+    if (rs == 1)
+      hipLaunchKernelGGL(SliceKernel<true>, grid, dim3(256), 0, stream, x, cs, R, k, RP, KP, s,
+                         crt, y);
+    else
+      hipLaunchKernelGGL(SliceKernel<false>, grid, dim3(256), 0, stream, x, rs, R, k, RP, KP, s,
+                         crt, y);
+    HWY_HIP_CHECK(hipGetLastError());
+  };
+  slice(a, a_rs, a_cs, m, MP, sa, a8);
+  slice(b, b_cs, b_rs, n, NP, sb, b8);
+
+  // 4. One INT8 GEMM per modulus, all in one launch.
+  ModGemmArgs g{};
+  g.a = a8;
+  g.b = b8;
+  g.r = rr;
+  g.a_plane = MP * KP;
+  g.b_plane = NP * KP;
+  g.r_plane = MP * NP;
+  g.kp = KP;
+  g.ldr = MP;
+  g.tiles_m = int(MP / kTileM);
+  g.tiles_n = int(NP / kTileN);
+  for (int l = 0; l < nmod; ++l) {
+    g.p[l] = crt.p[l];
+    g.inv_p[l] = crt.inv_p[l];
+  }
+  Mark(kPhaseModGemm, stream);
+  hipLaunchKernelGGL((ModGemmKernel<kTileM, kTileN, kTileK, 2, 4>),
+                     dim3(unsigned(g.tiles_m * g.tiles_n), unsigned(nmod)), dim3(512), 0, stream,
+                     g);
+  HWY_HIP_CHECK(hipGetLastError());
+
+  // 5. Reconstruction and the alpha/beta epilogue.
+  Mark(kPhaseReconstruct, stream);
+  const int64_t total = m * n;
+  hipLaunchKernelGGL(ReconstructKernel, dim3(unsigned(std::min<int64_t>((total + 255) / 256, 1 << 20))),
+                     dim3(256), 0, stream, rr, MP * NP, MP, m, n, sa, sb, crt, alpha, beta, c,
+                     ldc);
+  HWY_HIP_CHECK(hipGetLastError());
+  Mark(kNumPhases, stream);
+  return nmod;
+}
+
+}  // namespace ozaki
+}  // namespace hwy_amd
+
+using namespace hwy_amd;
+
+extern "C" {
+
+HWYBLAS_AMD_API int hwyblas_dgemm_emulated_dev(char transa, char transb, int m, int n, int k,
+                                               double alpha, const double* a, int lda,
+                                               const double* b, int ldb, double beta, double* c,
+                                               int ldc, int nmoduli, void* stream,
+                                               int* used_moduli) {
+  Op oa, ob;
+  const bool va = DecodeOp(transa, &oa), vb = DecodeOp(transb, &ob);
+  const int rows_a = oa == Op::kNoTrans ? m : k, rows_b = ob == Op::kNoTrans ? k : n;
+  int info = 0;
+  if (!va) info = 1;
+  else if (!vb) info = 2;
+  else if (m < 0) info = 3;
+  else if (n < 0) info = 4;
+  else if (k < 0) info = 5;
+  else if (lda < std::max(1, rows_a)) info = 8;
+  else if (ldb < std::max(1, rows_b)) info = 10;
+  else if (ldc < std::max(1, m)) info = 13;
+  if (info != 0) {
+    ReportError("DGEMM", info);
+    return info;
+  }
+  if (used_moduli) *used_moduli = 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (m == 0 || n == 0) return 0;
+  if ((k == 0 || alpha == 0.0) && beta == 1.0) return 0;
+  if (k == 0 || alpha == 0.0) {
+    ScaleDevice<double>(m, n, beta, c, ldc, s);
+    return 0;
+  }
+  const int used = ozaki::EmulatedDgemm(oa == Op::kTrans, ob == Op::kTrans, m, n, k, alpha, a,
+                                        lda, b, ldb, beta, c, ldc, nmoduli, s);
+  if (used_moduli) *used_moduli = used;
+  return 0;
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_profile_enable(int on) {
+  ozaki::t_timer.enabled = on != 0;
+  ozaki::t_timer.recorded = false;
+}
+
+HWYBLAS_AMD_API float hwyblas_amd_profile_ms(int phase) {
+  ozaki::PhaseTimer& t = ozaki::t_timer;
+  if (!t.recorded || phase < 0 || phase >= ozaki::kNumPhases) return -1.0f;
+  HWY_HIP_CHECK(hipEventSynchronize(t.ev[phase + 1]));
+  float ms = 0;
+  HWY_HIP_CHECK(hipEventElapsedTime(&ms, t.ev[phase], t.ev[phase + 1]));
+  return ms;
+}
+
+}  // extern "C"

@@ -173,6 +173,51 @@ HWYBLAS_AMD_API int hwyblas_strsm_dev(char side, char uplo, char transa,
                                       int ldb, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Emulated DGEMM on the INT8 matrix cores (no hwyBLAS counterpart). Same
+ * arguments, checks (reported as "DGEMM") and quick returns as
+ * hwyblas_dgemm_dev, FP64 in and out. The product is computed by the Ozaki
+ * scheme II: power-of-two row/column scaling to integers, symmetric residues
+ * modulo `nmoduli` pairwise coprime moduli <= 256 (2..24; 0 = chosen from a
+ * sampled estimate of the operands' dynamic range so that the error stays at
+ * native FP64 level), exact INT8 products and a Chinese-remainder
+ * reconstruction. Operands with NaN or Inf, and k >= 131072, use the native
+ * FP64 kernel. The call synchronises `stream` once (to read the operand
+ * statistics). *used_moduli (if not NULL) receives the number of moduli, or
+ * 0 when the native kernel ran. Returns 0 or the Fortran info. */
+HWYBLAS_AMD_API int hwyblas_dgemm_emulated_dev(char transa, char transb, int m, int n,
+                                               int k, double alpha, const double* a,
+                                               int lda, const double* b, int ldb,
+                                               double beta, double* c, int ldc,
+                                               int nmoduli, void* stream,
+                                               int* used_moduli);
+
+/* ------------------------------------------------------------------------ */
+/* Tile configurations of the plain GEMM kernel (no hwyBLAS counterpart: its
+ * blocking is compile-time, src/gemm_kernel-inl.h:72-74). For the autotuner
+ * and the cost model: the configurations are numbered 0..count-1; every one
+ * computes the same result (the k order is the same), only the time
+ * differs. The *_dev_cfg forms take m, n, k > 0 and valid leading
+ * dimensions (else they return -1 without reporting) and enqueue like the
+ * *_dev entry points. */
+HWYBLAS_AMD_API int hwyblas_gemm_config_count(void);
+/* Macro-tile bm x bn, k slice bk, waves per workgroup. 0, or -1 if cfg is
+ * out of range. */
+HWYBLAS_AMD_API int hwyblas_gemm_config_info(int cfg, int* bm, int* bn, int* bk,
+                                             int* waves);
+/* The configuration the Fortran/CBLAS/dev entry points use for a shape. */
+HWYBLAS_AMD_API int hwyblas_gemm_default_config(int dtype_bytes, char transa,
+                                                char transb, int m, int n, int k);
+HWYBLAS_AMD_API int hwyblas_dgemm_dev_cfg(int cfg, char transa, char transb, int m,
+                                          int n, int k, double alpha,
+                                          const double* a, int lda, const double* b,
+                                          int ldb, double beta, double* c, int ldc,
+                                          void* stream);
+HWYBLAS_AMD_API int hwyblas_sgemm_dev_cfg(int cfg, char transa, char transb, int m,
+                                          int n, int k, float alpha, const float* a,
+                                          int lda, const float* b, int ldb, float beta,
+                                          float* c, int ldc, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Device utilities (no hwyBLAS counterpart): memory, streams, timing and the
  * seeded generator shared with the CPU oracle, for callers and benchmarks
  * that hold no other HIP binding. Errors abort like any HIP failure.        */
@@ -204,6 +249,14 @@ HWYBLAS_AMD_API void hwyblas_amd_fill_uniform(void* x, int dtype_bytes,
                                               int64_t ld, uint64_t seed,
                                               double lo, double hi,
                                               void* stream);
+/* Per-phase timing of the calling thread's last emulated DGEMM, for
+ * profiling (off by default): 0 operand statistics and moduli choice,
+ * 1 scaling and INT8 slicing, 2 the INT8 GEMMs (one launch for all moduli),
+ * 3 CRT reconstruction and epilogue. HIP events are recorded on the call's
+ * stream; hwyblas_amd_profile_ms waits for them and returns milliseconds, or
+ * -1 if nothing was recorded. */
+HWYBLAS_AMD_API void hwyblas_amd_profile_enable(int on);
+HWYBLAS_AMD_API float hwyblas_amd_profile_ms(int phase);
 /* Library version string. */
 HWYBLAS_AMD_API const char* hwyblas_amd_version(void);
 

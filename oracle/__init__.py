@@ -61,6 +61,15 @@ def lib():
             q.restype = None
             q.argtypes = [_c_char, _c_char, _c_char, _c_char, _i, _i, s, _p, _i, _p, _i,
                           _p, _i, _p, _i, _p]
+        L.orc_ozaki_modulus.restype = _i
+        L.orc_ozaki_modulus.argtypes = [_i]
+        L.orc_ozaki_total_bits.restype = _i
+        L.orc_ozaki_total_bits.argtypes = [_i, ctypes.c_int64]
+        L.orc_ozaki_crt.restype = _d
+        L.orc_ozaki_crt.argtypes = [_i, _p]
+        L.orc_dgemm_ozaki.restype = None
+        L.orc_dgemm_ozaki.argtypes = [_c_char, _c_char, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p,
+                                      _i, _i]
         L.orc_plant_mistake.restype = None
         L.orc_plant_mistake.argtypes = [_i]
         _lib = L
@@ -139,6 +148,25 @@ def trsm_residual(side, uplo, ta, diag, m, n, alpha, a, lda, b0, ldb, x, ldx, wh
     fn(_b(side), _b(uplo), _b(ta), _b(diag), m, n, alpha, _ptr(a), lda, _ptr(b0), ldb,
        _ptr(x), ldx, _ptr(which), len(which), _ptr(ratio))
     return ratio
+
+
+def ozaki_moduli(nmod):
+    return [lib().orc_ozaki_modulus(l) for l in range(nmod)]
+
+
+def ozaki_total_bits(nmod, k):
+    return lib().orc_ozaki_total_bits(nmod, k)
+
+
+def ozaki_crt(nmod, residues):
+    r = np.ascontiguousarray(residues, dtype=np.int32)
+    return lib().orc_ozaki_crt(nmod, _ptr(r))
+
+
+def dgemm_ozaki(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nmod):
+    """In-place C update by the CPU restatement of the emulated DGEMM."""
+    lib().orc_dgemm_ozaki(_b(ta), _b(tb), m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta,
+                          _ptr(c), ldc, nmod)
 
 
 def plant_mistake(which):

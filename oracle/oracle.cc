@@ -547,6 +547,175 @@ void TrsmResidual(char side, char uplo, char ta, char diag, int m, int n,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Emulated DGEMM (Ozaki scheme II; Ozaki, Uchino, Imamura 2025), the CPU
+// restatement of the product's INT8 emulation, step for step.
+
+namespace ozk {
+
+constexpr int kMax = 24, kLimbs = 6;
+constexpr int kModuli[kMax] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217,
+                               211, 199, 197, 193, 191, 181, 179, 173, 167, 163, 157, 151};
+
+int64_t ModInv(int64_t a, int64_t m) {
+  for (int64_t y = 1; y < m; ++y)
+    if ((a * y) % m == 1) return y;
+  return 0;
+}
+
+struct Crt {
+  int nmod;
+  double P;
+  uint32_t frac[kMax][kLimbs];
+};
+
+Crt Make(int nmod) {
+  Crt c{};
+  c.nmod = nmod;
+  c.P = 1;
+  for (int l = 0; l < nmod; ++l) c.P *= kModuli[l];
+  for (int l = 0; l < nmod; ++l) {
+    const int64_t p = kModuli[l];
+    int64_t ml = 1;
+    for (int q = 0; q < nmod; ++q)
+      if (q != l) ml = (ml * kModuli[q]) % p;
+    uint64_t rem = uint64_t(ModInv(ml, p));
+    for (int t = kLimbs - 1; t >= 0; --t) {
+      const uint64_t num = rem << 32;
+      c.frac[l][t] = uint32_t(num / uint64_t(p));
+      rem = num % uint64_t(p);
+    }
+  }
+  return c;
+}
+
+int TotalBits(int nmod, int64_t k) {
+  double lp = 0;
+  for (int l = 0; l < nmod; ++l) lp += std::log2(double(kModuli[l]));
+  return int(std::floor(lp - 1.0 - std::log2(double(k)) - 1e-9));
+}
+
+// x mod p in [0, p).
+int64_t ModU(int64_t x, int64_t p) {
+  int64_t r = x % p;
+  return r < 0 ? r + p : r;
+}
+
+// Symmetric residue of round(v) (v = x 2^s), as the product computes it.
+int Residue(double v, int p) {
+  if (std::fabs(v) < 0x1p52) {
+    const double xi = std::nearbyint(v);
+    int r = int(ModU(int64_t(xi), p));
+    if (r >= p / 2) r -= p;
+    return r;
+  }
+  int e;
+  const double m = std::frexp(v, &e);
+  const int64_t mi = int64_t(std::ldexp(m, 53));
+  int64_t r = ModU(mi, p);
+  for (int E = e - 53; E > 0; --E) r = (r * 2) % p;
+  if (r >= p / 2) r -= p;
+  return int(r);
+}
+
+// The centred integer with residues res[l], as a double: the 192-bit
+// fixed-point sum of res[l] * frac[l] (mod 1), centred, times P.
+double Reconstruct(const Crt& c, const int* res) {
+  uint64_t acc[kLimbs] = {};
+  const int nuse = (g_mistake == 6) ? c.nmod - 1 : c.nmod;
+  for (int l = 0; l < nuse; ++l)
+    for (int t = 0; t < kLimbs; ++t) acc[t] += uint64_t(res[l]) * c.frac[l][t];
+  uint32_t limb[kLimbs];
+  uint64_t carry = 0;
+  for (int t = 0; t < kLimbs; ++t) {
+    const uint64_t v = acc[t] + carry;
+    limb[t] = uint32_t(v);
+    carry = v >> 32;
+  }
+  const bool neg = (limb[kLimbs - 1] >> 31) != 0;
+  if (neg) {
+    uint64_t borrow = 1;
+    for (int t = 0; t < kLimbs; ++t) {
+      const uint64_t v = uint64_t(uint32_t(~limb[t])) + borrow;
+      limb[t] = uint32_t(v);
+      borrow = v >> 32;
+    }
+  }
+  int top = kLimbs - 1;
+  while (top > 2 && limb[top] == 0) --top;
+  double d = (double(limb[top]) * 0x1p32 + double(limb[top - 1])) * 0x1p32 +
+             double(limb[top - 2]);
+  d = std::ldexp(d, 32 * (top - 2) - 32 * kLimbs);
+  const double x = d * c.P;
+  return neg ? -x : x;
+}
+
+// Scale exponent of a row with max |x| = f 2^E (f in [0.5, 1)): bits - E.
+int ScaleExp(double rowmax, int bits) {
+  if (!(rowmax > 0)) return 0;
+  int e;
+  std::frexp(rowmax, &e);
+  return bits - e;
+}
+
+void Dgemm(char ta, char tb, int m, int n, int k, double alpha, const double* a, int lda,
+           const double* b, int ldb, double beta, double* c, int ldc, int nmod) {
+  if (m == 0 || n == 0) return;
+  if ((k == 0 || alpha == 0.0) && beta == 1.0) return;
+  if (k == 0 || alpha == 0.0) {
+    Scale(m, n, beta, c, ldc);
+    return;
+  }
+  const Opd<double> A = General(ta, a, lda), B = General(tb, b, ldb);
+  const Crt crt = Make(nmod);
+  const int bits = std::max(TotalBits(nmod, k), 2);
+  const int ba = bits / 2, bb = bits - ba;
+  std::vector<int> sa(m), sb(n);
+  for (int i = 0; i < m; ++i) {
+    double mx = 0;
+    for (int p = 0; p < k; ++p) mx = std::max(mx, std::fabs(A.at(i, p)));
+    sa[i] = ScaleExp(mx, ba);
+  }
+  for (int j = 0; j < n; ++j) {
+    double mx = 0;
+    for (int p = 0; p < k; ++p) mx = std::max(mx, std::fabs(B.at(p, j)));
+    sb[j] = ScaleExp(mx, bb);
+  }
+  // Residue slices: ra[l][i][p], rb[l][j][p].
+  std::vector<int8_t> ra(size_t(nmod) * m * k), rb(size_t(nmod) * n * k);
+  for (int l = 0; l < nmod; ++l) {
+    const int p_l = kModuli[l];
+    for (int i = 0; i < m; ++i)
+      for (int p = 0; p < k; ++p) {
+        const double v = std::ldexp(A.at(i, p), sa[i]);
+        ra[(size_t(l) * m + i) * k + p] = int8_t(v == 0.0 ? 0 : Residue(v, p_l));
+      }
+    for (int j = 0; j < n; ++j)
+      for (int p = 0; p < k; ++p) {
+        const double v = std::ldexp(B.at(p, j), sb[j]);
+        rb[(size_t(l) * n + j) * k + p] = int8_t(v == 0.0 ? 0 : Residue(v, p_l));
+      }
+  }
+  std::vector<int> res(nmod);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < m; ++i) {
+      for (int l = 0; l < nmod; ++l) {
+        int64_t s = 0;
+        const int8_t* x = &ra[(size_t(l) * m + i) * k];
+        const int8_t* y = &rb[(size_t(l) * n + j) * k];
+        for (int p = 0; p < k; ++p) s += int64_t(x[p]) * y[p];
+        res[l] = int(ModU(s, kModuli[l]));
+      }
+      const double x = std::ldexp(Reconstruct(crt, res.data()), -(sa[i] + sb[j]));
+      double& cij = c[i + size_t(j) * ldc];
+      double v = x * alpha;
+      if (beta != 0.0) v = (beta == 1.0) ? v + cij : std::fma(cij, beta, v);
+      cij = v;
+    }
+}
+
+}  // namespace ozk
+
 }  // namespace
 
 extern "C" {
@@ -639,6 +808,20 @@ void orc_strsm_residual(char side, char uplo, char ta, char diag, int m, int n,
                         int count, double* ratio) {
   TrsmResidual(side, uplo, ta, diag, m, n, alpha, a, lda, b0, ldb, x, ldx,
                which, count, ratio);
+}
+
+int orc_ozaki_modulus(int l) { return (l >= 0 && l < ozk::kMax) ? ozk::kModuli[l] : 0; }
+
+int orc_ozaki_total_bits(int nmod, int64_t k) { return ozk::TotalBits(nmod, k); }
+
+double orc_ozaki_crt(int nmod, const int* res) {
+  return ozk::Reconstruct(ozk::Make(nmod), res);
+}
+
+void orc_dgemm_ozaki(char ta, char tb, int m, int n, int k, double alpha, const double* a,
+                     int lda, const double* b, int ldb, double beta, double* c, int ldc,
+                     int nmod) {
+  ozk::Dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nmod);
 }
 
 void orc_plant_mistake(int which) { g_mistake = which; }

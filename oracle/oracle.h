@@ -89,10 +89,28 @@ void orc_strsm_residual(char side, char uplo, char transa, char diag, int m,
                         const float* b0, int ldb, const float* x, int ldx,
                         const int* which, int count, double* ratio);
 
+/* ---- emulated DGEMM (Ozaki scheme II), restated on the CPU ---------------
+ * The same algorithm as the product's INT8 emulation: per-row/column power-
+ * of-two scaling to integers with bA + bB = floor(log2 P - 1 - log2 k) bits,
+ * symmetric residues modulo the first nmod of the pairwise coprime moduli
+ * 256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199,
+ * 197, 193, 191, 181, 179, 173, 167, 163, 157, 151, exact integer products
+ * reduced mod p, and the CRT reconstruction through 192-bit fixed-point
+ * fractions y_l / p_l. Deterministic bit for bit. */
+int orc_ozaki_modulus(int l);
+int orc_ozaki_total_bits(int nmod, int64_t k);
+/* Reconstructs the centred integer X (as a double) from its residues
+ * res[l] in [0, p_l), l < nmod. */
+double orc_ozaki_crt(int nmod, const int* res);
+void orc_dgemm_ozaki(char transa, char transb, int m, int n, int k, double alpha,
+                     const double* a, int lda, const double* b, int ldb, double beta,
+                     double* c, int ldc, int nmod);
+
 /* ---- planted mistakes (for tests that show the checks catch them) ------- */
 /* 0: none; 1: drop the last k term of every GEMM tile; 2: use op(A) with
  * the wrong transposition; 3: ignore beta; 4: TRSM skips the reciprocal of
- * the diagonal; 5: the double-double reference drops its low word. */
+ * the diagonal; 5: the double-double reference drops its low word; 6: the
+ * emulated DGEMM's CRT drops the last modulus. */
 void orc_plant_mistake(int which);
 
 #ifdef __cplusplus

@@ -258,3 +258,27 @@ def test_large_square_sampled(H, dtype, ta, tb):
     err = np.abs(got[rows, cols] - exp) / bnd
     tol = 4 * (n + 2) * np.finfo(dtype).eps
     assert np.max(err) <= tol, np.max(err) / tol
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_every_tile_config_gives_identical_bits(H, dtype):
+    """Every tile configuration runs the same k order, so all of them must
+    return bitwise the same C, on shapes with edge tiles in m, n and k."""
+    nconf = len(H.gemm_configs())
+    for (ta, tb), (m, n, k) in zip(TRANS * 2, [(129, 67, 33), (300, 257, 70), (64, 64, 16),
+                                               (1, 513, 260), (520, 3, 9), (200, 200, 300),
+                                               (17, 130, 1), (256, 128, 64)]):
+        a, lda, b, ldb, c, ldc = gemm_operands(ta, tb, m, n, k, 1, dtype, m + n + k)
+        da, db = H.DeviceArray.from_host(a), H.DeviceArray.from_host(b)
+        ref = None
+        for cfg in range(nconf):
+            dc = H.DeviceArray.from_host(c)
+            assert H.gemm_dev_cfg(cfg, ta, tb, m, n, k, 0.5, da, lda, db, ldb, -1.0, dc, ldc,
+                                  dtype=dtype) == 0
+            got = dc.to_host()
+            if ref is None:
+                ref = got
+                r = gemm_bound_ratio(got, ta, tb, m, n, k, 0.5, a, lda, b, ldb, -1.0, c, ldc)
+                assert r <= 1.0
+            assert np.array_equal(got, ref), (cfg, ta, tb, m, n, k)
+            assert np.array_equal(got[m:], c[m:])
