@@ -102,6 +102,9 @@ def main():
     ap.add_argument("--samples", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB device to device (a kernel of known traffic, "
+                         "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): the hot path is "
@@ -119,6 +122,11 @@ def main():
         d.fill_uniform(n, n, s, stream=stream)
     L.hwyblas_amd_stream_synchronize(stream)
     flops = 2.0 * n ** 3
+    if args.calibrate_copy:
+        src, dst = H.DeviceArray(1 << 27, 1), H.DeviceArray(1 << 27, 1)  # 2 x 1 GiB
+        L.hwyblas_amd_memcpy(dst.ptr, src.ptr, 1 << 30, 3)
+        H.synchronize()
+        del src, dst
 
     # Native FP64 MFMA DGEMM.
     def native_step():

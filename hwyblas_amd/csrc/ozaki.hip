@@ -27,6 +27,7 @@
 // FP64 kernel, as does k >= 131072.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "common.h"
@@ -36,7 +37,8 @@ namespace ozaki {
 namespace {
 
 constexpr int kMaxModuli = 24;
-constexpr int kLimbs = 6;  // 192-bit CRT fractions
+constexpr int kLimbs = 4;     // 160-bit CRT fractions in 40-bit limbs
+constexpr int kLimbBits = 40;
 // Pairwise coprime, largest first: 2^8, 3*5*17, 11*23, 251, 13*19, primes,
 // 7*31, primes.
 constexpr int kModuli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233,
@@ -47,7 +49,7 @@ struct Crt {
   int nmod;
   int p[kMaxModuli];
   double inv_p[kMaxModuli];
-  uint32_t frac[kMaxModuli][kLimbs];  // floor(2^192 y_l / p_l), little endian
+  double frac[kMaxModuli][kLimbs];  // floor(2^160 y_l / p_l) in 40-bit limbs, low first
   double P;                           // prod p_l, rounded
 };
 
@@ -83,12 +85,12 @@ Crt MakeCrt(int nmod) {
     for (int q = 0; q < nmod; ++q)
       if (q != l) ml = (ml * kModuli[q]) % p;
     const int64_t y = ModInverse(ml, p);
-    // floor(2^192 y / p) by long division in base 2^32, most significant
-    // limb first.
+    // floor(2^160 y / p) by long division in base 2^40, most significant
+    // limb first (rem < p < 2^9, so rem 2^40 fits in 64 bits).
     uint64_t rem = uint64_t(y);
     for (int t = kLimbs - 1; t >= 0; --t) {
-      const uint64_t num = rem << 32;
-      c.frac[l][t] = uint32_t(num / uint64_t(p));
+      const uint64_t num = rem << kLimbBits;
+      c.frac[l][t] = double(num / uint64_t(p));
       rem = num % uint64_t(p);
     }
   }
@@ -219,50 +221,65 @@ __global__ void ScaleExpKernel(const uint64_t* rowmax, int64_t R, int bits, int*
 }
 
 // Slices: Y[l][r][p] (row stride KP, plane stride RP * KP) = symmetric
-// residue mod p_l of round(x(r, p) 2^s_r), zero for r >= R or p >= K. One
-// block per 64 x 64 tile; the tile passes through LDS so that the global
-// reads are coalesced along whichever dimension of x is contiguous and the
-// byte writes along p.
-template <bool ROWS_CONTIG>
+// residue mod p_l of round(x(r, p) 2^s_r), zero for r >= R or p >= K. Every
+// thread produces 8 consecutive bytes of one row per modulus (16 threads
+// cover a row's 128 bytes: whole cache lines).
+__device__ __forceinline__ void SliceEight(const double (&x)[8], int sc, const Crt& crt,
+                                           int8_t* dst, int64_t plane) {
+  double v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = ldexp(x[q], sc);
+  for (int l = 0; l < crt.nmod; ++l) {
+    const int p = crt.p[l];
+    const double ip = crt.inv_p[l];
+    uint64_t word = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int res = (v[q] == 0.0) ? 0 : ResidueOf(v[q], p, ip);
+      word |= uint64_t(uint8_t(int8_t(res))) << (8 * q);
+    }
+    *reinterpret_cast<uint64_t*>(dst + l * plane) = word;
+  }
+}
+
+// x(r, p) = x[r * ld + p] (p contiguous): each thread reads its 8 elements
+// directly. Block: 16 rows x 128 p.
 __global__ void __launch_bounds__(256)
-    SliceKernel(const double* x, int64_t ld, int64_t R, int64_t K, int64_t RP, int64_t KP,
-                const int* s, Crt crt, int8_t* Y) {
-  __shared__ double T[64][65];
+    SliceRowsKernel(const double* x, int64_t ld, int64_t R, int64_t K, int64_t RP, int64_t KP,
+                    const int* s, Crt crt, int8_t* Y) {
   const int tid = threadIdx.x;
-  const int64_t r0 = int64_t(blockIdx.y) * 64, p0 = int64_t(blockIdx.x) * 64;
-  for (int idx = tid; idx < 64 * 64; idx += 256) {
-    int rr, pp;
-    if (ROWS_CONTIG) { rr = idx & 63; pp = idx >> 6; }
-    else { pp = idx & 63; rr = idx >> 6; }
+  const int64_t r = int64_t(blockIdx.y) * 16 + (tid >> 4);
+  const int64_t p0 = int64_t(blockIdx.x) * 128 + (tid & 15) * 8;
+  if (r >= RP) return;
+  double v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = (r < R && p0 + q < K) ? x[r * ld + p0 + q] : 0.0;
+  SliceEight(v, r < R ? s[r] : 0, crt, Y + r * KP + p0, RP * KP);
+}
+
+// x(r, p) = x[r + p * ld] (r contiguous): a 32 x 128 tile is transposed
+// through LDS, read coalesced along r.
+__global__ void __launch_bounds__(256)
+    SliceColsKernel(const double* x, int64_t ld, int64_t R, int64_t K, int64_t RP, int64_t KP,
+                    const int* s, Crt crt, int8_t* Y) {
+  constexpr int TR = 32, TP = 128;
+  __shared__ double T[TP][TR + 1];
+  const int tid = threadIdx.x;
+  const int64_t r0 = int64_t(blockIdx.y) * TR, p0 = int64_t(blockIdx.x) * TP;
+  for (int idx = tid; idx < TR * TP; idx += 256) {
+    const int rr = idx % TR, pp = idx / TR;
     const int64_t r = r0 + rr, p = p0 + pp;
-    double v = 0;
-    if (r < R && p < K) v = ROWS_CONTIG ? x[r + p * ld] : x[r * ld + p];
-    T[rr][pp] = v;
+    T[pp][rr] = (r < R && p < K) ? x[r + p * ld] : 0.0;
   }
   __syncthreads();
-  // Thread -> (row rr, 4 consecutive p): writes one 32-bit word per modulus.
-  const int rr = tid >> 4, pq = (tid & 15) * 4;
-  const int64_t plane = RP * KP;
-  for (int rb = 0; rb < 64; rb += 16) {
-    const int row = rr + rb;
-    const int64_t r = r0 + row;
-    if (r >= RP) continue;
-    const int sc = (r < R) ? s[r] : 0;
-    double v[4];
+  const int pq = (tid & 15) * 8;
+  for (int rr = tid >> 4; rr < TR; rr += 16) {
+    const int64_t r = r0 + rr;
+    if (r >= RP) break;
+    double v[8];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = ldexp(T[row][pq + q], sc);
-    int8_t* dst = Y + r * KP + p0 + pq;
-    for (int l = 0; l < crt.nmod; ++l) {
-      const int p = crt.p[l];
-      const double ip = crt.inv_p[l];
-      uint32_t word = 0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int res = (v[q] == 0.0) ? 0 : ResidueOf(v[q], p, ip);
-        word |= uint32_t(uint8_t(int8_t(res))) << (8 * q);
-      }
-      *reinterpret_cast<uint32_t*>(dst + l * plane) = word;
-    }
+    for (int q = 0; q < 8; ++q) v[q] = T[pq + q][rr];
+    SliceEight(v, r < R ? s[r] : 0, crt, Y + r * KP + p0 + pq, RP * KP);
   }
 }
 
@@ -410,55 +427,370 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmKernel(ModGemmA
   }
 }
 
+// The same product with the operand slices streamed straight into LDS
+// (global_load_lds_dwordx4: each wave writes 64 consecutive 16-byte chunks,
+// lane i at base + 16 i) through a STAGES-deep ring of LDS buffers, so that
+// loads for STAGES - 1 slices are in flight while one is multiplied and no
+// registers are spent on staging. Lane i of a wave's DMA picks the global
+// chunk that belongs at its LDS position under the swizzle (the inverse of
+// swz). The waves synchronise with s_barrier after waiting (vmcnt) for their
+// own DMA of the slice about to be read; the slot refilled after the barrier
+// is the one every wave finished reading in the previous iteration.
+template <int N>
+__device__ __forceinline__ void WaitVmcntImm() {
+  // s_waitcnt with only vmcnt constrained (expcnt 7, lgkmcnt 15 = no wait);
+  // vmcnt is 6 bits, split over [3:0] and [15:14].
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | (7 << 4) | (0xF << 8) | ((N >> 4) << 14));
+}
+
+// Waits until at most `slices` later slices (PER_SLICE DMAs each) are
+// outstanding.
+template <int PER_SLICE>
+__device__ __forceinline__ void WaitSlices(int slices) {
+  switch (slices) {
+    case 0: WaitVmcntImm<0>(); break;
+    case 1: WaitVmcntImm<PER_SLICE>(); break;
+    case 2: WaitVmcntImm<2 * PER_SLICE>(); break;
+    default: WaitVmcntImm<3 * PER_SLICE>(); break;
+  }
+}
+
+// One global_load_lds_dwordx4: this lane's 16 bytes at `src` land in LDS at
+// `lds_wave_base` + 16 * lane. Issued as inline assembly so that the
+// compiler, which would otherwise wait for every outstanding DMA before any
+// LDS read, leaves the waiting to WaitVmcnt. M0 (the LDS base) must be
+// written one instruction before the DMA reads it.
+__device__ __forceinline__ void DmaToLds(const void* src, const void* lds_wave_base) {
+  const unsigned base =
+      __builtin_amdgcn_readfirstlane(unsigned(reinterpret_cast<uintptr_t>(lds_wave_base)));
+  unsigned saved;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\ts_mov_b32 m0, %1\n\ts_nop 0\n\t"
+      "global_load_lds_dwordx4 %2, off\n\ts_mov_b32 m0, %0"
+      : "=&s"(saved)
+      : "s"(base), "v"(src)
+      : "memory");
+}
+
+__device__ __forceinline__ void BarrierNoFence() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES>
+__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmDmaKernel(ModGemmArgs g) {
+  constexpr int NT = 64 * WAVES_M * WAVES_N;
+  constexpr int NW = WAVES_M * WAVES_N;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+  constexpr int TM = WM / 32, TN = WN / 32;
+  constexpr int CH = BK / 16;   // 16-byte chunks per row
+  constexpr int RPB = 16 / CH;  // rows per 256-byte bank row
+  constexpr int A_DMA = BM * CH / NT;  // DMA instructions per thread per slice
+  constexpr int B_DMA = BN * CH / NT;
+  constexpr int PER_SLICE = A_DMA + B_DMA;
+  static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "load shape");
+  static_assert(PER_SLICE * 3 < 64 && STAGES <= 5, "vmcnt table");
+  __shared__ i32x4 As[STAGES][BM * CH];
+  __shared__ i32x4 Bs[STAGES][BN * CH];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+  const int l = blockIdx.y;
+  int tm, tn;
+  {
+    const int nblk = gridDim.x, bid = blockIdx.x;
+    const int xcd = bid % 8, idx = bid / 8, per = nblk / 8, rem = nblk % 8;
+    const int lid = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    const int group = 8 * g.tiles_n;
+    const int first_m = (lid / group) * 8;
+    const int gm = min(g.tiles_m - first_m, 8);
+    tm = first_m + (lid % group) % gm;
+    tn = (lid % group) / gm;
+  }
+  const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
+  const int8_t* A = g.a + l * g.a_plane + m0 * g.kp;
+  const int8_t* B = g.b + l * g.b_plane + n0 * g.kp;
+
+  // LDS chunk position P -> the (row, chunk) it holds: row = P / CH,
+  // c = (P % CH) ^ ((row / RPB) % CH). Lane offsets within each slice:
+  int64_t a_off[A_DMA], b_off[B_DMA];
+#pragma unroll
+  for (int q = 0; q < A_DMA; ++q) {
+    const int P = (q * NW + wave) * 64 + lane;
+    const int row = P / CH, c = (P % CH) ^ ((row / RPB) % CH);
+    a_off[q] = int64_t(row) * g.kp + c * 16;
+  }
+#pragma unroll
+  for (int q = 0; q < B_DMA; ++q) {
+    const int P = (q * NW + wave) * 64 + lane;
+    const int row = P / CH, c = (P % CH) ^ ((row / RPB) % CH);
+    b_off[q] = int64_t(row) * g.kp + c * 16;
+  }
+  auto issue = [&](int stage, int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < A_DMA; ++q) DmaToLds(A + a_off[q] + k0, &As[stage][(q * NW + wave) * 64]);
+#pragma unroll
+    for (int q = 0; q < B_DMA; ++q) DmaToLds(B + b_off[q] + k0, &Bs[stage][(q * NW + wave) * 64]);
+  };
+  auto swz = [](int row, int c) { return row * CH + (c ^ ((row / RPB) % CH)); };
+
+  i32x16 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = i32x16{};
+
+  const int ktiles = int(g.kp / BK);
+#pragma unroll
+  for (int s = 0; s < STAGES - 1; ++s)
+    if (s < ktiles) issue(s, int64_t(s) * BK);
+  const int lr = lane & 31, lh = lane >> 5;
+  for (int kt = 0; kt < ktiles; ++kt) {
+    // Slices issued after kt: min(STAGES - 2, ktiles - 1 - kt).
+    const int later = min(STAGES - 2, ktiles - 1 - kt);
+    WaitSlices<PER_SLICE>(later);
+    BarrierNoFence();
+    const int next = kt + STAGES - 1;
+    if (next < ktiles) issue(next % STAGES, int64_t(next) * BK);
+    const int cur = kt % STAGES;
+#pragma unroll
+    for (int ks = 0; ks < BK / 32; ++ks) {
+      const int c = ks * 2 + lh;
+      i32x4 af[TM], bf[TN];
+#pragma unroll
+      for (int ti = 0; ti < TM; ++ti) af[ti] = As[cur][swz(wm0 + ti * 32 + lr, c)];
+#pragma unroll
+      for (int tj = 0; tj < TN; ++tj) bf[tj] = Bs[cur][swz(wn0 + tj * 32 + lr, c)];
+#pragma unroll
+      for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TN; ++tj)
+          acc[ti][tj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
+    }
+  }
+
+  const int p = g.p[l];
+  const double ip = g.inv_p[l];
+  uint8_t* R = g.r + l * g.r_plane;
+#pragma unroll
+  for (int ti = 0; ti < TM; ++ti) {
+#pragma unroll
+    for (int tj = 0; tj < TN; ++tj) {
+      const int64_t col = n0 + wn0 + tj * 32 + lr;
+#pragma unroll
+      for (int grp = 0; grp < 4; ++grp) {
+        const int64_t row = m0 + wm0 + ti * 32 + 8 * grp + 4 * lh;
+        uint32_t word = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          word |= uint32_t(ModU(acc[ti][tj][grp * 4 + q], p, ip)) << (8 * q);
+        *reinterpret_cast<uint32_t*>(R + row + col * g.ldr) = word;
+      }
+    }
+  }
+}
+
+// As ModGemmDmaKernel, with the operand fragments double-buffered in
+// registers so that the LDS latency and the block barrier are hidden behind
+// MFMAs: the fragments of step ks + 1 are read while step ks multiplies, and
+// before the last step of slice kt the waves wait for slice kt + 1's DMA,
+// meet at the barrier, refill the slot of slice kt - 1 (read by every wave
+// in the previous iteration) and read slice kt + 1's first fragments.
+template <int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES>
+__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmPipeKernel(ModGemmArgs g) {
+  constexpr int NT = 64 * WAVES_M * WAVES_N;
+  constexpr int NW = WAVES_M * WAVES_N;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+  constexpr int TM = WM / 32, TN = WN / 32;
+  constexpr int CH = BK / 16;
+  constexpr int RPB = 16 / CH;
+  constexpr int KS = BK / 32;  // MFMA k steps per slice
+  constexpr int A_DMA = BM * CH / NT;
+  constexpr int B_DMA = BN * CH / NT;
+  constexpr int PER_SLICE = A_DMA + B_DMA;
+  static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "load shape");
+  static_assert(STAGES >= 3 && PER_SLICE * 3 < 64, "pipeline");
+  static_assert(KS % 2 == 0, "each slice starts in fragment buffer 0");
+  __shared__ i32x4 As[STAGES][BM * CH];
+  __shared__ i32x4 Bs[STAGES][BN * CH];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+  const int l = blockIdx.y;
+  int tm, tn;
+  {
+    const int nblk = gridDim.x, bid = blockIdx.x;
+    const int xcd = bid % 8, idx = bid / 8, per = nblk / 8, rem = nblk % 8;
+    const int lid = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    const int group = 8 * g.tiles_n;
+    const int first_m = (lid / group) * 8;
+    const int gm = min(g.tiles_m - first_m, 8);
+    tm = first_m + (lid % group) % gm;
+    tn = (lid % group) / gm;
+  }
+  const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
+  const int8_t* A = g.a + l * g.a_plane + m0 * g.kp;
+  const int8_t* B = g.b + l * g.b_plane + n0 * g.kp;
+
+  int64_t a_off[A_DMA], b_off[B_DMA];
+#pragma unroll
+  for (int q = 0; q < A_DMA; ++q) {
+    const int P = (q * NW + wave) * 64 + lane;
+    const int row = P / CH, c = (P % CH) ^ ((row / RPB) % CH);
+    a_off[q] = int64_t(row) * g.kp + c * 16;
+  }
+#pragma unroll
+  for (int q = 0; q < B_DMA; ++q) {
+    const int P = (q * NW + wave) * 64 + lane;
+    const int row = P / CH, c = (P % CH) ^ ((row / RPB) % CH);
+    b_off[q] = int64_t(row) * g.kp + c * 16;
+  }
+  auto issue = [&](int stage, int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < A_DMA; ++q) DmaToLds(A + a_off[q] + k0, &As[stage][(q * NW + wave) * 64]);
+#pragma unroll
+    for (int q = 0; q < B_DMA; ++q) DmaToLds(B + b_off[q] + k0, &Bs[stage][(q * NW + wave) * 64]);
+  };
+  auto swz = [](int row, int c) { return row * CH + (c ^ ((row / RPB) % CH)); };
+  const int lr = lane & 31, lh = lane >> 5;
+  i32x4 fa[2][TM], fb[2][TN];
+  auto read_frags = [&](int buf, int stage, int ks) {
+    const int c = ks * 2 + lh;
+#pragma unroll
+    for (int ti = 0; ti < TM; ++ti) fa[buf][ti] = As[stage][swz(wm0 + ti * 32 + lr, c)];
+#pragma unroll
+    for (int tj = 0; tj < TN; ++tj) fb[buf][tj] = Bs[stage][swz(wn0 + tj * 32 + lr, c)];
+  };
+
+  i32x16 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = i32x16{};
+
+  const int ktiles = int(g.kp / BK);
+#pragma unroll
+  for (int s = 0; s < STAGES - 1; ++s)
+    if (s < ktiles) issue(s, int64_t(s) * BK);
+  WaitSlices<PER_SLICE>(min(STAGES - 2, ktiles - 1));
+  BarrierNoFence();
+  read_frags(0, 0, 0);
+
+  for (int kt = 0; kt < ktiles; ++kt) {
+    const int cur = kt % STAGES;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int fc = ks & 1;  // KS is even: slice kt always starts in buffer 0
+      if (ks + 1 < KS) {
+        read_frags(fc ^ 1, cur, ks + 1);
+      } else if (kt + 1 < ktiles) {
+        // Slices issued after kt + 1 so far: kt + 2 .. kt + STAGES - 2.
+        WaitSlices<PER_SLICE>(min(STAGES - 3, ktiles - 2 - kt));
+        BarrierNoFence();
+        const int next = kt + STAGES - 1;
+        if (next < ktiles) issue(next % STAGES, int64_t(next) * BK);
+        read_frags(fc ^ 1, (kt + 1) % STAGES, 0);
+      }
+#pragma unroll
+      for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TN; ++tj)
+          acc[ti][tj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[fc][ti], fb[fc][tj], acc[ti][tj],
+                                                               0, 0, 0);
+    }
+  }
+
+  const int p = g.p[l];
+  const double ip = g.inv_p[l];
+  uint8_t* R = g.r + l * g.r_plane;
+#pragma unroll
+  for (int ti = 0; ti < TM; ++ti) {
+#pragma unroll
+    for (int tj = 0; tj < TN; ++tj) {
+      const int64_t col = n0 + wn0 + tj * 32 + lr;
+#pragma unroll
+      for (int grp = 0; grp < 4; ++grp) {
+        const int64_t row = m0 + wm0 + ti * 32 + 8 * grp + 4 * lh;
+        uint32_t word = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          word |= uint32_t(ModU(acc[ti][tj][grp * 4 + q], p, ip)) << (8 * q);
+        *reinterpret_cast<uint32_t*>(R + row + col * g.ldr) = word;
+      }
+    }
+  }
+}
+
 // C(i, j) = alpha 2^-(sa_i + sb_j) C'(i, j) + beta C(i, j), C' from the
-// residue planes through the 192-bit fixed-point CRT sum.
+// residue planes through the fixed-point CRT sum S = sum_l c_l F_l mod 2^160,
+// F_l = floor(2^160 y_l / p_l), held as four 40-bit limbs in doubles: every
+// partial sum of c_l F_lt (c_l < 256, N <= 24) stays below 2^53, so the
+// limb sums are exact FP64 FMAs. Carries are then propagated exactly, the
+// top limb is wrapped mod 2^40 and centred, and S / 2^160 times P is C'. Each
+// thread handles four consecutive rows of one column (one 32-bit residue
+// word per modulus).
+__device__ __forceinline__ double CrtValue(double (&S)[kLimbs], double P) {
+#pragma unroll
+  for (int t = 0; t + 1 < kLimbs; ++t) {
+    const double carry = floor(S[t] * 0x1p-40);
+    S[t] = __builtin_fma(-carry, 0x1p40, S[t]);
+    S[t + 1] += carry;
+  }
+  S[kLimbs - 1] = __builtin_fma(-floor(S[kLimbs - 1] * 0x1p-40), 0x1p40, S[kLimbs - 1]);
+  // A fraction >= 1/2 stands for a negative C': take 2^160 - S exactly (limb
+  // complement plus one), so that the conversion below adds non-negative
+  // terms only.
+  const bool neg = S[kLimbs - 1] >= 0x1p39;
+  if (neg) {
+    double carry = 1;
+#pragma unroll
+    for (int t = 0; t < kLimbs; ++t) {
+      S[t] = (0x1p40 - 1) - S[t] + carry;
+      carry = (S[t] >= 0x1p40) ? 1.0 : 0.0;
+      S[t] -= carry * 0x1p40;
+    }
+  }
+  double v = S[0] * 0x1p-160;
+  v = v + S[1] * 0x1p-120;
+  v = v + S[2] * 0x1p-80;
+  v = v + S[3] * 0x1p-40;
+  v = v * P;
+  return neg ? -v : v;
+}
+
 __global__ void __launch_bounds__(256)
     ReconstructKernel(const uint8_t* R, int64_t r_plane, int64_t ldr, int64_t m, int64_t n,
                       const int* sa, const int* sb, Crt crt, double alpha, double beta,
                       double* c, int64_t ldc) {
-  const int64_t total = m * n;
+  const int64_t quads = (m + 3) / 4;
+  const int64_t total = quads * n;
   for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
        idx += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t i = idx % m, j = idx / m;
-    uint64_t acc[kLimbs] = {};
-    const uint8_t* rp = R + i + j * ldr;
+    const int64_t i0 = (idx % quads) * 4, j = idx / quads;
+    double S[4][kLimbs] = {};
+    const uint8_t* rp = R + i0 + j * ldr;
     for (int l = 0; l < crt.nmod; ++l) {
-      const uint64_t cl = rp[l * r_plane];
+      const uint32_t word = *reinterpret_cast<const uint32_t*>(rp + l * r_plane);
 #pragma unroll
-      for (int t = 0; t < kLimbs; ++t) acc[t] += cl * crt.frac[l][t];
-    }
-    uint32_t limb[kLimbs];
-    uint64_t carry = 0;
+      for (int q = 0; q < 4; ++q) {
+        const double cl = double((word >> (8 * q)) & 0xFFu);
 #pragma unroll
-    for (int t = 0; t < kLimbs; ++t) {
-      const uint64_t v = acc[t] + carry;
-      limb[t] = uint32_t(v);
-      carry = v >> 32;
-    }
-    // Centre: the top bit set means a fraction in [1/2, 1), i.e. negative.
-    const bool neg = (limb[kLimbs - 1] >> 31) != 0;
-    if (neg) {
-      uint64_t borrow = 1;
-#pragma unroll
-      for (int t = 0; t < kLimbs; ++t) {
-        const uint64_t v = uint64_t(uint32_t(~limb[t])) + borrow;
-        limb[t] = uint32_t(v);
-        borrow = v >> 32;
+        for (int t = 0; t < kLimbs; ++t) S[q][t] = __builtin_fma(cl, crt.frac[l][t], S[q][t]);
       }
     }
-    // Magnitude (a 192-bit fraction) to double from its top three non-zero
-    // limbs.
-    int top = kLimbs - 1;
-    while (top > 2 && limb[top] == 0) --top;
-    double d = (double(limb[top]) * 0x1p32 + double(limb[top - 1])) * 0x1p32 +
-               double(limb[top - 2]);
-    d = ldexp(d, 32 * (top - 2) - 32 * kLimbs);
-    double x = d * crt.P;
-    x = ldexp(neg ? -x : x, -(sa[i] + sb[j]));
-    double* cij = c + i + j * ldc;
-    double v = x * alpha;
-    if (beta != 0.0) v = (beta == 1.0) ? v + *cij : __builtin_fma(*cij, beta, v);
-    *cij = v;
+    const int sbj = sb[j];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t i = i0 + q;
+      if (i >= m) break;
+      const double x = ldexp(CrtValue(S[q], crt.P), -(sa[i] + sbj));
+      double* cij = c + i + j * ldc;
+      double v = x * alpha;
+      if (beta != 0.0) v = (beta == 1.0) ? v + *cij : __builtin_fma(*cij, beta, v);
+      *cij = v;
+    }
   }
 }
 
@@ -579,6 +911,22 @@ void Mark(int phase_boundary, hipStream_t stream) {
 }
 
 constexpr int kTileM = 256, kTileN = 256, kTileK = 64;
+// k of the padded slices: a multiple of every variant's k step.
+constexpr int kPadK = 128;
+
+// INT8 GEMM variant (HWYBLAS_AMD_INT8_KERNEL). All stream the operand slices
+// and give the same bits; they differ in schedule. 3 (default, fastest on
+// MI355X at m = n = k = 16384): DMA into LDS, 8 waves of 128 x 64, k in
+// steps of 128 through two LDS stages. 0: the same with k steps of 64 and
+// four stages; 1: slices staged through registers; 2: 4 waves of 128 x 128;
+// 4: three stages of 64; 5: fragments double-buffered in registers.
+int ModGemmVariant() {
+  static const int v = [] {
+    const char* e = getenv("HWYBLAS_AMD_INT8_KERNEL");
+    return (e && *e >= '0' && *e <= '9') ? *e - '0' : 3;
+  }();
+  return v;
+}
 
 }  // namespace
 
@@ -619,7 +967,7 @@ int EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alph
   // op(A)(i, p) = a[i * a_rs + p * a_cs]; op(B)(p, j) = b[p * b_rs + j * b_cs].
   const int64_t a_rs = ta ? lda : 1, a_cs = ta ? 1 : lda;
   const int64_t b_rs = tb ? ldb : 1, b_cs = tb ? 1 : ldb;
-  const int64_t MP = AlignUp(m, kTileM), NP = AlignUp(n, kTileN), KP = AlignUp(k, kTileK);
+  const int64_t MP = AlignUp(m, kTileM), NP = AlignUp(n, kTileN), KP = AlignUp(k, kPadK);
 
   // Workspace layout.
   size_t off = 0;
@@ -703,13 +1051,12 @@ int EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alph
                      colmax, n, bb, sb, gmax + 1);
   auto slice = [&](const double* x, int64_t rs, int64_t cs, int64_t R, int64_t RP, const int* s,
                    int8_t* y) {
-    const dim3 grid(unsigned(KP / 64), unsigned(RP / 64));
     if (rs == 1)
-      hipLaunchKernelGGL(SliceKernel<true>, grid, dim3(256), 0, stream, x, cs, R, k, RP, KP, s,
-                         crt, y);
+      hipLaunchKernelGGL(SliceColsKernel, dim3(unsigned(KP / 128), unsigned(RP / 32)), dim3(256),
+                         0, stream, x, cs, R, k, RP, KP, s, crt, y);
     else
-      hipLaunchKernelGGL(SliceKernel<false>, grid, dim3(256), 0, stream, x, rs, R, k, RP, KP, s,
-                         crt, y);
+      hipLaunchKernelGGL(SliceRowsKernel, dim3(unsigned(KP / 128), unsigned(RP / 16)), dim3(256),
+                         0, stream, x, rs, R, k, RP, KP, s, crt, y);
     HWY_HIP_CHECK(hipGetLastError());
   };
   slice(a, a_rs, a_cs, m, MP, sa, a8);
@@ -732,14 +1079,39 @@ int EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alph
     g.inv_p[l] = crt.inv_p[l];
   }
   Mark(kPhaseModGemm, stream);
-  hipLaunchKernelGGL((ModGemmKernel<kTileM, kTileN, kTileK, 2, 4>),
-                     dim3(unsigned(g.tiles_m * g.tiles_n), unsigned(nmod)), dim3(512), 0, stream,
-                     g);
+  const dim3 grid(unsigned(g.tiles_m * g.tiles_n), unsigned(nmod));
+  switch (ModGemmVariant()) {
+    case 0:
+      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, kTileK, 2, 4, 4>), grid, dim3(512), 0,
+                         stream, g);
+      break;
+    case 1:
+      hipLaunchKernelGGL((ModGemmKernel<kTileM, kTileN, kTileK, 2, 4>), grid, dim3(512), 0,
+                         stream, g);
+      break;
+    case 2:
+      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, kTileK, 2, 2, 4>), grid, dim3(256), 0,
+                         stream, g);
+      break;
+
+    case 4:
+      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, kTileK, 2, 4, 3>), grid, dim3(512), 0,
+                         stream, g);
+      break;
+    case 5:
+      hipLaunchKernelGGL((ModGemmPipeKernel<kTileM, kTileN, kTileK, 2, 4, 4>), grid, dim3(512), 0,
+                         stream, g);
+      break;
+
+    default:  // 3
+      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2>), grid, dim3(512), 0,
+                         stream, g);
+  }
   HWY_HIP_CHECK(hipGetLastError());
 
   // 5. Reconstruction and the alpha/beta epilogue.
   Mark(kPhaseReconstruct, stream);
-  const int64_t total = m * n;
+  const int64_t total = ((m + 3) / 4) * n;
   hipLaunchKernelGGL(ReconstructKernel, dim3(unsigned(std::min<int64_t>((total + 255) / 256, 1 << 20))),
                      dim3(256), 0, stream, rr, MP * NP, MP, m, n, sa, sb, crt, alpha, beta, c,
                      ldc);

@@ -553,7 +553,7 @@ void TrsmResidual(char side, char uplo, char ta, char diag, int m, int n,
 
 namespace ozk {
 
-constexpr int kMax = 24, kLimbs = 6;
+constexpr int kMax = 24, kLimbs = 4;  // 160-bit fractions in 40-bit limbs
 constexpr int kModuli[kMax] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217,
                                211, 199, 197, 193, 191, 181, 179, 173, 167, 163, 157, 151};
 
@@ -566,7 +566,7 @@ int64_t ModInv(int64_t a, int64_t m) {
 struct Crt {
   int nmod;
   double P;
-  uint32_t frac[kMax][kLimbs];
+  double frac[kMax][kLimbs];  // floor(2^160 y_l / p_l), 40-bit limbs, low first
 };
 
 Crt Make(int nmod) {
@@ -581,8 +581,8 @@ Crt Make(int nmod) {
       if (q != l) ml = (ml * kModuli[q]) % p;
     uint64_t rem = uint64_t(ModInv(ml, p));
     for (int t = kLimbs - 1; t >= 0; --t) {
-      const uint64_t num = rem << 32;
-      c.frac[l][t] = uint32_t(num / uint64_t(p));
+      const uint64_t num = rem << 40;
+      c.frac[l][t] = double(num / uint64_t(p));
       rem = num % uint64_t(p);
     }
   }
@@ -618,36 +618,36 @@ int Residue(double v, int p) {
   return int(r);
 }
 
-// The centred integer with residues res[l], as a double: the 192-bit
-// fixed-point sum of res[l] * frac[l] (mod 1), centred, times P.
+// The centred integer with residues res[l], as a double: the fixed-point
+// sum S = sum_l res[l] F_l (mod 2^160) in four 40-bit limbs accumulated
+// exactly in doubles, carries propagated, the top limb wrapped and centred,
+// then (S / 2^160) P.
 double Reconstruct(const Crt& c, const int* res) {
-  uint64_t acc[kLimbs] = {};
+  double S[kLimbs] = {};
   const int nuse = (g_mistake == 6) ? c.nmod - 1 : c.nmod;
   for (int l = 0; l < nuse; ++l)
-    for (int t = 0; t < kLimbs; ++t) acc[t] += uint64_t(res[l]) * c.frac[l][t];
-  uint32_t limb[kLimbs];
-  uint64_t carry = 0;
-  for (int t = 0; t < kLimbs; ++t) {
-    const uint64_t v = acc[t] + carry;
-    limb[t] = uint32_t(v);
-    carry = v >> 32;
+    for (int t = 0; t < kLimbs; ++t) S[t] = std::fma(double(res[l]), c.frac[l][t], S[t]);
+  for (int t = 0; t + 1 < kLimbs; ++t) {
+    const double carry = std::floor(S[t] * 0x1p-40);
+    S[t] = std::fma(-carry, 0x1p40, S[t]);
+    S[t + 1] += carry;
   }
-  const bool neg = (limb[kLimbs - 1] >> 31) != 0;
-  if (neg) {
-    uint64_t borrow = 1;
+  S[kLimbs - 1] = std::fma(-std::floor(S[kLimbs - 1] * 0x1p-40), 0x1p40, S[kLimbs - 1]);
+  const bool neg = S[kLimbs - 1] >= 0x1p39;  // fraction >= 1/2: negative
+  if (neg) {  // 2^160 - S exactly: limb complement plus one
+    double carry = 1;
     for (int t = 0; t < kLimbs; ++t) {
-      const uint64_t v = uint64_t(uint32_t(~limb[t])) + borrow;
-      limb[t] = uint32_t(v);
-      borrow = v >> 32;
+      S[t] = (0x1p40 - 1) - S[t] + carry;
+      carry = (S[t] >= 0x1p40) ? 1.0 : 0.0;
+      S[t] -= carry * 0x1p40;
     }
   }
-  int top = kLimbs - 1;
-  while (top > 2 && limb[top] == 0) --top;
-  double d = (double(limb[top]) * 0x1p32 + double(limb[top - 1])) * 0x1p32 +
-             double(limb[top - 2]);
-  d = std::ldexp(d, 32 * (top - 2) - 32 * kLimbs);
-  const double x = d * c.P;
-  return neg ? -x : x;
+  double v = S[0] * 0x1p-160;
+  v = v + S[1] * 0x1p-120;
+  v = v + S[2] * 0x1p-80;
+  v = v + S[3] * 0x1p-40;
+  v = v * c.P;
+  return neg ? -v : v;
 }
 
 // Scale exponent of a row with max |x| = f 2^E (f in [0.5, 1)): bits - E.

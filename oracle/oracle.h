@@ -95,8 +95,9 @@ void orc_strsm_residual(char side, char uplo, char transa, char diag, int m,
  * symmetric residues modulo the first nmod of the pairwise coprime moduli
  * 256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199,
  * 197, 193, 191, 181, 179, 173, 167, 163, 157, 151, exact integer products
- * reduced mod p, and the CRT reconstruction through 192-bit fixed-point
- * fractions y_l / p_l. Deterministic bit for bit. */
+ * reduced mod p, and the CRT reconstruction through 160-bit fixed-point
+ * fractions y_l / p_l (four 40-bit limbs summed exactly in doubles).
+ * Deterministic bit for bit. */
 int orc_ozaki_modulus(int l);
 int orc_ozaki_total_bits(int nmod, int64_t k);
 /* Reconstructs the centred integer X (as a double) from its residues

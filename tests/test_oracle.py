@@ -215,3 +215,71 @@ def test_planted_reference_mistake_is_caught():
     oracle.plant_mistake(5)
     with pytest.raises(AssertionError):
         test_cancellation_needs_the_low_word()
+
+
+def test_ozaki_crt_matches_python_integers():
+    """The emulation's CRT step (oracle restatement) against exact Python
+    integers: for random X with |X| < P/2 (as the scaling guarantees), the
+    residues mod the first nmod moduli reconstruct X to within one rounding
+    plus the fixed-point truncation P N 256 2^-160."""
+    import random
+    from math import prod
+    rnd = random.Random(7)
+    for nmod in (2, 5, 9, 14, 20, 24):
+        mods = oracle.ozaki_moduli(nmod)
+        assert all(np.gcd(a, b) == 1 for i, a in enumerate(mods) for b in mods[i + 1:])
+        P = prod(mods)
+        lim = P // 2 - 1 - (P >> 30)
+        for t in range(300):
+            X = rnd.randrange(-lim, lim)
+            if t < 5:
+                X = [0, 1, -1, lim, -lim][t]
+            elif t < 120:
+                X = rnd.randrange(-min(2 ** 40, lim), min(2 ** 40, lim))
+            got = oracle.ozaki_crt(nmod, [X % p for p in mods])
+            tol = 6e-16 * abs(X) + P * nmod * 256 * 2.0 ** -160
+            assert abs(got - X) <= tol, (nmod, X, got)
+
+
+def test_ozaki_bits_rule():
+    """k 2^(bA+bB) < P / 2 with the largest such bit count."""
+    from math import log2, prod
+    for nmod in (3, 8, 14, 24):
+        P = prod(oracle.ozaki_moduli(nmod))
+        for k in (1, 100, 16384):
+            bits = oracle.ozaki_total_bits(nmod, k)
+            assert k * 2 ** bits < P // 2 and k * 2 ** (bits + 1) >= P // 2 * 0.999
+
+
+def test_ozaki_emulation_error_at_native_level():
+    """With 14 moduli the restated emulation's componentwise error is at the
+    typical level of native FP64 accumulation, sqrt(k) u; with 12 it is far
+    above it (the moduli matter). The native worst-case bound 4 (k + 2) u
+    holds for both."""
+    m, n, k = 64, 64, 1000
+    a = oracle.fill_uniform(m, k, 1)
+    b = oracle.fill_uniform(k, n, 2)
+    c = oracle.fill_uniform(m, n, 3)
+    exp, bnd = oracle.gemm_ref("N", "N", m, n, k, 1.5, a, m, b, k, 0.5, c, m)
+    eps = np.finfo(float).eps
+    for nmod, ok in ((14, True), (12, False)):
+        cc = c.copy(order="F")
+        oracle.dgemm_ozaki("N", "N", m, n, k, 1.5, a, m, b, k, 0.5, cc, m, nmod)
+        err = np.max(np.abs(cc - exp) / bnd)
+        assert err <= 4 * (k + 2) * eps
+        assert (err <= np.sqrt(k) * eps) == ok, (nmod, err / eps)
+    # Integer operands: the integer product is recovered exactly and only
+    # the final conversion rounds (a few ulps).
+    rng = np.random.default_rng(1)
+    ai = np.asfortranarray(rng.integers(-1000, 1000, (m, k)).astype(float))
+    bi = np.asfortranarray(rng.integers(-1000, 1000, (k, n)).astype(float))
+    cc = np.zeros((m, n), order="F")
+    oracle.dgemm_ozaki("N", "N", m, n, k, 1.0, ai, m, bi, k, 0.0, cc, m, 14)
+    want = ai @ bi
+    assert np.all(np.abs(cc - want) <= 4 * np.spacing(np.abs(want)))
+
+
+def test_planted_crt_mistake_is_caught():
+    oracle.plant_mistake(6)
+    with pytest.raises(AssertionError):
+        test_ozaki_crt_matches_python_integers()
