@@ -65,6 +65,172 @@ def time_steps(H, stream, step, steps, warmup):
     return e0.elapsed_ms(e1) / steps, wall
 
 
+def log_uniform_ints(rng, lo, hi, size):
+    """Integers log-uniform in [lo, hi]: floor(exp(U(ln lo, ln(hi + 1))))."""
+    return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi + 1), size))).astype(int).clip(lo, hi)
+
+
+class DeviceBatch:
+    """Operands of many column-major problems packed in three device arenas
+    (A, B, C), each filled in one launch with the seeded uniform[-1, 1]
+    generator (element t of an arena is uniform(seed, t))."""
+
+    def __init__(self, H, dtype, shapes_a, shapes_b, shapes_c, seed):
+        self.H, self.dtype = H, np.dtype(dtype)
+        self.arenas, self.offsets = [], []
+        for s, shapes in enumerate((shapes_a, shapes_b, shapes_c)):
+            sizes = np.array([max(1, r * c) for r, c in shapes], dtype=np.int64)
+            off = np.concatenate([[0], np.cumsum((sizes + 63) // 64 * 64)])
+            arena = H.DeviceArray(int(off[-1]), 1, dtype)
+            arena.fill_uniform(int(off[-1]), 1, seed + s)
+            self.arenas.append(arena)
+            self.offsets.append(off[:-1])
+        H.synchronize()
+
+    def ptr(self, which, t):
+        return self.arenas[which].ptr + int(self.offsets[which][t]) * self.dtype.itemsize
+
+    def download(self, which, t, rows, cols):
+        out = np.empty((rows, cols), dtype=self.dtype, order="F")
+        if out.size:
+            self.H.lib().hwyblas_amd_memcpy(out.ctypes.data, self.ptr(which, t), out.nbytes, 2)
+        return out
+
+
+def batched_gemm_leg(H, oracle, dtype, count, steps, warmup, stream, configs=None, check=16,
+                     seed=7):
+    """`count` uneven GEMMs, m, n, k log-uniform in [8, 512], random trans,
+    uniform[-1, 1], alpha = 1, beta = 0: one grouped launch replayed from a
+    hipGraph, against a loop of single-call hwyblas_?gemm_dev launches."""
+    rng = np.random.default_rng(seed)
+    m, n, k = (log_uniform_ints(rng, 8, 512, count) for _ in range(3))
+    ta = rng.integers(0, 2, count)
+    tb = rng.integers(0, 2, count)
+    sa = [(k[t], m[t]) if ta[t] else (m[t], k[t]) for t in range(count)]
+    sb = [(n[t], k[t]) if tb[t] else (k[t], n[t]) for t in range(count)]
+    sc = [(m[t], n[t]) for t in range(count)]
+    batch = DeviceBatch(H, dtype, sa, sb, sc, seed=100)
+    probs = [H.GemmProblem(b"T" if ta[t] else b"N", b"T" if tb[t] else b"N", int(m[t]),
+                           int(n[t]), int(k[t]), 1.0, 0.0, batch.ptr(0, t), int(sa[t][0]),
+                           batch.ptr(1, t), int(sb[t][0]), batch.ptr(2, t), int(m[t]))
+             for t in range(count)]
+    t0 = time.perf_counter()
+    plan = H.Plan("gemm", dtype, probs, configs=configs)
+    plan_s = time.perf_counter() - t0
+    flops = float(np.sum(2.0 * m * n * k))
+    nbytes = float(np.sum((m * k + k * n + m * n) * np.dtype(dtype).itemsize))
+
+    def grouped_step():
+        plan.launch(stream, graph=True)
+    g_ms, _ = time_steps(H, stream, grouped_step, steps, warmup)
+
+    pre = "d" if np.dtype(dtype) == np.float64 else "s"
+    f = getattr(H.lib(), f"hwyblas_{pre}gemm_dev")
+    import ctypes
+    args = [(p.transa, p.transb, p.m, p.n, p.k, 1.0, ctypes.c_void_p(p.a), p.lda,
+             ctypes.c_void_p(p.b), p.ldb, 0.0, ctypes.c_void_p(p.c), p.ldc, stream)
+            for p in probs]
+
+    def loop_step():
+        for a in args:
+            f(*a)
+    l_ms, _ = time_steps(H, stream, loop_step, max(1, steps // 4), 1)
+
+    # Grouped result of the last replay, then checked problem by problem.
+    plan.launch(stream, graph=True)
+    H.lib().hwyblas_amd_stream_synchronize(stream)
+    worst = 0.0
+    for t in rng.choice(count, size=min(check, count), replace=False):
+        a = batch.download(0, t, *sa[t])
+        b = batch.download(1, t, *sb[t])
+        c = batch.download(2, t, *sc[t])
+        exp, bnd = oracle.gemm_ref("T" if ta[t] else "N", "T" if tb[t] else "N", int(m[t]),
+                                   int(n[t]), int(k[t]), 1.0, a, sa[t][0], b, sb[t][0], 0.0,
+                                   None, int(m[t]))
+        tol = 4.0 * (k[t] + 2) * np.finfo(dtype).eps
+        worst = max(worst, float(np.max(np.abs(c.astype(np.float64) - exp) / (tol * bnd))))
+    cfg_hist = np.bincount([plan.problem_config(t) for t in range(count)],
+                           minlength=len(H.grouped_configs())).tolist()
+    out = {"count": count, "dtype": "f64" if pre == "d" else "f32",
+           "grouped_ms": round(g_ms, 4), "loop_ms": round(l_ms, 4),
+           "gemm_per_s": round(count / (g_ms * 1e-3), 1),
+           "loop_gemm_per_s": round(count / (l_ms * 1e-3), 1),
+           "speedup_vs_loop": round(l_ms / g_ms, 3),
+           "tflops": round(flops / (g_ms * 1e-3) / 1e12, 3),
+           "gb_per_s_min_traffic": round(nbytes / (g_ms * 1e-3) / 1e9, 1),
+           "flops": flops, "work_items": plan.work_items, "plan_create_s": round(plan_s, 3),
+           "worst_bound_ratio": worst, "checked": int(min(check, count)),
+           "config_histogram": cfg_hist}
+    plan.destroy()
+    del batch
+    return out
+
+
+def batched_trsm_leg(H, oracle, count, steps, warmup, stream, check=16, seed=11):
+    """`count` DTRSMs L/L/N/N, m log-uniform in [8, 256], n in [1, 64]; A
+    lower triangular with uniform[-1, 1] off-diagonal entries and diagonal
+    m + uniform[0, 1]: one grouped launch replayed from a hipGraph, against a
+    loop of single-call hwyblas_dtrsm_dev launches."""
+    rng = np.random.default_rng(seed)
+    m = log_uniform_ints(rng, 8, 256, count)
+    n = log_uniform_ints(rng, 1, 64, count)
+    batch = DeviceBatch(H, np.float64, [(x, x) for x in m], [(m[t], n[t]) for t in range(count)],
+                        [(1, 1)] * count, seed=200)
+    # Diagonal m + U[0, 1]: one host pass over the A arena's diagonal entries.
+    total = int(batch.offsets[0][-1] + m[-1] * m[-1])
+    host = np.empty(total)
+    H.lib().hwyblas_amd_memcpy(host.ctypes.data, batch.arenas[0].ptr, host.nbytes, 2)
+    for t in range(count):
+        o, mm = int(batch.offsets[0][t]), int(m[t])
+        idx = o + np.arange(mm) * (mm + 1)
+        host[idx] = mm + (host[idx] + 1.0) * 0.5
+    H.lib().hwyblas_amd_memcpy(batch.arenas[0].ptr, host.ctypes.data, host.nbytes, 1)
+    del host
+    B0 = H.DeviceArray(batch.arenas[1].ld, 1)
+    nb = batch.arenas[1].ld * 8
+    H.lib().hwyblas_amd_memcpy(B0.ptr, batch.arenas[1].ptr, nb, 3)
+    probs = [H.TrsmProblem(b"L", b"L", b"N", b"N", int(m[t]), int(n[t]), 1.0, batch.ptr(0, t),
+                           int(m[t]), batch.ptr(1, t), int(m[t])) for t in range(count)]
+    plan = H.Plan("trsm", np.float64, probs)
+    flops = float(np.sum(m.astype(float) ** 2 * n))
+
+    def grouped_step():
+        plan.launch(stream, graph=True)
+    g_ms, _ = time_steps(H, stream, grouped_step, steps, warmup)
+    import ctypes
+    f = H.lib().hwyblas_dtrsm_dev
+    args = [(b"L", b"L", b"N", b"N", p.m, p.n, 1.0, ctypes.c_void_p(p.a), p.lda,
+             ctypes.c_void_p(p.b), p.ldb, stream) for p in probs]
+
+    def loop_step():
+        for a in args:
+            f(*a)
+    l_ms, _ = time_steps(H, stream, loop_step, max(1, steps // 4), 1)
+    # One solve from the original right-hand sides, checked by residual.
+    H.lib().hwyblas_amd_memcpy(batch.arenas[1].ptr, B0.ptr, nb, 3)
+    plan.launch(stream, graph=True)
+    H.lib().hwyblas_amd_stream_synchronize(stream)
+    worst = 0.0
+    for t in rng.choice(count, size=min(check, count), replace=False):
+        mm, nn = int(m[t]), int(n[t])
+        a = batch.download(0, t, mm, mm)
+        x = batch.download(1, t, mm, nn)
+        b0 = np.empty((mm, nn), order="F")
+        H.lib().hwyblas_amd_memcpy(b0.ctypes.data, B0.ptr + int(batch.offsets[1][t]) * 8,
+                                   b0.nbytes, 2)
+        r = oracle.trsm_residual("L", "L", "N", "N", mm, nn, 1.0, a, mm, b0, mm, x, mm,
+                                 np.arange(nn))
+        worst = max(worst, float(np.max(r) / (8.0 * (mm + 2) * np.finfo(np.float64).eps)))
+    out = {"count": count, "dtype": "f64", "grouped_ms": round(g_ms, 4),
+           "loop_ms": round(l_ms, 4), "solves_per_s": round(count / (g_ms * 1e-3), 1),
+           "loop_solves_per_s": round(count / (l_ms * 1e-3), 1),
+           "speedup_vs_loop": round(l_ms / g_ms, 3),
+           "tflops": round(flops / (g_ms * 1e-3) / 1e12, 4), "work_items": plan.work_items,
+           "worst_residual_ratio": worst, "checked": int(min(check, count))}
+    plan.destroy()
+    return out
+
+
 def cpu_baseline(oracle, seconds=12.0):
     """The oracle's single-threaded restatement of hwyBLAS's blocked DGEMM,
     DGEMM NN 2048^3 on one pinned core, best of the calls that fit in about
@@ -92,6 +258,23 @@ def cpu_baseline(oracle, seconds=12.0):
                       f"{calls} calls, 1 pinned core"}
 
 
+def run_batched(H, oracle, args, stream):
+    out = {}
+    for dtype, name in ((np.float64, "gemm_f64"), (np.float32, "gemm_f32")):
+        r = batched_gemm_leg(H, oracle, dtype, args.batch_count, max(3, args.steps),
+                             args.warmup, stream)
+        log(f"batched {name}: {r['gemm_per_s']:.0f} GEMM/s ({r['grouped_ms']:.2f} ms), loop "
+            f"{r['loop_ms']:.2f} ms, {r['speedup_vs_loop']:.2f}x, {r['tflops']:.2f} TFLOP/s, "
+            f"worst bound ratio {r['worst_bound_ratio']:.3g}")
+        out[name] = r
+    r = batched_trsm_leg(H, oracle, args.batch_count, max(3, args.steps), args.warmup, stream)
+    log(f"batched trsm_f64: {r['solves_per_s']:.0f} solves/s ({r['grouped_ms']:.2f} ms), loop "
+        f"{r['loop_ms']:.2f} ms, {r['speedup_vs_loop']:.2f}x, worst residual ratio "
+        f"{r['worst_residual_ratio']:.3g}")
+    out["trsm_f64"] = r
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +285,9 @@ def main():
     ap.add_argument("--samples", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-only", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=10000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (a kernel of known traffic, "
                          "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
@@ -127,6 +313,11 @@ def main():
         L.hwyblas_amd_memcpy(dst.ptr, src.ptr, 1 << 30, 3)
         H.synchronize()
         del src, dst
+
+    if args.batched_only:
+        out = run_batched(H, oracle, args, stream)
+        print(json.dumps({"batched": out}), flush=True)
+        return
 
     # Native FP64 MFMA DGEMM.
     def native_step():
@@ -179,6 +370,8 @@ def main():
                "bound_4(k+2)u": 4 * (n + 2) * 2.0 ** -53}
         log(f"sampled error: emulated {e_emu:.3e}, native {e_nat:.3e}")
 
+    batched = None if args.no_batched else run_batched(H, oracle, args, stream)
+
     cpu = None if args.no_cpu else cpu_baseline(oracle)
     if cpu:
         log(f"cpu baseline {cpu['value'] * 1e3:.2f} GFLOP/s")
@@ -212,6 +405,7 @@ def main():
                             "2 m n k per modulus",
                      "frac": round(int8_tops / PEAK_INT8, 4), "traffic": None},
         "error": err,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

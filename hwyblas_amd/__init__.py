@@ -58,6 +58,16 @@ def _declare(L):
                                              _i, _p])
     fn("hwyblas_dgemm_emulated_dev", _i, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i,
                                           _i, _p, _pi])
+    fn("hwyblas_gemm_plan_create", _p, [_i, _i, _p, _p, _pi, _pi])
+    fn("hwyblas_trsm_plan_create", _p, [_i, _i, _p, _pi, _pi])
+    fn("hwyblas_plan_launch", None, [_p, _p])
+    fn("hwyblas_plan_launch_graph", None, [_p, _p])
+    fn("hwyblas_plan_work_items", _i, [_p])
+    fn("hwyblas_plan_problem_config", _i, [_p, _i])
+    fn("hwyblas_plan_destroy", None, [_p])
+    fn("hwyblas_grouped_config_count", _i, [])
+    fn("hwyblas_grouped_config_info", _i, [_i, _pi, _pi, _pi])
+    fn("hwyblas_grouped_default_config", _i, [_i, _i, _i, _i])
     fn("hwyblas_gemm_config_count", _i, [])
     fn("hwyblas_gemm_config_info", _i, [_i, _pi, _pi, _pi, _pi])
     fn("hwyblas_gemm_default_config", _i, [_i, _c, _c, _i, _i, _i])
@@ -112,6 +122,8 @@ def _ptr(x):
         return None
     if isinstance(x, DeviceArray):
         return ctypes.c_void_p(x.ptr)
+    if isinstance(x, int):  # a raw device address
+        return ctypes.c_void_p(x)
     return x.ctypes.data_as(ctypes.c_void_p)
 
 
@@ -274,3 +286,112 @@ def synchronize():
 
 def device_count():
     return lib().hwyblas_amd_device_count()
+
+
+# ---------------------------------------------------------------------------
+# Batched (grouped) plans
+
+class GemmProblem(ctypes.Structure):
+    """hwyblas_gemm_problem (include/hwyblas_amd.h)."""
+    _fields_ = [("transa", ctypes.c_char), ("transb", ctypes.c_char), ("m", _i), ("n", _i),
+                ("k", _i), ("alpha", _d), ("beta", _d), ("a", _p), ("lda", _i), ("b", _p),
+                ("ldb", _i), ("c", _p), ("ldc", _i)]
+
+
+class TrsmProblem(ctypes.Structure):
+    """hwyblas_trsm_problem (include/hwyblas_amd.h)."""
+    _fields_ = [("side", ctypes.c_char), ("uplo", ctypes.c_char), ("transa", ctypes.c_char),
+                ("diag", ctypes.c_char), ("m", _i), ("n", _i), ("alpha", _d), ("a", _p),
+                ("lda", _i), ("b", _p), ("ldb", _i)]
+
+
+class PlanError(ValueError):
+    def __init__(self, info, index):
+        super().__init__(f"problem {index}: invalid argument {info}")
+        self.info, self.index = info, index
+
+
+class Plan:
+    """A grouped GEMM or TRSM plan over device-resident problems."""
+
+    def __init__(self, kind, dtype, problems, configs=None):
+        self.kind, self.dtype = kind, np.dtype(dtype)
+        self.count = len(problems)
+        arr = (GemmProblem if kind == "gemm" else TrsmProblem) * max(1, self.count)
+        self._problems = arr(*problems)
+        info, bad = ctypes.c_int(0), ctypes.c_int(-1)
+        if kind == "gemm":
+            cfg = None
+            if configs is not None:
+                self._cfg = np.ascontiguousarray(configs, dtype=np.int32)
+                cfg = self._cfg.ctypes.data_as(ctypes.c_void_p)
+            self.ptr = lib().hwyblas_gemm_plan_create(self.dtype.itemsize, self.count,
+                                                      self._problems, cfg, ctypes.byref(info),
+                                                      ctypes.byref(bad))
+        else:
+            self.ptr = lib().hwyblas_trsm_plan_create(self.dtype.itemsize, self.count,
+                                                      self._problems, ctypes.byref(info),
+                                                      ctypes.byref(bad))
+        if not self.ptr:
+            raise PlanError(info.value, bad.value)
+
+    def launch(self, stream=None, graph=False):
+        f = lib().hwyblas_plan_launch_graph if graph else lib().hwyblas_plan_launch
+        f(self.ptr, stream)
+
+    @property
+    def work_items(self):
+        return lib().hwyblas_plan_work_items(self.ptr)
+
+    def problem_config(self, i):
+        return lib().hwyblas_plan_problem_config(self.ptr, i)
+
+    def destroy(self):
+        if getattr(self, "ptr", None):
+            lib().hwyblas_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def grouped_configs():
+    out = []
+    for i in range(lib().hwyblas_grouped_config_count()):
+        v = [ctypes.c_int() for _ in range(3)]
+        lib().hwyblas_grouped_config_info(i, *[ctypes.byref(x) for x in v])
+        out.append(tuple(x.value for x in v))
+    return out
+
+
+class Arena:
+    """One device allocation holding many column-major matrices (offsets in
+    elements), filled from host arrays with one copy."""
+
+    def __init__(self, host_blocks, dtype):
+        self.dtype = np.dtype(dtype)
+        self.offsets, total = [], 0
+        for b in host_blocks:
+            self.offsets.append(total)
+            total += (b.size + 31) // 32 * 32
+        self.total = max(total, 1)
+        host = np.zeros(self.total, dtype=self.dtype)
+        for off, b in zip(self.offsets, host_blocks):
+            host[off:off + b.size] = np.asfortranarray(b).ravel(order="F")
+        self.dev = DeviceArray(self.total, 1, self.dtype)
+        lib().hwyblas_amd_memcpy(self.dev.ptr, host.ctypes.data, host.nbytes, 1)
+
+    def ptr(self, i):
+        return self.dev.ptr + self.offsets[i] * self.dtype.itemsize
+
+    def download(self):
+        out = np.empty(self.total, dtype=self.dtype)
+        lib().hwyblas_amd_memcpy(out.ctypes.data, self.dev.ptr, out.nbytes, 2)
+        return out
+
+    def block(self, flat, i, shape):
+        off = self.offsets[i]
+        return flat[off:off + shape[0] * shape[1]].reshape(shape, order="F")

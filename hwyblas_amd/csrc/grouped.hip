@@ -1,0 +1,604 @@
+// Batched ("grouped") GEMM and TRSM over many independent, uneven problems
+// in one persistent launch.
+//
+// A plan uploads the problem descriptors and a work list once. Work items
+// are GEMM tiles (problem, tile row, tile column, tile configuration) or
+// TRSM chunks (problem, block of 64 right-hand sides). The launch is a tiny
+// kernel that resets the work counter followed by the persistent kernel. Its
+// grid fills the GPU and every wave is an independent worker: a wave claims
+// the next item from the counter (one atomic per item, by lane 0), runs it
+// and claims again until the list is exhausted, so no wave waits on another
+// and the load balances itself over uneven sizes. Items are listed largest
+// first. The two launches touch no host memory and use no memset, so a plan
+// can be captured in a hipGraph and replayed.
+//
+// GEMM tiles run on the MFMA units without LDS: each lane loads its MFMA
+// operand elements directly (for op(X) = X the 16 lanes of a row group read
+// 16 consecutive elements), k advances in steps of 4 x UK with the loads of
+// a step issued together. The epilogue applies hwyBLAS's Store rules
+// (src/gemm_kernel-inl.h:181-224). Per problem, (k == 0 or alpha == 0) with
+// beta == 1 adds no items (C untouched); alpha == 0 never reads A or B.
+//
+// TRSM chunks solve op(A) X = alpha B (left) or X op(A) = alpha B (right)
+// for up to 64 right-hand sides, one per lane: B := alpha B, then rows are
+// solved in blocks of 32 held in registers, each element as
+// (b - sum_q x_q t) * (1 / t_jj) with the sum fused in solve order, as
+// hwyBLAS's base kernel computes it (src/trsm-inl.h:107-174).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "grouped.h"
+
+namespace hwy_amd {
+namespace grouped {
+namespace {
+
+typedef double dx4 __attribute__((ext_vector_type(4)));
+typedef float fx4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using Acc = dx4;
+  static __device__ __forceinline__ Acc Run(double a, double b, Acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int Row(int lk, int r) { return lk + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+  using Acc = fx4;
+  static __device__ __forceinline__ Acc Run(float a, float b, Acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int Row(int lk, int r) { return 4 * lk + r; }
+};
+
+template <typename T>
+struct GemmDesc {
+  const T* a;
+  const T* b;
+  T* c;
+  int64_t lda, ldb, ldc;
+  int m, n, k;   // k = 0 when alpha == 0 (A and B are not read)
+  int ta, tb;
+  T alpha, beta;
+};
+
+struct WorkItem {
+  int problem;
+  int i0, j0;  // first row / column (GEMM) or first right-hand side (TRSM)
+  int cfg;
+};
+
+// One wave computes the BM x BN tile of C at (i0, j0).
+template <typename T, int BM, int BN, int UK>
+__device__ void GemmTile(const GemmDesc<T>& d, int i0, int j0) {
+  using M = Mfma<T>;
+  using Acc = typename M::Acc;
+  constexpr int TM = BM / 16, TN = BN / 16;
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  Acc acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = Acc{};
+
+  // op(A)(i, p) and op(B)(p, j) element offsets.
+  const int64_t a_is = d.ta ? d.lda : 1, a_ps = d.ta ? 1 : d.lda;
+  const int64_t b_ps = d.tb ? d.ldb : 1, b_js = d.tb ? 1 : d.ldb;
+  int rows[TM], cols[TN];
+  bool rok[TM], cok[TN];
+#pragma unroll
+  for (int ti = 0; ti < TM; ++ti) {
+    rows[ti] = i0 + ti * 16 + li;
+    rok[ti] = rows[ti] < d.m;
+  }
+#pragma unroll
+  for (int tj = 0; tj < TN; ++tj) {
+    cols[tj] = j0 + tj * 16 + li;
+    cok[tj] = cols[tj] < d.n;
+  }
+  for (int p0 = 0; p0 < d.k; p0 += 4 * UK) {
+    T af[UK][TM], bf[UK][TN];
+#pragma unroll
+    for (int u = 0; u < UK; ++u) {
+      const int p = p0 + 4 * u + lk;
+      const bool pok = p < d.k;
+#pragma unroll
+      for (int ti = 0; ti < TM; ++ti)
+        af[u][ti] = (rok[ti] && pok) ? d.a[rows[ti] * a_is + p * a_ps] : T(0);
+#pragma unroll
+      for (int tj = 0; tj < TN; ++tj)
+        bf[u][tj] = (cok[tj] && pok) ? d.b[p * b_ps + cols[tj] * b_js] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < UK; ++u)
+#pragma unroll
+      for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TN; ++tj) acc[ti][tj] = M::Run(af[u][ti], bf[u][tj], acc[ti][tj]);
+  }
+  const T alpha = d.alpha, beta = d.beta;
+  const bool beta_zero = beta == T(0), beta_one = beta == T(1);
+#pragma unroll
+  for (int ti = 0; ti < TM; ++ti) {
+#pragma unroll
+    for (int tj = 0; tj < TN; ++tj) {
+      if (!cok[tj]) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = i0 + ti * 16 + M::Row(lk, r);
+        if (row < d.m) {
+          T* cij = d.c + row + cols[tj] * d.ldc;
+          T v = d.k ? acc[ti][tj][r] * alpha : T(0);
+          if (!beta_zero) v = beta_one ? v + *cij : __builtin_fma(*cij, beta, v);
+          *cij = v;
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void RunGemmItem(const GemmDesc<T>& d, const WorkItem& w) {
+  switch (w.cfg) {
+#define HWY_GROUPED_CASE(ID, BM, BN, UK) \
+  case ID:                               \
+    GemmTile<T, BM, BN, UK>(d, w.i0, w.j0); break;
+    HWY_GROUPED_GEMM_CONFIGS(HWY_GROUPED_CASE)
+#undef HWY_GROUPED_CASE
+    default: break;
+  }
+}
+
+__global__ void ResetCounter(int* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+// Claims the next work item for this wave (the same index in every lane).
+__device__ __forceinline__ int ClaimItem(int* counter) {
+  int t = 0;
+  if ((threadIdx.x & 63) == 0) t = atomicAdd(counter, 1);
+  return __shfl(t, 0, 64);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    GroupedGemmKernel(const GemmDesc<T>* descs, const WorkItem* items, int nitems, int* counter) {
+  for (;;) {
+    const int t = ClaimItem(counter);
+    if (t >= nitems) break;
+    const WorkItem w = items[t];
+    const GemmDesc<T> d = descs[w.problem];
+    RunGemmItem<T>(d, w);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// TRSM
+
+template <typename T>
+struct TrsmDesc {
+  const T* a;
+  T* b;
+  int64_t lda, ldb;
+  int m, n;
+  int left, lower_solve;  // lower_solve: L(i, q) is lower triangular
+  int trans, unit;
+  int lower_stored;       // uplo
+  T alpha;
+};
+
+constexpr int kTrsmBlock = 32;
+
+// L(i, q): the solve matrix, op(A) for the left side and op(A)^T for the
+// right side; op(A)(i, q) = A(i, q) or A(q, i).
+template <typename T>
+__device__ __forceinline__ T SolveElem(const TrsmDesc<T>& d, int i, int q) {
+  const int r = d.left ? i : q, c = d.left ? q : i;   // op(A)(r, c)
+  const int64_t off = d.trans ? int64_t(c) + int64_t(r) * d.lda : int64_t(r) + int64_t(c) * d.lda;
+  return d.a[off];
+}
+
+// Solves for the right-hand sides v0 .. v0 + 63 (one per lane). Element e of
+// right-hand side v is B(e, v) (left) or B(v, e) (right).
+template <typename T>
+__device__ void TrsmChunk(const TrsmDesc<T>& d, int v0) {
+  const int lane = threadIdx.x & 63;
+  const int v = v0 + lane;
+  const int len = d.left ? d.m : d.n;      // system size
+  const int nv = d.left ? d.n : d.m;       // right-hand sides
+  const bool active = v < nv;
+  auto elem = [&](int e) -> T& {
+    return d.left ? d.b[int64_t(e) + int64_t(v) * d.ldb] : d.b[int64_t(v) + int64_t(e) * d.ldb];
+  };
+  const bool asc = d.lower_solve;
+  const int nblk = (len + kTrsmBlock - 1) / kTrsmBlock;
+  for (int bi = 0; bi < nblk; ++bi) {
+    // Block rows [s, s + cnt) in solve order; position r of the block is row
+    // s + r (ascending) or e1 - 1 - r (descending).
+    const int e0 = asc ? bi * kTrsmBlock : max(0, len - (bi + 1) * kTrsmBlock);
+    const int e1 = asc ? min(len, (bi + 1) * kTrsmBlock) : len - bi * kTrsmBlock;
+    const int cnt = e1 - e0;
+    T x[kTrsmBlock];
+#pragma unroll
+    for (int r = 0; r < kTrsmBlock; ++r) {
+      const int e = asc ? e0 + r : e1 - 1 - r;
+      x[r] = (active && r < cnt) ? elem(e) : T(0);
+      if (d.alpha != T(1)) x[r] = (d.alpha == T(0)) ? T(0) : x[r] * d.alpha;
+    }
+    if (d.alpha != T(0)) {
+      // Contributions of the rows solved in earlier blocks, in solve order.
+      const int solved = bi * kTrsmBlock;
+      for (int s = 0; s < solved; ++s) {
+        const int q = asc ? s : len - 1 - s;
+        const T xq = active ? elem(q) : T(0);
+#pragma unroll
+        for (int r = 0; r < kTrsmBlock; ++r) {
+          if (r < cnt) {
+            const int e = asc ? e0 + r : e1 - 1 - r;
+            x[r] = __builtin_fma(-SolveElem(d, e, q), xq, x[r]);
+          }
+        }
+      }
+      // The diagonal block, column by column.
+#pragma unroll
+      for (int r = 0; r < kTrsmBlock; ++r) {
+        if (r < cnt) {
+          const int e = asc ? e0 + r : e1 - 1 - r;
+          if (!d.unit) x[r] = x[r] * (T(1) / SolveElem(d, e, e));
+#pragma unroll
+          for (int r2 = r + 1; r2 < kTrsmBlock; ++r2) {
+            if (r2 < cnt) {
+              const int e2 = asc ? e0 + r2 : e1 - 1 - r2;
+              x[r2] = __builtin_fma(-SolveElem(d, e2, e), x[r], x[r2]);
+            }
+          }
+        }
+      }
+    }
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < kTrsmBlock; ++r)
+        if (r < cnt) elem(asc ? e0 + r : e1 - 1 - r) = x[r];
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    GroupedTrsmKernel(const TrsmDesc<T>* descs, const WorkItem* items, int nitems, int* counter) {
+  for (;;) {
+    const int t = ClaimItem(counter);
+    if (t >= nitems) break;
+    const WorkItem w = items[t];
+    const TrsmDesc<T> d = descs[w.problem];
+    TrsmChunk<T>(d, w.i0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plans
+
+int DeviceCUs() {
+  int dev = 0, cus = 0;
+  HWY_HIP_CHECK(hipGetDevice(&dev));
+  HWY_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cus;
+}
+
+}  // namespace
+
+struct Plan {
+  int kind;  // 0 GEMM, 1 TRSM
+  int dtype_bytes;
+  void* descs = nullptr;
+  WorkItem* items = nullptr;
+  int* counter = nullptr;
+  int nitems = 0;
+  int blocks = 0;
+  int device = -1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int> cfg_of_problem;
+};
+
+namespace {
+
+double GemmCost(const GemmConfig& c, int m, int n, int k) {
+  return double((m + c.bm - 1) / c.bm) * ((n + c.bn - 1) / c.bn) * c.bm * c.bn * double(k + 1);
+}
+
+template <typename T>
+Plan* MakeGemmPlan(int count, const hwyblas_gemm_problem* probs, const int* cfgs, int* info,
+                   int* bad) {
+  *info = 0;
+  *bad = -1;
+  std::vector<GemmDesc<T>> descs(count);
+  std::vector<WorkItem> items;
+  std::vector<double> cost;
+  std::vector<int> chosen(count, -1);
+  for (int i = 0; i < count; ++i) {
+    const hwyblas_gemm_problem& p = probs[i];
+    Op oa, ob;
+    const bool va = DecodeOp(p.transa, &oa), vb = DecodeOp(p.transb, &ob);
+    const int rows_a = oa == Op::kNoTrans ? p.m : p.k, rows_b = ob == Op::kNoTrans ? p.k : p.n;
+    int e = 0;
+    if (!va) e = 1;
+    else if (!vb) e = 2;
+    else if (p.m < 0) e = 3;
+    else if (p.n < 0) e = 4;
+    else if (p.k < 0) e = 5;
+    else if (p.lda < std::max(1, rows_a)) e = 8;
+    else if (p.ldb < std::max(1, rows_b)) e = 10;
+    else if (p.ldc < std::max(1, p.m)) e = 13;
+    if (e != 0) {
+      *info = e;
+      *bad = i;
+      ReportError(sizeof(T) == 8 ? "DGEMM" : "SGEMM", e);
+      return nullptr;
+    }
+    GemmDesc<T>& d = descs[i];
+    d.a = static_cast<const T*>(p.a);
+    d.b = static_cast<const T*>(p.b);
+    d.c = static_cast<T*>(p.c);
+    d.lda = p.lda;
+    d.ldb = p.ldb;
+    d.ldc = p.ldc;
+    d.m = p.m;
+    d.n = p.n;
+    d.ta = oa == Op::kTrans;
+    d.tb = ob == Op::kTrans;
+    d.alpha = T(p.alpha);
+    d.beta = T(p.beta);
+    const bool no_product = p.k == 0 || d.alpha == T(0);
+    d.k = no_product ? 0 : p.k;
+    if (p.m == 0 || p.n == 0) continue;
+    if (no_product && d.beta == T(1)) continue;
+    int cfg = cfgs ? cfgs[i] : -1;
+    if (cfg < 0 || cfg >= kNumGroupedGemmConfigs)
+      cfg = DefaultGroupedConfig(int(sizeof(T)), p.m, p.n, p.k);
+    chosen[i] = cfg;
+    const GemmConfig& gc = kGroupedGemmConfigs[cfg];
+    for (int j0 = 0; j0 < p.n; j0 += gc.bn)
+      for (int i0 = 0; i0 < p.m; i0 += gc.bm) {
+        items.push_back({i, i0, j0, cfg});
+        cost.push_back(GemmCost(gc, std::min(gc.bm, p.m - i0), std::min(gc.bn, p.n - j0), d.k));
+      }
+  }
+  // Largest items first (longest-processing-time order).
+  std::vector<int> order(items.size());
+  for (size_t t = 0; t < order.size(); ++t) order[t] = int(t);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  std::vector<WorkItem> sorted(items.size());
+  for (size_t t = 0; t < order.size(); ++t) sorted[t] = items[order[t]];
+
+  Plan* plan = new Plan();
+  plan->kind = 0;
+  plan->dtype_bytes = int(sizeof(T));
+  plan->nitems = int(sorted.size());
+  plan->cfg_of_problem = chosen;
+  HWY_HIP_CHECK(hipGetDevice(&plan->device));
+  HWY_HIP_CHECK(hipMalloc(&plan->descs, std::max<size_t>(1, descs.size()) * sizeof(GemmDesc<T>)));
+  HWY_HIP_CHECK(hipMalloc(&plan->items, std::max<size_t>(1, sorted.size()) * sizeof(WorkItem)));
+  HWY_HIP_CHECK(hipMalloc(&plan->counter, sizeof(int)));
+  if (!descs.empty())
+    HWY_HIP_CHECK(hipMemcpy(plan->descs, descs.data(), descs.size() * sizeof(GemmDesc<T>),
+                            hipMemcpyHostToDevice));
+  if (!sorted.empty())
+    HWY_HIP_CHECK(hipMemcpy(plan->items, sorted.data(), sorted.size() * sizeof(WorkItem),
+                            hipMemcpyHostToDevice));
+  plan->blocks = std::max(1, std::min(DeviceCUs() * 4, (plan->nitems + 3) / 4));
+  return plan;
+}
+
+template <typename T>
+Plan* MakeTrsmPlan(int count, const hwyblas_trsm_problem* probs, int* info, int* bad) {
+  *info = 0;
+  *bad = -1;
+  std::vector<TrsmDesc<T>> descs(count);
+  std::vector<WorkItem> items;
+  std::vector<double> cost;
+  for (int i = 0; i < count; ++i) {
+    const hwyblas_trsm_problem& p = probs[i];
+    bool left, lower, unit;
+    Op op;
+    const bool vs = DecodeSide(p.side, &left), vu = DecodeUplo(p.uplo, &lower);
+    const bool vt = DecodeOp(p.transa, &op), vd = DecodeDiag(p.diag, &unit);
+    const int rows_a = left ? p.m : p.n;
+    int e = 0;
+    if (!vs) e = 1;
+    else if (!vu) e = 2;
+    else if (!vt) e = 3;
+    else if (!vd) e = 4;
+    else if (p.m < 0) e = 5;
+    else if (p.n < 0) e = 6;
+    else if (p.lda < std::max(1, rows_a)) e = 9;
+    else if (p.ldb < std::max(1, p.m)) e = 11;
+    if (e != 0) {
+      *info = e;
+      *bad = i;
+      ReportError(sizeof(T) == 8 ? "DTRSM" : "STRSM", e);
+      return nullptr;
+    }
+    TrsmDesc<T>& d = descs[i];
+    d.a = static_cast<const T*>(p.a);
+    d.b = static_cast<T*>(p.b);
+    d.lda = p.lda;
+    d.ldb = p.ldb;
+    d.m = p.m;
+    d.n = p.n;
+    d.left = left;
+    d.trans = op == Op::kTrans;
+    d.unit = unit;
+    d.lower_stored = lower;
+    // op(A) is lower iff (uplo lower) xor trans; L = op(A) (left) or op(A)^T.
+    const bool op_lower = lower != d.trans;
+    d.lower_solve = left ? op_lower : !op_lower;
+    d.alpha = T(p.alpha);
+    if (p.m == 0 || p.n == 0) continue;
+    const int len = left ? p.m : p.n, nv = left ? p.n : p.m;
+    for (int v0 = 0; v0 < nv; v0 += 64) {
+      items.push_back({i, v0, 0, 0});
+      cost.push_back(double(len) * len);
+    }
+  }
+  std::vector<int> order(items.size());
+  for (size_t t = 0; t < order.size(); ++t) order[t] = int(t);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  std::vector<WorkItem> sorted(items.size());
+  for (size_t t = 0; t < order.size(); ++t) sorted[t] = items[order[t]];
+  Plan* plan = new Plan();
+  plan->kind = 1;
+  plan->dtype_bytes = int(sizeof(T));
+  plan->nitems = int(sorted.size());
+  HWY_HIP_CHECK(hipGetDevice(&plan->device));
+  HWY_HIP_CHECK(hipMalloc(&plan->descs, std::max<size_t>(1, descs.size()) * sizeof(TrsmDesc<T>)));
+  HWY_HIP_CHECK(hipMalloc(&plan->items, std::max<size_t>(1, sorted.size()) * sizeof(WorkItem)));
+  HWY_HIP_CHECK(hipMalloc(&plan->counter, sizeof(int)));
+  if (!descs.empty())
+    HWY_HIP_CHECK(hipMemcpy(plan->descs, descs.data(), descs.size() * sizeof(TrsmDesc<T>),
+                            hipMemcpyHostToDevice));
+  if (!sorted.empty())
+    HWY_HIP_CHECK(hipMemcpy(plan->items, sorted.data(), sorted.size() * sizeof(WorkItem),
+                            hipMemcpyHostToDevice));
+  plan->blocks = std::max(1, std::min(DeviceCUs() * 4, (plan->nitems + 3) / 4));
+  return plan;
+}
+
+}  // namespace
+
+void Enqueue(const Plan* plan, hipStream_t stream) {
+  if (plan->nitems == 0) return;
+  hipLaunchKernelGGL(ResetCounter, dim3(1), dim3(64), 0, stream, plan->counter);
+  const dim3 grid(unsigned(plan->blocks)), block(256);
+  if (plan->kind == 0) {
+    if (plan->dtype_bytes == 8)
+      hipLaunchKernelGGL(GroupedGemmKernel<double>, grid, block, 0, stream,
+                         static_cast<const GemmDesc<double>*>(plan->descs), plan->items,
+                         plan->nitems, plan->counter);
+    else
+      hipLaunchKernelGGL(GroupedGemmKernel<float>, grid, block, 0, stream,
+                         static_cast<const GemmDesc<float>*>(plan->descs), plan->items,
+                         plan->nitems, plan->counter);
+  } else {
+    if (plan->dtype_bytes == 8)
+      hipLaunchKernelGGL(GroupedTrsmKernel<double>, grid, block, 0, stream,
+                         static_cast<const TrsmDesc<double>*>(plan->descs), plan->items,
+                         plan->nitems, plan->counter);
+    else
+      hipLaunchKernelGGL(GroupedTrsmKernel<float>, grid, block, 0, stream,
+                         static_cast<const TrsmDesc<float>*>(plan->descs), plan->items,
+                         plan->nitems, plan->counter);
+  }
+  HWY_HIP_CHECK(hipGetLastError());
+}
+
+void LaunchGraph(Plan* plan, hipStream_t stream) {
+  if (plan->nitems == 0) return;
+  if (!plan->exec) {
+    // Capture on a private stream (the caller's may already be capturing).
+    hipStream_t cap;
+    HWY_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    HWY_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    Enqueue(plan, cap);
+    HWY_HIP_CHECK(hipStreamEndCapture(cap, &plan->graph));
+    HWY_HIP_CHECK(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+    HWY_HIP_CHECK(hipStreamDestroy(cap));
+  }
+  HWY_HIP_CHECK(hipGraphLaunch(plan->exec, stream));
+}
+
+void Destroy(Plan* plan) {
+  if (!plan) return;
+  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+  if (plan->graph) (void)hipGraphDestroy(plan->graph);
+  (void)hipFree(plan->descs);
+  (void)hipFree(plan->items);
+  (void)hipFree(plan->counter);
+  delete plan;
+}
+
+// Without tuning data: the largest tile that gives the problem at least a
+// few tiles, k unrolled when k is long.
+int DefaultGroupedConfig(int dtype_bytes, int m, int n, int k) {
+  (void)dtype_bytes;
+  const int uk = k >= 64 ? 1 : 0;
+  if (m >= 128 && n >= 128) return 8 + uk;  // 64 x 64
+  if (m >= 64 && n >= 64) return 2 + uk;    // 32 x 32
+  return 0 + uk;                            // 16 x 16
+}
+
+}  // namespace grouped
+}  // namespace hwy_amd
+
+using namespace hwy_amd;
+
+extern "C" {
+
+HWYBLAS_AMD_API void* hwyblas_gemm_plan_create(int dtype_bytes, int count,
+                                               const hwyblas_gemm_problem* problems,
+                                               const int* configs, int* info, int* bad_index) {
+  int i = 0, b = -1;
+  grouped::Plan* p = (dtype_bytes == 8)
+                         ? grouped::MakeGemmPlan<double>(count, problems, configs, &i, &b)
+                         : grouped::MakeGemmPlan<float>(count, problems, configs, &i, &b);
+  if (info) *info = i;
+  if (bad_index) *bad_index = b;
+  return p;
+}
+
+HWYBLAS_AMD_API void* hwyblas_trsm_plan_create(int dtype_bytes, int count,
+                                               const hwyblas_trsm_problem* problems, int* info,
+                                               int* bad_index) {
+  int i = 0, b = -1;
+  grouped::Plan* p = (dtype_bytes == 8)
+                         ? grouped::MakeTrsmPlan<double>(count, problems, &i, &b)
+                         : grouped::MakeTrsmPlan<float>(count, problems, &i, &b);
+  if (info) *info = i;
+  if (bad_index) *bad_index = b;
+  return p;
+}
+
+HWYBLAS_AMD_API void hwyblas_plan_launch(void* plan, void* stream) {
+  grouped::Enqueue(static_cast<grouped::Plan*>(plan), static_cast<hipStream_t>(stream));
+}
+
+HWYBLAS_AMD_API void hwyblas_plan_launch_graph(void* plan, void* stream) {
+  grouped::LaunchGraph(static_cast<grouped::Plan*>(plan), static_cast<hipStream_t>(stream));
+}
+
+HWYBLAS_AMD_API int hwyblas_plan_work_items(void* plan) {
+  return plan ? static_cast<grouped::Plan*>(plan)->nitems : 0;
+}
+
+HWYBLAS_AMD_API int hwyblas_plan_problem_config(void* plan, int problem) {
+  auto* p = static_cast<grouped::Plan*>(plan);
+  if (!p || problem < 0 || problem >= int(p->cfg_of_problem.size())) return -1;
+  return p->cfg_of_problem[problem];
+}
+
+HWYBLAS_AMD_API void hwyblas_plan_destroy(void* plan) {
+  grouped::Destroy(static_cast<grouped::Plan*>(plan));
+}
+
+HWYBLAS_AMD_API int hwyblas_grouped_config_count(void) { return grouped::kNumGroupedGemmConfigs; }
+
+HWYBLAS_AMD_API int hwyblas_grouped_config_info(int cfg, int* bm, int* bn, int* unroll) {
+  if (cfg < 0 || cfg >= grouped::kNumGroupedGemmConfigs) return -1;
+  *bm = grouped::kGroupedGemmConfigs[cfg].bm;
+  *bn = grouped::kGroupedGemmConfigs[cfg].bn;
+  *unroll = grouped::kGroupedGemmConfigs[cfg].uk;
+  return 0;
+}
+
+HWYBLAS_AMD_API int hwyblas_grouped_default_config(int dtype_bytes, int m, int n, int k) {
+  return grouped::DefaultGroupedConfig(dtype_bytes, m, n, k);
+}
+
+}  // extern "C"

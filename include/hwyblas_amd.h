@@ -192,6 +192,61 @@ HWYBLAS_AMD_API int hwyblas_dgemm_emulated_dev(char transa, char transb, int m, 
                                                int* used_moduli);
 
 /* ------------------------------------------------------------------------ */
+/* Batched (grouped) GEMM and TRSM over many independent, uneven problems (no
+ * hwyBLAS counterpart: hwyBLAS solves one problem per call). A plan checks
+ * every problem as the Fortran routine would (the first invalid one is
+ * reported under "DGEMM"/"SGEMM"/"DTRSM"/"STRSM" with its position; *info
+ * and *bad_index receive them and NULL is returned), uploads descriptors and
+ * a work list once, and then runs all problems in one persistent launch,
+ * directly or through a captured HIP graph. Matrices are device pointers;
+ * the problem array and `configs` are host memory. dtype_bytes is 8 (double)
+ * or 4 (float; alpha and beta are converted). configs[i] selects the tile
+ * configuration of GEMM problem i (see hwyblas_grouped_config_info; -1 or a
+ * NULL array: the built-in choice). Problems must not overlap in their
+ * output. Launches are asynchronous on `stream`; a plan may be launched
+ * again (the operands may change, not the problem list). */
+typedef struct hwyblas_gemm_problem {
+  char transa, transb;
+  int m, n, k;
+  double alpha, beta;
+  const void* a;
+  int lda;
+  const void* b;
+  int ldb;
+  void* c;
+  int ldc;
+} hwyblas_gemm_problem;
+
+typedef struct hwyblas_trsm_problem {
+  char side, uplo, transa, diag;
+  int m, n;
+  double alpha;
+  const void* a;
+  int lda;
+  void* b;
+  int ldb;
+} hwyblas_trsm_problem;
+
+HWYBLAS_AMD_API void* hwyblas_gemm_plan_create(int dtype_bytes, int count,
+                                               const hwyblas_gemm_problem* problems,
+                                               const int* configs, int* info,
+                                               int* bad_index);
+HWYBLAS_AMD_API void* hwyblas_trsm_plan_create(int dtype_bytes, int count,
+                                               const hwyblas_trsm_problem* problems,
+                                               int* info, int* bad_index);
+HWYBLAS_AMD_API void hwyblas_plan_launch(void* plan, void* stream);
+/* As hwyblas_plan_launch through a HIP graph captured at the first call and
+ * replayed afterwards. */
+HWYBLAS_AMD_API void hwyblas_plan_launch_graph(void* plan, void* stream);
+HWYBLAS_AMD_API int hwyblas_plan_work_items(void* plan);
+/* The tile configuration a GEMM plan uses for problem i (-1: no work). */
+HWYBLAS_AMD_API int hwyblas_plan_problem_config(void* plan, int problem);
+HWYBLAS_AMD_API void hwyblas_plan_destroy(void* plan);
+HWYBLAS_AMD_API int hwyblas_grouped_config_count(void);
+HWYBLAS_AMD_API int hwyblas_grouped_config_info(int cfg, int* bm, int* bn, int* unroll);
+HWYBLAS_AMD_API int hwyblas_grouped_default_config(int dtype_bytes, int m, int n, int k);
+
+/* ------------------------------------------------------------------------ */
 /* Tile configurations of the plain GEMM kernel (no hwyBLAS counterpart: its
  * blocking is compile-time, src/gemm_kernel-inl.h:72-74). For the autotuner
  * and the cost model: the configurations are numbered 0..count-1; every one

@@ -168,3 +168,42 @@ def test_handler_is_replaced_and_restored():
 def test_version_and_device_count_without_gpu():
     assert b"gfx950" in H.lib().hwyblas_amd_version()
     assert H.device_count() >= 0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("change,info", GEMM_CASES)
+def test_grouped_gemm_plan_rejects_invalid_problem(dtype, change, info):
+    """A plan is refused, with the problem's index and the single-call info
+    position, before any device work; the handler sees ?GEMM and the info."""
+    g = _gemm_args(**change)
+    good = H.GemmProblem(b"N", b"N", 4, 3, 2, 1.0, 0.0, None, 4, None, 2, None, 4)
+    bad = H.GemmProblem(g["ta"].encode(), g["tb"].encode(), g["m"], g["n"], g["k"], 1.0, 0.0,
+                        None, g["lda"], None, g["ldb"], None, g["ldc"])
+    with H.ErrorCapture() as cap:
+        with pytest.raises(H.PlanError) as e:
+            H.Plan("gemm", dtype, [good, good, bad, good])
+    assert (e.value.info, e.value.index) == (info, 2)
+    assert cap.calls == [("DGEMM" if dtype == np.float64 else "SGEMM", info)]
+
+
+@pytest.mark.parametrize("change,info", TRSM_CASES)
+def test_grouped_trsm_plan_rejects_invalid_problem(change, info):
+    t = dict(side="L", uplo="L", ta="N", diag="N", m=4, n=3, lda=4, ldb=4)
+    t.update(change)
+    bad = H.TrsmProblem(t["side"].encode(), t["uplo"].encode(), t["ta"].encode(),
+                        t["diag"].encode(), t["m"], t["n"], 1.0, None, t["lda"], None, t["ldb"])
+    with H.ErrorCapture() as cap:
+        with pytest.raises(H.PlanError) as e:
+            H.Plan("trsm", np.float64, [bad])
+    assert (e.value.info, e.value.index) == (info, 0)
+    assert cap.calls == [("DTRSM", info)]
+
+
+def test_grouped_config_table():
+    cfgs = H.grouped_configs()
+    assert len(cfgs) >= 8
+    for bm, bn, uk in cfgs:
+        assert bm % 16 == 0 and bn % 16 == 0 and uk >= 1
+    assert H.lib().hwyblas_grouped_config_info(len(cfgs), None, None, None) == -1
+    for m, n, k in [(8, 8, 8), (100, 70, 300), (512, 512, 512), (1, 512, 1)]:
+        assert 0 <= H.lib().hwyblas_grouped_default_config(8, m, n, k) < len(cfgs)
