@@ -68,6 +68,10 @@ def _declare(L):
     fn("hwyblas_grouped_config_count", _i, [])
     fn("hwyblas_grouped_config_info", _i, [_i, _pi, _pi, _pi])
     fn("hwyblas_grouped_default_config", _i, [_i, _i, _i, _i])
+    fn("hwyblas_amd_capture_begin", None, [_p])
+    fn("hwyblas_amd_capture_end", _p, [_p])
+    fn("hwyblas_amd_graph_launch", None, [_p, _p])
+    fn("hwyblas_amd_graph_destroy", None, [_p])
     fn("hwyblas_gemm_config_count", _i, [])
     fn("hwyblas_gemm_config_info", _i, [_i, _pi, _pi, _pi, _pi])
     fn("hwyblas_gemm_default_config", _i, [_i, _c, _c, _i, _i, _i])

@@ -141,6 +141,9 @@ HWYBLAS_AMD_API void hwyblas_amd_memcpy(void* dst, const void* src, size_t bytes
                           : kind == 2 ? hipMemcpyDeviceToHost
                                       : hipMemcpyDeviceToDevice;
   HWY_HIP_CHECK(hipMemcpy(dst, src, bytes, k));
+  // A device-to-device hipMemcpy may return before the copy ends; this call
+  // returns only once the bytes are in place.
+  HWY_HIP_CHECK(hipStreamSynchronize(nullptr));
 }
 
 HWYBLAS_AMD_API void hwyblas_amd_memset(void* dst, int value, size_t bytes) {
@@ -208,6 +211,28 @@ HWYBLAS_AMD_API void hwyblas_amd_fill_uniform(void* x, int dtype_bytes,
                        static_cast<float*>(x), rows, cols, ld, key, lo, hi - lo);
   }
   HWY_HIP_CHECK(hipGetLastError());
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_capture_begin(void* stream) {
+  HWY_HIP_CHECK(hipStreamBeginCapture(static_cast<hipStream_t>(stream),
+                                      hipStreamCaptureModeThreadLocal));
+}
+
+HWYBLAS_AMD_API void* hwyblas_amd_capture_end(void* stream) {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HWY_HIP_CHECK(hipStreamEndCapture(static_cast<hipStream_t>(stream), &graph));
+  HWY_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  HWY_HIP_CHECK(hipGraphDestroy(graph));
+  return exec;
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_graph_launch(void* exec, void* stream) {
+  HWY_HIP_CHECK(hipGraphLaunch(static_cast<hipGraphExec_t>(exec), static_cast<hipStream_t>(stream)));
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_graph_destroy(void* exec) {
+  if (exec) HWY_HIP_CHECK(hipGraphExecDestroy(static_cast<hipGraphExec_t>(exec)));
 }
 
 HWYBLAS_AMD_API const char* hwyblas_amd_version(void) { return "hwyblas_amd 0.1.0 (gfx950)"; }

@@ -280,7 +280,8 @@ HWYBLAS_AMD_API int hwyblas_sgemm_dev_cfg(int cfg, char transa, char transb, int
 HWYBLAS_AMD_API int hwyblas_amd_device_count(void);
 HWYBLAS_AMD_API void* hwyblas_amd_malloc(size_t bytes);
 HWYBLAS_AMD_API void hwyblas_amd_free(void* p);
-/* kind: 1 host->device, 2 device->host, 3 device->device. */
+/* kind: 1 host->device, 2 device->host, 3 device->device. Returns when the
+ * copy is complete. */
 HWYBLAS_AMD_API void hwyblas_amd_memcpy(void* dst, const void* src,
                                         size_t bytes, int kind);
 HWYBLAS_AMD_API void hwyblas_amd_memset(void* dst, int value, size_t bytes);
@@ -312,6 +313,13 @@ HWYBLAS_AMD_API void hwyblas_amd_fill_uniform(void* x, int dtype_bytes,
  * -1 if nothing was recorded. */
 HWYBLAS_AMD_API void hwyblas_amd_profile_enable(int on);
 HWYBLAS_AMD_API float hwyblas_amd_profile_ms(int phase);
+/* Stream capture into a hipGraph (for loops of calls on device pointers):
+ * begin on a stream, enqueue calls on it, end to get an instantiated graph,
+ * launch it any number of times, destroy it. */
+HWYBLAS_AMD_API void hwyblas_amd_capture_begin(void* stream);
+HWYBLAS_AMD_API void* hwyblas_amd_capture_end(void* stream);
+HWYBLAS_AMD_API void hwyblas_amd_graph_launch(void* exec, void* stream);
+HWYBLAS_AMD_API void hwyblas_amd_graph_destroy(void* exec);
 /* Library version string. */
 HWYBLAS_AMD_API const char* hwyblas_amd_version(void);
 
