@@ -66,12 +66,28 @@ def _declare(L):
     fn("hwyblas_plan_problem_config", _i, [_p, _i])
     fn("hwyblas_plan_destroy", None, [_p])
     fn("hwyblas_grouped_config_count", _i, [])
-    fn("hwyblas_grouped_config_info", _i, [_i, _pi, _pi, _pi])
+    fn("hwyblas_grouped_config_info", _i, [_i, _pi, _pi, _pi, _pi])
     fn("hwyblas_grouped_default_config", _i, [_i, _i, _i, _i])
     fn("hwyblas_amd_capture_begin", None, [_p])
     fn("hwyblas_amd_capture_end", _p, [_p])
     fn("hwyblas_amd_graph_launch", None, [_p, _p])
     fn("hwyblas_amd_graph_destroy", None, [_p])
+    _pf = ctypes.POINTER(ctypes.c_float)
+    fn("hwyblas_grouped_tune", _i, [_i, _i, _p, _i, _p, _p])
+    fn("hwyblas_grouped_tune_copies", _i, [_i, _i, _i, _i])
+    fn("hwyblas_grouped_features", None, [_i, _i, _i, _i, _i, _i, _p])
+    fn("hwyblas_grouped_choose_configs", _i, [_i, _i, _p, _i, _p, _p])
+    fn("hwyblas_mlp_create", _p, [_i, ctypes.c_uint64])
+    fn("hwyblas_mlp_destroy", None, [_p])
+    fn("hwyblas_mlp_num_params", _i, [_p])
+    fn("hwyblas_mlp_outputs", _i, [_p])
+    fn("hwyblas_mlp_get_params", None, [_p, _p])
+    fn("hwyblas_mlp_set_params", None, [_p, _p])
+    fn("hwyblas_mlp_predict", _i, [_p, _i, _p, _p, _p])
+    fn("hwyblas_mlp_gradients", _i, [_p, _i, _p, _p, _p, _p, _pf, _p])
+    fn("hwyblas_mlp_train", _i, [_p, _i, _p, _p, _p, _i, _i, ctypes.c_float, ctypes.c_uint64,
+                                 _p, _p])
+    fn("hwyblas_grouped_set_model", None, [_p])
     fn("hwyblas_gemm_config_count", _i, [])
     fn("hwyblas_gemm_config_info", _i, [_i, _pi, _pi, _pi, _pi])
     fn("hwyblas_gemm_default_config", _i, [_i, _c, _c, _i, _i, _i])
@@ -363,9 +379,10 @@ class Plan:
 
 
 def grouped_configs():
+    """[(bm, bn, unroll, block)] of the grouped GEMM tile configurations."""
     out = []
     for i in range(lib().hwyblas_grouped_config_count()):
-        v = [ctypes.c_int() for _ in range(3)]
+        v = [ctypes.c_int() for _ in range(4)]
         lib().hwyblas_grouped_config_info(i, *[ctypes.byref(x) for x in v])
         out.append(tuple(x.value for x in v))
     return out

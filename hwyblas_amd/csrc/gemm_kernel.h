@@ -91,31 +91,40 @@ __device__ __forceinline__ void TileCoords(int tiles_m, int tiles_n, int* tm,
   *tn = in_group / gm;
 }
 
-// Macro-tile kernel. TA/TB: op(A)/op(B) transposed. The block is
-// WAVES_M x WAVES_N waves; each wave computes (BM / WAVES_M) x (BN / WAVES_N)
-// as 16 x 16 MFMA tiles.
-template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WAVES_M,
-          int WAVES_N, int OCC>
-__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
-    __attribute__((amdgpu_waves_per_eu(OCC))) GemmKernel(GemmParams<T> p) {
+// LDS bytes one macro-tile needs (two buffers of each operand's slice).
+// An operand whose memory is contiguous along k is kept row-major with k
+// fastest ([row][BK + 1], the odd stride spreads the MFMA reads over the
+// banks); one contiguous along rows/columns is kept k-major ([k][BM + PAD]).
+template <typename T, bool TA, bool TB, int BM, int BN, int BK>
+struct TileLds {
+  static constexpr int kPadMN = 4;
+  static constexpr int A_LD = TA ? BK + 1 : BM + kPadMN;
+  static constexpr int A_SZ = TA ? BM * A_LD : BK * A_LD;
+  static constexpr int B_LD = TB ? BN + kPadMN : BK + 1;
+  static constexpr int B_SZ = TB ? BK * B_LD : BN * B_LD;
+  static constexpr int kElems = 2 * (A_SZ + B_SZ);
+};
+
+// One workgroup (WAVES_M x WAVES_N waves) computes the BM x BN macro-tile of
+// C at (m0, n0) over the whole k range; each wave computes
+// (BM / WAVES_M) x (BN / WAVES_N) as 16 x 16 MFMA tiles. `lds` holds
+// TileLds<...>::kElems elements. Every thread of the workgroup must call it.
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WAVES_M, int WAVES_N>
+__device__ __forceinline__ void MacroTile(const GemmParams<T>& p, int64_t m0, int64_t n0,
+                                          T* lds) {
   using M = Mfma<T>;
   using Acc = typename M::Acc;
+  using Lds = TileLds<T, TA, TB, BM, BN, BK>;
   constexpr int NT = 64 * WAVES_M * WAVES_N;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   constexpr int TM = WM / 16, TN = WN / 16;
   static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "tile shape");
-
-  // LDS layouts. An operand whose memory is contiguous along k is kept
-  // row-major with k fastest ([row][BK + 1], the odd stride spreads the MFMA
-  // reads over the banks); one contiguous along rows/columns is kept k-major
-  // ([k][BM + PAD]).
-  constexpr int kPadMN = 4;
-  constexpr int A_LD = TA ? BK + 1 : BM + kPadMN;
-  constexpr int A_SZ = TA ? BM * A_LD : BK * A_LD;
-  constexpr int B_LD = TB ? BN + kPadMN : BK + 1;
-  constexpr int B_SZ = TB ? BK * B_LD : BN * B_LD;
-  __shared__ T As[2][A_SZ];
-  __shared__ T Bs[2][B_SZ];
+  constexpr int A_LD = Lds::A_LD, A_SZ = Lds::A_SZ;
+  constexpr int B_LD = Lds::B_LD, B_SZ = Lds::B_SZ;
+  // Buffer b of op(A)'s slice starts at lds + b * A_SZ, of op(B)'s at
+  // lds + 2 * A_SZ + b * B_SZ.
+  T* const as_base = lds;
+  T* const bs_base = lds + 2 * A_SZ;
 
   // Global-load assignment: consecutive threads read consecutive addresses.
   constexpr int A_FAST = TA ? BK : BM;      // extent of the contiguous dim
@@ -130,10 +139,6 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
-
-  int tm, tn;
-  TileCoords(p.tiles_m, p.tiles_n, &tm, &tn);
-  const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
 
   // Per-thread coordinates of the elements it loads: (fast, slow + step r).
   const int a_fast = tid % A_FAST, a_slow = tid / A_FAST;
@@ -192,13 +197,13 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
     for (int r = 0; r < LA; ++r) {
       const int i = TA ? a_slow + A_SLOW_STEP * r : a_fast;
       const int pp = TA ? a_fast : a_slow + A_SLOW_STEP * r;
-      As[buf][TA ? i * A_LD + pp : pp * A_LD + i] = ra[r];
+      as_base[buf * A_SZ + (TA ? i * A_LD + pp : pp * A_LD + i)] = ra[r];
     }
 #pragma unroll
     for (int r = 0; r < LB; ++r) {
       const int j = TB ? b_fast : b_slow + B_SLOW_STEP * r;
       const int pp = TB ? b_slow + B_SLOW_STEP * r : b_fast;
-      Bs[buf][TB ? pp * B_LD + j : j * B_LD + pp] = rb[r];
+      bs_base[buf * B_SZ + (TB ? pp * B_LD + j : j * B_LD + pp)] = rb[r];
     }
   };
 
@@ -218,8 +223,8 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
     const int cur = kt & 1;
     const bool more = kt + 1 < ktiles;
     if (more) load_tile(int64_t(kt + 1) * BK);
-    const T* as = As[cur];
-    const T* bs = Bs[cur];
+    const T* as = as_base + cur * A_SZ;
+    const T* bs = bs_base + cur * B_SZ;
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T af[TM], bf[TN];
@@ -262,6 +267,18 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
       }
     }
   }
+}
+
+
+// Macro-tile kernel: one workgroup per tile, tiles mapped XCD-aware.
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WAVES_M,
+          int WAVES_N, int OCC>
+__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
+    __attribute__((amdgpu_waves_per_eu(OCC))) GemmKernel(GemmParams<T> p) {
+  __shared__ T lds[TileLds<T, TA, TB, BM, BN, BK>::kElems];
+  int tm, tn;
+  TileCoords(p.tiles_m, p.tiles_n, &tm, &tn);
+  MacroTile<T, TA, TB, BM, BN, BK, WAVES_M, WAVES_N>(p, int64_t(tm) * BM, int64_t(tn) * BN, lds);
 }
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WGM, int WGN, int OCC>

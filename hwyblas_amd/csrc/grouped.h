@@ -5,9 +5,9 @@
 
 #include "common.h"
 
-// X(id, BM, BN, UK): one wave computes a BM x BN tile, k advancing 4 * UK
-// per step.
-#define HWY_GROUPED_GEMM_CONFIGS(X) \
+// Wave tiles, X(id, BM, BN, UK): one wave computes a BM x BN tile, loading
+// its MFMA operands straight from memory, k advancing 4 * UK per step.
+#define HWY_GROUPED_WAVE_CONFIGS(X) \
   X(0, 16, 16, 1)                   \
   X(1, 16, 16, 4)                   \
   X(2, 32, 32, 1)                   \
@@ -19,17 +19,31 @@
   X(8, 64, 64, 1)                   \
   X(9, 64, 64, 2)
 
+// Workgroup tiles, X(id, BM, BN, BK): the four waves of a workgroup compute
+// a BM x BN tile (2 x 2 waves) with the plain kernel's LDS-staged macro-tile,
+// BK of k per stage.
+#define HWY_GROUPED_BLOCK_CONFIGS(X) \
+  X(10, 64, 64, 16)                  \
+  X(11, 128, 64, 16)                 \
+  X(12, 64, 128, 16)                 \
+  X(13, 128, 128, 16)
+
 namespace hwy_amd {
 namespace grouped {
 
 struct GemmConfig {
-  int bm, bn, uk;
+  int bm, bn;
+  int uk;     // wave tiles: k steps of 4 per load batch; workgroup tiles: BK
+  int block;  // 1: a workgroup tile
 };
 
 constexpr GemmConfig kGroupedGemmConfigs[] = {
-#define HWY_GROUPED_INFO(ID, BM, BN, UK) {BM, BN, UK},
-    HWY_GROUPED_GEMM_CONFIGS(HWY_GROUPED_INFO)
-#undef HWY_GROUPED_INFO
+#define HWY_GROUPED_WAVE_INFO(ID, BM, BN, UK) {BM, BN, UK, 0},
+    HWY_GROUPED_WAVE_CONFIGS(HWY_GROUPED_WAVE_INFO)
+#undef HWY_GROUPED_WAVE_INFO
+#define HWY_GROUPED_BLOCK_INFO(ID, BM, BN, BK) {BM, BN, BK, 1},
+    HWY_GROUPED_BLOCK_CONFIGS(HWY_GROUPED_BLOCK_INFO)
+#undef HWY_GROUPED_BLOCK_INFO
 };
 constexpr int kNumGroupedGemmConfigs = sizeof(kGroupedGemmConfigs) / sizeof(kGroupedGemmConfigs[0]);
 

@@ -1,23 +1,26 @@
 // Batched ("grouped") GEMM and TRSM over many independent, uneven problems
 // in one persistent launch.
 //
-// A plan uploads the problem descriptors and a work list once. Work items
-// are GEMM tiles (problem, tile row, tile column, tile configuration) or
-// TRSM chunks (problem, block of 64 right-hand sides). The launch is a tiny
-// kernel that resets the work counter followed by the persistent kernel. Its
-// grid fills the GPU and every wave is an independent worker: a wave claims
-// the next item from the counter (one atomic per item, by lane 0), runs it
-// and claims again until the list is exhausted, so no wave waits on another
-// and the load balances itself over uneven sizes. Items are listed largest
-// first. The two launches touch no host memory and use no memset, so a plan
-// can be captured in a hipGraph and replayed.
+// A plan uploads the problem descriptors and the work lists once. GEMM work
+// comes in two kinds: workgroup tiles (the four waves of a workgroup compute
+// one large tile with the plain kernel's LDS-staged macro-tile) and wave
+// tiles (one wave computes one small tile straight from memory). TRSM work
+// items are blocks of 64 right-hand sides of one problem. The launch is a
+// tiny kernel that resets the work counters followed by the persistent
+// kernel, whose grid fills the GPU. Workgroups first claim workgroup tiles
+// from one atomic counter (largest first) until none are left; then every
+// wave claims units of wave tiles from a second counter. A unit is a run of
+// consecutive small tiles of similar total cost, so one atomic serves many
+// tiny tiles, while the largest tiles stay units of their own and keep the
+// load balanced. No wave waits on another wave's work. The two launches
+// touch no host memory and use no memset, so a plan can be captured in a
+// hipGraph and replayed.
 //
-// GEMM tiles run on the MFMA units without LDS: each lane loads its MFMA
-// operand elements directly (for op(X) = X the 16 lanes of a row group read
-// 16 consecutive elements), k advances in steps of 4 x UK with the loads of
-// a step issued together. The epilogue applies hwyBLAS's Store rules
-// (src/gemm_kernel-inl.h:181-224). Per problem, (k == 0 or alpha == 0) with
-// beta == 1 adds no items (C untouched); alpha == 0 never reads A or B.
+// Both tile kinds sum k in the order of the single-call kernel (MFMA steps
+// of 4, ascending k), and the epilogue applies hwyBLAS's Store rules
+// (src/gemm_kernel-inl.h:181-224), so every tile configuration gives the
+// bits hwyblas_?gemm_dev gives. Per problem, (k == 0 or alpha == 0) with
+// beta == 1 adds no work (C untouched); alpha == 0 never reads A or B.
 //
 // TRSM chunks solve op(A) X = alpha B (left) or X op(A) = alpha B (right)
 // for up to 64 right-hand sides, one per lane: B := alpha B, then rows are
@@ -29,33 +32,17 @@
 #include <vector>
 
 #include "common.h"
+#include "gemm_kernel.h"
 #include "grouped.h"
 
 namespace hwy_amd {
 namespace grouped {
 namespace {
 
-typedef double dx4 __attribute__((ext_vector_type(4)));
-typedef float fx4 __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-  using Acc = dx4;
-  static __device__ __forceinline__ Acc Run(double a, double b, Acc c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int Row(int lk, int r) { return lk + 4 * r; }
-};
-template <>
-struct Mfma<float> {
-  using Acc = fx4;
-  static __device__ __forceinline__ Acc Run(float a, float b, Acc c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int Row(int lk, int r) { return 4 * lk + r; }
-};
+using gemm_detail::GemmParams;
+using gemm_detail::MacroTile;
+using gemm_detail::Mfma;
+using gemm_detail::TileLds;
 
 template <typename T>
 struct GemmDesc {
@@ -150,32 +137,108 @@ __device__ __forceinline__ void RunGemmItem(const GemmDesc<T>& d, const WorkItem
 #define HWY_GROUPED_CASE(ID, BM, BN, UK) \
   case ID:                               \
     GemmTile<T, BM, BN, UK>(d, w.i0, w.j0); break;
-    HWY_GROUPED_GEMM_CONFIGS(HWY_GROUPED_CASE)
+    HWY_GROUPED_WAVE_CONFIGS(HWY_GROUPED_CASE)
 #undef HWY_GROUPED_CASE
     default: break;
   }
 }
 
-__global__ void ResetCounter(int* counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+// Workgroup tile `w` of problem `d` (all threads of the workgroup call it).
+template <typename T, bool TA, bool TB>
+__device__ void BlockTile(const GemmParams<T>& p, const WorkItem& w, T* lds) {
+  switch (w.cfg) {
+#define HWY_GROUPED_BLOCK_CASE(ID, BM, BN, BK) \
+  case ID:                                     \
+    MacroTile<T, TA, TB, BM, BN, BK, 2, 2>(p, w.i0, w.j0, lds); break;
+    HWY_GROUPED_BLOCK_CONFIGS(HWY_GROUPED_BLOCK_CASE)
+#undef HWY_GROUPED_BLOCK_CASE
+    default: break;
+  }
 }
 
-// Claims the next work item for this wave (the same index in every lane).
+template <typename T>
+__device__ __forceinline__ void RunBlockItem(const GemmDesc<T>& d, const WorkItem& w, T* lds) {
+  GemmParams<T> p;
+  p.a = d.a;
+  p.b = d.b;
+  p.c = d.c;
+  p.m = d.m;
+  p.n = d.n;
+  p.k = d.k;
+  p.lda = d.lda;
+  p.ldb = d.ldb;
+  p.ldc = d.ldc;
+  p.alpha = d.alpha;
+  p.beta = d.beta;
+  p.tiles_m = p.tiles_n = 0;
+  if (!d.ta && !d.tb) BlockTile<T, false, false>(p, w, lds);
+  else if (!d.ta && d.tb) BlockTile<T, false, true>(p, w, lds);
+  else if (d.ta && !d.tb) BlockTile<T, true, false>(p, w, lds);
+  else BlockTile<T, true, true>(p, w, lds);
+}
+
+constexpr int CMax(int a, int b) { return a > b ? a : b; }
+
+// LDS elements of the largest workgroup tile over all transpositions.
+template <typename T>
+constexpr int BlockLdsElems() {
+  int mx = 1;
+#define HWY_GROUPED_LDS(ID, BM, BN, BK)                                \
+  mx = CMax(mx, TileLds<T, false, false, BM, BN, BK>::kElems);         \
+  mx = CMax(mx, TileLds<T, false, true, BM, BN, BK>::kElems);          \
+  mx = CMax(mx, TileLds<T, true, false, BM, BN, BK>::kElems);          \
+  mx = CMax(mx, TileLds<T, true, true, BM, BN, BK>::kElems);
+  HWY_GROUPED_BLOCK_CONFIGS(HWY_GROUPED_LDS)
+#undef HWY_GROUPED_LDS
+  return mx;
+}
+
+// Counters live 128 bytes apart so their atomics do not share a line.
+constexpr int kCounterStride = 32;
+constexpr int kNumCounters = 2;
+
+__global__ void ResetCounters(int* counters) {
+  if (threadIdx.x < kNumCounters && blockIdx.x == 0) counters[threadIdx.x * kCounterStride] = 0;
+}
+
+// Claims the next work index for this wave (the same index in every lane).
 __device__ __forceinline__ int ClaimItem(int* counter) {
   int t = 0;
   if ((threadIdx.x & 63) == 0) t = atomicAdd(counter, 1);
-  return __shfl(t, 0, 64);
+  return __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-    GroupedGemmKernel(const GemmDesc<T>* descs, const WorkItem* items, int nitems, int* counter) {
+    GroupedGemmKernel(const GemmDesc<T>* __restrict__ descs,
+                      const WorkItem* __restrict__ block_items, int n_block,
+                      const WorkItem* __restrict__ wave_items,
+                      const int* __restrict__ unit_start, int n_units, int* counters) {
+  __shared__ T lds[BlockLdsElems<T>()];
+  __shared__ int claimed;
+  // Phase 1: workgroup tiles, claimed by the workgroup as a whole.
+  if (n_block > 0) {
+    for (;;) {
+      if (threadIdx.x == 0) claimed = atomicAdd(&counters[0], 1);
+      __syncthreads();
+      const int t = __builtin_amdgcn_readfirstlane(claimed);
+      __syncthreads();
+      if (t >= n_block) break;
+      const WorkItem w = block_items[t];
+      const GemmDesc<T> d = descs[w.problem];
+      RunBlockItem<T>(d, w, lds);
+    }
+  }
+  // Phase 2: units of wave tiles, claimed by each wave.
   for (;;) {
-    const int t = ClaimItem(counter);
-    if (t >= nitems) break;
-    const WorkItem w = items[t];
-    const GemmDesc<T> d = descs[w.problem];
-    RunGemmItem<T>(d, w);
+    const int u = ClaimItem(&counters[kCounterStride]);
+    if (u >= n_units) break;
+    const int t1 = unit_start[u + 1];
+    for (int t = unit_start[u]; t < t1; ++t) {
+      const WorkItem w = wave_items[t];
+      const GemmDesc<T> d = descs[w.problem];
+      RunGemmItem<T>(d, w);
+    }
   }
 }
 
@@ -272,13 +335,17 @@ __device__ void TrsmChunk(const TrsmDesc<T>& d, int v0) {
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-    GroupedTrsmKernel(const TrsmDesc<T>* descs, const WorkItem* items, int nitems, int* counter) {
+    GroupedTrsmKernel(const TrsmDesc<T>* __restrict__ descs, const WorkItem* __restrict__ items,
+                      const int* __restrict__ unit_start, int n_units, int* counters) {
   for (;;) {
-    const int t = ClaimItem(counter);
-    if (t >= nitems) break;
-    const WorkItem w = items[t];
-    const TrsmDesc<T> d = descs[w.problem];
-    TrsmChunk<T>(d, w.i0);
+    const int u = ClaimItem(&counters[kCounterStride]);
+    if (u >= n_units) break;
+    const int t1 = unit_start[u + 1];
+    for (int t = unit_start[u]; t < t1; ++t) {
+      const WorkItem w = items[t];
+      const TrsmDesc<T> d = descs[w.problem];
+      TrsmChunk<T>(d, w.i0);
+    }
   }
 }
 
@@ -298,9 +365,10 @@ struct Plan {
   int kind;  // 0 GEMM, 1 TRSM
   int dtype_bytes;
   void* descs = nullptr;
-  WorkItem* items = nullptr;
-  int* counter = nullptr;
-  int nitems = 0;
+  WorkItem* items = nullptr;  // workgroup items, then wave items
+  int* units = nullptr;       // n_units + 1 starts into the wave items
+  int* counters = nullptr;
+  int n_block = 0, n_wave = 0, n_units = 0;
   int blocks = 0;
   int device = -1;
   hipGraph_t graph = nullptr;
@@ -310,8 +378,53 @@ struct Plan {
 
 namespace {
 
-double GemmCost(const GemmConfig& c, int m, int n, int k) {
-  return double((m + c.bm - 1) / c.bm) * ((n + c.bn - 1) / c.bn) * c.bm * c.bn * double(k + 1);
+// Estimated cost of a tile: its area times k plus a fixed cost per tile
+// (claim, descriptor load, epilogue) worth about 16 steps of k.
+double TileCost(const GemmConfig& c, int k) { return double(c.bm) * c.bn * double(k + 16); }
+
+// Workgroups of the persistent grid: two 256-thread workgroups per CU
+// (the kernels run at two waves per SIMD).
+constexpr int kBlocksPerCu = 2;
+
+// Largest first (longest-processing-time order).
+std::vector<int> ByCostDescending(const std::vector<double>& cost) {
+  std::vector<int> order(cost.size());
+  for (size_t t = 0; t < order.size(); ++t) order[t] = int(t);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  return order;
+}
+
+// Splits items (already in descending cost) into units of consecutive items:
+// a unit closes once it holds at least `target` cost, so large items stay
+// alone and runs of small ones share one claim.
+std::vector<int> BuildUnits(const std::vector<double>& sorted_cost, double target) {
+  std::vector<int> starts;
+  double acc = 0;
+  for (size_t t = 0; t < sorted_cost.size(); ++t) {
+    if (acc == 0) starts.push_back(int(t));
+    acc += sorted_cost[t];
+    if (acc >= target) acc = 0;
+  }
+  starts.push_back(int(sorted_cost.size()));
+  return starts;
+}
+
+// Uploads the work lists and sizes the grid.
+void UploadWork(Plan* plan, const void* descs, size_t desc_bytes, const std::vector<WorkItem>& items,
+                const std::vector<int>& units) {
+  HWY_HIP_CHECK(hipGetDevice(&plan->device));
+  HWY_HIP_CHECK(hipMalloc(&plan->descs, std::max<size_t>(1, desc_bytes)));
+  HWY_HIP_CHECK(hipMalloc(&plan->items, std::max<size_t>(1, items.size()) * sizeof(WorkItem)));
+  HWY_HIP_CHECK(hipMalloc(&plan->units, units.size() * sizeof(int)));
+  HWY_HIP_CHECK(hipMalloc(&plan->counters, kNumCounters * kCounterStride * sizeof(int)));
+  if (desc_bytes) HWY_HIP_CHECK(hipMemcpy(plan->descs, descs, desc_bytes, hipMemcpyHostToDevice));
+  if (!items.empty())
+    HWY_HIP_CHECK(hipMemcpy(plan->items, items.data(), items.size() * sizeof(WorkItem),
+                            hipMemcpyHostToDevice));
+  HWY_HIP_CHECK(hipMemcpy(plan->units, units.data(), units.size() * sizeof(int),
+                          hipMemcpyHostToDevice));
+  const int want = plan->n_block + (plan->n_units + 3) / 4;
+  plan->blocks = std::max(1, std::min(DeviceCUs() * kBlocksPerCu, want));
 }
 
 template <typename T>
@@ -320,8 +433,8 @@ Plan* MakeGemmPlan(int count, const hwyblas_gemm_problem* probs, const int* cfgs
   *info = 0;
   *bad = -1;
   std::vector<GemmDesc<T>> descs(count);
-  std::vector<WorkItem> items;
-  std::vector<double> cost;
+  std::vector<WorkItem> block_items, wave_items;
+  std::vector<double> block_cost, wave_cost;
   std::vector<int> chosen(count, -1);
   for (int i = 0; i < count; ++i) {
     const hwyblas_gemm_problem& p = probs[i];
@@ -362,38 +475,41 @@ Plan* MakeGemmPlan(int count, const hwyblas_gemm_problem* probs, const int* cfgs
     if (no_product && d.beta == T(1)) continue;
     int cfg = cfgs ? cfgs[i] : -1;
     if (cfg < 0 || cfg >= kNumGroupedGemmConfigs)
-      cfg = DefaultGroupedConfig(int(sizeof(T)), p.m, p.n, p.k);
+      cfg = DefaultGroupedConfig(int(sizeof(T)), p.m, p.n, d.k);
+    // Without a product only the epilogue runs (C = beta C): a wave tile.
+    if (no_product && kGroupedGemmConfigs[cfg].block) cfg = 9;
     chosen[i] = cfg;
     const GemmConfig& gc = kGroupedGemmConfigs[cfg];
+    auto& list = gc.block ? block_items : wave_items;
+    auto& cost = gc.block ? block_cost : wave_cost;
     for (int j0 = 0; j0 < p.n; j0 += gc.bn)
       for (int i0 = 0; i0 < p.m; i0 += gc.bm) {
-        items.push_back({i, i0, j0, cfg});
-        cost.push_back(GemmCost(gc, std::min(gc.bm, p.m - i0), std::min(gc.bn, p.n - j0), d.k));
+        list.push_back({i, i0, j0, cfg});
+        cost.push_back(TileCost(gc, d.k));
       }
   }
-  // Largest items first (longest-processing-time order).
-  std::vector<int> order(items.size());
-  for (size_t t = 0; t < order.size(); ++t) order[t] = int(t);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-  std::vector<WorkItem> sorted(items.size());
-  for (size_t t = 0; t < order.size(); ++t) sorted[t] = items[order[t]];
+  std::vector<WorkItem> items;
+  items.reserve(block_items.size() + wave_items.size());
+  for (int t : ByCostDescending(block_cost)) items.push_back(block_items[t]);
+  std::vector<double> sorted_wave_cost;
+  sorted_wave_cost.reserve(wave_items.size());
+  double total = 0;
+  for (int t : ByCostDescending(wave_cost)) {
+    items.push_back(wave_items[t]);
+    sorted_wave_cost.push_back(wave_cost[t]);
+    total += wave_cost[t];
+  }
+  const int resident_waves = DeviceCUs() * kBlocksPerCu * 4;
+  const std::vector<int> units = BuildUnits(sorted_wave_cost, total / (8.0 * resident_waves));
 
   Plan* plan = new Plan();
   plan->kind = 0;
   plan->dtype_bytes = int(sizeof(T));
-  plan->nitems = int(sorted.size());
+  plan->n_block = int(block_items.size());
+  plan->n_wave = int(wave_items.size());
+  plan->n_units = int(units.size()) - 1;
   plan->cfg_of_problem = chosen;
-  HWY_HIP_CHECK(hipGetDevice(&plan->device));
-  HWY_HIP_CHECK(hipMalloc(&plan->descs, std::max<size_t>(1, descs.size()) * sizeof(GemmDesc<T>)));
-  HWY_HIP_CHECK(hipMalloc(&plan->items, std::max<size_t>(1, sorted.size()) * sizeof(WorkItem)));
-  HWY_HIP_CHECK(hipMalloc(&plan->counter, sizeof(int)));
-  if (!descs.empty())
-    HWY_HIP_CHECK(hipMemcpy(plan->descs, descs.data(), descs.size() * sizeof(GemmDesc<T>),
-                            hipMemcpyHostToDevice));
-  if (!sorted.empty())
-    HWY_HIP_CHECK(hipMemcpy(plan->items, sorted.data(), sorted.size() * sizeof(WorkItem),
-                            hipMemcpyHostToDevice));
-  plan->blocks = std::max(1, std::min(DeviceCUs() * 4, (plan->nitems + 3) / 4));
+  UploadWork(plan, descs.data(), descs.size() * sizeof(GemmDesc<T>), items, units);
   return plan;
 }
 
@@ -448,59 +564,54 @@ Plan* MakeTrsmPlan(int count, const hwyblas_trsm_problem* probs, int* info, int*
       cost.push_back(double(len) * len);
     }
   }
-  std::vector<int> order(items.size());
-  for (size_t t = 0; t < order.size(); ++t) order[t] = int(t);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-  std::vector<WorkItem> sorted(items.size());
-  for (size_t t = 0; t < order.size(); ++t) sorted[t] = items[order[t]];
+  std::vector<WorkItem> sorted;
+  std::vector<double> sorted_cost;
+  for (int t : ByCostDescending(cost)) {
+    sorted.push_back(items[t]);
+    sorted_cost.push_back(cost[t]);
+  }
+  // Every chunk is a unit of its own (a chunk is already a long solve).
+  const std::vector<int> units = BuildUnits(sorted_cost, 0.0);
   Plan* plan = new Plan();
   plan->kind = 1;
   plan->dtype_bytes = int(sizeof(T));
-  plan->nitems = int(sorted.size());
-  HWY_HIP_CHECK(hipGetDevice(&plan->device));
-  HWY_HIP_CHECK(hipMalloc(&plan->descs, std::max<size_t>(1, descs.size()) * sizeof(TrsmDesc<T>)));
-  HWY_HIP_CHECK(hipMalloc(&plan->items, std::max<size_t>(1, sorted.size()) * sizeof(WorkItem)));
-  HWY_HIP_CHECK(hipMalloc(&plan->counter, sizeof(int)));
-  if (!descs.empty())
-    HWY_HIP_CHECK(hipMemcpy(plan->descs, descs.data(), descs.size() * sizeof(TrsmDesc<T>),
-                            hipMemcpyHostToDevice));
-  if (!sorted.empty())
-    HWY_HIP_CHECK(hipMemcpy(plan->items, sorted.data(), sorted.size() * sizeof(WorkItem),
-                            hipMemcpyHostToDevice));
-  plan->blocks = std::max(1, std::min(DeviceCUs() * 4, (plan->nitems + 3) / 4));
+  plan->n_wave = int(sorted.size());
+  plan->n_units = int(units.size()) - 1;
+  UploadWork(plan, descs.data(), descs.size() * sizeof(TrsmDesc<T>), sorted, units);
   return plan;
 }
 
 }  // namespace
 
 void Enqueue(const Plan* plan, hipStream_t stream) {
-  if (plan->nitems == 0) return;
-  hipLaunchKernelGGL(ResetCounter, dim3(1), dim3(64), 0, stream, plan->counter);
+  if (plan->n_block + plan->n_wave == 0) return;
+  hipLaunchKernelGGL(ResetCounters, dim3(1), dim3(64), 0, stream, plan->counters);
   const dim3 grid(unsigned(plan->blocks)), block(256);
+  const WorkItem* wave_items = plan->items + plan->n_block;
   if (plan->kind == 0) {
     if (plan->dtype_bytes == 8)
       hipLaunchKernelGGL(GroupedGemmKernel<double>, grid, block, 0, stream,
                          static_cast<const GemmDesc<double>*>(plan->descs), plan->items,
-                         plan->nitems, plan->counter);
+                         plan->n_block, wave_items, plan->units, plan->n_units, plan->counters);
     else
       hipLaunchKernelGGL(GroupedGemmKernel<float>, grid, block, 0, stream,
                          static_cast<const GemmDesc<float>*>(plan->descs), plan->items,
-                         plan->nitems, plan->counter);
+                         plan->n_block, wave_items, plan->units, plan->n_units, plan->counters);
   } else {
     if (plan->dtype_bytes == 8)
       hipLaunchKernelGGL(GroupedTrsmKernel<double>, grid, block, 0, stream,
-                         static_cast<const TrsmDesc<double>*>(plan->descs), plan->items,
-                         plan->nitems, plan->counter);
+                         static_cast<const TrsmDesc<double>*>(plan->descs), wave_items,
+                         plan->units, plan->n_units, plan->counters);
     else
       hipLaunchKernelGGL(GroupedTrsmKernel<float>, grid, block, 0, stream,
-                         static_cast<const TrsmDesc<float>*>(plan->descs), plan->items,
-                         plan->nitems, plan->counter);
+                         static_cast<const TrsmDesc<float>*>(plan->descs), wave_items,
+                         plan->units, plan->n_units, plan->counters);
   }
   HWY_HIP_CHECK(hipGetLastError());
 }
 
 void LaunchGraph(Plan* plan, hipStream_t stream) {
-  if (plan->nitems == 0) return;
+  if (plan->n_block + plan->n_wave == 0) return;
   if (!plan->exec) {
     // Capture on a private stream (the caller's may already be capturing).
     hipStream_t cap;
@@ -520,18 +631,18 @@ void Destroy(Plan* plan) {
   if (plan->graph) (void)hipGraphDestroy(plan->graph);
   (void)hipFree(plan->descs);
   (void)hipFree(plan->items);
-  (void)hipFree(plan->counter);
+  (void)hipFree(plan->units);
+  (void)hipFree(plan->counters);
   delete plan;
 }
 
-// Without tuning data: the largest tile that gives the problem at least a
-// few tiles, k unrolled when k is long.
+// Without tuning data: a 128 x 128 workgroup tile for large problems, else
+// the 64 x 64 wave tile (tiles of fewer rows cost more claims and loads than
+// they save in padding on the mixed batches measured).
 int DefaultGroupedConfig(int dtype_bytes, int m, int n, int k) {
   (void)dtype_bytes;
-  const int uk = k >= 64 ? 1 : 0;
-  if (m >= 128 && n >= 128) return 8 + uk;  // 64 x 64
-  if (m >= 64 && n >= 64) return 2 + uk;    // 32 x 32
-  return 0 + uk;                            // 16 x 16
+  if (m >= 96 && n >= 96 && k >= 32) return 13;
+  return 9;
 }
 
 }  // namespace grouped
@@ -574,7 +685,9 @@ HWYBLAS_AMD_API void hwyblas_plan_launch_graph(void* plan, void* stream) {
 }
 
 HWYBLAS_AMD_API int hwyblas_plan_work_items(void* plan) {
-  return plan ? static_cast<grouped::Plan*>(plan)->nitems : 0;
+  if (!plan) return 0;
+  const auto* p = static_cast<grouped::Plan*>(plan);
+  return p->n_block + p->n_wave;
 }
 
 HWYBLAS_AMD_API int hwyblas_plan_problem_config(void* plan, int problem) {
@@ -589,11 +702,13 @@ HWYBLAS_AMD_API void hwyblas_plan_destroy(void* plan) {
 
 HWYBLAS_AMD_API int hwyblas_grouped_config_count(void) { return grouped::kNumGroupedGemmConfigs; }
 
-HWYBLAS_AMD_API int hwyblas_grouped_config_info(int cfg, int* bm, int* bn, int* unroll) {
+HWYBLAS_AMD_API int hwyblas_grouped_config_info(int cfg, int* bm, int* bn, int* unroll,
+                                                int* block) {
   if (cfg < 0 || cfg >= grouped::kNumGroupedGemmConfigs) return -1;
   *bm = grouped::kGroupedGemmConfigs[cfg].bm;
   *bn = grouped::kGroupedGemmConfigs[cfg].bn;
   *unroll = grouped::kGroupedGemmConfigs[cfg].uk;
+  if (block) *block = grouped::kGroupedGemmConfigs[cfg].block;
   return 0;
 }
 

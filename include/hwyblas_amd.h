@@ -243,8 +243,62 @@ HWYBLAS_AMD_API int hwyblas_plan_work_items(void* plan);
 HWYBLAS_AMD_API int hwyblas_plan_problem_config(void* plan, int problem);
 HWYBLAS_AMD_API void hwyblas_plan_destroy(void* plan);
 HWYBLAS_AMD_API int hwyblas_grouped_config_count(void);
-HWYBLAS_AMD_API int hwyblas_grouped_config_info(int cfg, int* bm, int* bn, int* unroll);
+/* Tile configuration cfg: a bm x bn tile; block = 0: one wave computes it
+ * (unroll = k steps of 4 per load batch); block = 1: a workgroup of four
+ * waves computes it through LDS (unroll = k per LDS stage). */
+HWYBLAS_AMD_API int hwyblas_grouped_config_info(int cfg, int* bm, int* bn, int* unroll,
+                                                int* block);
 HWYBLAS_AMD_API int hwyblas_grouped_default_config(int dtype_bytes, int m, int n, int k);
+
+/* ------------------------------------------------------------------------ */
+/* Autotuner and cost model for the grouped tile configurations (no hwyBLAS
+ * counterpart: hwyBLAS fixes its blocking per SIMD target at compile time,
+ * src/gemm_kernel-inl.h:60-75). */
+
+/* Times every grouped configuration on each shape (5 ints per shape:
+ * transa != 0, transb != 0, m, n, k; m, n, k >= 1): a plan of
+ * hwyblas_grouped_tune_copies() problems of that shape with distinct device
+ * operands, launched `reps` times after a warm-up. Writes microseconds per
+ * problem to us[shape * hwyblas_grouped_config_count() + cfg]. Returns 0, or
+ * -1 on invalid arguments. */
+HWYBLAS_AMD_API int hwyblas_grouped_tune(int dtype_bytes, int nshapes, const int* shapes,
+                                         int reps, float* us, void* stream);
+HWYBLAS_AMD_API int hwyblas_grouped_tune_copies(int dtype_bytes, int m, int n, int k);
+/* The cost model's 6 input features of a shape. */
+HWYBLAS_AMD_API void hwyblas_grouped_features(int dtype_bytes, int transa, int transb, int m,
+                                              int n, int k, float* features);
+/* Fills configs[i] for each problem (host array, as for plan creation):
+ * policy 0 the built-in rule; 1 the installed cost model (falls back to 2
+ * when none is installed); 2 exhaustive tuning of each distinct shape, kept
+ * for the process. Returns 0, or -1 on invalid arguments. */
+HWYBLAS_AMD_API int hwyblas_grouped_choose_configs(int dtype_bytes, int count,
+                                                   const hwyblas_gemm_problem* problems,
+                                                   int policy, int* configs, void* stream);
+
+/* The cost model: a 4-layer MLP, width 64, FP32 (6 inputs -> 64 -> 64 -> 64
+ * -> `outputs`, ReLU), predicting log2(microseconds per problem) of each
+ * configuration. Parameters row-major W[out][in], in the order W1 b1 W2 b2
+ * W3 b3 W4 b4. Training, gradients and inference run on the GPU. */
+HWYBLAS_AMD_API void* hwyblas_mlp_create(int outputs, uint64_t seed);
+HWYBLAS_AMD_API void hwyblas_mlp_destroy(void* model);
+HWYBLAS_AMD_API int hwyblas_mlp_num_params(void* model);
+HWYBLAS_AMD_API int hwyblas_mlp_outputs(void* model);
+HWYBLAS_AMD_API void hwyblas_mlp_get_params(void* model, float* params);
+/* Replaces the parameters and resets the optimizer state. */
+HWYBLAS_AMD_API void hwyblas_mlp_set_params(void* model, const float* params);
+/* out[i * outputs + c]: prediction for features[i * 6 ...]. */
+HWYBLAS_AMD_API int hwyblas_mlp_predict(void* model, int n, const float* features, float* out,
+                                        void* stream);
+/* Loss 0.5 mean_i (out[i][cfg[i]] - target[i])^2 and its gradient (no update). */
+HWYBLAS_AMD_API int hwyblas_mlp_gradients(void* model, int n, const float* features,
+                                          const int* cfg, const float* target, float* grads,
+                                          float* loss, void* stream);
+/* Adam (beta 0.9, 0.999) over shuffled mini-batches of that loss. */
+HWYBLAS_AMD_API int hwyblas_mlp_train(void* model, int n, const float* features, const int* cfg,
+                                      const float* target, int epochs, int batch, float lr,
+                                      uint64_t seed, float* epoch_loss, void* stream);
+/* Installs the model policy 1 uses (NULL uninstalls); the caller keeps it alive. */
+HWYBLAS_AMD_API void hwyblas_grouped_set_model(void* model);
 
 /* ------------------------------------------------------------------------ */
 /* Tile configurations of the plain GEMM kernel (no hwyBLAS counterpart: its

@@ -202,8 +202,9 @@ def test_grouped_trsm_plan_rejects_invalid_problem(change, info):
 def test_grouped_config_table():
     cfgs = H.grouped_configs()
     assert len(cfgs) >= 8
-    for bm, bn, uk in cfgs:
-        assert bm % 16 == 0 and bn % 16 == 0 and uk >= 1
-    assert H.lib().hwyblas_grouped_config_info(len(cfgs), None, None, None) == -1
+    for bm, bn, uk, block in cfgs:
+        assert bm % 16 == 0 and bn % 16 == 0 and uk >= 1 and block in (0, 1)
+    assert any(c[3] for c in cfgs) and not all(c[3] for c in cfgs)
+    assert H.lib().hwyblas_grouped_config_info(len(cfgs), None, None, None, None) == -1
     for m, n, k in [(8, 8, 8), (100, 70, 300), (512, 512, 512), (1, 512, 1)]:
         assert 0 <= H.lib().hwyblas_grouped_default_config(8, m, n, k) < len(cfgs)

@@ -122,8 +122,10 @@ def test_grouped_gemm_uneven_batch(H, dtype):
         if p["m"] == 0 or p["n"] == 0 or ((p["k"] == 0 or p["alpha"] == 0) and p["beta"] == 1):
             assert cfg == -1
         else:
+            # alpha == 0 is planned as k == 0 (A and B are not read).
+            k_eff = 0 if p["alpha"] == 0.0 else p["k"]
             assert cfg == H.lib().hwyblas_grouped_default_config(np.dtype(dtype).itemsize, p["m"],
-                                                                 p["n"], p["k"])
+                                                                 p["n"], k_eff)
     print(f"{plan.work_items} work items, worst bound ratio {worst:.3g}")
 
 

@@ -283,3 +283,103 @@ def test_planted_crt_mistake_is_caught():
     oracle.plant_mistake(6)
     with pytest.raises(AssertionError):
         test_ozaki_crt_matches_python_integers()
+
+
+# ---------------------------------------------------------------------------
+# The cost model's CPU restatement (oracle/mlp.py)
+
+def test_mlp_forward_by_hand():
+    """A sparse parameter set whose output is worked out by hand."""
+    from oracle import mlp
+    outs = 4
+    p = np.zeros(mlp.num_params(outs))
+    w = {name: (off, shape) for name, off, shape in mlp.offsets(outs)}
+
+    def put(name, idx, v):
+        off, shape = w[name]
+        p[off + np.ravel_multi_index(idx, shape)] = v
+    put("W1", (0, 0), 2.0)     # h1[0] = relu(2 * 0.5) = 1
+    put("W1", (1, 3), -1.0)    # h1[1] = relu(-1) = 0
+    put("b1", (2,), 0.25)      # h1[2] = 0.25
+    put("W2", (0, 0), 3.0)     # h2[0] = 3 * 1 + 4 * 0.25 = 4
+    put("W2", (0, 2), 4.0)
+    put("W2", (1, 1), 5.0)     # h2[1] = relu(5 * 0 - 1) = 0
+    put("b2", (1,), -1.0)
+    put("W3", (5, 0), 0.5)     # h3[5] = 2
+    put("W3", (6, 1), 9.0)     # h3[6] = 0
+    put("W4", (2, 5), 1.5)     # out[2] = 1.5 * 2 + 0.125
+    put("b4", (2,), 0.125)
+    put("b4", (3,), -7.0)
+    x = np.array([[0.5, 0.0, 0.0, 1.0, 0.0, 1.0]])
+    out, (_, h1, h2, h3) = mlp.forward(p, outs, x)
+    assert np.array_equal(out[0], [0.0, 0.0, 3.125, -7.0])
+    assert (h1[0, 0], h1[0, 1], h1[0, 2]) == (1.0, 0.0, 0.25)
+    assert (h2[0, 0], h2[0, 1], h3[0, 5], h3[0, 6]) == (4.0, 0.0, 2.0, 0.0)
+
+
+def _mlp_case(seed=4, outs=5, n=7):
+    from oracle import mlp
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.3, mlp.num_params(outs))
+    x = rng.uniform(0, 1, (n, mlp.K_IN))
+    cfg = rng.integers(0, outs, n)
+    y = rng.normal(1.0, 0.5, n)
+    return p, outs, x, cfg, y
+
+
+def _fd_check(grad_fn, loss_fn, p, idx, h=1e-6):
+    worst = 0.0
+    g = grad_fn(p)
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = h
+        fd = (loss_fn(p + e) - loss_fn(p - e)) / (2 * h)
+        worst = max(worst, abs(fd - g[i]) / (abs(fd) + abs(g[i]) + 1e-7))
+    return worst
+
+
+def test_mlp_gradient_matches_finite_differences():
+    from oracle import mlp
+    p, outs, x, cfg, y = _mlp_case()
+    idx = np.random.default_rng(1).choice(len(p), 400, replace=False)
+    # Every block is covered, including the biases of the last layer.
+    idx = np.concatenate([idx, np.arange(len(p) - outs, len(p))])
+    worst = _fd_check(lambda q: mlp.loss_and_grad(q, outs, x, cfg, y)[1],
+                      lambda q: mlp.loss(q, outs, x, cfg, y), p, idx)
+    assert worst < 2e-5, worst
+    assert np.isclose(mlp.loss_and_grad(p, outs, x, cfg, y)[0], mlp.loss(p, outs, x, cfg, y))
+
+
+def test_mlp_gradient_check_catches_a_dropped_relu_mask():
+    """A backward pass that forgets one ReLU derivative fails the same check."""
+    from oracle import mlp
+    p, outs, x, cfg, y = _mlp_case()
+
+    def wrong_grad(q):
+        w = mlp.unpack(q, outs)
+        out, (xx, h1, h2, h3) = mlp.forward(q, outs, x)
+        n = len(cfg)
+        d4 = np.zeros_like(out)
+        d4[np.arange(n), cfg] = (out[np.arange(n), cfg] - y) / n
+        d3 = (d4 @ w["W4"]) * (h3 > 0)
+        d2 = d3 @ w["W3"]  # mask of layer 2 dropped
+        d1 = (d2 @ w["W2"]) * (h1 > 0)
+        g = {"W4": d4.T @ h3, "b4": d4.sum(0), "W3": d3.T @ h2, "b3": d3.sum(0),
+             "W2": d2.T @ h1, "b2": d2.sum(0), "W1": d1.T @ xx, "b1": d1.sum(0)}
+        return np.concatenate([g[name].ravel() for name, _, _ in mlp.offsets(outs)])
+    names = {name: (off, shape) for name, off, shape in mlp.offsets(outs)}
+    off, shape = names["b2"]
+    worst = _fd_check(wrong_grad, lambda q: mlp.loss(q, outs, x, cfg, y), p,
+                      np.arange(off, off + shape[0]))
+    assert worst > 1e-3
+
+
+def test_mlp_features_match_library():
+    """The library's host feature function and the restatement agree."""
+    import ctypes
+    from oracle import mlp
+    import hwyblas_amd as H
+    buf = (ctypes.c_float * 6)()
+    for args in [(8, 0, 1, 8, 512, 77), (4, 1, 0, 1, 300, 2), (8, 1, 1, 512, 512, 512)]:
+        H.lib().hwyblas_grouped_features(*args, buf)
+        assert np.allclose(np.array(buf[:]), mlp.features(*args), rtol=1e-6, atol=1e-7)
