@@ -122,8 +122,8 @@ class DeviceBatch:
         return out
 
 
-def batched_gemm_leg(H, oracle, dtype, count, steps, warmup, stream, configs=None, check=16,
-                     seed=7):
+def batched_gemm_leg(H, oracle, dtype, count, steps, warmup, stream, model=None, configs=None,
+                     check=16, seed=7):
     """`count` uneven GEMMs, m, n, k log-uniform in [8, 512], random trans,
     uniform[-1, 1], alpha = 1, beta = 0: one grouped launch replayed from a
     hipGraph, against a loop of single-call hwyblas_?gemm_dev launches."""
@@ -139,15 +139,38 @@ def batched_gemm_leg(H, oracle, dtype, count, steps, warmup, stream, configs=Non
                            int(n[t]), int(k[t]), 1.0, 0.0, batch.ptr(0, t), int(sa[t][0]),
                            batch.ptr(1, t), int(sb[t][0]), batch.ptr(2, t), int(m[t]))
              for t in range(count)]
-    t0 = time.perf_counter()
-    plan = H.Plan("gemm", dtype, probs, configs=configs)
-    plan_s = time.perf_counter() - t0
     flops = float(np.sum(2.0 * m * n * k))
     nbytes = float(np.sum((m * k + k * n + m * n) * np.dtype(dtype).itemsize))
+    arr = (H.GemmProblem * count)(*probs)
+    dbytes = np.dtype(dtype).itemsize
 
-    def grouped_step():
-        plan.launch(stream, graph=True)
-    g_ms, _ = time_steps(H, stream, grouped_step, steps, warmup)
+    def choose(policy):
+        cfg = np.zeros(count, dtype=np.int32)
+        assert H.lib().hwyblas_grouped_choose_configs(dbytes, count, arr, policy,
+                                                       cfg.ctypes.data, stream) == 0
+        return cfg
+
+    def time_plan(cfg):
+        t0 = time.perf_counter()
+        pl = H.Plan("gemm", dtype, probs, configs=cfg)
+        ps = time.perf_counter() - t0
+        ms, _ = time_steps(H, stream, lambda: pl.launch(stream, graph=True), steps, warmup)
+        return pl, ms, ps
+
+    # The built-in rule, then the configurations the cost model picks.
+    plan0, d_ms, _ = time_plan(choose(0))
+    plan0.destroy()
+    policy = "default"
+    if model is not None:
+        t0 = time.perf_counter()
+        cfg_used = choose(1)
+        choose_s = time.perf_counter() - t0
+        policy = "mlp"
+    else:
+        cfg_used, choose_s = choose(0), 0.0
+    if configs is not None:
+        cfg_used, policy = np.asarray(configs, dtype=np.int32), "given"
+    plan, g_ms, plan_s = time_plan(cfg_used)
 
     pre = "d" if np.dtype(dtype) == np.float64 else "s"
     f = getattr(H.lib(), f"hwyblas_{pre}gemm_dev")
@@ -177,8 +200,10 @@ def batched_gemm_leg(H, oracle, dtype, count, steps, warmup, stream, configs=Non
         worst = max(worst, float(np.max(np.abs(c.astype(np.float64) - exp) / (tol * bnd))))
     cfg_hist = np.bincount([plan.problem_config(t) for t in range(count)],
                            minlength=len(H.grouped_configs())).tolist()
-    out = {"count": count, "dtype": "f64" if pre == "d" else "f32",
-           "grouped_ms": round(g_ms, 4), "loop_ms": round(l_ms, 4),
+    out = {"count": count, "dtype": "f64" if pre == "d" else "f32", "configs": policy,
+           "grouped_ms": round(g_ms, 4), "grouped_ms_default_configs": round(d_ms, 4),
+           "gain_vs_default_configs": round(d_ms / g_ms, 3),
+           "choose_configs_s": round(choose_s, 3), "loop_ms": round(l_ms, 4),
            "gemm_per_s": round(count / (g_ms * 1e-3), 1),
            "loop_gemm_per_s": round(count / (l_ms * 1e-3), 1),
            "speedup_vs_loop": round(l_ms / g_ms, 3),
@@ -292,10 +317,18 @@ def cpu_baseline(oracle, seconds=12.0):
 
 def run_batched(H, oracle, args, stream):
     out = {}
+    model = None
+    if not args.no_model:
+        from hwyblas_amd import tune
+        try:
+            model = tune.load_model()
+        except (OSError, ValueError) as e:
+            log(f"no cost model ({e}); built-in configuration rule only")
     for dtype, name in ((np.float64, "gemm_f64"), (np.float32, "gemm_f32")):
         r = batched_gemm_leg(H, oracle, dtype, args.batch_count, max(3, args.steps),
-                             args.warmup, stream)
-        log(f"batched {name}: {r['gemm_per_s']:.0f} GEMM/s ({r['grouped_ms']:.2f} ms), loop "
+                             args.warmup, stream, model=model)
+        log(f"batched {name}: {r['gemm_per_s']:.0f} GEMM/s ({r['grouped_ms']:.2f} ms, configs "
+            f"{r['configs']}; built-in rule {r['grouped_ms_default_configs']:.2f} ms), loop "
             f"{r['loop_ms']:.2f} ms ({r['speedup_vs_loop']:.2f}x), loop graph "
             f"{r['loop_graph_ms']:.2f} ms ({r['speedup_vs_loop_graph']:.2f}x), "
             f"{r['tflops']:.2f} TFLOP/s, "
@@ -307,6 +340,9 @@ def run_batched(H, oracle, args, stream):
         f"{r['loop_graph_ms']:.2f} ms ({r['speedup_vs_loop_graph']:.2f}x), worst residual ratio "
         f"{r['worst_residual_ratio']:.3g}")
     out["trsm_f64"] = r
+    if model is not None:
+        H.lib().hwyblas_grouped_set_model(None)
+        model.destroy()
     return out
 
 
@@ -323,6 +359,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-only", action="store_true")
     ap.add_argument("--batch-count", type=int, default=10000)
+    ap.add_argument("--no-model", action="store_true",
+                    help="batched GEMM with the built-in configuration rule only")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (a kernel of known traffic, "
                          "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")

@@ -88,6 +88,10 @@ def _declare(L):
     fn("hwyblas_mlp_train", _i, [_p, _i, _p, _p, _p, _i, _i, ctypes.c_float, ctypes.c_uint64,
                                  _p, _p])
     fn("hwyblas_grouped_set_model", None, [_p])
+    fn("hwyblas_dgetrf_dev", _i, [_i, _i, _p, _i, _p, _i, _p])
+    fn("hwyblas_dgetrs_dev", _i, [ctypes.c_char, _i, _i, _p, _i, _p, _p, _i, _p])
+    fn("hwyblas_dgesv_ir_dev", _i, [_i, _i, _p, _i, _p, _i, _p, _p, _i, _p, _i, _i, _i, _p, _p,
+                                    _p, _p])
     fn("hwyblas_gemm_config_count", _i, [])
     fn("hwyblas_gemm_config_info", _i, [_i, _pi, _pi, _pi, _pi])
     fn("hwyblas_gemm_default_config", _i, [_i, _c, _c, _i, _i, _i])
@@ -416,3 +420,31 @@ class Arena:
     def block(self, flat, i, shape):
         off = self.offsets[i]
         return flat[off:off + shape[0] * shape[1]].reshape(shape, order="F")
+
+
+# ---------------------------------------------------------------------------
+# LU consumer
+
+def getrf_dev(a, update=0, stream=None):
+    """LU of the device matrix `a` (DeviceArray, m = a.ld rows unless given)
+    in place; returns (ipiv DeviceArray of int32, info)."""
+    m, n = a.ld, a.cols
+    ipiv = DeviceArray(max(1, min(m, n)), 1, np.int32)
+    info = lib().hwyblas_dgetrf_dev(m, n, a.ptr, a.ld, ipiv.ptr, update, stream)
+    return ipiv, info
+
+
+def gesv_ir_dev(a, b, update=0, itmax=5, stream=None):
+    """Mixed-precision solve of A X = B (device arrays); returns
+    (X DeviceArray, berr, nbe, iterations, info)."""
+    n, nrhs = a.ld, b.cols
+    af = DeviceArray(n, n)
+    x = DeviceArray(n, nrhs)
+    ipiv = DeviceArray(max(1, n), 1, np.int32)
+    berr = np.zeros(nrhs)
+    nbe = np.zeros(nrhs)
+    iters = np.zeros(nrhs, dtype=np.int32)
+    info = lib().hwyblas_dgesv_ir_dev(n, nrhs, a.ptr, a.ld, af.ptr, n, ipiv.ptr, b.ptr, b.ld,
+                                      x.ptr, n, update, itmax, berr.ctypes.data,
+                                      nbe.ctypes.data, iters.ctypes.data, stream)
+    return x, berr, nbe, iters, info

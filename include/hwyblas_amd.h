@@ -301,6 +301,38 @@ HWYBLAS_AMD_API int hwyblas_mlp_train(void* model, int n, const float* features,
 HWYBLAS_AMD_API void hwyblas_grouped_set_model(void* model);
 
 /* ------------------------------------------------------------------------ */
+/* LU consumer on device memory: hwyBLAS's LAPACK port calls dgemm_/dtrsm_
+ * from dgetrf/dgetrs/dgerfs (src/lapack/getrf.cc:216-339,
+ * src/lapack/gesvx.cc:333-463); these run the same algorithms on the GPU. */
+
+/* A = P L U with partial pivoting (the pivot of a column is its first entry
+ * of largest magnitude, as xIAMAX), the recursion of dgetrf2
+ * (src/lapack/getrf.cc:112-207) with 64-column panels. ipiv: device array of
+ * min(m, n) 1-based row indices. update: precision of the trailing-matrix
+ * GEMMs (0 FP64, 1 emulated FP64 on the INT8 matrix cores where k >= 1024,
+ * 2 FP32); panels and TRSMs are FP64. Returns info: 0, -i for an invalid
+ * argument i (reported as DGETRF), or the first exactly-zero pivot (1-based).
+ * Synchronizes the stream. */
+HWYBLAS_AMD_API int hwyblas_dgetrf_dev(int m, int n, double* a, int lda, int* ipiv,
+                                       int update, void* stream);
+/* op(A) X = B with the factorization (src/lapack/getrf.cc:287-339); X
+ * overwrites B. Returns 0 or -i (reported as DGETRS). Asynchronous. */
+HWYBLAS_AMD_API int hwyblas_dgetrs_dev(char trans, int n, int nrhs, const double* a, int lda,
+                                       const int* ipiv, double* b, int ldb, void* stream);
+/* A X = B for nrhs <= 16: af := LU of A (dgetrf with `update`), X := dgetrs,
+ * then FP64 iterative refinement against A as dgerfs does it
+ * (src/lapack/gesvx.cc:395-463): a column is corrected while its
+ * componentwise backward error berr exceeds eps, halves at least, and fewer
+ * than itmax corrections were made. Host outputs per right-hand side: berr,
+ * nbe = ||B - A X||inf / (||A||inf ||X||inf), iterations = corrections.
+ * Returns 0, getrf's info (> 0), or -i (reported as DGESVIR). Synchronizes. */
+HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int lda, double* af,
+                                         int ldaf, int* ipiv, const double* b, int ldb,
+                                         double* x, int ldx, int update, int itmax,
+                                         double* berr, double* nbe, int* iterations,
+                                         void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Tile configurations of the plain GEMM kernel (no hwyBLAS counterpart: its
  * blocking is compile-time, src/gemm_kernel-inl.h:72-74). For the autotuner
  * and the cost model: the configurations are numbered 0..count-1; every one

@@ -70,6 +70,14 @@ def lib():
         L.orc_dgemm_ozaki.restype = None
         L.orc_dgemm_ozaki.argtypes = [_c_char, _c_char, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p,
                                       _i, _i]
+        L.orc_dgetrf.restype = _i
+        L.orc_dgetrf.argtypes = [_i, _i, _p, _i, _p]
+        L.orc_dlaswp.restype = None
+        L.orc_dlaswp.argtypes = [_i, _p, _i, _i, _i, _p, _i]
+        L.orc_dgetrs.restype = None
+        L.orc_dgetrs.argtypes = [_c_char, _i, _i, _p, _i, _p, _p, _i]
+        L.orc_dgerfs_berr.restype = None
+        L.orc_dgerfs_berr.argtypes = [_i, _i, _p, _i, _p, _i, _p, _i, _p]
         L.orc_plant_mistake.restype = None
         L.orc_plant_mistake.argtypes = [_i]
         _lib = L
@@ -171,3 +179,46 @@ def dgemm_ozaki(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nmod):
 
 def plant_mistake(which):
     lib().orc_plant_mistake(which)
+
+
+def getrf(a):
+    """LU of a copy of `a` (Fortran order): (lu, ipiv 1-based, info)."""
+    lu_ = np.array(a, dtype=np.float64, order="F", copy=True)
+    m, n = lu_.shape
+    ipiv = np.zeros(max(1, min(m, n)), dtype=np.int32)
+    info = lib().orc_dgetrf(m, n, _ptr(lu_), max(1, m), _ptr(ipiv))
+    return lu_, ipiv[:min(m, n)], info
+
+
+def getrs(trans, lu_, ipiv, b):
+    x = np.array(b, dtype=np.float64, order="F", copy=True)
+    if x.ndim == 1:
+        x = x.reshape(-1, 1, order="F")
+    n, nrhs = lu_.shape[0], x.shape[1]
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    lib().orc_dgetrs(_b(trans), n, nrhs, _ptr(lu_), max(1, n), _ptr(ipiv), _ptr(x), max(1, n))
+    return x
+
+
+def gerfs_berr(a, b, x):
+    a = np.asfortranarray(a, dtype=np.float64)
+    b = np.asfortranarray(b, dtype=np.float64)
+    x = np.asfortranarray(x, dtype=np.float64)
+    n, nrhs = b.shape
+    out = np.zeros(nrhs)
+    lib().orc_dgerfs_berr(n, nrhs, _ptr(a), a.shape[0], _ptr(b), b.shape[0], _ptr(x),
+                          x.shape[0], _ptr(out))
+    return out
+
+
+def lu_factors(lu_, ipiv):
+    """(P, L, U) with A = P L U from the packed factorization."""
+    m, n = lu_.shape
+    k = min(m, n)
+    L = np.tril(lu_[:, :k], -1) + np.eye(m, k)
+    U = np.triu(lu_[:k, :])
+    perm = np.arange(m)
+    for i, p in enumerate(ipiv):
+        perm[[i, p - 1]] = perm[[p - 1, i]]
+    P = np.eye(m)[:, perm]
+    return P, L, U

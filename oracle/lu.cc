@@ -1,0 +1,183 @@
+// CPU restatement of hwyBLAS's LU with partial pivoting, its solve and the
+// backward error of gerfs (TEST INFRASTRUCTURE ONLY; see oracle.h).
+//
+// Follows src/lapack/getrf.cc (xLASWP :28-81, xGETRF2 :112-207, xGETRF
+// :216-280, xGETRS :287-339) with hwyBLAS's leaf kernel
+// (src/lapack/factor_kernels.cc:107-138, leaf 16 rows or columns,
+// factor_kernels.h:27), GETRF's block size 64 (src/lapack/environment.cc:
+// 86-88), and the componentwise backward error of xGERFS
+// (src/lapack/gesvx.cc:395-425). GEMM and TRSM are the oracle's own
+// restatements (oracle.cc).
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <utility>
+
+#include "oracle.h"
+
+extern int g_mistake;
+
+namespace {
+
+constexpr int kGetrfLeaf = 16;
+constexpr int kGetrfNb = 64;
+
+// Index of the first entry of largest magnitude; 0 when the first entry is
+// NaN; other NaNs are skipped (src/level1-inl.h:353-410, kMax = true).
+int IamaxFirst(int n, const double* x) {
+  const double first = std::fabs(x[0]);
+  if (std::isnan(first)) return 0;
+  int best = 0;
+  double bv = first;
+  for (int i = 1; i < n; ++i) {
+    const double v = std::fabs(x[i]);
+    if (v > bv) {
+      bv = v;
+      best = i;
+    }
+  }
+  return best;
+}
+
+// src/lapack/factor_kernels.cc:107-138 (0-based, ipiv 1-based).
+int GetrfLeaf(int m, int n, double* a, int lda, int* ipiv) {
+  auto at = [&](int i, int j) -> double& { return a[i + int64_t(j) * lda]; };
+  const double sfmin = std::numeric_limits<double>::min();
+  int info = 0;
+  for (int j = 0; j < std::min(m, n); ++j) {
+    const int p = j + IamaxFirst(m - j, &at(j, j));
+    ipiv[j] = p + 1;
+    const double pivot = at(p, j);
+    if (pivot != 0.0) {
+      if (p != j)  // (planted mistake 7: the columns left of j are not swapped)
+        for (int k = g_mistake == 7 ? j : 0; k < n; ++k) std::swap(at(j, k), at(p, k));
+      if (std::fabs(pivot) >= sfmin) {
+        const double r = 1.0 / pivot;
+        for (int i = j + 1; i < m; ++i) at(i, j) = at(i, j) * r;
+      } else {
+        for (int i = j + 1; i < m; ++i) at(i, j) = at(i, j) / pivot;
+      }
+    } else if (info == 0) {
+      info = j + 1;
+    }
+    for (int k = j + 1; k < n; ++k) {
+      const double alpha = -at(j, k);
+      for (int i = j + 1; i < m; ++i) at(i, k) = std::fma(alpha, at(i, j), at(i, k));
+    }
+  }
+  return info;
+}
+
+// src/lapack/getrf.cc:28-81 with 1-based k1, k2 (the 32-column blocking
+// there does not change the result).
+void Laswp(int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx) {
+  if (incx == 0) return;
+  auto at = [&](int i, int j) -> double& { return a[(i - 1) + int64_t(j - 1) * lda]; };
+  if (incx > 0) {
+    for (int i = k1, ix = k1; i <= k2; ++i, ix += incx) {
+      const int ip = ipiv[ix - 1];
+      if (ip != i)
+        for (int k = 1; k <= n; ++k) std::swap(at(i, k), at(ip, k));
+    }
+  } else {
+    for (int i = k2, ix = k1 + (k1 - k2) * incx; i >= k1; --i, ix += incx) {
+      const int ip = ipiv[ix - 1];
+      if (ip != i)
+        for (int k = 1; k <= n; ++k) std::swap(at(i, k), at(ip, k));
+    }
+  }
+}
+
+// src/lapack/getrf.cc:112-207.
+int Getrf2(int m, int n, double* a, int lda, int* ipiv) {
+  if (m == 0 || n == 0) return 0;
+  if (std::min(m, n) <= kGetrfLeaf) return GetrfLeaf(m, n, a, lda, ipiv);
+  int info = 0;
+  const int n1 = std::min(m, n) / 2;
+  const int n2 = n - n1;
+  int iinfo = Getrf2(m, n1, a, lda, ipiv);
+  if (info == 0 && iinfo > 0) info = iinfo;
+  double* a12 = a + int64_t(n1) * lda;
+  Laswp(n2, a12, lda, 1, n1, ipiv, 1);
+  orc_dtrsm('L', 'L', 'N', 'U', n1, n2, 1.0, a, lda, a12, lda);
+  orc_dgemm('N', 'N', m - n1, n2, n1, -1.0, a + n1, lda, a12, lda, 1.0, a12 + n1, lda);
+  iinfo = Getrf2(m - n1, n2, a12 + n1, lda, ipiv + n1);
+  if (info == 0 && iinfo > 0) info = iinfo + n1;
+  for (int i = n1; i < std::min(m, n); ++i) ipiv[i] += n1;
+  Laswp(n1, a, lda, n1 + 1, std::min(m, n), ipiv, 1);
+  return info;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_dlaswp(int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx) {
+  Laswp(n, a, lda, k1, k2, ipiv, incx);
+}
+
+// src/lapack/getrf.cc:216-280.
+int orc_dgetrf(int m, int n, double* a, int lda, int* ipiv) {
+  if (m == 0 || n == 0) return 0;
+  const int nb = kGetrfNb;
+  if (nb >= std::min(m, n)) return Getrf2(m, n, a, lda, ipiv);
+  int info = 0;
+  auto ptr = [&](int i, int j) { return a + (i - 1) + int64_t(j - 1) * lda; };
+  for (int j = 1; j <= std::min(m, n); j += nb) {
+    const int jb = std::min(std::min(m, n) - j + 1, nb);
+    const int iinfo = Getrf2(m - j + 1, jb, ptr(j, j), lda, ipiv + (j - 1));
+    if (info == 0 && iinfo > 0) info = iinfo + j - 1;
+    for (int i = j; i <= std::min(m, j + jb - 1); ++i) ipiv[i - 1] += j - 1;
+    Laswp(j - 1, a, lda, j, j + jb - 1, ipiv, 1);
+    if (j + jb <= n) {
+      Laswp(n - j - jb + 1, ptr(1, j + jb), lda, j, j + jb - 1, ipiv, 1);
+      orc_dtrsm('L', 'L', 'N', 'U', jb, n - j - jb + 1, 1.0, ptr(j, j), lda, ptr(j, j + jb), lda);
+      if (j + jb <= m)
+        orc_dgemm('N', 'N', m - j - jb + 1, n - j - jb + 1, jb, -1.0, ptr(j + jb, j), lda,
+                  ptr(j, j + jb), lda, 1.0, ptr(j + jb, j + jb), lda);
+    }
+  }
+  return info;
+}
+
+// src/lapack/getrf.cc:287-339.
+void orc_dgetrs(char trans, int n, int nrhs, const double* a, int lda, const int* ipiv,
+                double* b, int ldb) {
+  if (n == 0 || nrhs == 0) return;
+  if (trans == 'N' || trans == 'n') {
+    Laswp(nrhs, b, ldb, 1, n, ipiv, 1);
+    orc_dtrsm('L', 'L', 'N', 'U', n, nrhs, 1.0, a, lda, b, ldb);
+    orc_dtrsm('L', 'U', 'N', 'N', n, nrhs, 1.0, a, lda, b, ldb);
+  } else {
+    orc_dtrsm('L', 'U', 'T', 'N', n, nrhs, 1.0, a, lda, b, ldb);
+    orc_dtrsm('L', 'L', 'T', 'U', n, nrhs, 1.0, a, lda, b, ldb);
+    Laswp(nrhs, b, ldb, 1, n, ipiv, -1);
+  }
+}
+
+// Componentwise backward error of X (src/lapack/gesvx.cc:395-425, trans =
+// 'N'): berr_j = max_i |B - A X|_ij / (|A| |X| + |B|)_ij, with the safe1 /
+// safe2 guard. The residual is accumulated in long double.
+void orc_dgerfs_berr(int n, int nrhs, const double* a, int lda, const double* b, int ldb,
+                     const double* x, int ldx, double* berr) {
+  const double eps = std::numeric_limits<double>::epsilon() * 0.5;
+  const double safmin = std::numeric_limits<double>::min();
+  const double safe1 = double(n + 1) * safmin, safe2 = safe1 / eps;
+  for (int j = 0; j < nrhs; ++j) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+      long double r = b[i + int64_t(j) * ldb];
+      double w = std::fabs(b[i + int64_t(j) * ldb]);
+      for (int k = 0; k < n; ++k) {
+        const double aik = a[i + int64_t(k) * lda], xk = x[k + int64_t(j) * ldx];
+        r -= static_cast<long double>(aik) * xk;
+        w += std::fabs(aik) * std::fabs(xk);
+      }
+      const double ra = std::fabs(double(r));
+      s = std::max(s, w > safe2 ? ra / w : (ra + safe1) / (w + safe1));
+    }
+    berr[j] = s;
+  }
+}
+
+}  // extern "C"

@@ -30,9 +30,10 @@
 #include <immintrin.h>
 #endif
 
-namespace {
-
+// The planted mistake (orc_plant_mistake), shared with lu.cc.
 int g_mistake = 0;
+
+namespace {
 
 // ---------------------------------------------------------------------------
 // Operands: op(X)(i, p) = data[i * rs + p * cs]   (src/gemm.h:28-36)

@@ -107,11 +107,23 @@ void orc_dgemm_ozaki(char transa, char transb, int m, int n, int k, double alpha
                      const double* a, int lda, const double* b, int ldb, double beta,
                      double* c, int ldc, int nmod);
 
+/* ---- LU with partial pivoting and its solve (oracle/lu.cc) ---------------
+ * hwyBLAS's dgetrf (blocked NB = 64 over the recursive dgetrf2 with a leaf of
+ * 16), dlaswp (1-based k1..k2, incx = +-1), dgetrs and gerfs's componentwise
+ * backward error. ipiv is 1-based. */
+int orc_dgetrf(int m, int n, double* a, int lda, int* ipiv);
+void orc_dlaswp(int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx);
+void orc_dgetrs(char trans, int n, int nrhs, const double* a, int lda, const int* ipiv,
+                double* b, int ldb);
+void orc_dgerfs_berr(int n, int nrhs, const double* a, int lda, const double* b, int ldb,
+                     const double* x, int ldx, double* berr);
+
 /* ---- planted mistakes (for tests that show the checks catch them) ------- */
 /* 0: none; 1: drop the last k term of every GEMM tile; 2: use op(A) with
  * the wrong transposition; 3: ignore beta; 4: TRSM skips the reciprocal of
  * the diagonal; 5: the double-double reference drops its low word; 6: the
- * emulated DGEMM's CRT drops the last modulus. */
+ * emulated DGEMM's CRT drops the last modulus; 7: the LU leaf does not swap
+ * the columns left of the pivot column. */
 void orc_plant_mistake(int which);
 
 #ifdef __cplusplus

@@ -1,6 +1,7 @@
-"""Writes tests/golden/blas_fixtures.npz: small GEMM and TRSM cases whose
-expected outputs come from an independent library, numpy's BLAS (OpenBLAS)
-and scipy.linalg.solve_triangular (LAPACK trtrs over OpenBLAS), in this
+"""Writes tests/golden/blas_fixtures.npz: small GEMM, TRSM and LU cases
+whose expected outputs come from an independent library, numpy's BLAS
+(OpenBLAS), scipy.linalg.solve_triangular (LAPACK trtrs) and
+scipy.linalg.lu_factor / lu_solve (LAPACK getrf / getrs), in this
 container.
 
 hwyBLAS stores no golden vectors (its tests regenerate inputs from seeds and
@@ -75,9 +76,28 @@ def main():
                         tcases.append(f"{key},{dt},{side},{uplo},{ta},{diag},{m},{n},{alpha}")
                         idx += 1
     data["trsm_cases"] = np.array(tcases)
+    # LU with partial pivoting (LAPACK dgetrf over OpenBLAS) and the solve
+    # with it (dgetrs), square and rectangular; one matrix with a zero column.
+    lcases = []
+    for idx, (m, n) in enumerate([(1, 1), (7, 7), (16, 16), (33, 33), (64, 64), (100, 100),
+                                  (150, 150), (40, 20), (20, 40), (30, 30)]):
+        a = rng.standard_normal((m, n))
+        if idx == 9:
+            a[:, 4] = 0.0
+        lu, piv = scipy.linalg.lu_factor(a, check_finite=False)
+        key = f"lu{idx}"
+        data[key + "_a"], data[key + "_lu"] = a, lu
+        data[key + "_ipiv"] = (piv + 1).astype(np.int32)
+        if m == n and idx != 9:
+            b = rng.uniform(-1, 1, (n, 3))
+            data[key + "_b"] = b
+            data[key + "_x"] = scipy.linalg.lu_solve((lu, piv), b)
+            data[key + "_xt"] = scipy.linalg.lu_solve((lu, piv), b, trans=1)
+        lcases.append(f"{key},{m},{n}")
+    data["lu_cases"] = np.array(lcases)
     data["versions"] = np.array([f"numpy {np.__version__}", f"scipy {scipy.__version__}"])
     np.savez_compressed(OUT, **data)
-    print(f"wrote {OUT}: {len(cases)} gemm, {len(tcases)} trsm cases")
+    print(f"wrote {OUT}: {len(cases)} gemm, {len(tcases)} trsm, {len(lcases)} lu cases")
 
 
 if __name__ == "__main__":

@@ -208,3 +208,19 @@ def test_grouped_config_table():
     assert H.lib().hwyblas_grouped_config_info(len(cfgs), None, None, None, None) == -1
     for m, n, k in [(8, 8, 8), (100, 70, 300), (512, 512, 512), (1, 512, 1)]:
         assert 0 <= H.lib().hwyblas_grouped_default_config(8, m, n, k) < len(cfgs)
+
+
+def test_lu_entry_points_reject_invalid_arguments():
+    """Argument checks come before any device work (positions as LAPACK's)."""
+    with H.ErrorCapture() as cap:
+        assert H.lib().hwyblas_dgetrf_dev(-1, 4, None, 4, None, 0, None) == -1
+        assert H.lib().hwyblas_dgetrf_dev(4, 4, None, 3, None, 0, None) == -4
+        assert H.lib().hwyblas_dgetrf_dev(4, 4, None, 4, None, 7, None) == -6
+        assert H.lib().hwyblas_dgetrs_dev(b"X", 4, 1, None, 4, None, None, 4, None) == -1
+        assert H.lib().hwyblas_dgetrs_dev(b"N", 4, 1, None, 4, None, None, 3, None) == -8
+        assert H.lib().hwyblas_dgesv_ir_dev(4, 17, None, 4, None, 4, None, None, 4, None, 4, 0,
+                                            5, None, None, None, None) == -2
+    assert cap.calls == [("DGETRF", 1), ("DGETRF", 4), ("DGETRF", 6), ("DGETRS", 1),
+                         ("DGETRS", 8), ("DGESVIR", 2)]
+    # Empty problems return at once.
+    assert H.lib().hwyblas_dgetrf_dev(0, 5, None, 1, None, 0, None) == 0

@@ -383,3 +383,70 @@ def test_mlp_features_match_library():
     for args in [(8, 0, 1, 8, 512, 77), (4, 1, 0, 1, 300, 2), (8, 1, 1, 512, 512, 512)]:
         H.lib().hwyblas_grouped_features(*args, buf)
         assert np.allclose(np.array(buf[:]), mlp.features(*args), rtol=1e-6, atol=1e-7)
+
+
+# ---------------------------------------------------------------------------
+# LU with partial pivoting (oracle/lu.cc)
+
+def lu_fixture_cases():
+    for row in FIX["lu_cases"]:
+        key, m, n = str(row).split(",")
+        yield key, int(m), int(n)
+
+
+def test_getrf_matches_lapack_fixtures():
+    """Same pivots as LAPACK's dgetrf and the same factors to rounding; the
+    zero column gives info = its index."""
+    for key, m, n in lu_fixture_cases():
+        a = _f(FIX[key + "_a"])
+        lu, ipiv, info = oracle.getrf(a)
+        assert np.array_equal(ipiv, FIX[key + "_ipiv"]), key
+        ref = FIX[key + "_lu"]
+        assert np.max(np.abs(lu - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))), key
+        assert info == (5 if key == "lu9" else 0), (key, info)
+
+
+def test_getrs_matches_lapack_fixtures():
+    for key, m, n in lu_fixture_cases():
+        if key + "_b" not in FIX:
+            continue
+        lu, ipiv, _ = oracle.getrf(_f(FIX[key + "_a"]))
+        for trans, name in (("N", "_x"), ("T", "_xt")):
+            x = oracle.getrs(trans, lu, ipiv, FIX[key + "_b"])
+            ref = FIX[key + name]
+            assert np.max(np.abs(x - ref)) <= 1e-10 * max(1.0, np.max(np.abs(ref))), (key, trans)
+
+
+def test_getrf_residual_criterion_and_planted_swap_mistake():
+    """||P L U - A||_1 / (n ||A||_1 eps) < 30 (hwyBLAS tests/getrf_test.cc:42-96),
+    at a size that runs the blocked path; a leaf that does not swap the
+    columns left of the pivot column fails it."""
+    rng = np.random.default_rng(8)
+    n = 200
+    a = _f(rng.standard_normal((n, n)))
+    eps = np.finfo(np.float64).eps
+
+    def ratio():
+        lu, ipiv, _ = oracle.getrf(a)
+        P, L, U = oracle.lu_factors(lu, ipiv)
+        return np.abs(P @ L @ U - a).sum(0).max() / (n * np.abs(a).sum(0).max() * eps)
+    assert ratio() < 30
+    oracle.plant_mistake(7)
+    try:
+        assert ratio() > 1e6
+    finally:
+        oracle.plant_mistake(0)
+
+
+def test_gerfs_berr_closed_forms():
+    """For A = I: berr_j = max_i |b_ij - x_ij| / (|x_ij| + |b_ij|); an exact
+    solution of an integer system has berr 0."""
+    rng = np.random.default_rng(3)
+    n = 9
+    b = rng.uniform(-1, 1, (n, 2))
+    x = b + rng.uniform(-1e-3, 1e-3, (n, 2))
+    berr = oracle.gerfs_berr(np.eye(n), b, x)
+    assert np.allclose(berr, np.max(np.abs(b - x) / (np.abs(x) + np.abs(b)), axis=0), rtol=1e-14)
+    a = np.triu(rng.integers(1, 4, (n, n))).astype(float)
+    xe = rng.integers(-5, 5, (n, 2)).astype(float)
+    assert np.array_equal(oracle.gerfs_berr(a, a @ xe, xe), [0.0, 0.0])
