@@ -288,6 +288,185 @@ def batched_trsm_leg(H, oracle, count, steps, warmup, stream, check=16, seed=11)
     return out
 
 
+def plain_sweep(H, oracle, sizes=(4096, 8192, 16384), reps=3, samples=64):
+    """The plain kernels at scale: S/DGEMM m = n = k in `sizes`, every
+    transa/transb, column-major (hwyblas_?gemm_dev, event-timed) and row-major
+    (cblas_?gemm on device pointers, wall-timed, synchronous), uniform[-1, 1],
+    alpha = 1, beta = 0; then DTRSM m = n = 8192 for the 8 side/uplo/trans
+    combinations (diag N), A triangular with uniform[-1, 1] off-diagonal
+    entries and diagonal n + uniform[0, 1]. Sampled entries are checked
+    against the double-double reference (GEMM: componentwise bound ratio;
+    TRSM: residual ratio)."""
+    L = H.lib()
+    stream = L.hwyblas_amd_stream_create()
+    out = {"gemm": [], "trsm": []}
+    rng = np.random.default_rng(5)
+    for dtype, pre, peak in ((np.float64, "d", PEAK_FP64), (np.float32, "s", 157.3)):
+        for n in sizes:
+            dA, dB, dC = (H.DeviceArray(n, n, dtype) for _ in range(3))
+            dA.fill_uniform(n, n, 1, stream=stream)
+            dB.fill_uniform(n, n, 2, stream=stream)
+            L.hwyblas_amd_stream_synchronize(stream)
+            ha = oracle.fill_uniform(n, n, 1, dtype=dtype)
+            hb = oracle.fill_uniform(n, n, 2, dtype=dtype)
+            rows = rng.integers(0, n, samples)
+            cols = rng.integers(0, n, samples)
+            flops = 2.0 * n ** 3
+            for layout in ("col", "row"):
+                for ta in "NT":
+                    for tb in "NT":
+                        if layout == "col":
+                            def step():
+                                assert H.gemm_dev(ta, tb, n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n,
+                                                  stream=stream, dtype=dtype) == 0
+                            ms, _ = time_steps(H, stream, step, reps, 1)
+                        else:
+                            t_a = H.Trans if ta == "T" else H.NoTrans
+                            t_b = H.Trans if tb == "T" else H.NoTrans
+                            H.cblas_gemm(H.RowMajor, t_a, t_b, n, n, n, 1.0, dA, n, dB, n, 0.0,
+                                         dC, n, dtype=dtype)
+                            t0 = time.perf_counter()
+                            for _ in range(reps):
+                                H.cblas_gemm(H.RowMajor, t_a, t_b, n, n, n, 1.0, dA, n, dB, n,
+                                             0.0, dC, n, dtype=dtype)
+                            ms = (time.perf_counter() - t0) * 1e3 / reps
+                        got = np.empty(samples, dtype=dtype)
+                        for t in range(samples):
+                            L.hwyblas_amd_memcpy(got[t:].ctypes.data,
+                                                 dC.ptr + int(rows[t] + cols[t] * n) *
+                                                 np.dtype(dtype).itemsize,
+                                                 np.dtype(dtype).itemsize, 2)
+                        if layout == "col":
+                            exp, bnd = oracle.gemm_ref_entries(ta, tb, n, 1.0, ha, n, hb, n, 0.0,
+                                                               None, n, rows, cols)
+                        else:  # row-major C = op(A) op(B)  <=>  column-major C = op(B) op(A)
+                            exp, bnd = oracle.gemm_ref_entries(tb, ta, n, 1.0, hb, n, ha, n, 0.0,
+                                                               None, n, rows, cols)
+                        ratio = float(np.max(np.abs(got.astype(np.float64) - exp) /
+                                             (4.0 * (n + 2) * np.finfo(dtype).eps * bnd)))
+                        tf = flops / (ms * 1e-3) / 1e12
+                        r = {"dtype": pre, "n": n, "layout": layout, "transa": ta,
+                             "transb": tb, "ms": round(ms, 3), "tflops": round(tf, 2),
+                             "frac_peak": round(tf / peak, 3), "bound_ratio": ratio}
+                        log(f"{pre}gemm {layout} {ta}{tb} {n}: {ms:.2f} ms {tf:.1f} TFLOP/s "
+                            f"({tf / peak:.0%}), bound ratio {ratio:.3g}")
+                        out["gemm"].append(r)
+            del dA, dB, dC, ha, hb
+    n = 8192
+    ha = oracle.fill_uniform(n, n, 5)
+    ha[np.arange(n), np.arange(n)] = n + (ha[np.arange(n), np.arange(n)] + 1.0) * 0.5
+    hb = oracle.fill_uniform(n, n, 6)
+    dA = H.DeviceArray.from_host(ha)
+    dB0 = H.DeviceArray.from_host(hb)
+    dB = H.DeviceArray(n, n)
+    for side in "LR":
+        for uplo in "LU":
+            for ta in "NT":
+                def reset():
+                    L.hwyblas_amd_memcpy(dB.ptr, dB0.ptr, n * n * 8, 3)
+
+                def step():
+                    assert H.trsm_dev(side, uplo, ta, "N", n, n, 1.0, dA, n, dB, n,
+                                      stream=stream, dtype=np.float64) == 0
+                times = []
+                for _ in range(reps):
+                    reset()
+                    ms, _ = time_steps(H, stream, step, 1, 0)
+                    times.append(ms)
+                ms = min(times)
+                reset()
+                step()
+                L.hwyblas_amd_stream_synchronize(stream)
+                which = rng.integers(0, n, 3)
+                x = dB.to_host()
+                rr = oracle.trsm_residual(side, uplo, ta, "N", n, n, 1.0, ha, n, hb, n, x, n,
+                                          which)
+                ratio = float(np.max(rr) / (8.0 * (n + 2) * np.finfo(np.float64).eps))
+                tf = float(n) ** 3 / (ms * 1e-3) / 1e12
+                out["trsm"].append({"side": side, "uplo": uplo, "transa": ta, "n": n,
+                                    "ms": round(ms, 3), "tflops": round(tf, 2),
+                                    "frac_peak": round(tf / PEAK_FP64, 3),
+                                    "residual_ratio": ratio})
+                log(f"dtrsm {side}{uplo}{ta}N {n}: {ms:.2f} ms {tf:.1f} TFLOP/s, residual ratio "
+                    f"{ratio:.3g}")
+    L.hwyblas_amd_stream_destroy(stream)
+    return out
+
+
+DISTRIBUTIONS = {
+    "b": "+-u 2^e, u uniform in [1,2), e uniform integer in [-30,30] (seeds 11, 12)",
+    "c": "uniform[-1,1] (seeds 1, 2), rows of A and columns of B scaled by 10^s, "
+         "s uniform in [-8,8] (numpy seeds 21, 22)",
+}
+
+
+def distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, samples, nmod=0, steps=2):
+    """Emulated (moduli from the operands, or `nmod`) and native DGEMM on one
+    of the wide-range distributions; sampled componentwise errors against the
+    double-double reference on the same operands regenerated on the host."""
+    L = H.lib()
+    if dist == "b":
+        L.hwyblas_amd_fill_pow2(dA.ptr, n, n, n, 11, -30, 30, stream)
+        L.hwyblas_amd_fill_pow2(dB.ptr, n, n, n, 12, -30, 30, stream)
+    elif dist == "c":
+        rs = np.power(10.0, np.random.default_rng(21).uniform(-8, 8, n))
+        cs = np.power(10.0, np.random.default_rng(22).uniform(-8, 8, n))
+        drs = H.DeviceArray.from_host(rs.reshape(-1, 1))
+        dcs = H.DeviceArray.from_host(cs.reshape(-1, 1))
+        dA.fill_uniform(n, n, 1, stream=stream)
+        dB.fill_uniform(n, n, 2, stream=stream)
+        L.hwyblas_amd_scale_rows_cols(dA.ptr, n, n, n, drs.ptr, None, stream)
+        L.hwyblas_amd_scale_rows_cols(dB.ptr, n, n, n, None, dcs.ptr, stream)
+    else:  # "a"
+        dA.fill_uniform(n, n, SEEDS[0], stream=stream)
+        dB.fill_uniform(n, n, SEEDS[1], stream=stream)
+    L.hwyblas_amd_stream_synchronize(stream)
+    flops = 2.0 * n ** 3
+
+    def native_step():
+        assert H.gemm_dev("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n, stream=stream,
+                          dtype=np.float64) == 0
+    nat_ms, _ = time_steps(H, stream, native_step, steps, 1)
+    rng = np.random.default_rng(321)
+    rows = rng.integers(0, n, samples)
+    cols = rng.integers(0, n, samples)
+    nat_vals = gather(H, dC, n, rows, cols)
+    used = [0]
+
+    def emu_step():
+        info, u = H.dgemm_emulated_dev("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n,
+                                       nmoduli=nmod, stream=stream)
+        assert info == 0
+        used[0] = u
+    emu_ms, _ = time_steps(H, stream, emu_step, steps, 1)
+    emu_vals = gather(H, dC, n, rows, cols)
+    if dist == "b":
+        ha, hb = oracle.fill_pow2(n, n, 11), oracle.fill_pow2(n, n, 12)
+    else:
+        ha, hb = oracle.fill_uniform(n, n, 1), oracle.fill_uniform(n, n, 2)
+        if dist == "c":
+            ha *= rs[:, None]
+            hb *= cs[None, :]
+    exp, bnd = oracle.gemm_ref_entries("N", "N", n, 1.0, ha, n, hb, n, 0.0, None, n, rows, cols)
+    del ha, hb
+    e_nat = float(np.max(np.abs(nat_vals - exp) / bnd))
+    e_emu = float(np.max(np.abs(emu_vals - exp) / bnd))
+    nz = np.abs(exp) > 0
+    r_nat = float(np.max(np.abs(nat_vals - exp)[nz] / np.abs(exp)[nz]))
+    r_emu = float(np.max(np.abs(emu_vals - exp)[nz] / np.abs(exp)[nz]))
+    out = {"distribution": DISTRIBUTIONS.get(dist, "uniform[-1,1] (seeds 1, 2)"),
+           "moduli": used[0], "emulated_ms": round(emu_ms, 3), "native_ms": round(nat_ms, 3),
+           "emulated_tflops": round(flops / (emu_ms * 1e-3) / 1e12, 2),
+           "native_tflops": round(flops / (nat_ms * 1e-3) / 1e12, 2),
+           "emulated_max_componentwise_rel": e_emu, "native_max_componentwise_rel": e_nat,
+           "emulated_max_rel": r_emu, "native_max_rel": r_nat,
+           "ratio_emulated_to_native": e_emu / e_nat if e_nat > 0 else None,
+           "samples": int(samples)}
+    log(f"dist {dist}: {used[0]} moduli, emulated {out['emulated_tflops']} TF vs native "
+        f"{out['native_tflops']} TF; componentwise error emulated {e_emu:.3e} native {e_nat:.3e}")
+    return out
+
+
 def cpu_baseline(oracle, seconds=12.0):
     """The oracle's single-threaded restatement of hwyBLAS's blocked DGEMM,
     DGEMM NN 2048^3 on one pinned core, best of the calls that fit in about
@@ -358,7 +537,15 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-only", action="store_true")
+    ap.add_argument("--plain-sweep", action="store_true",
+                    help="S/DGEMM at 4096-16384 (every transposition, both layouts) and DTRSM "
+                         "8192 (every side/uplo/trans) instead of the headline")
     ap.add_argument("--batch-count", type=int, default=10000)
+    ap.add_argument("--dists", default="bc",
+                    help="wide-range operand distributions to run after the headline (b, c)")
+    ap.add_argument("--moduli-sweep", default="",
+                    help="comma-separated moduli counts: error and speed of the emulation at "
+                         "each, uniform operands")
     ap.add_argument("--no-model", action="store_true",
                     help="batched GEMM with the built-in configuration rule only")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -387,6 +574,9 @@ def main():
         H.synchronize()
         del src, dst
 
+    if args.plain_sweep:
+        print(json.dumps({"plain_sweep": plain_sweep(H, oracle)}), flush=True)
+        return
     if args.batched_only:
         out = run_batched(H, oracle, args, stream)
         print(json.dumps({"batched": out}), flush=True)
@@ -443,6 +633,15 @@ def main():
                "bound_4(k+2)u": 4 * (n + 2) * 2.0 ** -53}
         log(f"sampled error: emulated {e_emu:.3e}, native {e_nat:.3e}")
 
+    dists = {}
+    if not args.no_check:
+        for dist in args.dists:
+            dists[dist] = distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, args.samples)
+        for nm in [int(x) for x in args.moduli_sweep.split(",") if x]:
+            dists[f"a_moduli_{nm}"] = distribution_leg(H, oracle, "a", n, dA, dB, dC, stream,
+                                                       args.samples, nmod=nm)
+    del dA, dB, dC
+
     batched = None if args.no_batched else run_batched(H, oracle, args, stream)
 
     cpu = None if args.no_cpu else cpu_baseline(oracle)
@@ -482,6 +681,7 @@ def main():
                                      "copy-calibrated)",
                      "algorithmic_bytes": used[0] * 3 * n * n},
         "error": err,
+        "distributions": dists,
         "batched": batched,
         "cpu_baseline": cpu,
     }

@@ -68,6 +68,10 @@ def _declare(L):
     fn("hwyblas_grouped_config_count", _i, [])
     fn("hwyblas_grouped_config_info", _i, [_i, _pi, _pi, _pi, _pi])
     fn("hwyblas_grouped_default_config", _i, [_i, _i, _i, _i])
+    fn("hwyblas_amd_fill_pow2", None, [_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                       ctypes.c_uint64, _i, _i, _p])
+    fn("hwyblas_amd_scale_rows_cols", None, [_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                             _p, _p, _p])
     fn("hwyblas_amd_capture_begin", None, [_p])
     fn("hwyblas_amd_capture_end", _p, [_p])
     fn("hwyblas_amd_graph_launch", None, [_p, _p])

@@ -23,10 +23,10 @@
 // beta == 1 adds no work (C untouched); alpha == 0 never reads A or B.
 //
 // TRSM chunks solve op(A) X = alpha B (left) or X op(A) = alpha B (right)
-// for up to 64 right-hand sides, one per lane: B := alpha B, then rows are
-// solved in blocks of 32 held in registers, each element as
-// (b - sum_q x_q t) * (1 / t_jj) with the sum fused in solve order, as
-// hwyBLAS's base kernel computes it (src/trsm-inl.h:107-174).
+// for up to 64 right-hand sides, one per lane: B := alpha B, then blocks of
+// 16 rows are solved in LDS, each element as (b - sum_q l_iq x_q) *
+// (1 / l_ii) (hwyBLAS's base kernel, src/trsm-inl.h:107-174), and each
+// solved block updates the rows below it on the MFMA units.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -78,37 +78,67 @@ __device__ void GemmTile(const GemmDesc<T>& d, int i0, int j0) {
   // op(A)(i, p) and op(B)(p, j) element offsets.
   const int64_t a_is = d.ta ? d.lda : 1, a_ps = d.ta ? 1 : d.lda;
   const int64_t b_ps = d.tb ? d.ldb : 1, b_js = d.tb ? 1 : d.ldb;
-  int rows[TM], cols[TN];
+  int cols[TN];
   bool rok[TM], cok[TN];
+  const T* ap[TM];  // this lane's row of op(A) at k = lk
+  const T* bp[TN];  // this lane's column of op(B) at k = lk
 #pragma unroll
   for (int ti = 0; ti < TM; ++ti) {
-    rows[ti] = i0 + ti * 16 + li;
-    rok[ti] = rows[ti] < d.m;
+    const int row = i0 + ti * 16 + li;
+    rok[ti] = row < d.m;
+    ap[ti] = d.a + (rok[ti] ? int64_t(row) * a_is : 0) + int64_t(lk) * a_ps;
   }
 #pragma unroll
   for (int tj = 0; tj < TN; ++tj) {
     cols[tj] = j0 + tj * 16 + li;
     cok[tj] = cols[tj] < d.n;
+    bp[tj] = d.b + (cok[tj] ? int64_t(cols[tj]) * b_js : 0) + int64_t(lk) * b_ps;
   }
-  for (int p0 = 0; p0 < d.k; p0 += 4 * UK) {
-    T af[UK][TM], bf[UK][TN];
+  // Loads of one step (k in [p0, p0 + 4 UK)); out-of-range entries are 0.
+  auto load = [&](int p0, T (&af)[UK][TM], T (&bf)[UK][TN]) {
 #pragma unroll
     for (int u = 0; u < UK; ++u) {
-      const int p = p0 + 4 * u + lk;
-      const bool pok = p < d.k;
+      const int q = p0 + 4 * u;
+      const bool pok = q + lk < d.k;
 #pragma unroll
       for (int ti = 0; ti < TM; ++ti)
-        af[u][ti] = (rok[ti] && pok) ? d.a[rows[ti] * a_is + p * a_ps] : T(0);
+        af[u][ti] = (rok[ti] && pok) ? ap[ti][int64_t(q) * a_ps] : T(0);
 #pragma unroll
       for (int tj = 0; tj < TN; ++tj)
-        bf[u][tj] = (cok[tj] && pok) ? d.b[p * b_ps + cols[tj] * b_js] : T(0);
+        bf[u][tj] = (cok[tj] && pok) ? bp[tj][int64_t(q) * b_ps] : T(0);
     }
+  };
+  auto mma = [&](const T (&af)[UK][TM], const T (&bf)[UK][TN]) {
 #pragma unroll
     for (int u = 0; u < UK; ++u)
 #pragma unroll
       for (int ti = 0; ti < TM; ++ti)
 #pragma unroll
         for (int tj = 0; tj < TN; ++tj) acc[ti][tj] = M::Run(af[u][ti], bf[u][tj], acc[ti][tj]);
+  };
+  constexpr int kStep = 4 * UK;
+  // 32-bit registers of one stage of operands and of the accumulators.
+  constexpr int kWords = int(sizeof(T)) / 4;
+  constexpr int kStageRegs = UK * (TM + TN) * kWords, kAccRegs = TM * TN * 4 * kWords;
+  if constexpr (2 * kStageRegs + kAccRegs <= 80) {
+    // Two register stages: the next step's loads are in flight while the
+    // current step multiplies. k still advances in ascending steps of 4.
+    T af0[UK][TM], bf0[UK][TN], af1[UK][TM], bf1[UK][TN];
+    if (d.k > 0) load(0, af0, bf0);
+    for (int p0 = 0; p0 < d.k; p0 += 2 * kStep) {
+      if (p0 + kStep < d.k) load(p0 + kStep, af1, bf1);
+      mma(af0, bf0);
+      if (p0 + kStep >= d.k) break;
+      if (p0 + 2 * kStep < d.k) load(p0 + 2 * kStep, af0, bf0);
+      mma(af1, bf1);
+    }
+  } else {
+    // Larger tiles keep one stage (two would spill at two waves per SIMD).
+    for (int p0 = 0; p0 < d.k; p0 += kStep) {
+      T af[UK][TM], bf[UK][TN];
+      load(p0, af, bf);
+      mma(af, bf);
+    }
   }
   const T alpha = d.alpha, beta = d.beta;
   const bool beta_zero = beta == T(0), beta_one = beta == T(1);
@@ -257,7 +287,8 @@ struct TrsmDesc {
   T alpha;
 };
 
-constexpr int kTrsmBlock = 32;
+constexpr int kTrsmTile = 16;  // rows per block: one MFMA tile
+constexpr int kTrsmRhs = 64;   // right-hand sides per chunk: one per lane
 
 // L(i, q): the solve matrix, op(A) for the left side and op(A)^T for the
 // right side; op(A)(i, q) = A(i, q) or A(q, i).
@@ -268,68 +299,109 @@ __device__ __forceinline__ T SolveElem(const TrsmDesc<T>& d, int i, int q) {
   return d.a[off];
 }
 
-// Solves for the right-hand sides v0 .. v0 + 63 (one per lane). Element e of
-// right-hand side v is B(e, v) (left) or B(v, e) (right).
+// One wave's LDS: the current block of right-hand sides (solve rows x
+// right-hand sides), the block's triangle and its inverse diagonal.
 template <typename T>
-__device__ void TrsmChunk(const TrsmDesc<T>& d, int v0) {
-  const int lane = threadIdx.x & 63;
-  const int v = v0 + lane;
-  const int len = d.left ? d.m : d.n;      // system size
-  const int nv = d.left ? d.n : d.m;       // right-hand sides
-  const bool active = v < nv;
-  auto elem = [&](int e) -> T& {
-    return d.left ? d.b[int64_t(e) + int64_t(v) * d.ldb] : d.b[int64_t(v) + int64_t(e) * d.ldb];
-  };
+struct TrsmWaveLds {
+  T x[kTrsmTile][kTrsmRhs + 1];
+  T l[kTrsmTile][kTrsmTile + 1];
+  T inv[kTrsmTile];
+};
+
+// Orders this wave's LDS accesses across its lanes (no workgroup barrier:
+// the waves of a workgroup work on unrelated chunks).
+__device__ __forceinline__ void WaveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Solves for the right-hand sides v0 .. v0 + 63 of problem d. In solve order
+// e (ascending rows when L is lower, descending otherwise) L is lower
+// triangular. Blocks of 16 rows are solved in turn: the block's triangle is
+// staged in LDS and each lane solves its right-hand side there,
+// x_i = (b_i - sum_q l_iq x_q) * (1 / l_ii) (hwyBLAS's dot form with the
+// reciprocal, src/trsm-inl.h:107-174); then every later block is updated,
+// B_t -= L(t, b) X_b, on the MFMA units with X_b read from LDS.
+template <typename T>
+__device__ void TrsmChunk(const TrsmDesc<T>& d, int v0, TrsmWaveLds<T>& s) {
+  using M = Mfma<T>;
+  using Acc = typename M::Acc;
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int len = d.left ? d.m : d.n;  // system size
+  const int nv = d.left ? d.n : d.m;   // right-hand sides
+  const int nvc = min(kTrsmRhs, nv - v0);
   const bool asc = d.lower_solve;
-  const int nblk = (len + kTrsmBlock - 1) / kTrsmBlock;
-  for (int bi = 0; bi < nblk; ++bi) {
-    // Block rows [s, s + cnt) in solve order; position r of the block is row
-    // s + r (ascending) or e1 - 1 - r (descending).
-    const int e0 = asc ? bi * kTrsmBlock : max(0, len - (bi + 1) * kTrsmBlock);
-    const int e1 = asc ? min(len, (bi + 1) * kTrsmBlock) : len - bi * kTrsmBlock;
-    const int cnt = e1 - e0;
-    T x[kTrsmBlock];
-#pragma unroll
-    for (int r = 0; r < kTrsmBlock; ++r) {
-      const int e = asc ? e0 + r : e1 - 1 - r;
-      x[r] = (active && r < cnt) ? elem(e) : T(0);
-      if (d.alpha != T(1)) x[r] = (d.alpha == T(0)) ? T(0) : x[r] * d.alpha;
+  auto phys = [&](int e) { return asc ? e : len - 1 - e; };
+  // Element e of right-hand side v0 + c.
+  auto bptr = [&](int e, int c) -> T* {
+    const int64_t row = phys(e), v = v0 + c;
+    return d.left ? d.b + row + v * d.ldb : d.b + v + row * d.ldb;
+  };
+  auto lval = [&](int e, int q) { return SolveElem(d, phys(e), phys(q)); };
+
+  // B := alpha B; alpha == 0 leaves zeros without reading A
+  // (src/trsm.cc:172-175).
+  if (d.alpha != T(1)) {
+    if (lane < nvc)
+      for (int e = 0; e < len; ++e) {
+        T* x = bptr(e, lane);
+        *x = d.alpha == T(0) ? T(0) : *x * d.alpha;
+      }
+    if (d.alpha == T(0)) return;
+  }
+  const int nb = (len + kTrsmTile - 1) / kTrsmTile;
+  for (int b = 0; b < nb; ++b) {
+    const int e0 = b * kTrsmTile, cnt = min(kTrsmTile, len - e0);
+    // The diagonal block in LDS, then each lane's right-hand side.
+    for (int idx = lane; idx < kTrsmTile * kTrsmTile; idx += 64) {
+      const int i = idx / kTrsmTile, q = idx % kTrsmTile;
+      s.l[i][q] = (q < i && i < cnt) ? lval(e0 + i, e0 + q) : T(0);
     }
-    if (d.alpha != T(0)) {
-      // Contributions of the rows solved in earlier blocks, in solve order.
-      const int solved = bi * kTrsmBlock;
-      for (int s = 0; s < solved; ++s) {
-        const int q = asc ? s : len - 1 - s;
-        const T xq = active ? elem(q) : T(0);
+    if (lane < kTrsmTile)
+      s.inv[lane] = (lane < cnt && !d.unit) ? T(1) / lval(e0 + lane, e0 + lane) : T(1);
 #pragma unroll
-        for (int r = 0; r < kTrsmBlock; ++r) {
-          if (r < cnt) {
-            const int e = asc ? e0 + r : e1 - 1 - r;
-            x[r] = __builtin_fma(-SolveElem(d, e, q), xq, x[r]);
-          }
+    for (int i = 0; i < kTrsmTile; ++i)
+      s.x[i][lane] = (i < cnt && lane < nvc) ? *bptr(e0 + i, lane) : T(0);
+    WaveSync();
+    // Each lane solves its own column of s.x (rows past cnt stay 0).
+    for (int i = 0; i < cnt; ++i) {
+      T t = s.x[i][lane];
+      for (int q = 0; q < i; ++q) t = __builtin_fma(-s.l[i][q], s.x[q][lane], t);
+      s.x[i][lane] = t * s.inv[i];
+    }
+    if (lane < nvc)
+      for (int i = 0; i < cnt; ++i) *bptr(e0 + i, lane) = s.x[i][lane];
+    WaveSync();
+    // B_t -= L(t, b) X_b for every later block t.
+    for (int t = b + 1; t < nb; ++t) {
+      const int r0 = t * kTrsmTile;
+      Acc acc[kTrsmRhs / 16];
+#pragma unroll
+      for (int jt = 0; jt < kTrsmRhs / 16; ++jt) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + M::Row(lk, r), col = 16 * jt + li;
+          acc[jt][r] = (row < len && col < nvc) ? *bptr(row, col) : T(0);
         }
       }
-      // The diagonal block, column by column.
 #pragma unroll
-      for (int r = 0; r < kTrsmBlock; ++r) {
-        if (r < cnt) {
-          const int e = asc ? e0 + r : e1 - 1 - r;
-          if (!d.unit) x[r] = x[r] * (T(1) / SolveElem(d, e, e));
+      for (int u = 0; u < kTrsmTile / 4; ++u) {
+        const int i = r0 + li, q = 4 * u + lk;
+        const T a = (i < len && q < cnt) ? -lval(i, e0 + q) : T(0);
 #pragma unroll
-          for (int r2 = r + 1; r2 < kTrsmBlock; ++r2) {
-            if (r2 < cnt) {
-              const int e2 = asc ? e0 + r2 : e1 - 1 - r2;
-              x[r2] = __builtin_fma(-SolveElem(d, e2, e), x[r], x[r2]);
-            }
-          }
+        for (int jt = 0; jt < kTrsmRhs / 16; ++jt)
+          acc[jt] = M::Run(a, s.x[q][16 * jt + li], acc[jt]);
+      }
+#pragma unroll
+      for (int jt = 0; jt < kTrsmRhs / 16; ++jt) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + M::Row(lk, r), col = 16 * jt + li;
+          if (row < len && col < nvc) *bptr(row, col) = acc[jt][r];
         }
       }
     }
-    if (active) {
-#pragma unroll
-      for (int r = 0; r < kTrsmBlock; ++r)
-        if (r < cnt) elem(asc ? e0 + r : e1 - 1 - r) = x[r];
-    }
+    WaveSync();  // the next block overwrites the LDS
   }
 }
 
@@ -337,6 +409,8 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     GroupedTrsmKernel(const TrsmDesc<T>* __restrict__ descs, const WorkItem* __restrict__ items,
                       const int* __restrict__ unit_start, int n_units, int* counters) {
+  __shared__ TrsmWaveLds<T> lds[4];
+  TrsmWaveLds<T>& mine = lds[threadIdx.x >> 6];
   for (;;) {
     const int u = ClaimItem(&counters[kCounterStride]);
     if (u >= n_units) break;
@@ -344,7 +418,7 @@ __global__ void __launch_bounds__(256)
     for (int t = unit_start[u]; t < t1; ++t) {
       const WorkItem w = items[t];
       const TrsmDesc<T> d = descs[w.problem];
-      TrsmChunk<T>(d, w.i0);
+      TrsmChunk<T>(d, w.i0, mine);
     }
   }
 }

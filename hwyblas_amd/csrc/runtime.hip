@@ -109,6 +109,37 @@ __global__ void FillUniformKernel(T* x, int64_t rows, int64_t cols, int64_t ld,
   }
 }
 
+// x(i, j) = +-u 2^e: u in [1, 2) with 52 random bits and the sign from one
+// hash of the compact index, e uniform in [emin, emax] from a second.
+__global__ void FillPow2Kernel(double* x, int64_t rows, int64_t cols, int64_t ld, uint64_t key,
+                               uint64_t key2, int emin, int span) {
+  const int64_t total = rows * cols;
+  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t w = SplitMix64(key + uint64_t(idx));
+    const uint64_t w2 = SplitMix64(key2 + uint64_t(idx));
+    const double u = 1.0 + double(w >> 12) * 0x1.0p-52;
+    const int e = emin + int((w2 >> 32) % uint64_t(span));
+    const double v = ldexp(u, e);
+    const int64_t i = idx % rows, j = idx / rows;
+    x[i + j * ld] = (w & 1) ? -v : v;
+  }
+}
+
+// x(i, j) = (x(i, j) * rs[i]) * cs[j] (either scale may be absent).
+__global__ void ScaleRowsColsKernel(double* x, int64_t rows, int64_t cols, int64_t ld,
+                                    const double* rs, const double* cs) {
+  const int64_t total = rows * cols;
+  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = idx % rows, j = idx / rows;
+    double v = x[i + j * ld];
+    if (rs) v = v * rs[i];
+    if (cs) v = v * cs[j];
+    x[i + j * ld] = v;
+  }
+}
+
 }  // namespace
 }  // namespace hwy_amd
 
@@ -210,6 +241,27 @@ HWYBLAS_AMD_API void hwyblas_amd_fill_uniform(void* x, int dtype_bytes,
     hipLaunchKernelGGL(FillUniformKernel<float>, dim3(blocks), dim3(256), 0, s,
                        static_cast<float*>(x), rows, cols, ld, key, lo, hi - lo);
   }
+  HWY_HIP_CHECK(hipGetLastError());
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_fill_pow2(double* x, int64_t rows, int64_t cols, int64_t ld,
+                                           uint64_t seed, int emin, int emax, void* stream) {
+  if (rows <= 0 || cols <= 0 || emax < emin) return;
+  const uint64_t key = SplitMix64Host(seed ^ 0x5851F42D4C957F2Dull);
+  const uint64_t key2 = SplitMix64Host(seed ^ 0x2545F4914F6CDD1Dull);
+  const unsigned blocks = unsigned(std::min<int64_t>((rows * cols + 255) / 256, 1 << 16));
+  hipLaunchKernelGGL(FillPow2Kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     x, rows, cols, ld, key, key2, emin, emax - emin + 1);
+  HWY_HIP_CHECK(hipGetLastError());
+}
+
+HWYBLAS_AMD_API void hwyblas_amd_scale_rows_cols(double* x, int64_t rows, int64_t cols,
+                                                 int64_t ld, const double* row_scale,
+                                                 const double* col_scale, void* stream) {
+  if (rows <= 0 || cols <= 0) return;
+  const unsigned blocks = unsigned(std::min<int64_t>((rows * cols + 255) / 256, 1 << 16));
+  hipLaunchKernelGGL(ScaleRowsColsKernel, dim3(blocks), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), x, rows, cols, ld, row_scale, col_scale);
   HWY_HIP_CHECK(hipGetLastError());
 }
 

@@ -399,6 +399,16 @@ HWYBLAS_AMD_API void hwyblas_amd_fill_uniform(void* x, int dtype_bytes,
  * -1 if nothing was recorded. */
 HWYBLAS_AMD_API void hwyblas_amd_profile_enable(int on);
 HWYBLAS_AMD_API float hwyblas_amd_profile_ms(int phase);
+/* Seeded test data: x(i, j) = +-u 2^e with u uniform in [1, 2) (52 random
+ * bits), random sign and e a uniform integer in [emin, emax], a function of
+ * the seed and the compact index i + j * rows only. */
+HWYBLAS_AMD_API void hwyblas_amd_fill_pow2(double* x, int64_t rows, int64_t cols, int64_t ld,
+                                           uint64_t seed, int emin, int emax, void* stream);
+/* x(i, j) = (x(i, j) * row_scale[i]) * col_scale[j] (device vectors; NULL
+ * skips one). */
+HWYBLAS_AMD_API void hwyblas_amd_scale_rows_cols(double* x, int64_t rows, int64_t cols,
+                                                 int64_t ld, const double* row_scale,
+                                                 const double* col_scale, void* stream);
 /* Stream capture into a hipGraph (for loops of calls on device pointers):
  * begin on a stream, enqueue calls on it, end to get an instantiated graph,
  * launch it any number of times, destroy it. */

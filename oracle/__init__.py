@@ -70,6 +70,9 @@ def lib():
         L.orc_dgemm_ozaki.restype = None
         L.orc_dgemm_ozaki.argtypes = [_c_char, _c_char, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p,
                                       _i, _i]
+        L.orc_fill_pow2.restype = None
+        L.orc_fill_pow2.argtypes = [_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                    ctypes.c_uint64, _i, _i]
         L.orc_dgetrf.restype = _i
         L.orc_dgetrf.argtypes = [_i, _i, _p, _i, _p]
         L.orc_dlaswp.restype = None
@@ -222,3 +225,11 @@ def lu_factors(lu_, ipiv):
         perm[[i, p - 1]] = perm[[p - 1, i]]
     P = np.eye(m)[:, perm]
     return P, L, U
+
+
+def fill_pow2(rows, cols, seed, emin=-30, emax=30):
+    """+-u 2^e, u uniform in [1, 2), e uniform in [emin, emax] (Fortran order)."""
+    x = np.empty((rows, cols), dtype=np.float64, order="F")
+    lib().orc_fill_pow2(_ptr(x), rows, cols, rows, seed, emin, emax)
+    return x
+

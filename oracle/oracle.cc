@@ -741,6 +741,24 @@ void orc_fill_uniform_f64(double* x, int64_t rows, int64_t cols, int64_t ld,
       x[i + j * ld] = std::fma(orc_uniform01(seed, static_cast<uint64_t>(i + j * rows)), w, lo);
 }
 
+// The distribution "u 2^e" of the emulated-DGEMM workloads; identical bit
+// for bit to the library's device fill (hwyblas_amd_fill_pow2).
+void orc_fill_pow2(double* x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, int emin,
+                   int emax) {
+  const uint64_t key = orc_splitmix64(seed ^ 0x5851F42D4C957F2Dull);
+  const uint64_t key2 = orc_splitmix64(seed ^ 0x2545F4914F6CDD1Dull);
+  const uint64_t span = uint64_t(emax - emin + 1);
+  for (int64_t j = 0; j < cols; ++j)
+    for (int64_t i = 0; i < rows; ++i) {
+      const uint64_t idx = uint64_t(i + j * rows);
+      const uint64_t w = orc_splitmix64(key + idx);
+      const uint64_t w2 = orc_splitmix64(key2 + idx);
+      const double u = 1.0 + double(w >> 12) * 0x1.0p-52;
+      const double v = std::ldexp(u, emin + int((w2 >> 32) % span));
+      x[i + j * ld] = (w & 1) ? -v : v;
+    }
+}
+
 void orc_fill_uniform_f32(float* x, int64_t rows, int64_t cols, int64_t ld,
                           uint64_t seed, double lo, double hi) {
   const double w = hi - lo;

@@ -38,6 +38,10 @@ void orc_fill_uniform_f64(double* x, int64_t rows, int64_t cols, int64_t ld,
                           uint64_t seed, double lo, double hi);
 void orc_fill_uniform_f32(float* x, int64_t rows, int64_t cols, int64_t ld,
                           uint64_t seed, double lo, double hi);
+/* x(i,j) = +-u 2^e, u in [1,2) with 52 random bits, e uniform in [emin, emax]
+ * (two hashes of the compact index; the sign is bit 0 of the first). */
+void orc_fill_pow2(double* x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
+                   int emin, int emax);
 
 /* ---- hwyBLAS restatement (column-major, Fortran semantics; arguments are
  * assumed valid: validation is the ABI layer's job) ------------------------ */

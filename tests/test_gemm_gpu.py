@@ -282,3 +282,23 @@ def test_every_tile_config_gives_identical_bits(H, dtype):
                 assert r <= 1.0
             assert np.array_equal(got, ref), (cfg, ta, tb, m, n, k)
             assert np.array_equal(got[m:], c[m:])
+
+
+def test_device_pow2_fill_and_scaling_match_oracle(H):
+    """The wide-exponent test data (+-u 2^e) and the row/column scaling the
+    benchmark uses are bit for bit those the oracle regenerates on the host."""
+    m, n = 257, 130
+    d = H.DeviceArray(m + 1, n)
+    H.lib().hwyblas_amd_fill_pow2(d.ptr, m, n, m + 1, 77, -30, 30, None)
+    rs = np.power(10.0, np.random.default_rng(1).uniform(-8, 8, m))
+    cs = np.power(10.0, np.random.default_rng(2).uniform(-8, 8, n))
+    drs, dcs = H.DeviceArray.from_host(rs.reshape(-1, 1)), H.DeviceArray.from_host(cs.reshape(-1, 1))
+    H.synchronize()
+    host = oracle.fill_pow2(m, n, 77)
+    got = d.to_host()[:m]
+    assert np.array_equal(got, host)
+    e = np.frexp(np.abs(host))[1] - 1
+    assert e.min() == -30 and e.max() == 30 and (host < 0).any() and (host > 0).any()
+    H.lib().hwyblas_amd_scale_rows_cols(d.ptr, m, n, m + 1, drs.ptr, dcs.ptr, None)
+    H.synchronize()
+    assert np.array_equal(d.to_host()[:m], (host * rs[:, None]) * cs[None, :])
