@@ -116,3 +116,62 @@ def test_exhaustive_policy_uses_the_tuner(H, T):
     assert cfg[0] == cfg[2] and all(0 <= c < ncfg for c in cfg)
     us = T.tune(4, shapes[:2], reps=2)
     assert us.shape == (2, ncfg) and (us > 0).all()
+
+
+def test_trained_model_speeds_up_an_uneven_batch(H, T):
+    """The committed model (trained on the autotuner's samples) picks
+    configurations for an unseen uneven batch that run faster than the
+    built-in rule's; both plans give the same bits (every configuration sums
+    k in the same order)."""
+    import time
+    rng = np.random.default_rng(99)
+    count = 3000
+    shapes = T.sample_shapes(rng, count)
+    sa = [(k, m) if ta else (m, k) for ta, tb, m, n, k in shapes]
+    sb = [(n, k) if tb else (k, n) for ta, tb, m, n, k in shapes]
+    sizes = [(int(r) * int(c) + 63) // 64 * 64 for r, c in sa + sb]
+    offa = np.concatenate([[0], np.cumsum(sizes[:count])])
+    offb = np.concatenate([[0], np.cumsum(sizes[count:])])
+    offc = np.concatenate([[0], np.cumsum([(int(m) * int(n) + 63) // 64 * 64
+                                           for _, _, m, n, _ in shapes])])
+    A, B = H.DeviceArray(int(offa[-1]), 1), H.DeviceArray(int(offb[-1]), 1)
+    C0, C1 = H.DeviceArray(int(offc[-1]), 1), H.DeviceArray(int(offc[-1]), 1)
+    A.fill_uniform(int(offa[-1]), 1, 5)
+    B.fill_uniform(int(offb[-1]), 1, 6)
+
+    def problems(C):
+        return [H.GemmProblem(b"T" if ta else b"N", b"T" if tb else b"N", int(m), int(n),
+                              int(k), 1.0, 0.0, A.ptr + 8 * int(offa[t]), int(sa[t][0]),
+                              B.ptr + 8 * int(offb[t]), int(sb[t][0]), C.ptr + 8 * int(offc[t]),
+                              int(m)) for t, (ta, tb, m, n, k) in enumerate(shapes)]
+    arr = (H.GemmProblem * count)(*problems(C0))
+    model = T.load_model()
+    try:
+        cfg = {}
+        for policy in (0, 1):
+            c = np.zeros(count, dtype=np.int32)
+            assert H.lib().hwyblas_grouped_choose_configs(8, count, arr, policy, c.ctypes.data,
+                                                           None) == 0
+            cfg[policy] = c
+    finally:
+        H.lib().hwyblas_grouped_set_model(None)
+    assert (cfg[0] != cfg[1]).any()
+    plans = {0: H.Plan("gemm", np.float64, problems(C0), configs=cfg[0]),
+             1: H.Plan("gemm", np.float64, problems(C1), configs=cfg[1])}
+
+    def best_ms(plan):
+        plan.launch(graph=True)
+        H.synchronize()
+        best = np.inf
+        for _ in range(5):
+            t0 = time.perf_counter()
+            for _ in range(5):
+                plan.launch(graph=True)
+            H.synchronize()
+            best = min(best, (time.perf_counter() - t0) / 5 * 1e3)
+        return best
+    t_rule, t_model = best_ms(plans[0]), best_ms(plans[1])
+    print(f"{count} uneven DGEMMs: built-in rule {t_rule:.3f} ms, model {t_model:.3f} ms, "
+          f"gain {t_rule / t_model:.3f}x")
+    assert np.array_equal(C0.to_host(), C1.to_host())
+    assert t_model < 0.97 * t_rule
