@@ -34,9 +34,15 @@
 #include "common.h"
 #include "gemm_kernel.h"
 #include "grouped.h"
+#include "trsm_chunk.h"
 
 namespace hwy_amd {
 namespace grouped {
+
+using trsm_chunk::TrsmChunk;
+using trsm_chunk::TrsmDesc;
+using trsm_chunk::TrsmWaveLds;
+
 namespace {
 
 using gemm_detail::GemmParams;
@@ -276,136 +282,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 // TRSM
 
 template <typename T>
-struct TrsmDesc {
-  const T* a;
-  T* b;
-  int64_t lda, ldb;
-  int m, n;
-  int left, lower_solve;  // lower_solve: L(i, q) is lower triangular
-  int trans, unit;
-  int lower_stored;       // uplo
-  T alpha;
-};
-
-constexpr int kTrsmTile = 16;  // rows per block: one MFMA tile
-constexpr int kTrsmRhs = 64;   // right-hand sides per chunk: one per lane
-
-// L(i, q): the solve matrix, op(A) for the left side and op(A)^T for the
-// right side; op(A)(i, q) = A(i, q) or A(q, i).
-template <typename T>
-__device__ __forceinline__ T SolveElem(const TrsmDesc<T>& d, int i, int q) {
-  const int r = d.left ? i : q, c = d.left ? q : i;   // op(A)(r, c)
-  const int64_t off = d.trans ? int64_t(c) + int64_t(r) * d.lda : int64_t(r) + int64_t(c) * d.lda;
-  return d.a[off];
-}
-
-// One wave's LDS: the current block of right-hand sides (solve rows x
-// right-hand sides), the block's triangle and its inverse diagonal.
-template <typename T>
-struct TrsmWaveLds {
-  T x[kTrsmTile][kTrsmRhs + 1];
-  T l[kTrsmTile][kTrsmTile + 1];
-  T inv[kTrsmTile];
-};
-
-// Orders this wave's LDS accesses across its lanes (no workgroup barrier:
-// the waves of a workgroup work on unrelated chunks).
-__device__ __forceinline__ void WaveSync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Solves for the right-hand sides v0 .. v0 + 63 of problem d. In solve order
-// e (ascending rows when L is lower, descending otherwise) L is lower
-// triangular. Blocks of 16 rows are solved in turn: the block's triangle is
-// staged in LDS and each lane solves its right-hand side there,
-// x_i = (b_i - sum_q l_iq x_q) * (1 / l_ii) (hwyBLAS's dot form with the
-// reciprocal, src/trsm-inl.h:107-174); then every later block is updated,
-// B_t -= L(t, b) X_b, on the MFMA units with X_b read from LDS.
-template <typename T>
-__device__ void TrsmChunk(const TrsmDesc<T>& d, int v0, TrsmWaveLds<T>& s) {
-  using M = Mfma<T>;
-  using Acc = typename M::Acc;
-  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-  const int len = d.left ? d.m : d.n;  // system size
-  const int nv = d.left ? d.n : d.m;   // right-hand sides
-  const int nvc = min(kTrsmRhs, nv - v0);
-  const bool asc = d.lower_solve;
-  auto phys = [&](int e) { return asc ? e : len - 1 - e; };
-  // Element e of right-hand side v0 + c.
-  auto bptr = [&](int e, int c) -> T* {
-    const int64_t row = phys(e), v = v0 + c;
-    return d.left ? d.b + row + v * d.ldb : d.b + v + row * d.ldb;
-  };
-  auto lval = [&](int e, int q) { return SolveElem(d, phys(e), phys(q)); };
-
-  // B := alpha B; alpha == 0 leaves zeros without reading A
-  // (src/trsm.cc:172-175).
-  if (d.alpha != T(1)) {
-    if (lane < nvc)
-      for (int e = 0; e < len; ++e) {
-        T* x = bptr(e, lane);
-        *x = d.alpha == T(0) ? T(0) : *x * d.alpha;
-      }
-    if (d.alpha == T(0)) return;
-  }
-  const int nb = (len + kTrsmTile - 1) / kTrsmTile;
-  for (int b = 0; b < nb; ++b) {
-    const int e0 = b * kTrsmTile, cnt = min(kTrsmTile, len - e0);
-    // The diagonal block in LDS, then each lane's right-hand side.
-    for (int idx = lane; idx < kTrsmTile * kTrsmTile; idx += 64) {
-      const int i = idx / kTrsmTile, q = idx % kTrsmTile;
-      s.l[i][q] = (q < i && i < cnt) ? lval(e0 + i, e0 + q) : T(0);
-    }
-    if (lane < kTrsmTile)
-      s.inv[lane] = (lane < cnt && !d.unit) ? T(1) / lval(e0 + lane, e0 + lane) : T(1);
-#pragma unroll
-    for (int i = 0; i < kTrsmTile; ++i)
-      s.x[i][lane] = (i < cnt && lane < nvc) ? *bptr(e0 + i, lane) : T(0);
-    WaveSync();
-    // Each lane solves its own column of s.x (rows past cnt stay 0).
-    for (int i = 0; i < cnt; ++i) {
-      T t = s.x[i][lane];
-      for (int q = 0; q < i; ++q) t = __builtin_fma(-s.l[i][q], s.x[q][lane], t);
-      s.x[i][lane] = t * s.inv[i];
-    }
-    if (lane < nvc)
-      for (int i = 0; i < cnt; ++i) *bptr(e0 + i, lane) = s.x[i][lane];
-    WaveSync();
-    // B_t -= L(t, b) X_b for every later block t.
-    for (int t = b + 1; t < nb; ++t) {
-      const int r0 = t * kTrsmTile;
-      Acc acc[kTrsmRhs / 16];
-#pragma unroll
-      for (int jt = 0; jt < kTrsmRhs / 16; ++jt) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = r0 + M::Row(lk, r), col = 16 * jt + li;
-          acc[jt][r] = (row < len && col < nvc) ? *bptr(row, col) : T(0);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kTrsmTile / 4; ++u) {
-        const int i = r0 + li, q = 4 * u + lk;
-        const T a = (i < len && q < cnt) ? -lval(i, e0 + q) : T(0);
-#pragma unroll
-        for (int jt = 0; jt < kTrsmRhs / 16; ++jt)
-          acc[jt] = M::Run(a, s.x[q][16 * jt + li], acc[jt]);
-      }
-#pragma unroll
-      for (int jt = 0; jt < kTrsmRhs / 16; ++jt) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = r0 + M::Row(lk, r), col = 16 * jt + li;
-          if (row < len && col < nvc) *bptr(row, col) = acc[jt][r];
-        }
-      }
-    }
-    WaveSync();  // the next block overwrites the LDS
-  }
-}
-
-template <typename T>
 __global__ void __launch_bounds__(256)
     GroupedTrsmKernel(const TrsmDesc<T>* __restrict__ descs, const WorkItem* __restrict__ items,
                       const int* __restrict__ unit_start, int n_units, int* counters) {
@@ -617,18 +493,18 @@ Plan* MakeTrsmPlan(int count, const hwyblas_trsm_problem* probs, int* info, int*
       return nullptr;
     }
     TrsmDesc<T>& d = descs[i];
+    const bool trans = op == Op::kTrans;
     d.a = static_cast<const T*>(p.a);
     d.b = static_cast<T*>(p.b);
-    d.lda = p.lda;
+    d.rs = trans ? p.lda : 1;
+    d.cs = trans ? 1 : p.lda;
     d.ldb = p.ldb;
     d.m = p.m;
     d.n = p.n;
     d.left = left;
-    d.trans = op == Op::kTrans;
     d.unit = unit;
-    d.lower_stored = lower;
     // op(A) is lower iff (uplo lower) xor trans; L = op(A) (left) or op(A)^T.
-    const bool op_lower = lower != d.trans;
+    const bool op_lower = lower != trans;
     d.lower_solve = left ? op_lower : !op_lower;
     d.alpha = T(p.alpha);
     if (p.m == 0 || p.n == 0) continue;

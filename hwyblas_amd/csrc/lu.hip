@@ -5,8 +5,9 @@
 //
 // getrf follows xGETRF2's recursion (src/lapack/getrf.cc:112-207): factor
 // the left half of the columns, swap its pivots into the right half, solve
-// A12 := L11^-1 A12 (TRSM, FP64), update A22 -= A21 A12 (GEMM), factor A22,
-// swap its pivots into the left half. The recursion stops at panels of at
+// A12 := L11^-1 A12 (TRSM, FP64, its 64 x 64 diagonal blocks applied through
+// their inverses), update A22 -= A21 A12 (GEMM), factor A22, swap its pivots
+// into the left half. The recursion stops at panels of at
 // most kPanel columns, factored column by column as xGETF2 / hwyBLAS's leaf
 // kernel does (src/lapack/factor_kernels.cc:107-138): pivot = first entry of
 // largest magnitude (xIAMAX), row swap, scaling by the reciprocal of the
@@ -14,8 +15,9 @@
 // column step is one launch: every workgroup eliminates the current column
 // from its rows and finds the largest entry of the next column among them;
 // the last workgroup to finish (an atomic counter) reduces the candidates,
-// records the pivot and swaps the two rows of the panel. Halving the
-// columns keeps the GEMMs large, so the trailing updates can run on:
+// records the pivot and swaps the two rows across the whole matrix (xLASWP's
+// interchanges, applied as they are found). Halving the columns keeps the
+// GEMMs large, so the trailing updates can run on:
 //  - update 0: the FP64 MFMA GEMM;
 //  - update 1: the emulated FP64 GEMM (INT8 Ozaki scheme) when k is large
 //    enough for it to pay, else FP64;
@@ -47,7 +49,7 @@ namespace lu {
 namespace {
 
 constexpr int kPanel = 64;          // columns factored column by column
-constexpr int kStepRows = 256;      // rows per workgroup in a column step
+constexpr int kStepRows = 256;      // threads per workgroup in a column step
 constexpr int64_t kEmulatedMinK = 1024;  // below this the emulation does not pay
 
 struct Candidate {
@@ -60,40 +62,11 @@ __device__ __forceinline__ bool Better(double v, int row, double bv, int brow) {
   return v > bv || (v == bv && row < brow);
 }
 
-// One column step of the panel factorization (panel p: m x nb, rows
-// relative to the panel's first row, which is row `row0` of the matrix).
-// For j >= 0: eliminates column j from rows j + 1 .. m - 1 (the pivot row j
-// is already in place). For j + 1 < nb: finds the pivot of column j + 1;
-// the last workgroup records it and swaps rows of the panel.
-__global__ void __launch_bounds__(kStepRows)
-    PanelStepKernel(double* __restrict__ p, int64_t lda, int m, int nb, int j, int row0,
-                    int* __restrict__ ipiv, Candidate* __restrict__ cand,
-                    unsigned* __restrict__ counter, int* __restrict__ info, double sfmin) {
-  const int r = j + 1 + blockIdx.x * kStepRows + threadIdx.x;
-  const bool mine = r < m;
-  double next = 0.0;  // |p(r, j + 1)| after the elimination
-  if (mine) {
-    if (j >= 0) {
-      const double piv = p[j + int64_t(j) * lda];
-      double l = p[r + int64_t(j) * lda];
-      if (piv != 0.0) {
-        l = (fabs(piv) >= sfmin) ? l * (1.0 / piv) : l / piv;
-        p[r + int64_t(j) * lda] = l;
-      }
-      for (int c = j + 1; c < nb; ++c) {
-        double* x = p + r + int64_t(c) * lda;
-        *x = __builtin_fma(-p[j + int64_t(c) * lda], l, *x);
-      }
-    }
-    if (j + 1 < nb) next = fabs(p[r + int64_t(j + 1) * lda]);
-  }
-  // Pivots exist for the first min(m, nb) columns only.
-  if (j + 1 >= min(m, nb)) return;
+constexpr int kStepGroups = 4;                  // column groups per row
+constexpr int kGroupCols = kPanel / kStepGroups;  // 16 columns per thread
+constexpr int kBlockRows = kStepRows / kStepGroups;
 
-  // Block candidate (wave shuffles, then across the waves through LDS).
-  // NaNs never win (xIAMAX skips them), except in the first row (below).
-  double bv = (mine && !isnan(next)) ? next : -1.0;
-  int brow = mine ? r : INT_MAX;
+__device__ __forceinline__ void BlockBest(double& bv, int& brow, double* sv, int* srow) {
   for (int off = 32; off > 0; off >>= 1) {
     const double ov = __shfl_down(bv, off, 64);
     const int orow = __shfl_down(brow, off, 64);
@@ -102,21 +75,84 @@ __global__ void __launch_bounds__(kStepRows)
       brow = orow;
     }
   }
-  __shared__ double sv[kStepRows / 64];
-  __shared__ int srow[kStepRows / 64];
-  __shared__ bool last;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) {
     sv[wave] = bv;
     srow[wave] = brow;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
     for (int w = 1; w < kStepRows / 64; ++w)
       if (Better(sv[w], srow[w], bv, brow)) {
         bv = sv[w];
         brow = srow[w];
       }
+}
+
+// One column step of the panel factorization. The panel p is m x nb (rows
+// relative to its first row, which is row `row0` of the matrix; `full`
+// points at that row in column 0 of the matrix, ncols wide). A workgroup
+// owns 64 rows; its four waves split the panel's columns (16 each).
+//  - j >= 0: eliminates column j from rows j + 1 .. m - 1 (the pivot row j is
+//    already in place): l = p(r, j) / pivot, p(r, c) -= p(j, c) l.
+//  - j + 1 < min(m, nb): finds the pivot of column j + 1. The last workgroup
+//    to finish (atomic counter) reduces the candidates, records the pivot,
+//    and swaps the two rows of the panel. PanelSwapKernel then applies the
+//    panel's interchanges to every other column of the matrix (xGETRF2
+//    applies them to columns that do not change in between, so applying
+//    them once the panel is done gives the same matrix).
+__global__ void __launch_bounds__(kStepRows)
+    PanelStepKernel(double* __restrict__ p, int64_t lda, int m, int nb, int j,
+                    double* __restrict__ full, int ncols, int row0, int* __restrict__ ipiv,
+                    Candidate* __restrict__ cand, unsigned* __restrict__ counter,
+                    int* __restrict__ info, double sfmin) {
+  const int rl = threadIdx.x % kBlockRows, cg = threadIdx.x / kBlockRows;
+  const int r = j + 1 + blockIdx.x * kBlockRows + rl;
+  const bool mine = r < m;
+  const int c0 = cg * kGroupCols;
+  double next = -1.0;  // |p(r, j + 1)| after the elimination, if this thread holds it
+  if (j >= 0) {
+    // Every load before any store (the multiplier column is read by the
+    // four column groups and written by the first).
+    const double piv = p[j + int64_t(j) * lda];
+    const double lraw = mine ? p[r + int64_t(j) * lda] : 0.0;
+    double u[kGroupCols], v[kGroupCols];
+#pragma unroll
+    for (int t = 0; t < kGroupCols; ++t) {
+      const int c = c0 + t;
+      const bool use = c > j && c < nb;
+      u[t] = use ? p[j + int64_t(c) * lda] : 0.0;
+      v[t] = (use && mine) ? p[r + int64_t(c) * lda] : 0.0;
+    }
+    __syncthreads();
+    double l = lraw;
+    if (piv != 0.0) l = (fabs(piv) >= sfmin) ? lraw * (1.0 / piv) : lraw / piv;
+    if (mine) {
+      if (cg == 0 && piv != 0.0) p[r + int64_t(j) * lda] = l;
+#pragma unroll
+      for (int t = 0; t < kGroupCols; ++t) {
+        const int c = c0 + t;
+        if (c > j && c < nb) {
+          v[t] = __builtin_fma(-u[t], l, v[t]);
+          p[r + int64_t(c) * lda] = v[t];
+          if (c == j + 1) next = isnan(v[t]) ? -1.0 : fabs(v[t]);
+        }
+      }
+    }
+  } else if (mine && j + 1 >= c0 && j + 1 < c0 + kGroupCols) {
+    const double v0 = p[r + int64_t(j + 1) * lda];
+    next = isnan(v0) ? -1.0 : fabs(v0);
+  }
+  // Pivots exist for the first min(m, nb) columns only.
+  if (j + 1 >= min(m, nb)) return;
+
+  __shared__ double sv[kStepRows / 64];
+  __shared__ int srow[kStepRows / 64];
+  __shared__ bool last;
+  double bv = next;
+  int brow = next >= 0.0 ? r : INT_MAX;
+  BlockBest(bv, brow, sv, srow);
+  if (threadIdx.x == 0) {
     cand[blockIdx.x] = {bv, brow};
     __threadfence();
     last = atomicAdd(counter, 1u) == gridDim.x - 1;
@@ -124,19 +160,21 @@ __global__ void __launch_bounds__(kStepRows)
   __syncthreads();
   if (!last) return;
 
-  // The last workgroup: the pivot of column j + 1 and the row swap.
+  // The last workgroup: the best candidate, the pivot and the row swap.
+  __threadfence();
+  double gv = -1.0;
+  int grow = INT_MAX;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += kStepRows) {
+    const Candidate c = cand[b];
+    if (Better(c.v, c.row, gv, grow)) {
+      gv = c.v;
+      grow = c.row;
+    }
+  }
+  __syncthreads();
+  BlockBest(gv, grow, sv, srow);
   __shared__ int piv_row;
   if (threadIdx.x == 0) {
-    __threadfence();
-    double gv = -1.0;
-    int grow = INT_MAX;
-    for (unsigned b = 0; b < gridDim.x; ++b) {
-      const Candidate c = cand[b];
-      if (Better(c.v, c.row, gv, grow)) {
-        gv = c.v;
-        grow = c.row;
-      }
-    }
     // xIAMAX picks the first row when it holds a NaN or the column is zero.
     if (isnan(p[(j + 1) + int64_t(j + 1) * lda]) || !(gv > 0.0)) {
       grow = j + 1;
@@ -148,45 +186,98 @@ __global__ void __launch_bounds__(kStepRows)
   }
   __syncthreads();
   const int pr = piv_row;
-  if (pr != j + 1) {
-    for (int c = threadIdx.x; c < nb; c += kStepRows) {
-      double* x = p + (j + 1) + int64_t(c) * lda;
-      double* y = p + pr + int64_t(c) * lda;
-      const double t = *x;
-      *x = *y;
-      *y = t;
-    }
+  if (pr != j + 1 && threadIdx.x < nb) {
+    double* x = p + (j + 1) + int64_t(threadIdx.x) * lda;
+    double* y = p + pr + int64_t(threadIdx.x) * lda;
+    const double t = *x;
+    *x = *y;
+    *y = t;
   }
 }
 
-// Row interchanges (xLASWP, src/lapack/getrf.cc:28-81) on ncols columns:
-// rows k1 .. k2 - 1 in order (forward) or in reverse, row i with row
-// ipiv[i] - 1 + shift. One thread per column; the swaps of a column are
-// sequential.
-__global__ void LaswpKernel(double* __restrict__ a, int64_t lda, int ncols, int k1, int k2,
-                            const int* __restrict__ ipiv, int shift, int forward) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncols) return;
-  double* col = a + int64_t(c) * lda;
-  if (forward) {
-    for (int i = k1; i < k2; ++i) {
-      const int q = ipiv[i] - 1 + shift;
-      if (q != i) {
-        const double t = col[i];
-        col[i] = col[q];
-        col[q] = t;
-      }
-    }
-  } else {
-    for (int i = k2 - 1; i >= k1; --i) {
-      const int q = ipiv[i] - 1 + shift;
-      if (q != i) {
-        const double t = col[i];
-        col[i] = col[q];
-        col[q] = t;
-      }
-    }
+constexpr int kMaxAffected = 2 * kPanel;
+
+// Applies the row interchanges of one panel (count <= kPanel pivots, rows
+// relative to the panel's first row) to every column of the matrix outside
+// the panel's columns [pc0, pc0 + nb). The interchanges are composed into
+// "row L[x] receives old row S[x]" over the affected rows (the panel's top
+// rows and their partners); each thread then moves its column's affected
+// entries through LDS, all loads before all stores.
+__global__ void __launch_bounds__(64)
+    PanelSwapKernel(double* __restrict__ full, int64_t lda, int ncols, int pc0, int nb,
+                    const int* __restrict__ ipiv, int count) {
+  __shared__ int L[kMaxAffected], S[kMaxAffected];
+  __shared__ int naff;
+  __shared__ double buf[kMaxAffected][64 + 1];
+  const int lane = threadIdx.x;
+  if (lane < count) {
+    L[lane] = lane;
+    S[lane] = lane;
   }
+  // The whole wave composes the interchanges: lane t holds the t-th partner
+  // row below the panel's top rows (a ballot finds a partner already seen).
+  int extra_row = -1, extra_src = -1, na = count;
+  __syncthreads();
+  for (int i = 0; i < count; ++i) {
+    const int q = ipiv[i] - 1;
+    if (q < count) {
+      if (lane == 0) {
+        const int tmp = S[i];
+        S[i] = S[q];
+        S[q] = tmp;
+      }
+    } else {
+      const unsigned long long hit = __ballot(extra_row == q);
+      const int t = hit ? __ffsll(static_cast<long long>(hit)) - 1 : na - count;
+      if (!hit) {
+        if (lane == t) {
+          extra_row = q;
+          extra_src = q;
+        }
+        ++na;
+      }
+      // swap(S[i], source of row q)
+      const int si = __shfl(lane == 0 ? S[i] : 0, 0, 64);
+      const int sq = __shfl(extra_src, t, 64);
+      if (lane == t) extra_src = si;
+      if (lane == 0) S[i] = sq;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane < na - count) {
+    L[count + lane] = extra_row;
+    S[count + lane] = extra_src;
+  }
+  if (lane == 0) naff = na;
+  __syncthreads();
+  int c = blockIdx.x * 64 + lane;
+  if (c >= pc0) c += nb;  // skip the panel's own columns
+  if (c >= ncols) return;
+  const int total = naff;
+  double* col = full + int64_t(c) * lda;
+  for (int x = 0; x < total; ++x) buf[x][lane] = col[S[x]];
+  for (int x = 0; x < total; ++x) col[L[x]] = buf[x][lane];
+}
+
+// The row interchanges of xLASWP (src/lapack/getrf.cc:28-81) over rows
+// [0, n), composed into one permutation: row i of the result is row src[i]
+// of the input (forward order; the reverse order is the inverse). The
+// composition is cheap on the host; moving the rows is a parallel gather
+// through a scratch copy.
+__global__ void GatherRows(const double* __restrict__ b, int64_t ldb, int n, int ncols,
+                           const int* __restrict__ src, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = src[i];
+  for (int c = 0; c < ncols; ++c) out[i + int64_t(c) * n] = b[r + int64_t(c) * ldb];
+}
+
+__global__ void CopyBack(const double* __restrict__ in, int n, int ncols, double* __restrict__ b,
+                         int64_t ldb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < ncols; ++c) b[i + int64_t(c) * ldb] = in[i + int64_t(c) * n];
 }
 
 __global__ void AddToPivots(int* __restrict__ ipiv, int count, int add) {
@@ -230,7 +321,33 @@ unsigned GridFor(int64_t n, int threads = 256) {
   return unsigned(std::max<int64_t>(1, std::min<int64_t>((n + threads - 1) / threads, 1 << 16)));
 }
 
+// inv (64 x 64, dense, zeros above the diagonal) = the inverse of the r x r
+// unit lower triangle of a (r <= 64). Thread j computes column j by forward
+// substitution on e_j.
+__global__ void __launch_bounds__(64)
+    InvertUnitLowerKernel(const double* __restrict__ a, int64_t lda, int r,
+                          double* __restrict__ inv) {
+  __shared__ double l[kPanel][kPanel + 1];
+  const int j = threadIdx.x;
+  for (int i = 0; i < kPanel; ++i) l[i][j] = (j < i && i < r) ? a[i + int64_t(j) * lda] : 0.0;
+  __syncthreads();
+  double x[kPanel];
+#pragma unroll
+  for (int i = 0; i < kPanel; ++i) {
+    double v = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+    for (int q = 0; q < i; ++q) v = __builtin_fma(-l[i][q], x[q], v);
+    x[i] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < kPanel; ++i) inv[i + j * kPanel] = (i < r && j < r) ? x[i] : 0.0;
+}
+
 struct Work {
+  double* base = nullptr;  // the whole matrix
+  int ncols = 0;
+  double* linv = nullptr;  // inverses of the panels' unit lower triangles
+  double* temp = nullptr;  // kPanel x ncols
   Candidate* cand = nullptr;
   unsigned* counter = nullptr;
   int* info = nullptr;
@@ -238,25 +355,52 @@ struct Work {
   size_t f32_elems = 0;
 };
 
-void Laswp(double* a, int64_t lda, int ncols, int k1, int k2, const int* ipiv, int shift,
-           bool forward, hipStream_t s) {
-  if (ncols <= 0 || k2 <= k1) return;
-  hipLaunchKernelGGL(LaswpKernel, dim3(GridFor(ncols, 64)), dim3(64), 0, s, a, lda, ncols, k1, k2,
-                     ipiv, shift, forward ? 1 : 0);
-}
-
+// Panel a (m x nb) whose first row is row `row0` of the matrix.
 void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, hipStream_t s) {
   const double sfmin = std::numeric_limits<double>::min();
   const int mn = std::min(m, nb);
+  double* full = w.base + row0;
   // Step j eliminates column j (j >= 0) and pivots column j + 1 (if any).
   for (int j = -1; j < mn; ++j) {
     const int rows = m - (j + 1);
     if (j + 1 >= mn && rows <= 0) break;
-    const unsigned blocks = unsigned(std::max(1, (rows + kStepRows - 1) / kStepRows));
+    const unsigned blocks = unsigned(std::max(1, (rows + kBlockRows - 1) / kBlockRows));
     hipLaunchKernelGGL(PanelStepKernel, dim3(blocks), dim3(kStepRows), 0, s, a, lda, m, nb, j,
-                       row0, ipiv, w.cand, w.counter, w.info, sfmin);
+                       full, w.ncols, row0, ipiv, w.cand, w.counter, w.info, sfmin);
   }
+  // The panel's interchanges on every other column of the matrix.
+  const int pc0 = int((a - full) / lda);  // the panel's first column
+  const int others = w.ncols - nb;
+  if (others > 0)
+    hipLaunchKernelGGL(PanelSwapKernel, dim3((others + 63) / 64), dim3(64), 0, s, full, lda,
+                       w.ncols, pc0, nb, ipiv, mn);
+  // The inverse of the panel's unit lower triangle (final from here on: later
+  // interchanges only move rows below it), for the TRSMs with it.
+  hipLaunchKernelGGL(InvertUnitLowerKernel, dim3(1), dim3(kPanel), 0, s, a, lda, mn,
+                     w.linv + int64_t(row0 / kPanel) * kPanel * kPanel);
   HWY_HIP_CHECK(hipGetLastError());
+}
+
+// B (n1 x n2, ldb) := L^-1 B for the unit lower triangle L (n1 x n1) of the
+// factorization starting at matrix row g0 (a multiple of kPanel): the
+// recursion of the TRSM, with the diagonal blocks applied as GEMMs with the
+// inverses the panels left (rounding as X = inv(L_jj) B_j, the usual form of
+// a GPU LU's triangular solves).
+void TrsmUnitLower(const double* l, int64_t lda, int n1, double* b, int64_t ldb, int n2, int g0,
+                   Work& w, hipStream_t s) {
+  if (n1 <= 0 || n2 <= 0) return;
+  if (n1 <= kPanel) {
+    const double* inv = w.linv + int64_t(g0 / kPanel) * kPanel * kPanel;
+    GemmDevice<double>(false, false, n1, n2, n1, 1.0, inv, kPanel, b, ldb, 0.0, w.temp, n1, s);
+    HWY_HIP_CHECK(hipMemcpy2DAsync(b, size_t(ldb) * 8, w.temp, size_t(n1) * 8, size_t(n1) * 8,
+                                   n2, hipMemcpyDeviceToDevice, s));
+    return;
+  }
+  const int r1 = std::max(kPanel, (n1 / 2) / kPanel * kPanel);
+  TrsmUnitLower(l, lda, r1, b, ldb, n2, g0, w, s);
+  GemmDevice<double>(false, false, n1 - r1, n2, r1, -1.0, l + r1, lda, b, ldb, 1.0, b + r1, ldb,
+                     s);
+  TrsmUnitLower(l + r1 + int64_t(r1) * lda, lda, n1 - r1, b + r1, ldb, n2, g0 + r1, w, s);
 }
 
 void Update(int64_t m, int64_t n, int64_t k, const double* a21, const double* a12, double* a22,
@@ -295,20 +439,23 @@ void Recurse(double* a, int64_t lda, int m, int n, int row0, int* ipiv, int upda
              hipStream_t s) {
   const int mn = std::min(m, n);
   if (n <= kPanel || mn <= kPanel) {
-    Panel(a, lda, m, n, row0, ipiv, w, s);
+    // A panel of at most kPanel columns; a wide block (m <= kPanel < n)
+    // then has U12 = L11^-1 A12 left for its remaining columns.
+    const int nbp = std::min(n, kPanel);
+    Panel(a, lda, m, nbp, row0, ipiv, w, s);
+    if (n > nbp) TrsmUnitLower(a, lda, m, a + int64_t(nbp) * lda, lda, n - nbp, row0, w, s);
     return;
   }
   int n1 = std::max(kPanel, (mn / 2) / kPanel * kPanel);
   const int n2 = n - n1;
   Recurse(a, lda, m, n1, row0, ipiv, update, w, s);
-  Laswp(a + int64_t(n1) * lda, lda, n2, 0, n1, ipiv, 0, true, s);
-  TrsmDevice<double>(true, true, false, true, n1, n2, a, lda, a + int64_t(n1) * lda, lda, s);
+  // (The left half's row interchanges are already applied to every column.)
+  TrsmUnitLower(a, lda, n1, a + int64_t(n1) * lda, lda, n2, row0, w, s);
   Update(m - n1, n2, n1, a + n1, a + int64_t(n1) * lda, a + n1 + int64_t(n1) * lda, lda, update,
          w, s);
   Recurse(a + n1 + int64_t(n1) * lda, lda, m - n1, n2, row0 + n1, ipiv + n1, update, w, s);
   const int cnt = mn - n1;
   hipLaunchKernelGGL(AddToPivots, dim3(GridFor(cnt)), dim3(256), 0, s, ipiv + n1, cnt, n1);
-  Laswp(a, lda, n1, n1, mn, ipiv, 0, true, s);
 }
 
 }  // namespace
@@ -317,8 +464,13 @@ void Recurse(double* a, int64_t lda, int m, int n, int row0, int* ipiv, int upda
 int Getrf(int m, int n, double* a, int64_t lda, int* ipiv, int update, hipStream_t s) {
   if (m == 0 || n == 0) return 0;
   Work w;
-  const int max_blocks = (m + kStepRows - 1) / kStepRows + 1;
+  w.base = a;
+  w.ncols = n;
+  const int max_blocks = (m + kBlockRows - 1) / kBlockRows + 1;
   HWY_HIP_CHECK(hipMalloc(&w.cand, size_t(max_blocks) * sizeof(Candidate)));
+  const int panels = (std::min(m, n) + kPanel - 1) / kPanel;
+  HWY_HIP_CHECK(hipMalloc(&w.linv, size_t(panels) * kPanel * kPanel * sizeof(double)));
+  HWY_HIP_CHECK(hipMalloc(&w.temp, size_t(kPanel) * n * sizeof(double)));
   HWY_HIP_CHECK(hipMalloc(&w.counter, 256));
   HWY_HIP_CHECK(hipMalloc(&w.info, 256));
   hipLaunchKernelGGL(FillUnsigned, dim3(1), dim3(64), 0, s, w.counter, 1, 0u);
@@ -328,24 +480,80 @@ int Getrf(int m, int n, double* a, int64_t lda, int* ipiv, int update, hipStream
   HWY_HIP_CHECK(hipMemcpyAsync(&info, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
   HWY_HIP_CHECK(hipStreamSynchronize(s));
   for (void* p : {static_cast<void*>(w.cand), static_cast<void*>(w.counter),
-                  static_cast<void*>(w.info), static_cast<void*>(w.f32)})
+                  static_cast<void*>(w.info), static_cast<void*>(w.f32),
+                  static_cast<void*>(w.linv), static_cast<void*>(w.temp)})
     if (p) HWY_HIP_CHECK(hipFree(p));
   return info == INT_MAX ? 0 : info;
 }
 
-// xGETRS (src/lapack/getrf.cc:287-339) with device ipiv.
-void Getrs(bool trans, int n, int nrhs, const double* a, int64_t lda, const int* ipiv, double* b,
-           int64_t ldb, hipStream_t s) {
+// src[i] of the composed row interchanges ipiv[0..n) (1-based, forward).
+std::vector<int> PivotPermutation(int n, const int* ipiv_host) {
+  std::vector<int> src(n);
+  for (int i = 0; i < n; ++i) src[i] = i;
+  for (int i = 0; i < n; ++i) std::swap(src[i], src[ipiv_host[i] - 1]);
+  return src;
+}
+
+// Applies P^T (forward interchanges) or P (reverse) to the n x ncols B,
+// with src the forward permutation (device) and scratch n * ncols doubles.
+void Permute(bool forward, int n, int ncols, double* b, int64_t ldb, const int* src_dev,
+             const int* inv_dev, double* scratch, hipStream_t s) {
+  hipLaunchKernelGGL(GatherRows, dim3(GridFor(n)), dim3(256), 0, s, b, ldb, n, ncols,
+                     forward ? src_dev : inv_dev, scratch);
+  hipLaunchKernelGGL(CopyBack, dim3(GridFor(n)), dim3(256), 0, s, scratch, n, ncols, b, ldb);
+  HWY_HIP_CHECK(hipGetLastError());
+}
+
+// The row permutation of a factorization, on the device.
+struct Perm {
+  int* src = nullptr;
+  int* inv = nullptr;
+  double* scratch = nullptr;
+  int n = 0, ncols = 0;
+
+  Perm(int n_, int ncols_, const int* ipiv_dev, hipStream_t s) : n(n_), ncols(ncols_) {
+    std::vector<int> piv(n);
+    HWY_HIP_CHECK(hipMemcpyAsync(piv.data(), ipiv_dev, size_t(n) * sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+    HWY_HIP_CHECK(hipStreamSynchronize(s));
+    const std::vector<int> src_h = PivotPermutation(n, piv.data());
+    std::vector<int> inv_h(n);
+    for (int i = 0; i < n; ++i) inv_h[src_h[i]] = i;
+    HWY_HIP_CHECK(hipMalloc(&src, size_t(n) * sizeof(int)));
+    HWY_HIP_CHECK(hipMalloc(&inv, size_t(n) * sizeof(int)));
+    HWY_HIP_CHECK(hipMalloc(&scratch, size_t(n) * std::max(ncols, 1) * sizeof(double)));
+    HWY_HIP_CHECK(hipMemcpy(src, src_h.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice));
+    HWY_HIP_CHECK(hipMemcpy(inv, inv_h.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice));
+  }
+  ~Perm() {
+    (void)hipFree(src);
+    (void)hipFree(inv);
+    (void)hipFree(scratch);
+  }
+};
+
+// xGETRS (src/lapack/getrf.cc:287-339) with the permutation of ipiv.
+void GetrsPerm(bool trans, int n, int nrhs, const double* a, int64_t lda, const Perm& perm,
+               double* b, int64_t ldb, hipStream_t s) {
   if (n == 0 || nrhs == 0) return;
   if (!trans) {
-    Laswp(b, ldb, nrhs, 0, n, ipiv, 0, true, s);
+    Permute(true, n, nrhs, b, ldb, perm.src, perm.inv, perm.scratch, s);
     TrsmDevice<double>(true, true, false, true, n, nrhs, a, lda, b, ldb, s);
     TrsmDevice<double>(true, false, false, false, n, nrhs, a, lda, b, ldb, s);
   } else {
     TrsmDevice<double>(true, false, true, false, n, nrhs, a, lda, b, ldb, s);
     TrsmDevice<double>(true, true, true, true, n, nrhs, a, lda, b, ldb, s);
-    Laswp(b, ldb, nrhs, 0, n, ipiv, 0, false, s);
+    Permute(false, n, nrhs, b, ldb, perm.src, perm.inv, perm.scratch, s);
   }
+}
+
+// Synchronizes (the permutation is composed on the host).
+void Getrs(bool trans, int n, int nrhs, const double* a, int64_t lda, const int* ipiv, double* b,
+           int64_t ldb, hipStream_t s) {
+  if (n == 0 || nrhs == 0) return;
+  Perm perm(n, nrhs, ipiv, s);
+  GetrsPerm(trans, n, nrhs, a, lda, perm, b, ldb, s);
+  HWY_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------------------
@@ -523,7 +731,8 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
   if (info > 0) return info;
   HWY_HIP_CHECK(hipMemcpy2DAsync(x, size_t(ldx) * 8, b, size_t(ldb) * 8, size_t(n) * 8, nrhs,
                                  hipMemcpyDeviceToDevice, s));
-  lu::Getrs(false, n, nrhs, af, ldaf, ipiv, x, ldx, s);
+  const lu::Perm perm(n, nrhs, ipiv, s);
+  lu::GetrsPerm(false, n, nrhs, af, ldaf, perm, x, ldx, s);
 
   const int nchunks = (n + lu::kResChunk - 1) / lu::kResChunk;
   const int rblocks = (n + 255) / 256;
@@ -573,7 +782,7 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
     if (!any) break;
     HWY_HIP_CHECK(hipMemcpyAsync(active_d, active.data(), nrhs * sizeof(int),
                                  hipMemcpyHostToDevice, s));
-    lu::Getrs(false, n, nrhs, af, ldaf, ipiv, r, n, s);
+    lu::GetrsPerm(false, n, nrhs, af, ldaf, perm, r, n, s);
     hipLaunchKernelGGL(lu::AddMaskedKernel, dim3(rblocks), dim3(256), 0, s, x, int64_t(ldx), r,
                        int64_t(n), n, nrhs, active_d);
     HWY_HIP_CHECK(hipGetLastError());

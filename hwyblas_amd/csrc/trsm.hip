@@ -4,26 +4,16 @@
 // multiples of the base size until a diagonal block has at most kNB rows;
 // every off-diagonal update B2 -= T21 X1 (and its upper/right variants) is a
 // GEMM with alpha = -1, beta = 1 on the MFMA kernel, so almost all flops run
-// there. Diagonal blocks are solved by DiagSolveKernel:
-//  - the block of op(A) is loaded into LDS as a dense kNB x kNB matrix L in
-//    "solve orientation" (L = T for the left side, T^T for the right side;
-//    only the referenced triangle is read, the unit diagonal is never read);
-//  - each thread owns one right-hand side (a column of B for the left side,
-//    a row of B for the right side) in registers, staged through LDS so that
-//    global reads and writes are coalesced;
-//  - x_j = (b_j - sum_q x_q t(q, j)) * (1 / t(j, j)), the sum fused in solve
-//    order, as hwyBLAS's base kernel computes it (src/trsm-inl.h:107-174,
-//    reciprocal at :69). The update is done column by column (after x_q is
-//    final, every later x_j gets its fma), which performs the same fma
-//    sequence on each accumulator as the dot form but with independent
-//    chains across j.
-// Indices are padded to kNB so that register arrays are indexed by
-// compile-time constants: an ascending solve keeps the block at [0, r), a
-// descending one at [kNB - r, kNB); padding rows are zero with unit diagonal
-// and are never stored.
+// there. Diagonal blocks (order <= kNB) are solved by the wave-level solve
+// of trsm_chunk.h: one wave per 64 right-hand sides, blocks of 16 rows
+// solved in LDS with x_j = (b_j - sum_q x_q t(q, j)) * (1 / t(j, j)) (the
+// dot form of hwyBLAS's base kernel, src/trsm-inl.h:107-174, reciprocal at
+// :69), the rows below updated on the MFMA units. Only the referenced
+// triangle is read; a unit diagonal is never read.
 #include <algorithm>
 
 #include "common.h"
+#include "trsm_chunk.h"
 
 namespace hwy_amd {
 namespace {
@@ -45,103 +35,37 @@ TriView<T> SubView(const TriView<T>& t, int64_t i0, int64_t j0) {
   return s;
 }
 
-// Solves L x = b for NT right-hand sides per block: columns of B (LEFT,
-// L = T) or rows of B (right side, L = T^T). ASC: ascending solve order
-// (L lower). Both sides stage their right-hand sides through LDS, so global
-// reads and writes are coalesced and every register index is a constant.
-template <typename T, bool LEFT, bool ASC>
-__global__ void __launch_bounds__(64)
-    DiagSolveKernel(TriView<T> t, int r, T* b, int64_t ldb, int64_t other) {
-  constexpr int NT = 64;
-  __shared__ T Ls[kNB][kNB + 1];
-  __shared__ T inv[kNB];
-  __shared__ T Bs[NT][kNB + 1];
-  const int tid = threadIdx.x;
-  const int off = ASC ? 0 : kNB - r;  // padded index = real index + off
-
-  // L(i, q) = op(A)(i, q) (left) or op(A)(q, i) (right), strictly inside the
-  // referenced triangle; everything else is 0.
-  for (int idx = tid; idx < kNB * kNB; idx += NT) {
-    const int P = idx % kNB, Q = idx / kNB;
-    const int i = P - off, q = Q - off;
-    T v = T(0);
-    const bool in = (i >= 0 && i < r && q >= 0 && q < r) && (ASC ? q < i : q > i);
-    if (in) v = LEFT ? t.data[i * t.rs + q * t.cs] : t.data[q * t.rs + i * t.cs];
-    Ls[P][Q] = v;
-  }
-  for (int P = tid; P < kNB; P += NT) {
-    const int i = P - off;
-    T d = T(1);
-    if (!t.unit && i >= 0 && i < r) d = T(1) / t.data[i * t.rs + i * t.cs];
-    inv[P] = d;
-  }
-
-  // Right-hand side c of this block, element i: B(i, first + c) (left) or
-  // B(first + c, i) (right). Consecutive threads read consecutive addresses.
-  const int64_t first = int64_t(blockIdx.x) * NT;
-  const int cols = int(std::min<int64_t>(NT, other - first));
-  for (int idx = tid; idx < r * cols; idx += NT) {
-    int i, c;
-    if (LEFT) { i = idx % r; c = idx / r; }
-    else { c = idx % cols; i = idx / cols; }
-    const int64_t g = LEFT ? i + (first + c) * ldb : (first + c) + int64_t(i) * ldb;
-    Bs[c][i + off] = b[g];
-  }
-  __syncthreads();
-
-  T x[kNB];
-#pragma unroll
-  for (int P = 0; P < kNB; ++P) {
-    const int i = P - off;
-    x[P] = (i >= 0 && i < r && tid < cols) ? Bs[tid][P] : T(0);
-  }
-
-  const bool unit = t.unit;
-#pragma unroll
-  for (int s = 0; s < kNB; ++s) {
-    const int P = ASC ? s : kNB - 1 - s;
-    if (!unit) x[P] = x[P] * inv[P];
-#pragma unroll
-    for (int s2 = s + 1; s2 < kNB; ++s2) {
-      const int R = ASC ? s2 : kNB - 1 - s2;
-      x[R] = __builtin_fma(-Ls[R][P], x[P], x[R]);
-    }
-  }
-
-  __syncthreads();  // every thread has read its right-hand side from Bs
-#pragma unroll
-  for (int P = 0; P < kNB; ++P) Bs[tid][P] = x[P];
-  __syncthreads();
-  for (int idx = tid; idx < r * cols; idx += NT) {
-    int i, c;
-    if (LEFT) { i = idx % r; c = idx / r; }
-    else { c = idx % cols; i = idx / cols; }
-    const int64_t g = LEFT ? i + (first + c) * ldb : (first + c) + int64_t(i) * ldb;
-    b[g] = Bs[c][i + off];
-  }
+// Solves a diagonal block (order r <= kNB) for `other` right-hand sides:
+// one wave per 64 of them, four waves per workgroup.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    DiagChunkKernel(trsm_chunk::TrsmDesc<T> d, int nchunks) {
+  __shared__ trsm_chunk::TrsmWaveLds<T> lds[4];
+  const int wave = threadIdx.x >> 6;
+  const int chunk = blockIdx.x * 4 + wave;
+  if (chunk >= nchunks) return;
+  trsm_chunk::TrsmChunk<T>(d, chunk * 64, lds[wave]);
 }
 
 template <typename T>
 void DiagSolve(bool left, const TriView<T>& t, int r, T* b, int64_t ldb,
                int64_t other, hipStream_t stream) {
+  trsm_chunk::TrsmDesc<T> d;
+  d.a = t.data;
+  d.rs = t.rs;
+  d.cs = t.cs;
+  d.b = b;
+  d.ldb = ldb;
+  d.m = left ? r : int(other);
+  d.n = left ? int(other) : r;
+  d.left = left;
   // Solve orientation: L = T (left) or T^T (right); lower L solves ascending.
-  const bool asc = left ? t.lower : !t.lower;
-  const unsigned blocks = unsigned((other + 63) / 64);
-  if (left) {
-    if (asc)
-      hipLaunchKernelGGL((DiagSolveKernel<T, true, true>), dim3(blocks), dim3(64), 0, stream,
-                         t, r, b, ldb, other);
-    else
-      hipLaunchKernelGGL((DiagSolveKernel<T, true, false>), dim3(blocks), dim3(64), 0, stream,
-                         t, r, b, ldb, other);
-  } else {
-    if (asc)
-      hipLaunchKernelGGL((DiagSolveKernel<T, false, true>), dim3(blocks), dim3(64), 0, stream,
-                         t, r, b, ldb, other);
-    else
-      hipLaunchKernelGGL((DiagSolveKernel<T, false, false>), dim3(blocks), dim3(64), 0, stream,
-                         t, r, b, ldb, other);
-  }
+  d.lower_solve = left ? t.lower : !t.lower;
+  d.unit = t.unit;
+  d.alpha = T(1);
+  const int nchunks = int((other + 63) / 64);
+  hipLaunchKernelGGL(DiagChunkKernel<T>, dim3(unsigned((nchunks + 3) / 4)), dim3(256), 0, stream,
+                     d, nchunks);
   HWY_HIP_CHECK(hipGetLastError());
 }
 

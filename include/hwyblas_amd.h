@@ -316,7 +316,8 @@ HWYBLAS_AMD_API void hwyblas_grouped_set_model(void* model);
 HWYBLAS_AMD_API int hwyblas_dgetrf_dev(int m, int n, double* a, int lda, int* ipiv,
                                        int update, void* stream);
 /* op(A) X = B with the factorization (src/lapack/getrf.cc:287-339); X
- * overwrites B. Returns 0 or -i (reported as DGETRS). Asynchronous. */
+ * overwrites B. Returns 0 or -i (reported as DGETRS). Synchronizes (the row
+ * interchanges are composed into one permutation on the host). */
 HWYBLAS_AMD_API int hwyblas_dgetrs_dev(char trans, int n, int nrhs, const double* a, int lda,
                                        const int* ipiv, double* b, int ldb, void* stream);
 /* A X = B for nrhs <= 16: af := LU of A (dgetrf with `update`), X := dgetrs,
