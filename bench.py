@@ -123,7 +123,7 @@ class DeviceBatch:
 
 
 def batched_gemm_leg(H, oracle, dtype, count, steps, warmup, stream, model=None, configs=None,
-                     check=16, seed=7):
+                     check=16, seed=7, with_loop=True):
     """`count` uneven GEMMs, m, n, k log-uniform in [8, 512], random trans,
     uniform[-1, 1], alpha = 1, beta = 0: one grouped launch replayed from a
     hipGraph, against a loop of single-call hwyblas_?gemm_dev launches."""
@@ -182,8 +182,10 @@ def batched_gemm_leg(H, oracle, dtype, count, steps, warmup, stream, model=None,
     def loop_step():
         for a in args:
             f(*a)
-    l_ms, _ = time_steps(H, stream, loop_step, max(1, steps // 4), 1)
-    lg_ms = time_loop_graph(H, stream, loop_step, max(1, steps // 2))
+    l_ms = lg_ms = float("nan")
+    if with_loop:
+        l_ms, _ = time_steps(H, stream, loop_step, max(1, steps // 4), 1)
+        lg_ms = time_loop_graph(H, stream, loop_step, max(1, steps // 2))
 
     # Grouped result of the last replay, then checked problem by problem.
     plan.launch(stream, graph=True)
@@ -219,7 +221,7 @@ def batched_gemm_leg(H, oracle, dtype, count, steps, warmup, stream, model=None,
     return out
 
 
-def batched_trsm_leg(H, oracle, count, steps, warmup, stream, check=16, seed=11):
+def batched_trsm_leg(H, oracle, count, steps, warmup, stream, check=16, seed=11, with_loop=True):
     """`count` DTRSMs L/L/N/N, m log-uniform in [8, 256], n in [1, 64]; A
     lower triangular with uniform[-1, 1] off-diagonal entries and diagonal
     m + uniform[0, 1]: one grouped launch replayed from a hipGraph, against a
@@ -258,8 +260,10 @@ def batched_trsm_leg(H, oracle, count, steps, warmup, stream, check=16, seed=11)
     def loop_step():
         for a in args:
             f(*a)
-    l_ms, _ = time_steps(H, stream, loop_step, max(1, steps // 4), 1)
-    lg_ms = time_loop_graph(H, stream, loop_step, max(1, steps // 2))
+    l_ms = lg_ms = float("nan")
+    if with_loop:
+        l_ms, _ = time_steps(H, stream, loop_step, max(1, steps // 4), 1)
+        lg_ms = time_loop_graph(H, stream, loop_step, max(1, steps // 2))
     # One solve from the original right-hand sides, checked by residual.
     H.lib().hwyblas_amd_memcpy(batch.arenas[1].ptr, B0.ptr, nb, 3)
     H.synchronize()
@@ -467,6 +471,125 @@ def distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, samples, nmod=0, st
     return out
 
 
+def householder_blocks(n, seed, b=128):
+    """Stewart's (1980) random orthogonal Q = H_0 H_1 ... H_{n-2}: H_i reflects
+    a Gaussian vector x (rows i..n-1) to -sign(x_0) |x| e_i; this is the Q of
+    a Householder QR of a Gaussian matrix (equal in distribution). Yields
+    (row0, V, T) per block of b reflectors in compact WY form, the product of
+    the block's reflectors being I - V T V^T (V: rows row0..n-1)."""
+    rng = np.random.default_rng(seed)
+    for j0 in range(0, n - 1, b):
+        nb = min(b, n - 1 - j0)
+        rows = n - j0
+        V = np.zeros((rows, nb), order="F")
+        tau = np.zeros(nb)
+        for t in range(nb):
+            x = rng.standard_normal(rows - t)
+            v = x.copy()
+            v[0] += np.copysign(np.linalg.norm(x), x[0])
+            V[t:, t] = v
+            tau[t] = 2.0 / (v @ v)
+        G = V.T @ V
+        T = np.zeros((nb, nb), order="F")
+        for t in range(nb):
+            T[t, t] = tau[t]
+            if t:
+                T[:t, t] = -tau[t] * (T[:t, :t] @ G[:t, t])
+        yield j0, V, T
+
+
+def apply_orthogonal(H, dM, n, seed, side, stream):
+    """M := Q M (side "L") or M := M Q^T (side "R") with Stewart's Q, through
+    the library's DGEMM (blocks applied last to first)."""
+    blocks = list(householder_blocks(n, seed))
+    for j0, V, T in reversed(blocks):
+        rows, nb = V.shape
+        dV = H.DeviceArray.from_host(V)
+        dT = H.DeviceArray.from_host(T)
+        W = H.DeviceArray(nb, n) if side == "L" else H.DeviceArray(n, nb)
+        W2 = H.DeviceArray(nb, n) if side == "L" else H.DeviceArray(n, nb)
+        sub = dM.ptr + 8 * j0 if side == "L" else dM.ptr + 8 * j0 * n
+        if side == "L":  # W = V^T M[j0:, :]; W = T W; M[j0:, :] -= V W
+            H.gemm_dev("T", "N", nb, n, rows, 1.0, dV, rows, sub, n, 0.0, W, nb, stream=stream,
+                       dtype=np.float64)
+            H.gemm_dev("N", "N", nb, n, nb, 1.0, dT, nb, W, nb, 0.0, W2, nb, stream=stream,
+                       dtype=np.float64)
+            H.gemm_dev("N", "N", rows, n, nb, -1.0, dV, rows, W2, nb, 1.0, sub, n, stream=stream,
+                       dtype=np.float64)
+        else:  # Y = M[:, j0:] V; Y = Y T^T; M[:, j0:] -= Y V^T
+            H.gemm_dev("N", "N", n, nb, rows, 1.0, sub, n, dV, rows, 0.0, W, n, stream=stream,
+                       dtype=np.float64)
+            H.gemm_dev("N", "T", n, nb, nb, 1.0, W, n, dT, nb, 0.0, W2, n, stream=stream,
+                       dtype=np.float64)
+            H.gemm_dev("N", "T", n, rows, nb, -1.0, W2, n, dV, rows, 1.0, sub, n, stream=stream,
+                       dtype=np.float64)
+        H.lib().hwyblas_amd_stream_synchronize(stream)
+        del dV, dT, W, W2
+
+
+def lu_leg(H, n, kappas, stream, nrhs=16, reps=2):
+    """Mixed-precision LU solve: A = Q1 diag(sigma) Q2^T (sigma geometric from 1
+    to 10^kappa, Q1/Q2 Stewart-random orthogonal, seeds 31/32), B uniform[-1,1]
+    (seed 33), nrhs right-hand sides. For each trailing-update precision:
+    getrf + getrs + FP64 refinement, timed end to end; beside it the all-FP64
+    getrf + getrs without refinement."""
+    L = H.lib()
+    out = {}
+    for kappa in kappas:
+        t0 = time.perf_counter()
+        dA = H.DeviceArray(n, n)
+        sig = np.power(10.0, kappa * np.arange(n) / (n - 1))
+        host = np.zeros(n)  # build diag(sigma) column by column on the device
+        L.hwyblas_amd_memset(dA.ptr, 0, n * n * 8)
+        for i in range(n):
+            host[0] = sig[i]
+            L.hwyblas_amd_memcpy(dA.ptr + 8 * (i + i * n), host.ctypes.data, 8, 1)
+        apply_orthogonal(H, dA, n, 31, "L", stream)
+        apply_orthogonal(H, dA, n, 32, "R", stream)
+        dB = H.DeviceArray(n, nrhs)
+        dB.fill_uniform(n, nrhs, 33)
+        H.synchronize()
+        gen_s = time.perf_counter() - t0
+        res = {"generation_s": round(gen_s, 1)}
+        dAF, dX = H.DeviceArray(n, n), H.DeviceArray(n, nrhs)
+        dP = H.DeviceArray(n, 1, np.int32)
+        berr, nbe = np.zeros(nrhs), np.zeros(nrhs)
+        iters = np.zeros(nrhs, dtype=np.int32)
+        for update, name in ((0, "fp64"), (1, "emulated"), (2, "fp32")):
+            best = None
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                info = L.hwyblas_dgesv_ir_dev(n, nrhs, dA.ptr, n, dAF.ptr, n, dP.ptr, dB.ptr, n,
+                                              dX.ptr, n, update, 5, berr.ctypes.data,
+                                              nbe.ctypes.data, iters.ctypes.data, stream)
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            res[name] = {"ms": round(best * 1e3, 1), "solves_per_s": round(1.0 / best, 3),
+                         "berr_max": float(berr.max()), "nbe_max": float(nbe.max()),
+                         "iterations": [int(iters.min()), int(iters.max())], "info": int(info)}
+            log(f"lu n={n} kappa=1e{kappa} {name}: {best * 1e3:.1f} ms, berr {berr.max():.2e}, "
+                f"nbe {nbe.max():.2e}, corrections {iters.min()}-{iters.max()}")
+        # All-FP64 getrf + getrs, no refinement.
+        dF = H.DeviceArray(n, n)
+        X = H.DeviceArray(n, nrhs)
+        best = None
+        for _ in range(reps):
+            L.hwyblas_amd_memcpy(dF.ptr, dA.ptr, n * n * 8, 3)
+            L.hwyblas_amd_memcpy(X.ptr, dB.ptr, n * nrhs * 8, 3)
+            t0 = time.perf_counter()
+            ipiv, info = H.getrf_dev(dF, 0, stream=stream)
+            L.hwyblas_dgetrs_dev(b"N", n, nrhs, dF.ptr, n, ipiv.ptr, X.ptr, n, stream)
+            L.hwyblas_amd_stream_synchronize(stream)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        res["fp64_getrf_getrs_no_refinement"] = {"ms": round(best * 1e3, 1),
+                                                 "solves_per_s": round(1.0 / best, 3)}
+        log(f"lu n={n} kappa=1e{kappa} all-FP64 getrf+getrs: {best * 1e3:.1f} ms")
+        out[f"kappa_1e{kappa}"] = res
+        del dA, dB, dF, X, dAF, dX, dP
+    return out
+
+
 def cpu_baseline(oracle, seconds=12.0):
     """The oracle's single-threaded restatement of hwyBLAS's blocked DGEMM,
     DGEMM NN 2048^3 on one pinned core, best of the calls that fit in about
@@ -505,7 +628,7 @@ def run_batched(H, oracle, args, stream):
             log(f"no cost model ({e}); built-in configuration rule only")
     for dtype, name in ((np.float64, "gemm_f64"), (np.float32, "gemm_f32")):
         r = batched_gemm_leg(H, oracle, dtype, args.batch_count, max(3, args.steps),
-                             args.warmup, stream, model=model)
+                             args.warmup, stream, model=model, with_loop=not args.no_loop)
         log(f"batched {name}: {r['gemm_per_s']:.0f} GEMM/s ({r['grouped_ms']:.2f} ms, configs "
             f"{r['configs']}; built-in rule {r['grouped_ms_default_configs']:.2f} ms), loop "
             f"{r['loop_ms']:.2f} ms ({r['speedup_vs_loop']:.2f}x), loop graph "
@@ -513,7 +636,8 @@ def run_batched(H, oracle, args, stream):
             f"{r['tflops']:.2f} TFLOP/s, "
             f"worst bound ratio {r['worst_bound_ratio']:.3g}")
         out[name] = r
-    r = batched_trsm_leg(H, oracle, args.batch_count, max(3, args.steps), args.warmup, stream)
+    r = batched_trsm_leg(H, oracle, args.batch_count, max(3, args.steps), args.warmup, stream,
+                         with_loop=not args.no_loop)
     log(f"batched trsm_f64: {r['solves_per_s']:.0f} solves/s ({r['grouped_ms']:.2f} ms), loop "
         f"{r['loop_ms']:.2f} ms ({r['speedup_vs_loop']:.2f}x), loop graph "
         f"{r['loop_graph_ms']:.2f} ms ({r['speedup_vs_loop_graph']:.2f}x), worst residual ratio "
@@ -541,11 +665,17 @@ def main():
                     help="S/DGEMM at 4096-16384 (every transposition, both layouts) and DTRSM "
                          "8192 (every side/uplo/trans) instead of the headline")
     ap.add_argument("--batch-count", type=int, default=10000)
+    ap.add_argument("--lu", action="store_true",
+                    help="the mixed-precision LU solve (n = --lu-n, kappa 1e2, 1e6, 1e10) "
+                         "instead of the headline")
+    ap.add_argument("--lu-n", type=int, default=16384)
     ap.add_argument("--dists", default="bc",
                     help="wide-range operand distributions to run after the headline (b, c)")
     ap.add_argument("--moduli-sweep", default="",
                     help="comma-separated moduli counts: error and speed of the emulation at "
                          "each, uniform operands")
+    ap.add_argument("--no-loop", action="store_true",
+                    help="batched legs without the loop-of-single-calls comparison")
     ap.add_argument("--no-model", action="store_true",
                     help="batched GEMM with the built-in configuration rule only")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -574,6 +704,9 @@ def main():
         H.synchronize()
         del src, dst
 
+    if args.lu:
+        print(json.dumps({"lu": lu_leg(H, args.lu_n, (2, 6, 10), stream)}), flush=True)
+        return
     if args.plain_sweep:
         print(json.dumps({"plain_sweep": plain_sweep(H, oracle)}), flush=True)
         return
