@@ -36,15 +36,13 @@ TriView<T> SubView(const TriView<T>& t, int64_t i0, int64_t j0) {
 }
 
 // Solves a diagonal block (order r <= kNB) for `other` right-hand sides:
-// one wave per 64 of them, four waves per workgroup.
+// one single-wave workgroup per 64 of them, so the chunks spread over CUs.
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
     DiagChunkKernel(trsm_chunk::TrsmDesc<T> d, int nchunks) {
-  __shared__ trsm_chunk::TrsmWaveLds<T> lds[4];
-  const int wave = threadIdx.x >> 6;
-  const int chunk = blockIdx.x * 4 + wave;
-  if (chunk >= nchunks) return;
-  trsm_chunk::TrsmChunk<T>(d, chunk * 64, lds[wave]);
+  __shared__ trsm_chunk::TrsmWaveLds<T> lds;
+  if (int(blockIdx.x) >= nchunks) return;
+  trsm_chunk::TrsmChunk<T>(d, int(blockIdx.x) * 64, lds);
 }
 
 template <typename T>
@@ -64,8 +62,8 @@ void DiagSolve(bool left, const TriView<T>& t, int r, T* b, int64_t ldb,
   d.unit = t.unit;
   d.alpha = T(1);
   const int nchunks = int((other + 63) / 64);
-  hipLaunchKernelGGL(DiagChunkKernel<T>, dim3(unsigned((nchunks + 3) / 4)), dim3(256), 0, stream,
-                     d, nchunks);
+  hipLaunchKernelGGL(DiagChunkKernel<T>, dim3(unsigned(nchunks)), dim3(64), 0, stream, d,
+                     nchunks);
   HWY_HIP_CHECK(hipGetLastError());
 }
 

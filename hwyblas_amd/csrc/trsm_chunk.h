@@ -66,46 +66,66 @@ __device__ __forceinline__ void WaveSync() {
 // x_i = (b_i - sum_q l_iq x_q) * (1 / l_ii) (hwyBLAS's dot form with the
 // reciprocal, src/trsm-inl.h:107-174); then every later block is updated,
 // B_t -= L(t, b) X_b, on the MFMA units with X_b read from LDS.
-template <typename T>
-__device__ void TrsmChunk(const TrsmDesc<T>& d, int v0, TrsmWaveLds<T>& s) {
+template <typename T, bool LEFT>
+__device__ void TrsmChunkSide(const TrsmDesc<T>& d, int v0, TrsmWaveLds<T>& s) {
   using M = Mfma<T>;
   using Acc = typename M::Acc;
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-  const int len = d.left ? d.m : d.n;  // system size
-  const int nv = d.left ? d.n : d.m;   // right-hand sides
+  const int len = LEFT ? d.m : d.n;  // system size
+  const int nv = LEFT ? d.n : d.m;   // right-hand sides
   const int nvc = min(kTrsmRhs, nv - v0);
   const bool asc = d.lower_solve;
   auto phys = [&](int e) { return asc ? e : len - 1 - e; };
   // Element e of right-hand side v0 + c.
   auto bptr = [&](int e, int c) -> T* {
     const int64_t row = phys(e), v = v0 + c;
-    return d.left ? d.b + row + v * d.ldb : d.b + v + row * d.ldb;
+    return LEFT ? d.b + row + v * d.ldb : d.b + v + row * d.ldb;
   };
   auto lval = [&](int e, int q) { return SolveElem(d, phys(e), phys(q)); };
+
+  // Memory order: for the left side the elements of one right-hand side are
+  // contiguous (lanes walk rows), for the right side the right-hand sides
+  // are (lanes walk right-hand sides); loads and stores follow it.
+  constexpr bool left = LEFT;
 
   // B := alpha B; alpha == 0 leaves zeros without reading A
   // (src/trsm.cc:172-175).
   if (d.alpha != T(1)) {
-    if (lane < nvc)
+    if constexpr (left) {
+      for (int c = 0; c < nvc; ++c)
+        for (int e = lane; e < len; e += 64) {
+          T* x = bptr(e, c);
+          *x = d.alpha == T(0) ? T(0) : *x * d.alpha;
+        }
+    } else if (lane < nvc) {
       for (int e = 0; e < len; ++e) {
         T* x = bptr(e, lane);
         *x = d.alpha == T(0) ? T(0) : *x * d.alpha;
       }
+    }
     if (d.alpha == T(0)) return;
   }
   const int nb = (len + kTrsmTile - 1) / kTrsmTile;
   for (int b = 0; b < nb; ++b) {
     const int e0 = b * kTrsmTile, cnt = min(kTrsmTile, len - e0);
-    // The diagonal block in LDS, then each lane's right-hand side.
+    // The diagonal block in LDS, then the block of right-hand sides.
     for (int idx = lane; idx < kTrsmTile * kTrsmTile; idx += 64) {
       const int i = idx / kTrsmTile, q = idx % kTrsmTile;
       s.l[i][q] = (q < i && i < cnt) ? lval(e0 + i, e0 + q) : T(0);
     }
     if (lane < kTrsmTile)
       s.inv[lane] = (lane < cnt && !d.unit) ? T(1) / lval(e0 + lane, e0 + lane) : T(1);
-#pragma unroll
-    for (int i = 0; i < kTrsmTile; ++i)
-      s.x[i][lane] = (i < cnt && lane < nvc) ? *bptr(e0 + i, lane) : T(0);
+    if constexpr (left) {
+#pragma unroll 4
+      for (int it = 0; it < kTrsmRhs / 4; ++it) {
+        const int i = li, c = 4 * it + lk;
+        s.x[i][c] = (i < cnt && c < nvc) ? *bptr(e0 + i, c) : T(0);
+      }
+    } else {
+#pragma unroll 4
+      for (int i = 0; i < kTrsmTile; ++i)
+        s.x[i][lane] = (i < cnt && lane < nvc) ? *bptr(e0 + i, lane) : T(0);
+    }
     WaveSync();
     // Each lane solves its own column of s.x (rows past cnt stay 0).
     for (int i = 0; i < cnt; ++i) {
@@ -113,10 +133,19 @@ __device__ void TrsmChunk(const TrsmDesc<T>& d, int v0, TrsmWaveLds<T>& s) {
       for (int q = 0; q < i; ++q) t = __builtin_fma(-s.l[i][q], s.x[q][lane], t);
       s.x[i][lane] = t * s.inv[i];
     }
-    if (lane < nvc)
-      for (int i = 0; i < cnt; ++i) *bptr(e0 + i, lane) = s.x[i][lane];
     WaveSync();
-    // B_t -= L(t, b) X_b for every later block t.
+    if constexpr (left) {
+#pragma unroll 4
+      for (int it = 0; it < kTrsmRhs / 4; ++it) {
+        const int i = li, c = 4 * it + lk;
+        if (i < cnt && c < nvc) *bptr(e0 + i, c) = s.x[i][c];
+      }
+    } else if (lane < nvc) {
+      for (int i = 0; i < cnt; ++i) *bptr(e0 + i, lane) = s.x[i][lane];
+    }
+    // B_t -= L(t, b) X_b for every later block t. The MFMA tile is
+    // (right-hand sides x rows) for the left side, (rows x right-hand sides)
+    // for the right side, so that its loads and stores are contiguous.
     for (int t = b + 1; t < nb; ++t) {
       const int r0 = t * kTrsmTile;
       Acc acc[kTrsmRhs / 16];
@@ -124,7 +153,8 @@ __device__ void TrsmChunk(const TrsmDesc<T>& d, int v0, TrsmWaveLds<T>& s) {
       for (int jt = 0; jt < kTrsmRhs / 16; ++jt) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = r0 + M::Row(lk, r), col = 16 * jt + li;
+          const int row = left ? r0 + li : r0 + M::Row(lk, r);
+          const int col = left ? 16 * jt + M::Row(lk, r) : 16 * jt + li;
           acc[jt][r] = (row < len && col < nvc) ? *bptr(row, col) : T(0);
         }
       }
@@ -132,21 +162,34 @@ __device__ void TrsmChunk(const TrsmDesc<T>& d, int v0, TrsmWaveLds<T>& s) {
       for (int u = 0; u < kTrsmTile / 4; ++u) {
         const int i = r0 + li, q = 4 * u + lk;
         const T a = (i < len && q < cnt) ? -lval(i, e0 + q) : T(0);
+        if constexpr (left) {
 #pragma unroll
-        for (int jt = 0; jt < kTrsmRhs / 16; ++jt)
-          acc[jt] = M::Run(a, s.x[q][16 * jt + li], acc[jt]);
+          for (int jt = 0; jt < kTrsmRhs / 16; ++jt)
+            acc[jt] = M::Run(s.x[q][16 * jt + li], a, acc[jt]);
+        } else {
+#pragma unroll
+          for (int jt = 0; jt < kTrsmRhs / 16; ++jt)
+            acc[jt] = M::Run(a, s.x[q][16 * jt + li], acc[jt]);
+        }
       }
 #pragma unroll
       for (int jt = 0; jt < kTrsmRhs / 16; ++jt) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = r0 + M::Row(lk, r), col = 16 * jt + li;
+          const int row = left ? r0 + li : r0 + M::Row(lk, r);
+          const int col = left ? 16 * jt + M::Row(lk, r) : 16 * jt + li;
           if (row < len && col < nvc) *bptr(row, col) = acc[jt][r];
         }
       }
     }
     WaveSync();  // the next block overwrites the LDS
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void TrsmChunk(const TrsmDesc<T>& d, int v0, TrsmWaveLds<T>& s) {
+  if (d.left) TrsmChunkSide<T, true>(d, v0, s);
+  else TrsmChunkSide<T, false>(d, v0, s);
 }
 
 }  // namespace trsm_chunk
