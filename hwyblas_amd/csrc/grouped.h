@@ -21,12 +21,13 @@
 
 // Workgroup tiles, X(id, BM, BN, BK): the four waves of a workgroup compute
 // a BM x BN tile (2 x 2 waves) with the plain kernel's LDS-staged macro-tile,
-// BK of k per stage.
+// BK of k per stage. (A 128 x 128 tile would not fit the persistent kernel's
+// registers at two waves per SIMD beside the wave tiles.)
 #define HWY_GROUPED_BLOCK_CONFIGS(X) \
   X(10, 64, 64, 16)                  \
-  X(11, 128, 64, 16)                 \
-  X(12, 64, 128, 16)                 \
-  X(13, 128, 128, 16)
+  X(11, 128, 64, 8)                  \
+  X(12, 64, 128, 8)                  \
+  X(13, 64, 64, 32)
 
 namespace hwy_amd {
 namespace grouped {

@@ -69,7 +69,7 @@ struct WorkItem {
 
 // One wave computes the BM x BN tile of C at (i0, j0).
 template <typename T, int BM, int BN, int UK>
-__device__ void GemmTile(const GemmDesc<T>& d, int i0, int j0) {
+__device__ __forceinline__ void GemmTile(const GemmDesc<T>& d, int i0, int j0) {
   using M = Mfma<T>;
   using Acc = typename M::Acc;
   constexpr int TM = BM / 16, TN = BN / 16;
@@ -126,7 +126,7 @@ __device__ void GemmTile(const GemmDesc<T>& d, int i0, int j0) {
   // 32-bit registers of one stage of operands and of the accumulators.
   constexpr int kWords = int(sizeof(T)) / 4;
   constexpr int kStageRegs = UK * (TM + TN) * kWords, kAccRegs = TM * TN * 4 * kWords;
-  if constexpr (2 * kStageRegs + kAccRegs <= 80) {
+  if constexpr (2 * kStageRegs + kAccRegs <= 72) {
     // Two register stages: the next step's loads are in flight while the
     // current step multiplies. k still advances in ascending steps of 4.
     T af0[UK][TM], bf0[UK][TN], af1[UK][TM], bf1[UK][TN];
@@ -181,7 +181,7 @@ __device__ __forceinline__ void RunGemmItem(const GemmDesc<T>& d, const WorkItem
 
 // Workgroup tile `w` of problem `d` (all threads of the workgroup call it).
 template <typename T, bool TA, bool TB>
-__device__ void BlockTile(const GemmParams<T>& p, const WorkItem& w, T* lds) {
+__device__ __forceinline__ void BlockTile(const GemmParams<T>& p, const WorkItem& w, T* lds) {
   switch (w.cfg) {
 #define HWY_GROUPED_BLOCK_CASE(ID, BM, BN, BK) \
   case ID:                                     \
@@ -586,12 +586,12 @@ void Destroy(Plan* plan) {
   delete plan;
 }
 
-// Without tuning data: a 128 x 128 workgroup tile for large problems, else
+// Without tuning data: a 64 x 64 workgroup tile for large problems, else
 // the 64 x 64 wave tile (tiles of fewer rows cost more claims and loads than
 // they save in padding on the mixed batches measured).
 int DefaultGroupedConfig(int dtype_bytes, int m, int n, int k) {
   (void)dtype_bytes;
-  if (m >= 96 && n >= 96 && k >= 32) return 13;
+  if (m >= 96 && n >= 96 && k >= 32) return 10;
   return 9;
 }
 
