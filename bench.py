@@ -397,6 +397,9 @@ def plain_sweep(H, oracle, sizes=(4096, 8192, 16384), reps=3, samples=64):
     return out
 
 
+METHODS = {"int8": "INT8 Ozaki scheme II (v_mfma_i32_32x32x32_i8)",
+           "bf16": "BF16 slices, Ozaki scheme I (v_mfma_f32_32x32x16_bf16, exact FP32 "
+                   "chunks of 256)"}
 DISTRIBUTIONS = {
     "b": "+-u 2^e, u uniform in [1,2), e uniform integer in [-30,30] (seeds 11, 12)",
     "c": "uniform[-1,1] (seeds 1, 2), rows of A and columns of B scaled by 10^s, "
@@ -404,10 +407,13 @@ DISTRIBUTIONS = {
 }
 
 
-def distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, samples, nmod=0, steps=2):
-    """Emulated (moduli from the operands, or `nmod`) and native DGEMM on one
-    of the wide-range distributions; sampled componentwise errors against the
-    double-double reference on the same operands regenerated on the host."""
+def distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, samples, nmod=0, steps=2,
+                     method="int8"):
+    """Emulated (moduli or BF16 slices from the operands, or `nmod` of them)
+    and native DGEMM on one of the operand distributions; sampled
+    componentwise errors against the double-double reference on the same
+    operands regenerated on the host. method: "int8" (Ozaki scheme II) or
+    "bf16" (BF16 slices, Ozaki scheme I)."""
     L = H.lib()
     if dist == "b":
         L.hwyblas_amd_fill_pow2(dA.ptr, n, n, n, 11, -30, 30, stream)
@@ -438,8 +444,12 @@ def distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, samples, nmod=0, st
     used = [0]
 
     def emu_step():
-        info, u = H.dgemm_emulated_dev("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n,
-                                       nmoduli=nmod, stream=stream)
+        if method == "bf16":
+            info, u = H.dgemm_emulated_bf16_dev("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC,
+                                                n, slices=nmod, stream=stream)
+        else:
+            info, u = H.dgemm_emulated_dev("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n,
+                                           nmoduli=nmod, stream=stream)
         assert info == 0
         used[0] = u
     emu_ms, _ = time_steps(H, stream, emu_step, steps, 1)
@@ -458,15 +468,17 @@ def distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, samples, nmod=0, st
     nz = np.abs(exp) > 0
     r_nat = float(np.max(np.abs(nat_vals - exp)[nz] / np.abs(exp)[nz]))
     r_emu = float(np.max(np.abs(emu_vals - exp)[nz] / np.abs(exp)[nz]))
+    count_key = "slices" if method == "bf16" else "moduli"
     out = {"distribution": DISTRIBUTIONS.get(dist, "uniform[-1,1] (seeds 1, 2)"),
-           "moduli": used[0], "emulated_ms": round(emu_ms, 3), "native_ms": round(nat_ms, 3),
+           "method": METHODS[method],
+           count_key: used[0], "emulated_ms": round(emu_ms, 3), "native_ms": round(nat_ms, 3),
            "emulated_tflops": round(flops / (emu_ms * 1e-3) / 1e12, 2),
            "native_tflops": round(flops / (nat_ms * 1e-3) / 1e12, 2),
            "emulated_max_componentwise_rel": e_emu, "native_max_componentwise_rel": e_nat,
            "emulated_max_rel": r_emu, "native_max_rel": r_nat,
            "ratio_emulated_to_native": e_emu / e_nat if e_nat > 0 else None,
            "samples": int(samples)}
-    log(f"dist {dist}: {used[0]} moduli, emulated {out['emulated_tflops']} TF vs native "
+    log(f"dist {dist} ({method}): {used[0]} {count_key}, emulated {out['emulated_tflops']} TF vs native "
         f"{out['native_tflops']} TF; componentwise error emulated {e_emu:.3e} native {e_nat:.3e}")
     return out
 
@@ -674,6 +686,12 @@ def main():
     ap.add_argument("--moduli-sweep", default="",
                     help="comma-separated moduli counts: error and speed of the emulation at "
                          "each, uniform operands")
+    ap.add_argument("--bf16", default="a",
+                    help="distributions (a, b, c) to run the BF16-slice emulation on, slice "
+                         "count from the operands")
+    ap.add_argument("--bf16-sweep", default="",
+                    help="comma-separated BF16 slice counts: error and speed at each, uniform "
+                         "operands")
     ap.add_argument("--no-loop", action="store_true",
                     help="batched legs without the loop-of-single-calls comparison")
     ap.add_argument("--no-model", action="store_true",
@@ -773,6 +791,12 @@ def main():
         for nm in [int(x) for x in args.moduli_sweep.split(",") if x]:
             dists[f"a_moduli_{nm}"] = distribution_leg(H, oracle, "a", n, dA, dB, dC, stream,
                                                        args.samples, nmod=nm)
+        for dist in args.bf16:
+            dists[f"{dist}_bf16"] = distribution_leg(H, oracle, dist, n, dA, dB, dC, stream,
+                                                     args.samples, method="bf16")
+        for sl in [int(x) for x in args.bf16_sweep.split(",") if x]:
+            dists[f"a_bf16_slices_{sl}"] = distribution_leg(H, oracle, "a", n, dA, dB, dC, stream,
+                                                            args.samples, nmod=sl, method="bf16")
     del dA, dB, dC
 
     batched = None if args.no_batched else run_batched(H, oracle, args, stream)

@@ -58,6 +58,8 @@ def _declare(L):
                                              _i, _p])
     fn("hwyblas_dgemm_emulated_dev", _i, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i,
                                           _i, _p, _pi])
+    fn("hwyblas_dgemm_emulated_bf16_dev", _i, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p,
+                                               _i, _i, _p, _pi])
     fn("hwyblas_gemm_plan_create", _p, [_i, _i, _p, _p, _pi, _pi])
     fn("hwyblas_trsm_plan_create", _p, [_i, _i, _p, _pi, _pi])
     fn("hwyblas_plan_launch", None, [_p, _p])
@@ -216,6 +218,16 @@ def dgemm_emulated_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nmo
     info = lib().hwyblas_dgemm_emulated_dev(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(a),
                                             lda, _ptr(b), ldb, beta, _ptr(c), ldc, nmoduli,
                                             stream, ctypes.byref(used))
+    return info, used.value
+
+
+def dgemm_emulated_bf16_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, slices=0,
+                            stream=None):
+    """Emulated DGEMM (BF16 slices, Ozaki I); returns (info, slices used)."""
+    used = ctypes.c_int(0)
+    info = lib().hwyblas_dgemm_emulated_bf16_dev(ta.encode(), tb.encode(), m, n, k, alpha,
+                                                 _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc,
+                                                 slices, stream, ctypes.byref(used))
     return info, used.value
 
 

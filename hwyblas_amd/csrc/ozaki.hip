@@ -31,6 +31,7 @@
 #include <cstring>
 
 #include "common.h"
+#include "ozaki_shared.h"
 
 namespace hwy_amd {
 namespace ozaki {
@@ -157,69 +158,6 @@ __device__ __forceinline__ int ResidueOf(double v, int p, double inv_p) {
   return r;
 }
 
-__global__ void FillU64(uint64_t* x, int64_t n, uint64_t v) {
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x)
-    x[i] = v;
-}
-
-// Max |x(r, p)| per row r of a strided operand (element at x[r*rs + p*cs]),
-// as the bit pattern of a non-negative double (which orders like the
-// value; NaN orders above Inf). Rows contiguous (rs == 1): one thread per
-// row over a chunk of p. p contiguous (cs == 1): one wave per row.
-template <bool ROWS_CONTIG>
-__global__ void RowAbsMaxKernel(const double* x, int64_t ld, int64_t R, int64_t K,
-                                int64_t pchunk, uint64_t* out) {
-  if (ROWS_CONTIG) {
-    const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= R) return;
-    const int64_t p0 = int64_t(blockIdx.y) * pchunk;
-    const int64_t p1 = min(K, p0 + pchunk);
-    uint64_t mx = 0;
-    for (int64_t p = p0; p < p1; ++p) {
-      const uint64_t b = __double_as_longlong(fabs(x[r + p * ld]));
-      mx = b > mx ? b : mx;
-    }
-    if (p1 > p0) atomicMax(reinterpret_cast<unsigned long long*>(out + r), mx);
-  } else {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t r = int64_t(blockIdx.x) * (blockDim.x >> 6) + wave;
-    if (r >= R) return;
-    const int64_t p0 = int64_t(blockIdx.y) * pchunk;
-    const int64_t p1 = min(K, p0 + pchunk);
-    uint64_t mx = 0;
-    for (int64_t p = p0 + lane; p < p1; p += 64) {
-      const uint64_t b = __double_as_longlong(fabs(x[r * ld + p]));
-      mx = b > mx ? b : mx;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      const uint64_t o = __shfl_xor(mx, off, 64);
-      mx = o > mx ? o : mx;
-    }
-    if (lane == 0 && p1 > p0) atomicMax(reinterpret_cast<unsigned long long*>(out + r), mx);
-  }
-}
-
-// Per-row scale exponents s_r = bits - E_r, rowmax_r = f 2^E_r, f in
-// [0.5, 1) (0 for zero rows), and the global max bit pattern in *gmax.
-__global__ void ScaleExpKernel(const uint64_t* rowmax, int64_t R, int bits, int* s,
-                               uint64_t* gmax) {
-  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  uint64_t b = 0;
-  if (r < R) {
-    b = rowmax[r];
-    const double v = __longlong_as_double(b);
-    int e = 0;
-    if (v > 0 && b < 0x7FF0000000000000ull) frexp(v, &e);
-    s[r] = (v > 0) ? bits - e : 0;
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint64_t o = __shfl_xor(b, off, 64);
-    b = o > b ? o : b;
-  }
-  if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned long long*>(gmax), b);
-}
-
 // Slices: Y[l][r][p] (row stride KP, plane stride RP * KP) = symmetric
 // residue mod p_l of round(x(r, p) 2^s_r), zero for r >= R or p >= K. Every
 // thread produces 8 consecutive bytes of one row per modulus (16 threads
@@ -297,7 +235,6 @@ __global__ void __launch_bounds__(256)
 // row, RPB rows per 256-byte bank row) so the MFMA operand reads of 16
 // consecutive rows hit distinct banks.
 
-typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x16 __attribute__((ext_vector_type(16)));
 
 struct ModGemmArgs {
@@ -328,18 +265,8 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmKernel(ModGemmA
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
   const int l = blockIdx.y;
-  // XCD-aware raster (as in the FP64/FP32 kernel).
   int tm, tn;
-  {
-    const int nblk = gridDim.x, bid = blockIdx.x;
-    const int xcd = bid % 8, idx = bid / 8, per = nblk / 8, rem = nblk % 8;
-    const int lid = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-    const int group = 8 * g.tiles_n;
-    const int first_m = (lid / group) * 8;
-    const int gm = min(g.tiles_m - first_m, 8);
-    tm = first_m + (lid % group) % gm;
-    tn = (lid % group) / gm;
-  }
+  XcdTile(g.tiles_m, g.tiles_n, &tm, &tn);
   const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
   const int8_t* A = g.a + l * g.a_plane + m0 * g.kp;
   const int8_t* B = g.b + l * g.b_plane + n0 * g.kp;
@@ -436,49 +363,6 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmKernel(ModGemmA
 // swz). The waves synchronise with s_barrier after waiting (vmcnt) for their
 // own DMA of the slice about to be read; the slot refilled after the barrier
 // is the one every wave finished reading in the previous iteration.
-template <int N>
-__device__ __forceinline__ void WaitVmcntImm() {
-  // s_waitcnt with only vmcnt constrained (expcnt 7, lgkmcnt 15 = no wait);
-  // vmcnt is 6 bits, split over [3:0] and [15:14].
-  static_assert(N >= 0 && N < 64, "vmcnt range");
-  __builtin_amdgcn_s_waitcnt((N & 0xF) | (7 << 4) | (0xF << 8) | ((N >> 4) << 14));
-}
-
-// Waits until at most `slices` later slices (PER_SLICE DMAs each) are
-// outstanding.
-template <int PER_SLICE>
-__device__ __forceinline__ void WaitSlices(int slices) {
-  switch (slices) {
-    case 0: WaitVmcntImm<0>(); break;
-    case 1: WaitVmcntImm<PER_SLICE>(); break;
-    case 2: WaitVmcntImm<2 * PER_SLICE>(); break;
-    default: WaitVmcntImm<3 * PER_SLICE>(); break;
-  }
-}
-
-// One global_load_lds_dwordx4: this lane's 16 bytes at `src` land in LDS at
-// `lds_wave_base` + 16 * lane. Issued as inline assembly so that the
-// compiler, which would otherwise wait for every outstanding DMA before any
-// LDS read, leaves the waiting to WaitVmcnt. M0 (the LDS base) must be
-// written one instruction before the DMA reads it.
-__device__ __forceinline__ void DmaToLds(const void* src, const void* lds_wave_base) {
-  const unsigned base =
-      __builtin_amdgcn_readfirstlane(unsigned(reinterpret_cast<uintptr_t>(lds_wave_base)));
-  unsigned saved;
-  asm volatile(
-      "s_mov_b32 %0, m0\n\ts_mov_b32 m0, %1\n\ts_nop 0\n\t"
-      "global_load_lds_dwordx4 %2, off\n\ts_mov_b32 m0, %0"
-      : "=&s"(saved)
-      : "s"(base), "v"(src)
-      : "memory");
-}
-
-__device__ __forceinline__ void BarrierNoFence() {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
-
 template <int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES>
 __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmDmaKernel(ModGemmArgs g) {
   constexpr int NT = 64 * WAVES_M * WAVES_N;
@@ -499,16 +383,7 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmDmaKernel(ModGe
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
   const int l = blockIdx.y;
   int tm, tn;
-  {
-    const int nblk = gridDim.x, bid = blockIdx.x;
-    const int xcd = bid % 8, idx = bid / 8, per = nblk / 8, rem = nblk % 8;
-    const int lid = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-    const int group = 8 * g.tiles_n;
-    const int first_m = (lid / group) * 8;
-    const int gm = min(g.tiles_m - first_m, 8);
-    tm = first_m + (lid % group) % gm;
-    tn = (lid % group) / gm;
-  }
+  XcdTile(g.tiles_m, g.tiles_n, &tm, &tn);
   const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
   const int8_t* A = g.a + l * g.a_plane + m0 * g.kp;
   const int8_t* B = g.b + l * g.b_plane + n0 * g.kp;
@@ -620,16 +495,7 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmPipeKernel(ModG
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
   const int l = blockIdx.y;
   int tm, tn;
-  {
-    const int nblk = gridDim.x, bid = blockIdx.x;
-    const int xcd = bid % 8, idx = bid / 8, per = nblk / 8, rem = nblk % 8;
-    const int lid = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-    const int group = 8 * g.tiles_n;
-    const int first_m = (lid / group) * 8;
-    const int gm = min(g.tiles_m - first_m, 8);
-    tm = first_m + (lid % group) % gm;
-    tn = (lid % group) / gm;
-  }
+  XcdTile(g.tiles_m, g.tiles_n, &tm, &tn);
   const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
   const int8_t* A = g.a + l * g.a_plane + m0 * g.kp;
   const int8_t* B = g.b + l * g.b_plane + n0 * g.kp;
@@ -794,122 +660,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Sampled statistics for the choice of moduli: for sample rows i of op(A)
-// (blockIdx.x) and sample columns j of op(B): sum_p |a_ip| |b_pj|,
-// sum_p |a_ip|, sum_p |b_pj|, max_p |a_ip| and max_p |b_pj|.
-constexpr int kSamples = 32;
-struct SampleOut {
-  double sab[kSamples][kSamples];
-  double sa[kSamples], sb[kSamples], ma[kSamples], mb[kSamples];
-  uint64_t amax_bits, bmax_bits;
-};
-
-__global__ void __launch_bounds__(256)
-    SampleKernel(const double* a, int64_t a_rs, int64_t a_cs, const double* b, int64_t b_rs,
-                 int64_t b_cs, int64_t m, int64_t n, int64_t k, SampleOut* out) {
-  const int si = blockIdx.x;
-  const int64_t i = (m - 1) * si / (kSamples - 1 > 0 ? kSamples - 1 : 1);
-  __shared__ double red[256];
-  double s_ab[kSamples] = {};
-  double s_a = 0, m_a = 0;
-  double s_b[kSamples] = {}, m_b[kSamples] = {};
-  for (int64_t p = threadIdx.x; p < k; p += blockDim.x) {
-    const double av = fabs(a[i * a_rs + p * a_cs]);
-    s_a += av;
-    m_a = fmax(m_a, av);
-#pragma unroll
-    for (int sj = 0; sj < kSamples; ++sj) {
-      const int64_t j = (n - 1) * sj / (kSamples - 1);
-      const double bv = fabs(b[p * b_rs + j * b_cs]);
-      s_ab[sj] += av * bv;
-      if (si == 0) {
-        s_b[sj] += bv;
-        m_b[sj] = fmax(m_b[sj], bv);
-      }
-    }
-  }
-  auto reduce = [&](double v, bool is_max) {
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (threadIdx.x < s)
-        red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + s])
-                                  : red[threadIdx.x] + red[threadIdx.x + s];
-      __syncthreads();
-    }
-    const double r = red[0];
-    __syncthreads();
-    return r;
-  };
-  for (int sj = 0; sj < kSamples; ++sj) {
-    const double v = reduce(s_ab[sj], false);
-    if (threadIdx.x == 0) out->sab[si][sj] = v;
-  }
-  const double sa = reduce(s_a, false), ma = reduce(m_a, true);
-  if (threadIdx.x == 0) {
-    out->sa[si] = sa;
-    out->ma[si] = ma;
-  }
-  if (si == 0) {
-    for (int sj = 0; sj < kSamples; ++sj) {
-      const double sb = reduce(s_b[sj], false), mb = reduce(m_b[sj], true);
-      if (threadIdx.x == 0) {
-        out->sb[sj] = sb;
-        out->mb[sj] = mb;
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Workspace: one grow-only device buffer per thread and device.
-
-struct Workspace {
-  void* ptr = nullptr;
-  size_t cap = 0;
-  int device = -1;
-};
-thread_local Workspace t_ws[64];
-
-void* GetWorkspace(size_t bytes, hipStream_t stream) {
-  int dev = 0;
-  HWY_HIP_CHECK(hipGetDevice(&dev));
-  Workspace& w = t_ws[dev];
-  if (w.cap < bytes) {
-    if (w.ptr) {
-      HWY_HIP_CHECK(hipStreamSynchronize(stream));
-      HWY_HIP_CHECK(hipFree(w.ptr));
-    }
-    const size_t rounded = (bytes + (size_t(64) << 20) - 1) & ~((size_t(64) << 20) - 1);
-    HWY_HIP_CHECK(hipMalloc(&w.ptr, rounded));
-    w.cap = rounded;
-    w.device = dev;
-  }
-  return w.ptr;
-}
-
-size_t AlignUp(size_t x, size_t a) { return (x + a - 1) / a * a; }
-
-// Optional per-phase timing (hwyblas_amd_profile_*): HIP events recorded on
-// the call's stream around the phases of the last emulated call.
-enum Phase { kPhaseStats = 0, kPhaseSlice = 1, kPhaseModGemm = 2, kPhaseReconstruct = 3,
-             kNumPhases = 4 };
-struct PhaseTimer {
-  bool enabled = false;
-  bool recorded = false;
-  hipEvent_t ev[kNumPhases + 1] = {};
-};
-thread_local PhaseTimer t_timer;
-
-void Mark(int phase_boundary, hipStream_t stream) {
-  PhaseTimer& t = t_timer;
-  if (!t.enabled) return;
-  if (!t.ev[0])
-    for (auto& e : t.ev) HWY_HIP_CHECK(hipEventCreate(&e));
-  HWY_HIP_CHECK(hipEventRecord(t.ev[phase_boundary], stream));
-  if (phase_boundary == kNumPhases) t.recorded = true;
-}
-
 constexpr int kTileM = 256, kTileN = 256, kTileK = 64;
 // k of the padded slices: a multiple of every variant's k step.
 constexpr int kPadK = 128;
@@ -930,6 +680,51 @@ int ModGemmVariant() {
 
 }  // namespace
 
+namespace {
+struct Workspace {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  int device = -1;
+};
+thread_local Workspace t_ws[64];
+}  // namespace
+
+void* GetWorkspace(size_t bytes, hipStream_t stream, bool* grown) {
+  int dev = 0;
+  HWY_HIP_CHECK(hipGetDevice(&dev));
+  Workspace& w = t_ws[dev];
+  if (grown) *grown = w.cap < bytes;
+  if (w.cap < bytes) {
+    if (w.ptr) {
+      HWY_HIP_CHECK(hipStreamSynchronize(stream));
+      HWY_HIP_CHECK(hipFree(w.ptr));
+    }
+    const size_t rounded = (bytes + (size_t(64) << 20) - 1) & ~((size_t(64) << 20) - 1);
+    HWY_HIP_CHECK(hipMalloc(&w.ptr, rounded));
+    w.cap = rounded;
+    w.device = dev;
+  }
+  return w.ptr;
+}
+
+struct PhaseTimer {
+  bool enabled = false;
+  bool recorded = false;
+  hipEvent_t ev[kNumPhases + 1] = {};
+};
+thread_local PhaseTimer t_timer;
+
+void Mark(int phase_boundary, hipStream_t stream) {
+  PhaseTimer& t = t_timer;
+  if (!t.enabled) return;
+  if (!t.ev[0])
+    for (auto& e : t.ev) HWY_HIP_CHECK(hipEventCreate(&e));
+  HWY_HIP_CHECK(hipEventRecord(t.ev[phase_boundary], stream));
+  if (phase_boundary == kNumPhases) t.recorded = true;
+}
+
+void ResetPhases() { t_timer.recorded = false; }
+
 // Chooses the number of moduli from the sampled statistics: the per-row
 // fixed-point truncation perturbs C(i, j) by at most
 //   2^-(bA+1) maxrow|a_i| sum_p |b_pj| + 2^-(bB+1) maxcol|b_j| sum_p |a_ip|,
@@ -937,14 +732,8 @@ int ModGemmVariant() {
 // sum_p |a_ip||b_pj|; the smallest N whose bits keep the first within the
 // second on every sampled (i, j) is used.
 int ChooseModuli(const SampleOut& s, int64_t k) {
-  double worst_a = 0, worst_b = 0;
-  for (int i = 0; i < kSamples; ++i)
-    for (int j = 0; j < kSamples; ++j) {
-      const double denom = s.sab[i][j];
-      if (denom <= 0) continue;
-      worst_a = std::max(worst_a, s.ma[i] * s.sb[j] / denom);
-      worst_b = std::max(worst_b, s.mb[j] * s.sa[i] / denom);
-    }
+  double worst_a, worst_b, worst_ab;
+  WorstRatios(s, k, &worst_a, &worst_b, &worst_ab);
   const double u = 0x1p-53;
   const double target = std::sqrt(double(k)) * u;
   for (int nmod = 8; nmod <= kMaxModuli; ++nmod) {
@@ -959,7 +748,7 @@ int ChooseModuli(const SampleOut& s, int64_t k) {
 int EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
                   const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                   double* c, int64_t ldc, int nmod, hipStream_t stream) {
-  t_timer.recorded = false;
+  ResetPhases();
   if (k >= 131072) {
     GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
     return 0;
@@ -1032,7 +821,6 @@ int EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alph
   if (nmod <= 0)
     HWY_HIP_CHECK(hipMemcpyAsync(&host, sample, sizeof(host), hipMemcpyDeviceToHost, stream));
   HWY_HIP_CHECK(hipStreamSynchronize(stream));
-  const uint64_t kInfBits = 0x7FF0000000000000ull;
   if (gm[0] >= kInfBits || gm[1] >= kInfBits) {
     GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
     return 0;

@@ -191,6 +191,26 @@ HWYBLAS_AMD_API int hwyblas_dgemm_emulated_dev(char transa, char transb, int m, 
                                                int nmoduli, void* stream,
                                                int* used_moduli);
 
+/* Emulated DGEMM on the BF16 matrix cores (no hwyBLAS counterpart): the
+ * Ozaki scheme I with BF16 slices. Same arguments, checks and quick returns
+ * as hwyblas_dgemm_emulated_dev. Rows of op(A) and columns of op(B) are
+ * scaled by powers of two below 1 in magnitude and cut into `slices`
+ * (1..16; larger values use 16; 0 = chosen from a sampled estimate of the
+ * operands' dynamic range so that the error stays at native FP64 level)
+ * 8-bit integer digits, exact in BF16; the s (s + 1) / 2 digit products
+ * with t + u <= s + 1 run on the BF16 MFMA with exact FP32 accumulation
+ * over k chunks of 256, summed in FP64 in a fixed order (the result is
+ * reproducible bit for bit). Operands with NaN or Inf use the native FP64
+ * kernel. Synchronises `stream` once. *used_slices (if not NULL) receives
+ * the slice count, or 0 when the native kernel ran. Returns 0 or the
+ * Fortran info. */
+HWYBLAS_AMD_API int hwyblas_dgemm_emulated_bf16_dev(char transa, char transb, int m, int n,
+                                                    int k, double alpha, const double* a,
+                                                    int lda, const double* b, int ldb,
+                                                    double beta, double* c, int ldc,
+                                                    int slices, void* stream,
+                                                    int* used_slices);
+
 /* ------------------------------------------------------------------------ */
 /* Batched (grouped) GEMM and TRSM over many independent, uneven problems (no
  * hwyBLAS counterpart: hwyBLAS solves one problem per call). A plan checks

@@ -70,6 +70,11 @@ def lib():
         L.orc_dgemm_ozaki.restype = None
         L.orc_dgemm_ozaki.argtypes = [_c_char, _c_char, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p,
                                       _i, _i]
+        L.orc_dgemm_ozaki_bf16.restype = None
+        L.orc_dgemm_ozaki_bf16.argtypes = [_c_char, _c_char, _i, _i, _i, _d, _p, _i, _p, _i,
+                                           _d, _p, _i, _i]
+        L.orc_bf16_digits.restype = None
+        L.orc_bf16_digits.argtypes = [_d, _i, _i, _p]
         L.orc_fill_pow2.restype = None
         L.orc_fill_pow2.argtypes = [_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                     ctypes.c_uint64, _i, _i]
@@ -178,6 +183,19 @@ def dgemm_ozaki(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nmod):
     """In-place C update by the CPU restatement of the emulated DGEMM."""
     lib().orc_dgemm_ozaki(_b(ta), _b(tb), m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta,
                           _ptr(c), ldc, nmod)
+
+
+def dgemm_ozaki_bf16(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, slices):
+    """In-place C update by the CPU restatement of the BF16-slice emulated DGEMM."""
+    lib().orc_dgemm_ozaki_bf16(_b(ta), _b(tb), m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb,
+                               beta, _ptr(c), ldc, slices)
+
+
+def bf16_digits(x, sc, slices):
+    """The integer digits d_1..d_s of x 2^sc (|x 2^sc| < 1)."""
+    d = np.zeros(slices, dtype=np.int32)
+    lib().orc_bf16_digits(float(x), sc, slices, _ptr(d))
+    return d
 
 
 def plant_mistake(which):

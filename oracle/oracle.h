@@ -111,6 +111,20 @@ void orc_dgemm_ozaki(char transa, char transb, int m, int n, int k, double alpha
                      const double* a, int lda, const double* b, int ldb, double beta,
                      double* c, int ldc, int nmod);
 
+/* ---- emulated DGEMM with BF16 slices (Ozaki scheme I), restated on the CPU
+ * (oracle/emulated_bf16.cc) ------------------------------------------------
+ * The product's algorithm: rows of op(A) / columns of op(B) scaled by
+ * 2^-(E+1) (max magnitude f 2^E, f in [0.5, 1)), `slices` balanced 8-bit
+ * digits per entry (d_t = rint(256 r), ties to even; r = 256 r - d_t;
+ * |d_t| <= 128), the pairs t + u <= s + 1 by level from s + 1 down to 2
+ * (t ascending), exact chunk sums over k chunks of 1024 accumulated as
+ * acc = fma(chunk, 2^-8l, acc), then C = alpha 2^(E_i + F_j + 2) acc + beta C.
+ * Deterministic bit for bit. */
+void orc_bf16_digits(double x, int sc, int slices, int* digits);
+void orc_dgemm_ozaki_bf16(char transa, char transb, int m, int n, int k, double alpha,
+                          const double* a, int lda, const double* b, int ldb, double beta,
+                          double* c, int ldc, int slices);
+
 /* ---- LU with partial pivoting and its solve (oracle/lu.cc) ---------------
  * hwyBLAS's dgetrf (blocked NB = 64 over the recursive dgetrf2 with a leaf of
  * 16), dlaswp (1-based k1..k2, incx = +-1), dgetrs and gerfs's componentwise
@@ -127,7 +141,8 @@ void orc_dgerfs_berr(int n, int nrhs, const double* a, int lda, const double* b,
  * the wrong transposition; 3: ignore beta; 4: TRSM skips the reciprocal of
  * the diagonal; 5: the double-double reference drops its low word; 6: the
  * emulated DGEMM's CRT drops the last modulus; 7: the LU leaf does not swap
- * the columns left of the pivot column. */
+ * the columns left of the pivot column; 8: the BF16-slice emulation leaves
+ * out its top level of slice pairs (t + u = s + 1). */
 void orc_plant_mistake(int which);
 
 #ifdef __cplusplus

@@ -286,6 +286,107 @@ def test_planted_crt_mistake_is_caught():
 
 
 # ---------------------------------------------------------------------------
+# The BF16-slice emulation's restatement (oracle/emulated_bf16.cc)
+
+def test_bf16_digits_known_answers():
+    """Balanced base-256 digits (nearest, ties to even): worked by hand, and
+    for random values of magnitude below 1/2 every digit is an integer of
+    magnitude <= 128 and sum_t d_t 256^-t is within 256^-s / 2 of x (exact
+    rationals)."""
+    from fractions import Fraction
+    assert list(oracle.bf16_digits(0.375, 0, 3)) == [96, 0, 0]
+    assert list(oracle.bf16_digits(-0.375, 0, 2)) == [-96, 0]
+    # x 2^-1 = 24 256^-1 + 2.5 256^-2 + 0.5 256^-3: 256 r = 24.0098 -> 24;
+    # then 2.50195 -> 3 (not a tie), leaving -0.498; 256 (-0.498) = -127.5
+    # -> -128 (tie to even), leaving 0.5; then 128.
+    x = 3 * 2.0 ** -4 + 5 * 2.0 ** -16 + 2.0 ** -24
+    assert list(oracle.bf16_digits(x, -1, 4)) == [24, 3, -128, 128]
+    assert list(oracle.bf16_digits(x, -1, 5)) == [24, 3, -128, 128, 0]
+    rng = np.random.default_rng(5)
+    for v in rng.uniform(-0.5, 0.5, 200):
+        for s in (3, 7, 16):
+            d = oracle.bf16_digits(v, 0, s)
+            assert np.all(np.abs(d) <= 128)
+            total = sum(Fraction(int(dt), 256 ** (t + 1)) for t, dt in enumerate(d))
+            assert abs(Fraction(v) - total) <= Fraction(1, 2 * 256 ** s)
+
+
+def _two_digit_integers(rng, rows, cols):
+    """Integers of magnitude < 2^12 with a full-range entry in every row and
+    column: scaled by 2^-13 each is x / 8192 = d1 / 256 + d2 / 65536 with
+    d1 = rint(x / 32) and an integer d2 (x - 32 d1 is an integer of
+    magnitude <= 16): exactly two digits."""
+    x = rng.integers(-(2 ** 12) + 1, 2 ** 12, (rows, cols)).astype(float)
+    x[:, 0] = 2 ** 12 - 1
+    x[0, :] = -(2 ** 12) + 1
+    return np.asfortranarray(x)
+
+
+def test_bf16_emulation_exact_on_two_digit_integers():
+    """Two-digit entries need the digit pairs of levels 2..4 (t + u <= 4):
+    with 3 slices every needed pair is present and C = A B exactly (every
+    partial sum is a multiple of 2^-32 below 2^9); with 2 slices the pair
+    (2, 2) of level 4 is left out and C is not exact."""
+    rng = np.random.default_rng(11)
+    m, n, k = 23, 19, 2600  # three k chunks, the last ragged
+    a = _two_digit_integers(rng, m, k)
+    b = _two_digit_integers(rng, k, n)
+    want = (a.astype(np.int64) @ b.astype(np.int64)).astype(float)
+    for s, exact in ((3, True), (5, True), (2, False)):
+        c = np.zeros((m, n), order="F")
+        oracle.dgemm_ozaki_bf16("N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m, s)
+        assert np.array_equal(c, want) == exact, s
+    # The transposed operand forms give the same product.
+    at, bt = np.asfortranarray(a.T), np.asfortranarray(b.T)
+    c = np.zeros((m, n), order="F")
+    oracle.dgemm_ozaki_bf16("T", "T", m, n, k, 1.0, at, k, bt, n, 0.0, c, m, 3)
+    assert np.array_equal(c, want)
+
+
+def test_planted_bf16_level_mistake_is_caught():
+    oracle.plant_mistake(8)
+    with pytest.raises(AssertionError):
+        test_bf16_emulation_exact_on_two_digit_integers()
+
+
+def test_bf16_emulation_error_falls_with_slices():
+    """Against the double-double reference on uniform operands: the
+    componentwise error (relative to |alpha| |A| |B| + |beta| |C|) falls by
+    about 2^-8 per slice until it reaches FP64 rounding; 7 slices are within
+    native FP64's typical sqrt(k) u, 5 are not, and the native worst-case
+    bound 4 (k + 2) u holds from 7 on."""
+    m, n, k = 48, 40, 700
+    a = oracle.fill_uniform(m, k, 21)
+    b = oracle.fill_uniform(k, n, 22)
+    c = oracle.fill_uniform(m, n, 23)
+    exp, bnd = oracle.gemm_ref("N", "N", m, n, k, 1.5, a, m, b, k, -0.5, c, m)
+    eps = np.finfo(float).eps
+    errs = {}
+    for s in range(3, 10):
+        cc = c.copy(order="F")
+        oracle.dgemm_ozaki_bf16("N", "N", m, n, k, 1.5, a, m, b, k, -0.5, cc, m, s)
+        errs[s] = np.max(np.abs(cc - exp) / bnd)
+    for s in range(3, 6):
+        assert errs[s + 1] <= errs[s] * 2.0 ** -6, (s, errs)
+    assert errs[7] <= np.sqrt(k) * eps and errs[5] > np.sqrt(k) * eps, errs
+    for s in range(7, 10):
+        assert errs[s] <= 4 * (k + 2) * eps
+
+
+def test_bf16_emulation_quick_returns_and_beta():
+    m, n, k = 5, 4, 3
+    a = oracle.fill_uniform(m, k, 1)
+    b = oracle.fill_uniform(k, n, 2)
+    c = np.full((m, n), np.nan, order="F")
+    oracle.dgemm_ozaki_bf16("N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m, 7)
+    assert np.isfinite(c).all()  # beta = 0: C is not read
+    c2 = np.full((m, n), 2.0, order="F")
+    nan = np.full_like(a, np.nan)
+    oracle.dgemm_ozaki_bf16("N", "N", m, n, k, 0.0, nan, m, nan, k, 3.0, c2, m, 7)
+    assert np.all(c2 == 6.0)  # alpha = 0: C = beta C, A and B unread
+
+
+# ---------------------------------------------------------------------------
 # The cost model's CPU restatement (oracle/mlp.py)
 
 def test_mlp_forward_by_hand():
