@@ -33,7 +33,10 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -49,6 +52,7 @@ namespace lu {
 namespace {
 
 constexpr int kPanel = 64;          // columns factored column by column
+constexpr int kGetrfInternalError = -1000;  // a panel barrier timed out
 constexpr int kStepRows = 256;      // threads per workgroup in a column step
 constexpr int64_t kEmulatedMinK = 1024;  // below this the emulation does not pay
 
@@ -343,6 +347,280 @@ __global__ void __launch_bounds__(64)
   for (int i = 0; i < kPanel; ++i) inv[i + j * kPanel] = (i < r && j < r) ? x[i] : 0.0;
 }
 
+// The whole panel factorization in one cooperative launch: the panel stays
+// in registers and every column step ends in one grid-wide barrier instead
+// of a kernel boundary. Workgroup g (512 threads) owns rows
+// [256 g, 256 g + 256): wave w holds the 16 columns 16 (w % 4) .. + 15 of
+// the rows 128 (w / 4) + 64 q + lane, q < 2. Per column j, as
+// PanelStepKernel: the waves holding column j form the multipliers, every
+// wave updates its columns; the waves holding column j + 1 find the
+// workgroup's best candidate (xIAMAX order), and the workgroup publishes a
+// record {|value|, row, the row's 64 values}; the owner of row j + 1
+// publishes that row too. After the barrier every workgroup loads all
+// records in one pass (one memory round trip), reduces the candidates itself
+// (the same result everywhere), takes the pivot row's values and swaps the
+// two rows in its registers. Records alternate between two slots: a slot is
+// rewritten two barriers after it was read. Records, the arrival counter and
+// the polls are agent-scope atomic accesses (coherent across the XCDs'
+// L2 caches without cache-wide write-back or invalidation fences, which
+// would dominate the step); there are few, large workgroups because the
+// barrier's cost grows with their number. Each wait is bounded (kSpinLimit
+// polls); a timed-out wait sets
+// *failed and every workgroup returns (the launch is cooperative, so all
+// workgroups are resident and the bound is never reached in a working run).
+constexpr int kPersistThreads = 512;
+constexpr int kPersistRows = 256;               // rows per workgroup
+constexpr int kPersistRpt = kPersistRows / 128;  // rows per thread (two row halves)
+constexpr int kPersistMaxGroups = 128;          // records loaded through LDS
+constexpr int kRecord = kPanel + 2;             // |value|, row, 64 values
+constexpr unsigned kSpinLimit = 1u << 22;       // polls (a few seconds at most)
+
+struct PersistArgs {
+  double* p;
+  int64_t lda;
+  int m, nb, row0;
+  int* ipiv;
+  int* info;
+  double sfmin;
+  double* rec;      // [2][gridDim.x + 1][kRecord]; slot gridDim.x: row j + 1
+  unsigned* bar;    // arrivals (0 at launch)
+  unsigned* failed;
+};
+
+// Arrive, then wait until `target` arrivals in all. Every thread's record
+// stores (agent-scope atomics, written through to the coherent level) have
+// completed (vmcnt) before thread 0 arrives; the records are read after the
+// wait with agent-scope atomic loads. False when the wait timed out (here or
+// in another workgroup).
+__device__ bool GridSync(unsigned* bar, unsigned target, unsigned* failed) {
+  __shared__ int ok;
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int good = 1;
+    unsigned polls = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+          ++polls > kSpinLimit) {
+        __hip_atomic_exchange(failed, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        good = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    ok = good;
+  }
+  asm volatile("" ::: "memory");
+  __syncthreads();
+  return ok != 0;
+}
+
+// Wavefront reductions of the device library (ockl).
+extern "C" __device__ unsigned long long __ockl_wfred_max_u64(unsigned long long);
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+
+// The wave's best (value, row) in xIAMAX order: values are non-negative
+// magnitudes or -1 (no candidate / NaN); ties go to the smaller row. The
+// magnitude's bits order like its value, so the largest key
+// (bits + 1, 0 for -1) picks the value and a minimum over the lanes holding
+// it picks the row.
+__device__ __forceinline__ void WaveBest(double& v, int& row) {
+  const unsigned long long key =
+      v < 0.0 ? 0ull : static_cast<unsigned long long>(__double_as_longlong(v)) + 1ull;
+  const unsigned long long kmax = __ockl_wfred_max_u64(key);
+  row = __ockl_wfred_min_i32(key == kmax ? row : INT_MAX);
+  v = kmax == 0ull ? -1.0 : __longlong_as_double(static_cast<long long>(kmax - 1ull));
+}
+
+__device__ __forceinline__ void PutRecord(double* x, double v) {
+  __hip_atomic_store(x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double GetRecord(const double* x) {
+  return __hip_atomic_load(const_cast<double*>(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// x[t] for a run-time t < kGroupCols without indexing registers.
+__device__ __forceinline__ double Pick(const double (&x)[kGroupCols], int t) {
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i < kGroupCols; ++i) r = (i == t) ? x[i] : r;
+  return r;
+}
+
+// x[t] = y where `put`, for a run-time t < kGroupCols.
+__device__ __forceinline__ void PutIf(double (&x)[kGroupCols], int t, bool put, double y) {
+#pragma unroll
+  for (int i = 0; i < kGroupCols; ++i) x[i] = (put && i == t) ? y : x[i];
+}
+
+__global__ void __launch_bounds__(kPersistThreads) PanelPersistentKernel(PersistArgs g) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cg = wave % kStepGroups, half = wave / kStepGroups;
+  const int c0 = cg * kGroupCols;
+  const int G = gridDim.x, gid = blockIdx.x;
+  const int m = g.m, nb = g.nb, mn = min(m, nb);
+  const int64_t lda = g.lda;
+  __shared__ double su[kPanel];                   // the pivot row
+  __shared__ double sl[kPersistRows];             // multipliers of the current column
+  __shared__ double srec[kPersistMaxGroups + 1][kRecord];
+  __shared__ double hv[2];
+  __shared__ int hrow[2];
+  __shared__ int s_piv;
+
+  int rows[kPersistRpt];
+  double v[kPersistRpt][kGroupCols];
+#pragma unroll
+  for (int q = 0; q < kPersistRpt; ++q) {
+    rows[q] = gid * kPersistRows + half * (kPersistRows / 2) + q * 64 + lane;
+#pragma unroll
+    for (int t = 0; t < kGroupCols; ++t) {
+      const int c = c0 + t;
+      v[q][t] = (rows[q] < m && c < nb) ? g.p[rows[q] + int64_t(c) * lda] : 0.0;
+    }
+  }
+  const int lbase = half * (kPersistRows / 2);  // this wave's rows in sl
+
+  for (int j = -1; j < mn; ++j) {
+    if (j >= 0) {
+      // Eliminate column j with the pivot row (row j, in place).
+      const double piv = su[j];
+      if (cg == j / kGroupCols) {
+#pragma unroll
+        for (int q = 0; q < kPersistRpt; ++q) {
+          const double lraw = Pick(v[q], j % kGroupCols);
+          double l = lraw;
+          if (piv != 0.0) l = (fabs(piv) >= g.sfmin) ? lraw * (1.0 / piv) : lraw / piv;
+          const bool below = rows[q] > j && rows[q] < m;
+          PutIf(v[q], j % kGroupCols, below && piv != 0.0, l);
+          sl[lbase + q * 64 + lane] = below ? l : 0.0;
+        }
+      }
+      __syncthreads();
+      double u[kGroupCols];
+#pragma unroll
+      for (int t = 0; t < kGroupCols; ++t) u[t] = su[c0 + t];
+#pragma unroll
+      for (int q = 0; q < kPersistRpt; ++q) {
+        if (rows[q] > j && rows[q] < m) {
+          const double l = sl[lbase + q * 64 + lane];
+#pragma unroll
+          for (int t = 0; t < kGroupCols; ++t) {
+            const int c = c0 + t;
+            if (c > j && c < nb) v[q][t] = __builtin_fma(-u[t], l, v[q][t]);
+          }
+        }
+      }
+    }
+    const int jn = j + 1;
+    if (jn >= mn) break;
+    const int buf = jn & 1;
+    double* slot = g.rec + int64_t(buf) * (G + 1) * kRecord;
+    // This workgroup's candidate for column jn: each of the two waves holding
+    // the column reduces its rows, then the better of the two.
+    if (cg == jn / kGroupCols) {
+      double bv = -1.0;
+      int brow = INT_MAX;
+#pragma unroll
+      for (int q = 0; q < kPersistRpt; ++q) {
+        if (rows[q] >= jn && rows[q] < m) {
+          const double x = Pick(v[q], jn % kGroupCols);
+          const double val = isnan(x) ? -1.0 : fabs(x);
+          if (Better(val, rows[q], bv, brow)) {
+            bv = val;
+            brow = rows[q];
+          }
+        }
+      }
+      WaveBest(bv, brow);
+      if (lane == 0) {
+        hv[half] = bv;
+        hrow[half] = brow;
+      }
+    }
+    __syncthreads();
+    double bv = hv[0];
+    int br = hrow[0];
+    if (Better(hv[1], hrow[1], bv, br)) {
+      bv = hv[1];
+      br = hrow[1];
+    }
+    if (threadIdx.x == 0) {
+      PutRecord(slot + int64_t(gid) * kRecord, bv);
+      PutRecord(slot + int64_t(gid) * kRecord + 1, double(br));
+    }
+#pragma unroll
+    for (int q = 0; q < kPersistRpt; ++q) {
+      if (rows[q] == br) {
+#pragma unroll
+        for (int t = 0; t < kGroupCols; ++t)
+          PutRecord(slot + int64_t(gid) * kRecord + 2 + c0 + t, v[q][t]);
+      }
+      if (rows[q] == jn) {
+#pragma unroll
+        for (int t = 0; t < kGroupCols; ++t)
+          PutRecord(slot + int64_t(G) * kRecord + 2 + c0 + t, v[q][t]);
+      }
+    }
+    if (!GridSync(g.bar, unsigned(jn + 1) * unsigned(G), g.failed)) return;
+
+    // Every record in one pass, then the pivot (the same in every workgroup).
+    for (int idx = threadIdx.x; idx < (G + 1) * kRecord; idx += kPersistThreads)
+      (&srec[0][0])[idx] = GetRecord(slot + idx);
+    __syncthreads();
+    if (wave == 0) {
+      // G <= 128 candidates, two per lane.
+      double gv = -1.0;
+      int grow = INT_MAX;
+      for (int b = lane; b < G; b += 64) {
+        const double cv = srec[b][0];
+        const int crow = int(srec[b][1]);
+        if (Better(cv, crow, gv, grow)) {
+          gv = cv;
+          grow = crow;
+        }
+      }
+      WaveBest(gv, grow);
+      if (lane == 0) {
+        // xIAMAX picks the first row when it holds a NaN or the column is zero.
+        if (isnan(srec[G][2 + jn]) || !(gv > 0.0)) {
+          grow = jn;
+          if (gv == 0.0 && gid == 0) atomicMin(g.info, g.row0 + jn + 1);
+        }
+        if (gid == 0) g.ipiv[jn] = grow + 1;
+        s_piv = grow;
+      }
+    }
+    __syncthreads();
+    const int pr = s_piv;
+    const double* prow = (pr == jn) ? &srec[G][2] : &srec[pr / kPersistRows][2];
+    if (threadIdx.x < kPanel) su[threadIdx.x] = prow[threadIdx.x];
+    if (pr != jn) {
+#pragma unroll
+      for (int q = 0; q < kPersistRpt; ++q) {
+        if (rows[q] == jn) {
+#pragma unroll
+          for (int t = 0; t < kGroupCols; ++t) v[q][t] = prow[c0 + t];
+        } else if (rows[q] == pr) {
+#pragma unroll
+          for (int t = 0; t < kGroupCols; ++t) v[q][t] = srec[G][2 + c0 + t];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int q = 0; q < kPersistRpt; ++q) {
+    if (rows[q] >= m) continue;
+#pragma unroll
+    for (int t = 0; t < kGroupCols; ++t) {
+      const int c = c0 + t;
+      if (c < nb) g.p[rows[q] + int64_t(c) * lda] = v[q][t];
+    }
+  }
+}
+
 struct Work {
   double* base = nullptr;  // the whole matrix
   int ncols = 0;
@@ -353,14 +631,18 @@ struct Work {
   int* info = nullptr;
   float* f32 = nullptr;
   size_t f32_elems = 0;
+  // The cooperative panel kernel (when every workgroup fits on the device).
+  int persist_max_groups = 0;
+  double* prec = nullptr;      // [2][groups + 1][kRecord]
+  unsigned* sync = nullptr;    // [0]: arrivals, [64]: failed
 };
 
-// Panel a (m x nb) whose first row is row `row0` of the matrix.
-void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, hipStream_t s) {
+// The panel factorization as one launch per column step.
+void PanelSteps(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w,
+                hipStream_t s) {
   const double sfmin = std::numeric_limits<double>::min();
   const int mn = std::min(m, nb);
   double* full = w.base + row0;
-  // Step j eliminates column j (j >= 0) and pivots column j + 1 (if any).
   for (int j = -1; j < mn; ++j) {
     const int rows = m - (j + 1);
     if (j + 1 >= mn && rows <= 0) break;
@@ -368,6 +650,25 @@ void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, 
     hipLaunchKernelGGL(PanelStepKernel, dim3(blocks), dim3(kStepRows), 0, s, a, lda, m, nb, j,
                        full, w.ncols, row0, ipiv, w.cand, w.counter, w.info, sfmin);
   }
+}
+
+// Panel a (m x nb) whose first row is row `row0` of the matrix.
+void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, hipStream_t s) {
+  const double sfmin = std::numeric_limits<double>::min();
+  const int mn = std::min(m, nb);
+  double* full = w.base + row0;
+  const int groups = (m + kPersistRows - 1) / kPersistRows;
+  if (groups <= w.persist_max_groups) {
+    hipLaunchKernelGGL(FillUnsigned, dim3(1), dim3(64), 0, s, w.sync, 1, 0u);
+    PersistArgs g{a, lda, m, nb, row0, ipiv, w.info, sfmin, w.prec, w.sync, w.sync + 64};
+    void* args[] = {&g};
+    HWY_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(PanelPersistentKernel),
+                                             dim3(groups), dim3(kPersistThreads), args, 0, s));
+  } else {
+    // Step j eliminates column j (j >= 0) and pivots column j + 1 (if any).
+    PanelSteps(a, lda, m, nb, row0, ipiv, w, s);
+  }
+  // The panel's interchanges on every other column of the matrix.
   // The panel's interchanges on every other column of the matrix.
   const int pc0 = int((a - full) / lda);  // the panel's first column
   const int others = w.ncols - nb;
@@ -458,9 +759,24 @@ void Recurse(double* a, int64_t lda, int m, int n, int row0, int* ipiv, int upda
   hipLaunchKernelGGL(AddToPivots, dim3(GridFor(cnt)), dim3(256), 0, s, ipiv + n1, cnt, n1);
 }
 
+// How many workgroups of PanelPersistentKernel can be resident at once on
+// the current device (0: use one launch per column step).
+int PersistentPanelCapacity() {
+  const char* e = getenv("HWYBLAS_AMD_LU_PANEL");
+  if (e && std::string(e) == "steps") return 0;
+  int dev = 0, cus = 0, coop = 0, per_cu = 0;
+  HWY_HIP_CHECK(hipGetDevice(&dev));
+  HWY_HIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  HWY_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HWY_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(PanelPersistentKernel), kPersistThreads, 0));
+  return coop ? std::min(per_cu * cus, kPersistMaxGroups) : 0;
+}
+
 }  // namespace
 
-// Returns info (0, or the first zero pivot, 1-based). Synchronizes.
+// Returns info (0, or the first zero pivot, 1-based, or kGetrfInternalError).
+// Synchronizes.
 int Getrf(int m, int n, double* a, int64_t lda, int* ipiv, int update, hipStream_t s) {
   if (m == 0 || n == 0) return 0;
   Work w;
@@ -475,14 +791,34 @@ int Getrf(int m, int n, double* a, int64_t lda, int* ipiv, int update, hipStream
   HWY_HIP_CHECK(hipMalloc(&w.info, 256));
   hipLaunchKernelGGL(FillUnsigned, dim3(1), dim3(64), 0, s, w.counter, 1, 0u);
   hipLaunchKernelGGL(FillInt, dim3(1), dim3(64), 0, s, w.info, 1, INT_MAX);
+  // The cooperative panel kernel serves every panel whose workgroups are all
+  // resident at once (HWYBLAS_AMD_LU_PANEL=steps: one launch per column).
+  const int groups = (m + kPersistRows - 1) / kPersistRows;
+  const int cap = PersistentPanelCapacity();
+  if (cap > 0) {
+    w.persist_max_groups = std::min(groups, cap);
+    const int g = w.persist_max_groups;
+    HWY_HIP_CHECK(hipMalloc(&w.prec, size_t(2) * (g + 1) * kRecord * sizeof(double)));
+    HWY_HIP_CHECK(hipMalloc(&w.sync, 512));
+    hipLaunchKernelGGL(FillUnsigned, dim3(1), dim3(128), 0, s, w.sync, 128, 0u);
+  }
   Recurse(a, lda, m, n, 0, ipiv, update, w, s);
   int info = INT_MAX;
+  unsigned failed = 0;
   HWY_HIP_CHECK(hipMemcpyAsync(&info, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (w.sync)
+    HWY_HIP_CHECK(hipMemcpyAsync(&failed, w.sync + 64, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                 s));
   HWY_HIP_CHECK(hipStreamSynchronize(s));
   for (void* p : {static_cast<void*>(w.cand), static_cast<void*>(w.counter),
                   static_cast<void*>(w.info), static_cast<void*>(w.f32),
-                  static_cast<void*>(w.linv), static_cast<void*>(w.temp)})
+                  static_cast<void*>(w.linv), static_cast<void*>(w.temp),
+                  static_cast<void*>(w.prec), static_cast<void*>(w.sync)})
     if (p) HWY_HIP_CHECK(hipFree(p));
+  if (failed) {
+    fprintf(stderr, "hwyblas_amd: dgetrf's panel kernel timed out at a grid barrier\n");
+    return kGetrfInternalError;
+  }
   return info == INT_MAX ? 0 : info;
 }
 
@@ -728,7 +1064,7 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
   HWY_HIP_CHECK(hipMemcpy2DAsync(af, size_t(ldaf) * 8, a, size_t(lda) * 8, size_t(n) * 8, n,
                                  hipMemcpyDeviceToDevice, s));
   info = lu::Getrf(n, n, af, ldaf, ipiv, update, s);
-  if (info > 0) return info;
+  if (info != 0) return info;
   HWY_HIP_CHECK(hipMemcpy2DAsync(x, size_t(ldx) * 8, b, size_t(ldb) * 8, size_t(n) * 8, nrhs,
                                  hipMemcpyDeviceToDevice, s));
   const lu::Perm perm(n, nrhs, ipiv, s);

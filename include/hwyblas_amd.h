@@ -330,9 +330,13 @@ HWYBLAS_AMD_API void hwyblas_grouped_set_model(void* model);
  * (src/lapack/getrf.cc:112-207) with 64-column panels. ipiv: device array of
  * min(m, n) 1-based row indices. update: precision of the trailing-matrix
  * GEMMs (0 FP64, 1 emulated FP64 on the INT8 matrix cores where k >= 1024,
- * 2 FP32); panels and TRSMs are FP64. Returns info: 0, -i for an invalid
- * argument i (reported as DGETRF), or the first exactly-zero pivot (1-based).
- * Synchronizes the stream. */
+ * 2 FP32); panels and TRSMs are FP64. A panel is factored by one
+ * cooperative launch when all its workgroups fit on the device at once
+ * (HWYBLAS_AMD_LU_PANEL=steps selects one launch per column instead).
+ * Returns info: 0, -i for an invalid argument i (reported as DGETRF), the
+ * first exactly-zero pivot (1-based), or -1000 if a wait inside the
+ * cooperative panel kernel exceeded its bound (not expected; the factors
+ * are then invalid). Synchronizes the stream. */
 HWYBLAS_AMD_API int hwyblas_dgetrf_dev(int m, int n, double* a, int lda, int* ipiv,
                                        int update, void* stream);
 /* op(A) X = B with the factorization (src/lapack/getrf.cc:287-339); X
@@ -346,7 +350,8 @@ HWYBLAS_AMD_API int hwyblas_dgetrs_dev(char trans, int n, int nrhs, const double
  * componentwise backward error berr exceeds eps, halves at least, and fewer
  * than itmax corrections were made. Host outputs per right-hand side: berr,
  * nbe = ||B - A X||inf / (||A||inf ||X||inf), iterations = corrections.
- * Returns 0, getrf's info (> 0), or -i (reported as DGESVIR). Synchronizes. */
+ * Returns 0, getrf's info (> 0, or -1000 as there), or -i (reported as
+ * DGESVIR). Synchronizes. */
 HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int lda, double* af,
                                          int ldaf, int* ipiv, const double* b, int ldb,
                                          double* x, int ldx, int update, int itmax,

@@ -40,6 +40,40 @@ def test_getrf_matches_oracle(H, n):
     assert lu_ratio(a, lu, piv) < 30
 
 
+@pytest.mark.parametrize("m,n", [(1000, 1000), (2100, 700), (5000, 300), (90, 300), (129, 129)])
+def test_cooperative_panel_equals_column_steps(H, m, n, monkeypatch):
+    """The cooperative panel kernel (default) and the one-launch-per-column
+    panel give the same factors and pivots bit for bit, also with a zero
+    column, an exact tie between candidate rows of different workgroups and
+    a NaN."""
+    rng = np.random.default_rng(m + 3 * n)
+    a = rng.standard_normal((m, n))
+    a[:, min(n - 1, 40)] = 0.0
+    a[5, 3] = a[m - 2, 3] = 50.0  # tie: the smaller row wins
+    outs = []
+    for mode in (None, "steps"):
+        if mode:
+            monkeypatch.setenv("HWYBLAS_AMD_LU_PANEL", mode)
+        else:
+            monkeypatch.delenv("HWYBLAS_AMD_LU_PANEL", raising=False)
+        outs.append(gpu_getrf(H, a))
+    (lu0, p0, i0), (lu1, p1, i1) = outs
+    assert i0 == i1 and np.array_equal(p0, p1)
+    assert np.array_equal(lu0, lu1)
+    b = a.copy()
+    b[m // 2, 7] = np.nan
+    outs = []
+    for mode in (None, "steps"):
+        if mode:
+            monkeypatch.setenv("HWYBLAS_AMD_LU_PANEL", mode)
+        else:
+            monkeypatch.delenv("HWYBLAS_AMD_LU_PANEL", raising=False)
+        outs.append(gpu_getrf(H, b))
+    (lu0, p0, i0), (lu1, p1, i1) = outs
+    assert i0 == i1 and np.array_equal(p0, p1)
+    assert np.array_equal(lu0, lu1, equal_nan=True)
+
+
 @pytest.mark.parametrize("m,n", [(300, 100), (100, 300), (700, 129), (65, 64)])
 def test_getrf_rectangular(H, m, n):
     rng = np.random.default_rng(m * 7 + n)
