@@ -868,12 +868,51 @@ struct Perm {
   }
 };
 
+// The two triangular solves of getrs for few right-hand sides, blocked
+// right-looking by kPanel rows: a diagonal-block solve (substitution, as
+// the TRSM's) and one update of all remaining rows, a GEMM of depth kPanel
+// whose many row tiles keep the whole GPU busy (the recursive TRSM's deep
+// updates of depth n/2 have one tile column and stream A through few
+// workgroups).
+void SolveLowerUnitBlocked(int n, int nrhs, const double* a, int64_t lda, double* b, int64_t ldb,
+                           hipStream_t s) {
+  for (int i0 = 0; i0 < n; i0 += kPanel) {
+    const int r = std::min(kPanel, n - i0);
+    TrsmDevice<double>(true, true, false, true, r, nrhs, a + i0 + int64_t(i0) * lda, lda, b + i0,
+                       ldb, s);
+    if (i0 + r < n)
+      GemmDevice<double>(false, false, n - i0 - r, nrhs, r, -1.0,
+                         a + (i0 + r) + int64_t(i0) * lda, lda, b + i0, ldb, 1.0, b + i0 + r,
+                         ldb, s);
+  }
+}
+
+void SolveUpperBlocked(int n, int nrhs, const double* a, int64_t lda, double* b, int64_t ldb,
+                       hipStream_t s) {
+  const int nblk = (n + kPanel - 1) / kPanel;
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    const int i0 = blk * kPanel, r = std::min(kPanel, n - i0);
+    TrsmDevice<double>(true, false, false, false, r, nrhs, a + i0 + int64_t(i0) * lda, lda,
+                       b + i0, ldb, s);
+    if (i0 > 0)
+      GemmDevice<double>(false, false, i0, nrhs, r, -1.0, a + int64_t(i0) * lda, lda, b + i0, ldb,
+                         1.0, b, ldb, s);
+  }
+}
+
+constexpr int kBlockedSolveMaxRhs = 64;
+
 // xGETRS (src/lapack/getrf.cc:287-339) with the permutation of ipiv.
 void GetrsPerm(bool trans, int n, int nrhs, const double* a, int64_t lda, const Perm& perm,
                double* b, int64_t ldb, hipStream_t s) {
   if (n == 0 || nrhs == 0) return;
   if (!trans) {
     Permute(true, n, nrhs, b, ldb, perm.src, perm.inv, perm.scratch, s);
+    if (nrhs <= kBlockedSolveMaxRhs) {
+      SolveLowerUnitBlocked(n, nrhs, a, lda, b, ldb, s);
+      SolveUpperBlocked(n, nrhs, a, lda, b, ldb, s);
+      return;
+    }
     TrsmDevice<double>(true, true, false, true, n, nrhs, a, lda, b, ldb, s);
     TrsmDevice<double>(true, false, false, false, n, nrhs, a, lda, b, ldb, s);
   } else {

@@ -5,7 +5,8 @@
 // every off-diagonal update B2 -= T21 X1 (and its upper/right variants) is a
 // GEMM with alpha = -1, beta = 1 on the MFMA kernel, so almost all flops run
 // there. Diagonal blocks (order <= kNB) are solved by the wave-level solve
-// of trsm_chunk.h: one wave per 64 right-hand sides, blocks of 16 rows
+// of trsm_chunk.h (DiagNarrowLeftKernel for a left side with at most 256
+// right-hand sides): one wave per 64 right-hand sides, blocks of 16 rows
 // solved in LDS with x_j = (b_j - sum_q x_q t(q, j)) * (1 / t(j, j)) (the
 // dot form of hwyBLAS's base kernel, src/trsm-inl.h:107-174, reciprocal at
 // :69), the rows below updated on the MFMA units. Only the referenced
@@ -45,6 +46,87 @@ __global__ void __launch_bounds__(64)
   trsm_chunk::TrsmChunk<T>(d, int(blockIdx.x) * 64, lds);
 }
 
+// A left-side diagonal block (order r <= kNB) for a few right-hand sides:
+// a workgroup of eight waves takes kNarrowRhs right-hand sides, kWaveRhs per
+// wave; lane e holds solve row e of its wave's right-hand sides in
+// registers, and the block's triangle sits in LDS. Step i scales row i by
+// 1 / t(i, i) and broadcasts it, and every later row subtracts t(e, i) x_i
+// with one FMA per right-hand side; so x_i = (b_i - sum_{q<i} t(i, q) x_q)
+// (1 / t(i, i)) with the sum accumulated by FMAs in ascending q, the dot
+// form of hwyBLAS's base kernel (src/trsm-inl.h:107-174, reciprocal at
+// :69). B is read and written once; every load is issued before the first
+// is waited for.
+constexpr int kWaveRhs = 2;
+constexpr int kNarrowWaves = 8;
+constexpr int kNarrowRhs = kWaveRhs * kNarrowWaves;
+constexpr int kNarrowMaxRhs = 256;  // above this the MFMA chunk kernel wins
+
+// Lane i's value in every lane (i wave-uniform): v_readlane per 32 bits.
+template <typename T>
+__device__ __forceinline__ T Broadcast(T v, int i) {
+  if constexpr (sizeof(T) == 8) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(int(bits & 0xFFFFFFFFll), i);
+    const int hi = __builtin_amdgcn_readlane(int(bits >> 32), i);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                static_cast<unsigned int>(lo));
+  } else {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64 * kNarrowWaves)
+    DiagNarrowLeftKernel(trsm_chunk::TrsmDesc<T> d) {
+  __shared__ T st[kNB][kNB + 1];  // st[e][q] = L(e, q), q < e, in solve order
+  __shared__ T sinv[kNB];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = d.m, c0 = int(blockIdx.x) * kNarrowRhs + wave * kWaveRhs;
+  const int nvc = min(kWaveRhs, d.n - c0);  // may be <= 0 for the last waves
+  const bool asc = d.lower_solve;
+  auto phys = [&](int e) { return asc ? e : r - 1 - e; };
+  T x[kWaveRhs];
+  const int64_t row = lane < r ? phys(lane) : 0;
+#pragma unroll
+  for (int c = 0; c < kWaveRhs; ++c)
+    x[c] = (lane < r && c < nvc) ? d.b[row + int64_t(c0 + c) * d.ldb] : T(0);
+  // Wave w loads the triangle's columns q = w, w + 8, ... of its lane's row.
+  constexpr int kCols = kNB / kNarrowWaves;
+  T trow[kCols];
+#pragma unroll
+  for (int u = 0; u < kCols; ++u) {
+    const int q = wave + kNarrowWaves * u;
+    trow[u] = (lane < r && q < lane) ? trsm_chunk::SolveElem(d, phys(lane), phys(q)) : T(0);
+  }
+  if (wave == 0) {
+    const T diag =
+        (lane < r && !d.unit) ? trsm_chunk::SolveElem(d, phys(lane), phys(lane)) : T(1);
+    sinv[lane] = T(1) / diag;
+  }
+#pragma unroll
+  for (int u = 0; u < kCols; ++u) st[lane][wave + kNarrowWaves * u] = trow[u];
+  __syncthreads();
+  if (nvc <= 0) return;
+  for (int i = 0; i < r; ++i) {
+    const T inv = sinv[i];
+    if (lane == i) {
+#pragma unroll
+      for (int c = 0; c < kWaveRhs; ++c) x[c] = x[c] * inv;
+    }
+    const T t = (lane > i && lane < r) ? st[lane][i] : T(0);
+#pragma unroll
+    for (int c = 0; c < kWaveRhs; ++c) {
+      const T xi = Broadcast(x[c], i);
+      if (lane > i) x[c] = __builtin_fma(-t, xi, x[c]);
+    }
+  }
+  if (lane < r) {
+#pragma unroll
+    for (int c = 0; c < kWaveRhs; ++c)
+      if (c < nvc) d.b[row + int64_t(c0 + c) * d.ldb] = x[c];
+  }
+}
+
 template <typename T>
 void DiagSolve(bool left, const TriView<T>& t, int r, T* b, int64_t ldb,
                int64_t other, hipStream_t stream) {
@@ -61,6 +143,13 @@ void DiagSolve(bool left, const TriView<T>& t, int r, T* b, int64_t ldb,
   d.lower_solve = left ? t.lower : !t.lower;
   d.unit = t.unit;
   d.alpha = T(1);
+  if (left && other <= kNarrowMaxRhs) {
+    hipLaunchKernelGGL(DiagNarrowLeftKernel<T>,
+                       dim3(unsigned((other + kNarrowRhs - 1) / kNarrowRhs)),
+                       dim3(64 * kNarrowWaves), 0, stream, d);
+    HWY_HIP_CHECK(hipGetLastError());
+    return;
+  }
   const int nchunks = int((other + 63) / 64);
   hipLaunchKernelGGL(DiagChunkKernel<T>, dim3(unsigned(nchunks)), dim3(64), 0, stream, d,
                      nchunks);
