@@ -52,6 +52,8 @@ struct Crt {
   double inv_p[kMaxModuli];
   double frac[kMaxModuli][kLimbs];  // floor(2^160 y_l / p_l) in 40-bit limbs, low first
   double P;                           // prod p_l, rounded
+  double pow32[kMaxModuli];           // 2^32 mod p_l
+  double pow64[kMaxModuli];           // 2^64 mod p_l
 };
 
 // ---------------------------------------------------------------------------
@@ -76,6 +78,9 @@ Crt MakeCrt(int nmod) {
     c.p[l] = kModuli[l];
     c.inv_p[l] = 1.0 / kModuli[l];
     c.P *= kModuli[l];
+    const uint64_t p32 = (uint64_t(1) << 32) % uint64_t(kModuli[l]);
+    c.pow32[l] = double(p32);
+    c.pow64[l] = double((p32 * p32) % uint64_t(kModuli[l]));
     logp += std::log2(double(kModuli[l]));
   }
   (void)logp;
@@ -162,18 +167,78 @@ __device__ __forceinline__ int ResidueOf(double v, int p, double inv_p) {
 // residue mod p_l of round(x(r, p) 2^s_r), zero for r >= R or p >= K. Every
 // thread produces 8 consecutive bytes of one row per modulus (16 threads
 // cover a row's 128 bytes: whole cache lines).
+// An integer 2^52 <= |v| < 2^96 as exact 32-bit limbs l0 + l1 2^32 + l2 2^64
+// of |v|: its residue mod p is that of l0 + l1 (2^32 mod p) + l2 (2^64 mod p),
+// an exact FP64 integer below 2^41, reduced once per modulus.
+struct Limbs {
+  double l0, l1, l2;
+};
+__device__ __forceinline__ Limbs SplitLimbs(double v) {
+  const double a = fabs(v);
+  Limbs s;
+  s.l2 = floor(a * 0x1p-64);
+  const double a1 = a - s.l2 * 0x1p64;
+  s.l1 = floor(a1 * 0x1p-32);
+  s.l0 = a1 - s.l1 * 0x1p32;
+  return s;
+}
+__device__ __forceinline__ int LimbResidue(const Limbs& s, bool neg, int p, double inv_p,
+                                           double pow32, double pow64) {
+  const double t = __builtin_fma(s.l2, pow64, __builtin_fma(s.l1, pow32, s.l0));
+  const double q = floor(t * inv_p);
+  double rr = __builtin_fma(-q, double(p), t);
+  if (rr < 0.0) rr += p;
+  if (rr >= p) rr -= p;
+  int r = int(rr);
+  if (neg && r != 0) r = p - r;
+  if (r >= (p >> 1)) r -= p;
+  return r;
+}
+
 __device__ __forceinline__ void SliceEight(const double (&x)[8], int sc, const Crt& crt,
                                            int8_t* dst, int64_t plane) {
   double v[8];
+  bool any_big = false;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) v[q] = ldexp(x[q], sc);
+  for (int q = 0; q < 8; ++q) {
+    v[q] = ldexp(x[q], sc);
+    any_big |= !(fabs(v[q]) < 0x1p52);
+  }
+  if (!any_big) {
+    for (int l = 0; l < crt.nmod; ++l) {
+      const int p = crt.p[l];
+      const double ip = crt.inv_p[l];
+      uint64_t word = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int res = (v[q] == 0.0) ? 0 : SymMod(rint(v[q]), p, ip);
+        word |= uint64_t(uint8_t(int8_t(res))) << (8 * q);
+      }
+      *reinterpret_cast<uint64_t*>(dst + l * plane) = word;
+    }
+    return;
+  }
+  // Some |v| >= 2^52 (many moduli): 32-bit limbs for 2^52 <= |v| < 2^96, the
+  // general path beyond.
+  int kind[8];
+  Limbs limbs[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double a = fabs(v[q]);
+    kind[q] = a < 0x1p52 ? 0 : (a < 0x1p96 ? 1 : 2);
+    limbs[q] = SplitLimbs(kind[q] == 1 ? v[q] : 0.0);
+  }
   for (int l = 0; l < crt.nmod; ++l) {
     const int p = crt.p[l];
     const double ip = crt.inv_p[l];
     uint64_t word = 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const int res = (v[q] == 0.0) ? 0 : ResidueOf(v[q], p, ip);
+      const int res = (v[q] == 0.0) ? 0
+                      : kind[q] == 0 ? SymMod(rint(v[q]), p, ip)
+                      : kind[q] == 1 ? LimbResidue(limbs[q], v[q] < 0.0, p, ip, crt.pow32[l],
+                                                   crt.pow64[l])
+                                     : ResidueOf(v[q], p, ip);
       word |= uint64_t(uint8_t(int8_t(res))) << (8 * q);
     }
     *reinterpret_cast<uint64_t*>(dst + l * plane) = word;
@@ -363,7 +428,8 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmKernel(ModGemmA
 // swz). The waves synchronise with s_barrier after waiting (vmcnt) for their
 // own DMA of the slice about to be read; the slot refilled after the barrier
 // is the one every wave finished reading in the previous iteration.
-template <int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES>
+template <int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES, bool PRIO = false,
+          int LATE = 0>
 __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmDmaKernel(ModGemmArgs g) {
   constexpr int NT = 64 * WAVES_M * WAVES_N;
   constexpr int NW = WAVES_M * WAVES_N;
@@ -403,11 +469,17 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmDmaKernel(ModGe
     const int row = P / CH, c = (P % CH) ^ ((row / RPB) % CH);
     b_off[q] = int64_t(row) * g.kp + c * 16;
   }
-  auto issue = [&](int stage, int64_t k0) {
+  auto issue_a = [&](int stage, int64_t k0) {
 #pragma unroll
     for (int q = 0; q < A_DMA; ++q) DmaToLds(A + a_off[q] + k0, &As[stage][(q * NW + wave) * 64]);
+  };
+  auto issue_b = [&](int stage, int64_t k0) {
 #pragma unroll
     for (int q = 0; q < B_DMA; ++q) DmaToLds(B + b_off[q] + k0, &Bs[stage][(q * NW + wave) * 64]);
+  };
+  auto issue = [&](int stage, int64_t k0) {
+    issue_a(stage, k0);
+    issue_b(stage, k0);
   };
   auto swz = [](int row, int c) { return row * CH + (c ^ ((row / RPB) % CH)); };
 
@@ -428,10 +500,19 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmDmaKernel(ModGe
     WaitSlices<PER_SLICE>(later);
     BarrierNoFence();
     const int next = kt + STAGES - 1;
-    if (next < ktiles) issue(next % STAGES, int64_t(next) * BK);
+    if (LATE == 0 && next < ktiles) issue(next % STAGES, int64_t(next) * BK);
     const int cur = kt % STAGES;
+    if (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int ks = 0; ks < BK / 32; ++ks) {
+      // (LATE = 1: the next slice's loads go out after the first step's
+      // MFMAs, not in one burst with every wave's first LDS reads; 2: after
+      // the second; 3: A's after the first, B's after the second.)
+      if (next < ktiles) {
+        if ((LATE == 1 && ks == 1) || (LATE == 2 && ks == 2)) issue(next % STAGES, int64_t(next) * BK);
+        if (LATE == 3 && ks == 1) issue_a(next % STAGES, int64_t(next) * BK);
+        if (LATE == 3 && ks == 2) issue_b(next % STAGES, int64_t(next) * BK);
+      }
       const int c = ks * 2 + lh;
       i32x4 af[TM], bf[TN];
 #pragma unroll
@@ -444,6 +525,7 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmDmaKernel(ModGe
         for (int tj = 0; tj < TN; ++tj)
           acc[ti][tj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
     }
+    if (PRIO) __builtin_amdgcn_s_setprio(0);
   }
 
   const int p = g.p[l];
@@ -665,15 +747,20 @@ constexpr int kTileM = 256, kTileN = 256, kTileK = 64;
 constexpr int kPadK = 128;
 
 // INT8 GEMM variant (HWYBLAS_AMD_INT8_KERNEL). All stream the operand slices
-// and give the same bits; they differ in schedule. 3 (default, fastest on
+// and give the same bits; they differ in schedule. 9 (default, fastest on
 // MI355X at m = n = k = 16384): DMA into LDS, 8 waves of 128 x 64, k in
-// steps of 128 through two LDS stages. 0: the same with k steps of 64 and
-// four stages; 1: slices staged through registers; 2: 4 waves of 128 x 128;
-// 4: three stages of 64; 5: fragments double-buffered in registers.
+// steps of 128 through two LDS stages, the next slice's A loads issued
+// after the first MFMA step and its B loads after the second (not in one
+// burst with every wave's first LDS reads). 3: the same with all loads
+// issued right after the barrier; 0: k steps of 64 and four stages; 1:
+// slices staged through registers; 2: 4 waves of 128 x 128; 4: three stages
+// of 64; 5: fragments double-buffered in registers; 6: as 3 with the waves'
+// priority raised while they issue MFMAs; 7: loads after the first MFMA
+// step; 8: after the second.
 int ModGemmVariant() {
   static const int v = [] {
     const char* e = getenv("HWYBLAS_AMD_INT8_KERNEL");
-    return (e && *e >= '0' && *e <= '9') ? *e - '0' : 3;
+    return (e && *e >= '0' && *e <= '9') ? *e - '0' : 9;
   }();
   return v;
 }
@@ -890,10 +977,26 @@ int EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alph
       hipLaunchKernelGGL((ModGemmPipeKernel<kTileM, kTileN, kTileK, 2, 4, 4>), grid, dim3(512), 0,
                          stream, g);
       break;
+    case 6:
+      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2, true>), grid, dim3(512),
+                         0, stream, g);
+      break;
+    case 7:
+      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2, false, 1>), grid,
+                         dim3(512), 0, stream, g);
+      break;
+    case 8:
+      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2, false, 2>), grid,
+                         dim3(512), 0, stream, g);
+      break;
 
-    default:  // 3
+    case 3:
       hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2>), grid, dim3(512), 0,
                          stream, g);
+      break;
+    default:  // 9
+      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2, false, 3>), grid,
+                         dim3(512), 0, stream, g);
   }
   HWY_HIP_CHECK(hipGetLastError());
 
