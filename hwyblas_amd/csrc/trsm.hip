@@ -4,13 +4,12 @@
 // multiples of the base size until a diagonal block has at most kNB rows;
 // every off-diagonal update B2 -= T21 X1 (and its upper/right variants) is a
 // GEMM with alpha = -1, beta = 1 on the MFMA kernel, so almost all flops run
-// there. Diagonal blocks (order <= kNB) are solved by the wave-level solve
-// of trsm_chunk.h (DiagNarrowLeftKernel for a left side with at most 256
-// right-hand sides): one wave per 64 right-hand sides, blocks of 16 rows
-// solved in LDS with x_j = (b_j - sum_q x_q t(q, j)) * (1 / t(j, j)) (the
-// dot form of hwyBLAS's base kernel, src/trsm-inl.h:107-174, reciprocal at
-// :69), the rows below updated on the MFMA units. Only the referenced
-// triangle is read; a unit diagonal is never read.
+// there. Diagonal blocks (order <= kNB) are solved by DiagNarrowKernel: the
+// block's rows across a wave's lanes, a few right-hand sides per wave in
+// registers, x_j = (b_j - sum_q t(j, q) x_q) * (1 / t(j, j)) with the sum as
+// FMAs in ascending q (the dot form of hwyBLAS's base kernel,
+// src/trsm-inl.h:107-174, reciprocal at :69). Only the referenced triangle
+// is read; a unit diagonal is never read.
 #include <algorithm>
 
 #include "common.h"
@@ -36,20 +35,10 @@ TriView<T> SubView(const TriView<T>& t, int64_t i0, int64_t j0) {
   return s;
 }
 
-// Solves a diagonal block (order r <= kNB) for `other` right-hand sides:
-// one single-wave workgroup per 64 of them, so the chunks spread over CUs.
-template <typename T>
-__global__ void __launch_bounds__(64)
-    DiagChunkKernel(trsm_chunk::TrsmDesc<T> d, int nchunks) {
-  __shared__ trsm_chunk::TrsmWaveLds<T> lds;
-  if (int(blockIdx.x) >= nchunks) return;
-  trsm_chunk::TrsmChunk<T>(d, int(blockIdx.x) * 64, lds);
-}
-
-// A left-side diagonal block (order r <= kNB) for a few right-hand sides:
-// a workgroup of eight waves takes kNarrowRhs right-hand sides, kWaveRhs per
-// wave; lane e holds solve row e of its wave's right-hand sides in
-// registers, and the block's triangle sits in LDS. Step i scales row i by
+// A diagonal block (order r <= kNB): a workgroup of eight waves takes
+// kNarrowRhs right-hand sides (columns of B for the left side, rows for the
+// right side), kWaveRhs per wave; lane e holds solve row e of its wave's
+// right-hand sides in registers, and the block's triangle sits in LDS. Step i scales row i by
 // 1 / t(i, i) and broadcasts it, and every later row subtracts t(e, i) x_i
 // with one FMA per right-hand side; so x_i = (b_i - sum_{q<i} t(i, q) x_q)
 // (1 / t(i, i)) with the sum accumulated by FMAs in ascending q, the dot
@@ -59,7 +48,6 @@ __global__ void __launch_bounds__(64)
 constexpr int kWaveRhs = 2;
 constexpr int kNarrowWaves = 8;
 constexpr int kNarrowRhs = kWaveRhs * kNarrowWaves;
-constexpr int kNarrowMaxRhs = 256;  // above this the MFMA chunk kernel wins
 
 // Lane i's value in every lane (i wave-uniform): v_readlane per 32 bits.
 template <typename T>
@@ -75,21 +63,26 @@ __device__ __forceinline__ T Broadcast(T v, int i) {
   }
 }
 
-template <typename T>
+template <typename T, bool LEFT>
 __global__ void __launch_bounds__(64 * kNarrowWaves)
-    DiagNarrowLeftKernel(trsm_chunk::TrsmDesc<T> d) {
+    DiagNarrowKernel(trsm_chunk::TrsmDesc<T> d) {
   __shared__ T st[kNB][kNB + 1];  // st[e][q] = L(e, q), q < e, in solve order
   __shared__ T sinv[kNB];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int r = d.m, c0 = int(blockIdx.x) * kNarrowRhs + wave * kWaveRhs;
-  const int nvc = min(kWaveRhs, d.n - c0);  // may be <= 0 for the last waves
+  const int r = LEFT ? d.m : d.n;      // the system's order
+  const int nrhs = LEFT ? d.n : d.m;   // right-hand sides: columns (left) or rows of B
+  const int c0 = int(blockIdx.x) * kNarrowRhs + wave * kWaveRhs;
+  const int nvc = min(kWaveRhs, nrhs - c0);  // may be <= 0 for the last waves
   const bool asc = d.lower_solve;
   auto phys = [&](int e) { return asc ? e : r - 1 - e; };
   T x[kWaveRhs];
-  const int64_t row = lane < r ? phys(lane) : 0;
+  const int64_t e_phys = lane < r ? phys(lane) : 0;
+  // Element (solve row e, right-hand side c) of B.
+  auto at = [&](int c) -> T* {
+    return LEFT ? d.b + e_phys + int64_t(c0 + c) * d.ldb : d.b + (c0 + c) + e_phys * d.ldb;
+  };
 #pragma unroll
-  for (int c = 0; c < kWaveRhs; ++c)
-    x[c] = (lane < r && c < nvc) ? d.b[row + int64_t(c0 + c) * d.ldb] : T(0);
+  for (int c = 0; c < kWaveRhs; ++c) x[c] = (lane < r && c < nvc) ? *at(c) : T(0);
   // Wave w loads the triangle's columns q = w, w + 8, ... of its lane's row.
   constexpr int kCols = kNB / kNarrowWaves;
   T trow[kCols];
@@ -123,7 +116,7 @@ __global__ void __launch_bounds__(64 * kNarrowWaves)
   if (lane < r) {
 #pragma unroll
     for (int c = 0; c < kWaveRhs; ++c)
-      if (c < nvc) d.b[row + int64_t(c0 + c) * d.ldb] = x[c];
+      if (c < nvc) *at(c) = x[c];
   }
 }
 
@@ -143,16 +136,12 @@ void DiagSolve(bool left, const TriView<T>& t, int r, T* b, int64_t ldb,
   d.lower_solve = left ? t.lower : !t.lower;
   d.unit = t.unit;
   d.alpha = T(1);
-  if (left && other <= kNarrowMaxRhs) {
-    hipLaunchKernelGGL(DiagNarrowLeftKernel<T>,
-                       dim3(unsigned((other + kNarrowRhs - 1) / kNarrowRhs)),
-                       dim3(64 * kNarrowWaves), 0, stream, d);
-    HWY_HIP_CHECK(hipGetLastError());
-    return;
-  }
-  const int nchunks = int((other + 63) / 64);
-  hipLaunchKernelGGL(DiagChunkKernel<T>, dim3(unsigned(nchunks)), dim3(64), 0, stream, d,
-                     nchunks);
+  const dim3 grid(unsigned((other + kNarrowRhs - 1) / kNarrowRhs));
+  if (left)
+    hipLaunchKernelGGL((DiagNarrowKernel<T, true>), grid, dim3(64 * kNarrowWaves), 0, stream, d);
+  else
+    hipLaunchKernelGGL((DiagNarrowKernel<T, false>), grid, dim3(64 * kNarrowWaves), 0, stream,
+                       d);
   HWY_HIP_CHECK(hipGetLastError());
 }
 
