@@ -260,7 +260,17 @@ __global__ void __launch_bounds__(64)
   if (c >= ncols) return;
   const int total = naff;
   double* col = full + int64_t(c) * lda;
-  for (int x = 0; x < total; ++x) buf[x][lane] = col[S[x]];
+  // Loads in batches of 16 in flight (each lane's rows are scattered: one
+  // cache line each), then the stores.
+  constexpr int kBatch = 16;
+  for (int x0 = 0; x0 < total; x0 += kBatch) {
+    double v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = (x0 + u < total) ? col[S[x0 + u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (x0 + u < total) buf[x0 + u][lane] = v[u];
+  }
   for (int x = 0; x < total; ++x) col[L[x]] = buf[x][lane];
 }
 
@@ -333,7 +343,14 @@ __global__ void __launch_bounds__(64)
                           double* __restrict__ inv) {
   __shared__ double l[kPanel][kPanel + 1];
   const int j = threadIdx.x;
-  for (int i = 0; i < kPanel; ++i) l[i][j] = (j < i && i < r) ? a[i + int64_t(j) * lda] : 0.0;
+  {
+    // Column j of the triangle, every load in flight before the first store.
+    double colv[kPanel];
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) colv[i] = (j < i && i < r) ? a[i + int64_t(j) * lda] : 0.0;
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) l[i][j] = colv[i];
+  }
   __syncthreads();
   double x[kPanel];
 #pragma unroll
@@ -385,6 +402,7 @@ struct PersistArgs {
   double* rec;      // [2][gridDim.x + 1][kRecord]; slot gridDim.x: row j + 1
   unsigned* bar;    // arrivals (0 at launch)
   unsigned* failed;
+  double* linv;     // the inverse of the panel's unit lower triangle (kPanel x kPanel)
 };
 
 // Arrive, then wait until `target` arrivals in all. Every thread's record
@@ -619,6 +637,42 @@ __global__ void __launch_bounds__(kPersistThreads) PanelPersistentKernel(Persist
       if (c < nb) g.p[rows[q] + int64_t(c) * lda] = v[q][t];
     }
   }
+
+  // Workgroup 0 holds the final unit lower triangle L11 (rows and columns
+  // 0 .. mn - 1; rows 0 .. 63 are its waves' q = 0, half = 0 rows) and
+  // inverts it for the TRSMs with it: L11 X = I with lane e holding row e
+  // of eight right-hand sides per wave, x_e -= L(e, i) x_i in ascending i
+  // (the FMA order of InvertUnitLowerKernel's forward substitution).
+  if (gid != 0) return;
+  __syncthreads();
+  double (*st)[kPanel + 1] = reinterpret_cast<double (*)[kPanel + 1]>(&srec[0][0]);
+  if (half == 0) {
+#pragma unroll
+    for (int t = 0; t < kGroupCols; ++t) st[lane][c0 + t] = v[0][t];
+  }
+  __syncthreads();
+  constexpr int kInvRhs = kPanel / (kPersistThreads / 64);  // 8 per wave
+  double x[kInvRhs];
+  const int cb = wave * kInvRhs;
+#pragma unroll
+  for (int u = 0; u < kInvRhs; ++u) x[u] = (lane == cb + u) ? 1.0 : 0.0;
+  for (int i = 0; i < mn; ++i) {
+    const double t = (lane > i && lane < mn) ? st[lane][i] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kInvRhs; ++u) {
+      const long long bits = __double_as_longlong(x[u]);
+      const int lo = __builtin_amdgcn_readlane(int(bits & 0xFFFFFFFFll), i);
+      const int hi = __builtin_amdgcn_readlane(int(bits >> 32), i);
+      const double xi =
+          __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+      if (lane > i && lane < mn) x[u] = __builtin_fma(-t, xi, x[u]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kInvRhs; ++u) {
+    const int col = cb + u;
+    g.linv[lane + col * kPanel] = (lane < mn && col < mn) ? x[u] : 0.0;
+  }
 }
 
 struct Work {
@@ -658,9 +712,12 @@ void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, 
   const int mn = std::min(m, nb);
   double* full = w.base + row0;
   const int groups = (m + kPersistRows - 1) / kPersistRows;
-  if (groups <= w.persist_max_groups) {
+  const bool persistent = groups <= w.persist_max_groups;
+  if (persistent) {
     hipLaunchKernelGGL(FillUnsigned, dim3(1), dim3(64), 0, s, w.sync, 1, 0u);
-    PersistArgs g{a, lda, m, nb, row0, ipiv, w.info, sfmin, w.prec, w.sync, w.sync + 64};
+    PersistArgs g{a,          lda,  m,          nb,      row0,
+                  ipiv,       w.info, sfmin,    w.prec,  w.sync,
+                  w.sync + 64, w.linv + int64_t(row0 / kPanel) * kPanel * kPanel};
     void* args[] = {&g};
     HWY_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(PanelPersistentKernel),
                                              dim3(groups), dim3(kPersistThreads), args, 0, s));
@@ -676,9 +733,11 @@ void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, 
     hipLaunchKernelGGL(PanelSwapKernel, dim3((others + 63) / 64), dim3(64), 0, s, full, lda,
                        w.ncols, pc0, nb, ipiv, mn);
   // The inverse of the panel's unit lower triangle (final from here on: later
-  // interchanges only move rows below it), for the TRSMs with it.
-  hipLaunchKernelGGL(InvertUnitLowerKernel, dim3(1), dim3(kPanel), 0, s, a, lda, mn,
-                     w.linv + int64_t(row0 / kPanel) * kPanel * kPanel);
+  // interchanges only move rows below it), for the TRSMs with it; the
+  // cooperative panel kernel has already written it.
+  if (!persistent)
+    hipLaunchKernelGGL(InvertUnitLowerKernel, dim3(1), dim3(kPanel), 0, s, a, lda, mn,
+                       w.linv + int64_t(row0 / kPanel) * kPanel * kPanel);
   HWY_HIP_CHECK(hipGetLastError());
 }
 
