@@ -40,6 +40,7 @@
 #include <vector>
 
 #include "common.h"
+#include "gemm_kernel.h"
 
 namespace hwy_amd {
 namespace ozaki {
@@ -675,11 +676,56 @@ __global__ void __launch_bounds__(kPersistThreads) PanelPersistentKernel(Persist
   }
 }
 
+// B (r x n2, ldb) := inv B in place for a kPanel x kPanel inverse (zero
+// beyond r): workgroup g stages inv and its 64 columns of B in LDS (every
+// load issued before the first wait) and multiplies on the FP64 MFMA
+// (v_mfma_f64_16x16x4: wave w owns output rows 16 w .. 16 w + 15 of all four
+// column tiles). Each workgroup reads all of its columns before it writes.
+__global__ void __launch_bounds__(256)
+    InvMulKernel(const double* __restrict__ inv, int r, double* __restrict__ b, int64_t ldb,
+                 int n2) {
+  __shared__ double si[kPanel][kPanel + 1];
+  __shared__ double sb[kPanel][kPanel + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int j0 = blockIdx.x * kPanel;
+  constexpr int kPer = kPanel * kPanel / 256;
+  double ti[kPer], tb[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int idx = tid + 256 * u, i = idx % kPanel, c = idx / kPanel;
+    ti[u] = inv[i + c * kPanel];
+    tb[u] = (i < r && j0 + c < n2) ? b[i + int64_t(j0 + c) * ldb] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int idx = tid + 256 * u, i = idx % kPanel, c = idx / kPanel;
+    si[i][c] = ti[u];
+    sb[i][c] = tb[u];
+  }
+  __syncthreads();
+  using M = gemm_detail::Mfma<double>;
+  M::Acc acc[4] = {};
+  const int li = lane & 15, lk = lane >> 4;
+  for (int k0 = 0; k0 < kPanel; k0 += 4) {
+    const double a = si[16 * wave + li][k0 + lk];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) acc[ct] = M::Run(a, sb[k0 + lk][16 * ct + li], acc[ct]);
+  }
+#pragma unroll
+  for (int ct = 0; ct < 4; ++ct) {
+    const int col = j0 + 16 * ct + li;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int row = 16 * wave + M::Row(lk, rr);
+      if (row < r && col < n2) b[row + int64_t(col) * ldb] = acc[ct][rr];
+    }
+  }
+}
+
 struct Work {
   double* base = nullptr;  // the whole matrix
   int ncols = 0;
   double* linv = nullptr;  // inverses of the panels' unit lower triangles
-  double* temp = nullptr;  // kPanel x ncols
   Candidate* cand = nullptr;
   unsigned* counter = nullptr;
   int* info = nullptr;
@@ -743,17 +789,17 @@ void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, 
 
 // B (n1 x n2, ldb) := L^-1 B for the unit lower triangle L (n1 x n1) of the
 // factorization starting at matrix row g0 (a multiple of kPanel): the
-// recursion of the TRSM, with the diagonal blocks applied as GEMMs with the
-// inverses the panels left (rounding as X = inv(L_jj) B_j, the usual form of
-// a GPU LU's triangular solves).
+// recursion of the TRSM, with the diagonal blocks applied in place by
+// InvMulKernel with the inverses the panels left (rounding as
+// X = inv(L_jj) B_j, the usual form of a GPU LU's triangular solves).
 void TrsmUnitLower(const double* l, int64_t lda, int n1, double* b, int64_t ldb, int n2, int g0,
                    Work& w, hipStream_t s) {
   if (n1 <= 0 || n2 <= 0) return;
   if (n1 <= kPanel) {
     const double* inv = w.linv + int64_t(g0 / kPanel) * kPanel * kPanel;
-    GemmDevice<double>(false, false, n1, n2, n1, 1.0, inv, kPanel, b, ldb, 0.0, w.temp, n1, s);
-    HWY_HIP_CHECK(hipMemcpy2DAsync(b, size_t(ldb) * 8, w.temp, size_t(n1) * 8, size_t(n1) * 8,
-                                   n2, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(InvMulKernel, dim3(unsigned((n2 + kPanel - 1) / kPanel)), dim3(256), 0, s,
+                       inv, n1, b, ldb, n2);
+    HWY_HIP_CHECK(hipGetLastError());
     return;
   }
   const int r1 = std::max(kPanel, (n1 / 2) / kPanel * kPanel);
@@ -845,7 +891,6 @@ int Getrf(int m, int n, double* a, int64_t lda, int* ipiv, int update, hipStream
   HWY_HIP_CHECK(hipMalloc(&w.cand, size_t(max_blocks) * sizeof(Candidate)));
   const int panels = (std::min(m, n) + kPanel - 1) / kPanel;
   HWY_HIP_CHECK(hipMalloc(&w.linv, size_t(panels) * kPanel * kPanel * sizeof(double)));
-  HWY_HIP_CHECK(hipMalloc(&w.temp, size_t(kPanel) * n * sizeof(double)));
   HWY_HIP_CHECK(hipMalloc(&w.counter, 256));
   HWY_HIP_CHECK(hipMalloc(&w.info, 256));
   hipLaunchKernelGGL(FillUnsigned, dim3(1), dim3(64), 0, s, w.counter, 1, 0u);
@@ -871,7 +916,7 @@ int Getrf(int m, int n, double* a, int64_t lda, int* ipiv, int update, hipStream
   HWY_HIP_CHECK(hipStreamSynchronize(s));
   for (void* p : {static_cast<void*>(w.cand), static_cast<void*>(w.counter),
                   static_cast<void*>(w.info), static_cast<void*>(w.f32),
-                  static_cast<void*>(w.linv), static_cast<void*>(w.temp),
+                  static_cast<void*>(w.linv),
                   static_cast<void*>(w.prec), static_cast<void*>(w.sync)})
     if (p) HWY_HIP_CHECK(hipFree(p));
   if (failed) {
