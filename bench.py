@@ -543,8 +543,9 @@ def lu_leg(H, n, kappas, stream, nrhs=16, reps=2):
     """Mixed-precision LU solve: A = Q1 diag(sigma) Q2^T (sigma geometric from 1
     to 10^kappa, Q1/Q2 Stewart-random orthogonal, seeds 31/32), B uniform[-1,1]
     (seed 33), nrhs right-hand sides. For each trailing-update precision:
-    getrf + getrs + FP64 refinement, timed end to end; beside it the all-FP64
-    getrf + getrs without refinement."""
+    getrf + getrs + FP64 refinement + the forward error bound, timed end to
+    end (and without the bound); beside it the all-FP64 getrf + getrs
+    without refinement."""
     L = H.lib()
     out = {}
     for kappa in kappas:
@@ -565,22 +566,31 @@ def lu_leg(H, n, kappas, stream, nrhs=16, reps=2):
         res = {"generation_s": round(gen_s, 1)}
         dAF, dX = H.DeviceArray(n, n), H.DeviceArray(n, nrhs)
         dP = H.DeviceArray(n, 1, np.int32)
-        berr, nbe = np.zeros(nrhs), np.zeros(nrhs)
+        ferr, berr, nbe = np.zeros(nrhs), np.zeros(nrhs), np.zeros(nrhs)
         iters = np.zeros(nrhs, dtype=np.int32)
         for update, name in ((0, "fp64"), (1, "emulated"), (2, "fp32")):
-            best = None
-            for _ in range(reps):
-                t0 = time.perf_counter()
-                info = L.hwyblas_dgesv_ir_dev(n, nrhs, dA.ptr, n, dAF.ptr, n, dP.ptr, dB.ptr, n,
-                                              dX.ptr, n, update, 5, berr.ctypes.data,
-                                              nbe.ctypes.data, iters.ctypes.data, stream)
-                dt = time.perf_counter() - t0
-                best = dt if best is None else min(best, dt)
+            times = {}
+            for with_ferr in (False, True):  # gerfs computes the forward error bound too
+                best = None
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    info = L.hwyblas_dgesv_ir_dev(
+                        n, nrhs, dA.ptr, n, dAF.ptr, n, dP.ptr, dB.ptr, n, dX.ptr, n, update, 5,
+                        ferr.ctypes.data if with_ferr else None, berr.ctypes.data,
+                        nbe.ctypes.data, iters.ctypes.data, stream)
+                    dt = time.perf_counter() - t0
+                    best = dt if best is None else min(best, dt)
+                times[with_ferr] = best
+            best = times[True]
             res[name] = {"ms": round(best * 1e3, 1), "solves_per_s": round(1.0 / best, 3),
+                         "ms_without_ferr": round(times[False] * 1e3, 1),
                          "berr_max": float(berr.max()), "nbe_max": float(nbe.max()),
+                         "ferr_max": float(ferr.max()),
                          "iterations": [int(iters.min()), int(iters.max())], "info": int(info)}
-            log(f"lu n={n} kappa=1e{kappa} {name}: {best * 1e3:.1f} ms, berr {berr.max():.2e}, "
-                f"nbe {nbe.max():.2e}, corrections {iters.min()}-{iters.max()}")
+            log(f"lu n={n} kappa=1e{kappa} {name}: {best * 1e3:.1f} ms with ferr "
+                f"({times[False] * 1e3:.1f} without), berr {berr.max():.2e}, "
+                f"nbe {nbe.max():.2e}, ferr {ferr.max():.2e}, "
+                f"corrections {iters.min()}-{iters.max()}")
         # All-FP64 getrf + getrs, no refinement.
         dF = H.DeviceArray(n, n)
         X = H.DeviceArray(n, nrhs)

@@ -97,7 +97,7 @@ def _declare(L):
     fn("hwyblas_dgetrf_dev", _i, [_i, _i, _p, _i, _p, _i, _p])
     fn("hwyblas_dgetrs_dev", _i, [ctypes.c_char, _i, _i, _p, _i, _p, _p, _i, _p])
     fn("hwyblas_dgesv_ir_dev", _i, [_i, _i, _p, _i, _p, _i, _p, _p, _i, _p, _i, _i, _i, _p, _p,
-                                    _p, _p])
+                                    _p, _p, _p])
     fn("hwyblas_gemm_config_count", _i, [])
     fn("hwyblas_gemm_config_info", _i, [_i, _pi, _pi, _pi, _pi])
     fn("hwyblas_gemm_default_config", _i, [_i, _c, _c, _i, _i, _i])
@@ -450,17 +450,26 @@ def getrf_dev(a, update=0, stream=None):
     return ipiv, info
 
 
-def gesv_ir_dev(a, b, update=0, itmax=5, stream=None):
+def gesv_ir_dev(a, b, update=0, itmax=5, stream=None, with_ferr=False, factors=False):
     """Mixed-precision solve of A X = B (device arrays); returns
-    (X DeviceArray, berr, nbe, iterations, info)."""
+    (X DeviceArray, berr, nbe, iterations, info), with ferr (the forward
+    error bound per column) appended when with_ferr, and the factors
+    (af DeviceArray, ipiv DeviceArray) appended when factors."""
     n, nrhs = a.ld, b.cols
     af = DeviceArray(n, n)
     x = DeviceArray(n, nrhs)
     ipiv = DeviceArray(max(1, n), 1, np.int32)
-    berr = np.zeros(nrhs)
-    nbe = np.zeros(nrhs)
-    iters = np.zeros(nrhs, dtype=np.int32)
+    ferr = np.zeros(max(1, nrhs))
+    berr = np.zeros(max(1, nrhs))
+    nbe = np.zeros(max(1, nrhs))
+    iters = np.zeros(max(1, nrhs), dtype=np.int32)
     info = lib().hwyblas_dgesv_ir_dev(n, nrhs, a.ptr, a.ld, af.ptr, n, ipiv.ptr, b.ptr, b.ld,
-                                      x.ptr, n, update, itmax, berr.ctypes.data,
+                                      x.ptr, n, update, itmax,
+                                      ferr.ctypes.data if with_ferr else None, berr.ctypes.data,
                                       nbe.ctypes.data, iters.ctypes.data, stream)
-    return x, berr, nbe, iters, info
+    out = (x, berr[:nrhs], nbe[:nrhs], iters[:nrhs], info)
+    if with_ferr:
+        out += (ferr[:nrhs],)
+    if factors:
+        out += (af, ipiv)
+    return out

@@ -1004,6 +1004,38 @@ void SolveUpperBlocked(int n, int nrhs, const double* a, int64_t lda, double* b,
   }
 }
 
+// op(U) = U^T is lower triangular: blocks go forward, and the update of
+// the rows below a block reads the block's rows of U right of the
+// diagonal, transposed.
+void SolveUpperTransBlocked(int n, int nrhs, const double* a, int64_t lda, double* b,
+                            int64_t ldb, hipStream_t s) {
+  for (int i0 = 0; i0 < n; i0 += kPanel) {
+    const int r = std::min(kPanel, n - i0);
+    TrsmDevice<double>(true, false, true, false, r, nrhs, a + i0 + int64_t(i0) * lda, lda,
+                       b + i0, ldb, s);
+    if (i0 + r < n)
+      GemmDevice<double>(true, false, n - i0 - r, nrhs, r, -1.0,
+                         a + i0 + int64_t(i0 + r) * lda, lda, b + i0, ldb, 1.0, b + i0 + r,
+                         ldb, s);
+  }
+}
+
+// op(L) = L^T is unit upper triangular: blocks go backward, and the update
+// of the rows above a block reads the block's rows of L left of the
+// diagonal, transposed.
+void SolveLowerUnitTransBlocked(int n, int nrhs, const double* a, int64_t lda, double* b,
+                                int64_t ldb, hipStream_t s) {
+  const int nblk = (n + kPanel - 1) / kPanel;
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    const int i0 = blk * kPanel, r = std::min(kPanel, n - i0);
+    TrsmDevice<double>(true, true, true, true, r, nrhs, a + i0 + int64_t(i0) * lda, lda, b + i0,
+                       ldb, s);
+    if (i0 > 0)
+      GemmDevice<double>(true, false, i0, nrhs, r, -1.0, a + i0, lda, b + i0, ldb, 1.0, b, ldb,
+                         s);
+  }
+}
+
 constexpr int kBlockedSolveMaxRhs = 64;
 
 // xGETRS (src/lapack/getrf.cc:287-339) with the permutation of ipiv.
@@ -1020,8 +1052,13 @@ void GetrsPerm(bool trans, int n, int nrhs, const double* a, int64_t lda, const 
     TrsmDevice<double>(true, true, false, true, n, nrhs, a, lda, b, ldb, s);
     TrsmDevice<double>(true, false, false, false, n, nrhs, a, lda, b, ldb, s);
   } else {
-    TrsmDevice<double>(true, false, true, false, n, nrhs, a, lda, b, ldb, s);
-    TrsmDevice<double>(true, true, true, true, n, nrhs, a, lda, b, ldb, s);
+    if (nrhs <= kBlockedSolveMaxRhs) {
+      SolveUpperTransBlocked(n, nrhs, a, lda, b, ldb, s);
+      SolveLowerUnitTransBlocked(n, nrhs, a, lda, b, ldb, s);
+    } else {
+      TrsmDevice<double>(true, false, true, false, n, nrhs, a, lda, b, ldb, s);
+      TrsmDevice<double>(true, true, true, true, n, nrhs, a, lda, b, ldb, s);
+    }
     Permute(false, n, nrhs, b, ldb, perm.src, perm.inv, perm.scratch, s);
   }
 }
@@ -1077,13 +1114,16 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// R = B - sum rp; W = |B| + sum wp; per workgroup and right-hand side the
+// R = B - sum rp; D = |B| + sum wp; per workgroup and right-hand side the
 // largest berr term and the largest |R| (columns of `stats`: 2 * nrhs).
+// With w_out, also the weights of the forward error bound,
+// W = |R| + nzeps D, plus safe1 where D <= safe2 (leading dimension n).
 __global__ void __launch_bounds__(256)
     ResidualFinishKernel(const double* __restrict__ rp, const double* __restrict__ wp,
                          int nchunks, const double* __restrict__ b, int64_t ldb, int n, int nrhs,
                          double* __restrict__ r_out, int64_t ldr, double safe1, double safe2,
-                         double* __restrict__ stats) {
+                         double* __restrict__ stats, double* __restrict__ w_out,
+                         double nzeps) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ double red[256];
   for (int j = 0; j < nrhs; ++j) {
@@ -1098,6 +1138,8 @@ __global__ void __launch_bounds__(256)
       const double rij = bij - sr;
       const double wij = fabs(bij) + sw;
       r_out[i + int64_t(j) * ldr] = rij;
+      if (w_out)
+        w_out[i + int64_t(j) * n] = fabs(rij) + nzeps * wij + (wij > safe2 ? 0.0 : safe1);
       berr = wij > safe2 ? fabs(rij) / wij : (fabs(rij) + safe1) / (wij + safe1);
       rabs = fabs(rij);
     }
@@ -1123,6 +1165,22 @@ __global__ void AddMaskedKernel(double* __restrict__ x, int64_t ldx, const doubl
     if (active[j]) x[i + int64_t(j) * ldx] += d[i + int64_t(j) * ldd];
 }
 
+// The right-hand side each packed column of the forward-error products
+// belongs to.
+struct ColumnList {
+  int col[kMaxRhs];
+};
+
+// z(:, c) *= w(:, list.col[c]) for the packed columns c (grid.y).
+__global__ void __launch_bounds__(256)
+    ScaleByWeightsKernel(double* __restrict__ z, int n, const double* __restrict__ w,
+                         ColumnList list) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = blockIdx.y;
+  z[i + int64_t(c) * n] *= w[i + int64_t(list.col[c]) * n];
+}
+
 // Largest row sum of |A| (the infinity norm).
 __global__ void RowAbsSumKernel(const double* __restrict__ a, int64_t lda, int n,
                                 double* __restrict__ out) {
@@ -1131,6 +1189,171 @@ __global__ void RowAbsSumKernel(const double* __restrict__ a, int64_t lda, int n
   double s = 0.0;
   for (int k = 0; k < n; ++k) s += fabs(a[i + int64_t(k) * lda]);
   out[i] = s;
+}
+
+// Hager's estimate of ||M||_1 with Higham's refinements, the steps of
+// xLACN2 (src/lapack/trsen.cc:2235-2366), driven from the host for one
+// vector: Start() and Next() name the product wanted next in x(), which
+// the caller replaces by M x (kTimesM) or M^T x (kTimesMt) before calling
+// Next() again, until kDone. Every estimate is |M w|_1 / |w|_1 for some w.
+class NormEstimator {
+ public:
+  enum Request { kDone = 0, kTimesM = 1, kTimesMt = 2 };
+
+  explicit NormEstimator(int n) : n_(n), x_(n), sign_(n) {}
+  double* x() { return x_.data(); }
+  double estimate() const { return est_; }
+
+  Request Start() {
+    std::fill(x_.begin(), x_.end(), 1.0 / double(n_));
+    stage_ = kAfterFirst;
+    return kTimesM;
+  }
+
+  Request Next() {
+    switch (stage_) {
+      case kAfterFirst:  // x = M (1/n)
+        if (n_ == 1) {
+          est_ = std::fabs(x_[0]);
+          return kDone;
+        }
+        est_ = Norm1();
+        return AskSigns(kAfterFirstT);
+      case kAfterFirstT:  // x = M^T sign
+        j_ = FirstLargest();
+        iter_ = 2;
+        return AskUnit();
+      case kAfterUnit: {  // x = M e_j
+        const double prev = est_;
+        est_ = Norm1();
+        bool repeated = true;
+        for (int i = 0; i < n_ && repeated; ++i) repeated = Sign(x_[i]) == sign_[i];
+        if (repeated || est_ <= prev) return AskAlternating();
+        return AskSigns(kAfterLoopT);
+      }
+      case kAfterLoopT: {  // x = M^T sign
+        const int last = j_;
+        j_ = FirstLargest();
+        if (x_[last] != std::fabs(x_[j_]) && iter_ < kMaxIter) {
+          ++iter_;
+          return AskUnit();
+        }
+        return AskAlternating();
+      }
+      case kAfterAlternating: {  // x = M b
+        const double t = 2.0 * (Norm1() / double(3 * n_));
+        if (t > est_) est_ = t;
+        return kDone;
+      }
+    }
+    return kDone;
+  }
+
+ private:
+  enum Stage { kAfterFirst, kAfterFirstT, kAfterUnit, kAfterLoopT, kAfterAlternating };
+  static constexpr int kMaxIter = 5;
+
+  static int Sign(double t) { return t >= 0.0 ? 1 : -1; }
+  double Norm1() const {
+    double s = 0.0;
+    for (double t : x_) s += std::fabs(t);
+    return s;
+  }
+  // First index of the largest magnitude; 0 when x[0] is NaN, other NaNs
+  // skipped (as xIAMAX).
+  int FirstLargest() const {
+    if (std::isnan(x_[0])) return 0;
+    int best = 0;
+    double bv = std::fabs(x_[0]);
+    for (int i = 1; i < n_; ++i)
+      if (std::fabs(x_[i]) > bv) {
+        bv = std::fabs(x_[i]);
+        best = i;
+      }
+    return best;
+  }
+  Request AskSigns(Stage next) {
+    for (int i = 0; i < n_; ++i) {
+      sign_[i] = Sign(x_[i]);
+      x_[i] = double(sign_[i]);
+    }
+    stage_ = next;
+    return kTimesMt;
+  }
+  Request AskUnit() {
+    std::fill(x_.begin(), x_.end(), 0.0);
+    x_[j_] = 1.0;
+    stage_ = kAfterUnit;
+    return kTimesM;
+  }
+  // b_i = (-1)^i (1 + i / (n - 1)).
+  Request AskAlternating() {
+    for (int i = 0; i < n_; ++i) x_[i] = (i % 2 ? -1.0 : 1.0) * (1.0 + double(i) / double(n_ - 1));
+    stage_ = kAfterAlternating;
+    return kTimesM;
+  }
+
+  int n_;
+  std::vector<double> x_;
+  std::vector<int> sign_;
+  Stage stage_ = kAfterFirst;
+  double est_ = 0.0;
+  int j_ = 0, iter_ = 0;
+};
+
+// gerfs's forward error bound (src/lapack/gesvx.cc:465-518) for every
+// right-hand side at once: est_j = ||inv(A) diag(W_j)||inf, estimated as
+// ||M_j||_1 with M_j = diag(W_j) inv(A)^T. The estimators of all columns
+// advance together; each round packs the vectors that want M x (getrs with
+// A^T, then times W) and those that want M^T x (times W, then getrs with
+// A) into z, one solve per kind. w: the weights (n x nrhs, leading
+// dimension n); z: n x nrhs scratch.
+void ForwardErrorEstimates(int n, int nrhs, const double* af, int64_t ldaf, const Perm& perm,
+                           const double* w, double* z, hipStream_t s, double* est) {
+  std::vector<NormEstimator> ests;
+  std::vector<NormEstimator::Request> req(nrhs);
+  for (int j = 0; j < nrhs; ++j) {
+    ests.emplace_back(n);
+    req[j] = ests[j].Start();
+  }
+  std::vector<double> packed(size_t(n) * nrhs);
+  const int rblocks = (n + 255) / 256;
+  for (;;) {
+    ColumnList first{}, second{};
+    int c1 = 0, c2 = 0;
+    for (int j = 0; j < nrhs; ++j) {
+      if (req[j] == NormEstimator::kTimesM) first.col[c1++] = j;
+      if (req[j] == NormEstimator::kTimesMt) second.col[c2++] = j;
+    }
+    if (c1 + c2 == 0) break;
+    for (int c = 0; c < c1 + c2; ++c) {
+      const int j = c < c1 ? first.col[c] : second.col[c - c1];
+      std::copy(ests[j].x(), ests[j].x() + n, packed.data() + size_t(c) * n);
+    }
+    HWY_HIP_CHECK(hipMemcpyAsync(z, packed.data(), size_t(n) * (c1 + c2) * 8,
+                                 hipMemcpyHostToDevice, s));
+    double* z2 = z + size_t(c1) * n;
+    if (c1 > 0) {
+      GetrsPerm(true, n, c1, af, ldaf, perm, z, n, s);
+      hipLaunchKernelGGL(ScaleByWeightsKernel, dim3(rblocks, c1), dim3(256), 0, s, z, n, w,
+                         first);
+    }
+    if (c2 > 0) {
+      hipLaunchKernelGGL(ScaleByWeightsKernel, dim3(rblocks, c2), dim3(256), 0, s, z2, n, w,
+                         second);
+      GetrsPerm(false, n, c2, af, ldaf, perm, z2, n, s);
+    }
+    HWY_HIP_CHECK(hipGetLastError());
+    HWY_HIP_CHECK(hipMemcpyAsync(packed.data(), z, size_t(n) * (c1 + c2) * 8,
+                                 hipMemcpyDeviceToHost, s));
+    HWY_HIP_CHECK(hipStreamSynchronize(s));
+    for (int c = 0; c < c1 + c2; ++c) {
+      const int j = c < c1 ? first.col[c] : second.col[c - c1];
+      std::copy(packed.data() + size_t(c) * n, packed.data() + size_t(c + 1) * n, ests[j].x());
+      req[j] = ests[j].Next();
+    }
+  }
+  for (int j = 0; j < nrhs; ++j) est[j] = ests[j].estimate();
 }
 
 }  // namespace lu
@@ -1180,15 +1403,17 @@ HWYBLAS_AMD_API int hwyblas_dgetrs_dev(char trans, int n, int nrhs, const double
 }
 
 // Solves A X = B: af := LU of A (trailing updates in the precision
-// `update`), X := getrs(B), then FP64 refinement against A. berr, nbe
-// (normwise backward error ||B - A X||inf / (||A||inf ||X||inf)) and
-// iterations (corrections applied) per right-hand side, on the host.
+// `update`), X := getrs(B), then FP64 refinement against A. ferr (forward
+// error bound, when not null), berr, nbe (normwise backward error
+// ||B - A X||inf / (||A||inf ||X||inf)) and iterations (corrections
+// applied) per right-hand side, on the host.
 // Returns getrf's info (> 0: singular, nothing solved) or < 0 on an
 // invalid argument.
 HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int lda, double* af,
                                          int ldaf, int* ipiv, const double* b, int ldb, double* x,
-                                         int ldx, int update, int itmax, double* berr,
-                                         double* nbe, int* iterations, void* stream) {
+                                         int ldx, int update, int itmax, double* ferr,
+                                         double* berr, double* nbe, int* iterations,
+                                         void* stream) {
   int info = 0;
   if (n < 0) info = -1;
   else if (nrhs < 0 || nrhs > lu::kMaxRhs) info = -2;
@@ -1202,7 +1427,15 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
     ReportError("DGESVIR", -info);
     return info;
   }
-  if (n == 0 || nrhs == 0) return 0;
+  if (n == 0 || nrhs == 0) {
+    for (int j = 0; j < nrhs; ++j) {
+      if (ferr) ferr[j] = 0.0;
+      if (berr) berr[j] = 0.0;
+      if (nbe) nbe[j] = 0.0;
+      if (iterations) iterations[j] = 0;
+    }
+    return 0;
+  }
   hipStream_t s = StreamOr(stream);
   HWY_HIP_CHECK(hipMemcpy2DAsync(af, size_t(ldaf) * 8, a, size_t(lda) * 8, size_t(n) * 8, n,
                                  hipMemcpyDeviceToDevice, s));
@@ -1215,8 +1448,12 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
 
   const int nchunks = (n + lu::kResChunk - 1) / lu::kResChunk;
   const int rblocks = (n + 255) / 256;
-  double *rp, *wp, *r, *stats, *rowsum;
+  double *rp, *wp, *r, *stats, *rowsum, *wts = nullptr, *z = nullptr;
   int* active_d;
+  if (ferr) {
+    HWY_HIP_CHECK(hipMalloc(&wts, size_t(n) * nrhs * 8));
+    HWY_HIP_CHECK(hipMalloc(&z, size_t(n) * nrhs * 8));
+  }
   HWY_HIP_CHECK(hipMalloc(&rp, size_t(nchunks) * lu::kMaxRhs * n * 8));
   HWY_HIP_CHECK(hipMalloc(&wp, size_t(nchunks) * lu::kMaxRhs * n * 8));
   HWY_HIP_CHECK(hipMalloc(&r, size_t(n) * nrhs * 8));
@@ -1233,7 +1470,8 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
     hipLaunchKernelGGL(lu::ResidualPartialKernel, dim3(rblocks, nchunks), dim3(256), 0, s, a,
                        int64_t(lda), n, x, int64_t(ldx), nrhs, rp, wp);
     hipLaunchKernelGGL(lu::ResidualFinishKernel, dim3(rblocks), dim3(256), 0, s, rp, wp, nchunks,
-                       b, int64_t(ldb), n, nrhs, r, int64_t(n), safe1, safe2, stats);
+                       b, int64_t(ldb), n, nrhs, r, int64_t(n), safe1, safe2, stats, wts,
+                       double(n + 1) * eps);
     HWY_HIP_CHECK(hipGetLastError());
     HWY_HIP_CHECK(hipMemcpyAsync(host_stats.data(), stats, host_stats.size() * 8,
                                  hipMemcpyDeviceToHost, s));
@@ -1266,6 +1504,9 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
                        int64_t(n), n, nrhs, active_d);
     HWY_HIP_CHECK(hipGetLastError());
   }
+  // The last residual pass was of the final X: its weights bound the error.
+  std::vector<double> est(nrhs, 0.0);
+  if (ferr) lu::ForwardErrorEstimates(n, nrhs, af, ldaf, perm, wts, z, s, est.data());
   // Normwise backward error of the final X.
   hipLaunchKernelGGL(lu::RowAbsSumKernel, dim3(rblocks), dim3(256), 0, s, a, int64_t(lda), n,
                      rowsum);
@@ -1279,12 +1520,13 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
     double xn = 0.0;
     for (int i = 0; i < n; ++i) xn = std::max(xn, std::fabs(xs[size_t(j) * n + i]));
     if (nbe) nbe[j] = (anorm > 0 && xn > 0) ? rinf[j] / (anorm * xn) : 0.0;
+    if (ferr) ferr[j] = xn != 0.0 ? est[j] / xn : est[j];
     if (iterations) iterations[j] = count[j] - 1;
   }
   for (void* p : {static_cast<void*>(rp), static_cast<void*>(wp), static_cast<void*>(r),
                   static_cast<void*>(stats), static_cast<void*>(rowsum),
-                  static_cast<void*>(active_d)})
-    HWY_HIP_CHECK(hipFree(p));
+                  static_cast<void*>(active_d), static_cast<void*>(wts), static_cast<void*>(z)})
+    if (p) HWY_HIP_CHECK(hipFree(p));
   return 0;
 }
 

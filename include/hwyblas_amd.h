@@ -348,15 +348,19 @@ HWYBLAS_AMD_API int hwyblas_dgetrs_dev(char trans, int n, int nrhs, const double
  * then FP64 iterative refinement against A as dgerfs does it
  * (src/lapack/gesvx.cc:395-463): a column is corrected while its
  * componentwise backward error berr exceeds eps, halves at least, and fewer
- * than itmax corrections were made. Host outputs per right-hand side: berr,
- * nbe = ||B - A X||inf / (||A||inf ||X||inf), iterations = corrections.
- * Returns 0, getrf's info (> 0, or -1000 as there), or -i (reported as
- * DGESVIR). Synchronizes. */
+ * than itmax corrections were made. Then, when ferr is not NULL, dgerfs's
+ * forward error bound (src/lapack/gesvx.cc:465-518): ferr_j estimates
+ * || |inv(A)| (|R| + (n + 1) eps (|A| |X| + |B|)) ||inf / max|X(:, j)| by
+ * the xLACN2 estimator (src/lapack/trsen.cc:2235-2366), its products solved
+ * with af on the GPU. Host outputs per right-hand side: ferr, berr,
+ * nbe = ||B - A X||inf / (||A||inf ||X||inf), iterations = corrections
+ * (nbe and iterations may be NULL). Returns 0, getrf's info (> 0, or -1000
+ * as there), or -i (reported as DGESVIR). Synchronizes. */
 HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int lda, double* af,
                                          int ldaf, int* ipiv, const double* b, int ldb,
                                          double* x, int ldx, int update, int itmax,
-                                         double* berr, double* nbe, int* iterations,
-                                         void* stream);
+                                         double* ferr, double* berr, double* nbe,
+                                         int* iterations, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Tile configurations of the plain GEMM kernel (no hwyBLAS counterpart: its

@@ -86,6 +86,10 @@ def lib():
         L.orc_dgetrs.argtypes = [_c_char, _i, _i, _p, _i, _p, _p, _i]
         L.orc_dgerfs_berr.restype = None
         L.orc_dgerfs_berr.argtypes = [_i, _i, _p, _i, _p, _i, _p, _i, _p]
+        L.orc_dlacn2.restype = None
+        L.orc_dlacn2.argtypes = [_i, _p, _p, _p, _p, _p, _p]
+        L.orc_dgerfs.restype = _i
+        L.orc_dgerfs.argtypes = [_c_char, _i, _i, _p, _i, _p, _i, _p, _p, _i, _p, _i, _p, _p]
         L.orc_plant_mistake.restype = None
         L.orc_plant_mistake.argtypes = [_i]
         _lib = L
@@ -230,6 +234,44 @@ def gerfs_berr(a, b, x):
     lib().orc_dgerfs_berr(n, nrhs, _ptr(a), a.shape[0], _ptr(b), b.shape[0], _ptr(x),
                           x.shape[0], _ptr(out))
     return out
+
+
+def lacn2(n, matvec, rmatvec):
+    """Drives the oracle's xLACN2 with the callbacks matvec(x) = M x and
+    rmatvec(x) = M^T x; returns (estimate of ||M||_1, v, number of products)."""
+    v = np.zeros(max(1, n))
+    x = np.zeros(max(1, n))
+    isgn = np.zeros(max(1, n), dtype=np.int32)
+    est = np.zeros(1)
+    kase = np.zeros(1, dtype=np.int32)
+    isave = np.zeros(3, dtype=np.int32)
+    calls = 0
+    while True:
+        lib().orc_dlacn2(n, _ptr(v), _ptr(x), _ptr(isgn), _ptr(est), _ptr(kase), _ptr(isave))
+        if kase[0] == 0:
+            return float(est[0]), v[:n].copy(), calls
+        x[:n] = matvec(x[:n].copy()) if kase[0] == 1 else rmatvec(x[:n].copy())
+        calls += 1
+
+
+def gerfs(trans, a, lu_, ipiv, b, x):
+    """xGERFS: (refined copy of x, ferr, berr) for op(A) X = B, with the
+    oracle's LU (lu_, 1-based ipiv) of A."""
+    a = np.asfortranarray(a, dtype=np.float64)
+    lu_ = np.asfortranarray(lu_, dtype=np.float64)
+    b = np.asfortranarray(b, dtype=np.float64)
+    if b.ndim == 1:
+        b = b.reshape(-1, 1, order="F")
+    xr = np.array(x, dtype=np.float64, order="F", copy=True).reshape(b.shape, order="F")
+    n, nrhs = b.shape
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    ferr, berr = np.zeros(max(1, nrhs)), np.zeros(max(1, nrhs))
+    info = lib().orc_dgerfs(_b(trans), n, nrhs, _ptr(a), max(1, n), _ptr(lu_), max(1, n),
+                            _ptr(ipiv), _ptr(b), max(1, n), _ptr(xr), max(1, n), _ptr(ferr),
+                            _ptr(berr))
+    if info != 0:
+        raise ValueError(f"gerfs: invalid argument {-info}")
+    return xr, ferr[:nrhs], berr[:nrhs]
 
 
 def lu_factors(lu_, ipiv):

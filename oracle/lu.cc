@@ -5,13 +5,16 @@
 // :216-280, xGETRS :287-339) with hwyBLAS's leaf kernel
 // (src/lapack/factor_kernels.cc:107-138, leaf 16 rows or columns,
 // factor_kernels.h:27), GETRF's block size 64 (src/lapack/environment.cc:
-// 86-88), and the componentwise backward error of xGERFS
-// (src/lapack/gesvx.cc:395-425). GEMM and TRSM are the oracle's own
-// restatements (oracle.cc).
+// 86-88), the componentwise backward error of xGERFS
+// (src/lapack/gesvx.cc:395-425), the whole of xGERFS with its forward
+// error bound (src/lapack/gesvx.cc:333-521) and the norm estimator xLACN2
+// it drives (src/lapack/trsen.cc:2235-2366). GEMM and TRSM are the
+// oracle's own restatements (oracle.cc).
 #include <algorithm>
 #include <cmath>
 #include <limits>
 #include <utility>
+#include <vector>
 
 #include "oracle.h"
 
@@ -178,6 +181,191 @@ void orc_dgerfs_berr(int n, int nrhs, const double* a, int lda, const double* b,
     }
     berr[j] = s;
   }
+}
+
+// xLACN2 for real data (src/lapack/trsen.cc:2235-2366): Hager's 1-norm
+// estimator with Higham's refinements, by reverse communication. On return
+// with kase = 1 the caller replaces x by M x, with kase = 2 by M^T x, and
+// calls again; kase = 0 ends it with est (and v with M v = w,
+// est = |v|_1 / |w|_1). isave[0] names the product the next call receives
+// (1: M x0 with x0 = 1/n; 2: M^T sign; 3: M e_j; 4: M^T sign inside the
+// main loop; 5: M b with the alternating vector b), isave[1] is j
+// (1-based), isave[2] counts the main-loop iterations (at most 5). The sums
+// |y|_1 run in index order (hwyBLAS's Asum is vectorized, so its last bit
+// can differ). Planted mistake 9: the alternating-vector stage is skipped.
+void orc_dlacn2(int n, double* v, double* x, int* isgn, double* est, int* kase, int* isave) {
+  constexpr int kItmax = 5;
+  auto sign = [](double t) { return t >= 0.0 ? 1.0 : -1.0; };
+  auto asum = [n](const double* y) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += std::fabs(y[i]);
+    return s;
+  };
+  auto ask_unit = [&](int j) {  // x = e_j, then M x
+    std::fill(x, x + n, 0.0);
+    x[j - 1] = 1.0;
+    *kase = 1;
+    isave[0] = 3;
+  };
+  auto ask_signs = [&](int step) {  // x = sign(x), then M^T x
+    for (int i = 0; i < n; ++i) {
+      x[i] = sign(x[i]);
+      isgn[i] = int(x[i]);
+    }
+    *kase = 2;
+    isave[0] = step;
+  };
+  auto ask_alternating = [&]() {  // b_i = (-1)^i (1 + i / (n - 1)), then M b
+    if (g_mistake == 9) {
+      *kase = 0;
+      return;
+    }
+    double s = 1.0;
+    for (int i = 0; i < n; ++i) {
+      x[i] = s * (1.0 + double(i) / double(n - 1));
+      s = -s;
+    }
+    *kase = 1;
+    isave[0] = 5;
+  };
+  if (*kase == 0) {
+    std::fill(x, x + n, 1.0 / double(n));
+    *kase = 1;
+    isave[0] = 1;
+    return;
+  }
+  switch (isave[0]) {
+    case 2:  // x = M^T sign(M x0)
+      isave[1] = IamaxFirst(n, x) + 1;
+      isave[2] = 2;
+      ask_unit(isave[1]);
+      return;
+    case 3: {  // x = M e_j
+      std::copy(x, x + n, v);
+      const double estold = *est;
+      *est = asum(v);
+      bool repeated = true;
+      for (int i = 0; i < n && repeated; ++i) repeated = int(sign(x[i])) == isgn[i];
+      if (repeated || *est <= estold) {
+        ask_alternating();
+        return;
+      }
+      ask_signs(4);
+      return;
+    }
+    case 4: {  // x = M^T sign(M e_j)
+      const int jlast = isave[1];
+      isave[1] = IamaxFirst(n, x) + 1;
+      if (x[jlast - 1] != std::fabs(x[isave[1] - 1]) && isave[2] < kItmax) {
+        ++isave[2];
+        ask_unit(isave[1]);
+        return;
+      }
+      ask_alternating();
+      return;
+    }
+    case 5: {  // x = M b
+      const double temp = 2.0 * (asum(x) / double(3 * n));
+      if (temp > *est) {
+        std::copy(x, x + n, v);
+        *est = temp;
+      }
+      *kase = 0;
+      return;
+    }
+    default:  // 1 (and, as there, any other value): x = M x0
+      if (n == 1) {
+        v[0] = x[0];
+        *est = std::fabs(v[0]);
+        *kase = 0;
+        return;
+      }
+      *est = asum(x);
+      ask_signs(2);
+      return;
+  }
+}
+
+// xGERFS for real data (src/lapack/gesvx.cc:333-521). Per column j: the
+// FP64 residual R = B - op(A) X (:412-413; each row a dot product in index
+// order), berr with the safe1 / safe2 guard (:415-450), a correction
+// X += getrs(op, R) while berr > eps, berr at least halved and at most 5
+// corrections made (:339, :457-463); then FERR = ||inv(op(A)) diag(W)||inf /
+// max|X(:, j)| with W = |R| + nz eps (|op(A)| |X| + |B|), nz = n + 1, plus
+// safe1 where |op(A)| |X| + |B| <= safe2 (:465-487), the norm estimated by
+// orc_dlacn2 on M = diag(W) inv(op(A))^T (kase 1: getrs with the other
+// transposition, then times W; kase 2: times W, then getrs with op) (:488-
+// 518). Returns -1 for an unknown trans. Planted mistakes: 10, kase 1 solves
+// with op itself; 11, nz = n.
+int orc_dgerfs(char trans, int n, int nrhs, const double* a, int lda, const double* af,
+               int ldaf, const int* ipiv, const double* b, int ldb, double* x, int ldx,
+               double* ferr, double* berr) {
+  const bool notran = trans == 'N' || trans == 'n';
+  if (!notran && trans != 'T' && trans != 't' && trans != 'C' && trans != 'c') return -1;
+  if (n == 0 || nrhs == 0) {
+    for (int j = 0; j < nrhs; ++j) ferr[j] = berr[j] = 0.0;
+    return 0;
+  }
+  const char transn = notran ? 'N' : 'T', transt = notran ? 'T' : 'N';
+  const char kase1 = g_mistake == 10 ? transn : transt;
+  const double nz = g_mistake == 11 ? double(n) : double(n + 1);
+  const double eps = std::numeric_limits<double>::epsilon() * 0.5;  // dlamch('E')
+  const double safmin = std::numeric_limits<double>::min();
+  const double safe1 = nz * safmin, safe2 = safe1 / eps;
+  auto opa = [&](int i, int k) {
+    return notran ? a[i + int64_t(k) * lda] : a[k + int64_t(i) * lda];
+  };
+  std::vector<double> r(n), w(n), v(n);
+  std::vector<int> isgn(n);
+  for (int j = 0; j < nrhs; ++j) {
+    double* xj = x + int64_t(j) * ldx;
+    const double* bj = b + int64_t(j) * ldb;
+    int count = 1;
+    double lstres = 3.0;
+    for (;;) {
+      for (int i = 0; i < n; ++i) {
+        double t = 0.0, d = std::fabs(bj[i]);
+        for (int k = 0; k < n; ++k) {
+          t += opa(i, k) * xj[k];
+          d += std::fabs(opa(i, k)) * std::fabs(xj[k]);
+        }
+        r[i] = bj[i] - t;
+        w[i] = d;
+      }
+      double s = 0.0;
+      for (int i = 0; i < n; ++i)
+        s = std::max(s, w[i] > safe2 ? std::fabs(r[i]) / w[i]
+                                     : (std::fabs(r[i]) + safe1) / (w[i] + safe1));
+      berr[j] = s;
+      if (s > eps && 2.0 * s <= lstres && count <= 5) {
+        orc_dgetrs(trans, n, 1, af, ldaf, ipiv, r.data(), n);
+        for (int i = 0; i < n; ++i) xj[i] += r[i];
+        lstres = s;
+        ++count;
+        continue;
+      }
+      break;
+    }
+    for (int i = 0; i < n; ++i)
+      w[i] = std::fabs(r[i]) + nz * eps * w[i] + (w[i] > safe2 ? 0.0 : safe1);
+    int kase = 0, isave[3] = {0, 0, 0};
+    double est = 0.0;
+    for (;;) {
+      orc_dlacn2(n, v.data(), r.data(), isgn.data(), &est, &kase, isave);
+      if (kase == 0) break;
+      if (kase == 1) {
+        orc_dgetrs(kase1, n, 1, af, ldaf, ipiv, r.data(), n);
+        for (int i = 0; i < n; ++i) r[i] *= w[i];
+      } else {
+        for (int i = 0; i < n; ++i) r[i] *= w[i];
+        orc_dgetrs(transn, n, 1, af, ldaf, ipiv, r.data(), n);
+      }
+    }
+    double xn = 0.0;
+    for (int i = 0; i < n; ++i) xn = std::max(xn, std::fabs(xj[i]));
+    ferr[j] = xn != 0.0 ? est / xn : est;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -135,6 +135,15 @@ void orc_dgetrs(char trans, int n, int nrhs, const double* a, int lda, const int
                 double* b, int ldb);
 void orc_dgerfs_berr(int n, int nrhs, const double* a, int lda, const double* b, int ldb,
                      const double* x, int ldx, double* berr);
+/* xLACN2 (reverse-communication 1-norm estimate; kase 1: x := M x, kase 2:
+ * x := M^T x; isave[3] keeps the state) and xGERFS (refines X in place with
+ * the LU af/ipiv of A; per column the forward error bound ferr and the
+ * componentwise backward error berr; trans N, T or C; returns -1 for any
+ * other trans). */
+void orc_dlacn2(int n, double* v, double* x, int* isgn, double* est, int* kase, int* isave);
+int orc_dgerfs(char trans, int n, int nrhs, const double* a, int lda, const double* af,
+               int ldaf, const int* ipiv, const double* b, int ldb, double* x, int ldx,
+               double* ferr, double* berr);
 
 /* ---- planted mistakes (for tests that show the checks catch them) ------- */
 /* 0: none; 1: drop the last k term of every GEMM tile; 2: use op(A) with
@@ -142,7 +151,9 @@ void orc_dgerfs_berr(int n, int nrhs, const double* a, int lda, const double* b,
  * the diagonal; 5: the double-double reference drops its low word; 6: the
  * emulated DGEMM's CRT drops the last modulus; 7: the LU leaf does not swap
  * the columns left of the pivot column; 8: the BF16-slice emulation leaves
- * out its top level of slice pairs (t + u = s + 1). */
+ * out its top level of slice pairs (t + u = s + 1); 9: the 1-norm estimator
+ * skips its alternating-vector stage; 10: gerfs's forward error bound solves
+ * with op(A) where op(A)^T belongs; 11: that bound takes nz = n. */
 void orc_plant_mistake(int which);
 
 #ifdef __cplusplus

@@ -83,3 +83,33 @@ def trsm_residual_ratio(side, uplo, ta, diag, m, n, alpha, a, lda, b0, ldb, x, w
     r = oracle.trsm_residual(side, uplo, ta, diag, m, n, alpha, a, lda, b0, ldb, x, ldb, which)
     tol = 8.0 * (ka + 2) * eps(b0.dtype)
     return float(np.max(r) / tol) if len(r) else 0.0
+
+
+def permuted_triangular_system(n, nrhs, seed, trans="N"):
+    """(A, X, B) with A = P U: U upper triangular with integer entries, a
+    power-of-two diagonal at least the row's off-diagonal sum and
+    off-diagonals in {0, -1, -2}; P a random row permutation; X integer,
+    nonzero, |X| <= 50; B = op(A) X, exact. Partial pivoting recovers P with
+    L = I, every solve is exact integer arithmetic (so refinement stops with
+    R = 0), and inv(op(A)) >= 0 entrywise."""
+    rng = np.random.default_rng(seed)
+    u = np.triu(-rng.integers(0, 3, (n, n)).astype(float) * (rng.random((n, n)) < 0.2), 1)
+    rs = -u.sum(1)
+    u[np.arange(n), np.arange(n)] = 2.0 ** np.ceil(np.log2(rs + 1)) * rng.choice([1, 2], n)
+    a = np.asfortranarray(u[rng.permutation(n)])
+    x = rng.integers(1, 51, (n, nrhs)).astype(float) * rng.choice([-1.0, 1.0], (n, nrhs))
+    opa = a if trans == "N" else a.T
+    return a, np.asfortranarray(x), np.asfortranarray(opa @ x)
+
+
+def forward_bound(opa, x, b):
+    """gerfs's forward error bound for an exact solution (R = 0), by its
+    defining formula (src/lapack/gesvx.cc:469-476) with an explicit
+    inverse: || |inv(op(A))| W ||inf / max|X| per column, W = nz eps
+    (|op(A)| |X| + |B|) + safe1 where that is <= safe2, nz = n + 1."""
+    n = opa.shape[0]
+    e = np.finfo(np.float64).eps / 2
+    safe1 = (n + 1) * np.finfo(np.float64).tiny
+    d = np.abs(opa) @ np.abs(x) + np.abs(b)
+    w = (n + 1) * e * d + np.where(d > safe1 / e, 0.0, safe1)
+    return (np.abs(np.linalg.inv(opa)) @ w).max(0) / np.abs(x).max(0)

@@ -219,7 +219,7 @@ def test_lu_entry_points_reject_invalid_arguments():
         assert H.lib().hwyblas_dgetrs_dev(b"X", 4, 1, None, 4, None, None, 4, None) == -1
         assert H.lib().hwyblas_dgetrs_dev(b"N", 4, 1, None, 4, None, None, 3, None) == -8
         assert H.lib().hwyblas_dgesv_ir_dev(4, 17, None, 4, None, 4, None, None, 4, None, 4, 0,
-                                            5, None, None, None, None) == -2
+                                            5, None, None, None, None, None) == -2
     assert cap.calls == [("DGETRF", 1), ("DGETRF", 4), ("DGETRF", 6), ("DGETRS", 1),
                          ("DGETRS", 8), ("DGESVIR", 2)]
     # Empty problems return at once.

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
+from helpers import forward_bound, permuted_triangular_system
 
 pytestmark = pytest.mark.gpu
 EPS = np.finfo(np.float64).eps
@@ -107,9 +108,11 @@ def test_getrf_nan_first_row_pivot(H):
 
 
 @pytest.mark.parametrize("trans", ["N", "T"])
-def test_getrs_matches_oracle(H, trans):
+@pytest.mark.parametrize("n,nrhs", [(400, 5), (333, 17), (200, 70)])
+def test_getrs_matches_oracle(H, trans, n, nrhs):
+    """Blocked solves (nrhs <= 64, ragged last block at n = 333) and the
+    recursive TRSMs (nrhs = 70), both transpositions."""
     rng = np.random.default_rng(11)
-    n, nrhs = 400, 5
     a = rng.standard_normal((n, n))
     b = rng.uniform(-1, 1, (n, nrhs))
     da = H.DeviceArray.from_host(np.asfortranarray(a))
@@ -170,3 +173,78 @@ def test_gesv_refinement(H, update, kappa):
     assert np.all(nbe < 10 * EPS)
     if update == 2:
         assert iters.min() >= 1  # FP32 factors need corrections
+
+
+@pytest.mark.parametrize("update", [0, 1, 2])
+def test_gesv_forward_bound_closed_form(H, update):
+    """Permuted triangular integer systems are solved exactly (R = 0), and
+    inv(A) >= 0 makes the estimated matrix nonnegative, so the estimate is
+    exact: ferr equals the bound's defining formula, evaluated with an
+    explicit inverse. n = 2100 runs its top-level update (k = 1050) on the
+    emulated GEMM for update 1."""
+    for n, nrhs in ((1, 1), (7, 3), (300, 16), (2100, 16)):
+        a, xt, b = permuted_triangular_system(n, nrhs, n + update)
+        da = H.DeviceArray.from_host(a)
+        db = H.DeviceArray.from_host(b)
+        x, berr, nbe, iters, info, ferr = H.gesv_ir_dev(da, db, update=update, with_ferr=True)
+        assert info == 0
+        assert np.array_equal(x.to_host(), xt), n
+        assert np.all(berr == 0) and np.all(iters == 0)
+        want = forward_bound(a, xt, b)
+        assert np.allclose(ferr, want, rtol=1e-12, atol=0), (n, ferr / want - 1)
+
+
+@pytest.mark.parametrize("kappa", [2, 6, 10])
+def test_gesv_forward_bound_matches_oracle(H, kappa):
+    """A = Q1 diag(sigma) Q2^T: the GPU's ferr for its refined X against the
+    oracle's xGERFS on the same X (its own LU and solves), within 5%: the
+    weights differ only in |R| at the rounding level, a few percent of the
+    (n + 1) eps term at n = 512. The bound grows with the condition number."""
+    n, nrhs = 512, 16
+    a = conditioned(n, kappa, 200 + kappa)
+    b = np.random.default_rng(8).uniform(-1, 1, (n, nrhs))
+    da = H.DeviceArray.from_host(np.asfortranarray(a))
+    db = H.DeviceArray.from_host(np.asfortranarray(b))
+    x, berr, nbe, iters, info, ferr = H.gesv_ir_dev(da, db, with_ferr=True)
+    assert info == 0
+    xh = x.to_host()
+    lu, piv, _ = oracle.getrf(a)
+    _, oferr, oberr = oracle.gerfs("N", a, lu, piv, b, xh)
+    print(f"kappa 1e{kappa}: ferr {ferr.max():.3g} (oracle {oferr.max():.3g})")
+    assert np.allclose(ferr, oferr, rtol=0.05, atol=0)
+    assert np.all(ferr < 10.0 ** kappa * 10 * n * EPS)
+
+
+def test_gesv_forward_bound_near_underflow(H):
+    """B scaled by 2^-982 (hwyBLAS's near-underflow case): safe1 enters the
+    weights and ferr >= 0.3 safe1 / (||A||inf ||X||inf)
+    (tests/lapack/linear_test.cc:541-558), and matches the oracle's."""
+    n, nrhs = 300, 4
+    a = conditioned(n, 3, 5)
+    b = np.random.default_rng(9).uniform(-1, 1, (n, nrhs)) * 2.0 ** -982
+    da = H.DeviceArray.from_host(np.asfortranarray(a))
+    db = H.DeviceArray.from_host(np.asfortranarray(b))
+    x, berr, nbe, iters, info, ferr = H.gesv_ir_dev(da, db, with_ferr=True)
+    assert info == 0
+    xh = x.to_host()
+    safe1 = (n + 1) * np.finfo(np.float64).tiny
+    xn = np.abs(xh).max(0)
+    assert np.all(ferr >= 0.3 * safe1 / (np.abs(a).sum(1).max() * xn))
+    lu, piv, _ = oracle.getrf(a)
+    _, oferr, _ = oracle.gerfs("N", a, lu, piv, b, xh)
+    assert np.allclose(ferr, oferr, rtol=0.05, atol=0)
+
+
+def test_gesv_without_ferr_leaves_the_rest_unchanged(H):
+    """ferr = NULL skips the estimate; X, berr and nbe are the same."""
+    n, nrhs = 257, 5
+    a = conditioned(n, 4, 6)
+    b = np.random.default_rng(10).uniform(-1, 1, (n, nrhs))
+    da = H.DeviceArray.from_host(np.asfortranarray(a))
+    db = H.DeviceArray.from_host(np.asfortranarray(b))
+    x0, berr0, nbe0, it0, info0 = H.gesv_ir_dev(da, db)
+    x1, berr1, nbe1, it1, info1, ferr = H.gesv_ir_dev(da, db, with_ferr=True)
+    assert info0 == info1 == 0
+    assert np.array_equal(x0.to_host(), x1.to_host())
+    assert np.array_equal(berr0, berr1) and np.array_equal(nbe0, nbe1)
+    assert np.all(ferr > 0)

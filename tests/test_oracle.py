@@ -16,7 +16,8 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import gemm_bound_ratio, trsm_operands, trsm_residual_ratio
+from helpers import (forward_bound, gemm_bound_ratio, permuted_triangular_system,
+                     trsm_operands, trsm_residual_ratio)
 
 FIX = np.load(os.path.join(os.path.dirname(__file__), "golden", "blas_fixtures.npz"))
 
@@ -551,3 +552,120 @@ def test_gerfs_berr_closed_forms():
     a = np.triu(rng.integers(1, 4, (n, n))).astype(float)
     xe = rng.integers(-5, 5, (n, 2)).astype(float)
     assert np.array_equal(oracle.gerfs_berr(a, a @ xe, xe), [0.0, 0.0])
+
+
+# ---- xLACN2 and xGERFS's forward error bound --------------------------------
+
+def _lacn2(m):
+    return oracle.lacn2(m.shape[0], lambda x: m @ x, lambda x: m.T @ x)
+
+
+def test_lacn2_known_answers():
+    """n = 1: |m|. Nonnegative (nonsymmetric) M: the sign vector is all
+    ones, M^T 1 holds the column sums, so the estimate is the exact 1-norm
+    after M x0, M^T 1, M e_j and the alternating vector (4 products); a
+    diagonal with mixed signs: exact max |d|."""
+    est, v, calls = _lacn2(np.array([[-3.5]]))
+    assert est == 3.5 and calls == 1 and v[0] == -3.5
+    m = np.array([[1.0, 7, 0, 2], [0, 1, 5, 0], [3, 0, 1, 1], [0, 0, 4, 2]])
+    est, v, calls = _lacn2(m)
+    assert est == np.abs(m).sum(0).max() == 10.0 and calls == 4
+    assert np.array_equal(v, m[:, 2])  # the column the estimate came from
+    d = np.diag([2.0, -9.0, 4.0, -1.0, 3.0])
+    assert _lacn2(d)[0] == 9.0
+
+
+def test_lacn2_alternating_stage_by_hand():
+    """M = [[-1,0,-4],[0,3,-1],[4,-3,2]]: M x0 = (-5/3, 2/3, 1) (est 10/3),
+    M^T sign = (5, 0, 5) -> j = 1, M e_1 = (-1, 0, 4) (est 5, same signs:
+    converged), then b = (1, -1.5, 2), M b = (-9, -6.5, 12.5) and
+    2 |M b|_1 / (3 n) = 56/9 > 5. Without that stage the estimate is 5."""
+    m = np.array([[-1.0, 0, -4], [0, 3, -1], [4, -3, 2]])
+    est, v, calls = _lacn2(m)
+    assert abs(est - 56 / 9) <= 1e-15 and calls == 4
+    assert np.array_equal(v, [-9.0, -6.5, 12.5])
+    oracle.plant_mistake(9)
+    assert _lacn2(m)[0] == 5.0
+
+
+def test_lacn2_lower_bound_on_random_matrices():
+    """Every estimate is |M w|_1 / |w|_1 for some w, so it never exceeds
+    ||M||_1; on Gaussian matrices it stays within a factor 2 of it."""
+    rng = np.random.default_rng(5)
+    ratios = []
+    for _ in range(200):
+        n = int(rng.integers(2, 40))
+        m = rng.standard_normal((n, n))
+        est, v, calls = _lacn2(m)
+        assert calls <= 11
+        ratios.append(est / np.abs(m).sum(0).max())
+    ratios = np.array(ratios)
+    assert ratios.max() <= 1 + 1e-14 and ratios.min() > 0.5
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_gerfs_forward_bound_closed_form(trans):
+    """Exact solutions of permuted triangular integer systems: berr = 0 and
+    ferr equals the bound's defining formula (inv(op(A)) >= 0, so the
+    estimated matrix is nonnegative and the estimate exact)."""
+    for n in (1, 2, 6, 40, 300):
+        a, xt, b = permuted_triangular_system(n, 3, n, trans)
+        lu, piv, info = oracle.getrf(a)
+        assert info == 0
+        x0 = oracle.getrs(trans, lu, piv, b)
+        assert np.array_equal(x0, xt)
+        x, ferr, berr = oracle.gerfs(trans, a, lu, piv, b, x0)
+        assert np.array_equal(x, xt) and np.all(berr == 0)
+        want = forward_bound(a if trans == "N" else a.T, xt, b)
+        assert np.allclose(ferr, want, rtol=1e-12, atol=0), (n, ferr, want)
+
+
+@pytest.mark.parametrize("mistake", [10, 11])
+def test_planted_ferr_mistakes_are_caught(mistake):
+    oracle.plant_mistake(mistake)
+    off = []
+    for trans in "NT":
+        a, xt, b = permuted_triangular_system(40, 3, 40, trans)
+        lu, piv, _ = oracle.getrf(a)
+        _, ferr, _ = oracle.gerfs(trans, a, lu, piv, b, xt)
+        off.append(np.abs(ferr / forward_bound(a if trans == "N" else a.T, xt, b) - 1).max())
+    assert max(off) > 1e-3
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_gerfs_refinement_criteria(trans):
+    """hwyBLAS's refinement test (tests/lapack/linear_test.cc:498-560), with
+    exact integer solutions: X = X_true (1 + 1e-6 u) is refined to about
+    the rounding error and its error bounded by ferr; at B scale 1,
+    berr < 4 (n + 1) eps and ferr < 1e3 n eps; at 2^-982 safe1 dominates
+    and ferr >= 0.3 safe1 / (||op(A)||inf ||X||inf)."""
+    e = np.finfo(np.float64).eps / 2
+    rng = np.random.default_rng(19)
+    for n in (1, 6, 40):
+        for bscale in (1.0, 2.0 ** -982):
+            a = rng.integers(-5, 6, (n, n)).astype(float) + np.diag(np.full(n, 6.0 * n))
+            opa = a if trans == "N" else a.T
+            xt = rng.integers(-50, 51, (n, 2)).astype(float) * bscale
+            b = opa @ xt
+            lu, piv, _ = oracle.getrf(a)
+            x0 = xt * (1 + 1e-6 * rng.uniform(-1, 1, xt.shape))
+            x, ferr, berr = oracle.gerfs(trans, a, lu, piv, b, x0)
+            for j in range(2):
+                xn = np.abs(x[:, j]).max()
+                err = np.abs(x[:, j] - xt[:, j]).max()
+                assert err / xn < 100 * n * e
+                assert ferr[j] >= 0.3 * err / xn
+                if bscale == 1.0:
+                    assert berr[j] < 4 * (n + 1) * e and ferr[j] < 1e3 * n * e
+                else:
+                    safe1 = (n + 1) * np.finfo(np.float64).tiny
+                    assert ferr[j] >= 0.3 * safe1 / (np.abs(opa).sum(1).max() * xn)
+
+
+def test_gerfs_quick_return_and_bad_trans():
+    a = np.eye(3)
+    lu, piv, _ = oracle.getrf(a)
+    x, ferr, berr = oracle.gerfs("N", a, lu, piv, np.zeros((3, 0)), np.zeros((3, 0)))
+    assert ferr.size == 0 and berr.size == 0
+    with pytest.raises(ValueError):
+        oracle.gerfs("X", a, lu, piv, np.ones((3, 1)), np.ones((3, 1)))
