@@ -944,12 +944,210 @@ void Permute(bool forward, int n, int ncols, double* b, int64_t ldb, const int* 
   HWY_HIP_CHECK(hipGetLastError());
 }
 
-// The row permutation of a factorization, on the device.
+// Inverses of the kPanel x kPanel diagonal blocks of the factors (block b:
+// rows and columns [kPanel b, kPanel b + r)), one workgroup per block and
+// one thread per column j of the inverse, every block in one launch: for L
+// (unit lower) x_i = delta_ij - sum_{q < i} L(i, q) x_q, i ascending; for U
+// (upper) x_i = (delta_ij - sum_{q > i} U(i, q) x_q) / U(i, i), i
+// descending. out: per block kPanel x kPanel column-major, zero beyond r.
+template <bool UPPER>
+__global__ void __launch_bounds__(kPanel)
+    InvertDiagBlocksKernel(const double* __restrict__ a, int64_t lda, int n,
+                           double* __restrict__ out) {
+  __shared__ double t[kPanel][kPanel + 1];
+  const int j = threadIdx.x, i0 = blockIdx.x * kPanel, r = min(kPanel, n - i0);
+  const double* d = a + i0 + int64_t(i0) * lda;
+  {
+    double colv[kPanel];
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i)
+      colv[i] = (j < r && i < r && (UPPER ? i <= j : i > j)) ? d[i + int64_t(j) * lda] : 0.0;
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) t[i][j] = colv[i];
+  }
+  __syncthreads();
+  double x[kPanel];
+  if (UPPER) {
+#pragma unroll
+    for (int i = kPanel - 1; i >= 0; --i) {
+      double v = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+      for (int q = i + 1; q < kPanel; ++q) v = __builtin_fma(-t[i][q], x[q], v);
+      x[i] = i < r ? v / t[i][i] : 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) {
+      double v = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+      for (int q = 0; q < i; ++q) v = __builtin_fma(-t[i][q], x[q], v);
+      x[i] = v;
+    }
+  }
+  double* o = out + int64_t(blockIdx.x) * kPanel * kPanel;
+#pragma unroll
+  for (int i = 0; i < kPanel; ++i) o[i + j * kPanel] = (i < r && j < r) ? x[i] : 0.0;
+}
+
+constexpr int kStepRhs = 16;  // right-hand sides per workgroup of InvStepKernel
+
+// One block step of a blocked triangular solve with the block's inverse:
+// x_i = M b_i for the diagonal block rows [i0, i0 + r) (M = minv, or its
+// transpose when MTRANS), written to x by workgroup 0 of each
+// right-hand-side chunk, then b(s, :) -= T(s, i0 .. i0 + r - 1) x_i for this
+// workgroup's kPanel rows s of [u0, u1), in place. T(s, c) = a[s + c ld]
+// (COLS: T is the stored matrix) or a[s ld + c] (T is its transpose). b's
+// rows [i0, i0 + r) are only read and the updated rows are disjoint from
+// them, so no workgroup reads what another writes. Every global load is
+// issued before the first use (one memory round trip; out-of-range
+// elements read a valid address and are replaced by zero). Sums run in
+// ascending index order.
+template <bool COLS, bool MTRANS>
+__global__ void __launch_bounds__(256)
+    InvStepKernel(const double* __restrict__ a, int64_t ld, const double* __restrict__ minv,
+                  int i0, int r, int u0, int u1, double* __restrict__ b, int64_t ldb,
+                  double* __restrict__ x, int64_t ldx, int nrhs) {
+  constexpr int kPer = kPanel * kPanel / 256;       // M and T elements per thread
+  constexpr int kPerB = kPanel * kStepRhs / 256;   // b_i elements per thread
+  __shared__ double sm[kPanel][kPanel + 1];  // M, then the slab of T
+  __shared__ double sb[kPanel][kStepRhs];
+  __shared__ double sx[kPanel][kStepRhs + 1];
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.y * kStepRhs, nj = min(kStepRhs, nrhs - j0);
+  const int s0 = u0 + blockIdx.x * kPanel;
+  const int sn = max(0, min(kPanel, u1 - s0));
+  const int e = tid % kPanel, jg = (tid / kPanel) * 4;  // row e, right-hand sides jg .. + 3
+  // Element (p, w) of a kPanel x kPanel tile for the q-th load of a thread:
+  // p = idx % kPanel runs along consecutive threads.
+  double mv[kPer], tv[kPer], bv[kPerB], acc[4];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) mv[q] = minv[tid + q * 256];
+#pragma unroll
+  for (int q = 0; q < kPerB; ++q) {
+    const int idx = tid + q * 256, k = idx % kPanel, jj = idx / kPanel;
+    const bool ok = k < r && jj < nj;
+    const double v = b[ok ? (i0 + k) + int64_t(j0 + jj) * ldb : int64_t(i0)];
+    bv[q] = ok ? v : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    // COLS: p indexes rows s (contiguous), w columns; else the reverse.
+    const int idx = tid + q * 256, p = idx % kPanel, w = idx / kPanel;
+    const int ss = COLS ? p : w, k = COLS ? w : p;
+    const bool ok = ss < sn && k < r;
+    const int64_t off = COLS ? int64_t(s0 + ss) + int64_t(i0 + k) * ld
+                             : int64_t(s0 + ss) * ld + int64_t(i0 + k);
+    const double v = a[ok ? off : 0];
+    tv[q] = ok ? v : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const bool ok = e < sn && jg + u < nj;
+    const double v = b[ok ? (s0 + e) + int64_t(j0 + jg + u) * ldb : int64_t(i0)];
+    acc[u] = ok ? v : 0.0;
+  }
+
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int idx = tid + q * 256, p = idx % kPanel, w = idx / kPanel;  // M(p, w)
+    if (MTRANS) sm[w][p] = mv[q];
+    else sm[p][w] = mv[q];
+  }
+#pragma unroll
+  for (int q = 0; q < kPerB; ++q) {
+    const int idx = tid + q * 256;
+    sb[idx % kPanel][idx / kPanel] = bv[q];
+  }
+  __syncthreads();
+  {
+    double xa[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k = 0; k < kPanel; ++k) {
+      const double mk = sm[e][k];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xa[u] = __builtin_fma(mk, sb[k][jg + u], xa[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      sx[e][jg + u] = xa[u];
+      if (blockIdx.x == 0 && e < r && jg + u < nj) x[(i0 + e) + int64_t(j0 + jg + u) * ldx] = xa[u];
+    }
+  }
+  if (sn == 0) return;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int idx = tid + q * 256, p = idx % kPanel, w = idx / kPanel;
+    if (COLS) sm[p][w] = tv[q];
+    else sm[w][p] = tv[q];
+  }
+  __syncthreads();
+  if (e >= sn) return;
+#pragma unroll 4
+  for (int k = 0; k < r; ++k) {
+    const double t = sm[e][k];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = __builtin_fma(-t, sx[k][jg + u], acc[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (jg + u < nj) b[(s0 + e) + int64_t(j0 + jg + u) * ldb] = acc[u];
+}
+
+// A triangular solve of few right-hand sides by kPanel-row blocks with the
+// blocks' inverses (minv, kPanel x kPanel each, transposed when mtrans):
+// forward (T lower: blocks ascending, rows below updated) or backward (T
+// upper: blocks descending, rows above updated). b is consumed (its unsolved
+// rows updated in place); the solution goes to x. One launch per block.
+// trans: T is the transpose of the stored triangle, and M the transpose of
+// the stored inverses (both are then read transposed).
+void SolveWithBlockInverses(bool forward, int n, int nrhs, const double* a, int64_t lda,
+                            bool trans, const double* minv, double* b, int64_t ldb, double* x,
+                            int64_t ldx, hipStream_t s) {
+  const int nblk = (n + kPanel - 1) / kPanel;
+  const unsigned ychunks = unsigned((nrhs + kStepRhs - 1) / kStepRhs);
+  for (int t = 0; t < nblk; ++t) {
+    const int blk = forward ? t : nblk - 1 - t;
+    const int i0 = blk * kPanel, r = std::min(kPanel, n - i0);
+    const int u0 = forward ? i0 + r : 0, u1 = forward ? n : i0;
+    const dim3 grid(unsigned(std::max(1, (u1 - u0 + kPanel - 1) / kPanel)), ychunks);
+    const double* m = minv + int64_t(blk) * kPanel * kPanel;
+    if (trans)
+      hipLaunchKernelGGL((InvStepKernel<false, true>), grid, dim3(256), 0, s, a, lda, m, i0, r,
+                         u0, u1, b, ldb, x, ldx, nrhs);
+    else
+      hipLaunchKernelGGL((InvStepKernel<true, false>), grid, dim3(256), 0, s, a, lda, m, i0, r,
+                         u0, u1, b, ldb, x, ldx, nrhs);
+  }
+  HWY_HIP_CHECK(hipGetLastError());
+}
+
+// The row permutation of a factorization, on the device, and (once asked
+// for) the inverses of its factors' diagonal blocks.
 struct Perm {
   int* src = nullptr;
   int* inv = nullptr;
   double* scratch = nullptr;
   int n = 0, ncols = 0;
+  mutable double* block_inv = nullptr;  // [2][blocks][kPanel x kPanel]: L's, then U's
+  mutable const double* block_inv_of = nullptr;
+
+  // The diagonal-block inverses of the factors in a (computed for the
+  // first a asked about; a Perm serves one factorization).
+  const double* BlockInverses(const double* a, int64_t lda, hipStream_t s) const {
+    const int nblk = (n + kPanel - 1) / kPanel;
+    if (block_inv_of != a) {
+      if (!block_inv)
+        HWY_HIP_CHECK(hipMalloc(&block_inv, size_t(2) * nblk * kPanel * kPanel * sizeof(double)));
+      hipLaunchKernelGGL(InvertDiagBlocksKernel<false>, dim3(nblk), dim3(kPanel), 0, s, a, lda,
+                         n, block_inv);
+      hipLaunchKernelGGL(InvertDiagBlocksKernel<true>, dim3(nblk), dim3(kPanel), 0, s, a, lda, n,
+                         block_inv + int64_t(nblk) * kPanel * kPanel);
+      HWY_HIP_CHECK(hipGetLastError());
+      block_inv_of = a;
+    }
+    return block_inv;
+  }
 
   Perm(int n_, int ncols_, const int* ipiv_dev, hipStream_t s) : n(n_), ncols(ncols_) {
     std::vector<int> piv(n);
@@ -969,6 +1167,7 @@ struct Perm {
     (void)hipFree(src);
     (void)hipFree(inv);
     (void)hipFree(scratch);
+    if (block_inv) (void)hipFree(block_inv);
   }
 };
 
@@ -1039,9 +1238,33 @@ void SolveLowerUnitTransBlocked(int n, int nrhs, const double* a, int64_t lda, d
 constexpr int kBlockedSolveMaxRhs = 64;
 
 // xGETRS (src/lapack/getrf.cc:287-339) with the permutation of ipiv.
+// For nrhs <= kBlockedSolveMaxRhs the triangular solves go by kPanel-row
+// blocks with the diagonal blocks' inverses, one launch per block
+// (HWYBLAS_AMD_GETRS=trsm: a TRSM per diagonal block plus a GEMM update).
+bool GetrsWithTrsm() {
+  const char* e = getenv("HWYBLAS_AMD_GETRS");
+  return e && std::string(e) == "trsm";
+}
+
 void GetrsPerm(bool trans, int n, int nrhs, const double* a, int64_t lda, const Perm& perm,
                double* b, int64_t ldb, hipStream_t s) {
   if (n == 0 || nrhs == 0) return;
+  if (nrhs <= kBlockedSolveMaxRhs && nrhs <= perm.ncols && !GetrsWithTrsm()) {
+    const double* inv = perm.BlockInverses(a, lda, s);
+    const double* linv = inv;
+    const double* uinv = inv + int64_t((n + kPanel - 1) / kPanel) * kPanel * kPanel;
+    double* y = perm.scratch;  // n x nrhs, free between the permutations
+    if (!trans) {
+      Permute(true, n, nrhs, b, ldb, perm.src, perm.inv, perm.scratch, s);
+      SolveWithBlockInverses(true, n, nrhs, a, lda, false, linv, b, ldb, y, n, s);
+      SolveWithBlockInverses(false, n, nrhs, a, lda, false, uinv, y, n, b, ldb, s);
+    } else {
+      SolveWithBlockInverses(true, n, nrhs, a, lda, true, uinv, b, ldb, y, n, s);
+      SolveWithBlockInverses(false, n, nrhs, a, lda, true, linv, y, n, b, ldb, s);
+      Permute(false, n, nrhs, b, ldb, perm.src, perm.inv, perm.scratch, s);
+    }
+    return;
+  }
   if (!trans) {
     Permute(true, n, nrhs, b, ldb, perm.src, perm.inv, perm.scratch, s);
     if (nrhs <= kBlockedSolveMaxRhs) {

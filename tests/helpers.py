@@ -88,12 +88,16 @@ def trsm_residual_ratio(side, uplo, ta, diag, m, n, alpha, a, lda, b0, ldb, x, w
 def permuted_triangular_system(n, nrhs, seed, trans="N"):
     """(A, X, B) with A = P U: U upper triangular with integer entries, a
     power-of-two diagonal at least the row's off-diagonal sum and
-    off-diagonals in {0, -1, -2}; P a random row permutation; X integer,
-    nonzero, |X| <= 50; B = op(A) X, exact. Partial pivoting recovers P with
-    L = I, every solve is exact integer arithmetic (so refinement stops with
-    R = 0), and inv(op(A)) >= 0 entrywise."""
+    off-diagonals in {0, -1, -2}, none inside the 64 x 64 diagonal blocks
+    (so a block solve by substitution or by the block's inverse is exact);
+    P a random row permutation; X integer, nonzero, |X| <= 50;
+    B = op(A) X, exact. Partial pivoting recovers P with L = I, every solve
+    is exact integer arithmetic (so refinement stops with R = 0), and
+    inv(op(A)) >= 0 entrywise."""
     rng = np.random.default_rng(seed)
     u = np.triu(-rng.integers(0, 3, (n, n)).astype(float) * (rng.random((n, n)) < 0.2), 1)
+    blk = np.arange(n) // 64
+    u[blk[:, None] == blk[None, :]] = 0.0
     rs = -u.sum(1)
     u[np.arange(n), np.arange(n)] = 2.0 ** np.ceil(np.log2(rs + 1)) * rng.choice([1, 2], n)
     a = np.asfortranarray(u[rng.permutation(n)])

@@ -108,10 +108,17 @@ def test_getrf_nan_first_row_pivot(H):
 
 
 @pytest.mark.parametrize("trans", ["N", "T"])
-@pytest.mark.parametrize("n,nrhs", [(400, 5), (333, 17), (200, 70)])
-def test_getrs_matches_oracle(H, trans, n, nrhs):
-    """Blocked solves (nrhs <= 64, ragged last block at n = 333) and the
-    recursive TRSMs (nrhs = 70), both transpositions."""
+@pytest.mark.parametrize("n,nrhs", [(400, 5), (333, 17), (200, 70), (1, 3), (64, 64)])
+@pytest.mark.parametrize("mode", [None, "trsm"])
+def test_getrs_matches_oracle(H, trans, n, nrhs, mode, monkeypatch):
+    """Blocked solves with the diagonal blocks' inverses (nrhs <= 64, ragged
+    last block at n = 333, several right-hand-side chunks at 17 and 64), the
+    blocked TRSM solves (HWYBLAS_AMD_GETRS=trsm) and the recursive TRSMs
+    (nrhs = 70), both transpositions."""
+    if mode:
+        monkeypatch.setenv("HWYBLAS_AMD_GETRS", mode)
+    else:
+        monkeypatch.delenv("HWYBLAS_AMD_GETRS", raising=False)
     rng = np.random.default_rng(11)
     a = rng.standard_normal((n, n))
     b = rng.uniform(-1, 1, (n, nrhs))
