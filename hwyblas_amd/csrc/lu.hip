@@ -1000,26 +1000,31 @@ constexpr int kStepRhs = 16;  // right-hand sides per workgroup of InvStepKernel
 // rows [i0, i0 + r) are only read and the updated rows are disjoint from
 // them, so no workgroup reads what another writes. Every global load is
 // issued before the first use (one memory round trip; out-of-range
-// elements read a valid address and are replaced by zero). Sums run in
-// ascending index order.
+// elements read a valid address and are replaced by zero). Both products
+// run on the FP64 MFMA (v_mfma_f64_16x16x4: wave w owns rows 16 w .. + 15
+// of the 16 right-hand sides), the update as D = b + (-T) x with b in the
+// accumulator; M and T are zero beyond r, so every product spans kPanel.
 template <bool COLS, bool MTRANS>
 __global__ void __launch_bounds__(256)
     InvStepKernel(const double* __restrict__ a, int64_t ld, const double* __restrict__ minv,
                   int i0, int r, int u0, int u1, double* __restrict__ b, int64_t ldb,
                   double* __restrict__ x, int64_t ldx, int nrhs) {
+  static_assert(kStepRhs == 16 && kPanel == 64, "one 16 x 16 MFMA tile per wave");
   constexpr int kPer = kPanel * kPanel / 256;       // M and T elements per thread
   constexpr int kPerB = kPanel * kStepRhs / 256;   // b_i elements per thread
-  __shared__ double sm[kPanel][kPanel + 1];  // M, then the slab of T
-  __shared__ double sb[kPanel][kStepRhs];
+  using MF = gemm_detail::Mfma<double>;
+  __shared__ double sm[kPanel][kPanel + 1];  // M, then the slab of -T
+  __shared__ double sb[kPanel][kStepRhs + 1];
   __shared__ double sx[kPanel][kStepRhs + 1];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
   const int j0 = blockIdx.y * kStepRhs, nj = min(kStepRhs, nrhs - j0);
   const int s0 = u0 + blockIdx.x * kPanel;
   const int sn = max(0, min(kPanel, u1 - s0));
-  const int e = tid % kPanel, jg = (tid / kPanel) * 4;  // row e, right-hand sides jg .. + 3
   // Element (p, w) of a kPanel x kPanel tile for the q-th load of a thread:
   // p = idx % kPanel runs along consecutive threads.
-  double mv[kPer], tv[kPer], bv[kPerB], acc[4];
+  double mv[kPer], tv[kPer], bv[kPerB];
+  MF::Acc acc;
 #pragma unroll
   for (int q = 0; q < kPer; ++q) mv[q] = minv[tid + q * 256];
 #pragma unroll
@@ -1038,13 +1043,14 @@ __global__ void __launch_bounds__(256)
     const int64_t off = COLS ? int64_t(s0 + ss) + int64_t(i0 + k) * ld
                              : int64_t(s0 + ss) * ld + int64_t(i0 + k);
     const double v = a[ok ? off : 0];
-    tv[q] = ok ? v : 0.0;
+    tv[q] = ok ? -v : 0.0;
   }
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const bool ok = e < sn && jg + u < nj;
-    const double v = b[ok ? (s0 + e) + int64_t(j0 + jg + u) * ldb : int64_t(i0)];
-    acc[u] = ok ? v : 0.0;
+  for (int rr = 0; rr < 4; ++rr) {
+    const int row = 16 * wave + MF::Row(lk, rr);
+    const bool ok = row < sn && li < nj;
+    const double v = b[ok ? (s0 + row) + int64_t(j0 + li) * ldb : int64_t(i0)];
+    acc[rr] = ok ? v : 0.0;
   }
 
 #pragma unroll
@@ -1060,17 +1066,15 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   {
-    double xa[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int k = 0; k < kPanel; ++k) {
-      const double mk = sm[e][k];
+    MF::Acc xa = {};
 #pragma unroll
-      for (int u = 0; u < 4; ++u) xa[u] = __builtin_fma(mk, sb[k][jg + u], xa[u]);
-    }
+    for (int k0 = 0; k0 < kPanel; k0 += 4)
+      xa = MF::Run(sm[16 * wave + li][k0 + lk], sb[k0 + lk][li], xa);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      sx[e][jg + u] = xa[u];
-      if (blockIdx.x == 0 && e < r && jg + u < nj) x[(i0 + e) + int64_t(j0 + jg + u) * ldx] = xa[u];
+    for (int rr = 0; rr < 4; ++rr) {
+      const int row = 16 * wave + MF::Row(lk, rr);
+      sx[row][li] = xa[rr];
+      if (blockIdx.x == 0 && row < r && li < nj) x[(i0 + row) + int64_t(j0 + li) * ldx] = xa[rr];
     }
   }
   if (sn == 0) return;
@@ -1082,16 +1086,14 @@ __global__ void __launch_bounds__(256)
     else sm[w][p] = tv[q];
   }
   __syncthreads();
-  if (e >= sn) return;
-#pragma unroll 4
-  for (int k = 0; k < r; ++k) {
-    const double t = sm[e][k];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc[u] = __builtin_fma(-t, sx[k][jg + u], acc[u]);
+  for (int k0 = 0; k0 < kPanel; k0 += 4)
+    acc = MF::Run(sm[16 * wave + li][k0 + lk], sx[k0 + lk][li], acc);
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) {
+    const int row = 16 * wave + MF::Row(lk, rr);
+    if (row < sn && li < nj) b[(s0 + row) + int64_t(j0 + li) * ldb] = acc[rr];
   }
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-    if (jg + u < nj) b[(s0 + e) + int64_t(j0 + jg + u) * ldb] = acc[u];
 }
 
 // A triangular solve of few right-hand sides by kPanel-row blocks with the
@@ -1302,11 +1304,13 @@ constexpr int kMaxRhs = 16;
 constexpr int kResChunk = 512;  // columns of A per workgroup of the residual
 
 // Partial sums over the columns [k0, k0 + kResChunk) of A: rp = sum a x,
-// wp = sum |a| |x| for every row (one thread each) and right-hand side.
+// wp = sum |a| |x| for every row (one thread each) and right-hand side;
+// with ap, also the row's sum of |a| over the chunk (for ||A||inf).
 __global__ void __launch_bounds__(256)
     ResidualPartialKernel(const double* __restrict__ a, int64_t lda, int n,
                           const double* __restrict__ x, int64_t ldx, int nrhs,
-                          double* __restrict__ rp, double* __restrict__ wp) {
+                          double* __restrict__ rp, double* __restrict__ wp,
+                          double* __restrict__ ap) {
   __shared__ double xs[kResChunk][kMaxRhs];
   const int k0 = blockIdx.y * kResChunk;
   const int kn = min(kResChunk, n - k0);
@@ -1321,9 +1325,11 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int j = 0; j < kMaxRhs; ++j) r[j] = w[j] = 0.0;
   const double* ai = a + i + int64_t(k0) * lda;
+  double sa = 0.0;
   for (int kk = 0; kk < kn; ++kk) {
     const double v = ai[int64_t(kk) * lda];
     const double av = fabs(v);
+    sa += av;
 #pragma unroll
     for (int j = 0; j < kMaxRhs; ++j) {
       r[j] = __builtin_fma(v, xs[kk][j], r[j]);
@@ -1335,6 +1341,7 @@ __global__ void __launch_bounds__(256)
     rp[base + int64_t(j) * n] = r[j];
     wp[base + int64_t(j) * n] = w[j];
   }
+  if (ap) ap[int64_t(blockIdx.y) * n + i] = sa;
 }
 
 // R = B - sum rp; D = |B| + sum wp; per workgroup and right-hand side the
@@ -1404,13 +1411,13 @@ __global__ void __launch_bounds__(256)
   z[i + int64_t(c) * n] *= w[i + int64_t(list.col[c]) * n];
 }
 
-// Largest row sum of |A| (the infinity norm).
-__global__ void RowAbsSumKernel(const double* __restrict__ a, int64_t lda, int n,
+// Row sums of |A| from the residual pass's per-chunk sums (chunks in order).
+__global__ void RowAbsSumKernel(const double* __restrict__ ap, int nchunks, int n,
                                 double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (int k = 0; k < n; ++k) s += fabs(a[i + int64_t(k) * lda]);
+  for (int c = 0; c < nchunks; ++c) s += ap[int64_t(c) * n + i];
   out[i] = s;
 }
 
@@ -1671,7 +1678,7 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
 
   const int nchunks = (n + lu::kResChunk - 1) / lu::kResChunk;
   const int rblocks = (n + 255) / 256;
-  double *rp, *wp, *r, *stats, *rowsum, *wts = nullptr, *z = nullptr;
+  double *rp, *wp, *r, *stats, *rowsum, *rowpart, *wts = nullptr, *z = nullptr;
   int* active_d;
   if (ferr) {
     HWY_HIP_CHECK(hipMalloc(&wts, size_t(n) * nrhs * 8));
@@ -1682,6 +1689,7 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
   HWY_HIP_CHECK(hipMalloc(&r, size_t(n) * nrhs * 8));
   HWY_HIP_CHECK(hipMalloc(&stats, size_t(rblocks) * 2 * nrhs * 8));
   HWY_HIP_CHECK(hipMalloc(&rowsum, size_t(n) * 8));
+  HWY_HIP_CHECK(hipMalloc(&rowpart, size_t(nchunks) * n * 8));
   HWY_HIP_CHECK(hipMalloc(&active_d, lu::kMaxRhs * sizeof(int)));
   const double eps = std::numeric_limits<double>::epsilon() * 0.5;  // LAPACK's dlamch('E')
   const double safmin = std::numeric_limits<double>::min();
@@ -1689,9 +1697,10 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
   std::vector<double> host_stats(size_t(rblocks) * 2 * nrhs);
   std::vector<double> lstres(nrhs, 3.0), rinf(nrhs, 0.0);
   std::vector<int> count(nrhs, 1), active(nrhs, 1), done(nrhs, 0);
-  for (;;) {
+  for (bool first = true;; first = false) {
     hipLaunchKernelGGL(lu::ResidualPartialKernel, dim3(rblocks, nchunks), dim3(256), 0, s, a,
-                       int64_t(lda), n, x, int64_t(ldx), nrhs, rp, wp);
+                       int64_t(lda), n, x, int64_t(ldx), nrhs, rp, wp,
+                       first ? rowpart : nullptr);
     hipLaunchKernelGGL(lu::ResidualFinishKernel, dim3(rblocks), dim3(256), 0, s, rp, wp, nchunks,
                        b, int64_t(ldb), n, nrhs, r, int64_t(n), safe1, safe2, stats, wts,
                        double(n + 1) * eps);
@@ -1731,7 +1740,7 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
   std::vector<double> est(nrhs, 0.0);
   if (ferr) lu::ForwardErrorEstimates(n, nrhs, af, ldaf, perm, wts, z, s, est.data());
   // Normwise backward error of the final X.
-  hipLaunchKernelGGL(lu::RowAbsSumKernel, dim3(rblocks), dim3(256), 0, s, a, int64_t(lda), n,
+  hipLaunchKernelGGL(lu::RowAbsSumKernel, dim3(rblocks), dim3(256), 0, s, rowpart, nchunks, n,
                      rowsum);
   std::vector<double> rows(n), xs(size_t(n) * nrhs);
   HWY_HIP_CHECK(hipMemcpyAsync(rows.data(), rowsum, size_t(n) * 8, hipMemcpyDeviceToHost, s));
@@ -1748,7 +1757,8 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
   }
   for (void* p : {static_cast<void*>(rp), static_cast<void*>(wp), static_cast<void*>(r),
                   static_cast<void*>(stats), static_cast<void*>(rowsum),
-                  static_cast<void*>(active_d), static_cast<void*>(wts), static_cast<void*>(z)})
+                  static_cast<void*>(rowpart), static_cast<void*>(active_d),
+                  static_cast<void*>(wts), static_cast<void*>(z)})
     if (p) HWY_HIP_CHECK(hipFree(p));
   return 0;
 }
