@@ -246,7 +246,8 @@ __device__ __forceinline__ void SliceEight(const double (&x)[8], int sc, const C
 }
 
 // x(r, p) = x[r * ld + p] (p contiguous): each thread reads its 8 elements
-// directly. Block: 16 rows x 128 p.
+// directly, all loads in flight at once (out-of-range elements read x[0]
+// and become zero). Block: 16 rows x 128 p.
 __global__ void __launch_bounds__(256)
     SliceRowsKernel(const double* x, int64_t ld, int64_t R, int64_t K, int64_t RP, int64_t KP,
                     const int* s, Crt crt, int8_t* Y) {
@@ -256,7 +257,11 @@ __global__ void __launch_bounds__(256)
   if (r >= RP) return;
   double v[8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) v[q] = (r < R && p0 + q < K) ? x[r * ld + p0 + q] : 0.0;
+  for (int q = 0; q < 8; ++q) {
+    const bool ok = r < R && p0 + q < K;
+    const double t = x[ok ? r * ld + p0 + q : 0];
+    v[q] = ok ? t : 0.0;
+  }
   SliceEight(v, r < R ? s[r] : 0, crt, Y + r * KP + p0, RP * KP);
 }
 
@@ -265,14 +270,25 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     SliceColsKernel(const double* x, int64_t ld, int64_t R, int64_t K, int64_t RP, int64_t KP,
                     const int* s, Crt crt, int8_t* Y) {
-  constexpr int TR = 32, TP = 128;
+  constexpr int TR = 32, TP = 128, PER = TR * TP / 256;
   __shared__ double T[TP][TR + 1];
   const int tid = threadIdx.x;
   const int64_t r0 = int64_t(blockIdx.y) * TR, p0 = int64_t(blockIdx.x) * TP;
-  for (int idx = tid; idx < TR * TP; idx += 256) {
-    const int rr = idx % TR, pp = idx / TR;
+  // Every load in flight before the first LDS store (out-of-range elements
+  // read x[0] and become zero).
+  double tv[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int idx = tid + q * 256, rr = idx % TR, pp = idx / TR;
     const int64_t r = r0 + rr, p = p0 + pp;
-    T[pp][rr] = (r < R && p < K) ? x[r + p * ld] : 0.0;
+    const bool ok = r < R && p < K;
+    const double t = x[ok ? r + p * ld : 0];
+    tv[q] = ok ? t : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int idx = tid + q * 256;
+    T[idx / TR][idx % TR] = tv[q];
   }
   __syncthreads();
   const int pq = (tid & 15) * 8;
@@ -719,11 +735,17 @@ __global__ void __launch_bounds__(256)
     const int64_t i0 = (idx % quads) * 4, j = idx / quads;
     double S[4][kLimbs] = {};
     const uint8_t* rp = R + i0 + j * ldr;
-    for (int l = 0; l < crt.nmod; ++l) {
-      const uint32_t word = *reinterpret_cast<const uint32_t*>(rp + l * r_plane);
+    // Every plane's word in flight at once (nmod is uniform).
+    uint32_t words[kMaxModuli];
+#pragma unroll
+    for (int l = 0; l < kMaxModuli; ++l)
+      words[l] = l < crt.nmod ? *reinterpret_cast<const uint32_t*>(rp + l * r_plane) : 0u;
+#pragma unroll
+    for (int l = 0; l < kMaxModuli; ++l) {
+      if (l >= crt.nmod) break;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double cl = double((word >> (8 * q)) & 0xFFu);
+        const double cl = double((words[l] >> (8 * q)) & 0xFFu);
 #pragma unroll
         for (int t = 0; t < kLimbs; ++t) S[q][t] = __builtin_fma(cl, crt.frac[l][t], S[q][t]);
       }
