@@ -483,6 +483,7 @@ __global__ void __launch_bounds__(kPersistThreads) PanelPersistentKernel(Persist
   __shared__ double su[kPanel];                   // the pivot row
   __shared__ double sl[kPersistRows];             // multipliers of the current column
   __shared__ double srec[kPersistMaxGroups + 1][kRecord];
+  __shared__ double spub[2][kPanel];             // the candidate row, row jn
   __shared__ double hv[2];
   __shared__ int hrow[2];
   __shared__ int s_piv;
@@ -564,28 +565,55 @@ __global__ void __launch_bounds__(kPersistThreads) PanelPersistentKernel(Persist
       bv = hv[1];
       br = hrow[1];
     }
-    if (threadIdx.x == 0) {
-      PutRecord(slot + int64_t(gid) * kRecord, bv);
-      PutRecord(slot + int64_t(gid) * kRecord + 1, double(br));
-    }
+    // The candidate row and row jn go through LDS, so that every thread
+    // publishes at most one value (the compiler waits for earlier
+    // agent-scope stores before each group of them).
 #pragma unroll
     for (int q = 0; q < kPersistRpt; ++q) {
       if (rows[q] == br) {
 #pragma unroll
-        for (int t = 0; t < kGroupCols; ++t)
-          PutRecord(slot + int64_t(gid) * kRecord + 2 + c0 + t, v[q][t]);
+        for (int t = 0; t < kGroupCols; ++t) spub[0][c0 + t] = v[q][t];
       }
       if (rows[q] == jn) {
 #pragma unroll
-        for (int t = 0; t < kGroupCols; ++t)
-          PutRecord(slot + int64_t(G) * kRecord + 2 + c0 + t, v[q][t]);
+        for (int t = 0; t < kGroupCols; ++t) spub[1][c0 + t] = v[q][t];
+      }
+    }
+    __syncthreads();
+    {
+      const int t = threadIdx.x;
+      const bool own_jn = jn >= gid * kPersistRows && jn < (gid + 1) * kPersistRows && jn < m;
+      if (t < kPanel) {
+        if (br != INT_MAX) PutRecord(slot + int64_t(gid) * kRecord + 2 + t, spub[0][t]);
+      } else if (t < 2 * kPanel) {
+        if (own_jn) PutRecord(slot + int64_t(G) * kRecord + 2 + (t - kPanel), spub[1][t - kPanel]);
+      } else if (t == 2 * kPanel) {
+        PutRecord(slot + int64_t(gid) * kRecord, bv);
+      } else if (t == 2 * kPanel + 1) {
+        PutRecord(slot + int64_t(gid) * kRecord + 1, double(br));
       }
     }
     if (!GridSync(g.bar, unsigned(jn + 1) * unsigned(G), g.failed)) return;
 
-    // Every record in one pass, then the pivot (the same in every workgroup).
-    for (int idx = threadIdx.x; idx < (G + 1) * kRecord; idx += kPersistThreads)
-      (&srec[0][0])[idx] = GetRecord(slot + idx);
+    // Every record in one pass (all of a thread's loads in flight before
+    // the first LDS store; the guard is uniform), then the pivot (the same
+    // in every workgroup).
+    {
+      constexpr int kRecLoads =
+          ((kPersistMaxGroups + 1) * kRecord + kPersistThreads - 1) / kPersistThreads;
+      const int nrec = (G + 1) * kRecord;
+      double rv[kRecLoads];
+#pragma unroll
+      for (int q = 0; q < kRecLoads; ++q) {
+        const int idx = threadIdx.x + q * kPersistThreads;
+        if (q * kPersistThreads < nrec) rv[q] = GetRecord(slot + (idx < nrec ? idx : 0));
+      }
+#pragma unroll
+      for (int q = 0; q < kRecLoads; ++q) {
+        const int idx = threadIdx.x + q * kPersistThreads;
+        if (q * kPersistThreads < nrec && idx < nrec) (&srec[0][0])[idx] = rv[q];
+      }
+    }
     __syncthreads();
     if (wave == 0) {
       // G <= 128 candidates, two per lane.
