@@ -691,6 +691,8 @@ def main():
                     help="the mixed-precision LU solve (n = --lu-n, kappa 1e2, 1e6, 1e10) "
                          "instead of the headline")
     ap.add_argument("--lu-n", type=int, default=16384)
+    ap.add_argument("--lu-kappas", default="2,6,10",
+                    help="comma-separated log10 condition numbers for --lu")
     ap.add_argument("--dists", default="bc",
                     help="wide-range operand distributions to run after the headline (b, c)")
     ap.add_argument("--moduli-sweep", default="",
@@ -733,7 +735,8 @@ def main():
         del src, dst
 
     if args.lu:
-        print(json.dumps({"lu": lu_leg(H, args.lu_n, (2, 6, 10), stream)}), flush=True)
+        kappas = tuple(int(x) for x in args.lu_kappas.split(",") if x)
+        print(json.dumps({"lu": lu_leg(H, args.lu_n, kappas, stream)}), flush=True)
         return
     if args.plain_sweep:
         print(json.dumps({"plain_sweep": plain_sweep(H, oracle)}), flush=True)
