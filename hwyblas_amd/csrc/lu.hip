@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(kStepRows)
       grow = j + 1;
       if (gv == 0.0) atomicMin(info, row0 + j + 2);
     }
-    ipiv[j + 1] = grow + 1;
+    ipiv[j + 1] = row0 + grow + 1;  // 1-based row of the whole matrix
     piv_row = grow;
     *counter = 0;
   }
@@ -210,7 +210,7 @@ constexpr int kMaxAffected = 2 * kPanel;
 // entries through LDS, all loads before all stores.
 __global__ void __launch_bounds__(64)
     PanelSwapKernel(double* __restrict__ full, int64_t lda, int ncols, int pc0, int nb,
-                    const int* __restrict__ ipiv, int count) {
+                    const int* __restrict__ ipiv, int count, int row0) {
   __shared__ int L[kMaxAffected], S[kMaxAffected];
   __shared__ int naff;
   __shared__ double buf[kMaxAffected][64 + 1];
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(64)
   int extra_row = -1, extra_src = -1, na = count;
   __syncthreads();
   for (int i = 0; i < count; ++i) {
-    const int q = ipiv[i] - 1;
+    const int q = ipiv[i] - 1 - row0;  // relative to the panel's first row
     if (q < count) {
       if (lane == 0) {
         const int tmp = S[i];
@@ -295,10 +295,6 @@ __global__ void CopyBack(const double* __restrict__ in, int n, int ncols, double
   for (int c = 0; c < ncols; ++c) b[i + int64_t(c) * ldb] = in[i + int64_t(c) * n];
 }
 
-__global__ void AddToPivots(int* __restrict__ ipiv, int count, int add) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) ipiv[i] += add;
-}
 
 __global__ void FillInt(int* __restrict__ x, int count, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -401,9 +397,10 @@ struct PersistArgs {
   int* info;
   double sfmin;
   double* rec;      // [2][gridDim.x + 1][kRecord]; slot gridDim.x: row j + 1
-  unsigned* bar;    // arrivals (0 at launch)
+  unsigned* bar;    // arrivals (bar_base at launch: the count runs on over panels)
   unsigned* failed;
   double* linv;     // the inverse of the panel's unit lower triangle (kPanel x kPanel)
+  unsigned bar_base;
 };
 
 // Arrive, then wait until `target` arrivals in all. Every thread's record
@@ -593,7 +590,7 @@ __global__ void __launch_bounds__(kPersistThreads) PanelPersistentKernel(Persist
         PutRecord(slot + int64_t(gid) * kRecord + 1, double(br));
       }
     }
-    if (!GridSync(g.bar, unsigned(jn + 1) * unsigned(G), g.failed)) return;
+    if (!GridSync(g.bar, g.bar_base + unsigned(jn + 1) * unsigned(G), g.failed)) return;
 
     // Every record in one pass (all of a thread's loads in flight before
     // the first LDS store; the guard is uniform), then the pivot (the same
@@ -634,7 +631,7 @@ __global__ void __launch_bounds__(kPersistThreads) PanelPersistentKernel(Persist
           grow = jn;
           if (gv == 0.0 && gid == 0) atomicMin(g.info, g.row0 + jn + 1);
         }
-        if (gid == 0) g.ipiv[jn] = grow + 1;
+        if (gid == 0) g.ipiv[jn] = g.row0 + grow + 1;  // 1-based row of the whole matrix
         s_piv = grow;
       }
     }
@@ -763,6 +760,7 @@ struct Work {
   int persist_max_groups = 0;
   double* prec = nullptr;      // [2][groups + 1][kRecord]
   unsigned* sync = nullptr;    // [0]: arrivals, [64]: failed
+  unsigned arrivals = 0;       // the arrival count after the launches so far
 };
 
 // The panel factorization as one launch per column step.
@@ -788,10 +786,11 @@ void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, 
   const int groups = (m + kPersistRows - 1) / kPersistRows;
   const bool persistent = groups <= w.persist_max_groups;
   if (persistent) {
-    hipLaunchKernelGGL(FillUnsigned, dim3(1), dim3(64), 0, s, w.sync, 1, 0u);
-    PersistArgs g{a,          lda,  m,          nb,      row0,
-                  ipiv,       w.info, sfmin,    w.prec,  w.sync,
-                  w.sync + 64, w.linv + int64_t(row0 / kPanel) * kPanel * kPanel};
+    PersistArgs g{a,           lda,    m,     nb,     row0,
+                  ipiv,        w.info, sfmin, w.prec, w.sync,
+                  w.sync + 64, w.linv + int64_t(row0 / kPanel) * kPanel * kPanel,
+                  w.arrivals};
+    w.arrivals += unsigned(mn) * unsigned(groups);  // one arrival per workgroup and column
     void* args[] = {&g};
     HWY_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(PanelPersistentKernel),
                                              dim3(groups), dim3(kPersistThreads), args, 0, s));
@@ -805,7 +804,7 @@ void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, 
   const int others = w.ncols - nb;
   if (others > 0)
     hipLaunchKernelGGL(PanelSwapKernel, dim3((others + 63) / 64), dim3(64), 0, s, full, lda,
-                       w.ncols, pc0, nb, ipiv, mn);
+                       w.ncols, pc0, nb, ipiv, mn, row0);
   // The inverse of the panel's unit lower triangle (final from here on: later
   // interchanges only move rows below it), for the TRSMs with it; the
   // cooperative panel kernel has already written it.
@@ -887,9 +886,8 @@ void Recurse(double* a, int64_t lda, int m, int n, int row0, int* ipiv, int upda
   TrsmUnitLower(a, lda, n1, a + int64_t(n1) * lda, lda, n2, row0, w, s);
   Update(m - n1, n2, n1, a + n1, a + int64_t(n1) * lda, a + n1 + int64_t(n1) * lda, lda, update,
          w, s);
+  // (Pivots are written as rows of the whole matrix, so none needs an offset.)
   Recurse(a + n1 + int64_t(n1) * lda, lda, m - n1, n2, row0 + n1, ipiv + n1, update, w, s);
-  const int cnt = mn - n1;
-  hipLaunchKernelGGL(AddToPivots, dim3(GridFor(cnt)), dim3(256), 0, s, ipiv + n1, cnt, n1);
 }
 
 // How many workgroups of PanelPersistentKernel can be resident at once on
