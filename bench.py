@@ -399,7 +399,7 @@ def plain_sweep(H, oracle, sizes=(4096, 8192, 16384), reps=3, samples=64):
 
 METHODS = {"int8": "INT8 Ozaki scheme II (v_mfma_i32_32x32x32_i8)",
            "bf16": "BF16 slices, Ozaki scheme I (v_mfma_f32_32x32x16_bf16, exact FP32 "
-                   "chunks of 256)"}
+                   "chunks of 1024)"}
 DISTRIBUTIONS = {
     "b": "+-u 2^e, u uniform in [1,2), e uniform integer in [-30,30] (seeds 11, 12)",
     "c": "uniform[-1,1] (seeds 1, 2), rows of A and columns of B scaled by 10^s, "
