@@ -841,7 +841,7 @@ def main():
                    "speedup_vs_native": round(emu_tf / nat_tf, 3),
                    "native_roofline_frac_fp64": round(nat_tf / PEAK_FP64, 4),
                    "phases_ms": phases},
-        "roofline": {"bound": "mfma", "kernel": "ModGemmKernel (INT8, all moduli)",
+        "roofline": {"bound": "mfma", "kernel": "ModGemmDmaKernel (INT8, all moduli)",
                      "achieved": round(int8_tops, 2), "peak": PEAK_INT8, "unit": "TFLOP/s",
                      "ops": "int8 multiply-adds counted as 2 ops (TOP/s); algorithmic "
                             "2 m n k per modulus",
