@@ -184,6 +184,43 @@ def trsm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, dtype=None):
       _sp(alpha, dtype), _ptr(a), _ip(lda), _ptr(b), _ip(ldb))
 
 
+def symm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, dtype=None):
+    """Fortran ?symm_ (host arrays or DeviceArrays); C in place."""
+    dtype = dtype or c.dtype
+    f = getattr(lib(), f"{_pre(dtype)}symm_")
+    f(side.encode(), uplo.encode(), _ip(m), _ip(n), _sp(alpha, dtype), _ptr(a), _ip(lda),
+      _ptr(b), _ip(ldb), _sp(beta, dtype), _ptr(c), _ip(ldc))
+
+
+def syrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc, dtype=None):
+    dtype = dtype or c.dtype
+    f = getattr(lib(), f"{_pre(dtype)}syrk_")
+    f(uplo.encode(), trans.encode(), _ip(n), _ip(k), _sp(alpha, dtype), _ptr(a), _ip(lda),
+      _sp(beta, dtype), _ptr(c), _ip(ldc))
+
+
+def syr2k(uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc, dtype=None):
+    dtype = dtype or c.dtype
+    f = getattr(lib(), f"{_pre(dtype)}syr2k_")
+    f(uplo.encode(), trans.encode(), _ip(n), _ip(k), _sp(alpha, dtype), _ptr(a), _ip(lda),
+      _ptr(b), _ip(ldb), _sp(beta, dtype), _ptr(c), _ip(ldc))
+
+
+def gemmtr(uplo, ta, tb, n, k, alpha, a, lda, b, ldb, beta, c, ldc, dtype=None, name="gemmtr"):
+    """Fortran ?gemmtr_ (or, with name="gemmt", ?gemmt_)."""
+    dtype = dtype or c.dtype
+    f = getattr(lib(), f"{_pre(dtype)}{name}_")
+    f(uplo.encode(), ta.encode(), tb.encode(), _ip(n), _ip(k), _sp(alpha, dtype), _ptr(a),
+      _ip(lda), _ptr(b), _ip(ldb), _sp(beta, dtype), _ptr(c), _ip(ldc))
+
+
+def trmm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, dtype=None):
+    dtype = dtype or b.dtype
+    f = getattr(lib(), f"{_pre(dtype)}trmm_")
+    f(side.encode(), uplo.encode(), ta.encode(), diag.encode(), _ip(m), _ip(n),
+      _sp(alpha, dtype), _ptr(a), _ip(lda), _ptr(b), _ip(ldb))
+
+
 def cblas_gemm(layout, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, dtype=None):
     dtype = dtype or c.dtype
     f = getattr(lib(), f"cblas_{_pre(dtype)}gemm")

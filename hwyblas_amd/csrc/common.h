@@ -86,13 +86,14 @@ void SetCblasContext(const CblasContext* context);
 // grow-only device staging buffers, reused across calls like hwyBLAS's
 // thread-local scratch slots (src/common.cc:46-64).
 
-enum StagingSlot : int { kSlotA = 0, kSlotB = 1, kSlotC = 2, kNumSlots = 3 };
+// kSlotD: a dense copy of a structured operand (SYMM, TRMM).
+enum StagingSlot : int { kSlotA = 0, kSlotB = 1, kSlotC = 2, kSlotD = 3, kNumSlots = 4 };
 
 struct ThreadContext {
   hipStream_t stream = nullptr;
   int device = -1;
-  void* buf[kNumSlots] = {nullptr, nullptr, nullptr};
-  size_t cap[kNumSlots] = {0, 0, 0};
+  void* buf[kNumSlots] = {nullptr, nullptr, nullptr, nullptr};
+  size_t cap[kNumSlots] = {0, 0, 0, 0};
 };
 
 ThreadContext& Context();
@@ -121,6 +122,23 @@ template <>
 void GemmDevice<float>(bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
                        const float* a, int64_t lda, const float* b, int64_t ldb, float beta,
                        float* c, int64_t ldc, hipStream_t stream);
+
+// As GemmDevice, computing and writing only part of C: 1 its lower
+// triangle (row >= col), 2 its upper triangle (row <= col), 0 all of it.
+template <typename T>
+void GemmDevicePart(int part, bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha,
+                    const T* a, int64_t lda, const T* b, int64_t ldb, T beta, T* c,
+                    int64_t ldc, hipStream_t stream);
+template <>
+void GemmDevicePart<double>(int part, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                            double alpha, const double* a, int64_t lda, const double* b,
+                            int64_t ldb, double beta, double* c, int64_t ldc,
+                            hipStream_t stream);
+template <>
+void GemmDevicePart<float>(int part, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                           float alpha, const float* a, int64_t lda, const float* b,
+                           int64_t ldb, float beta, float* c, int64_t ldc,
+                           hipStream_t stream);
 
 // Tile configuration GemmDevice uses for a shape (an index into the table of
 // gemm_configs.h).

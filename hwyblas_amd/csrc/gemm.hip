@@ -8,12 +8,13 @@
 
 namespace hwy_amd {
 
+// part: 0 all of C, 1 its lower triangle, 2 its upper triangle.
 void GemmLaunchF64(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
                    const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
-                   double* c, int64_t ldc, hipStream_t stream);
+                   double* c, int64_t ldc, hipStream_t stream, int part = 0);
 void GemmLaunchF32(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
                    const float* a, int64_t lda, const float* b, int64_t ldb, float beta,
-                   float* c, int64_t ldc, hipStream_t stream);
+                   float* c, int64_t ldc, hipStream_t stream, int part = 0);
 
 namespace {
 
@@ -56,6 +57,24 @@ void GemmDevice<float>(bool ta, bool tb, int64_t m, int64_t n, int64_t k, float 
                        float* c, int64_t ldc, hipStream_t stream) {
   const int cfg = ChooseGemmConfig(4, ta, tb, m, n, k);
   GemmLaunchF32(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+
+template <>
+void GemmDevicePart<double>(int part, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                            double alpha, const double* a, int64_t lda, const double* b,
+                            int64_t ldb, double beta, double* c, int64_t ldc,
+                            hipStream_t stream) {
+  const int cfg = ChooseGemmConfig(8, ta, tb, m, n, k);
+  GemmLaunchF64(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, part);
+}
+
+template <>
+void GemmDevicePart<float>(int part, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                           float alpha, const float* a, int64_t lda, const float* b,
+                           int64_t ldb, float beta, float* c, int64_t ldc,
+                           hipStream_t stream) {
+  const int cfg = ChooseGemmConfig(4, ta, tb, m, n, k);
+  GemmLaunchF32(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, part);
 }
 
 namespace {

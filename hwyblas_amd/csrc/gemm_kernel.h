@@ -66,6 +66,10 @@ struct GemmParams {
   int64_t m, n, k, lda, ldb, ldc;
   T alpha, beta;
   int tiles_m, tiles_n;
+  // The part of C computed and written: 0 all of it, 1 the lower triangle
+  // (row >= col), 2 the upper triangle (row <= col). Tiles entirely outside
+  // the part do no work.
+  int part = 0;
 };
 
 constexpr int kXcds = 8;    // MI355X: 8 XCDs, each with its own L2
@@ -258,7 +262,8 @@ __device__ __forceinline__ void MacroTile(const GemmParams<T>& p, int64_t m0, in
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t row = m0 + wm0 + ti * 16 + M::Row(lk, r);
-        if (row < p.m && col < p.n) {
+        const bool in_part = p.part == 0 || (p.part == 1 ? row >= col : row <= col);
+        if (row < p.m && col < p.n && in_part) {
           T* cij = p.c + row + col * p.ldc;
           T v = acc[ti][tj][r] * alpha;
           if (!beta_zero) v = beta_one ? v + *cij : __builtin_fma(*cij, beta, v);
@@ -278,7 +283,10 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
   __shared__ T lds[TileLds<T, TA, TB, BM, BN, BK>::kElems];
   int tm, tn;
   TileCoords(p.tiles_m, p.tiles_n, &tm, &tn);
-  MacroTile<T, TA, TB, BM, BN, BK, WAVES_M, WAVES_N>(p, int64_t(tm) * BM, int64_t(tn) * BN, lds);
+  const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
+  // A tile entirely on the other side of the diagonal (uniform exit).
+  if ((p.part == 1 && m0 + BM - 1 < n0) || (p.part == 2 && m0 > n0 + BN - 1)) return;
+  MacroTile<T, TA, TB, BM, BN, BK, WAVES_M, WAVES_N>(p, m0, n0, lds);
 }
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WGM, int WGN, int OCC>

@@ -78,6 +78,75 @@ HWYBLAS_AMD_API void dtrsm_(const char* side, const char* uplo,
                             const hwyblas_int* lda, double* b,
                             const hwyblas_int* ldb);
 
+/* The other real Level-3 routines, on the same GEMM kernel, with hwyBLAS's
+ * argument checks, info positions and quick returns (src/level3.cc:25-345).
+ * Host or device operands as for dgemm_; synchronous.
+ *  - ?symm_: C := alpha A B + beta C (side L) or alpha B A + beta C (R), A
+ *    symmetric, read from its uplo triangle. Replaces ssymm_/dsymm_
+ *    (include/hwyblas/hwyblas.h:591-602).
+ *  - ?syrk_: the uplo triangle of C := alpha A A^T + beta C (trans N) or
+ *    alpha A^T A + beta C (T, C); the other triangle is not written.
+ *    Replaces ssyrk_/dsyrk_ (hwyblas.h:632-641).
+ *  - ?syr2k_: the uplo triangle of C := alpha (op(A) op(B)^T + op(B) op(A)^T)
+ *    + beta C. Replaces ssyr2k_/dsyr2k_ (hwyblas.h:668-679).
+ *  - ?gemmtr_ and ?gemmt_ (its other name, reported as ?GEMMT): the uplo
+ *    triangle of C := alpha op(A) op(B) + beta C, op(A) n x k. Replace
+ *    sgemmtr_/dgemmtr_/sgemmt_/dgemmt_ (hwyblas.h:711-748).
+ *  - ?trmm_: B := alpha op(A) B (side L) or alpha B op(A), A triangular
+ *    (diag U: unit, not read). Replaces strmm_/dtrmm_ (hwyblas.h:764-774). */
+HWYBLAS_AMD_API void ssymm_(const char* side, const char* uplo, const hwyblas_int* m,
+                            const hwyblas_int* n, const float* alpha, const float* a,
+                            const hwyblas_int* lda, const float* b, const hwyblas_int* ldb,
+                            const float* beta, float* c, const hwyblas_int* ldc);
+HWYBLAS_AMD_API void dsymm_(const char* side, const char* uplo, const hwyblas_int* m,
+                            const hwyblas_int* n, const double* alpha, const double* a,
+                            const hwyblas_int* lda, const double* b, const hwyblas_int* ldb,
+                            const double* beta, double* c, const hwyblas_int* ldc);
+HWYBLAS_AMD_API void ssyrk_(const char* uplo, const char* trans, const hwyblas_int* n,
+                            const hwyblas_int* k, const float* alpha, const float* a,
+                            const hwyblas_int* lda, const float* beta, float* c,
+                            const hwyblas_int* ldc);
+HWYBLAS_AMD_API void dsyrk_(const char* uplo, const char* trans, const hwyblas_int* n,
+                            const hwyblas_int* k, const double* alpha, const double* a,
+                            const hwyblas_int* lda, const double* beta, double* c,
+                            const hwyblas_int* ldc);
+HWYBLAS_AMD_API void ssyr2k_(const char* uplo, const char* trans, const hwyblas_int* n,
+                             const hwyblas_int* k, const float* alpha, const float* a,
+                             const hwyblas_int* lda, const float* b, const hwyblas_int* ldb,
+                             const float* beta, float* c, const hwyblas_int* ldc);
+HWYBLAS_AMD_API void dsyr2k_(const char* uplo, const char* trans, const hwyblas_int* n,
+                             const hwyblas_int* k, const double* alpha, const double* a,
+                             const hwyblas_int* lda, const double* b, const hwyblas_int* ldb,
+                             const double* beta, double* c, const hwyblas_int* ldc);
+HWYBLAS_AMD_API void sgemmtr_(const char* uplo, const char* transa, const char* transb,
+                              const hwyblas_int* n, const hwyblas_int* k, const float* alpha,
+                              const float* a, const hwyblas_int* lda, const float* b,
+                              const hwyblas_int* ldb, const float* beta, float* c,
+                              const hwyblas_int* ldc);
+HWYBLAS_AMD_API void dgemmtr_(const char* uplo, const char* transa, const char* transb,
+                              const hwyblas_int* n, const hwyblas_int* k, const double* alpha,
+                              const double* a, const hwyblas_int* lda, const double* b,
+                              const hwyblas_int* ldb, const double* beta, double* c,
+                              const hwyblas_int* ldc);
+HWYBLAS_AMD_API void sgemmt_(const char* uplo, const char* transa, const char* transb,
+                             const hwyblas_int* n, const hwyblas_int* k, const float* alpha,
+                             const float* a, const hwyblas_int* lda, const float* b,
+                             const hwyblas_int* ldb, const float* beta, float* c,
+                             const hwyblas_int* ldc);
+HWYBLAS_AMD_API void dgemmt_(const char* uplo, const char* transa, const char* transb,
+                             const hwyblas_int* n, const hwyblas_int* k, const double* alpha,
+                             const double* a, const hwyblas_int* lda, const double* b,
+                             const hwyblas_int* ldb, const double* beta, double* c,
+                             const hwyblas_int* ldc);
+HWYBLAS_AMD_API void strmm_(const char* side, const char* uplo, const char* transa,
+                            const char* diag, const hwyblas_int* m, const hwyblas_int* n,
+                            const float* alpha, const float* a, const hwyblas_int* lda,
+                            float* b, const hwyblas_int* ldb);
+HWYBLAS_AMD_API void dtrmm_(const char* side, const char* uplo, const char* transa,
+                            const char* diag, const hwyblas_int* m, const hwyblas_int* n,
+                            const double* alpha, const double* a, const hwyblas_int* lda,
+                            double* b, const hwyblas_int* ldb);
+
 /* ------------------------------------------------------------------------ */
 /* Error handling (include/hwyblas/hwyblas.h:888-902, cblas.h:45)           */
 

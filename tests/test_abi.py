@@ -224,3 +224,64 @@ def test_lu_entry_points_reject_invalid_arguments():
                          ("DGETRS", 8), ("DGESVIR", 2)]
     # Empty problems return at once.
     assert H.lib().hwyblas_dgetrf_dev(0, 5, None, 1, None, 0, None) == 0
+
+
+# SYMM, SYRK, SYR2K, GEMMTR/GEMMT, TRMM: the info position of each invalid
+# argument, in hwyBLAS's order (src/level3.cc), reported before any device
+# work (no GPU needed). Valid defaults are overridden one at a time.
+LEVEL3_CASES = [
+    ("symm", dict(side="X"), 1), ("symm", dict(uplo="X"), 2), ("symm", dict(m=-1), 3),
+    ("symm", dict(n=-1), 4), ("symm", dict(lda=3), 7), ("symm", dict(ldb=3), 9),
+    ("symm", dict(ldc=3), 12), ("symm", dict(side="R", n=5, lda=4), 7),
+    ("syrk", dict(uplo="X"), 1), ("syrk", dict(trans="X"), 2), ("syrk", dict(n=-1), 3),
+    ("syrk", dict(k=-1), 4), ("syrk", dict(lda=3), 7), ("syrk", dict(trans="T", lda=1), 7),
+    ("syrk", dict(ldc=3), 10),
+    ("syr2k", dict(uplo="X"), 1), ("syr2k", dict(trans="X"), 2), ("syr2k", dict(n=-1), 3),
+    ("syr2k", dict(k=-1), 4), ("syr2k", dict(lda=3), 7), ("syr2k", dict(ldb=3), 9),
+    ("syr2k", dict(ldc=3), 12),
+    ("gemmtr", dict(uplo="X"), 1), ("gemmtr", dict(ta="X"), 2), ("gemmtr", dict(tb="X"), 3),
+    ("gemmtr", dict(n=-1), 4), ("gemmtr", dict(k=-1), 5), ("gemmtr", dict(lda=3), 8),
+    ("gemmtr", dict(ldb=1), 10), ("gemmtr", dict(ldc=3), 13),
+    ("trmm", dict(side="X"), 1), ("trmm", dict(uplo="X"), 2), ("trmm", dict(ta="X"), 3),
+    ("trmm", dict(diag="X"), 4), ("trmm", dict(m=-1), 5), ("trmm", dict(n=-1), 6),
+    ("trmm", dict(lda=3), 9), ("trmm", dict(ldb=3), 11),
+]
+
+
+@pytest.mark.parametrize("routine,change,info", LEVEL3_CASES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_level3_invalid_arguments(routine, change, info, dtype):
+    """m = n = 4, k = 2: every operand's leading dimension valid at 4 (B of
+    GEMMTR N x N at 2), then one argument made invalid."""
+    g = dict(side="L", uplo="L", trans="N", ta="N", tb="N", diag="N", m=4, n=4, k=2, lda=4,
+             ldb=4, ldc=4)
+    if routine == "gemmtr":
+        g["ldb"] = 2
+    g.update(change)
+    x = np.zeros(64, dtype=dtype)
+    with H.ErrorCapture() as cap:
+        if routine == "symm":
+            H.symm(g["side"], g["uplo"], g["m"], g["n"], 1.0, x, g["lda"], x, g["ldb"], 1.0, x,
+                   g["ldc"], dtype=dtype)
+        elif routine == "syrk":
+            H.syrk(g["uplo"], g["trans"], g["n"], g["k"], 1.0, x, g["lda"], 1.0, x, g["ldc"],
+                   dtype=dtype)
+        elif routine == "syr2k":
+            H.syr2k(g["uplo"], g["trans"], g["n"], g["k"], 1.0, x, g["lda"], x, g["ldb"], 1.0,
+                    x, g["ldc"], dtype=dtype)
+        elif routine == "gemmtr":
+            H.gemmtr(g["uplo"], g["ta"], g["tb"], g["n"], g["k"], 1.0, x, g["lda"], x, g["ldb"],
+                     1.0, x, g["ldc"], dtype=dtype)
+        else:
+            H.trmm(g["side"], g["uplo"], g["ta"], g["diag"], g["m"], g["n"], 1.0, x, g["lda"], x,
+                   g["ldb"], dtype=dtype)
+    name = ("D" if dtype == np.float64 else "S") + routine.upper()
+    assert cap.calls == [(name, info)]
+    assert np.all(x == 0)
+
+
+def test_gemmt_reports_its_own_name():
+    with H.ErrorCapture() as cap:
+        H.gemmtr("X", "N", "N", 4, 2, 1.0, None, 4, None, 2, 1.0, None, 4, dtype=np.float64,
+                 name="gemmt")
+    assert cap.calls == [("DGEMMT", 1)]
