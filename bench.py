@@ -612,6 +612,70 @@ def lu_leg(H, n, kappas, stream, nrhs=16, reps=2):
     return out
 
 
+def level3_leg(H, n=8192, reps=3):
+    """DSYMM (L, U), DSYRK (L, N), DSYR2K (L, N), DGEMMTR (L, N, T) and DTRMM
+    (L, U, N, N) at m = n = k = `n` on device operands through the Fortran
+    entries (synchronous; best of `reps` wall-clock calls), uniform[-1, 1],
+    alpha = 1, beta = 0.5 (TRMM: alpha = 1). Algorithmic flops: SYMM 2 n^3,
+    SYRK n^2 (n + 1), SYR2K 2 n^2 (n + 1), GEMMTR n^2 (n + 1), TRMM n^3; beside
+    them DGEMM 2 n^3. Check: SYRK's triangle equals the same tile
+    configuration's DGEMM A A^T bit for bit."""
+    L = H.lib()
+    d = np.float64
+    A, B, C, X = (H.DeviceArray(n, n, d) for _ in range(4))
+    A.fill_uniform(n, n, 1)
+    B.fill_uniform(n, n, 2)
+    C.fill_uniform(n, n, 3)
+    H.synchronize()
+
+    def best(call):
+        call()
+        H.synchronize()
+        t = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            call()
+            H.synchronize()
+            dt = time.perf_counter() - t0
+            t = dt if t is None else min(t, dt)
+        return t
+
+    cases = {
+        "dgemm": (lambda: H.gemm("N", "N", n, n, n, 1.0, A, n, B, n, 0.5, C, n, dtype=d),
+                  2.0 * n ** 3),
+        "dsymm": (lambda: H.symm("L", "U", n, n, 1.0, A, n, B, n, 0.5, C, n, dtype=d),
+                  2.0 * n ** 3),
+        "dsyrk": (lambda: H.syrk("L", "N", n, n, 1.0, A, n, 0.5, C, n, dtype=d),
+                  float(n) * n * (n + 1)),
+        "dsyr2k": (lambda: H.syr2k("L", "N", n, n, 1.0, A, n, B, n, 0.5, C, n, dtype=d),
+                   2.0 * n * n * (n + 1)),
+        "dgemmtr": (lambda: H.gemmtr("L", "N", "T", n, n, 1.0, A, n, B, n, 0.5, C, n, dtype=d),
+                    float(n) * n * (n + 1)),
+        "dtrmm": (lambda: H.trmm("L", "U", "N", "N", n, n, 1.0, A, n, X, n, dtype=d),
+                  float(n) ** 3),
+    }
+    out = {"n": n}
+    for name, (call, flops) in cases.items():
+        if name == "dtrmm":
+            X.fill_uniform(n, n, 4)
+            H.synchronize()
+        t = best(call)
+        tf = flops / t / 1e12
+        out[name] = {"ms": round(t * 1e3, 2), "tflops": round(tf, 2),
+                     "frac_fp64_peak": round(tf / PEAK_FP64, 3)}
+        log(f"level3 {name} n={n}: {t * 1e3:.2f} ms, {tf:.1f} TFLOP/s")
+    # SYRK's lower triangle against DGEMM A A^T (beta = 0 on both).
+    C.fill_uniform(n, n, 3)
+    H.syrk("L", "N", n, n, 1.0, A, n, 0.0, C, n, dtype=d)
+    G = H.DeviceArray(n, n, d)
+    H.gemm("N", "T", n, n, n, 1.0, A, n, A, n, 0.0, G, n, dtype=d)
+    H.synchronize()
+    c, g = C.to_host(), G.to_host()
+    low = np.tril(np.ones((n, n), dtype=bool))
+    out["syrk_equals_gemm_triangle"] = bool(np.array_equal(c[low], g[low]))
+    return out
+
+
 def cpu_baseline(oracle, seconds=12.0):
     """The oracle's single-threaded restatement of hwyBLAS's blocked DGEMM,
     DGEMM NN 2048^3 on one pinned core, best of the calls that fit in about
@@ -690,6 +754,8 @@ def main():
     ap.add_argument("--lu", action="store_true",
                     help="the mixed-precision LU solve (n = --lu-n, kappa 1e2, 1e6, 1e10) "
                          "instead of the headline")
+    ap.add_argument("--level3", action="store_true",
+                    help="DSYMM/DSYRK/DSYR2K/DGEMMTR/DTRMM at n = 8192 beside DGEMM")
     ap.add_argument("--lu-n", type=int, default=16384)
     ap.add_argument("--lu-kappas", default="2,6,10",
                     help="comma-separated log10 condition numbers for --lu")
@@ -740,6 +806,9 @@ def main():
         return
     if args.plain_sweep:
         print(json.dumps({"plain_sweep": plain_sweep(H, oracle)}), flush=True)
+        return
+    if args.level3:
+        print(json.dumps({"level3": level3_leg(H)}), flush=True)
         return
     if args.batched_only:
         out = run_batched(H, oracle, args, stream)

@@ -140,6 +140,25 @@ void GemmDevicePart<float>(int part, bool ta, bool tb, int64_t m, int64_t n, int
                            int64_t ldb, float beta, float* c, int64_t ldc,
                            hipStream_t stream);
 
+// As GemmDevice, when one factor is triangular with explicit zeros: each
+// tile multiplies only over the k slices that meet its nonzero k-range
+// (krange as GemmParams::krange: 1 k >= the tile's first row, 2 k <= its
+// last row, 3 k <= its last column, 4 k >= its first column).
+template <typename T>
+void GemmDeviceKRange(int krange, bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha,
+                      const T* a, int64_t lda, const T* b, int64_t ldb, T beta, T* c,
+                      int64_t ldc, hipStream_t stream);
+template <>
+void GemmDeviceKRange<double>(int krange, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                              double alpha, const double* a, int64_t lda, const double* b,
+                              int64_t ldb, double beta, double* c, int64_t ldc,
+                              hipStream_t stream);
+template <>
+void GemmDeviceKRange<float>(int krange, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                             float alpha, const float* a, int64_t lda, const float* b,
+                             int64_t ldb, float beta, float* c, int64_t ldc,
+                             hipStream_t stream);
+
 // Tile configuration GemmDevice uses for a shape (an index into the table of
 // gemm_configs.h).
 int ChooseGemmConfig(int dtype_bytes, bool ta, bool tb, int64_t m, int64_t n, int64_t k);

@@ -8,13 +8,16 @@
 
 namespace hwy_amd {
 
-// part: 0 all of C, 1 its lower triangle, 2 its upper triangle.
+// part: 0 all of C, 1 its lower triangle, 2 its upper triangle; krange: a
+// triangular factor's zero k-range skipped per tile (GemmParams::krange).
 void GemmLaunchF64(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
                    const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
-                   double* c, int64_t ldc, hipStream_t stream, int part = 0);
+                   double* c, int64_t ldc, hipStream_t stream, int part = 0,
+                   int krange = 0);
 void GemmLaunchF32(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
                    const float* a, int64_t lda, const float* b, int64_t ldb, float beta,
-                   float* c, int64_t ldc, hipStream_t stream, int part = 0);
+                   float* c, int64_t ldc, hipStream_t stream, int part = 0,
+                   int krange = 0);
 
 namespace {
 
@@ -59,12 +62,23 @@ void GemmDevice<float>(bool ta, bool tb, int64_t m, int64_t n, int64_t k, float 
   GemmLaunchF32(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
 }
 
+namespace {
+
+// A triangle of C runs on square tiles, so that only the tiles meeting it
+// are launched (the 128 x 128 x 16 two-waves-per-SIMD tile otherwise).
+int PartConfig(int cfg, int part) {
+  if (part == 0 || kGemmConfigs[cfg].bm == kGemmConfigs[cfg].bn) return cfg;
+  return 8;
+}
+
+}  // namespace
+
 template <>
 void GemmDevicePart<double>(int part, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
                             double alpha, const double* a, int64_t lda, const double* b,
                             int64_t ldb, double beta, double* c, int64_t ldc,
                             hipStream_t stream) {
-  const int cfg = ChooseGemmConfig(8, ta, tb, m, n, k);
+  const int cfg = PartConfig(ChooseGemmConfig(8, ta, tb, m, n, k), part);
   GemmLaunchF64(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, part);
 }
 
@@ -73,8 +87,26 @@ void GemmDevicePart<float>(int part, bool ta, bool tb, int64_t m, int64_t n, int
                            float alpha, const float* a, int64_t lda, const float* b,
                            int64_t ldb, float beta, float* c, int64_t ldc,
                            hipStream_t stream) {
-  const int cfg = ChooseGemmConfig(4, ta, tb, m, n, k);
+  const int cfg = PartConfig(ChooseGemmConfig(4, ta, tb, m, n, k), part);
   GemmLaunchF32(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, part);
+}
+
+template <>
+void GemmDeviceKRange<double>(int krange, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                              double alpha, const double* a, int64_t lda, const double* b,
+                              int64_t ldb, double beta, double* c, int64_t ldc,
+                              hipStream_t stream) {
+  const int cfg = ChooseGemmConfig(8, ta, tb, m, n, k);
+  GemmLaunchF64(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, 0, krange);
+}
+
+template <>
+void GemmDeviceKRange<float>(int krange, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                             float alpha, const float* a, int64_t lda, const float* b,
+                             int64_t ldb, float beta, float* c, int64_t ldc,
+                             hipStream_t stream) {
+  const int cfg = ChooseGemmConfig(4, ta, tb, m, n, k);
+  GemmLaunchF32(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, 0, krange);
 }
 
 namespace {

@@ -5,8 +5,9 @@ namespace hwy_amd {
 
 void GemmLaunchF32(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
                    const float* a, int64_t lda, const float* b, int64_t ldb, float beta,
-                   float* c, int64_t ldc, hipStream_t stream, int part) {
+                   float* c, int64_t ldc, hipStream_t stream, int part, int krange) {
   gemm_detail::GemmParams<float> prm{a, b, c, m, n, k, lda, ldb, ldc, alpha, beta, 0, 0, part};
+  prm.krange = krange;
   gemm_detail::LaunchAny<float>(cfg, ta, tb, prm, stream);
 }
 

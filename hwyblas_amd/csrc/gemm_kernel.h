@@ -70,6 +70,13 @@ struct GemmParams {
   // (row >= col), 2 the upper triangle (row <= col). Tiles entirely outside
   // the part do no work.
   int part = 0;
+  // Set by Launch for a triangle of a square C with square tiles: the grid
+  // holds only the T (T + 1) / 2 tiles that meet the triangle.
+  int tri_grid = 0;
+  // A triangular factor's zero k-range, skipped per tile (only the k slices
+  // meeting [kb, ke) run): 0 none; 1 kb = the tile's first row; 2 ke = its
+  // last row + 1; 3 ke = its last column + 1; 4 kb = its first column.
+  int krange = 0;
 };
 
 constexpr int kXcds = 8;    // MI355X: 8 XCDs, each with its own L2
@@ -80,13 +87,18 @@ constexpr int kGroupM = 8;  // tile rows per raster group
 // contiguous run of the raster order, and the raster order walks groups of
 // kGroupM tile rows column by column: tiles that share A row panels and B
 // column panels then run close together in time on the same L2.
-__device__ __forceinline__ void TileCoords(int tiles_m, int tiles_n, int* tm,
-                                           int* tn) {
+// The workgroup's position in a run that gives each XCD (ids congruent mod
+// 8, dispatched round-robin) a contiguous share.
+__device__ __forceinline__ int XcdRunIndex() {
   const int nblk = gridDim.x, bid = blockIdx.x;
   const int xcd = bid % kXcds, idx = bid / kXcds;
   const int per = nblk / kXcds, rem = nblk % kXcds;
-  const int lid = xcd < rem ? xcd * (per + 1) + idx
-                            : rem * (per + 1) + (xcd - rem) * per + idx;
+  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+__device__ __forceinline__ void TileCoords(int tiles_m, int tiles_n, int* tm,
+                                           int* tn) {
+  const int lid = XcdRunIndex();
   const int group = kGroupM * tiles_n;
   const int first_m = (lid / group) * kGroupM;
   const int gm = min(tiles_m - first_m, kGroupM);
@@ -217,15 +229,24 @@ __device__ __forceinline__ void MacroTile(const GemmParams<T>& p, int64_t m0, in
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = Acc{};
 
-  const int ktiles = int((p.k + BK - 1) / BK);
-  load_tile(0);
+  int kt0 = 0, kt1 = int((p.k + BK - 1) / BK);
+  if (p.krange) {
+    int64_t kb = 0, ke = p.k;
+    if (p.krange == 1) kb = m0;
+    else if (p.krange == 2) ke = m0 + BM;
+    else if (p.krange == 3) ke = n0 + BN;
+    else kb = n0;
+    kt0 = int(min(kb, p.k) / BK);
+    kt1 = int((min(ke, p.k) + BK - 1) / BK);
+  }
+  load_tile(int64_t(kt0) * BK);
   store_lds(0);
   __syncthreads();
 
   const int li = lane & 15, lk = lane >> 4;
-  for (int kt = 0; kt < ktiles; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < ktiles;
+  for (int kt = kt0; kt < kt1; ++kt) {
+    const int cur = (kt - kt0) & 1;
+    const bool more = kt + 1 < kt1;
     if (more) load_tile(int64_t(kt + 1) * BK);
     const T* as = as_base + cur * A_SZ;
     const T* bs = bs_base + cur * B_SZ;
@@ -282,7 +303,22 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
     __attribute__((amdgpu_waves_per_eu(OCC))) GemmKernel(GemmParams<T> p) {
   __shared__ T lds[TileLds<T, TA, TB, BM, BN, BK>::kElems];
   int tm, tn;
-  TileCoords(p.tiles_m, p.tiles_n, &tm, &tn);
+  if (p.tri_grid) {
+    // Tile t of the triangle, row by row: r (r + 1) / 2 <= t < (r + 1) (r + 2) / 2.
+    const int t = XcdRunIndex();
+    int r = int((sqrtf(8.0f * float(t) + 1.0f) - 1.0f) * 0.5f);
+    while ((r + 1) * (r + 2) / 2 <= t) ++r;
+    while (r * (r + 1) / 2 > t) --r;
+    const int c = t - r * (r + 1) / 2;
+    tm = p.part == 1 ? r : c;
+    tn = p.part == 1 ? c : r;
+  } else if (p.krange == 1 || p.krange == 2) {
+    // Work that varies by tile row: the raster bands run over tile columns,
+    // so every XCD's share spans all rows.
+    TileCoords(p.tiles_n, p.tiles_m, &tn, &tm);
+  } else {
+    TileCoords(p.tiles_m, p.tiles_n, &tm, &tn);
+  }
   const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
   // A tile entirely on the other side of the diagonal (uniform exit).
   if ((p.part == 1 && m0 + BM - 1 < n0) || (p.part == 2 && m0 > n0 + BN - 1)) return;
@@ -294,7 +330,12 @@ void Launch(const GemmParams<T>& prm, hipStream_t stream) {
   GemmParams<T> q = prm;
   q.tiles_m = int((q.m + BM - 1) / BM);
   q.tiles_n = int((q.n + BN - 1) / BN);
-  const int64_t blocks = int64_t(q.tiles_m) * q.tiles_n;
+  int64_t blocks = int64_t(q.tiles_m) * q.tiles_n;
+  q.tri_grid = 0;
+  if (q.part != 0 && BM == BN && q.m == q.n) {
+    q.tri_grid = 1;
+    blocks = int64_t(q.tiles_m) * (q.tiles_m + 1) / 2;
+  }
   hipLaunchKernelGGL((GemmKernel<T, TA, TB, BM, BN, BK, WGM, WGN, OCC>), dim3(unsigned(blocks)),
                      dim3(64 * WGM * WGN), 0, stream, q);
   HWY_HIP_CHECK(hipGetLastError());

@@ -321,11 +321,18 @@ void TrmmHost(const char* name, const char* sidep, const char* uplop, const char
     T* x = static_cast<T*>(Staging(kSlotC, size_t(m) * n * sizeof(T)));
     HWY_HIP_CHECK(hipMemcpy2DAsync(x, size_t(m) * sizeof(T), ob.dev, size_t(ob.ld) * sizeof(T),
                                    size_t(m) * sizeof(T), n, hipMemcpyDeviceToDevice, s));
+    // op(A) is upper triangular when (uplo U, N) or (uplo L, T). Its zero
+    // k-range is skipped per tile, as hwyBLAS's packing does
+    // (src/gemm-inl.h:66-130): left, row i needs k >= i (upper) or k <= i;
+    // right, column j needs k <= j (upper) or k >= j.
     const bool t = op == Op::kTrans;
+    const bool upper = lower == t;
     if (left)
-      GemmDevice<T>(t, false, m, n, m, alpha, tri, ra, x, m, T(0), ob.dev, ob.ld, s);
+      GemmDeviceKRange<T>(upper ? 1 : 2, t, false, m, n, m, alpha, tri, ra, x, m, T(0), ob.dev,
+                          ob.ld, s);
     else
-      GemmDevice<T>(false, t, m, n, n, alpha, x, m, tri, ra, T(0), ob.dev, ob.ld, s);
+      GemmDeviceKRange<T>(upper ? 3 : 4, false, t, m, n, n, alpha, x, m, tri, ra, T(0), ob.dev,
+                          ob.ld, s);
   }
   ob.Finish(s);
 }
