@@ -52,6 +52,12 @@ def _declare(L):
         fn(f"{pre}trsm_", None, [ctypes.c_char_p] * 4 + [_pi, _pi, _p, _p, _pi, _p, _pi])
         fn(f"cblas_{pre}gemm", None, [_i, _i, _i, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i])
         fn(f"cblas_{pre}trsm", None, [_i, _i, _i, _i, _i, _i, _i, s, _p, _i, _p, _i])
+        for name in ("gemmtr", "gemmt"):
+            fn(f"cblas_{pre}{name}", None, [_i, _i, _i, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i])
+        fn(f"cblas_{pre}symm", None, [_i, _i, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i])
+        fn(f"cblas_{pre}syrk", None, [_i, _i, _i, _i, _i, s, _p, _i, s, _p, _i])
+        fn(f"cblas_{pre}syr2k", None, [_i, _i, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i])
+        fn(f"cblas_{pre}trmm", None, [_i, _i, _i, _i, _i, _i, _i, s, _p, _i, _p, _i])
         fn(f"hwyblas_{pre}gemm_dev", _i, [_c, _c, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i, _p])
         fn(f"hwyblas_{pre}trsm_dev", _i, [_c, _c, _c, _c, _i, _i, s, _p, _i, _p, _i, _p])
         fn(f"hwyblas_{pre}gemm_dev_cfg", _i, [_i, _c, _c, _i, _i, _i, s, _p, _i, _p, _i, s, _p,
@@ -225,6 +231,12 @@ def cblas_gemm(layout, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, dty
     dtype = dtype or c.dtype
     f = getattr(lib(), f"cblas_{_pre(dtype)}gemm")
     f(layout, ta, tb, m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc)
+
+
+def cblas_level3(routine, *args, dtype):
+    """cblas_{s,d}{routine} with its CBLAS arguments (enums as ints)."""
+    f = getattr(lib(), f"cblas_{_pre(dtype)}{routine}")
+    f(*[_ptr(x) if isinstance(x, (np.ndarray, DeviceArray)) else x for x in args])
 
 
 def cblas_trsm(layout, side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, dtype=None):

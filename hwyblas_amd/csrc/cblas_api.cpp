@@ -51,17 +51,32 @@ char SideChar(CBLAS_SIDE s) { return s == CblasLeft ? 'L' : s == CblasRight ? 'R
 char FlipUplo(char c) { return c == 'U' ? 'L' : c == 'L' ? 'U' : c; }
 char FlipSide(char c) { return c == 'L' ? 'R' : c == 'R' ? 'L' : c; }
 
+// Row-major SYRK/SYR2K take the other transposition (real: C is T).
+char FlipRankTrans(char c) { return c == 'N' ? 'T' : (c == 'T' || c == 'C') ? 'N' : c; }
+
 template <typename T>
 struct Fortran;
 template <>
 struct Fortran<double> {
   static constexpr auto gemm = dgemm_;
   static constexpr auto trsm = dtrsm_;
+  static constexpr auto gemmtr = dgemmtr_;
+  static constexpr auto gemmt = dgemmt_;
+  static constexpr auto symm = dsymm_;
+  static constexpr auto syrk = dsyrk_;
+  static constexpr auto syr2k = dsyr2k_;
+  static constexpr auto trmm = dtrmm_;
 };
 template <>
 struct Fortran<float> {
   static constexpr auto gemm = sgemm_;
   static constexpr auto trsm = strsm_;
+  static constexpr auto gemmtr = sgemmtr_;
+  static constexpr auto gemmt = sgemmt_;
+  static constexpr auto symm = ssymm_;
+  static constexpr auto syrk = ssyrk_;
+  static constexpr auto syr2k = ssyr2k_;
+  static constexpr auto trmm = strmm_;
 };
 
 template <typename T>
@@ -98,10 +113,165 @@ void Trsm(const char* name, CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO upl
   Fortran<T>::trsm(&sc, &uc, &tc, &dc, &m, &n, &alpha, a, &lda, b, &ldb);
 }
 
+// GEMMTR (and its GEMMT name, `alias`). Row-major: the triangle of C^T =
+// op(B)^T op(A)^T, the other uplo, A and B swapped with their
+// transpositions. Positions: layout 1, uplo 2, transa 3, transb 4, n 5,
+// k 6, alpha 7, a 8, lda 9, b 10, ldb 11, beta 12, c 13, ldc 14.
+template <typename T>
+void Gemmtr(const char* name, bool alias, CBLAS_LAYOUT layout, CBLAS_UPLO uplo,
+            CBLAS_TRANSPOSE transa, CBLAS_TRANSPOSE transb, blasint n, blasint k, T alpha,
+            const T* a, blasint lda, const T* b, blasint ldb, T beta, T* c, blasint ldc) {
+  if (!ValidLayout(name, layout)) return;
+  const char ta = TransChar(transa), tb = TransChar(transb), uc = UploChar(uplo);
+  auto f = alias ? Fortran<T>::gemmt : Fortran<T>::gemmtr;
+  if (layout == CblasColMajor) {
+    CblasScope scope(name, {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14});
+    f(&uc, &ta, &tb, &n, &k, &alpha, a, &lda, b, &ldb, &beta, c, &ldc);
+  } else {
+    const char ur = FlipUplo(uc);
+    CblasScope scope(name, {2, 4, 3, 5, 6, 7, 10, 11, 8, 9, 12, 13, 14});
+    f(&ur, &tb, &ta, &n, &k, &alpha, b, &ldb, a, &lda, &beta, c, &ldc);
+  }
+}
+
+// SYMM. Row-major: C^T = B^T A (side flipped), A's storage is the other
+// triangle of the same symmetric matrix, m and n swapped. Positions: layout
+// 1, side 2, uplo 3, m 4, n 5, alpha 6, a 7, lda 8, b 9, ldb 10, beta 11,
+// c 12, ldc 13.
+template <typename T>
+void Symm(const char* name, CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, blasint m,
+          blasint n, T alpha, const T* a, blasint lda, const T* b, blasint ldb, T beta, T* c,
+          blasint ldc) {
+  if (!ValidLayout(name, layout)) return;
+  char sc = SideChar(side), uc = UploChar(uplo);
+  const bool row = layout == CblasRowMajor;
+  if (row) {
+    sc = FlipSide(sc);
+    uc = FlipUplo(uc);
+    std::swap(m, n);
+  }
+  CblasScope scope(name, row ? std::initializer_list<int>{2, 3, 5, 4, 6, 7, 8, 9, 10, 11, 12, 13}
+                             : std::initializer_list<int>{2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13});
+  Fortran<T>::symm(&sc, &uc, &m, &n, &alpha, a, &lda, b, &ldb, &beta, c, &ldc);
+}
+
+// SYRK. Row-major: C^T is the same product of the column-major A^T: the
+// other triangle and transposition. Positions: layout 1, uplo 2, trans 3,
+// n 4, k 5, alpha 6, a 7, lda 8, beta 9, c 10, ldc 11.
+template <typename T>
+void Syrk(const char* name, CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans,
+          blasint n, blasint k, T alpha, const T* a, blasint lda, T beta, T* c, blasint ldc) {
+  if (!ValidLayout(name, layout)) return;
+  char uc = UploChar(uplo), tc = TransChar(trans);
+  if (layout == CblasRowMajor) {
+    uc = FlipUplo(uc);
+    tc = FlipRankTrans(tc);
+  }
+  CblasScope scope(name, {2, 3, 4, 5, 6, 7, 8, 9, 10, 11});
+  Fortran<T>::syrk(&uc, &tc, &n, &k, &alpha, a, &lda, &beta, c, &ldc);
+}
+
+// SYR2K: as SYRK. Positions: layout 1, uplo 2, trans 3, n 4, k 5, alpha 6,
+// a 7, lda 8, b 9, ldb 10, beta 11, c 12, ldc 13.
+template <typename T>
+void Syr2k(const char* name, CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans,
+           blasint n, blasint k, T alpha, const T* a, blasint lda, const T* b, blasint ldb,
+           T beta, T* c, blasint ldc) {
+  if (!ValidLayout(name, layout)) return;
+  char uc = UploChar(uplo), tc = TransChar(trans);
+  if (layout == CblasRowMajor) {
+    uc = FlipUplo(uc);
+    tc = FlipRankTrans(tc);
+  }
+  CblasScope scope(name, {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13});
+  Fortran<T>::syr2k(&uc, &tc, &n, &k, &alpha, a, &lda, b, &ldb, &beta, c, &ldc);
+}
+
+// TRMM: as TRSM (the other side and triangle, op unchanged).
+template <typename T>
+void Trmm(const char* name, CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo,
+          CBLAS_TRANSPOSE transa, CBLAS_DIAG diag, blasint m, blasint n, T alpha, const T* a,
+          blasint lda, T* b, blasint ldb) {
+  if (!ValidLayout(name, layout)) return;
+  char sc = SideChar(side), uc = UploChar(uplo);
+  const char tc = TransChar(transa), dc = DiagChar(diag);
+  const bool row = layout == CblasRowMajor;
+  if (row) {
+    sc = FlipSide(sc);
+    uc = FlipUplo(uc);
+    std::swap(m, n);
+  }
+  CblasScope scope(name, row ? std::initializer_list<int>{2, 3, 4, 5, 7, 6, 8, 9, 10, 11, 12}
+                             : std::initializer_list<int>{2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12});
+  Fortran<T>::trmm(&sc, &uc, &tc, &dc, &m, &n, &alpha, a, &lda, b, &ldb);
+}
+
 }  // namespace
 }  // namespace hwy_amd
 
 extern "C" {
+
+#define HWY_CBLAS_GEMMTR(NAME, T, ALIAS)                                                      \
+  HWYBLAS_AMD_API void NAME(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,                 \
+                            const CBLAS_TRANSPOSE transa, const CBLAS_TRANSPOSE transb,       \
+                            const hwyblas_int n, const hwyblas_int k, const T alpha,          \
+                            const T* a, const hwyblas_int lda, const T* b,                    \
+                            const hwyblas_int ldb, const T beta, T* c, const hwyblas_int ldc) { \
+    hwy_amd::Gemmtr<T>(#NAME, ALIAS, layout, uplo, transa, transb, n, k, alpha, a, lda, b,    \
+                       ldb, beta, c, ldc);                                                    \
+  }
+HWY_CBLAS_GEMMTR(cblas_sgemmtr, float, false)
+HWY_CBLAS_GEMMTR(cblas_dgemmtr, double, false)
+HWY_CBLAS_GEMMTR(cblas_sgemmt, float, true)
+HWY_CBLAS_GEMMTR(cblas_dgemmt, double, true)
+#undef HWY_CBLAS_GEMMTR
+
+#define HWY_CBLAS_SYMM(NAME, T)                                                               \
+  HWYBLAS_AMD_API void NAME(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,                 \
+                            const CBLAS_UPLO uplo, const hwyblas_int m, const hwyblas_int n,  \
+                            const T alpha, const T* a, const hwyblas_int lda, const T* b,     \
+                            const hwyblas_int ldb, const T beta, T* c, const hwyblas_int ldc) { \
+    hwy_amd::Symm<T>(#NAME, layout, side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);   \
+  }
+HWY_CBLAS_SYMM(cblas_ssymm, float)
+HWY_CBLAS_SYMM(cblas_dsymm, double)
+#undef HWY_CBLAS_SYMM
+
+#define HWY_CBLAS_SYRK(NAME, T)                                                               \
+  HWYBLAS_AMD_API void NAME(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,                 \
+                            const CBLAS_TRANSPOSE trans, const hwyblas_int n,                 \
+                            const hwyblas_int k, const T alpha, const T* a,                   \
+                            const hwyblas_int lda, const T beta, T* c, const hwyblas_int ldc) { \
+    hwy_amd::Syrk<T>(#NAME, layout, uplo, trans, n, k, alpha, a, lda, beta, c, ldc);          \
+  }
+HWY_CBLAS_SYRK(cblas_ssyrk, float)
+HWY_CBLAS_SYRK(cblas_dsyrk, double)
+#undef HWY_CBLAS_SYRK
+
+#define HWY_CBLAS_SYR2K(NAME, T)                                                              \
+  HWYBLAS_AMD_API void NAME(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,                 \
+                            const CBLAS_TRANSPOSE trans, const hwyblas_int n,                 \
+                            const hwyblas_int k, const T alpha, const T* a,                   \
+                            const hwyblas_int lda, const T* b, const hwyblas_int ldb,         \
+                            const T beta, T* c, const hwyblas_int ldc) {                      \
+    hwy_amd::Syr2k<T>(#NAME, layout, uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc); \
+  }
+HWY_CBLAS_SYR2K(cblas_ssyr2k, float)
+HWY_CBLAS_SYR2K(cblas_dsyr2k, double)
+#undef HWY_CBLAS_SYR2K
+
+#define HWY_CBLAS_TRMM(NAME, T)                                                               \
+  HWYBLAS_AMD_API void NAME(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,                 \
+                            const CBLAS_UPLO uplo, const CBLAS_TRANSPOSE transa,              \
+                            const CBLAS_DIAG diag, const hwyblas_int m, const hwyblas_int n,  \
+                            const T alpha, const T* a, const hwyblas_int lda, T* b,           \
+                            const hwyblas_int ldb) {                                          \
+    hwy_amd::Trmm<T>(#NAME, layout, side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);   \
+  }
+HWY_CBLAS_TRMM(cblas_strmm, float)
+HWY_CBLAS_TRMM(cblas_dtrmm, double)
+#undef HWY_CBLAS_TRMM
+
 
 HWYBLAS_AMD_API void cblas_sgemm(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOSE transa,
                                  const CBLAS_TRANSPOSE transb, const hwyblas_int m,

@@ -213,6 +213,76 @@ HWYBLAS_AMD_API void cblas_dtrsm(const CBLAS_LAYOUT layout,
                                  const hwyblas_int n, const double alpha,
                                  const double* a, const hwyblas_int lda,
                                  double* b, const hwyblas_int ldb);
+/* CBLAS forms of the other real Level-3 routines (src/cblas.cc:543-701;
+ * declarations include/hwyblas/cblas.h:641-840). Row-major: GEMMTR flips
+ * uplo and swaps A and B with their transpositions; SYMM flips side and
+ * uplo and swaps m and n; SYRK and SYR2K flip uplo and the transposition
+ * (N <-> T); TRMM flips side and uplo and swaps m and n. Errors are reported
+ * at CBLAS positions (layout = 1). */
+HWYBLAS_AMD_API void cblas_sgemmtr(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,
+                                   const CBLAS_TRANSPOSE transa, const CBLAS_TRANSPOSE transb,
+                                   const hwyblas_int n, const hwyblas_int k, const float alpha,
+                                   const float* a, const hwyblas_int lda, const float* b,
+                                   const hwyblas_int ldb, const float beta, float* c,
+                                   const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_dgemmtr(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,
+                                   const CBLAS_TRANSPOSE transa, const CBLAS_TRANSPOSE transb,
+                                   const hwyblas_int n, const hwyblas_int k, const double alpha,
+                                   const double* a, const hwyblas_int lda, const double* b,
+                                   const hwyblas_int ldb, const double beta, double* c,
+                                   const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_sgemmt(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,
+                                  const CBLAS_TRANSPOSE transa, const CBLAS_TRANSPOSE transb,
+                                  const hwyblas_int n, const hwyblas_int k, const float alpha,
+                                  const float* a, const hwyblas_int lda, const float* b,
+                                  const hwyblas_int ldb, const float beta, float* c,
+                                  const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_dgemmt(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,
+                                  const CBLAS_TRANSPOSE transa, const CBLAS_TRANSPOSE transb,
+                                  const hwyblas_int n, const hwyblas_int k, const double alpha,
+                                  const double* a, const hwyblas_int lda, const double* b,
+                                  const hwyblas_int ldb, const double beta, double* c,
+                                  const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_ssymm(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,
+                                 const CBLAS_UPLO uplo, const hwyblas_int m, const hwyblas_int n,
+                                 const float alpha, const float* a, const hwyblas_int lda,
+                                 const float* b, const hwyblas_int ldb, const float beta,
+                                 float* c, const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_dsymm(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,
+                                 const CBLAS_UPLO uplo, const hwyblas_int m, const hwyblas_int n,
+                                 const double alpha, const double* a, const hwyblas_int lda,
+                                 const double* b, const hwyblas_int ldb, const double beta,
+                                 double* c, const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_ssyrk(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,
+                                 const CBLAS_TRANSPOSE trans, const hwyblas_int n,
+                                 const hwyblas_int k, const float alpha, const float* a,
+                                 const hwyblas_int lda, const float beta, float* c,
+                                 const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_dsyrk(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,
+                                 const CBLAS_TRANSPOSE trans, const hwyblas_int n,
+                                 const hwyblas_int k, const double alpha, const double* a,
+                                 const hwyblas_int lda, const double beta, double* c,
+                                 const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_ssyr2k(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,
+                                  const CBLAS_TRANSPOSE trans, const hwyblas_int n,
+                                  const hwyblas_int k, const float alpha, const float* a,
+                                  const hwyblas_int lda, const float* b, const hwyblas_int ldb,
+                                  const float beta, float* c, const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_dsyr2k(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,
+                                  const CBLAS_TRANSPOSE trans, const hwyblas_int n,
+                                  const hwyblas_int k, const double alpha, const double* a,
+                                  const hwyblas_int lda, const double* b, const hwyblas_int ldb,
+                                  const double beta, double* c, const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_strmm(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,
+                                 const CBLAS_UPLO uplo, const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_DIAG diag, const hwyblas_int m, const hwyblas_int n,
+                                 const float alpha, const float* a, const hwyblas_int lda,
+                                 float* b, const hwyblas_int ldb);
+HWYBLAS_AMD_API void cblas_dtrmm(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,
+                                 const CBLAS_UPLO uplo, const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_DIAG diag, const hwyblas_int m, const hwyblas_int n,
+                                 const double alpha, const double* a, const hwyblas_int lda,
+                                 double* b, const hwyblas_int ldb);
 
 /* ------------------------------------------------------------------------ */
 /* Device-pointer entry points (no hwyBLAS counterpart: the reference has no

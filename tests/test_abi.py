@@ -285,3 +285,39 @@ def test_gemmt_reports_its_own_name():
         H.gemmtr("X", "N", "N", 4, 2, 1.0, None, 4, None, 2, 1.0, None, 4, dtype=np.float64,
                  name="gemmt")
     assert cap.calls == [("DGEMMT", 1)]
+
+
+@pytest.mark.parametrize("layout,routine,args,position", [
+    # Row-major SYMM swaps m and n: an invalid m is still CBLAS position 4.
+    ("row", "symm", dict(m=-1), 4), ("col", "symm", dict(m=-1), 4),
+    ("row", "symm", dict(n=-1), 5), ("col", "symm", dict(ldc=1), 13),
+    ("row", "syrk", dict(k=-1), 5), ("col", "syrk", dict(ldc=1), 11),
+    ("row", "syr2k", dict(ldb=1), 10),
+    # Row-major GEMMTR swaps the operands: transb is still CBLAS position 4.
+    ("row", "gemmtr", dict(tb=0), 4), ("row", "gemmtr", dict(ldb=1), 11),
+    ("col", "gemmtr", dict(lda=1), 9),
+    ("row", "trmm", dict(m=-1), 6), ("row", "trmm", dict(n=-1), 7),
+    ("col", "trmm", dict(diag=0), 5),
+])
+def test_cblas_level3_error_positions(layout, routine, args, position):
+    g = dict(side=H.Left, uplo=H.Lower, trans=H.NoTrans, ta=H.NoTrans, tb=H.NoTrans,
+             diag=H.NonUnit, m=4, n=4, k=4, lda=4, ldb=4, ldc=4)
+    g.update(args)
+    lay = H.RowMajor if layout == "row" else H.ColMajor
+    with H.ErrorCapture() as cap:
+        if routine == "symm":
+            H.cblas_level3("symm", lay, g["side"], g["uplo"], g["m"], g["n"], 1.0, None,
+                           g["lda"], None, g["ldb"], 1.0, None, g["ldc"], dtype=np.float64)
+        elif routine == "syrk":
+            H.cblas_level3("syrk", lay, g["uplo"], g["trans"], g["n"], g["k"], 1.0, None,
+                           g["lda"], 1.0, None, g["ldc"], dtype=np.float64)
+        elif routine == "syr2k":
+            H.cblas_level3("syr2k", lay, g["uplo"], g["trans"], g["n"], g["k"], 1.0, None,
+                           g["lda"], None, g["ldb"], 1.0, None, g["ldc"], dtype=np.float64)
+        elif routine == "gemmtr":
+            H.cblas_level3("gemmtr", lay, g["uplo"], g["ta"], g["tb"], g["n"], g["k"], 1.0,
+                           None, g["lda"], None, g["ldb"], 1.0, None, g["ldc"], dtype=np.float64)
+        else:
+            H.cblas_level3("trmm", lay, g["side"], g["uplo"], g["ta"], g["diag"], g["m"],
+                           g["n"], 1.0, None, g["lda"], None, g["ldb"], dtype=np.float64)
+    assert cap.calls == [(f"cblas_d{routine}", position)]

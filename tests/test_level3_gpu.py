@@ -204,3 +204,77 @@ def test_device_operands(H):
     want = b.copy(order="F")
     H.trmm("L", "U", "T", "N", n, 9, 2.0, t, n, want, n)
     assert np.array_equal(db.to_host(), want)
+
+
+def _rowmajor(rows, cols, seed, dtype):
+    """A row-major rows x cols matrix (C order) with uniform[-1, 1] entries."""
+    return np.ascontiguousarray(rand(rows, cols, seed, dtype))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("layout", ["row", "col"])
+def test_cblas_level3(H, dtype, layout):
+    """cblas_?symm/syrk/syr2k/gemmtr/gemmt/trmm in both layouts against a
+    float64 numpy evaluation of the same mathematics (relative 1e-12 for
+    FP64, 2e-5 for FP32, scaled by the inputs' magnitudes); untouched parts
+    stay untouched."""
+    lay = H.RowMajor if layout == "row" else H.ColMajor
+    order = "C" if layout == "row" else "F"
+    tol = 1e-12 if dtype == np.float64 else 2e-5
+    mk = (lambda r, c, sd: np.asarray(rand(r, c, sd, dtype), order=order).copy(order=order))
+
+    def ld(x):
+        return x.shape[1] if layout == "row" else x.shape[0]
+
+    m, n, k = 37, 29, 23
+    f64 = lambda x: x.astype(np.float64)
+    # SYMM, side L, uplo L: only the lower triangle of A is read.
+    a = mk(m, m, 21)
+    low = tri_mask(m, True)
+    s_full = np.where(low, f64(a), f64(a).T)
+    a[~low] = np.nan
+    b, c = mk(m, n, 22), mk(m, n, 23)
+    want = 1.5 * s_full @ f64(b) - 0.5 * f64(c)
+    H.cblas_level3("symm", lay, H.Left, H.Lower, m, n, 1.5, a, ld(a), b, ld(b), -0.5, c, ld(c),
+                   dtype=dtype)
+    assert np.max(np.abs(f64(c) - want)) <= tol * m * 4
+    # SYRK, uplo U, trans T: C (n x n) upper := A^T A + C, A is k x n.
+    a = mk(k, n, 24)
+    c = mk(n, n, 25)
+    c0 = f64(c)
+    up = tri_mask(n, False)
+    H.cblas_level3("syrk", lay, H.Upper, H.Trans, n, k, 1.0, a, ld(a), 1.0, c, ld(c), dtype=dtype)
+    want = f64(a).T @ f64(a) + c0
+    assert np.max(np.abs(f64(c)[up] - want[up])) <= tol * k * 4
+    assert np.array_equal(f64(c)[~up], c0[~up])
+    # SYR2K, uplo L, trans N: C lower := A B^T + B A^T (A, B n x k).
+    a, b = mk(n, k, 26), mk(n, k, 27)
+    c = mk(n, n, 28)
+    c0 = f64(c)
+    lo = tri_mask(n, True)
+    H.cblas_level3("syr2k", lay, H.Lower, H.NoTrans, n, k, 1.0, a, ld(a), b, ld(b), 0.0, c, ld(c),
+                   dtype=dtype)
+    want = f64(a) @ f64(b).T + f64(b) @ f64(a).T
+    assert np.max(np.abs(f64(c)[lo] - want[lo])) <= tol * k * 8
+    assert np.array_equal(f64(c)[~lo], c0[~lo])
+    # GEMMTR / GEMMT, uplo U, A^T (k x n stored) times B (k x n).
+    for name in ("gemmtr", "gemmt"):
+        a, b = mk(k, n, 29), mk(k, n, 30)
+        c = mk(n, n, 31)
+        c0 = f64(c)
+        H.cblas_level3(name, lay, H.Upper, H.Trans, H.NoTrans, n, k, 2.0, a, ld(a), b, ld(b), 0.5,
+                       c, ld(c), dtype=dtype)
+        want = 2.0 * f64(a).T @ f64(b) + 0.5 * c0
+        assert np.max(np.abs(f64(c)[up] - want[up])) <= tol * k * 8, name
+        assert np.array_equal(f64(c)[~up], c0[~up])
+    # TRMM, side R, uplo U, A^T, unit diagonal: B (m x n) := B op(A).
+    a = mk(n, n, 32)
+    t = np.where(tri_mask(n, False), f64(a), 0.0)
+    np.fill_diagonal(t, 1.0)
+    a[~tri_mask(n, False)] = np.nan
+    np.fill_diagonal(a, np.nan)
+    b = mk(m, n, 33)
+    want = f64(b) @ t.T
+    H.cblas_level3("trmm", lay, H.Right, H.Upper, H.Trans, H.Unit, m, n, 1.0, a, ld(a), b, ld(b),
+                   dtype=dtype)
+    assert np.max(np.abs(f64(b) - want)) <= tol * n * 4
