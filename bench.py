@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 METRIC = "DGEMM TFLOP/s @16384^3 emulated vs native FP64 (err<=2x native); batched GEMM/s"
 PEAK_FP64 = 78.6      # MI355X dense FP64 matrix, TFLOP/s
 PEAK_INT8 = 5000.0    # MI355X dense INT8 matrix, TOP/s
+PEAK_HBM = 8.0        # MI355X HBM3E, TB/s
 SEEDS = (1, 2, 3)
 
 
@@ -664,6 +665,21 @@ def level3_leg(H, n=8192, reps=3):
         out[name] = {"ms": round(t * 1e3, 2), "tflops": round(tf, 2),
                      "frac_fp64_peak": round(tf / PEAK_FP64, 3)}
         log(f"level3 {name} n={n}: {t * 1e3:.2f} ms, {tf:.1f} TFLOP/s")
+    # DGEMV at 16384 x 16384 (HBM-bound: 8 m n bytes of A per call).
+    gm = 16384
+    GA = H.DeviceArray(gm, gm, d)
+    GA.fill_uniform(gm, gm, 5)
+    gx, gy = H.DeviceArray(gm, 1, d), H.DeviceArray(gm, 1, d)
+    gx.fill_uniform(gm, 1, 6)
+    gy.fill_uniform(gm, 1, 7)
+    H.synchronize()
+    for tr in "NT":
+        t = best(lambda: H.gemv(tr, gm, gm, 1.0, GA, gm, gx, 1, 0.5, gy, 1, dtype=d))
+        gbs = 8.0 * gm * gm / t / 1e9
+        out[f"dgemv_{tr}"] = {"n": gm, "ms": round(t * 1e3, 3), "gb_per_s": round(gbs, 1),
+                              "frac_hbm_peak": round(gbs / (PEAK_HBM * 1e3), 3)}
+        log(f"level3 dgemv {tr} n={gm}: {t * 1e3:.3f} ms, {gbs:.0f} GB/s")
+    del GA
     # SYRK's lower triangle against DGEMM A A^T (beta = 0 on both).
     C.fill_uniform(n, n, 3)
     H.syrk("L", "N", n, n, 1.0, A, n, 0.0, C, n, dtype=d)

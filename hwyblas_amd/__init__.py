@@ -54,6 +54,7 @@ def _declare(L):
         fn(f"cblas_{pre}trsm", None, [_i, _i, _i, _i, _i, _i, _i, s, _p, _i, _p, _i])
         for name in ("gemmtr", "gemmt"):
             fn(f"cblas_{pre}{name}", None, [_i, _i, _i, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i])
+        fn(f"cblas_{pre}gemv", None, [_i, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i])
         fn(f"cblas_{pre}symm", None, [_i, _i, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i])
         fn(f"cblas_{pre}syrk", None, [_i, _i, _i, _i, _i, s, _p, _i, s, _p, _i])
         fn(f"cblas_{pre}syr2k", None, [_i, _i, _i, _i, _i, s, _p, _i, _p, _i, s, _p, _i])
@@ -231,6 +232,14 @@ def cblas_gemm(layout, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, dty
     dtype = dtype or c.dtype
     f = getattr(lib(), f"cblas_{_pre(dtype)}gemm")
     f(layout, ta, tb, m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc)
+
+
+def gemv(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, dtype=None):
+    """Fortran ?gemv_ (host arrays or DeviceArrays); y in place."""
+    dtype = dtype or y.dtype
+    f = getattr(lib(), f"{_pre(dtype)}gemv_")
+    f(trans.encode(), _ip(m), _ip(n), _sp(alpha, dtype), _ptr(a), _ip(lda), _ptr(x), _ip(incx),
+      _sp(beta, dtype), _ptr(y), _ip(incy))
 
 
 def cblas_level3(routine, *args, dtype):

@@ -58,6 +58,7 @@ template <typename T>
 struct Fortran;
 template <>
 struct Fortran<double> {
+  static constexpr auto gemv = dgemv_;
   static constexpr auto gemm = dgemm_;
   static constexpr auto trsm = dtrsm_;
   static constexpr auto gemmtr = dgemmtr_;
@@ -69,6 +70,7 @@ struct Fortran<double> {
 };
 template <>
 struct Fortran<float> {
+  static constexpr auto gemv = sgemv_;
   static constexpr auto gemm = sgemm_;
   static constexpr auto trsm = strsm_;
   static constexpr auto gemmtr = sgemmtr_;
@@ -111,6 +113,25 @@ void Trsm(const char* name, CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO upl
   CblasScope scope(name, row ? std::initializer_list<int>{2, 3, 4, 5, 7, 6, 8, 9, 10, 11, 12}
                              : std::initializer_list<int>{2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12});
   Fortran<T>::trsm(&sc, &uc, &tc, &dc, &m, &n, &alpha, a, &lda, b, &ldb);
+}
+
+// GEMV. Row-major: op(A) of the row-major A is the other op of the
+// column-major A^T (n x m). Positions: layout 1, trans 2, m 3, n 4, alpha 5,
+// a 6, lda 7, x 8, incx 9, beta 10, y 11, incy 12.
+template <typename T>
+void Gemv(const char* name, CBLAS_LAYOUT layout, CBLAS_TRANSPOSE trans, blasint m, blasint n,
+          T alpha, const T* a, blasint lda, const T* x, blasint incx, T beta, T* y,
+          blasint incy) {
+  if (!ValidLayout(name, layout)) return;
+  char tc = TransChar(trans);
+  const bool row = layout == CblasRowMajor;
+  if (row) {
+    tc = tc == 'N' ? 'T' : (tc == 'T' || tc == 'C') ? 'N' : tc;
+    std::swap(m, n);
+  }
+  CblasScope scope(name, row ? std::initializer_list<int>{2, 4, 3, 5, 6, 7, 8, 9, 10, 11, 12}
+                             : std::initializer_list<int>{2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12});
+  Fortran<T>::gemv(&tc, &m, &n, &alpha, a, &lda, x, &incx, &beta, y, &incy);
 }
 
 // GEMMTR (and its GEMMT name, `alias`). Row-major: the triangle of C^T =
@@ -210,6 +231,18 @@ void Trmm(const char* name, CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO upl
 }  // namespace hwy_amd
 
 extern "C" {
+
+#define HWY_CBLAS_GEMV(NAME, T)                                                               \
+  HWYBLAS_AMD_API void NAME(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOSE trans,           \
+                            const hwyblas_int m, const hwyblas_int n, const T alpha,          \
+                            const T* a, const hwyblas_int lda, const T* x,                    \
+                            const hwyblas_int incx, const T beta, T* y,                       \
+                            const hwyblas_int incy) {                                         \
+    hwy_amd::Gemv<T>(#NAME, layout, trans, m, n, alpha, a, lda, x, incx, beta, y, incy);      \
+  }
+HWY_CBLAS_GEMV(cblas_sgemv, float)
+HWY_CBLAS_GEMV(cblas_dgemv, double)
+#undef HWY_CBLAS_GEMV
 
 #define HWY_CBLAS_GEMMTR(NAME, T, ALIAS)                                                      \
   HWYBLAS_AMD_API void NAME(const CBLAS_LAYOUT layout, const CBLAS_UPLO uplo,                 \

@@ -78,6 +78,20 @@ HWYBLAS_AMD_API void dtrsm_(const char* side, const char* uplo,
                             const hwyblas_int* lda, double* b,
                             const hwyblas_int* ldb);
 
+/* y := alpha op(A) x + beta y on the GPU (hwyBLAS src/gemv.cc:39-84: info 1
+ * trans, 2 m, 3 n, 6 lda, 8 incx, 11 incy; quick return for m = 0, n = 0 or
+ * alpha = 0 and beta = 1; beta = 0 never reads y; a negative increment walks
+ * the vector from its end). Host or device operands as for dgemm_;
+ * synchronous. Replaces sgemv_/dgemv_ (include/hwyblas/hwyblas.h:219-228). */
+HWYBLAS_AMD_API void sgemv_(const char* trans, const hwyblas_int* m, const hwyblas_int* n,
+                            const float* alpha, const float* a, const hwyblas_int* lda,
+                            const float* x, const hwyblas_int* incx, const float* beta,
+                            float* y, const hwyblas_int* incy);
+HWYBLAS_AMD_API void dgemv_(const char* trans, const hwyblas_int* m, const hwyblas_int* n,
+                            const double* alpha, const double* a, const hwyblas_int* lda,
+                            const double* x, const hwyblas_int* incx, const double* beta,
+                            double* y, const hwyblas_int* incy);
+
 /* The other real Level-3 routines, on the same GEMM kernel, with hwyBLAS's
  * argument checks, info positions and quick returns (src/level3.cc:25-345).
  * Host or device operands as for dgemm_; synchronous.
@@ -213,6 +227,18 @@ HWYBLAS_AMD_API void cblas_dtrsm(const CBLAS_LAYOUT layout,
                                  const hwyblas_int n, const double alpha,
                                  const double* a, const hwyblas_int lda,
                                  double* b, const hwyblas_int ldb);
+/* GEMV (src/cblas.cc:279-330; include/hwyblas/cblas.h:227-240): row-major
+ * takes the other transposition and swaps m and n. */
+HWYBLAS_AMD_API void cblas_sgemv(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOSE trans,
+                                 const hwyblas_int m, const hwyblas_int n, const float alpha,
+                                 const float* a, const hwyblas_int lda, const float* x,
+                                 const hwyblas_int incx, const float beta, float* y,
+                                 const hwyblas_int incy);
+HWYBLAS_AMD_API void cblas_dgemv(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOSE trans,
+                                 const hwyblas_int m, const hwyblas_int n, const double alpha,
+                                 const double* a, const hwyblas_int lda, const double* x,
+                                 const hwyblas_int incx, const double beta, double* y,
+                                 const hwyblas_int incy);
 /* CBLAS forms of the other real Level-3 routines (src/cblas.cc:543-701;
  * declarations include/hwyblas/cblas.h:641-840). Row-major: GEMMTR flips
  * uplo and swaps A and B with their transpositions; SYMM flips side and

@@ -321,3 +321,21 @@ def test_cblas_level3_error_positions(layout, routine, args, position):
             H.cblas_level3("trmm", lay, g["side"], g["uplo"], g["ta"], g["diag"], g["m"],
                            g["n"], 1.0, None, g["lda"], None, g["ldb"], dtype=np.float64)
     assert cap.calls == [(f"cblas_d{routine}", position)]
+
+
+@pytest.mark.parametrize("change,info", [
+    (dict(trans="X"), 1), (dict(m=-1), 2), (dict(n=-1), 3), (dict(lda=2), 6),
+    (dict(incx=0), 8), (dict(incy=0), 11)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_gemv_invalid_arguments(change, info, dtype):
+    g = dict(trans="N", m=3, n=2, lda=3, incx=1, incy=1)
+    g.update(change)
+    x = np.zeros(8, dtype=dtype)
+    with H.ErrorCapture() as cap:
+        H.gemv(g["trans"], g["m"], g["n"], 1.0, x, g["lda"], x, g["incx"], 1.0, x, g["incy"],
+               dtype=dtype)
+    assert cap.calls == [(("D" if dtype == np.float64 else "S") + "GEMV", info)]
+    with H.ErrorCapture() as cap:  # CBLAS row-major swaps m and n; positions stay
+        H.cblas_level3("gemv", H.RowMajor, H.NoTrans, 3, -1, 1.0, None, 3, None, 1, 1.0, None,
+                       1, dtype=dtype)
+    assert cap.calls == [(f"cblas_{'d' if dtype == np.float64 else 's'}gemv", 4)]
