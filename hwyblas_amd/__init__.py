@@ -242,6 +242,31 @@ def gemv(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, dtype=None):
       _sp(beta, dtype), _ptr(y), _ip(incy))
 
 
+def potrf(uplo, n, a, lda, dtype=None):
+    """Fortran ?potrf_ (host array or DeviceArray) in place; returns info."""
+    dtype = dtype or a.dtype
+    info = ctypes.c_int(0)
+    getattr(lib(), f"{_pre(dtype)}potrf_")(uplo.encode(), _ip(n), _ptr(a), _ip(lda),
+                                          ctypes.byref(info))
+    return info.value
+
+
+def potrs(uplo, n, nrhs, a, lda, b, ldb, dtype=None):
+    dtype = dtype or b.dtype
+    info = ctypes.c_int(0)
+    getattr(lib(), f"{_pre(dtype)}potrs_")(uplo.encode(), _ip(n), _ip(nrhs), _ptr(a), _ip(lda),
+                                          _ptr(b), _ip(ldb), ctypes.byref(info))
+    return info.value
+
+
+def posv(uplo, n, nrhs, a, lda, b, ldb, dtype=None):
+    dtype = dtype or b.dtype
+    info = ctypes.c_int(0)
+    getattr(lib(), f"{_pre(dtype)}posv_")(uplo.encode(), _ip(n), _ip(nrhs), _ptr(a), _ip(lda),
+                                         _ptr(b), _ip(ldb), ctypes.byref(info))
+    return info.value
+
+
 def cblas_level3(routine, *args, dtype):
     """cblas_{s,d}{routine} with its CBLAS arguments (enums as ints)."""
     f = getattr(lib(), f"cblas_{_pre(dtype)}{routine}")

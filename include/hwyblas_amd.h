@@ -486,6 +486,32 @@ HWYBLAS_AMD_API int hwyblas_mlp_train(void* model, int n, const float* features,
 HWYBLAS_AMD_API void hwyblas_grouped_set_model(void* model);
 
 /* ------------------------------------------------------------------------ */
+/* Cholesky on the GPU (hwyBLAS src/lapack/potrf.cc, posv.cc; LAPACK
+ * signatures): ?potrf_ factors A = L L^T (uplo L) or U^T U (U) in place, the
+ * other triangle untouched; info 0, -i for argument i (1 uplo, 2 n, 4 lda;
+ * reported as ?POTRF), or the order of the first leading minor that is not
+ * positive (or NaN; entries past that column are then not specified).
+ * ?potrs_ solves with the factor (info positions 1 uplo, 2 n, 3 nrhs, 5 lda,
+ * 7 ldb); ?posv_ factors, then solves if the factorization succeeded. Host
+ * or device operands as for dgemm_; synchronous. */
+HWYBLAS_AMD_API void spotrf_(const char* uplo, const hwyblas_int* n, float* a,
+                             const hwyblas_int* lda, hwyblas_int* info);
+HWYBLAS_AMD_API void dpotrf_(const char* uplo, const hwyblas_int* n, double* a,
+                             const hwyblas_int* lda, hwyblas_int* info);
+HWYBLAS_AMD_API void spotrs_(const char* uplo, const hwyblas_int* n, const hwyblas_int* nrhs,
+                             const float* a, const hwyblas_int* lda, float* b,
+                             const hwyblas_int* ldb, hwyblas_int* info);
+HWYBLAS_AMD_API void dpotrs_(const char* uplo, const hwyblas_int* n, const hwyblas_int* nrhs,
+                             const double* a, const hwyblas_int* lda, double* b,
+                             const hwyblas_int* ldb, hwyblas_int* info);
+HWYBLAS_AMD_API void sposv_(const char* uplo, const hwyblas_int* n, const hwyblas_int* nrhs,
+                            float* a, const hwyblas_int* lda, float* b, const hwyblas_int* ldb,
+                            hwyblas_int* info);
+HWYBLAS_AMD_API void dposv_(const char* uplo, const hwyblas_int* n, const hwyblas_int* nrhs,
+                            double* a, const hwyblas_int* lda, double* b, const hwyblas_int* ldb,
+                            hwyblas_int* info);
+
+/* ------------------------------------------------------------------------ */
 /* LU consumer on device memory: hwyBLAS's LAPACK port calls dgemm_/dtrsm_
  * from dgetrf/dgetrs/dgerfs (src/lapack/getrf.cc:216-339,
  * src/lapack/gesvx.cc:333-463); these run the same algorithms on the GPU. */

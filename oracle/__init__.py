@@ -88,6 +88,8 @@ def lib():
         L.orc_dgerfs_berr.argtypes = [_i, _i, _p, _i, _p, _i, _p, _i, _p]
         L.orc_dlacn2.restype = None
         L.orc_dlacn2.argtypes = [_i, _p, _p, _p, _p, _p, _p]
+        L.orc_dpotrf2.restype = _i
+        L.orc_dpotrf2.argtypes = [_c_char, _i, _p, _i]
         L.orc_dgerfs.restype = _i
         L.orc_dgerfs.argtypes = [_c_char, _i, _i, _p, _i, _p, _i, _p, _p, _i, _p, _i, _p, _p]
         L.orc_plant_mistake.restype = None
@@ -272,6 +274,14 @@ def gerfs(trans, a, lu_, ipiv, b, x):
     if info != 0:
         raise ValueError(f"gerfs: invalid argument {-info}")
     return xr, ferr[:nrhs], berr[:nrhs]
+
+
+def potrf2(uplo, a):
+    """xPOTRF2 of a copy of `a` (Fortran order): (factored copy, info)."""
+    f = np.array(a, dtype=np.float64, order="F", copy=True)
+    n = f.shape[0]
+    info = lib().orc_dpotrf2(_b(uplo), n, _ptr(f), max(1, n))
+    return f, info
 
 
 def lu_factors(lu_, ipiv):

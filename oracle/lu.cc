@@ -368,4 +368,50 @@ int orc_dgerfs(char trans, int n, int nrhs, const double* a, int lda, const doub
   return 0;
 }
 
+// xPOTRF2 (src/lapack/potrf.cc:49-122) as LAPACK states it: the recursion
+// n1 = n / 2 down to n = 1 (without hwyBLAS's leaf kernel), A21 := A21 L11^-T
+// (or A12 := U11^-T A12) by the oracle's TRSM, then the triangle of A22
+// minus A21 A21^T (A12^T A12), each entry's dot product in index order, then
+// A22. Returns 0, or the order of the first leading minor that is not
+// positive (or NaN). Planted mistake 12: the update uses row i twice
+// (A21(i, :) A21(i, :)^T).
+int orc_dpotrf2(char uplo, int n, double* a, int lda) {
+  if (n == 0) return 0;
+  const bool upper = uplo == 'U' || uplo == 'u';
+  if (n == 1) {
+    if (!(a[0] > 0.0)) return 1;
+    a[0] = std::sqrt(a[0]);
+    return 0;
+  }
+  const int n1 = n / 2, n2 = n - n1;
+  int info = orc_dpotrf2(uplo, n1, a, lda);
+  if (info != 0) return info;
+  double* a22 = a + n1 + int64_t(n1) * lda;
+  if (upper) {
+    double* a12 = a + int64_t(n1) * lda;
+    orc_dtrsm('L', 'U', 'T', 'N', n1, n2, 1.0, a, lda, a12, lda);
+    for (int j = 0; j < n2; ++j)
+      for (int i = 0; i <= j; ++i) {
+        const int ii = g_mistake == 12 ? j : i;
+        double acc = 0.0;
+        for (int k = 0; k < n1; ++k)
+          acc = std::fma(a12[k + int64_t(ii) * lda], a12[k + int64_t(j) * lda], acc);
+        a22[i + int64_t(j) * lda] = -acc + a22[i + int64_t(j) * lda];
+      }
+  } else {
+    double* a21 = a + n1;
+    orc_dtrsm('R', 'L', 'T', 'N', n2, n1, 1.0, a, lda, a21, lda);
+    for (int j = 0; j < n2; ++j)
+      for (int i = j; i < n2; ++i) {
+        const int jj = g_mistake == 12 ? i : j;
+        double acc = 0.0;
+        for (int k = 0; k < n1; ++k)
+          acc = std::fma(a21[i + int64_t(k) * lda], a21[jj + int64_t(k) * lda], acc);
+        a22[i + int64_t(j) * lda] = -acc + a22[i + int64_t(j) * lda];
+      }
+  }
+  info = orc_dpotrf2(uplo, n2, a22, lda);
+  return info != 0 ? info + n1 : 0;
+}
+
 }  // extern "C"

@@ -144,6 +144,10 @@ void orc_dlacn2(int n, double* v, double* x, int* isgn, double* est, int* kase, 
 int orc_dgerfs(char trans, int n, int nrhs, const double* a, int lda, const double* af,
                int ldaf, const int* ipiv, const double* b, int ldb, double* x, int ldx,
                double* ferr, double* berr);
+/* xPOTRF2 (Cholesky by recursive halving down to n = 1) in place: the uplo
+ * triangle of A becomes L (A = L L^T) or U (A = U^T U); returns 0 or the
+ * order of the first non-positive (or NaN) leading minor. */
+int orc_dpotrf2(char uplo, int n, double* a, int lda);
 
 /* ---- planted mistakes (for tests that show the checks catch them) ------- */
 /* 0: none; 1: drop the last k term of every GEMM tile; 2: use op(A) with
@@ -153,7 +157,8 @@ int orc_dgerfs(char trans, int n, int nrhs, const double* a, int lda, const doub
  * the columns left of the pivot column; 8: the BF16-slice emulation leaves
  * out its top level of slice pairs (t + u = s + 1); 9: the 1-norm estimator
  * skips its alternating-vector stage; 10: gerfs's forward error bound solves
- * with op(A) where op(A)^T belongs; 11: that bound takes nz = n. */
+ * with op(A) where op(A)^T belongs; 11: that bound takes nz = n; 12: the
+ * Cholesky update multiplies a row of the off-diagonal block by itself. */
 void orc_plant_mistake(int which);
 
 #ifdef __cplusplus

@@ -339,3 +339,24 @@ def test_gemv_invalid_arguments(change, info, dtype):
         H.cblas_level3("gemv", H.RowMajor, H.NoTrans, 3, -1, 1.0, None, 3, None, 1, 1.0, None,
                        1, dtype=dtype)
     assert cap.calls == [(f"cblas_{'d' if dtype == np.float64 else 's'}gemv", 4)]
+
+
+@pytest.mark.parametrize("routine,change,info", [
+    ("potrf", dict(uplo="X"), 1), ("potrf", dict(n=-1), 2), ("potrf", dict(lda=2), 4),
+    ("potrs", dict(uplo="X"), 1), ("potrs", dict(n=-1), 2), ("potrs", dict(nrhs=-1), 3),
+    ("potrs", dict(lda=2), 5), ("potrs", dict(ldb=2), 7),
+    ("posv", dict(uplo="X"), 1), ("posv", dict(nrhs=-1), 3), ("posv", dict(ldb=1), 7)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_cholesky_invalid_arguments(routine, change, info, dtype):
+    g = dict(uplo="L", n=3, nrhs=2, lda=3, ldb=3)
+    g.update(change)
+    x = np.zeros(16, dtype=dtype)
+    with H.ErrorCapture() as cap:
+        if routine == "potrf":
+            got = H.potrf(g["uplo"], g["n"], x, g["lda"], dtype=dtype)
+        elif routine == "potrs":
+            got = H.potrs(g["uplo"], g["n"], g["nrhs"], x, g["lda"], x, g["ldb"], dtype=dtype)
+        else:
+            got = H.posv(g["uplo"], g["n"], g["nrhs"], x, g["lda"], x, g["ldb"], dtype=dtype)
+    assert got == -info
+    assert cap.calls == [(("D" if dtype == np.float64 else "S") + routine.upper(), info)]

@@ -669,3 +669,51 @@ def test_gerfs_quick_return_and_bad_trans():
     assert ferr.size == 0 and berr.size == 0
     with pytest.raises(ValueError):
         oracle.gerfs("X", a, lu, piv, np.ones((3, 1)), np.ones((3, 1)))
+
+
+# ---- xPOTRF2 ------------------------------------------------------------------
+
+def _unit_lower(n, seed):
+    rng = np.random.default_rng(seed)
+    low = np.tril(rng.integers(-3, 4, (n, n)).astype(float), -1) + np.eye(n)
+    return low
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_potrf2_exact_on_unit_triangular_products(uplo):
+    """A = L L^T with an integer unit lower L: every pivot is 1 and every
+    step exact, so the factor is L itself (U = L^T for uplo U); the other
+    triangle is untouched."""
+    for n in (1, 2, 5, 17, 40):
+        low = _unit_lower(n, n)
+        a = low @ low.T
+        f, info = oracle.potrf2(uplo, a)
+        assert info == 0
+        tri = np.tril if uplo == "L" else np.triu
+        assert np.array_equal(tri(f), low if uplo == "L" else low.T)
+        other = np.triu(np.ones((n, n), dtype=bool), 1) if uplo == "L" else \
+            np.tril(np.ones((n, n), dtype=bool), -1)
+        assert np.array_equal(f[other], a[other])
+
+
+def test_potrf2_against_numpy_and_info():
+    rng = np.random.default_rng(3)
+    n = 60
+    m = rng.standard_normal((n, n))
+    a = m @ m.T + n * np.eye(n)
+    f, info = oracle.potrf2("L", a)
+    assert info == 0
+    ref = np.linalg.cholesky(a)
+    assert np.max(np.abs(np.tril(f) - ref)) <= 1e-12 * np.max(np.abs(ref))
+    d = np.diag([4.0, 9.0, -1.0, 16.0])
+    assert oracle.potrf2("L", d)[1] == 3
+    d[2, 2] = np.nan
+    assert oracle.potrf2("U", d)[1] == 3
+    assert oracle.potrf2("L", np.zeros((0, 0)))[1] == 0
+
+
+def test_planted_potrf_mistake_is_caught():
+    oracle.plant_mistake(12)
+    low = _unit_lower(17, 17)
+    f, info = oracle.potrf2("L", low @ low.T)
+    assert info != 0 or not np.array_equal(np.tril(f), low)
