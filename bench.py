@@ -619,7 +619,8 @@ def level3_leg(H, n=8192, reps=3):
     entries (synchronous; best of `reps` wall-clock calls), uniform[-1, 1],
     alpha = 1, beta = 0.5 (TRMM: alpha = 1). Algorithmic flops: SYMM 2 n^3,
     SYRK n^2 (n + 1), SYR2K 2 n^2 (n + 1), GEMMTR n^2 (n + 1), TRMM n^3; beside
-    them DGEMM 2 n^3. Check: SYRK's triangle equals the same tile
+    them DGEMM 2 n^3. Then DGEMV N and T at 16384 (bytes of A) and DPOTRF (L)
+    at 16384 (n^3 / 3 flops). Check: SYRK's triangle equals the same tile
     configuration's DGEMM A A^T bit for bit."""
     L = H.lib()
     d = np.float64
@@ -680,6 +681,32 @@ def level3_leg(H, n=8192, reps=3):
                               "frac_hbm_peak": round(gbs / (PEAK_HBM * 1e3), 3)}
         log(f"level3 dgemv {tr} n={gm}: {t * 1e3:.3f} ms, {gbs:.0f} GB/s")
     del GA
+    # DPOTRF (L) at 16384 on A = G G^T / n + I (SPD), G uniform[-1, 1], formed
+    # on the device with the library's DGEMM; n^3 / 3 flops.
+    pn = 16384
+    G = H.DeviceArray(pn, pn, d)
+    G.fill_uniform(pn, pn, 11)
+    P = H.DeviceArray(pn, pn, d)
+    Q = H.DeviceArray(pn, pn, d)
+    L.hwyblas_amd_memset(P.ptr, 0, pn * pn * 8)
+    one = np.ones(1)
+    for i in range(pn):
+        L.hwyblas_amd_memcpy(P.ptr + 8 * (i + i * pn), one.ctypes.data, 8, 1)
+    H.gemm("N", "T", pn, pn, pn, 1.0 / pn, G, pn, G, pn, 1.0, P, pn, dtype=d)
+    H.synchronize()
+    times, info = [], None
+    for _ in range(2):
+        L.hwyblas_amd_memcpy(Q.ptr, P.ptr, pn * pn * 8, 3)
+        H.synchronize()
+        t0 = time.perf_counter()
+        info = H.potrf("L", pn, Q, pn, dtype=d)
+        times.append(time.perf_counter() - t0)
+    t = min(times)
+    tf = pn ** 3 / 3.0 / t / 1e12
+    out["dpotrf"] = {"n": pn, "ms": round(t * 1e3, 1), "tflops": round(tf, 2),
+                     "frac_fp64_peak": round(tf / PEAK_FP64, 3), "info": int(info)}
+    log(f"level3 dpotrf n={pn}: {t * 1e3:.1f} ms, {tf:.1f} TFLOP/s, info {info}")
+    del G, P, Q
     # SYRK's lower triangle against DGEMM A A^T (beta = 0 on both).
     C.fill_uniform(n, n, 3)
     H.syrk("L", "N", n, n, 1.0, A, n, 0.0, C, n, dtype=d)
@@ -771,7 +798,8 @@ def main():
                     help="the mixed-precision LU solve (n = --lu-n, kappa 1e2, 1e6, 1e10) "
                          "instead of the headline")
     ap.add_argument("--level3", action="store_true",
-                    help="DSYMM/DSYRK/DSYR2K/DGEMMTR/DTRMM at n = 8192 beside DGEMM")
+                    help="DSYMM/DSYRK/DSYR2K/DGEMMTR/DTRMM at n = 8192 beside DGEMM, "
+                         "DGEMV and DPOTRF at 16384")
     ap.add_argument("--lu-n", type=int, default=16384)
     ap.add_argument("--lu-kappas", default="2,6,10",
                     help="comma-separated log10 condition numbers for --lu")
