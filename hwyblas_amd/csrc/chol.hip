@@ -19,50 +19,89 @@ namespace hwy_amd {
 namespace {
 
 constexpr int kCholLeaf = 64;
+// Columns the leaf factors together in registers between LDS exchanges.
+constexpr int kCholPanel = 8;
+
+// Lane `lane`'s v in every lane (lane wave-uniform): v_readlane per 32 bits.
+__device__ inline double LaneValue(double v, int lane) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(bits & 0xFFFFFFFFll), lane);
+  const int hi = __builtin_amdgcn_readlane(int(bits >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+__device__ inline float LaneValue(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
 
 // In-place Cholesky of the n x n (n <= kCholLeaf) diagonal block at a: the
 // lower triangle (A = L L^T) or the upper one (A = U^T U, handled as the
 // lower factor of its transpose). A non-positive or NaN pivot j records
 // offset + j + 1 in *info (atomic min) and stops; a block whose *info is
 // already set does nothing.
-template <typename T>
+//
+// One wave; lane i holds row i of the block in registers (every index a
+// compile-time constant), padded to kCholLeaf with the identity. Columns are
+// finished kCholPanel at a time, left-looking: a panel first takes the
+// products of the finished columns (from LDS, kept column-major so a row's
+// panel entries are one broadcast read), then factors its own columns with
+// the diagonal and the column entries passed between lanes by v_readlane.
+// Each entry sees the same sequence of fused multiply-adds as the
+// right-looking column order: a_ij -= l_ik l_jk for k = 0, 1, ..., then
+// l_ij = a_ij / sqrt(a_jj).
+template <typename T, bool UPPER>
 __global__ void __launch_bounds__(kCholLeaf)
-    CholLeafKernel(T* __restrict__ a, int64_t lda, int n, bool upper, int offset,
-                   int* __restrict__ info) {
-  __shared__ T t[kCholLeaf][kCholLeaf + 1];
-  __shared__ int failed;
+    CholLeafKernel(T* __restrict__ a, int64_t lda, int n, int offset, int* __restrict__ info) {
+  __shared__ __align__(16) T lcol[kCholLeaf][kCholLeaf];  // lcol[c][r] = l_rc
+  if (__builtin_amdgcn_readfirstlane(*info) != INT_MAX) return;
   const int i = threadIdx.x;
-  if (i == 0) failed = (*info != INT_MAX);
-  __syncthreads();
-  if (failed) return;
-  // t[r][c], r >= c: the block's lower factor in the making.
-  for (int c = 0; c < n; ++c)
-    if (i < n) t[i][c] = (i >= c) ? (upper ? a[c + int64_t(i) * lda] : a[i + int64_t(c) * lda]) : T(0);
-  __syncthreads();
-  int fail_at = -1;
-  for (int j = 0; j < n; ++j) {
-    const T d = t[j][j];
-    if (!(d > T(0))) {
-      fail_at = j;
-      break;
+  const int ri = min(i, n - 1);
+  T r[kCholLeaf];
+#pragma unroll
+  for (int c = 0; c < kCholLeaf; ++c) {
+    const int rc = min(c, n - 1);
+    r[c] = UPPER ? a[rc + int64_t(ri) * lda] : a[ri + int64_t(rc) * lda];
+  }
+#pragma unroll
+  for (int c = 0; c < kCholLeaf; ++c)
+    if (i >= n || c >= n) r[c] = (i == c) ? T(1) : T(0);
+  int done = kCholLeaf;
+#pragma unroll
+  for (int p = 0; p < kCholLeaf; p += kCholPanel) {
+#pragma unroll 4
+    for (int k = 0; k < p; ++k) {
+      const T lik = lcol[k][i];
+#pragma unroll
+      for (int c = 0; c < kCholPanel; ++c)
+        r[p + c] = __builtin_fma(-lik, lcol[k][p + c], r[p + c]);
     }
-    const T ljj = sqrt(d);
-    if (i > j && i < n) t[i][j] = t[i][j] / ljj;
-    __syncthreads();
-    if (i == j) t[j][j] = ljj;
-    if (i > j && i < n) {
-      const T lij = t[i][j];
-      for (int k = j + 1; k <= i; ++k) t[i][k] = __builtin_fma(-lij, t[k][j], t[i][k]);
+#pragma unroll
+    for (int c = 0; c < kCholPanel; ++c) {
+      const int j = p + c;
+      const T d = LaneValue(r[j], j);
+      if (!(d > T(0))) {
+        done = j;
+        break;
+      }
+      const T ljj = sqrt(d);
+      const T scaled = r[j] / ljj;
+      r[j] = i > j ? scaled : (i == j ? ljj : r[j]);
+#pragma unroll
+      for (int c2 = c + 1; c2 < kCholPanel; ++c2)
+        r[p + c2] = __builtin_fma(-r[j], LaneValue(r[j], p + c2), r[p + c2]);
+      lcol[j][i] = r[j];
     }
+    if (done < kCholLeaf) break;
     __syncthreads();
   }
-  if (fail_at >= 0 && i == 0) atomicMin(info, offset + fail_at + 1);
-  // Write back every column before the failure (all of them on success).
-  const int done = fail_at >= 0 ? fail_at : n;
-  for (int c = 0; c < done; ++c) {
-    if (i < n && i >= c) {
-      if (upper) a[c + int64_t(i) * lda] = t[i][c];
-      else a[i + int64_t(c) * lda] = t[i][c];
+  if (done < n && i == 0) atomicMin(info, offset + done + 1);
+  // Lane i writes its own row: every column before the failure (all of them
+  // on success).
+#pragma unroll
+  for (int c = 0; c < kCholLeaf; ++c) {
+    if (c < done && c < n && i >= c && i < n) {
+      if (UPPER) a[c + int64_t(i) * lda] = r[c];
+      else a[i + int64_t(c) * lda] = r[c];
     }
   }
 }
@@ -70,8 +109,12 @@ __global__ void __launch_bounds__(kCholLeaf)
 template <typename T>
 void CholRecurse(bool upper, int n, T* a, int64_t lda, int offset, int* info, hipStream_t s) {
   if (n <= kCholLeaf) {
-    hipLaunchKernelGGL(CholLeafKernel<T>, dim3(1), dim3(kCholLeaf), 0, s, a, lda, n, upper, offset,
-                       info);
+    if (upper)
+      hipLaunchKernelGGL((CholLeafKernel<T, true>), dim3(1), dim3(kCholLeaf), 0, s, a, lda, n,
+                         offset, info);
+    else
+      hipLaunchKernelGGL((CholLeafKernel<T, false>), dim3(1), dim3(kCholLeaf), 0, s, a, lda, n,
+                         offset, info);
     HWY_HIP_CHECK(hipGetLastError());
     return;
   }
