@@ -100,17 +100,21 @@ __global__ void __launch_bounds__(64 * kNarrowWaves)
   for (int u = 0; u < kCols; ++u) st[lane][wave + kNarrowWaves * u] = trow[u];
   __syncthreads();
   if (nvc <= 0) return;
-  for (int i = 0; i < r; ++i) {
-    const T inv = sinv[i];
-    if (lane == i) {
+  // The lane's row of the triangle and its own reciprocal in registers, so
+  // the solve's dependent chain is only scale -> v_readlane -> FMA. Steps at
+  // or past r touch zeros only (padding rows and columns are 0, their
+  // diagonal 1).
+  T tcol[kNB];
 #pragma unroll
-      for (int c = 0; c < kWaveRhs; ++c) x[c] = x[c] * inv;
-    }
-    const T t = (lane > i && lane < r) ? st[lane][i] : T(0);
+  for (int i = 0; i < kNB; ++i) tcol[i] = st[lane][i];
+  const T myinv = sinv[lane];
+#pragma unroll
+  for (int i = 0; i < kNB; ++i) {
 #pragma unroll
     for (int c = 0; c < kWaveRhs; ++c) {
+      if (lane == i) x[c] = x[c] * myinv;
       const T xi = Broadcast(x[c], i);
-      if (lane > i) x[c] = __builtin_fma(-t, xi, x[c]);
+      if (lane > i) x[c] = __builtin_fma(-tcol[i], xi, x[c]);
     }
   }
   if (lane < r) {
