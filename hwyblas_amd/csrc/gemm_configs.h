@@ -19,7 +19,9 @@
   X(7, 32, 32, 16, 1, 1, 1)      \
   X(8, 128, 128, 16, 2, 2, 2)    \
   X(9, 128, 128, 32, 2, 2, 1)    \
-  X(10, 256, 128, 8, 4, 2, 1)
+  X(10, 256, 128, 8, 4, 2, 1)    \
+  X(11, 32, 32, 32, 1, 1, 1)     \
+  X(12, 32, 32, 64, 1, 1, 1)
 
 namespace hwy_amd {
 
