@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "DGEMM TFLOP/s @16384^3 emulated vs native FP64 (err<=2x native); batched GEMM/s"
 PEAK_FP64 = 78.6      # MI355X dense FP64 matrix, TFLOP/s
+PEAK_FP32 = 157.3     # MI355X dense FP32 matrix, TFLOP/s
 PEAK_INT8 = 5000.0    # MI355X dense INT8 matrix, TOP/s
 PEAK_HBM = 8.0        # MI355X HBM3E, TB/s
 SEEDS = (1, 2, 3)
@@ -613,6 +614,69 @@ def lu_leg(H, n, kappas, stream, nrhs=16, reps=2):
     return out
 
 
+def zgemm_leg(H, n=8192, reps=3):
+    """ZGEMM and CGEMM at n^3 on device operands (interleaved complex, each
+    filled through its real 2n x n view, uniform [-1, 1]), alpha = 1 - 0.5i,
+    beta = 0.5 + 0.25i; 8 n^3 real flops per product, against the FP64 (FP32)
+    MFMA peak."""
+    out = {}
+    for name, rt, ct, peak in (("zgemm", np.float64, np.complex128, PEAK_FP64),
+                               ("cgemm", np.float32, np.complex64, PEAK_FP32)):
+        arrs = []
+        for seed in (21, 22, 23):
+            x = H.DeviceArray(2 * n, n, rt)
+            x.fill_uniform(2 * n, n, seed)
+            arrs.append(x)
+        A, B, C = arrs
+        H.synchronize()
+        for ta, tb in (("N", "N"), ("C", "N"), ("N", "T")):
+            times = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                info = H.zgemm_dev(ta, tb, n, n, n, 1.0 - 0.5j, A.ptr, n, B.ptr, n, 0.5 + 0.25j,
+                                   C.ptr, n, dtype=ct)
+                H.synchronize()
+                times.append(time.perf_counter() - t0)
+                assert info == 0
+            t = min(times)
+            tf = 8.0 * n ** 3 / t / 1e12
+            out[f"{name}_{ta}{tb}"] = {"n": n, "ms": round(t * 1e3, 2), "tflops": round(tf, 2),
+                                       "frac_peak": round(tf / peak, 3)}
+            log(f"{name} {ta}{tb} n={n}: {t * 1e3:.2f} ms, {tf:.1f} TFLOP/s")
+        del A, B, C, arrs
+    return out
+
+
+def potrf_leg(H, pn=16384, reps=2):
+    """DPOTRF (lower) at pn on A = G G^T / pn + I (SPD), G uniform[-1, 1] seed
+    11, formed on the device with the library's DGEMM; the best of `reps`
+    factorizations of fresh copies, n^3 / 3 flops."""
+    L = H.lib()
+    d = np.float64
+    G = H.DeviceArray(pn, pn, d)
+    G.fill_uniform(pn, pn, 11)
+    P = H.DeviceArray(pn, pn, d)
+    Q = H.DeviceArray(pn, pn, d)
+    L.hwyblas_amd_memset(P.ptr, 0, pn * pn * 8)
+    one = np.ones(1)
+    for i in range(pn):
+        L.hwyblas_amd_memcpy(P.ptr + 8 * (i + i * pn), one.ctypes.data, 8, 1)
+    H.gemm("N", "T", pn, pn, pn, 1.0 / pn, G, pn, G, pn, 1.0, P, pn, dtype=d)
+    H.synchronize()
+    times, info = [], None
+    for _ in range(reps):
+        L.hwyblas_amd_memcpy(Q.ptr, P.ptr, pn * pn * 8, 3)
+        H.synchronize()
+        t0 = time.perf_counter()
+        info = H.potrf("L", pn, Q, pn, dtype=d)
+        times.append(time.perf_counter() - t0)
+    t = min(times)
+    tf = pn ** 3 / 3.0 / t / 1e12
+    log(f"dpotrf n={pn}: {t * 1e3:.1f} ms, {tf:.1f} TFLOP/s, info {info}")
+    return {"n": pn, "ms": round(t * 1e3, 1), "tflops": round(tf, 2),
+            "frac_fp64_peak": round(tf / PEAK_FP64, 3), "info": int(info)}
+
+
 def level3_leg(H, n=8192, reps=3):
     """DSYMM (L, U), DSYRK (L, N), DSYR2K (L, N), DGEMMTR (L, N, T) and DTRMM
     (L, U, N, N) at m = n = k = `n` on device operands through the Fortran
@@ -681,32 +745,8 @@ def level3_leg(H, n=8192, reps=3):
                               "frac_hbm_peak": round(gbs / (PEAK_HBM * 1e3), 3)}
         log(f"level3 dgemv {tr} n={gm}: {t * 1e3:.3f} ms, {gbs:.0f} GB/s")
     del GA
-    # DPOTRF (L) at 16384 on A = G G^T / n + I (SPD), G uniform[-1, 1], formed
-    # on the device with the library's DGEMM; n^3 / 3 flops.
-    pn = 16384
-    G = H.DeviceArray(pn, pn, d)
-    G.fill_uniform(pn, pn, 11)
-    P = H.DeviceArray(pn, pn, d)
-    Q = H.DeviceArray(pn, pn, d)
-    L.hwyblas_amd_memset(P.ptr, 0, pn * pn * 8)
-    one = np.ones(1)
-    for i in range(pn):
-        L.hwyblas_amd_memcpy(P.ptr + 8 * (i + i * pn), one.ctypes.data, 8, 1)
-    H.gemm("N", "T", pn, pn, pn, 1.0 / pn, G, pn, G, pn, 1.0, P, pn, dtype=d)
-    H.synchronize()
-    times, info = [], None
-    for _ in range(2):
-        L.hwyblas_amd_memcpy(Q.ptr, P.ptr, pn * pn * 8, 3)
-        H.synchronize()
-        t0 = time.perf_counter()
-        info = H.potrf("L", pn, Q, pn, dtype=d)
-        times.append(time.perf_counter() - t0)
-    t = min(times)
-    tf = pn ** 3 / 3.0 / t / 1e12
-    out["dpotrf"] = {"n": pn, "ms": round(t * 1e3, 1), "tflops": round(tf, 2),
-                     "frac_fp64_peak": round(tf / PEAK_FP64, 3), "info": int(info)}
-    log(f"level3 dpotrf n={pn}: {t * 1e3:.1f} ms, {tf:.1f} TFLOP/s, info {info}")
-    del G, P, Q
+    out["dpotrf"] = potrf_leg(H)
+    out.update(zgemm_leg(H))
     # SYRK's lower triangle against DGEMM A A^T (beta = 0 on both).
     C.fill_uniform(n, n, 3)
     H.syrk("L", "N", n, n, 1.0, A, n, 0.0, C, n, dtype=d)
@@ -799,7 +839,11 @@ def main():
                          "instead of the headline")
     ap.add_argument("--level3", action="store_true",
                     help="DSYMM/DSYRK/DSYR2K/DGEMMTR/DTRMM at n = 8192 beside DGEMM, "
-                         "DGEMV and DPOTRF at 16384")
+                         "DGEMV and DPOTRF at 16384, ZGEMM/CGEMM at 8192")
+    ap.add_argument("--potrf", action="store_true",
+                    help="only DPOTRF (lower) at n = 16384")
+    ap.add_argument("--zgemm", action="store_true",
+                    help="only ZGEMM and CGEMM at n = 8192")
     ap.add_argument("--lu-n", type=int, default=16384)
     ap.add_argument("--lu-kappas", default="2,6,10",
                     help="comma-separated log10 condition numbers for --lu")
@@ -853,6 +897,12 @@ def main():
         return
     if args.level3:
         print(json.dumps({"level3": level3_leg(H)}), flush=True)
+        return
+    if args.potrf:
+        print(json.dumps({"dpotrf": potrf_leg(H)}), flush=True)
+        return
+    if args.zgemm:
+        print(json.dumps({"zgemm": zgemm_leg(H)}), flush=True)
         return
     if args.batched_only:
         out = run_batched(H, oracle, args, stream)
