@@ -63,6 +63,12 @@ def _declare(L):
         fn(f"hwyblas_{pre}trsm_dev", _i, [_c, _c, _c, _c, _i, _i, s, _p, _i, _p, _i, _p])
         fn(f"hwyblas_{pre}gemm_dev_cfg", _i, [_i, _c, _c, _i, _i, _i, s, _p, _i, _p, _i, s, _p,
                                              _i, _p])
+    for pre in ("z", "c"):
+        fn(f"{pre}gemm_", None, [ctypes.c_char_p, ctypes.c_char_p, _pi, _pi, _pi, _p, _p, _pi,
+                                 _p, _pi, _p, _p, _pi])
+        fn(f"cblas_{pre}gemm", None, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i])
+        fn(f"hwyblas_{pre}gemm_dev", _i, [_c, _c, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i,
+                                          _p])
     fn("hwyblas_dgemm_emulated_dev", _i, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i,
                                           _i, _p, _pi])
     fn("hwyblas_dgemm_emulated_bf16_dev", _i, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p,
@@ -277,6 +283,44 @@ def cblas_trsm(layout, side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, dtype=
     dtype = dtype or b.dtype
     f = getattr(lib(), f"cblas_{_pre(dtype)}trsm")
     f(layout, side, uplo, ta, diag, m, n, alpha, _ptr(a), lda, _ptr(b), ldb)
+
+
+def _zpre(dtype):
+    return "z" if np.dtype(dtype) == np.complex128 else "c"
+
+
+def _zscalar(v, dtype):
+    """(re, im) of a complex scalar as a real array of the type's precision."""
+    rt = np.float64 if np.dtype(dtype) == np.complex128 else np.float32
+    v = complex(v)
+    return np.array([v.real, v.imag], dtype=rt)
+
+
+def zgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, dtype=None):
+    """Fortran cgemm_/zgemm_ on numpy complex (host) arrays or DeviceArrays;
+    C in place."""
+    dtype = dtype or c.dtype
+    al, be = _zscalar(alpha, dtype), _zscalar(beta, dtype)
+    getattr(lib(), f"{_zpre(dtype)}gemm_")(
+        ta.encode(), tb.encode(), _ip(m), _ip(n), _ip(k), _ptr(al), _ptr(a), _ip(lda), _ptr(b),
+        _ip(ldb), _ptr(be), _ptr(c), _ip(ldc))
+
+
+def cblas_zgemm(layout, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, dtype=None):
+    """cblas_cgemm/cblas_zgemm (enums as ints, scalars as Python complex)."""
+    dtype = dtype or c.dtype
+    al, be = _zscalar(alpha, dtype), _zscalar(beta, dtype)
+    getattr(lib(), f"cblas_{_zpre(dtype)}gemm")(layout, ta, tb, m, n, k, _ptr(al), _ptr(a), lda,
+                                                _ptr(b), ldb, _ptr(be), _ptr(c), ldc)
+
+
+def zgemm_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None, dtype=None):
+    """hwyblas_{c,z}gemm_dev on device operands; returns the info."""
+    dtype = dtype or c.dtype
+    al, be = _zscalar(alpha, dtype), _zscalar(beta, dtype)
+    return getattr(lib(), f"hwyblas_{_zpre(dtype)}gemm_dev")(
+        ta.encode(), tb.encode(), m, n, k, _ptr(al), _ptr(a), lda, _ptr(b), ldb, _ptr(be),
+        _ptr(c), ldc, stream)
 
 
 def gemm_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None, dtype=None):

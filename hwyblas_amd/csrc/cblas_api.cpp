@@ -97,6 +97,35 @@ void Gemm(const char* name, CBLAS_LAYOUT layout, CBLAS_TRANSPOSE transa,
   }
 }
 
+// Complex GEMM: the same mapping and positions as Gemm, the scalars and
+// matrices as pointers to interleaved (re, im) reals.
+template <typename R>
+void ComplexGemm(const char* name, CBLAS_LAYOUT layout, CBLAS_TRANSPOSE transa,
+                 CBLAS_TRANSPOSE transb, blasint m, blasint n, blasint k, const void* alpha,
+                 const void* a, blasint lda, const void* b, blasint ldb, const void* beta,
+                 void* c, blasint ldc) {
+  using F = void (*)(const char*, const char*, const blasint*, const blasint*, const blasint*,
+                     const R*, const R*, const blasint*, const R*, const blasint*, const R*,
+                     R*, const blasint*);
+  F gemm;
+  if constexpr (sizeof(R) == 8) gemm = zgemm_;
+  else gemm = cgemm_;
+  if (!ValidLayout(name, layout)) return;
+  const char ta = TransChar(transa), tb = TransChar(transb);
+  const R* al = static_cast<const R*>(alpha);
+  const R* be = static_cast<const R*>(beta);
+  const R* ar = static_cast<const R*>(a);
+  const R* br = static_cast<const R*>(b);
+  R* cr = static_cast<R*>(c);
+  if (layout == CblasColMajor) {
+    CblasScope scope(name, {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14});
+    gemm(&ta, &tb, &m, &n, &k, al, ar, &lda, br, &ldb, be, cr, &ldc);
+  } else {
+    CblasScope scope(name, {3, 2, 5, 4, 6, 7, 10, 11, 8, 9, 12, 13, 14});
+    gemm(&tb, &ta, &n, &m, &k, al, br, &ldb, ar, &lda, be, cr, &ldc);
+  }
+}
+
 template <typename T>
 void Trsm(const char* name, CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo,
           CBLAS_TRANSPOSE transa, CBLAS_DIAG diag, blasint m, blasint n, T alpha,
@@ -324,6 +353,26 @@ HWYBLAS_AMD_API void cblas_dgemm(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOS
                                  const hwyblas_int ldc) {
   hwy_amd::Gemm<double>("cblas_dgemm", layout, transa, transb, m, n, k, alpha, a, lda, b,
                         ldb, beta, c, ldc);
+}
+
+HWYBLAS_AMD_API void cblas_cgemm(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_TRANSPOSE transb, const hwyblas_int m,
+                                 const hwyblas_int n, const hwyblas_int k, const void* alpha,
+                                 const void* a, const hwyblas_int lda, const void* b,
+                                 const hwyblas_int ldb, const void* beta, void* c,
+                                 const hwyblas_int ldc) {
+  hwy_amd::ComplexGemm<float>("cblas_cgemm", layout, transa, transb, m, n, k, alpha, a, lda,
+                              b, ldb, beta, c, ldc);
+}
+
+HWYBLAS_AMD_API void cblas_zgemm(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_TRANSPOSE transb, const hwyblas_int m,
+                                 const hwyblas_int n, const hwyblas_int k, const void* alpha,
+                                 const void* a, const hwyblas_int lda, const void* b,
+                                 const hwyblas_int ldb, const void* beta, void* c,
+                                 const hwyblas_int ldc) {
+  hwy_amd::ComplexGemm<double>("cblas_zgemm", layout, transa, transb, m, n, k, alpha, a, lda,
+                               b, ldb, beta, c, ldc);
 }
 
 HWYBLAS_AMD_API void cblas_strsm(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,

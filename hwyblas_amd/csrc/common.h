@@ -86,14 +86,24 @@ void SetCblasContext(const CblasContext* context);
 // grow-only device staging buffers, reused across calls like hwyBLAS's
 // thread-local scratch slots (src/common.cc:46-64).
 
-// kSlotD: a dense copy of a structured operand (SYMM, TRMM).
-enum StagingSlot : int { kSlotA = 0, kSlotB = 1, kSlotC = 2, kSlotD = 3, kNumSlots = 4 };
+// kSlotD: a dense copy of a structured operand (SYMM, TRMM). kSlotPackA,
+// kSlotPackB, kSlotProd: the complex GEMM's real packs and real product.
+enum StagingSlot : int {
+  kSlotA = 0,
+  kSlotB = 1,
+  kSlotC = 2,
+  kSlotD = 3,
+  kSlotPackA = 4,
+  kSlotPackB = 5,
+  kSlotProd = 6,
+  kNumSlots = 7
+};
 
 struct ThreadContext {
   hipStream_t stream = nullptr;
   int device = -1;
-  void* buf[kNumSlots] = {nullptr, nullptr, nullptr, nullptr};
-  size_t cap[kNumSlots] = {0, 0, 0, 0};
+  void* buf[kNumSlots] = {};
+  size_t cap[kNumSlots] = {};
 };
 
 ThreadContext& Context();

@@ -637,6 +637,57 @@ HWYBLAS_AMD_API void hwyblas_amd_capture_begin(void* stream);
 HWYBLAS_AMD_API void* hwyblas_amd_capture_end(void* stream);
 HWYBLAS_AMD_API void hwyblas_amd_graph_launch(void* exec, void* stream);
 HWYBLAS_AMD_API void hwyblas_amd_graph_destroy(void* exec);
+/* ------------------------------------------------------------------------ */
+/* Complex GEMM (hwyBLAS src/gemm.cc:73-132, :160-176; signatures as
+ * include/hwyblas/hwyblas.h:576-587 and cblas.h:623-638). C := alpha op(A)
+ * op(B) + beta C, op = N, T or C (conjugate transpose); matrices of
+ * interleaved (re, im) reals with leading dimensions in complex elements,
+ * alpha and beta as pointers to (re, im). Info positions as for ?gemm_
+ * (1 transa, 2 transb, 3 m, 4 n, 5 k, 8 lda, 10 ldb, 13 ldc). Host or device
+ * operands; the Fortran and CBLAS forms synchronize. Workspace from the
+ * calling thread's device buffers: 2 (m + n) k + 4 m n reals. */
+HWYBLAS_AMD_API void cgemm_(const char* transa, const char* transb,
+                            const hwyblas_int* m, const hwyblas_int* n,
+                            const hwyblas_int* k, const float* alpha,
+                            const float* a, const hwyblas_int* lda, const float* b,
+                            const hwyblas_int* ldb, const float* beta, float* c,
+                            const hwyblas_int* ldc);
+HWYBLAS_AMD_API void zgemm_(const char* transa, const char* transb,
+                            const hwyblas_int* m, const hwyblas_int* n,
+                            const hwyblas_int* k, const double* alpha,
+                            const double* a, const hwyblas_int* lda,
+                            const double* b, const hwyblas_int* ldb,
+                            const double* beta, double* c, const hwyblas_int* ldc);
+/* Row-major swaps A and B and m and n (src/cblas.cc:517-537, :1607-1630). */
+HWYBLAS_AMD_API void cblas_cgemm(const CBLAS_LAYOUT layout,
+                                 const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_TRANSPOSE transb, const hwyblas_int m,
+                                 const hwyblas_int n, const hwyblas_int k,
+                                 const void* alpha, const void* a,
+                                 const hwyblas_int lda, const void* b,
+                                 const hwyblas_int ldb, const void* beta, void* c,
+                                 const hwyblas_int ldc);
+HWYBLAS_AMD_API void cblas_zgemm(const CBLAS_LAYOUT layout,
+                                 const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_TRANSPOSE transb, const hwyblas_int m,
+                                 const hwyblas_int n, const hwyblas_int k,
+                                 const void* alpha, const void* a,
+                                 const hwyblas_int lda, const void* b,
+                                 const hwyblas_int ldb, const void* beta, void* c,
+                                 const hwyblas_int ldc);
+/* Device-pointer forms (no hwyBLAS counterpart), as hwyblas_dgemm_dev with
+ * alpha and beta as host pointers to (re, im). Asynchronous on `stream`;
+ * calls that share the calling thread's workspace must be ordered (one
+ * stream, or a synchronization between them). */
+HWYBLAS_AMD_API int hwyblas_cgemm_dev(char transa, char transb, int m, int n, int k,
+                                      const float* alpha, const float* a, int lda,
+                                      const float* b, int ldb, const float* beta, float* c,
+                                      int ldc, void* stream);
+HWYBLAS_AMD_API int hwyblas_zgemm_dev(char transa, char transb, int m, int n, int k,
+                                      const double* alpha, const double* a, int lda,
+                                      const double* b, int ldb, const double* beta,
+                                      double* c, int ldc, void* stream);
+
 /* Library version string. */
 HWYBLAS_AMD_API const char* hwyblas_amd_version(void);
 

@@ -360,3 +360,57 @@ def test_cholesky_invalid_arguments(routine, change, info, dtype):
             got = H.posv(g["uplo"], g["n"], g["nrhs"], x, g["lda"], x, g["ldb"], dtype=dtype)
     assert got == -info
     assert cap.calls == [(("D" if dtype == np.float64 else "S") + routine.upper(), info)]
+
+
+ZDTYPES = [np.complex128, np.complex64]
+ZGEMM_CASES = GEMM_CASES + [(dict(ta="C", lda=1), 8), (dict(tb="C", ldb=2), 10)]
+
+
+@pytest.mark.parametrize("dtype", ZDTYPES)
+@pytest.mark.parametrize("change,info", ZGEMM_CASES)
+def test_zgemm_invalid_arguments(dtype, change, info):
+    """cgemm_/zgemm_ check as ?gemm_ (src/gemm.cc:84-106), 'C' accepted."""
+    g = _gemm_args(**change)
+    a = np.ones((8, 8), dtype=dtype, order="F")
+    c = np.full((8, 8), 5.0 + 1.0j, dtype=dtype, order="F")
+    c0 = c.copy()
+    with H.ErrorCapture() as cap:
+        H.zgemm(g["ta"], g["tb"], g["m"], g["n"], g["k"], 1.0, a, g["lda"], a, g["ldb"], 0.0, c,
+                g["ldc"])
+    name = "ZGEMM" if dtype == np.complex128 else "CGEMM"
+    assert cap.calls == [(name, info)]
+    assert np.array_equal(c, c0)
+
+
+@pytest.mark.parametrize("layout,change,pos", [
+    (H.ColMajor, dict(m=-1), 4), (H.RowMajor, dict(m=-1), 4), (H.RowMajor, dict(n=-1), 5),
+    (H.ColMajor, dict(lda=1), 9), (H.RowMajor, dict(ldb=1), 11), (H.RowMajor, dict(ldc=1), 14),
+    (99, dict(), 1)])
+def test_cblas_zgemm_positions(layout, change, pos):
+    """cblas_zgemm reports CBLAS positions; row-major swaps A/B and m/n
+    (src/cblas.cc:517-537)."""
+    g = dict(m=4, n=3, k=2, lda=4, ldb=4, ldc=4)
+    g.update(change)
+    a = np.ones((8, 8), dtype=np.complex128, order="F")
+    c = np.full((8, 8), 2.0, dtype=np.complex128, order="F")
+    c0 = c.copy()
+    with H.ErrorCapture() as cap:
+        H.cblas_zgemm(layout, H.NoTrans, H.NoTrans, g["m"], g["n"], g["k"], 1.0, a, g["lda"], a,
+                      g["ldb"], 0.0, c, g["ldc"])
+    assert cap.calls == [("cblas_zgemm", pos)]
+    assert np.array_equal(c, c0)
+
+
+def test_zgemm_quick_returns_need_no_device():
+    """m = 0, n = 0, or (k = 0 or alpha = 0) with beta = 1 return before any
+    device work; C is untouched."""
+    a = np.ones((4, 4), dtype=np.complex128, order="F")
+    c = np.full((4, 4), 3.0 - 2.0j, dtype=np.complex128, order="F")
+    c0 = c.copy()
+    with H.ErrorCapture() as cap:
+        H.zgemm("N", "N", 0, 4, 4, 1.0, a, 4, a, 4, 2.0, c, 4)
+        H.zgemm("N", "C", 4, 0, 4, 1.0, a, 4, a, 4, 2.0, c, 4)
+        H.zgemm("C", "N", 4, 4, 0, 1.0, a, 4, a, 4, 1.0, c, 4)
+        H.zgemm("T", "T", 4, 4, 4, 0.0, a, 4, a, 4, 1.0, c, 4)
+    assert cap.calls == []
+    assert np.array_equal(c, c0)
