@@ -615,7 +615,7 @@ def lu_leg(H, n, kappas, stream, nrhs=16, reps=2):
 
 
 def zgemm_leg(H, n=8192, reps=3):
-    """ZGEMM and CGEMM at n^3 on device operands (interleaved complex, each
+    """ZGEMM and CGEMM, then ZTRSM and CTRSM, at n on device operands (interleaved complex, each
     filled through its real 2n x n view, uniform [-1, 1]), alpha = 1 - 0.5i,
     beta = 0.5 + 0.25i; 8 n^3 real flops per product, against the FP64 (FP32)
     MFMA peak."""
@@ -643,6 +643,29 @@ def zgemm_leg(H, n=8192, reps=3):
             out[f"{name}_{ta}{tb}"] = {"n": n, "ms": round(t * 1e3, 2), "tflops": round(tf, 2),
                                        "frac_peak": round(tf / peak, 3)}
             log(f"{name} {ta}{tb} n={n}: {t * 1e3:.2f} ms, {tf:.1f} TFLOP/s")
+        # TRSM on A's triangles with the diagonal set to 2n (well conditioned),
+        # 4 n^3 real flops; B is solved in place rep after rep.
+        tname = name.replace("gemm", "trsm")
+        diag = np.array([2.0 * n, 0.0], dtype=rt)
+        isz = np.dtype(rt).itemsize
+        for i in range(n):
+            H.lib().hwyblas_amd_memcpy(A.ptr + 2 * isz * (i + i * n), diag.ctypes.data,
+                                       diag.nbytes, 1)
+        H.synchronize()
+        for side, uplo, tr in (("L", "L", "N"), ("R", "U", "C")):
+            times = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                info = H.ztrsm_dev(side, uplo, tr, "N", n, n, 1.0, A.ptr, n, C.ptr, n, dtype=ct)
+                H.synchronize()
+                times.append(time.perf_counter() - t0)
+                assert info == 0
+            t = min(times)
+            tf = 4.0 * n ** 3 / t / 1e12
+            out[f"{tname}_{side}{uplo}{tr}"] = {"n": n, "ms": round(t * 1e3, 2),
+                                               "tflops": round(tf, 2),
+                                               "frac_peak": round(tf / peak, 3)}
+            log(f"{tname} {side}{uplo}{tr} n={n}: {t * 1e3:.2f} ms, {tf:.1f} TFLOP/s")
         del A, B, C, arrs
     return out
 
@@ -839,11 +862,11 @@ def main():
                          "instead of the headline")
     ap.add_argument("--level3", action="store_true",
                     help="DSYMM/DSYRK/DSYR2K/DGEMMTR/DTRMM at n = 8192 beside DGEMM, "
-                         "DGEMV and DPOTRF at 16384, ZGEMM/CGEMM at 8192")
+                         "DGEMV and DPOTRF at 16384, ZGEMM/CGEMM and ZTRSM/CTRSM at 8192")
     ap.add_argument("--potrf", action="store_true",
                     help="only DPOTRF (lower) at n = 16384")
     ap.add_argument("--zgemm", action="store_true",
-                    help="only ZGEMM and CGEMM at n = 8192")
+                    help="only ZGEMM/CGEMM and ZTRSM/CTRSM at n = 8192")
     ap.add_argument("--lu-n", type=int, default=16384)
     ap.add_argument("--lu-kappas", default="2,6,10",
                     help="comma-separated log10 condition numbers for --lu")

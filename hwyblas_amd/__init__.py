@@ -69,6 +69,9 @@ def _declare(L):
         fn(f"cblas_{pre}gemm", None, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i])
         fn(f"hwyblas_{pre}gemm_dev", _i, [_c, _c, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i,
                                           _p])
+        fn(f"{pre}trsm_", None, [ctypes.c_char_p] * 4 + [_pi, _pi, _p, _p, _pi, _p, _pi])
+        fn(f"cblas_{pre}trsm", None, [_i, _i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i])
+        fn(f"hwyblas_{pre}trsm_dev", _i, [_c, _c, _c, _c, _i, _i, _p, _p, _i, _p, _i, _p])
     fn("hwyblas_dgemm_emulated_dev", _i, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i,
                                           _i, _p, _pi])
     fn("hwyblas_dgemm_emulated_bf16_dev", _i, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p,
@@ -321,6 +324,30 @@ def zgemm_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None,
     return getattr(lib(), f"hwyblas_{_zpre(dtype)}gemm_dev")(
         ta.encode(), tb.encode(), m, n, k, _ptr(al), _ptr(a), lda, _ptr(b), ldb, _ptr(be),
         _ptr(c), ldc, stream)
+
+
+def ztrsm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, dtype=None):
+    """Fortran ctrsm_/ztrsm_ (numpy complex arrays or DeviceArrays); B in place."""
+    dtype = dtype or b.dtype
+    al = _zscalar(alpha, dtype)
+    getattr(lib(), f"{_zpre(dtype)}trsm_")(
+        side.encode(), uplo.encode(), ta.encode(), diag.encode(), _ip(m), _ip(n), _ptr(al),
+        _ptr(a), _ip(lda), _ptr(b), _ip(ldb))
+
+
+def cblas_ztrsm(layout, side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, dtype=None):
+    dtype = dtype or b.dtype
+    al = _zscalar(alpha, dtype)
+    getattr(lib(), f"cblas_{_zpre(dtype)}trsm")(layout, side, uplo, ta, diag, m, n, _ptr(al),
+                                                _ptr(a), lda, _ptr(b), ldb)
+
+
+def ztrsm_dev(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, stream=None, dtype=None):
+    dtype = dtype or b.dtype
+    al = _zscalar(alpha, dtype)
+    return getattr(lib(), f"hwyblas_{_zpre(dtype)}trsm_dev")(
+        side.encode(), uplo.encode(), ta.encode(), diag.encode(), m, n, _ptr(al), _ptr(a), lda,
+        _ptr(b), ldb, stream)
 
 
 def gemm_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None, dtype=None):

@@ -144,6 +144,31 @@ void Trsm(const char* name, CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO upl
   Fortran<T>::trsm(&sc, &uc, &tc, &dc, &m, &n, &alpha, a, &lda, b, &ldb);
 }
 
+// Complex TRSM: the same mapping and positions as Trsm.
+template <typename R>
+void ComplexTrsm(const char* name, CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo,
+                 CBLAS_TRANSPOSE transa, CBLAS_DIAG diag, blasint m, blasint n,
+                 const void* alpha, const void* a, blasint lda, void* b, blasint ldb) {
+  using F = void (*)(const char*, const char*, const char*, const char*, const blasint*,
+                     const blasint*, const R*, const R*, const blasint*, R*, const blasint*);
+  F trsm;
+  if constexpr (sizeof(R) == 8) trsm = ztrsm_;
+  else trsm = ctrsm_;
+  if (!ValidLayout(name, layout)) return;
+  char sc = SideChar(side), uc = UploChar(uplo);
+  const char tc = TransChar(transa), dc = DiagChar(diag);
+  const bool row = layout == CblasRowMajor;
+  if (row) {
+    sc = FlipSide(sc);
+    uc = FlipUplo(uc);
+    std::swap(m, n);
+  }
+  CblasScope scope(name, row ? std::initializer_list<int>{2, 3, 4, 5, 7, 6, 8, 9, 10, 11, 12}
+                             : std::initializer_list<int>{2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12});
+  trsm(&sc, &uc, &tc, &dc, &m, &n, static_cast<const R*>(alpha), static_cast<const R*>(a), &lda,
+       static_cast<R*>(b), &ldb);
+}
+
 // GEMV. Row-major: op(A) of the row-major A is the other op of the
 // column-major A^T (n x m). Positions: layout 1, trans 2, m 3, n 4, alpha 5,
 // a 6, lda 7, x 8, incx 9, beta 10, y 11, incy 12.
@@ -373,6 +398,24 @@ HWYBLAS_AMD_API void cblas_zgemm(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOS
                                  const hwyblas_int ldc) {
   hwy_amd::ComplexGemm<double>("cblas_zgemm", layout, transa, transb, m, n, k, alpha, a, lda,
                                b, ldb, beta, c, ldc);
+}
+
+HWYBLAS_AMD_API void cblas_ctrsm(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,
+                                 const CBLAS_UPLO uplo, const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_DIAG diag, const hwyblas_int m,
+                                 const hwyblas_int n, const void* alpha, const void* a,
+                                 const hwyblas_int lda, void* b, const hwyblas_int ldb) {
+  hwy_amd::ComplexTrsm<float>("cblas_ctrsm", layout, side, uplo, transa, diag, m, n, alpha, a,
+                              lda, b, ldb);
+}
+
+HWYBLAS_AMD_API void cblas_ztrsm(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,
+                                 const CBLAS_UPLO uplo, const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_DIAG diag, const hwyblas_int m,
+                                 const hwyblas_int n, const void* alpha, const void* a,
+                                 const hwyblas_int lda, void* b, const hwyblas_int ldb) {
+  hwy_amd::ComplexTrsm<double>("cblas_ztrsm", layout, side, uplo, transa, diag, m, n, alpha, a,
+                               lda, b, ldb);
 }
 
 HWYBLAS_AMD_API void cblas_strsm(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,

@@ -1,7 +1,10 @@
-// Complex GEMM, C := alpha op(A) op(B) + beta C with op = N, T or C
-// (conjugate transpose): cgemm_, zgemm_ and their device-pointer forms
-// (hwyBLAS src/gemm.cc:73-132 for the checks, info positions and quick
-// returns, src/gemm.cc:160-176 for the entries).
+// Complex Level-3 on the GPU: GEMM (cgemm_, zgemm_) and TRSM (ctrsm_,
+// ztrsm_), with their device-pointer forms. op = N, T or C (conjugate
+// transpose).
+//
+// GEMM, C := alpha op(A) op(B) + beta C (hwyBLAS src/gemm.cc:73-132 for the
+// checks, info positions and quick returns, src/gemm.cc:160-176 for the
+// entries).
 //
 // Arithmetic (hwyBLAS src/gemm_kernel-inl.h:228-424): the four real sums
 // sum ar br, sum ai br, sum ar bi, sum ai bi are accumulated separately and
@@ -15,6 +18,15 @@
 // so that their product (2m x 2n) holds [ar br, ar bi; ai br, ai bi] in its
 // four quadrants, at the 8mnk flops of the complex product. A combining pass
 // reads the quadrants and C and writes C. Workspace: 2(m + n)k + 4mn reals.
+//
+// TRSM, op(A) X = alpha B or X op(A) = alpha B (hwyBLAS src/trsm.cc:87-180):
+// B := alpha B first (alpha == 0 leaves zeros and never reads A), then the
+// triangle is halved at multiples of 64 down to diagonal blocks of at most
+// 64; each off-diagonal update is the complex GEMM above with alpha = -1,
+// beta = 1, and each diagonal block is solved by one kernel: a wave per
+// right-hand side, lane e holding solve row e, the block's triangle in LDS,
+// x_i = (b_i - sum_{q<i} t(i, q) x_q) (1 / t(i, i)) with the inverse from a
+// complex division (src/trsm-inl.h:60-70, conjugated diagonal for op = C).
 #include <algorithm>
 
 #include "common.h"
@@ -292,6 +304,294 @@ int ZgemmDev(const char* name, char transa, char transb, int m, int n, int k, co
   return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// TRSM.
+
+constexpr int kZNB = 64;     // largest diagonal block
+constexpr int kZWaves = 4;   // right-hand sides per workgroup (one per wave)
+
+// 1 / z by Smith's scaling (no overflow for representable quotients).
+template <typename C2>
+__device__ __forceinline__ C2 CRecip(C2 z) {
+  C2 out;
+  if (fabs(z.x) >= fabs(z.y)) {
+    const auto r = z.y / z.x, den = z.x + z.y * r;
+    out.x = 1 / den;
+    out.y = -r / den;
+  } else {
+    const auto r = z.x / z.y, den = z.x * r + z.y;
+    out.x = r / den;
+    out.y = -1 / den;
+  }
+  return out;
+}
+
+__device__ __forceinline__ double LaneBroadcast(double v, int i) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(bits & 0xFFFFFFFFll), i);
+  const int hi = __builtin_amdgcn_readlane(int(bits >> 32), i);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ float LaneBroadcast(float v, int i) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
+}
+
+// The triangle op(A) as a strided view: element (i, j) = data[i rs + j cs],
+// conjugated when conj.
+template <typename R>
+struct ZTri {
+  const typename Vec2<R>::type* data;
+  int64_t rs, cs;
+  bool lower, unit, conj;
+};
+
+template <typename R>
+ZTri<R> ZSub(const ZTri<R>& t, int64_t i0, int64_t j0) {
+  ZTri<R> s = t;
+  s.data += i0 * t.rs + j0 * t.cs;
+  return s;
+}
+
+template <typename R>
+struct ZDiagDesc {
+  const typename Vec2<R>::type* a;
+  int64_t rs, cs;
+  typename Vec2<R>::type* b;
+  int64_t ldb;
+  int r, nrhs;
+  bool asc, unit, conj;
+};
+
+// One diagonal block (order r <= kZNB). The solve matrix is L = T (left) or
+// T^T (right), walked ascending for a lower L; a right-hand side is a column
+// of B (left) or a row (right).
+template <typename R, bool LEFT>
+__global__ void __launch_bounds__(64 * kZWaves) ZDiagKernel(ZDiagDesc<R> d) {
+  using C2 = typename Vec2<R>::type;
+  __shared__ C2 st[kZNB][kZNB + 1];  // st[e][q] = L(e, q), q < e, in solve order
+  __shared__ C2 sinv[kZNB];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = d.r;
+  auto phys = [&](int e) { return d.asc ? e : r - 1 - e; };
+  auto lval = [&](int e, int q) {
+    const int64_t i = LEFT ? phys(e) : phys(q), j = LEFT ? phys(q) : phys(e);
+    C2 v = d.a[i * d.rs + j * d.cs];
+    if (d.conj) v.y = -v.y;
+    return v;
+  };
+  for (int q = wave; q < kZNB; q += kZWaves) {
+    C2 v;
+    v.x = R(0);
+    v.y = R(0);
+    if (lane < r && q < lane) v = lval(lane, q);
+    st[lane][q] = v;
+  }
+  if (wave == 0) {
+    C2 inv;
+    inv.x = R(1);
+    inv.y = R(0);
+    if (lane < r && !d.unit) inv = CRecip(lval(lane, lane));
+    sinv[lane] = inv;
+  }
+  __syncthreads();
+  const int c = int(blockIdx.x) * kZWaves + wave;
+  if (c >= d.nrhs) return;
+  const int64_t e_phys = lane < r ? phys(lane) : 0;
+  C2* at = LEFT ? d.b + e_phys + int64_t(c) * d.ldb : d.b + c + e_phys * d.ldb;
+  C2 x;
+  x.x = R(0);
+  x.y = R(0);
+  if (lane < r) x = *at;
+  const C2 myinv = sinv[lane];
+  for (int i = 0; i < r; ++i) {
+    if (lane == i) {
+      const R xr = x.x * myinv.x - x.y * myinv.y;
+      const R xi = x.y * myinv.x + x.x * myinv.y;
+      x.x = xr;
+      x.y = xi;
+    }
+    const R br = LaneBroadcast(x.x, i), bi = LaneBroadcast(x.y, i);
+    if (lane > i) {
+      const C2 t = st[lane][i];
+      x.x = __builtin_fma(-t.x, br, __builtin_fma(t.y, bi, x.x));
+      x.y = __builtin_fma(-t.x, bi, __builtin_fma(-t.y, br, x.y));
+    }
+  }
+  if (lane < r) *at = x;
+}
+
+template <typename R>
+void ZDiagSolve(bool left, const ZTri<R>& t, int r, typename Vec2<R>::type* b, int64_t ldb,
+                int64_t other, hipStream_t s) {
+  ZDiagDesc<R> d;
+  d.a = t.data;
+  d.rs = t.rs;
+  d.cs = t.cs;
+  d.b = b;
+  d.ldb = ldb;
+  d.r = r;
+  d.nrhs = int(other);
+  // L = T (left) or T^T (right); a lower L solves ascending.
+  d.asc = left ? t.lower : !t.lower;
+  d.unit = t.unit;
+  d.conj = t.conj;
+  const dim3 grid(unsigned((other + kZWaves - 1) / kZWaves));
+  if (left)
+    hipLaunchKernelGGL((ZDiagKernel<R, true>), grid, dim3(64 * kZWaves), 0, s, d);
+  else
+    hipLaunchKernelGGL((ZDiagKernel<R, false>), grid, dim3(64 * kZWaves), 0, s, d);
+  HWY_HIP_CHECK(hipGetLastError());
+}
+
+// The GEMM operand and op of a strided view (one of rs, cs is 1).
+template <typename R>
+void ViewOperand(const ZTri<R>& t, ZOp* op, int64_t* ld) {
+  if (t.rs == 1) {
+    *op = ZOp::kN;
+    *ld = t.cs;
+  } else {
+    *op = t.conj ? ZOp::kC : ZOp::kT;
+    *ld = t.rs;
+  }
+}
+
+template <typename R>
+void ZRecurse(bool left, const ZTri<R>& t, int64_t r, int64_t other,
+              typename Vec2<R>::type* b, int64_t ldb, hipStream_t s) {
+  if (r <= kZNB) {
+    ZDiagSolve<R>(left, t, int(r), b, ldb, other, s);
+    return;
+  }
+  const R minus_one[2] = {R(-1), R(0)}, one[2] = {R(1), R(0)};
+  const int64_t r1 = std::max<int64_t>(kZNB, (r / 2) / kZNB * kZNB);
+  const int64_t r2 = r - r1;
+  const ZTri<R> t22 = ZSub(t, r1, r1);
+  const ZTri<R> tof = t.lower ? ZSub(t, r1, 0) : ZSub(t, 0, r1);
+  ZOp op;
+  int64_t ldt;
+  ViewOperand(tof, &op, &ldt);
+  if (left) {
+    auto* b1 = b;
+    auto* b2 = b + r1;
+    if (t.lower) {
+      ZRecurse<R>(left, t, r1, other, b1, ldb, s);
+      ZgemmDevice<R>(op, ZOp::kN, r2, other, r1, minus_one, tof.data, ldt, b1, ldb, one, b2, ldb, s);
+      ZRecurse<R>(left, t22, r2, other, b2, ldb, s);
+    } else {
+      ZRecurse<R>(left, t22, r2, other, b2, ldb, s);
+      ZgemmDevice<R>(op, ZOp::kN, r1, other, r2, minus_one, tof.data, ldt, b2, ldb, one, b1, ldb, s);
+      ZRecurse<R>(left, t, r1, other, b1, ldb, s);
+    }
+  } else {
+    auto* b1 = b;
+    auto* b2 = b + r1 * ldb;
+    if (t.lower) {
+      ZRecurse<R>(left, t22, r2, other, b2, ldb, s);
+      ZgemmDevice<R>(ZOp::kN, op, other, r1, r2, minus_one, b2, ldb, tof.data, ldt, one, b1, ldb, s);
+      ZRecurse<R>(left, t, r1, other, b1, ldb, s);
+    } else {
+      ZRecurse<R>(left, t, r1, other, b1, ldb, s);
+      ZgemmDevice<R>(ZOp::kN, op, other, r2, r1, minus_one, b1, ldb, tof.data, ldt, one, b2, ldb, s);
+      ZRecurse<R>(left, t22, r2, other, b2, ldb, s);
+    }
+  }
+}
+
+// Info of the first invalid TRSM argument, or 0 (src/trsm.cc:145-168).
+blasint ZtrsmInfo(char side, char uplo, char transa, char diag, blasint m, blasint n,
+                  blasint lda, blasint ldb, bool* left, bool* lower, ZOp* op, bool* unit) {
+  const bool vs = DecodeSide(side, left);
+  const bool vu = DecodeUplo(uplo, lower);
+  const bool vt = DecodeZOp(transa, op);
+  const bool vd = DecodeDiag(diag, unit);
+  const blasint ka = *left ? m : n;
+  if (!vs) return 1;
+  if (!vu) return 2;
+  if (!vt) return 3;
+  if (!vd) return 4;
+  if (m < 0) return 5;
+  if (n < 0) return 6;
+  if (lda < std::max(1, ka)) return 9;
+  if (ldb < std::max(1, m)) return 11;
+  return 0;
+}
+
+// B := alpha B, then the solve (m, n > 0; device operands).
+template <typename R>
+void ZtrsmDevice(bool left, bool lower, ZOp op, bool unit, int64_t m, int64_t n, const R* alpha,
+                 const typename Vec2<R>::type* a, int64_t lda, typename Vec2<R>::type* b,
+                 int64_t ldb, hipStream_t s) {
+  if (!IsOneC(alpha)) ZScale<R>(m, n, alpha, b, ldb, s);
+  if (IsZeroC(alpha)) return;
+  ZTri<R> t;
+  const bool trans = op != ZOp::kN;
+  t.data = a;
+  t.rs = trans ? lda : 1;
+  t.cs = trans ? 1 : lda;
+  t.lower = trans ? !lower : lower;
+  t.unit = unit;
+  t.conj = op == ZOp::kC;
+  ZRecurse<R>(left, t, left ? m : n, left ? n : m, b, ldb, s);
+}
+
+template <typename R>
+void ZtrsmHost(const char* name, const char* sidep, const char* uplop, const char* transp,
+               const char* diagp, const blasint* mp, const blasint* np, const R* alpha,
+               const R* a, const blasint* ldap, R* b, const blasint* ldbp) {
+  using C2 = typename Vec2<R>::type;
+  const blasint m = *mp, n = *np, lda = *ldap, ldb = *ldbp;
+  bool left, lower, unit;
+  ZOp op;
+  const blasint info =
+      ZtrsmInfo(*sidep, *uplop, *transp, *diagp, m, n, lda, ldb, &left, &lower, &op, &unit);
+  if (info != 0) {
+    ReportError(name, info);
+    return;
+  }
+  if (m == 0 || n == 0) return;
+  ThreadContext& ctx = Context();
+  hipStream_t s = ctx.stream;
+  const bool b_dev = IsDevicePointer(b);
+  C2* db = reinterpret_cast<C2*>(b);
+  int64_t ldb_d = ldb;
+  if (!b_dev) {
+    db = static_cast<C2*>(Staging(kSlotB, size_t(m) * n * sizeof(C2)));
+    ldb_d = m;
+    if (!IsZeroC(alpha)) CopyIn2D(db, reinterpret_cast<const C2*>(b), m, n, ldb, s);
+  }
+  const C2* da = reinterpret_cast<const C2*>(a);
+  int64_t lda_d = lda;
+  if (!IsZeroC(alpha) && !IsDevicePointer(a)) {
+    const int64_t ka = left ? m : n;
+    C2* buf = static_cast<C2*>(Staging(kSlotA, size_t(ka) * ka * sizeof(C2)));
+    CopyIn2D(buf, da, ka, ka, lda, s);
+    da = buf;
+    lda_d = ka;
+  }
+  ZtrsmDevice<R>(left, lower, op, unit, m, n, alpha, da, lda_d, db, ldb_d, s);
+  if (!b_dev) CopyOut2D(reinterpret_cast<C2*>(b), ldb, db, m, n, s);
+  HWY_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <typename R>
+int ZtrsmDev(const char* name, char side, char uplo, char transa, char diag, int m, int n,
+             const R* alpha, const R* a, int lda, R* b, int ldb, hipStream_t s) {
+  bool left, lower, unit;
+  ZOp op;
+  const blasint info =
+      ZtrsmInfo(side, uplo, transa, diag, m, n, lda, ldb, &left, &lower, &op, &unit);
+  if (info != 0) {
+    ReportError(name, info);
+    return info;
+  }
+  if (m == 0 || n == 0) return 0;
+  using C2 = typename Vec2<R>::type;
+  ZtrsmDevice<R>(left, lower, op, unit, m, n, alpha, reinterpret_cast<const C2*>(a), lda,
+                 reinterpret_cast<C2*>(b), ldb, s);
+  return 0;
+}
+
 }  // namespace
 }  // namespace hwy_amd
 
@@ -331,6 +631,34 @@ HWYBLAS_AMD_API int hwyblas_zgemm_dev(char transa, char transb, int m, int n, in
                                       int ldc, void* stream) {
   return hwy_amd::ZgemmDev<double>("ZGEMM", transa, transb, m, n, k, alpha, a, lda, b, ldb,
                                    beta, c, ldc, static_cast<hipStream_t>(stream));
+}
+
+HWYBLAS_AMD_API void ctrsm_(const char* side, const char* uplo, const char* transa,
+                            const char* diag, const blasint* m, const blasint* n,
+                            const float* alpha, const float* a, const blasint* lda, float* b,
+                            const blasint* ldb) {
+  hwy_amd::ZtrsmHost<float>("CTRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+}
+
+HWYBLAS_AMD_API void ztrsm_(const char* side, const char* uplo, const char* transa,
+                            const char* diag, const blasint* m, const blasint* n,
+                            const double* alpha, const double* a, const blasint* lda, double* b,
+                            const blasint* ldb) {
+  hwy_amd::ZtrsmHost<double>("ZTRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+}
+
+HWYBLAS_AMD_API int hwyblas_ctrsm_dev(char side, char uplo, char transa, char diag, int m, int n,
+                                      const float* alpha, const float* a, int lda, float* b,
+                                      int ldb, void* stream) {
+  return hwy_amd::ZtrsmDev<float>("CTRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb,
+                                  static_cast<hipStream_t>(stream));
+}
+
+HWYBLAS_AMD_API int hwyblas_ztrsm_dev(char side, char uplo, char transa, char diag, int m, int n,
+                                      const double* alpha, const double* a, int lda, double* b,
+                                      int ldb, void* stream) {
+  return hwy_amd::ZtrsmDev<double>("ZTRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b,
+                                   ldb, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -688,6 +688,42 @@ HWYBLAS_AMD_API int hwyblas_zgemm_dev(char transa, char transb, int m, int n, in
                                       const double* b, int ldb, const double* beta,
                                       double* c, int ldc, void* stream);
 
+/* Complex TRSM (hwyBLAS src/trsm.cc:87-180; signatures as
+ * include/hwyblas/hwyblas.h:797-806 and cblas.h:872-885): op(A) X = alpha B
+ * (side L) or X op(A) = alpha B (side R), op = N, T or C, interleaved
+ * (re, im) storage, alpha as a pointer to (re, im). Info positions as for
+ * ?trsm_ (1 side, 2 uplo, 3 transa, 4 diag, 5 m, 6 n, 9 lda, 11 ldb);
+ * alpha == 0 sets B to zero without reading A. Host or device operands; the
+ * Fortran and CBLAS forms synchronize. Row-major CBLAS flips side and uplo
+ * and swaps m and n. */
+HWYBLAS_AMD_API void ctrsm_(const char* side, const char* uplo, const char* transa,
+                            const char* diag, const hwyblas_int* m, const hwyblas_int* n,
+                            const float* alpha, const float* a, const hwyblas_int* lda,
+                            float* b, const hwyblas_int* ldb);
+HWYBLAS_AMD_API void ztrsm_(const char* side, const char* uplo, const char* transa,
+                            const char* diag, const hwyblas_int* m, const hwyblas_int* n,
+                            const double* alpha, const double* a, const hwyblas_int* lda,
+                            double* b, const hwyblas_int* ldb);
+HWYBLAS_AMD_API void cblas_ctrsm(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,
+                                 const CBLAS_UPLO uplo, const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_DIAG diag, const hwyblas_int m,
+                                 const hwyblas_int n, const void* alpha, const void* a,
+                                 const hwyblas_int lda, void* b, const hwyblas_int ldb);
+HWYBLAS_AMD_API void cblas_ztrsm(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,
+                                 const CBLAS_UPLO uplo, const CBLAS_TRANSPOSE transa,
+                                 const CBLAS_DIAG diag, const hwyblas_int m,
+                                 const hwyblas_int n, const void* alpha, const void* a,
+                                 const hwyblas_int lda, void* b, const hwyblas_int ldb);
+/* Device-pointer forms (no hwyBLAS counterpart), as hwyblas_dtrsm_dev with
+ * alpha as a host pointer to (re, im); the same workspace rule as
+ * hwyblas_zgemm_dev. */
+HWYBLAS_AMD_API int hwyblas_ctrsm_dev(char side, char uplo, char transa, char diag, int m, int n,
+                                      const float* alpha, const float* a, int lda, float* b,
+                                      int ldb, void* stream);
+HWYBLAS_AMD_API int hwyblas_ztrsm_dev(char side, char uplo, char transa, char diag, int m, int n,
+                                      const double* alpha, const double* a, int lda, double* b,
+                                      int ldb, void* stream);
+
 /* Library version string. */
 HWYBLAS_AMD_API const char* hwyblas_amd_version(void);
 

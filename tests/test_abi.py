@@ -414,3 +414,37 @@ def test_zgemm_quick_returns_need_no_device():
         H.zgemm("T", "T", 4, 4, 4, 0.0, a, 4, a, 4, 1.0, c, 4)
     assert cap.calls == []
     assert np.array_equal(c, c0)
+
+
+ZTRSM_CASES = [  # (src/trsm.cc:151-168), 'C' accepted
+    (dict(side="X"), 1), (dict(uplo="X"), 2), (dict(ta="X"), 3), (dict(diag="X"), 4),
+    (dict(m=-1), 5), (dict(n=-1), 6), (dict(lda=3), 9), (dict(side="R", lda=2), 9),
+    (dict(ldb=3), 11), (dict(ta="C", lda=3), 9),
+]
+
+
+@pytest.mark.parametrize("dtype", ZDTYPES)
+@pytest.mark.parametrize("change,info", ZTRSM_CASES)
+def test_ztrsm_invalid_arguments(dtype, change, info):
+    t = dict(side="L", uplo="L", ta="N", diag="N", m=4, n=3, lda=4, ldb=4)
+    t.update(change)
+    a = np.ones((8, 8), dtype=dtype, order="F")
+    b = np.full((8, 8), 2.0 - 1.0j, dtype=dtype, order="F")
+    b0 = b.copy()
+    with H.ErrorCapture() as cap:
+        H.ztrsm(t["side"], t["uplo"], t["ta"], t["diag"], t["m"], t["n"], 1.0, a, t["lda"], b,
+                t["ldb"])
+    name = "ZTRSM" if dtype == np.complex128 else "CTRSM"
+    assert cap.calls == [(name, info)]
+    assert np.array_equal(b, b0)
+
+
+def test_cblas_ztrsm_row_major_positions():
+    """Row-major cblas_ztrsm swaps m and n: a bad ldb is CBLAS position 12,
+    m < 0 position 6."""
+    a = np.ones((8, 8), dtype=np.complex128, order="F")
+    b = np.full((8, 8), 2.0, dtype=np.complex128, order="F")
+    with H.ErrorCapture() as cap:
+        H.cblas_ztrsm(H.RowMajor, H.Left, H.Lower, H.NoTrans, H.NonUnit, 4, 3, 1.0, a, 4, b, 2)
+        H.cblas_ztrsm(H.RowMajor, H.Left, H.Lower, H.NoTrans, H.NonUnit, -1, 3, 1.0, a, 4, b, 3)
+    assert cap.calls == [("cblas_ztrsm", 12), ("cblas_ztrsm", 6)]
