@@ -63,6 +63,13 @@ def _declare(L):
         fn(f"hwyblas_{pre}trsm_dev", _i, [_c, _c, _c, _c, _i, _i, s, _p, _i, _p, _i, _p])
         fn(f"hwyblas_{pre}gemm_dev_cfg", _i, [_i, _c, _c, _i, _i, _i, s, _p, _i, _p, _i, s, _p,
                                              _i, _p])
+    fn("LAPACKE_set_nancheck", None, [_i])
+    fn("LAPACKE_get_nancheck", _i, [])
+    for pre in ("d", "s"):
+        for w in ("", "_work"):
+            fn(f"LAPACKE_{pre}potrf{w}", _i, [_i, ctypes.c_char, _i, _p, _i])
+            fn(f"LAPACKE_{pre}potrs{w}", _i, [_i, ctypes.c_char, _i, _i, _p, _i, _p, _i])
+            fn(f"LAPACKE_{pre}posv{w}", _i, [_i, ctypes.c_char, _i, _i, _p, _i, _p, _i])
     for pre in ("z", "c"):
         fn(f"{pre}gemm_", None, [ctypes.c_char_p, ctypes.c_char_p, _pi, _pi, _pi, _p, _p, _pi,
                                  _p, _pi, _p, _p, _pi])
@@ -348,6 +355,14 @@ def ztrsm_dev(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, stream=None, dt
     return getattr(lib(), f"hwyblas_{_zpre(dtype)}trsm_dev")(
         side.encode(), uplo.encode(), ta.encode(), diag.encode(), m, n, _ptr(al), _ptr(a), lda,
         _ptr(b), ldb, stream)
+
+
+def lapacke(routine, layout, uplo, *args, dtype=None, work=False):
+    """LAPACKE_{s,d}{potrf,potrs,posv}[_work] on host arrays; returns info.
+    args after uplo as in C (n, [nrhs,] a, lda, [b, ldb])."""
+    dtype = np.dtype(dtype or next(x.dtype for x in args if isinstance(x, np.ndarray)))
+    f = getattr(lib(), f"LAPACKE_{_pre(dtype)}{routine}{'_work' if work else ''}")
+    return f(layout, uplo.encode(), *[_ptr(x) if isinstance(x, np.ndarray) else x for x in args])
 
 
 def gemm_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None, dtype=None):

@@ -724,6 +724,53 @@ HWYBLAS_AMD_API int hwyblas_ztrsm_dev(char side, char uplo, char transa, char di
                                       const double* alpha, const double* a, int lda, double* b,
                                       int ldb, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* LAPACKE for the Cholesky routines (hwyBLAS src/lapack/lapacke.cc:112-133,
+ * 2065-2092, src/lapack/lapacke_symmetric.cc:22-70; signatures as
+ * include/hwyblas/lapacke.h:100-106, 620-740). matrix_layout 101 (row-major)
+ * or 102 (column-major); an invalid layout returns -1. Unless the NaN check is
+ * off (LAPACKE_set_nancheck(0), or LAPACKE_NANCHECK=0 in the environment), a
+ * NaN in A's referenced triangle or in B returns minus that argument's
+ * position. Otherwise the return is the routine's info, one lower for an
+ * invalid argument. Row-major operands are copied to column-major
+ * temporaries: a short leading dimension returns minus its position in the
+ * _work form. Host operands. */
+HWYBLAS_AMD_API void LAPACKE_xerbla(const char* name, hwyblas_int info);
+HWYBLAS_AMD_API void LAPACKE_set_nancheck(int flag);
+HWYBLAS_AMD_API int LAPACKE_get_nancheck(void);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_spotrf(int matrix_layout, char uplo, hwyblas_int n, float* a,
+                                           hwyblas_int lda);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dpotrf(int matrix_layout, char uplo, hwyblas_int n, double* a,
+                                           hwyblas_int lda);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_spotrf_work(int matrix_layout, char uplo, hwyblas_int n,
+                                                float* a, hwyblas_int lda);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dpotrf_work(int matrix_layout, char uplo, hwyblas_int n,
+                                                double* a, hwyblas_int lda);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_spotrs(int matrix_layout, char uplo, hwyblas_int n,
+                                           hwyblas_int nrhs, const float* a, hwyblas_int lda,
+                                           float* b, hwyblas_int ldb);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dpotrs(int matrix_layout, char uplo, hwyblas_int n,
+                                           hwyblas_int nrhs, const double* a, hwyblas_int lda,
+                                           double* b, hwyblas_int ldb);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_spotrs_work(int matrix_layout, char uplo, hwyblas_int n,
+                                                hwyblas_int nrhs, const float* a,
+                                                hwyblas_int lda, float* b, hwyblas_int ldb);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dpotrs_work(int matrix_layout, char uplo, hwyblas_int n,
+                                                hwyblas_int nrhs, const double* a,
+                                                hwyblas_int lda, double* b, hwyblas_int ldb);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_sposv(int matrix_layout, char uplo, hwyblas_int n,
+                                          hwyblas_int nrhs, float* a, hwyblas_int lda, float* b,
+                                          hwyblas_int ldb);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dposv(int matrix_layout, char uplo, hwyblas_int n,
+                                          hwyblas_int nrhs, double* a, hwyblas_int lda, double* b,
+                                          hwyblas_int ldb);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_sposv_work(int matrix_layout, char uplo, hwyblas_int n,
+                                               hwyblas_int nrhs, float* a, hwyblas_int lda,
+                                               float* b, hwyblas_int ldb);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dposv_work(int matrix_layout, char uplo, hwyblas_int n,
+                                               hwyblas_int nrhs, double* a, hwyblas_int lda,
+                                               double* b, hwyblas_int ldb);
+
 /* Library version string. */
 HWYBLAS_AMD_API const char* hwyblas_amd_version(void);
 

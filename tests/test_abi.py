@@ -448,3 +448,43 @@ def test_cblas_ztrsm_row_major_positions():
         H.cblas_ztrsm(H.RowMajor, H.Left, H.Lower, H.NoTrans, H.NonUnit, 4, 3, 1.0, a, 4, b, 2)
         H.cblas_ztrsm(H.RowMajor, H.Left, H.Lower, H.NoTrans, H.NonUnit, -1, 3, 1.0, a, 4, b, 3)
     assert cap.calls == [("cblas_ztrsm", 12), ("cblas_ztrsm", 6)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lapacke_checks_before_the_routine(dtype, capfd):
+    """LAPACKE layout (-1), NaN (minus the argument's position) and row-major
+    leading-dimension checks (src/lapack/lapacke.cc:112-133,
+    lapacke_symmetric.cc:22-70, lapacke_util.h:420-445) return before any
+    device work; NaN in the unreferenced triangle is not looked at by the
+    check."""
+    p = "d" if dtype == np.float64 else "s"
+    H.lib().LAPACKE_set_nancheck(1)
+    a = np.eye(4, dtype=dtype, order="F")
+    b = np.ones((4, 2), dtype=dtype, order="F")
+    assert H.lapacke("potrf", 99, "L", 4, a, 4) == -1
+    assert H.lapacke("posv", 7, "U", 4, 2, a, 4, b, 4) == -1
+    out = capfd.readouterr().out
+    assert f"Wrong parameter 1 in LAPACKE_{p}potrf" in out
+    assert f"Wrong parameter 1 in LAPACKE_{p}posv" in out
+    an = a.copy(order="F")
+    an[3, 1] = np.nan  # lower triangle, column-major
+    assert H.lapacke("potrf", H.ColMajor, "L", 4, an, 4) == -4
+    assert H.lapacke("potrs", H.ColMajor, "L", 4, 2, an, 4, b, 4) == -5
+    bn = b.copy(order="F")
+    bn[2, 1] = np.nan
+    assert H.lapacke("posv", H.ColMajor, "U", 4, 2, a, 4, bn, 4) == -7
+    assert H.lapacke("potrs", H.RowMajor, "L", 4, 2, a, 4, np.ascontiguousarray(bn), 2) == -7
+    # Row-major _work: a leading dimension shorter than the columns.
+    assert H.lapacke("potrf", H.RowMajor, "L", 4, a, 3, work=True) == -5
+    assert H.lapacke("potrs", H.RowMajor, "L", 4, 2, a, 3, b, 2, work=True) == -6
+    assert H.lapacke("posv", H.RowMajor, "L", 4, 2, a, 4, b, 1, work=True) == -8
+    out = capfd.readouterr().out
+    assert f"Wrong parameter 5 in LAPACKE_{p}potrf_work" in out
+    assert f"Wrong parameter 8 in LAPACKE_{p}posv_work" in out
+
+
+def test_lapacke_nancheck_switch():
+    H.lib().LAPACKE_set_nancheck(0)
+    assert H.lib().LAPACKE_get_nancheck() == 0
+    H.lib().LAPACKE_set_nancheck(5)
+    assert H.lib().LAPACKE_get_nancheck() == 1
