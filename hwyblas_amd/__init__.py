@@ -70,6 +70,13 @@ def _declare(L):
             fn(f"LAPACKE_{pre}potrf{w}", _i, [_i, ctypes.c_char, _i, _p, _i])
             fn(f"LAPACKE_{pre}potrs{w}", _i, [_i, ctypes.c_char, _i, _i, _p, _i, _p, _i])
             fn(f"LAPACKE_{pre}posv{w}", _i, [_i, ctypes.c_char, _i, _i, _p, _i, _p, _i])
+    fn("dgetrf_", None, [_pi, _pi, _p, _pi, _p, _pi])
+    fn("dgetrs_", None, [ctypes.c_char_p, _pi, _pi, _p, _pi, _p, _p, _pi, _pi])
+    fn("dgesv_", None, [_pi, _pi, _p, _pi, _p, _p, _pi, _pi])
+    for w in ("", "_work"):
+        fn(f"LAPACKE_dgetrf{w}", _i, [_i, _i, _i, _p, _i, _p])
+        fn(f"LAPACKE_dgetrs{w}", _i, [_i, ctypes.c_char, _i, _i, _p, _i, _p, _p, _i])
+        fn(f"LAPACKE_dgesv{w}", _i, [_i, _i, _i, _p, _i, _p, _p, _i])
     for pre in ("z", "c"):
         fn(f"{pre}gemm_", None, [ctypes.c_char_p, ctypes.c_char_p, _pi, _pi, _pi, _p, _p, _pi,
                                  _p, _pi, _p, _p, _pi])
@@ -355,6 +362,36 @@ def ztrsm_dev(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, stream=None, dt
     return getattr(lib(), f"hwyblas_{_zpre(dtype)}trsm_dev")(
         side.encode(), uplo.encode(), ta.encode(), diag.encode(), m, n, _ptr(al), _ptr(a), lda,
         _ptr(b), ldb, stream)
+
+
+def getrf(m, n, a, lda, ipiv):
+    """Fortran dgetrf_ (host arrays or DeviceArrays); returns info."""
+    info = ctypes.c_int(0)
+    lib().dgetrf_(_ip(m), _ip(n), _ptr(a), _ip(lda), _ptr(ipiv), ctypes.byref(info))
+    return info.value
+
+
+def getrs(trans, n, nrhs, a, lda, ipiv, b, ldb):
+    info = ctypes.c_int(0)
+    lib().dgetrs_(trans.encode(), _ip(n), _ip(nrhs), _ptr(a), _ip(lda), _ptr(ipiv), _ptr(b),
+                  _ip(ldb), ctypes.byref(info))
+    return info.value
+
+
+def gesv(n, nrhs, a, lda, ipiv, b, ldb):
+    info = ctypes.c_int(0)
+    lib().dgesv_(_ip(n), _ip(nrhs), _ptr(a), _ip(lda), _ptr(ipiv), _ptr(b), _ip(ldb),
+                 ctypes.byref(info))
+    return info.value
+
+
+def lapacke_lu(routine, layout, *args, work=False):
+    """LAPACKE_d{getrf,getrs,gesv}[_work] on host arrays; returns info. Args
+    after the layout as in C (chars as str)."""
+    f = getattr(lib(), f"LAPACKE_d{routine}{'_work' if work else ''}")
+    conv = [_ptr(x) if isinstance(x, np.ndarray) else (x.encode() if isinstance(x, str) else x)
+            for x in args]
+    return f(layout, *conv)
 
 
 def lapacke(routine, layout, uplo, *args, dtype=None, work=False):

@@ -1,6 +1,7 @@
-// LAPACKE (C interface) for the Cholesky routines: LAPACKE_{s,d}potrf,
-// potrs, posv and their _work forms, with LAPACKE_xerbla and the NaN-check
-// switch (hwyBLAS src/lapack/lapacke.cc:112-133, 2065-2092;
+// LAPACKE (C interface) for the Cholesky routines (LAPACKE_{s,d}potrf,
+// potrs, posv and their _work forms) and the LU solver (LAPACKE_dgetrf,
+// dgetrs, dgesv and their _work forms), with LAPACKE_xerbla and the
+// NaN-check switch (hwyBLAS src/lapack/lapacke.cc:112-133, 2065-2092;
 // src/lapack/lapacke_symmetric.cc:22-70; src/lapack/lapacke_util.h:89-119,
 // 395-480).
 //
@@ -248,10 +249,111 @@ blasint Posv(int layout, char uplo, blasint n, blasint nrhs, T* a, blasint lda, 
   return PosvWork(layout, uplo, n, nrhs, a, lda, b, ldb);
 }
 
+// LU (double only, as the Fortran entries): hwyBLAS
+// src/lapack/lapacke_linear.cc:14-75, lapacke.cc:40-58.
+blasint GetrfWork(int layout, blasint m, blasint n, double* a, blasint lda, blasint* ipiv) {
+  Arg<double> args[1] = {{a, m, n, lda, -5, false, true}};
+  return CallWork<double, 1>(layout, "LAPACKE_dgetrf_work", 'L', args,
+                             [&](double** p, blasint* ld) {
+                               blasint info = 0;
+                               dgetrf_(&m, &n, p[0], &ld[0], ipiv, &info);
+                               return info;
+                             });
+}
+
+blasint Getrf(int layout, blasint m, blasint n, double* a, blasint lda, blasint* ipiv) {
+  if (!ValidLayout(layout)) {
+    Xerbla("LAPACKE_dgetrf", -1);
+    return -1;
+  }
+  if (NanCheck() && GeneralHasNan(layout, m, n, a, lda)) return -4;
+  return GetrfWork(layout, m, n, a, lda, ipiv);
+}
+
+blasint GetrsWork(int layout, char trans, blasint n, blasint nrhs, const double* a, blasint lda,
+                  const blasint* ipiv, double* b, blasint ldb) {
+  Arg<double> args[2] = {{const_cast<double*>(a), n, n, lda, -6, false, false},
+                         {b, n, nrhs, ldb, -9, false, true}};
+  return CallWork<double, 2>(layout, "LAPACKE_dgetrs_work", 'L', args,
+                             [&](double** p, blasint* ld) {
+                               blasint info = 0;
+                               dgetrs_(&trans, &n, &nrhs, p[0], &ld[0], ipiv, p[1], &ld[1],
+                                       &info);
+                               return info;
+                             });
+}
+
+blasint Getrs(int layout, char trans, blasint n, blasint nrhs, const double* a, blasint lda,
+              const blasint* ipiv, double* b, blasint ldb) {
+  if (!ValidLayout(layout)) {
+    Xerbla("LAPACKE_dgetrs", -1);
+    return -1;
+  }
+  if (NanCheck()) {
+    if (GeneralHasNan(layout, n, n, a, lda)) return -5;
+    if (GeneralHasNan(layout, n, nrhs, b, ldb)) return -8;
+  }
+  return GetrsWork(layout, trans, n, nrhs, a, lda, ipiv, b, ldb);
+}
+
+blasint GesvWork(int layout, blasint n, blasint nrhs, double* a, blasint lda, blasint* ipiv,
+                 double* b, blasint ldb) {
+  Arg<double> args[2] = {{a, n, n, lda, -5, false, true}, {b, n, nrhs, ldb, -8, false, true}};
+  return CallWork<double, 2>(layout, "LAPACKE_dgesv_work", 'L', args,
+                             [&](double** p, blasint* ld) {
+                               blasint info = 0;
+                               dgesv_(&n, &nrhs, p[0], &ld[0], ipiv, p[1], &ld[1], &info);
+                               return info;
+                             });
+}
+
+blasint Gesv(int layout, blasint n, blasint nrhs, double* a, blasint lda, blasint* ipiv,
+             double* b, blasint ldb) {
+  if (!ValidLayout(layout)) {
+    Xerbla("LAPACKE_dgesv", -1);
+    return -1;
+  }
+  if (NanCheck()) {
+    if (GeneralHasNan(layout, n, n, a, lda)) return -4;
+    if (GeneralHasNan(layout, n, nrhs, b, ldb)) return -7;
+  }
+  return GesvWork(layout, n, nrhs, a, lda, ipiv, b, ldb);
+}
+
 }  // namespace
 }  // namespace hwy_amd
 
 extern "C" {
+
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgetrf(int layout, hwyblas_int m, hwyblas_int n, double* a,
+                                           hwyblas_int lda, hwyblas_int* ipiv) {
+  return hwy_amd::Getrf(layout, m, n, a, lda, ipiv);
+}
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgetrf_work(int layout, hwyblas_int m, hwyblas_int n,
+                                                double* a, hwyblas_int lda, hwyblas_int* ipiv) {
+  return hwy_amd::GetrfWork(layout, m, n, a, lda, ipiv);
+}
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgetrs(int layout, char trans, hwyblas_int n,
+                                           hwyblas_int nrhs, const double* a, hwyblas_int lda,
+                                           const hwyblas_int* ipiv, double* b, hwyblas_int ldb) {
+  return hwy_amd::Getrs(layout, trans, n, nrhs, a, lda, ipiv, b, ldb);
+}
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgetrs_work(int layout, char trans, hwyblas_int n,
+                                                hwyblas_int nrhs, const double* a,
+                                                hwyblas_int lda, const hwyblas_int* ipiv,
+                                                double* b, hwyblas_int ldb) {
+  return hwy_amd::GetrsWork(layout, trans, n, nrhs, a, lda, ipiv, b, ldb);
+}
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgesv(int layout, hwyblas_int n, hwyblas_int nrhs,
+                                          double* a, hwyblas_int lda, hwyblas_int* ipiv,
+                                          double* b, hwyblas_int ldb) {
+  return hwy_amd::Gesv(layout, n, nrhs, a, lda, ipiv, b, ldb);
+}
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgesv_work(int layout, hwyblas_int n, hwyblas_int nrhs,
+                                               double* a, hwyblas_int lda, hwyblas_int* ipiv,
+                                               double* b, hwyblas_int ldb) {
+  return hwy_amd::GesvWork(layout, n, nrhs, a, lda, ipiv, b, ldb);
+}
 
 HWYBLAS_AMD_API void LAPACKE_xerbla(const char* name, hwyblas_int info) {
   if (info == -1010) {

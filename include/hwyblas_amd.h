@@ -771,6 +771,41 @@ HWYBLAS_AMD_API hwyblas_int LAPACKE_dposv_work(int matrix_layout, char uplo, hwy
                                                hwyblas_int nrhs, double* a, hwyblas_int lda,
                                                double* b, hwyblas_int ldb);
 
+/* LU through LAPACK's Fortran and LAPACKE interfaces (hwyBLAS
+ * src/lapack/getrf.cc:112-367, :500-540; src/lapack/lapacke_linear.cc:14-75;
+ * signatures as include/hwyblas/lapack.h:93-95, 139-142, 158-161 and
+ * lapacke.h). FP64 only: the factorization is hwyblas_dgetrf_dev with FP64
+ * updates, the solve hwyblas_dgetrs_dev. Info positions as LAPACK's: dgetrf
+ * 1 m, 2 n, 4 lda; dgetrs 1 trans, 2 n, 3 nrhs, 5 lda, 8 ldb; dgesv 1 n,
+ * 2 nrhs, 4 lda, 7 ldb. Host or device operands (matrices and pivots);
+ * each call synchronizes. The LAPACKE forms follow the rules stated for
+ * the Cholesky routines above (host operands). */
+HWYBLAS_AMD_API void dgetrf_(const hwyblas_int* m, const hwyblas_int* n, double* a,
+                             const hwyblas_int* lda, hwyblas_int* ipiv, hwyblas_int* info);
+HWYBLAS_AMD_API void dgetrs_(const char* trans, const hwyblas_int* n, const hwyblas_int* nrhs,
+                             const double* a, const hwyblas_int* lda, const hwyblas_int* ipiv,
+                             double* b, const hwyblas_int* ldb, hwyblas_int* info);
+HWYBLAS_AMD_API void dgesv_(const hwyblas_int* n, const hwyblas_int* nrhs, double* a,
+                            const hwyblas_int* lda, hwyblas_int* ipiv, double* b,
+                            const hwyblas_int* ldb, hwyblas_int* info);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgetrf(int matrix_layout, hwyblas_int m, hwyblas_int n,
+                                           double* a, hwyblas_int lda, hwyblas_int* ipiv);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgetrf_work(int matrix_layout, hwyblas_int m, hwyblas_int n,
+                                                double* a, hwyblas_int lda, hwyblas_int* ipiv);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgetrs(int matrix_layout, char trans, hwyblas_int n,
+                                           hwyblas_int nrhs, const double* a, hwyblas_int lda,
+                                           const hwyblas_int* ipiv, double* b, hwyblas_int ldb);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgetrs_work(int matrix_layout, char trans, hwyblas_int n,
+                                                hwyblas_int nrhs, const double* a,
+                                                hwyblas_int lda, const hwyblas_int* ipiv,
+                                                double* b, hwyblas_int ldb);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgesv(int matrix_layout, hwyblas_int n, hwyblas_int nrhs,
+                                          double* a, hwyblas_int lda, hwyblas_int* ipiv, double* b,
+                                          hwyblas_int ldb);
+HWYBLAS_AMD_API hwyblas_int LAPACKE_dgesv_work(int matrix_layout, hwyblas_int n, hwyblas_int nrhs,
+                                               double* a, hwyblas_int lda, hwyblas_int* ipiv,
+                                               double* b, hwyblas_int ldb);
+
 /* Library version string. */
 HWYBLAS_AMD_API const char* hwyblas_amd_version(void);
 

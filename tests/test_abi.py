@@ -488,3 +488,34 @@ def test_lapacke_nancheck_switch():
     assert H.lib().LAPACKE_get_nancheck() == 0
     H.lib().LAPACKE_set_nancheck(5)
     assert H.lib().LAPACKE_get_nancheck() == 1
+
+
+def test_fortran_lu_argument_checks():
+    """dgetrf_/dgetrs_/dgesv_ report LAPACK's positions (src/lapack/getrf.cc,
+    :341-367) before any device work; LAPACKE_d{getrf,getrs,gesv} check the
+    layout, NaNs and row-major leading dimensions first."""
+    a = np.eye(4, order="F")
+    b = np.ones((4, 2), order="F")
+    piv = np.zeros(4, dtype=np.int32)
+    with H.ErrorCapture() as cap:
+        assert H.getrf(-1, 4, a, 4, piv) == -1
+        assert H.getrf(4, 4, a, 3, piv) == -4
+        assert H.getrs("X", 4, 2, a, 4, piv, b, 4) == -1
+        assert H.getrs("N", 4, 2, a, 4, piv, b, 3) == -8
+        assert H.gesv(4, -1, a, 4, piv, b, 4) == -2
+        assert H.gesv(4, 2, a, 4, piv, b, 3) == -7
+    assert cap.calls == [("DGETRF", 1), ("DGETRF", 4), ("DGETRS", 1), ("DGETRS", 8),
+                         ("DGESV", 2), ("DGESV", 7)]
+    H.lib().LAPACKE_set_nancheck(1)
+    assert H.lapacke_lu("getrf", 3, 4, 4, a, 4, piv) == -1
+    an = a.copy(order="F")
+    an[1, 2] = np.nan
+    assert H.lapacke_lu("getrf", H.ColMajor, 4, 4, an, 4, piv) == -4
+    assert H.lapacke_lu("getrs", H.ColMajor, "N", 4, 2, an, 4, piv, b, 4) == -5
+    assert H.lapacke_lu("gesv", H.RowMajor, 4, 2, an, 4, piv, b, 2) == -4
+    bn = b.copy(order="F")
+    bn[0, 0] = np.nan
+    assert H.lapacke_lu("gesv", H.ColMajor, 4, 2, a, 4, piv, bn, 4) == -7
+    assert H.lapacke_lu("getrf", H.RowMajor, 4, 4, a, 3, piv, work=True) == -5
+    assert H.lapacke_lu("getrs", H.RowMajor, "N", 4, 2, a, 4, piv, b, 1, work=True) == -9
+    assert H.lapacke_lu("gesv", H.RowMajor, 4, 2, a, 2, piv, b, 2, work=True) == -5
