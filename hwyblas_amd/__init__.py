@@ -78,6 +78,8 @@ def _declare(L):
         fn(f"LAPACKE_dgetrs{w}", _i, [_i, ctypes.c_char, _i, _i, _p, _i, _p, _p, _i])
         fn(f"LAPACKE_dgesv{w}", _i, [_i, _i, _i, _p, _i, _p, _p, _i])
     for pre in ("z", "c"):
+        fn(f"{pre}gemv_", None, [ctypes.c_char_p, _pi, _pi, _p, _p, _pi, _p, _pi, _p, _p, _pi])
+        fn(f"cblas_{pre}gemv", None, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i])
         fn(f"{pre}gemm_", None, [ctypes.c_char_p, ctypes.c_char_p, _pi, _pi, _pi, _p, _p, _pi,
                                  _p, _pi, _p, _p, _pi])
         fn(f"cblas_{pre}gemm", None, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i])
@@ -338,6 +340,22 @@ def zgemm_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None,
     return getattr(lib(), f"hwyblas_{_zpre(dtype)}gemm_dev")(
         ta.encode(), tb.encode(), m, n, k, _ptr(al), _ptr(a), lda, _ptr(b), ldb, _ptr(be),
         _ptr(c), ldc, stream)
+
+
+def zgemv(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, dtype=None):
+    """Fortran cgemv_/zgemv_ (numpy complex arrays or DeviceArrays); y in place."""
+    dtype = dtype or y.dtype
+    al, be = _zscalar(alpha, dtype), _zscalar(beta, dtype)
+    getattr(lib(), f"{_zpre(dtype)}gemv_")(
+        trans.encode(), _ip(m), _ip(n), _ptr(al), _ptr(a), _ip(lda), _ptr(x), _ip(incx),
+        _ptr(be), _ptr(y), _ip(incy))
+
+
+def cblas_zgemv(layout, trans, m, n, alpha, a, lda, x, incx, beta, y, incy, dtype=None):
+    dtype = dtype or y.dtype
+    al, be = _zscalar(alpha, dtype), _zscalar(beta, dtype)
+    getattr(lib(), f"cblas_{_zpre(dtype)}gemv")(layout, trans, m, n, _ptr(al), _ptr(a), lda,
+                                                _ptr(x), incx, _ptr(be), _ptr(y), incy)
 
 
 def ztrsm(side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb, dtype=None):

@@ -169,6 +169,27 @@ void ComplexTrsm(const char* name, CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_U
        static_cast<R*>(b), &ldb);
 }
 
+// Complex GEMV. Row-major: N and T swap, and ConjTrans becomes 'R'
+// (conjugated, not transposed) on the column-major storage A^T; m and n swap.
+// Positions as Gemv.
+template <typename R>
+void ComplexGemv(const char* name, const char* fname, CBLAS_LAYOUT layout, CBLAS_TRANSPOSE trans,
+                 blasint m, blasint n, const void* alpha, const void* a, blasint lda,
+                 const void* x, blasint incx, const void* beta, void* y, blasint incy) {
+  if (!ValidLayout(name, layout)) return;
+  char tc = TransChar(trans);
+  const bool row = layout == CblasRowMajor;
+  if (row) {
+    tc = tc == 'N' ? 'T' : tc == 'T' ? 'N' : tc == 'C' ? 'R' : tc;
+    std::swap(m, n);
+  }
+  CblasScope scope(name, row ? std::initializer_list<int>{2, 4, 3, 5, 6, 7, 8, 9, 10, 11, 12}
+                             : std::initializer_list<int>{2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12});
+  ZgemvEntry<R>(fname, tc, m, n, static_cast<const R*>(alpha), static_cast<const R*>(a), lda,
+                static_cast<const R*>(x), incx, static_cast<const R*>(beta), static_cast<R*>(y),
+                incy);
+}
+
 // GEMV. Row-major: op(A) of the row-major A is the other op of the
 // column-major A^T (n x m). Positions: layout 1, trans 2, m 3, n 4, alpha 5,
 // a 6, lda 7, x 8, incx 9, beta 10, y 11, incy 12.
@@ -416,6 +437,24 @@ HWYBLAS_AMD_API void cblas_ztrsm(const CBLAS_LAYOUT layout, const CBLAS_SIDE sid
                                  const hwyblas_int lda, void* b, const hwyblas_int ldb) {
   hwy_amd::ComplexTrsm<double>("cblas_ztrsm", layout, side, uplo, transa, diag, m, n, alpha, a,
                                lda, b, ldb);
+}
+
+HWYBLAS_AMD_API void cblas_cgemv(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOSE trans,
+                                 const hwyblas_int m, const hwyblas_int n, const void* alpha,
+                                 const void* a, const hwyblas_int lda, const void* x,
+                                 const hwyblas_int incx, const void* beta, void* y,
+                                 const hwyblas_int incy) {
+  hwy_amd::ComplexGemv<float>("cblas_cgemv", "CGEMV", layout, trans, m, n, alpha, a, lda, x,
+                              incx, beta, y, incy);
+}
+
+HWYBLAS_AMD_API void cblas_zgemv(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOSE trans,
+                                 const hwyblas_int m, const hwyblas_int n, const void* alpha,
+                                 const void* a, const hwyblas_int lda, const void* x,
+                                 const hwyblas_int incx, const void* beta, void* y,
+                                 const hwyblas_int incy) {
+  hwy_amd::ComplexGemv<double>("cblas_zgemv", "ZGEMV", layout, trans, m, n, alpha, a, lda, x,
+                               incx, beta, y, incy);
 }
 
 HWYBLAS_AMD_API void cblas_strsm(const CBLAS_LAYOUT layout, const CBLAS_SIDE side,

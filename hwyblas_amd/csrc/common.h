@@ -106,6 +106,12 @@ struct ThreadContext {
   size_t cap[kNumSlots] = {};
 };
 
+// Complex GEMV with op 'N', 'T', 'C' or 'R' (conjugated, not transposed),
+// Fortran-convention checks reported as `name` (gemv.hip).
+template <typename R>
+void ZgemvEntry(const char* name, char op, blasint m, blasint n, const R* alpha, const R* a,
+                blasint lda, const R* x, blasint incx, const R* beta, R* y, blasint incy);
+
 ThreadContext& Context();
 void* Staging(int slot, size_t bytes);
 

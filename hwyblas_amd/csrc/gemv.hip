@@ -210,7 +210,196 @@ void GemvHost(const char* name, const char* transp, const blasint* mp, const bla
   HWY_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// ---------------------------------------------------------------------------
+// Complex GEMV (cgemv_, zgemv_; the CBLAS forms go through ZgemvEntry). op:
+// 'N', 'T', 'C', or 'R' = conjugated and not transposed (row-major CBLAS
+// ConjTrans is 'R' on the column-major storage, so that x and y need no
+// conjugated copies). Products are complex FMAs in ascending k; the store is
+// r = alpha acc; beta == 1: r + y; beta != 0: r + beta y; beta == 0 never
+// reads y.
+
+template <typename R>
+struct Cplx2;
+template <>
+struct Cplx2<double> {
+  using type = double2;
+};
+template <>
+struct Cplx2<float> {
+  using type = float2;
+};
+
+template <typename C2>
+__device__ __forceinline__ void CFma(C2 a, C2 x, bool conj_a, C2* acc) {
+  const auto ai = conj_a ? -a.y : a.y;
+  acc->x = __builtin_fma(a.x, x.x, __builtin_fma(-ai, x.y, acc->x));
+  acc->y = __builtin_fma(a.x, x.y, __builtin_fma(ai, x.x, acc->y));
+}
+
+template <typename R, bool CONJ>
+__global__ void __launch_bounds__(kGemvRows)
+    ZGemvNPartialKernel(int64_t m, int64_t n, const typename Cplx2<R>::type* __restrict__ a,
+                        int64_t lda, const typename Cplx2<R>::type* __restrict__ x,
+                        typename Cplx2<R>::type* __restrict__ part) {
+  const int64_t i = int64_t(blockIdx.x) * kGemvRows + threadIdx.x;
+  if (i >= m) return;
+  const int64_t j0 = int64_t(blockIdx.y) * kGemvChunk;
+  const int64_t j1 = j0 + kGemvChunk < n ? j0 + kGemvChunk : n;
+  typename Cplx2<R>::type acc;
+  acc.x = R(0);
+  acc.y = R(0);
+#pragma unroll 4
+  for (int64_t j = j0; j < j1; ++j) CFma(a[i + j * lda], x[j], CONJ, &acc);
+  part[int64_t(blockIdx.y) * m + i] = acc;
+}
+
+template <typename C2, typename R>
+__device__ __forceinline__ C2 ZStore(C2 acc, R ar, R ai, R br, R bi, int beta_kind, const C2* y) {
+  C2 r;
+  r.x = acc.x * ar - acc.y * ai;
+  r.y = acc.y * ar + acc.x * ai;
+  if (beta_kind == 1) {
+    r.x = r.x + y->x;
+    r.y = r.y + y->y;
+  } else if (beta_kind == 2) {
+    r.x = r.x + (y->x * br - y->y * bi);
+    r.y = r.y + (y->y * br + y->x * bi);
+  }
+  return r;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    ZGemvNFinishKernel(int64_t m, int64_t nchunks, const typename Cplx2<R>::type* __restrict__ part,
+                       R ar, R ai, R br, R bi, int beta_kind,
+                       typename Cplx2<R>::type* __restrict__ y) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= m) return;
+  typename Cplx2<R>::type acc;
+  acc.x = R(0);
+  acc.y = R(0);
+  for (int64_t c = 0; c < nchunks; ++c) {
+    acc.x += part[c * m + i].x;
+    acc.y += part[c * m + i].y;
+  }
+  y[i] = ZStore(acc, ar, ai, br, bi, beta_kind, &y[i]);
+}
+
+// op T or C: one wave per column, lanes striding down it.
+template <typename R, bool CONJ>
+__global__ void __launch_bounds__(256)
+    ZGemvTKernel(int64_t m, int64_t n, const typename Cplx2<R>::type* __restrict__ a, int64_t lda,
+                 const typename Cplx2<R>::type* __restrict__ x, R ar, R ai, R br, R bi,
+                 int beta_kind, typename Cplx2<R>::type* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;
+  typename Cplx2<R>::type acc;
+  acc.x = R(0);
+  acc.y = R(0);
+  const auto* col = a + j * lda;
+  for (int64_t i = lane; i < m; i += 64) CFma(col[i], x[i], CONJ, &acc);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    acc.x += __shfl_xor(acc.x, off);
+    acc.y += __shfl_xor(acc.y, off);
+  }
+  if (lane == 0) y[j] = ZStore(acc, ar, ai, br, bi, beta_kind, &y[j]);
+}
+
+template <typename R>
+void ZgemvHost(const char* name, char opc, blasint m, blasint n, const R* alpha, const R* a,
+               blasint lda, const R* x, blasint incx, const R* beta, R* y, blasint incy) {
+  using C2 = typename Cplx2<R>::type;
+  const char oc = char(opc & ~0x20);
+  blasint info = 0;
+  if (oc != 'N' && oc != 'T' && oc != 'C' && oc != 'R') info = 1;
+  else if (m < 0) info = 2;
+  else if (n < 0) info = 3;
+  else if (lda < std::max(1, m)) info = 6;
+  else if (incx == 0) info = 8;
+  else if (incy == 0) info = 11;
+  if (info != 0) return ReportError(name, info);
+  const bool alpha0 = alpha[0] == R(0) && alpha[1] == R(0);
+  const bool beta0 = beta[0] == R(0) && beta[1] == R(0);
+  const bool beta1 = beta[0] == R(1) && beta[1] == R(0);
+  if (m == 0 || n == 0 || (alpha0 && beta1)) return;
+  const int beta_kind = beta0 ? 0 : beta1 ? 1 : 2;
+
+  hipStream_t s = Context().stream;
+  const bool trans = oc == 'T' || oc == 'C';
+  const bool conj = oc == 'C' || oc == 'R';
+  const int64_t len_x = trans ? m : n, len_y = trans ? n : m;
+  std::vector<C2> tmp_x, tmp_y;
+  C2* y2 = reinterpret_cast<C2*>(y);
+  C2* dy = ContiguousVector(y2, len_y, incy, !beta0, kSlotC, s, &tmp_y);
+  if (alpha0) {
+    hipLaunchKernelGGL(ZGemvNFinishKernel<R>, dim3(Blocks(len_y, 256)), dim3(256), 0, s, len_y,
+                       int64_t(0), static_cast<const C2*>(nullptr), alpha[0], alpha[1], beta[0],
+                       beta[1], beta_kind, dy);
+  } else {
+    const C2* dx = ContiguousVector(reinterpret_cast<const C2*>(x), len_x, incx, true, kSlotB, s,
+                                    &tmp_x);
+    const C2* da = reinterpret_cast<const C2*>(a);
+    int64_t lda_d = lda;
+    if (!IsDevicePointer(a)) {
+      C2* buf = static_cast<C2*>(Staging(kSlotA, size_t(m) * n * sizeof(C2)));
+      HWY_HIP_CHECK(hipMemcpy2DAsync(buf, size_t(m) * sizeof(C2), da, size_t(lda) * sizeof(C2),
+                                     size_t(m) * sizeof(C2), n, hipMemcpyHostToDevice, s));
+      da = buf;
+      lda_d = m;
+    }
+    if (!trans) {
+      const int64_t nchunks = (n + kGemvChunk - 1) / kGemvChunk;
+      C2* part = static_cast<C2*>(Staging(kSlotD, size_t(nchunks) * m * sizeof(C2)));
+      const dim3 grid(unsigned(Blocks(m, kGemvRows)), unsigned(nchunks));
+      if (conj)
+        hipLaunchKernelGGL((ZGemvNPartialKernel<R, true>), grid, dim3(kGemvRows), 0, s, m, n, da,
+                           lda_d, dx, part);
+      else
+        hipLaunchKernelGGL((ZGemvNPartialKernel<R, false>), grid, dim3(kGemvRows), 0, s, m, n, da,
+                           lda_d, dx, part);
+      hipLaunchKernelGGL(ZGemvNFinishKernel<R>, dim3(Blocks(m, 256)), dim3(256), 0, s, m,
+                         nchunks, part, alpha[0], alpha[1], beta[0], beta[1], beta_kind, dy);
+    } else {
+      if (conj)
+        hipLaunchKernelGGL((ZGemvTKernel<R, true>), dim3(Blocks(n, 4)), dim3(256), 0, s, m, n,
+                           da, lda_d, dx, alpha[0], alpha[1], beta[0], beta[1], beta_kind, dy);
+      else
+        hipLaunchKernelGGL((ZGemvTKernel<R, false>), dim3(Blocks(n, 4)), dim3(256), 0, s, m, n,
+                           da, lda_d, dx, alpha[0], alpha[1], beta[0], beta[1], beta_kind, dy);
+    }
+  }
+  HWY_HIP_CHECK(hipGetLastError());
+  if (dy != y2) {
+    if (IsDevicePointer(y)) {
+      hipLaunchKernelGGL(ScatterKernel<C2>, dim3(Blocks(len_y, 256)), dim3(256), 0, s, dy, len_y,
+                         int64_t(incy), y2);
+      HWY_HIP_CHECK(hipGetLastError());
+    } else {
+      tmp_y.resize(size_t(len_y));
+      HWY_HIP_CHECK(hipMemcpyAsync(tmp_y.data(), dy, size_t(len_y) * sizeof(C2),
+                                   hipMemcpyDeviceToHost, s));
+      HWY_HIP_CHECK(hipStreamSynchronize(s));
+      for (int64_t i = 0; i < len_y; ++i) y2[VecIndex(i, len_y, incy)] = tmp_y[size_t(i)];
+    }
+  }
+  HWY_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 }  // namespace
+
+template <typename R>
+void ZgemvEntry(const char* name, char op, blasint m, blasint n, const R* alpha, const R* a,
+                blasint lda, const R* x, blasint incx, const R* beta, R* y, blasint incy) {
+  ZgemvHost<R>(name, op, m, n, alpha, a, lda, x, incx, beta, y, incy);
+}
+template void ZgemvEntry<float>(const char*, char, blasint, blasint, const float*, const float*,
+                                blasint, const float*, blasint, const float*, float*, blasint);
+template void ZgemvEntry<double>(const char*, char, blasint, blasint, const double*,
+                                 const double*, blasint, const double*, blasint, const double*,
+                                 double*, blasint);
+
 }  // namespace hwy_amd
 
 using hwy_amd::blasint;
@@ -229,6 +418,23 @@ HWYBLAS_AMD_API void dgemv_(const char* trans, const blasint* m, const blasint* 
                             const double* x, const blasint* incx, const double* beta,
                             double* y, const blasint* incy) {
   hwy_amd::GemvHost("DGEMV", trans, m, n, alpha, a, lda, x, incx, beta, y, incy);
+}
+
+HWYBLAS_AMD_API void cgemv_(const char* trans, const blasint* m, const blasint* n,
+                            const float* alpha, const float* a, const blasint* lda,
+                            const float* x, const blasint* incx, const float* beta, float* y,
+                            const blasint* incy) {
+  // 'R' is internal to the CBLAS row-major mapping, not a Fortran option.
+  const char t = (*trans == 'R' || *trans == 'r') ? '?' : *trans;
+  hwy_amd::ZgemvEntry<float>("CGEMV", t, *m, *n, alpha, a, *lda, x, *incx, beta, y, *incy);
+}
+
+HWYBLAS_AMD_API void zgemv_(const char* trans, const blasint* m, const blasint* n,
+                            const double* alpha, const double* a, const blasint* lda,
+                            const double* x, const blasint* incx, const double* beta,
+                            double* y, const blasint* incy) {
+  const char t = (*trans == 'R' || *trans == 'r') ? '?' : *trans;
+  hwy_amd::ZgemvEntry<double>("ZGEMV", t, *m, *n, alpha, a, *lda, x, *incx, beta, y, *incy);
 }
 
 }  // extern "C"

@@ -806,6 +806,33 @@ HWYBLAS_AMD_API hwyblas_int LAPACKE_dgesv_work(int matrix_layout, hwyblas_int n,
                                                double* a, hwyblas_int lda, hwyblas_int* ipiv,
                                                double* b, hwyblas_int ldb);
 
+/* Complex GEMV (hwyBLAS src/gemv.cc:39-94, src/cblas.cc:279-316, 1030-1053;
+ * signatures as include/hwyblas/hwyblas.h:229-238 and cblas.h:241-252):
+ * y := alpha op(A) x + beta y, op = N, T or C; interleaved (re, im) storage,
+ * alpha and beta as
+ * pointers to (re, im). Info positions as ?gemv_ (1 trans, 2 m, 3 n, 6 lda,
+ * 8 incx, 11 incy). Row-major CBLAS ConjTrans runs as the conjugated,
+ * untransposed product on the column-major storage (no conjugated copies of
+ * x or y). Host or device operands; synchronous. */
+HWYBLAS_AMD_API void cgemv_(const char* trans, const hwyblas_int* m, const hwyblas_int* n,
+                            const float* alpha, const float* a, const hwyblas_int* lda,
+                            const float* x, const hwyblas_int* incx, const float* beta, float* y,
+                            const hwyblas_int* incy);
+HWYBLAS_AMD_API void zgemv_(const char* trans, const hwyblas_int* m, const hwyblas_int* n,
+                            const double* alpha, const double* a, const hwyblas_int* lda,
+                            const double* x, const hwyblas_int* incx, const double* beta,
+                            double* y, const hwyblas_int* incy);
+HWYBLAS_AMD_API void cblas_cgemv(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOSE trans,
+                                 const hwyblas_int m, const hwyblas_int n, const void* alpha,
+                                 const void* a, const hwyblas_int lda, const void* x,
+                                 const hwyblas_int incx, const void* beta, void* y,
+                                 const hwyblas_int incy);
+HWYBLAS_AMD_API void cblas_zgemv(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOSE trans,
+                                 const hwyblas_int m, const hwyblas_int n, const void* alpha,
+                                 const void* a, const hwyblas_int lda, const void* x,
+                                 const hwyblas_int incx, const void* beta, void* y,
+                                 const hwyblas_int incy);
+
 /* Library version string. */
 HWYBLAS_AMD_API const char* hwyblas_amd_version(void);
 
