@@ -767,7 +767,23 @@ def level3_leg(H, n=8192, reps=3):
         out[f"dgemv_{tr}"] = {"n": gm, "ms": round(t * 1e3, 3), "gb_per_s": round(gbs, 1),
                               "frac_hbm_peak": round(gbs / (PEAK_HBM * 1e3), 3)}
         log(f"level3 dgemv {tr} n={gm}: {t * 1e3:.3f} ms, {gbs:.0f} GB/s")
-    del GA
+    # ZGEMV on the same memory read as a complex 8192 x 16384 matrix (ld 8192
+    # complex): 16 m n bytes of A per call; x and y hold 16384 complex
+    # elements each (enough for either op).
+    zm, zn = gm // 2, gm
+    zx, zy = H.DeviceArray(2 * gm, 1, d), H.DeviceArray(2 * gm, 1, d)
+    zx.fill_uniform(2 * gm, 1, 8)
+    zy.fill_uniform(2 * gm, 1, 9)
+    H.synchronize()
+    for tr in "NC":
+        t = best(lambda: H.zgemv(tr, zm, zn, 1.0 - 0.5j, GA.ptr, zm, zx.ptr, 1, 0.5, zy.ptr, 1,
+                                 dtype=np.complex128))
+        gbs = 16.0 * zm * zn / t / 1e9
+        out[f"zgemv_{tr}"] = {"m": zm, "n": zn, "ms": round(t * 1e3, 3),
+                              "gb_per_s": round(gbs, 1),
+                              "frac_hbm_peak": round(gbs / (PEAK_HBM * 1e3), 3)}
+        log(f"level3 zgemv {tr} {zm}x{zn}: {t * 1e3:.3f} ms, {gbs:.0f} GB/s")
+    del GA, zx, zy
     out["dpotrf"] = potrf_leg(H)
     out.update(zgemm_leg(H))
     # SYRK's lower triangle against DGEMM A A^T (beta = 0 on both).
