@@ -404,7 +404,10 @@ __global__ void __launch_bounds__(64 * kZWaves) ZDiagKernel(ZDiagDesc<R> d) {
   x.y = R(0);
   if (lane < r) x = *at;
   const C2 myinv = sinv[lane];
-  for (int i = 0; i < r; ++i) {
+  // All kZNB steps, unrolled: steps at or past r touch zeros only (padding
+  // rows hold x = 0 and reciprocal 1, padding triangle entries are 0).
+#pragma unroll
+  for (int i = 0; i < kZNB; ++i) {
     if (lane == i) {
       const R xr = x.x * myinv.x - x.y * myinv.y;
       const R xi = x.y * myinv.x + x.x * myinv.y;
