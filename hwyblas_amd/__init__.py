@@ -78,6 +78,9 @@ def _declare(L):
         fn(f"LAPACKE_dgetrs{w}", _i, [_i, ctypes.c_char, _i, _i, _p, _i, _p, _p, _i])
         fn(f"LAPACKE_dgesv{w}", _i, [_i, _i, _i, _p, _i, _p, _p, _i])
     for pre in ("z", "c"):
+        for r in ("syrk_", "herk_"):
+            fn(f"{pre}{r}", None, [ctypes.c_char_p, ctypes.c_char_p, _pi, _pi, _p, _p, _pi, _p,
+                                   _p, _pi])
         fn(f"{pre}gemv_", None, [ctypes.c_char_p, _pi, _pi, _p, _p, _pi, _p, _pi, _p, _p, _pi])
         fn(f"cblas_{pre}gemv", None, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i])
         fn(f"{pre}gemm_", None, [ctypes.c_char_p, ctypes.c_char_p, _pi, _pi, _pi, _p, _p, _pi,
@@ -340,6 +343,20 @@ def zgemm_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None,
     return getattr(lib(), f"hwyblas_{_zpre(dtype)}gemm_dev")(
         ta.encode(), tb.encode(), m, n, k, _ptr(al), _ptr(a), lda, _ptr(b), ldb, _ptr(be),
         _ptr(c), ldc, stream)
+
+
+def zrank_k(routine, uplo, trans, n, k, alpha, a, lda, beta, c, ldc, dtype=None):
+    """Fortran {c,z}syrk_ (complex alpha, beta) or {c,z}herk_ (real alpha,
+    beta) on numpy complex arrays or DeviceArrays; C in place."""
+    dtype = dtype or c.dtype
+    if routine == "herk":
+        rt = np.float64 if np.dtype(dtype) == np.complex128 else np.float32
+        al, be = np.array([alpha], dtype=rt), np.array([beta], dtype=rt)
+    else:
+        al, be = _zscalar(alpha, dtype), _zscalar(beta, dtype)
+    getattr(lib(), f"{_zpre(dtype)}{routine}_")(
+        uplo.encode(), trans.encode(), _ip(n), _ip(k), _ptr(al), _ptr(a), _ip(lda), _ptr(be),
+        _ptr(c), _ip(ldc))
 
 
 def zgemv(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, dtype=None):

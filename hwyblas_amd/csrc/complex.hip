@@ -108,19 +108,28 @@ __device__ __forceinline__ void CMulScalar(R xr, R xi, R sr, R si, R* outr, R* o
 }
 
 // C(i, j) from the four quadrants of the real product p (2m x 2n, ld 2m).
+// part: 0 all of C, 1 its lower triangle (i >= j), 2 its upper one; the
+// other entries are not touched. real_diag: the diagonal's imaginary parts
+// are set to zero (HERK). p == nullptr: no product (C := beta C).
 template <typename R>
 __global__ void __launch_bounds__(kZThreads)
     ZCombineKernel(const R* __restrict__ p, int64_t m, int64_t n, R ar, R ai, R br, R bi,
-                   int beta_kind, typename Vec2<R>::type* __restrict__ c, int64_t ldc) {
+                   int beta_kind, typename Vec2<R>::type* __restrict__ c, int64_t ldc, int part,
+                   bool real_diag) {
   const int64_t i = int64_t(blockIdx.x) * kZThreads + threadIdx.x;
   if (i >= m) return;
   const int64_t ldp = 2 * m;
   for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
-    const R rr = p[i + j * ldp];              // sum ar br
-    const R ir = p[m + i + j * ldp];          // sum ai br
-    const R ri = p[i + (n + j) * ldp];        // sum ar bi
-    const R ii = p[m + i + (n + j) * ldp];    // sum ai bi
-    const R abr = rr - ii, abi = ir + ri;
+    if ((part == 1 && i < j) || (part == 2 && i > j)) continue;
+    R abr = R(0), abi = R(0);
+    if (p) {
+      const R rr = p[i + j * ldp];              // sum ar br
+      const R ir = p[m + i + j * ldp];          // sum ai br
+      const R ri = p[i + (n + j) * ldp];        // sum ar bi
+      const R ii = p[m + i + (n + j) * ldp];    // sum ai bi
+      abr = rr - ii;
+      abi = ir + ri;
+    }
     R xr, xi;
     CMulScalar(abr, abi, ar, ai, &xr, &xi);
     auto& cij = c[i + j * ldc];
@@ -137,7 +146,7 @@ __global__ void __launch_bounds__(kZThreads)
     }
     typename Vec2<R>::type out;
     out.x = xr;
-    out.y = xi;
+    out.y = (real_diag && i == j) ? R(0) : xi;
     cij = out;
   }
 }
@@ -191,7 +200,7 @@ template <typename R>
 void ZgemmDevice(ZOp oa, ZOp ob, int64_t m, int64_t n, int64_t k, const R* alpha,
                  const typename Vec2<R>::type* a, int64_t lda, const typename Vec2<R>::type* b,
                  int64_t ldb, const R* beta, typename Vec2<R>::type* c, int64_t ldc,
-                 hipStream_t s) {
+                 hipStream_t s, int part = 0, bool real_diag = false) {
   R* pa = static_cast<R*>(Staging(kSlotPackA, size_t(2 * m) * k * sizeof(R)));
   R* pb = static_cast<R*>(Staging(kSlotPackB, size_t(2 * n) * k * sizeof(R)));
   R* p = static_cast<R*>(Staging(kSlotProd, size_t(4 * m) * n * sizeof(R)));
@@ -202,7 +211,7 @@ void ZgemmDevice(ZOp oa, ZOp ob, int64_t m, int64_t n, int64_t k, const R* alpha
                         : (beta[0] == R(1) && beta[1] == R(0)) ? 1
                                                                 : 2;
   hipLaunchKernelGGL((ZCombineKernel<R>), ZGrid(m, n), dim3(kZThreads), 0, s, p, m, n,
-                     alpha[0], alpha[1], beta[0], beta[1], beta_kind, c, ldc);
+                     alpha[0], alpha[1], beta[0], beta[1], beta_kind, c, ldc, part, real_diag);
   HWY_HIP_CHECK(hipGetLastError());
 }
 
@@ -595,6 +604,75 @@ int ZtrsmDev(const char* name, char side, char uplo, char transa, char diag, int
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// SYRK and HERK (hwyBLAS src/level3.cc:78-135): C := alpha op(A) op(A)' +
+// beta C on the uplo triangle, ' = ^T (SYRK: trans N or T, complex alpha and
+// beta) or ^H (HERK: trans N or C, real alpha and beta, the diagonal's
+// imaginary parts set to zero). The product is the complex GEMM above over
+// all of n x n (twice the triangle's flops); the combining pass writes the
+// triangle only.
+template <typename R>
+void ZSyrkHost(const char* name, bool herm, const char* uplop, const char* transp,
+               const blasint* np, const blasint* kp, const R* alpha_in, const R* a,
+               const blasint* ldap, const R* beta_in, R* c, const blasint* ldcp) {
+  using C2 = typename Vec2<R>::type;
+  const blasint n = *np, k = *kp, lda = *ldap, ldc = *ldcp;
+  bool lower = false;
+  const bool vu = DecodeUplo(*uplop, &lower);
+  ZOp op;
+  bool vt = DecodeZOp(*transp, &op);
+  if (vt) vt = op != (herm ? ZOp::kT : ZOp::kC);
+  const bool no_trans = op == ZOp::kN;
+  const blasint rows_a = no_trans ? n : k;
+  blasint info = 0;
+  if (!vu) info = 1;
+  else if (!vt) info = 2;
+  else if (n < 0) info = 3;
+  else if (k < 0) info = 4;
+  else if (lda < std::max(1, rows_a)) info = 7;
+  else if (ldc < std::max(1, n)) info = 10;
+  if (info != 0) {
+    ReportError(name, info);
+    return;
+  }
+  const R alpha[2] = {alpha_in[0], herm ? R(0) : alpha_in[1]};
+  const R beta[2] = {beta_in[0], herm ? R(0) : beta_in[1]};
+  const bool no_product = k == 0 || IsZeroC(alpha);
+  if (n == 0 || (no_product && IsOneC(beta))) return;
+  hipStream_t s = Context().stream;
+  const bool c_dev = IsDevicePointer(c);
+  C2* dc = reinterpret_cast<C2*>(c);
+  int64_t ldc_d = ldc;
+  if (!c_dev) {
+    dc = static_cast<C2*>(Staging(kSlotC, size_t(n) * n * sizeof(C2)));
+    ldc_d = n;
+    CopyIn2D(dc, reinterpret_cast<const C2*>(c), n, n, ldc, s);  // the other triangle is kept
+  }
+  const int part = lower ? 1 : 2;
+  if (no_product) {
+    const int beta_kind = IsZeroC(beta) ? 0 : IsOneC(beta) ? 1 : 2;
+    hipLaunchKernelGGL((ZCombineKernel<R>), ZGrid(n, n), dim3(kZThreads), 0, s,
+                       static_cast<const R*>(nullptr), int64_t(n), int64_t(n), R(0), R(0),
+                       beta[0], beta[1], beta_kind, dc, ldc_d, part, herm);
+    HWY_HIP_CHECK(hipGetLastError());
+  } else {
+    const C2* da = reinterpret_cast<const C2*>(a);
+    int64_t lda_d = lda;
+    const int64_t ra = rows_a, ca = no_trans ? k : n;
+    if (!IsDevicePointer(a)) {
+      C2* buf = static_cast<C2*>(Staging(kSlotA, size_t(ra) * ca * sizeof(C2)));
+      CopyIn2D(buf, da, ra, ca, lda, s);
+      da = buf;
+      lda_d = ra;
+    }
+    const ZOp second = herm ? ZOp::kC : ZOp::kT;
+    ZgemmDevice<R>(no_trans ? ZOp::kN : second, no_trans ? second : ZOp::kN, n, n, k, alpha, da,
+                   lda_d, da, lda_d, beta, dc, ldc_d, s, part, herm);
+  }
+  if (!c_dev) CopyOut2D(reinterpret_cast<C2*>(c), ldc, dc, n, n, s);
+  HWY_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 }  // namespace
 }  // namespace hwy_amd
 
@@ -662,6 +740,34 @@ HWYBLAS_AMD_API int hwyblas_ztrsm_dev(char side, char uplo, char transa, char di
                                       int ldb, void* stream) {
   return hwy_amd::ZtrsmDev<double>("ZTRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b,
                                    ldb, static_cast<hipStream_t>(stream));
+}
+
+HWYBLAS_AMD_API void csyrk_(const char* uplo, const char* trans, const blasint* n,
+                            const blasint* k, const float* alpha, const float* a,
+                            const blasint* lda, const float* beta, float* c, const blasint* ldc) {
+  hwy_amd::ZSyrkHost<float>("CSYRK", false, uplo, trans, n, k, alpha, a, lda, beta, c, ldc);
+}
+
+HWYBLAS_AMD_API void zsyrk_(const char* uplo, const char* trans, const blasint* n,
+                            const blasint* k, const double* alpha, const double* a,
+                            const blasint* lda, const double* beta, double* c,
+                            const blasint* ldc) {
+  hwy_amd::ZSyrkHost<double>("ZSYRK", false, uplo, trans, n, k, alpha, a, lda, beta, c, ldc);
+}
+
+HWYBLAS_AMD_API void cherk_(const char* uplo, const char* trans, const blasint* n,
+                            const blasint* k, const float* alpha, const float* a,
+                            const blasint* lda, const float* beta, float* c, const blasint* ldc) {
+  const float al[2] = {*alpha, 0.0f}, be[2] = {*beta, 0.0f};
+  hwy_amd::ZSyrkHost<float>("CHERK", true, uplo, trans, n, k, al, a, lda, be, c, ldc);
+}
+
+HWYBLAS_AMD_API void zherk_(const char* uplo, const char* trans, const blasint* n,
+                            const blasint* k, const double* alpha, const double* a,
+                            const blasint* lda, const double* beta, double* c,
+                            const blasint* ldc) {
+  const double al[2] = {*alpha, 0.0}, be[2] = {*beta, 0.0};
+  hwy_amd::ZSyrkHost<double>("ZHERK", true, uplo, trans, n, k, al, a, lda, be, c, ldc);
 }
 
 }  // extern "C"

@@ -833,6 +833,31 @@ HWYBLAS_AMD_API void cblas_zgemv(const CBLAS_LAYOUT layout, const CBLAS_TRANSPOS
                                  const hwyblas_int incx, const void* beta, void* y,
                                  const hwyblas_int incy);
 
+/* Complex SYRK and HERK (hwyBLAS src/level3.cc:78-135; signatures as
+ * include/hwyblas/hwyblas.h:642-663): C := alpha op(A) op(A)' + beta C on the
+ * uplo triangle, ' = ^T (?syrk_: trans N or T, complex alpha and beta as
+ * pointers to (re, im)) or ^H (?herk_: trans N or C, real alpha and beta,
+ * the diagonal's imaginary parts set to zero). Info positions 1 uplo,
+ * 2 trans, 3 n, 4 k, 7 lda, 10 ldc. The other triangle is not touched. The
+ * product runs as the complex GEMM over all of C (twice the triangle's
+ * flops). Host or device operands; synchronous. */
+HWYBLAS_AMD_API void csyrk_(const char* uplo, const char* trans, const hwyblas_int* n,
+                            const hwyblas_int* k, const float* alpha, const float* a,
+                            const hwyblas_int* lda, const float* beta, float* c,
+                            const hwyblas_int* ldc);
+HWYBLAS_AMD_API void zsyrk_(const char* uplo, const char* trans, const hwyblas_int* n,
+                            const hwyblas_int* k, const double* alpha, const double* a,
+                            const hwyblas_int* lda, const double* beta, double* c,
+                            const hwyblas_int* ldc);
+HWYBLAS_AMD_API void cherk_(const char* uplo, const char* trans, const hwyblas_int* n,
+                            const hwyblas_int* k, const float* alpha, const float* a,
+                            const hwyblas_int* lda, const float* beta, float* c,
+                            const hwyblas_int* ldc);
+HWYBLAS_AMD_API void zherk_(const char* uplo, const char* trans, const hwyblas_int* n,
+                            const hwyblas_int* k, const double* alpha, const double* a,
+                            const hwyblas_int* lda, const double* beta, double* c,
+                            const hwyblas_int* ldc);
+
 /* Library version string. */
 HWYBLAS_AMD_API const char* hwyblas_amd_version(void);
 

@@ -519,3 +519,21 @@ def test_fortran_lu_argument_checks():
     assert H.lapacke_lu("getrf", H.RowMajor, 4, 4, a, 3, piv, work=True) == -5
     assert H.lapacke_lu("getrs", H.RowMajor, "N", 4, 2, a, 4, piv, b, 1, work=True) == -9
     assert H.lapacke_lu("gesv", H.RowMajor, 4, 2, a, 2, piv, b, 2, work=True) == -5
+
+
+@pytest.mark.parametrize("routine,change,info", [
+    ("herk", dict(uplo="X"), 1), ("herk", dict(trans="T"), 2), ("syrk", dict(trans="C"), 2),
+    ("syrk", dict(n=-1), 3), ("herk", dict(k=-1), 4), ("syrk", dict(lda=3), 7),
+    ("herk", dict(trans="C", lda=1), 7), ("syrk", dict(ldc=3), 10)])
+def test_zsyrk_zherk_invalid_arguments(routine, change, info):
+    """Complex SYRK takes N or T, HERK N or C (src/level3.cc:93-115)."""
+    g = dict(uplo="L", trans="N", n=4, k=2, lda=4, ldc=4)
+    g.update(change)
+    a = np.ones((8, 8), dtype=np.complex128, order="F")
+    c = np.full((8, 8), 3.0 + 1.0j, dtype=np.complex128, order="F")
+    c0 = c.copy()
+    with H.ErrorCapture() as cap:
+        H.zrank_k(routine, g["uplo"], g["trans"], g["n"], g["k"], 1.0, a, g["lda"], 0.5, c,
+                  g["ldc"])
+    assert cap.calls == [("Z" + routine.upper(), info)]
+    assert np.array_equal(c, c0)
