@@ -42,35 +42,17 @@ constexpr int64_t kTuneOutputElems = int64_t(1) << 25;
 constexpr size_t kTuneBudgetBytes = size_t(3) << 29;  // 1.5 GiB
 constexpr int kMaxCopies = 8192;
 
-struct TuneWorkspace {
-  int device = -1;
-  void* ptr = nullptr;
-  size_t bytes = 0;
-};
-
-TuneWorkspace& Workspace() {
-  thread_local TuneWorkspace w;
-  return w;
-}
-
-// Operand memory for a tuning run, filled once with uniform[-1, 1] values
-// (timing does not depend on them, but NaN or subnormal data could).
+// Operand memory for a tuning run, filled with uniform[-1, 1] values when
+// (re)allocated (timing does not depend on them, but NaN or subnormal data
+// could).
 void* EnsureWorkspace(size_t bytes, hipStream_t s) {
-  TuneWorkspace& w = Workspace();
-  int dev = 0;
-  HWY_HIP_CHECK(hipGetDevice(&dev));
-  if (w.device == dev && w.bytes >= bytes) return w.ptr;
-  if (w.ptr) {
-    HWY_HIP_CHECK(hipStreamSynchronize(s));
-    HWY_HIP_CHECK(hipFree(w.ptr));
-  }
   const size_t rounded = (bytes + (size_t(64) << 20) - 1) & ~((size_t(64) << 20) - 1);
-  HWY_HIP_CHECK(hipMalloc(&w.ptr, rounded));
-  w.bytes = rounded;
-  w.device = dev;
-  hwyblas_amd_fill_uniform(w.ptr, 4, int64_t(rounded / 4), 1, int64_t(rounded / 4), 12345, -1.0,
-                           1.0, s);
-  return w.ptr;
+  bool grown = false;
+  void* p = Staging(kSlotTune, rounded, s, &grown);
+  if (grown)
+    hwyblas_amd_fill_uniform(p, 4, int64_t(rounded / 4), 1, int64_t(rounded / 4), 12345, -1.0,
+                             1.0, s);
+  return p;
 }
 
 int CopiesFor(int dtype_bytes, int m, int n, int k) {

@@ -12,7 +12,7 @@
 
 #include <algorithm>
 
-#include "common.h"
+#include "entries.h"
 
 namespace hwy_amd {
 namespace {
@@ -72,7 +72,7 @@ blasint TrsmInfo(char side, char uplo, char transa, char diag, blasint m,
 }
 
 template <typename T>
-void GemmHost(const char* name, const char* transa, const char* transb,
+void GemmHost(const Caller& who, const char* transa, const char* transb,
               const blasint* mp, const blasint* np, const blasint* kp,
               const T* alphap, const T* a, const blasint* ldap, const T* b,
               const blasint* ldbp, const T* betap, T* c, const blasint* ldcp) {
@@ -80,7 +80,7 @@ void GemmHost(const char* name, const char* transa, const char* transb,
   Op oa, ob;
   const blasint info = GemmInfo(*transa, *transb, m, n, k, lda, ldb, ldc, &oa, &ob);
   if (info != 0) {
-    ReportError(name, info);
+    Reject(who, info);
     return;
   }
   const T alpha = *alphap, beta = *betap;
@@ -125,13 +125,13 @@ void GemmHost(const char* name, const char* transa, const char* transb,
 }
 
 template <typename T>
-int GemmDev(const char* name, char transa, char transb, int m, int n, int k,
+int GemmDev(const Caller& who, char transa, char transb, int m, int n, int k,
             T alpha, const T* a, int lda, const T* b, int ldb, T beta, T* c,
             int ldc, hipStream_t s) {
   Op oa, ob;
   const blasint info = GemmInfo(transa, transb, m, n, k, lda, ldb, ldc, &oa, &ob);
   if (info != 0) {
-    ReportError(name, info);
+    Reject(who, info);
     return info;
   }
   if (m == 0 || n == 0) return 0;
@@ -146,7 +146,7 @@ int GemmDev(const char* name, char transa, char transb, int m, int n, int k,
 }
 
 template <typename T>
-void TrsmHost(const char* name, const char* sidep, const char* uplop,
+void TrsmHost(const Caller& who, const char* sidep, const char* uplop,
               const char* transp, const char* diagp, const blasint* mp,
               const blasint* np, const T* alphap, const T* a,
               const blasint* ldap, T* b, const blasint* ldbp) {
@@ -156,7 +156,7 @@ void TrsmHost(const char* name, const char* sidep, const char* uplop,
   const blasint info = TrsmInfo(*sidep, *uplop, *transp, *diagp, m, n, lda, ldb,
                                 &left, &lower, &op, &unit);
   if (info != 0) {
-    ReportError(name, info);
+    Reject(who, info);
     return;
   }
   const T alpha = *alphap;
@@ -192,7 +192,7 @@ void TrsmHost(const char* name, const char* sidep, const char* uplop,
 }
 
 template <typename T>
-int TrsmDev(const char* name, char side, char uplo, char transa, char diag,
+int TrsmDev(const Caller& who, char side, char uplo, char transa, char diag,
             int m, int n, T alpha, const T* a, int lda, T* b, int ldb,
             hipStream_t s) {
   bool left, lower, unit;
@@ -200,7 +200,7 @@ int TrsmDev(const char* name, char side, char uplo, char transa, char diag,
   const blasint info =
       TrsmInfo(side, uplo, transa, diag, m, n, lda, ldb, &left, &lower, &op, &unit);
   if (info != 0) {
-    ReportError(name, info);
+    Reject(who, info);
     return info;
   }
   if (m == 0 || n == 0) return 0;
@@ -211,6 +211,23 @@ int TrsmDev(const char* name, char side, char uplo, char transa, char diag,
 }
 
 }  // namespace
+
+#define HWY_DEFINE_CORES(T)                                                                   \
+  void Gemm(const Caller& who, const char* transa, const char* transb, const blasint* m,      \
+            const blasint* n, const blasint* k, const T* alpha, const T* a,                   \
+            const blasint* lda, const T* b, const blasint* ldb, const T* beta, T* c,          \
+            const blasint* ldc) {                                                             \
+    GemmHost(who, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);              \
+  }                                                                                           \
+  void Trsm(const Caller& who, const char* side, const char* uplo, const char* transa,        \
+            const char* diag, const blasint* m, const blasint* n, const T* alpha, const T* a, \
+            const blasint* lda, T* b, const blasint* ldb) {                                   \
+    TrsmHost(who, side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);                     \
+  }
+HWY_DEFINE_CORES(float)
+HWY_DEFINE_CORES(double)
+#undef HWY_DEFINE_CORES
+
 }  // namespace hwy_amd
 
 using hwy_amd::blasint;
@@ -222,7 +239,7 @@ HWYBLAS_AMD_API void sgemm_(const char* transa, const char* transb,
                             const float* alpha, const float* a, const blasint* lda,
                             const float* b, const blasint* ldb, const float* beta,
                             float* c, const blasint* ldc) {
-  hwy_amd::GemmHost("SGEMM", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  hwy_amd::Gemm("SGEMM", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
 }
 
 HWYBLAS_AMD_API void dgemm_(const char* transa, const char* transb,
@@ -230,21 +247,21 @@ HWYBLAS_AMD_API void dgemm_(const char* transa, const char* transb,
                             const double* alpha, const double* a, const blasint* lda,
                             const double* b, const blasint* ldb, const double* beta,
                             double* c, const blasint* ldc) {
-  hwy_amd::GemmHost("DGEMM", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  hwy_amd::Gemm("DGEMM", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
 }
 
 HWYBLAS_AMD_API void strsm_(const char* side, const char* uplo, const char* transa,
                             const char* diag, const blasint* m, const blasint* n,
                             const float* alpha, const float* a, const blasint* lda,
                             float* b, const blasint* ldb) {
-  hwy_amd::TrsmHost("STRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+  hwy_amd::Trsm("STRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
 }
 
 HWYBLAS_AMD_API void dtrsm_(const char* side, const char* uplo, const char* transa,
                             const char* diag, const blasint* m, const blasint* n,
                             const double* alpha, const double* a, const blasint* lda,
                             double* b, const blasint* ldb) {
-  hwy_amd::TrsmHost("DTRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+  hwy_amd::Trsm("DTRSM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
 }
 
 HWYBLAS_AMD_API int hwyblas_dgemm_dev(char transa, char transb, int m, int n, int k,

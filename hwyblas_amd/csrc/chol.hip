@@ -219,7 +219,7 @@ void PotrfHost(const char* name, const char* uplo, const blasint* n, T* a, const
                blasint* info) {
   bool upper = false;
   *info = -PotrfInfo(*uplo, *n, *lda, &upper);
-  if (*info != 0) return ReportError(name, -*info);
+  if (*info != 0) return Reject(name, -*info);
   if (*n == 0) return;
   hipStream_t s = Context().stream;
   DevMatrix<T> ma(a, *n, *n, *lda, kSlotA, s);
@@ -233,7 +233,7 @@ void PotrsHost(const char* name, const char* uplo, const blasint* n, const blasi
                const T* a, const blasint* lda, T* b, const blasint* ldb, blasint* info) {
   bool upper = false;
   *info = -PosvInfo(*uplo, *n, *nrhs, *lda, *ldb, &upper);
-  if (*info != 0) return ReportError(name, -*info);
+  if (*info != 0) return Reject(name, -*info);
   if (*n == 0 || *nrhs == 0) return;
   hipStream_t s = Context().stream;
   DevMatrix<T> ma(const_cast<T*>(a), *n, *n, *lda, kSlotA, s);
@@ -249,7 +249,7 @@ void PosvHost(const char* name, const char* uplo, const blasint* n, const blasin
               const blasint* lda, T* b, const blasint* ldb, blasint* info) {
   bool upper = false;
   *info = -PosvInfo(*uplo, *n, *nrhs, *lda, *ldb, &upper);
-  if (*info != 0) return ReportError(name, -*info);
+  if (*info != 0) return Reject(name, -*info);
   if (*n == 0) return;
   hipStream_t s = Context().stream;
   DevMatrix<T> ma(a, *n, *n, *lda, kSlotA, s);

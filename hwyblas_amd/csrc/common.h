@@ -54,21 +54,31 @@ inline bool DecodeDiag(char c, bool* unit) {
 }
 
 // ---------------------------------------------------------------------------
-// Error reporting (errors.cpp)
+// Invalid-argument reports (errors.cpp)
 
-// Reports invalid argument `info` (1-based) of routine `name`: through the
-// installed handler, else cblas_xerbla with the mapped CBLAS position while a
-// CBLAS call is active on this thread, else xerbla_. Never aborts.
-void ReportError(const char* name, blasint info);
-bool CallErrorHandler(const char* name, blasint position);
-
-// Fortran argument i (1-based) of the routine running under a CBLAS wrapper
-// is CBLAS argument positions[i - 1].
-struct CblasContext {
-  const char* name = nullptr;
-  const int* positions = nullptr;
+// Whom a routine reports its first invalid argument to. A Fortran caller is
+// identified by the routine name, and positions are Fortran argument
+// numbers. A CBLAS caller is identified by the CBLAS name, and Fortran
+// argument i was made from CBLAS argument cblas_arg[i - 1] (the layout is
+// CBLAS argument 1), so the position reported is the one the program passed.
+struct Caller {
+  const char* name;
+  const unsigned char* cblas_arg = nullptr;
+  // Implicit: a Fortran routine name is a complete identity.
+  Caller(const char* fortran_name) : name(fortran_name) {}  // NOLINT
+  Caller(const char* cblas_name, const unsigned char* from_cblas)
+      : name(cblas_name), cblas_arg(from_cblas) {}
+  // The position reported for Fortran argument `info` (1-based).
+  blasint Position(blasint info) const;
 };
-void SetCblasContext(const CblasContext* context);
+
+// Reports invalid argument `info` (1-based, in the Fortran numbering of the
+// routine): to the installed handler if any, else printed by cblas_xerbla
+// (CBLAS callers) or xerbla_ (Fortran callers). Never aborts.
+void Reject(const Caller& who, blasint info);
+
+// Reports a CBLAS layout that is neither row- nor column-major (argument 1).
+void RejectLayout(const char* name, int layout);
 
 // Prints the HIP error and aborts: device failures have no BLAS error code,
 // and a routine that cannot run must not return as if it had.
@@ -88,6 +98,8 @@ void SetCblasContext(const CblasContext* context);
 
 // kSlotD: a dense copy of a structured operand (SYMM, TRMM). kSlotPackA,
 // kSlotPackB, kSlotProd: the complex GEMM's real packs and real product.
+// kSlotEmulated: the emulated DGEMM's statistics, slices and residues.
+// kSlotTune: the autotuner's operand memory.
 enum StagingSlot : int {
   kSlotA = 0,
   kSlotB = 1,
@@ -96,7 +108,9 @@ enum StagingSlot : int {
   kSlotPackA = 4,
   kSlotPackB = 5,
   kSlotProd = 6,
-  kNumSlots = 7
+  kSlotEmulated = 7,
+  kSlotTune = 8,
+  kNumSlots = 9
 };
 
 struct ThreadContext {
@@ -106,14 +120,13 @@ struct ThreadContext {
   size_t cap[kNumSlots] = {};
 };
 
-// Complex GEMV with op 'N', 'T', 'C' or 'R' (conjugated, not transposed),
-// Fortran-convention checks reported as `name` (gemv.hip).
-template <typename R>
-void ZgemvEntry(const char* name, char op, blasint m, blasint n, const R* alpha, const R* a,
-                blasint lda, const R* x, blasint incx, const R* beta, R* y, blasint incy);
-
 ThreadContext& Context();
-void* Staging(int slot, size_t bytes);
+// The calling thread's staging buffer `slot` on the current device, grown
+// to at least `bytes`. Growing waits for the thread's stream and, if given,
+// `user_stream` (work on either may still read the old buffer); `grown`
+// (optional) tells whether the buffer was reallocated (its contents lost).
+void* Staging(int slot, size_t bytes, hipStream_t user_stream = nullptr,
+              bool* grown = nullptr);
 
 // Whether p can be dereferenced by kernels on the current device (device or
 // managed memory); host memory, including pinned memory, is staged.

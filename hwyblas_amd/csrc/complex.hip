@@ -29,7 +29,7 @@
 // complex division (src/trsm-inl.h:60-70, conjugated diagonal for op = C).
 #include <algorithm>
 
-#include "common.h"
+#include "entries.h"
 
 namespace hwy_amd {
 namespace {
@@ -238,7 +238,7 @@ void CopyOut2D(T* dst, int64_t ld, const T* src, int64_t rows, int64_t cols, hip
 
 // Fortran entry: host or device operands; synchronizes.
 template <typename R>
-void ZgemmHost(const char* name, const char* transa, const char* transb, const blasint* mp,
+void ZgemmHost(const Caller& who, const char* transa, const char* transb, const blasint* mp,
                const blasint* np, const blasint* kp, const R* alpha, const R* a,
                const blasint* ldap, const R* b, const blasint* ldbp, const R* beta, R* c,
                const blasint* ldcp) {
@@ -247,7 +247,7 @@ void ZgemmHost(const char* name, const char* transa, const char* transb, const b
   ZOp oa, ob;
   const blasint info = ZgemmInfo(*transa, *transb, m, n, k, lda, ldb, ldc, &oa, &ob);
   if (info != 0) {
-    ReportError(name, info);
+    Reject(who, info);
     return;
   }
   if (m == 0 || n == 0) return;
@@ -291,14 +291,14 @@ void ZgemmHost(const char* name, const char* transa, const char* transb, const b
 
 // Device-pointer form: asynchronous on `stream`; returns the info.
 template <typename R>
-int ZgemmDev(const char* name, char transa, char transb, int m, int n, int k, const R* alpha,
+int ZgemmDev(const Caller& who, char transa, char transb, int m, int n, int k, const R* alpha,
              const R* a, int lda, const R* b, int ldb, const R* beta, R* c, int ldc,
              hipStream_t s) {
   using C2 = typename Vec2<R>::type;
   ZOp oa, ob;
   const blasint info = ZgemmInfo(transa, transb, m, n, k, lda, ldb, ldc, &oa, &ob);
   if (info != 0) {
-    ReportError(name, info);
+    Reject(who, info);
     return info;
   }
   if (m == 0 || n == 0) return 0;
@@ -548,7 +548,7 @@ void ZtrsmDevice(bool left, bool lower, ZOp op, bool unit, int64_t m, int64_t n,
 }
 
 template <typename R>
-void ZtrsmHost(const char* name, const char* sidep, const char* uplop, const char* transp,
+void ZtrsmHost(const Caller& who, const char* sidep, const char* uplop, const char* transp,
                const char* diagp, const blasint* mp, const blasint* np, const R* alpha,
                const R* a, const blasint* ldap, R* b, const blasint* ldbp) {
   using C2 = typename Vec2<R>::type;
@@ -558,7 +558,7 @@ void ZtrsmHost(const char* name, const char* sidep, const char* uplop, const cha
   const blasint info =
       ZtrsmInfo(*sidep, *uplop, *transp, *diagp, m, n, lda, ldb, &left, &lower, &op, &unit);
   if (info != 0) {
-    ReportError(name, info);
+    Reject(who, info);
     return;
   }
   if (m == 0 || n == 0) return;
@@ -587,14 +587,14 @@ void ZtrsmHost(const char* name, const char* sidep, const char* uplop, const cha
 }
 
 template <typename R>
-int ZtrsmDev(const char* name, char side, char uplo, char transa, char diag, int m, int n,
+int ZtrsmDev(const Caller& who, char side, char uplo, char transa, char diag, int m, int n,
              const R* alpha, const R* a, int lda, R* b, int ldb, hipStream_t s) {
   bool left, lower, unit;
   ZOp op;
   const blasint info =
       ZtrsmInfo(side, uplo, transa, diag, m, n, lda, ldb, &left, &lower, &op, &unit);
   if (info != 0) {
-    ReportError(name, info);
+    Reject(who, info);
     return info;
   }
   if (m == 0 || n == 0) return 0;
@@ -612,7 +612,7 @@ int ZtrsmDev(const char* name, char side, char uplo, char transa, char diag, int
 // all of n x n (twice the triangle's flops); the combining pass writes the
 // triangle only.
 template <typename R>
-void ZSyrkHost(const char* name, bool herm, const char* uplop, const char* transp,
+void ZSyrkHost(const Caller& who, bool herm, const char* uplop, const char* transp,
                const blasint* np, const blasint* kp, const R* alpha_in, const R* a,
                const blasint* ldap, const R* beta_in, R* c, const blasint* ldcp) {
   using C2 = typename Vec2<R>::type;
@@ -632,7 +632,7 @@ void ZSyrkHost(const char* name, bool herm, const char* uplop, const char* trans
   else if (lda < std::max(1, rows_a)) info = 7;
   else if (ldc < std::max(1, n)) info = 10;
   if (info != 0) {
-    ReportError(name, info);
+    Reject(who, info);
     return;
   }
   const R alpha[2] = {alpha_in[0], herm ? R(0) : alpha_in[1]};
@@ -674,6 +674,23 @@ void ZSyrkHost(const char* name, bool herm, const char* uplop, const char* trans
 }
 
 }  // namespace
+
+#define HWY_DEFINE_CORES(R)                                                                   \
+  void ComplexGemm(const Caller& who, const char* transa, const char* transb,                 \
+                   const blasint* m, const blasint* n, const blasint* k, const R* alpha,      \
+                   const R* a, const blasint* lda, const R* b, const blasint* ldb,            \
+                   const R* beta, R* c, const blasint* ldc) {                                 \
+    ZgemmHost<R>(who, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);          \
+  }                                                                                           \
+  void ComplexTrsm(const Caller& who, const char* side, const char* uplo,                     \
+                   const char* transa, const char* diag, const blasint* m, const blasint* n,  \
+                   const R* alpha, const R* a, const blasint* lda, R* b, const blasint* ldb) { \
+    ZtrsmHost<R>(who, side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);                 \
+  }
+HWY_DEFINE_CORES(float)
+HWY_DEFINE_CORES(double)
+#undef HWY_DEFINE_CORES
+
 }  // namespace hwy_amd
 
 using hwy_amd::blasint;

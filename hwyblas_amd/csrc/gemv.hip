@@ -12,7 +12,7 @@
 #include <algorithm>
 #include <vector>
 
-#include "common.h"
+#include "entries.h"
 
 namespace hwy_amd {
 namespace {
@@ -152,7 +152,7 @@ T* ContiguousVector(const T* v, int64_t len, int64_t inc, bool copy_in, int slot
 }
 
 template <typename T>
-void GemvHost(const char* name, const char* transp, const blasint* mp, const blasint* np,
+void GemvHost(const Caller& who, const char* transp, const blasint* mp, const blasint* np,
               const T* alphap, const T* a, const blasint* ldap, const T* x,
               const blasint* incxp, const T* betap, T* y, const blasint* incyp) {
   const blasint m = *mp, n = *np, lda = *ldap, incx = *incxp, incy = *incyp;
@@ -164,7 +164,7 @@ void GemvHost(const char* name, const char* transp, const blasint* mp, const bla
   else if (lda < std::max(1, m)) info = 6;
   else if (incx == 0) info = 8;
   else if (incy == 0) info = 11;
-  if (info != 0) return ReportError(name, info);
+  if (info != 0) return Reject(who, info);
   const T alpha = *alphap, beta = *betap;
   if (m == 0 || n == 0 || (alpha == T(0) && beta == T(1))) return;
 
@@ -211,7 +211,7 @@ void GemvHost(const char* name, const char* transp, const blasint* mp, const bla
 }
 
 // ---------------------------------------------------------------------------
-// Complex GEMV (cgemv_, zgemv_; the CBLAS forms go through ZgemvEntry). op:
+// Complex GEMV (cgemv_, zgemv_; the CBLAS forms go through ComplexGemv). op:
 // 'N', 'T', 'C', or 'R' = conjugated and not transposed (row-major CBLAS
 // ConjTrans is 'R' on the column-major storage, so that x and y need no
 // conjugated copies). Products are complex FMAs in ascending k; the store is
@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(256)
 }
 
 template <typename R>
-void ZgemvHost(const char* name, char opc, blasint m, blasint n, const R* alpha, const R* a,
+void ZgemvHost(const Caller& who, char opc, blasint m, blasint n, const R* alpha, const R* a,
                blasint lda, const R* x, blasint incx, const R* beta, R* y, blasint incy) {
   using C2 = typename Cplx2<R>::type;
   const char oc = char(opc & ~0x20);
@@ -319,7 +319,7 @@ void ZgemvHost(const char* name, char opc, blasint m, blasint n, const R* alpha,
   else if (lda < std::max(1, m)) info = 6;
   else if (incx == 0) info = 8;
   else if (incy == 0) info = 11;
-  if (info != 0) return ReportError(name, info);
+  if (info != 0) return Reject(who, info);
   const bool alpha0 = alpha[0] == R(0) && alpha[1] == R(0);
   const bool beta0 = beta[0] == R(0) && beta[1] == R(0);
   const bool beta1 = beta[0] == R(1) && beta[1] == R(0);
@@ -389,16 +389,20 @@ void ZgemvHost(const char* name, char opc, blasint m, blasint n, const R* alpha,
 
 }  // namespace
 
-template <typename R>
-void ZgemvEntry(const char* name, char op, blasint m, blasint n, const R* alpha, const R* a,
-                blasint lda, const R* x, blasint incx, const R* beta, R* y, blasint incy) {
-  ZgemvHost<R>(name, op, m, n, alpha, a, lda, x, incx, beta, y, incy);
-}
-template void ZgemvEntry<float>(const char*, char, blasint, blasint, const float*, const float*,
-                                blasint, const float*, blasint, const float*, float*, blasint);
-template void ZgemvEntry<double>(const char*, char, blasint, blasint, const double*,
-                                 const double*, blasint, const double*, blasint, const double*,
-                                 double*, blasint);
+#define HWY_DEFINE_CORES(T)                                                                   \
+  void Gemv(const Caller& who, const char* trans, const blasint* m, const blasint* n,         \
+            const T* alpha, const T* a, const blasint* lda, const T* x, const blasint* incx,  \
+            const T* beta, T* y, const blasint* incy) {                                       \
+    GemvHost(who, trans, m, n, alpha, a, lda, x, incx, beta, y, incy);                        \
+  }                                                                                           \
+  void ComplexGemv(const Caller& who, const char* op, const blasint* m, const blasint* n,     \
+                   const T* alpha, const T* a, const blasint* lda, const T* x,                \
+                   const blasint* incx, const T* beta, T* y, const blasint* incy) {           \
+    ZgemvHost<T>(who, *op, *m, *n, alpha, a, *lda, x, *incx, beta, y, *incy);                 \
+  }
+HWY_DEFINE_CORES(float)
+HWY_DEFINE_CORES(double)
+#undef HWY_DEFINE_CORES
 
 }  // namespace hwy_amd
 
@@ -410,14 +414,14 @@ HWYBLAS_AMD_API void sgemv_(const char* trans, const blasint* m, const blasint* 
                             const float* alpha, const float* a, const blasint* lda,
                             const float* x, const blasint* incx, const float* beta, float* y,
                             const blasint* incy) {
-  hwy_amd::GemvHost("SGEMV", trans, m, n, alpha, a, lda, x, incx, beta, y, incy);
+  hwy_amd::Gemv("SGEMV", trans, m, n, alpha, a, lda, x, incx, beta, y, incy);
 }
 
 HWYBLAS_AMD_API void dgemv_(const char* trans, const blasint* m, const blasint* n,
                             const double* alpha, const double* a, const blasint* lda,
                             const double* x, const blasint* incx, const double* beta,
                             double* y, const blasint* incy) {
-  hwy_amd::GemvHost("DGEMV", trans, m, n, alpha, a, lda, x, incx, beta, y, incy);
+  hwy_amd::Gemv("DGEMV", trans, m, n, alpha, a, lda, x, incx, beta, y, incy);
 }
 
 HWYBLAS_AMD_API void cgemv_(const char* trans, const blasint* m, const blasint* n,
@@ -426,7 +430,7 @@ HWYBLAS_AMD_API void cgemv_(const char* trans, const blasint* m, const blasint* 
                             const blasint* incy) {
   // 'R' is internal to the CBLAS row-major mapping, not a Fortran option.
   const char t = (*trans == 'R' || *trans == 'r') ? '?' : *trans;
-  hwy_amd::ZgemvEntry<float>("CGEMV", t, *m, *n, alpha, a, *lda, x, *incx, beta, y, *incy);
+  hwy_amd::ComplexGemv("CGEMV", &t, m, n, alpha, a, lda, x, incx, beta, y, incy);
 }
 
 HWYBLAS_AMD_API void zgemv_(const char* trans, const blasint* m, const blasint* n,
@@ -434,7 +438,7 @@ HWYBLAS_AMD_API void zgemv_(const char* trans, const blasint* m, const blasint* 
                             const double* x, const blasint* incx, const double* beta,
                             double* y, const blasint* incy) {
   const char t = (*trans == 'R' || *trans == 'r') ? '?' : *trans;
-  hwy_amd::ZgemvEntry<double>("ZGEMV", t, *m, *n, alpha, a, *lda, x, *incx, beta, y, *incy);
+  hwy_amd::ComplexGemv("ZGEMV", &t, m, n, alpha, a, lda, x, incx, beta, y, incy);
 }
 
 }  // extern "C"

@@ -403,7 +403,7 @@ Plan* MakeGemmPlan(int count, const hwyblas_gemm_problem* probs, const int* cfgs
     if (e != 0) {
       *info = e;
       *bad = i;
-      ReportError(sizeof(T) == 8 ? "DGEMM" : "SGEMM", e);
+      Reject(sizeof(T) == 8 ? "DGEMM" : "SGEMM", e);
       return nullptr;
     }
     GemmDesc<T>& d = descs[i];
@@ -489,7 +489,7 @@ Plan* MakeTrsmPlan(int count, const hwyblas_trsm_problem* probs, int* info, int*
     if (e != 0) {
       *info = e;
       *bad = i;
-      ReportError(sizeof(T) == 8 ? "DTRSM" : "STRSM", e);
+      Reject(sizeof(T) == 8 ? "DTRSM" : "STRSM", e);
       return nullptr;
     }
     TrsmDesc<T>& d = descs[i];

@@ -112,14 +112,14 @@ extern "C" {
 HWYBLAS_AMD_API void dgetrf_(const hwyblas_int* m, const hwyblas_int* n, double* a,
                              const hwyblas_int* lda, hwyblas_int* ipiv, hwyblas_int* info) {
   *info = hwy_amd::Getrf(*m, *n, a, *lda, ipiv);
-  if (*info < 0 && *info != -1000) hwy_amd::ReportError("DGETRF", -*info);
+  if (*info < 0 && *info != -1000) hwy_amd::Reject("DGETRF", -*info);
 }
 
 HWYBLAS_AMD_API void dgetrs_(const char* trans, const hwyblas_int* n, const hwyblas_int* nrhs,
                              const double* a, const hwyblas_int* lda, const hwyblas_int* ipiv,
                              double* b, const hwyblas_int* ldb, hwyblas_int* info) {
   *info = hwy_amd::Getrs(*trans, *n, *nrhs, a, *lda, ipiv, b, *ldb);
-  if (*info < 0) hwy_amd::ReportError("DGETRS", -*info);
+  if (*info < 0) hwy_amd::Reject("DGETRS", -*info);
 }
 
 HWYBLAS_AMD_API void dgesv_(const hwyblas_int* n, const hwyblas_int* nrhs, double* a,
@@ -133,7 +133,7 @@ HWYBLAS_AMD_API void dgesv_(const hwyblas_int* n, const hwyblas_int* nrhs, doubl
   else if (*ldb < std::max(1, N)) e = -7;
   if (e != 0) {
     *info = e;
-    hwy_amd::ReportError("DGESV", -e);
+    hwy_amd::Reject("DGESV", -e);
     return;
   }
   *info = hwy_amd::Getrf(N, N, a, *lda, ipiv);

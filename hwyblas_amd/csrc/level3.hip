@@ -13,7 +13,7 @@
 // always copy C in, so its other triangle goes back unchanged.
 #include <algorithm>
 
-#include "common.h"
+#include "entries.h"
 
 namespace hwy_amd {
 namespace {
@@ -127,7 +127,7 @@ struct OutputC {
 
 // src/level3.cc:25-72.
 template <typename T>
-void SymmHost(const char* name, const char* sidep, const char* uplop, const blasint* mp,
+void SymmHost(const Caller& who, const char* sidep, const char* uplop, const blasint* mp,
               const blasint* np, const T* alphap, const T* a, const blasint* ldap, const T* b,
               const blasint* ldbp, const T* betap, T* c, const blasint* ldcp) {
   const blasint m = *mp, n = *np, lda = *ldap, ldb = *ldbp, ldc = *ldcp;
@@ -143,7 +143,7 @@ void SymmHost(const char* name, const char* sidep, const char* uplop, const blas
   else if (lda < std::max(1, ra)) info = 7;
   else if (ldb < std::max(1, m)) info = 9;
   else if (ldc < std::max(1, m)) info = 12;
-  if (info != 0) return ReportError(name, info);
+  if (info != 0) return Reject(who, info);
   const T alpha = *alphap, beta = *betap;
   if (m == 0 || n == 0 || (alpha == T(0) && beta == T(1))) return;
   hipStream_t s = Context().stream;
@@ -165,7 +165,7 @@ void SymmHost(const char* name, const char* sidep, const char* uplop, const blas
 
 // src/level3.cc:81-139 (real SYRK: trans N, T or C).
 template <typename T>
-void SyrkHost(const char* name, const char* uplop, const char* transp, const blasint* np,
+void SyrkHost(const Caller& who, const char* uplop, const char* transp, const blasint* np,
               const blasint* kp, const T* alphap, const T* a, const blasint* ldap,
               const T* betap, T* c, const blasint* ldcp) {
   const blasint n = *np, k = *kp, lda = *ldap, ldc = *ldcp;
@@ -182,7 +182,7 @@ void SyrkHost(const char* name, const char* uplop, const char* transp, const bla
   else if (k < 0) info = 4;
   else if (lda < std::max(1, ra)) info = 7;
   else if (ldc < std::max(1, n)) info = 10;
-  if (info != 0) return ReportError(name, info);
+  if (info != 0) return Reject(who, info);
   const T alpha = *alphap, beta = *betap;
   const int part = lower ? 1 : 2;
   if (n == 0 || ((alpha == T(0) || k == 0) && beta == T(1))) return;
@@ -203,7 +203,7 @@ void SyrkHost(const char* name, const char* uplop, const char* transp, const bla
 // src/level3.cc:152-214 (real SYR2K): two passes over the triangle, the
 // second adding alpha op(B) op(A)^T to the first's result.
 template <typename T>
-void Syr2kHost(const char* name, const char* uplop, const char* transp, const blasint* np,
+void Syr2kHost(const Caller& who, const char* uplop, const char* transp, const blasint* np,
                const blasint* kp, const T* alphap, const T* a, const blasint* ldap, const T* b,
                const blasint* ldbp, const T* betap, T* c, const blasint* ldcp) {
   const blasint n = *np, k = *kp, lda = *ldap, ldb = *ldbp, ldc = *ldcp;
@@ -221,7 +221,7 @@ void Syr2kHost(const char* name, const char* uplop, const char* transp, const bl
   else if (lda < std::max(1, rows)) info = 7;
   else if (ldb < std::max(1, rows)) info = 9;
   else if (ldc < std::max(1, n)) info = 12;
-  if (info != 0) return ReportError(name, info);
+  if (info != 0) return Reject(who, info);
   const T alpha = *alphap, beta = *betap;
   const int part = lower ? 1 : 2;
   if (n == 0 || ((alpha == T(0) || k == 0) && beta == T(1))) return;
@@ -244,7 +244,7 @@ void Syr2kHost(const char* name, const char* uplop, const char* transp, const bl
 
 // src/level3.cc:219-268.
 template <typename T>
-void GemmtrHost(const char* name, const char* uplop, const char* transap, const char* transbp,
+void GemmtrHost(const Caller& who, const char* uplop, const char* transap, const char* transbp,
                 const blasint* np, const blasint* kp, const T* alphap, const T* a,
                 const blasint* ldap, const T* b, const blasint* ldbp, const T* betap, T* c,
                 const blasint* ldcp) {
@@ -264,7 +264,7 @@ void GemmtrHost(const char* name, const char* uplop, const char* transap, const 
   else if (lda < std::max(1, ra)) info = 8;
   else if (ldb < std::max(1, rb)) info = 10;
   else if (ldc < std::max(1, n)) info = 13;
-  if (info != 0) return ReportError(name, info);
+  if (info != 0) return Reject(who, info);
   const T alpha = *alphap, beta = *betap;
   const int part = lower ? 1 : 2;
   if (n == 0 || ((alpha == T(0) || k == 0) && beta == T(1))) return;
@@ -286,7 +286,7 @@ void GemmtrHost(const char* name, const char* uplop, const char* transap, const 
 // src/level3.cc:273-345: B := alpha op(A) B (left) or alpha B op(A), from a
 // copy of B's input; alpha == 0 writes zeros without reading A or B.
 template <typename T>
-void TrmmHost(const char* name, const char* sidep, const char* uplop, const char* transp,
+void TrmmHost(const Caller& who, const char* sidep, const char* uplop, const char* transp,
               const char* diagp, const blasint* mp, const blasint* np, const T* alphap,
               const T* a, const blasint* ldap, T* b, const blasint* ldbp) {
   const blasint m = *mp, n = *np, lda = *ldap, ldb = *ldbp;
@@ -306,7 +306,7 @@ void TrmmHost(const char* name, const char* sidep, const char* uplop, const char
   else if (n < 0) info = 6;
   else if (lda < std::max(1, ra)) info = 9;
   else if (ldb < std::max(1, m)) info = 11;
-  if (info != 0) return ReportError(name, info);
+  if (info != 0) return Reject(who, info);
   const T alpha = *alphap;
   if (m == 0 || n == 0) return;
   hipStream_t s = Context().stream;
@@ -338,6 +338,38 @@ void TrmmHost(const char* name, const char* sidep, const char* uplop, const char
 }
 
 }  // namespace
+
+#define HWY_DEFINE_CORES(T)                                                                   \
+  void Trmm(const Caller& who, const char* side, const char* uplo, const char* transa,        \
+            const char* diag, const blasint* m, const blasint* n, const T* alpha, const T* a, \
+            const blasint* lda, T* b, const blasint* ldb) {                                   \
+    TrmmHost(who, side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);                     \
+  }                                                                                           \
+  void Symm(const Caller& who, const char* side, const char* uplo, const blasint* m,          \
+            const blasint* n, const T* alpha, const T* a, const blasint* lda, const T* b,     \
+            const blasint* ldb, const T* beta, T* c, const blasint* ldc) {                    \
+    SymmHost(who, side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);                     \
+  }                                                                                           \
+  void Syrk(const Caller& who, const char* uplo, const char* trans, const blasint* n,         \
+            const blasint* k, const T* alpha, const T* a, const blasint* lda, const T* beta,  \
+            T* c, const blasint* ldc) {                                                       \
+    SyrkHost(who, uplo, trans, n, k, alpha, a, lda, beta, c, ldc);                            \
+  }                                                                                           \
+  void Syr2k(const Caller& who, const char* uplo, const char* trans, const blasint* n,        \
+             const blasint* k, const T* alpha, const T* a, const blasint* lda, const T* b,    \
+             const blasint* ldb, const T* beta, T* c, const blasint* ldc) {                   \
+    Syr2kHost(who, uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc);                   \
+  }                                                                                           \
+  void Gemmtr(const Caller& who, const char* uplo, const char* transa, const char* transb,    \
+              const blasint* n, const blasint* k, const T* alpha, const T* a,                 \
+              const blasint* lda, const T* b, const blasint* ldb, const T* beta, T* c,        \
+              const blasint* ldc) {                                                           \
+    GemmtrHost(who, uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc);         \
+  }
+HWY_DEFINE_CORES(float)
+HWY_DEFINE_CORES(double)
+#undef HWY_DEFINE_CORES
+
 }  // namespace hwy_amd
 
 using hwy_amd::blasint;
@@ -348,44 +380,44 @@ HWYBLAS_AMD_API void ssymm_(const char* side, const char* uplo, const blasint* m
                             const blasint* n, const float* alpha, const float* a,
                             const blasint* lda, const float* b, const blasint* ldb,
                             const float* beta, float* c, const blasint* ldc) {
-  hwy_amd::SymmHost("SSYMM", side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);
+  hwy_amd::Symm("SSYMM", side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);
 }
 HWYBLAS_AMD_API void dsymm_(const char* side, const char* uplo, const blasint* m,
                             const blasint* n, const double* alpha, const double* a,
                             const blasint* lda, const double* b, const blasint* ldb,
                             const double* beta, double* c, const blasint* ldc) {
-  hwy_amd::SymmHost("DSYMM", side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);
+  hwy_amd::Symm("DSYMM", side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);
 }
 HWYBLAS_AMD_API void ssyrk_(const char* uplo, const char* trans, const blasint* n,
                             const blasint* k, const float* alpha, const float* a,
                             const blasint* lda, const float* beta, float* c,
                             const blasint* ldc) {
-  hwy_amd::SyrkHost("SSYRK", uplo, trans, n, k, alpha, a, lda, beta, c, ldc);
+  hwy_amd::Syrk("SSYRK", uplo, trans, n, k, alpha, a, lda, beta, c, ldc);
 }
 HWYBLAS_AMD_API void dsyrk_(const char* uplo, const char* trans, const blasint* n,
                             const blasint* k, const double* alpha, const double* a,
                             const blasint* lda, const double* beta, double* c,
                             const blasint* ldc) {
-  hwy_amd::SyrkHost("DSYRK", uplo, trans, n, k, alpha, a, lda, beta, c, ldc);
+  hwy_amd::Syrk("DSYRK", uplo, trans, n, k, alpha, a, lda, beta, c, ldc);
 }
 HWYBLAS_AMD_API void ssyr2k_(const char* uplo, const char* trans, const blasint* n,
                              const blasint* k, const float* alpha, const float* a,
                              const blasint* lda, const float* b, const blasint* ldb,
                              const float* beta, float* c, const blasint* ldc) {
-  hwy_amd::Syr2kHost("SSYR2K", uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  hwy_amd::Syr2k("SSYR2K", uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
 }
 HWYBLAS_AMD_API void dsyr2k_(const char* uplo, const char* trans, const blasint* n,
                              const blasint* k, const double* alpha, const double* a,
                              const blasint* lda, const double* b, const blasint* ldb,
                              const double* beta, double* c, const blasint* ldc) {
-  hwy_amd::Syr2kHost("DSYR2K", uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  hwy_amd::Syr2k("DSYR2K", uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
 }
 HWYBLAS_AMD_API void sgemmtr_(const char* uplo, const char* transa, const char* transb,
                               const blasint* n, const blasint* k, const float* alpha,
                               const float* a, const blasint* lda, const float* b,
                               const blasint* ldb, const float* beta, float* c,
                               const blasint* ldc) {
-  hwy_amd::GemmtrHost("SGEMMTR", uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c,
+  hwy_amd::Gemmtr("SGEMMTR", uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c,
                       ldc);
 }
 HWYBLAS_AMD_API void dgemmtr_(const char* uplo, const char* transa, const char* transb,
@@ -393,7 +425,7 @@ HWYBLAS_AMD_API void dgemmtr_(const char* uplo, const char* transa, const char* 
                               const double* a, const blasint* lda, const double* b,
                               const blasint* ldb, const double* beta, double* c,
                               const blasint* ldc) {
-  hwy_amd::GemmtrHost("DGEMMTR", uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c,
+  hwy_amd::Gemmtr("DGEMMTR", uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c,
                       ldc);
 }
 HWYBLAS_AMD_API void sgemmt_(const char* uplo, const char* transa, const char* transb,
@@ -401,7 +433,7 @@ HWYBLAS_AMD_API void sgemmt_(const char* uplo, const char* transa, const char* t
                              const float* a, const blasint* lda, const float* b,
                              const blasint* ldb, const float* beta, float* c,
                              const blasint* ldc) {
-  hwy_amd::GemmtrHost("SGEMMT", uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c,
+  hwy_amd::Gemmtr("SGEMMT", uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c,
                       ldc);
 }
 HWYBLAS_AMD_API void dgemmt_(const char* uplo, const char* transa, const char* transb,
@@ -409,20 +441,20 @@ HWYBLAS_AMD_API void dgemmt_(const char* uplo, const char* transa, const char* t
                              const double* a, const blasint* lda, const double* b,
                              const blasint* ldb, const double* beta, double* c,
                              const blasint* ldc) {
-  hwy_amd::GemmtrHost("DGEMMT", uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c,
+  hwy_amd::Gemmtr("DGEMMT", uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c,
                       ldc);
 }
 HWYBLAS_AMD_API void strmm_(const char* side, const char* uplo, const char* transa,
                             const char* diag, const blasint* m, const blasint* n,
                             const float* alpha, const float* a, const blasint* lda, float* b,
                             const blasint* ldb) {
-  hwy_amd::TrmmHost("STRMM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+  hwy_amd::Trmm("STRMM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
 }
 HWYBLAS_AMD_API void dtrmm_(const char* side, const char* uplo, const char* transa,
                             const char* diag, const blasint* m, const blasint* n,
                             const double* alpha, const double* a, const blasint* lda,
                             double* b, const blasint* ldb) {
-  hwy_amd::TrmmHost("DTRMM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
+  hwy_amd::Trmm("DTRMM", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb);
 }
 
 }  // extern "C"

@@ -1633,7 +1633,7 @@ HWYBLAS_AMD_API int hwyblas_dgetrf_dev(int m, int n, double* a, int lda, int* ip
   else if (lda < std::max(1, m)) info = -4;
   else if (update < 0 || update > 2) info = -6;
   if (info != 0) {
-    ReportError("DGETRF", -info);
+    Reject("DGETRF", -info);
     return info;
   }
   if (m == 0 || n == 0) return 0;
@@ -1650,7 +1650,7 @@ HWYBLAS_AMD_API int hwyblas_dgetrs_dev(char trans, int n, int nrhs, const double
   else if (lda < std::max(1, n)) info = -5;
   else if (ldb < std::max(1, n)) info = -8;
   if (info != 0) {
-    ReportError("DGETRS", -info);
+    Reject("DGETRS", -info);
     return info;
   }
   if (n == 0 || nrhs == 0) return 0;
@@ -1680,7 +1680,7 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
   else if (update < 0 || update > 2) info = -12;
   else if (itmax < 0) info = -13;
   if (info != 0) {
-    ReportError("DGESVIR", -info);
+    Reject("DGESVIR", -info);
     return info;
   }
   if (n == 0 || nrhs == 0) {

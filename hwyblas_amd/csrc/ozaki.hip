@@ -789,31 +789,8 @@ int ModGemmVariant() {
 
 }  // namespace
 
-namespace {
-struct Workspace {
-  void* ptr = nullptr;
-  size_t cap = 0;
-  int device = -1;
-};
-thread_local Workspace t_ws[64];
-}  // namespace
-
 void* GetWorkspace(size_t bytes, hipStream_t stream, bool* grown) {
-  int dev = 0;
-  HWY_HIP_CHECK(hipGetDevice(&dev));
-  Workspace& w = t_ws[dev];
-  if (grown) *grown = w.cap < bytes;
-  if (w.cap < bytes) {
-    if (w.ptr) {
-      HWY_HIP_CHECK(hipStreamSynchronize(stream));
-      HWY_HIP_CHECK(hipFree(w.ptr));
-    }
-    const size_t rounded = (bytes + (size_t(64) << 20) - 1) & ~((size_t(64) << 20) - 1);
-    HWY_HIP_CHECK(hipMalloc(&w.ptr, rounded));
-    w.cap = rounded;
-    w.device = dev;
-  }
-  return w.ptr;
+  return Staging(kSlotEmulated, bytes, stream, grown);
 }
 
 struct PhaseTimer {
@@ -1058,7 +1035,7 @@ HWYBLAS_AMD_API int hwyblas_dgemm_emulated_dev(char transa, char transb, int m, 
   else if (ldb < std::max(1, rows_b)) info = 10;
   else if (ldc < std::max(1, m)) info = 13;
   if (info != 0) {
-    ReportError("DGEMM", info);
+    Reject("DGEMM", info);
     return info;
   }
   if (used_moduli) *used_moduli = 0;

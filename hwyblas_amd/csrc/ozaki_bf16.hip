@@ -506,7 +506,7 @@ HWYBLAS_AMD_API int hwyblas_dgemm_emulated_bf16_dev(char transa, char transb, in
   else if (ldb < std::max(1, rows_b)) info = 10;
   else if (ldc < std::max(1, m)) info = 13;
   if (info != 0) {
-    ReportError("DGEMM", info);
+    Reject("DGEMM", info);
     return info;
   }
   if (used_slices) *used_slices = 0;

@@ -16,9 +16,10 @@ namespace ozaki {
 // ---------------------------------------------------------------------------
 // Host side, defined once (ozaki.hip).
 
-// One grow-only device buffer per thread and device; a larger request frees
-// the old buffer (contents lost) after synchronizing `stream`, and sets
-// *grown when given.
+// The calling thread's emulated-DGEMM buffer on the current device (its
+// kSlotEmulated staging buffer): grow-only; a larger request frees the old
+// buffer (contents lost) after synchronizing `stream`, and sets *grown when
+// given.
 void* GetWorkspace(size_t bytes, hipStream_t stream, bool* grown = nullptr);
 
 // Optional per-phase timing (hwyblas_amd_profile_*): HIP events recorded on

@@ -1,14 +1,26 @@
 // Per-thread HIP context, device staging buffers, pointer classification and
 // the exported device utilities (memory, streams, events, the seeded fill).
 //
-// Each calling thread gets, per device, a stream of default priority and
-// three grow-only staging buffers (A, B, C), the device counterpart of
-// hwyBLAS's thread-local scratch slots (src/common.cc:46-64): reused across
-// calls, freed first when they must grow (to keep peak memory low), never
-// shrunk. The Fortran/CBLAS routines are therefore reentrant across threads
-// without locks, as hwyBLAS's are.
+// Each calling thread uses, per device, a stream of default priority and
+// grow-only staging buffers, the device counterpart of hwyBLAS's
+// thread-local scratch slots (src/common.cc:46-64): reused across calls,
+// freed first when they must grow (to keep peak memory low), never shrunk.
+// The Fortran/CBLAS routines are therefore reentrant across threads without
+// locks on the call path.
+//
+// The HIP resources are never released from a destructor: a thread-exit or
+// process-exit destructor may run after the HIP runtime (or a profiler
+// wrapped around it) has begun tearing down, and a HIP call from there can
+// crash the exit. Instead, a thread that ends hands its set of contexts to a
+// pool under a mutex, and the next thread that needs a set takes one from
+// the pool. Device memory and streams thus stay bounded by the largest
+// number of threads that called the library at once, and whatever is held
+// at process exit is reclaimed with the process.
 #include <stdio.h>
 #include <stdlib.h>
+
+#include <mutex>
+#include <vector>
 
 #include "common.h"
 
@@ -18,22 +30,50 @@ namespace {
 
 constexpr int kMaxDevices = 64;
 
-struct PerThread {
+struct ContextSet {
   ThreadContext ctx[kMaxDevices];
-  ~PerThread() {
-    for (int d = 0; d < kMaxDevices; ++d) {
-      ThreadContext& c = ctx[d];
-      if (c.device < 0) continue;
-      // Best effort at thread exit: the runtime may already be shutting down.
-      if (hipSetDevice(d) != hipSuccess) continue;
-      for (int s = 0; s < kNumSlots; ++s)
-        if (c.buf[s]) (void)hipFree(c.buf[s]);
-      if (c.stream) (void)hipStreamDestroy(c.stream);
-    }
-  }
 };
 
-thread_local PerThread t_state;
+// Heap-allocated and never destroyed, so that threads ending during static
+// destruction still find them.
+std::mutex& PoolMutex() {
+  static std::mutex* mu = new std::mutex;
+  return *mu;
+}
+std::vector<ContextSet*>& Pool() {
+  static std::vector<ContextSet*>* pool = new std::vector<ContextSet*>;
+  return *pool;
+}
+
+// The calling thread's set, taken from the pool (or created) on first use
+// and returned to the pool when the thread ends. Work a returned stream
+// still holds stays ordered before whatever the next owner enqueues on it.
+class ThreadSlot {
+ public:
+  ContextSet& Get() {
+    if (set_ == nullptr) {
+      std::lock_guard<std::mutex> lock(PoolMutex());
+      std::vector<ContextSet*>& pool = Pool();
+      if (pool.empty()) {
+        set_ = new ContextSet;
+      } else {
+        set_ = pool.back();
+        pool.pop_back();
+      }
+    }
+    return *set_;
+  }
+  ~ThreadSlot() {
+    if (set_ == nullptr) return;
+    std::lock_guard<std::mutex> lock(PoolMutex());
+    Pool().push_back(set_);
+  }
+
+ private:
+  ContextSet* set_ = nullptr;
+};
+
+thread_local ThreadSlot t_slot;
 
 }  // namespace
 
@@ -44,7 +84,7 @@ ThreadContext& Context() {
     fprintf(stderr, "hwyblas_amd: device %d out of range\n", dev);
     abort();
   }
-  ThreadContext& c = t_state.ctx[dev];
+  ThreadContext& c = t_slot.Get().ctx[dev];
   if (c.device < 0) {
     HWY_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     c.device = dev;
@@ -52,12 +92,15 @@ ThreadContext& Context() {
   return c;
 }
 
-void* Staging(int slot, size_t bytes) {
+void* Staging(int slot, size_t bytes, hipStream_t user_stream, bool* grown) {
   ThreadContext& c = Context();
   if (bytes == 0) bytes = 1;
+  if (grown != nullptr) *grown = c.cap[slot] < bytes;
   if (c.cap[slot] < bytes) {
     if (c.buf[slot]) {
+      // The old buffer may still be read by work queued on either stream.
       HWY_HIP_CHECK(hipStreamSynchronize(c.stream));
+      if (user_stream != nullptr) HWY_HIP_CHECK(hipStreamSynchronize(user_stream));
       HWY_HIP_CHECK(hipFree(c.buf[slot]));
       c.buf[slot] = nullptr;
       c.cap[slot] = 0;

@@ -537,3 +537,39 @@ def test_zsyrk_zherk_invalid_arguments(routine, change, info):
                   g["ldc"])
     assert cap.calls == [("Z" + routine.upper(), info)]
     assert np.array_equal(c, c0)
+
+
+def test_error_positions_under_concurrent_callers():
+    """Many threads report invalid arguments at once, some through CBLAS
+    row-major calls (remapped positions), some through the Fortran entry:
+    each report carries its own caller's name and position (no state is
+    shared between calls)."""
+    import collections
+    import threading
+
+    a = np.ones((8, 8), order="F")
+    c = np.zeros((8, 8), order="F")
+    # (call, expected report): row-major ldb < n is CBLAS position 11,
+    # row-major m < 0 is position 4, Fortran lda < m is 8.
+    cases = [
+        (lambda: H.cblas_gemm(H.RowMajor, H.NoTrans, H.NoTrans, 4, 3, 2, 1.0, a, 2, a, 2,
+                              0.0, c, 3), ("cblas_dgemm", 11)),
+        (lambda: H.cblas_gemm(H.RowMajor, H.NoTrans, H.NoTrans, -1, 3, 2, 1.0, a, 2, a, 3,
+                              0.0, c, 3), ("cblas_dgemm", 4)),
+        (lambda: H.gemm("N", "N", 4, 4, 4, 1.0, a, 2, a, 4, 0.0, c, 4), ("DGEMM", 8)),
+        (lambda: H.cblas_trsm(H.RowMajor, H.Left, H.Lower, H.NoTrans, H.NonUnit, 4, 3, 1.0, a,
+                              4, c, 2), ("cblas_dtrsm", 12)),
+    ]
+    rounds = 200
+    with H.ErrorCapture() as cap:
+        def worker(call):
+            for _ in range(rounds):
+                call()
+        threads = [threading.Thread(target=worker, args=(call,)) for call, _ in cases]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+    got = collections.Counter(cap.calls)
+    assert got == collections.Counter({want: rounds for _, want in cases})
+    assert np.all(c == 0)
