@@ -717,3 +717,70 @@ def test_planted_potrf_mistake_is_caught():
     low = _unit_lower(17, 17)
     f, info = oracle.potrf2("L", low @ low.T)
     assert info != 0 or not np.array_equal(np.tril(f), low)
+
+
+# ---- the checkers themselves, pinned in both directions -------------------
+
+@pytest.mark.parametrize("side,uplo,ta", list(itertools.product("LR", "LU", "NT")))
+def test_trsm_residual_rejects_a_perturbed_solution(side, uplo, ta):
+    """The residual checker behind every TRSM parity test accepts the solved
+    X and rejects the same X with one entry moved by 100 times the bound (in
+    the row whose residual that entry drives through the diagonal)."""
+    m, n = 60, 23
+    a, lda, b, ldb = trsm_operands(side, uplo, "N", m, n, np.float64, 91)
+    x = b.copy(order="F")
+    oracle.trsm(side, uplo, ta, "N", m, n, 1.0, a, lda, x, ldb)
+    assert trsm_residual_ratio(side, uplo, ta, "N", m, n, 1.0, a, lda, b, ldb, x) <= 1.0
+    ka = m if side == "L" else n
+    tol = 8.0 * (ka + 2) * np.finfo(np.float64).eps
+    p, w = 17, 5  # X(p, w) for the left side, X(w, p) for the right
+    dense = np.where(np.isnan(a[:ka, :ka]), 0.0, a[:ka, :ka])
+    opa = dense if ta == "N" else dense.T
+    if side == "L":
+        scale = np.abs(opa[p]) @ np.abs(x[:m, w]) + abs(b[p, w])
+        delta = 100 * tol * scale / abs(opa[p, p])
+        y = x.copy(order="F")
+        y[p, w] += delta
+    else:
+        scale = np.abs(x[w, :n]) @ np.abs(opa[:, p]) + abs(b[w, p])
+        delta = 100 * tol * scale / abs(opa[p, p])
+        y = x.copy(order="F")
+        y[w, p] += delta
+    assert trsm_residual_ratio(side, uplo, ta, "N", m, n, 1.0, a, lda, b, ldb, y) > 50.0
+
+
+def test_trsm_residual_rejects_the_skipped_reciprocal():
+    """Planted mistake 4 (the base kernel skips the diagonal's reciprocal)
+    is caught by the residual checker alone, with no fixture involved."""
+    for side, uplo, ta in itertools.product("LR", "LU", "NT"):
+        m, n = 70, 19
+        a, lda, b, ldb = trsm_operands(side, uplo, "N", m, n, np.float64, 92)
+        x = b.copy(order="F")
+        oracle.plant_mistake(4)
+        try:
+            oracle.trsm(side, uplo, ta, "N", m, n, 1.0, a, lda, x, ldb)
+        finally:
+            oracle.plant_mistake(0)
+        assert trsm_residual_ratio(side, uplo, ta, "N", m, n, 1.0, a, lda, b, ldb, x) > 1.0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("ta,tb", list(itertools.product("NT", "NT")))
+def test_entry_reference_equals_the_full_reference(ta, tb, dtype):
+    """gemm_ref_entries (behind every large-shape test, smoke and the bench)
+    gives, entry for entry, the same expected value and error scale as the
+    full double-double reference gemm_ref."""
+    m, n, k = 37, 29, 301
+    a = oracle.fill_uniform(m if ta == "N" else k, k if ta == "N" else m, 93, dtype=dtype)
+    b = oracle.fill_uniform(k if tb == "N" else n, n if tb == "N" else k, 94, dtype=dtype)
+    c = oracle.fill_uniform(m, n, 95, dtype=dtype)
+    for beta in (0.0, -0.75):
+        exp, bnd = oracle.gemm_ref(ta, tb, m, n, k, 1.25, a, a.shape[0], b, b.shape[0], beta,
+                                   c, m)
+        rng = np.random.default_rng(96)
+        rows = rng.integers(0, m, 200)
+        cols = rng.integers(0, n, 200)
+        e2, b2 = oracle.gemm_ref_entries(ta, tb, k, 1.25, a, a.shape[0], b, b.shape[0], beta, c,
+                                         m, rows, cols)
+        assert np.array_equal(e2, exp[rows, cols])
+        assert np.array_equal(b2, bnd[rows, cols])
