@@ -63,6 +63,51 @@ def gather(H, dC, n, rows, cols):
     return got
 
 
+def error_entries(n, samples, seed, full=16):
+    """The checked entries: `full` whole rows, `full` whole columns and
+    `samples` random entries of an n x n result."""
+    rng = np.random.default_rng(seed)
+    R = rng.choice(n, full, replace=False)
+    Cc = rng.choice(n, full, replace=False)
+    sr = rng.integers(0, n, samples)
+    sc = rng.integers(0, n, samples)
+    return R, Cc, sr, sc
+
+
+def reference_entries(oracle, ha, hb, n, R, Cc, sr, sc):
+    """Double-double reference (value, |A||B| scale) of the checked entries of
+    A B: the rows through a contiguous copy of A's rows, the columns and the
+    sampled entries through a transposed copy of A (contiguous dot products)."""
+    a_rows = np.asfortranarray(ha[R, :].T)  # op = T: op(A)(i, p) = a_rows[p + i n]
+    ri = np.repeat(np.arange(len(R), dtype=np.int32), n)
+    rj = np.tile(np.arange(n, dtype=np.int32), len(R))
+    e_r, b_r = oracle.gemm_ref_entries("T", "N", n, 1.0, a_rows, n, hb, n, 0.0, None, n, ri, rj)
+    del a_rows
+    haT = np.asfortranarray(ha.T)
+    ci = np.tile(np.arange(n, dtype=np.int32), len(Cc))
+    cj = np.repeat(np.asarray(Cc, dtype=np.int32), n)
+    e_c, b_c = oracle.gemm_ref_entries("T", "N", n, 1.0, haT, n, hb, n, 0.0, None, n, ci, cj)
+    e_s, b_s = oracle.gemm_ref_entries("T", "N", n, 1.0, haT, n, hb, n, 0.0, None, n, sr, sc)
+    del haT
+    return np.concatenate([e_r, e_c, e_s]), np.concatenate([b_r, b_c, b_s])
+
+
+def result_entries(H, dC, n, R, Cc, sr, sc):
+    """The same entries of a device result (whole matrix copied once)."""
+    c = dC.to_host()
+    out = np.concatenate([c[R, :].ravel(), c[:, Cc].T.ravel(), c[sr, sc]])
+    del c
+    return out
+
+
+def error_summary(got, exp, bnd):
+    """Max componentwise error |C - C_ref| / (|A||B|) and max plain relative
+    error |C - C_ref| / |C_ref| (entries with C_ref != 0)."""
+    err = np.abs(got - exp)
+    nz = exp != 0
+    return float(np.max(err / bnd)), float(np.max(err[nz] / np.abs(exp[nz])))
+
+
 def time_steps(H, stream, step, steps, warmup):
     for _ in range(warmup):
         step()
@@ -439,23 +484,21 @@ def distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, samples, nmod=0, st
         assert H.gemm_dev("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n, stream=stream,
                           dtype=np.float64) == 0
     nat_ms, _ = time_steps(H, stream, native_step, steps, 1)
-    rng = np.random.default_rng(321)
-    rows = rng.integers(0, n, samples)
-    cols = rng.integers(0, n, samples)
-    nat_vals = gather(H, dC, n, rows, cols)
-    used = [0]
+    R, Cc, sr, sc = error_entries(n, samples, 321)
+    nat_vals = result_entries(H, dC, n, R, Cc, sr, sc)
 
-    def emu_step():
+    def emu_step(report=False):
         if method == "bf16":
             info, u = H.dgemm_emulated_bf16_dev("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC,
-                                                n, slices=nmod, stream=stream)
+                                                n, slices=nmod, stream=stream, report=report)
         else:
             info, u = H.dgemm_emulated_dev("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n,
-                                           nmoduli=nmod, stream=stream)
+                                           nmoduli=nmod, stream=stream, report=report)
         assert info == 0
-        used[0] = u
+        return u
     emu_ms, _ = time_steps(H, stream, emu_step, steps, 1)
-    emu_vals = gather(H, dC, n, rows, cols)
+    used = [emu_step(report=True)]
+    emu_vals = result_entries(H, dC, n, R, Cc, sr, sc)
     if dist == "b":
         ha, hb = oracle.fill_pow2(n, n, 11), oracle.fill_pow2(n, n, 12)
     else:
@@ -463,13 +506,10 @@ def distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, samples, nmod=0, st
         if dist == "c":
             ha *= rs[:, None]
             hb *= cs[None, :]
-    exp, bnd = oracle.gemm_ref_entries("N", "N", n, 1.0, ha, n, hb, n, 0.0, None, n, rows, cols)
+    exp, bnd = reference_entries(oracle, ha, hb, n, R, Cc, sr, sc)
     del ha, hb
-    e_nat = float(np.max(np.abs(nat_vals - exp) / bnd))
-    e_emu = float(np.max(np.abs(emu_vals - exp) / bnd))
-    nz = np.abs(exp) > 0
-    r_nat = float(np.max(np.abs(nat_vals - exp)[nz] / np.abs(exp)[nz]))
-    r_emu = float(np.max(np.abs(emu_vals - exp)[nz] / np.abs(exp)[nz]))
+    e_nat, r_nat = error_summary(nat_vals, exp, bnd)
+    e_emu, r_emu = error_summary(emu_vals, exp, bnd)
     count_key = "slices" if method == "bf16" else "moduli"
     out = {"distribution": DISTRIBUTIONS.get(dist, "uniform[-1,1] (seeds 1, 2)"),
            "method": METHODS[method],
@@ -479,9 +519,12 @@ def distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, samples, nmod=0, st
            "emulated_max_componentwise_rel": e_emu, "native_max_componentwise_rel": e_nat,
            "emulated_max_rel": r_emu, "native_max_rel": r_nat,
            "ratio_emulated_to_native": e_emu / e_nat if e_nat > 0 else None,
-           "samples": int(samples)}
+           "ratio_rel_emulated_to_native": r_emu / r_nat if r_nat > 0 else None,
+           "checked": f"16 full rows, 16 full columns and {samples} random entries "
+                      f"({len(exp)} entries)"}
     log(f"dist {dist} ({method}): {used[0]} {count_key}, emulated {out['emulated_tflops']} TF vs native "
-        f"{out['native_tflops']} TF; componentwise error emulated {e_emu:.3e} native {e_nat:.3e}")
+        f"{out['native_tflops']} TF; componentwise error emulated {e_emu:.3e} native {e_nat:.3e}; "
+        f"relative error emulated {r_emu:.3e} native {r_nat:.3e}")
     return out
 
 
@@ -955,20 +998,18 @@ def main():
     nat_ms, _ = time_steps(H, stream, native_step, max(2, steps // 2), 1)
     nat_tf = flops / (nat_ms * 1e-3) / 1e12
     log(f"native DGEMM {nat_ms:.2f} ms/step = {nat_tf:.2f} TFLOP/s")
-    rng = np.random.default_rng(123)
-    rows = rng.integers(0, n, args.samples)
-    cols = rng.integers(0, n, args.samples)
-    nat_vals = gather(H, dC, n, rows, cols) if not args.no_check else None
+    R, Cc, sr, sc = error_entries(n, args.samples, 123)
+    nat_vals = result_entries(H, dC, n, R, Cc, sr, sc) if not args.no_check else None
 
-    # Emulated DGEMM (the headline).
-    used = [0]
-
-    def emu_step():
+    # Emulated DGEMM (the headline): timed without the count pointer (the
+    # call does not synchronize); one more call reports the count.
+    def emu_step(report=False):
         info, u = H.dgemm_emulated_dev("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n,
-                                       nmoduli=args.nmod, stream=stream)
+                                       nmoduli=args.nmod, stream=stream, report=report)
         assert info == 0
-        used[0] = u
+        return u
     emu_ms, wall = time_steps(H, stream, emu_step, steps, warmup)
+    used = [emu_step(report=True)]
     emu_tf = flops / (emu_ms * 1e-3) / 1e12
     log(f"emulated DGEMM ({used[0]} moduli) {emu_ms:.2f} ms/step = {emu_tf:.2f} TFLOP/s, "
         f"{emu_tf / nat_tf:.2f}x native; wall {wall:.2f}s")
@@ -985,19 +1026,23 @@ def main():
 
     err = None
     if not args.no_check:
-        emu_vals = gather(H, dC, n, rows, cols)
+        emu_vals = result_entries(H, dC, n, R, Cc, sr, sc)
         ha = oracle.fill_uniform(n, n, SEEDS[0])
         hb = oracle.fill_uniform(n, n, SEEDS[1])
-        exp, bnd = oracle.gemm_ref_entries("N", "N", n, 1.0, ha, n, hb, n, 0.0, None, n, rows,
-                                           cols)
+        exp, bnd = reference_entries(oracle, ha, hb, n, R, Cc, sr, sc)
         del ha, hb
-        e_nat = float(np.max(np.abs(nat_vals - exp) / bnd))
-        e_emu = float(np.max(np.abs(emu_vals - exp) / bnd))
+        e_nat, r_nat = error_summary(nat_vals, exp, bnd)
+        e_emu, r_emu = error_summary(emu_vals, exp, bnd)
         err = {"emulated_max_componentwise_rel": e_emu, "native_max_componentwise_rel": e_nat,
                "ratio_emulated_to_native": e_emu / e_nat if e_nat > 0 else None,
-               "samples": int(args.samples), "reference": "double-double (oracle)",
+               "emulated_max_rel": r_emu, "native_max_rel": r_nat,
+               "ratio_rel_emulated_to_native": r_emu / r_nat if r_nat > 0 else None,
+               "checked": f"16 full rows, 16 full columns and {args.samples} random entries "
+                          f"({len(exp)} entries)",
+               "reference": "double-double (oracle)",
                "bound_4(k+2)u": 4 * (n + 2) * 2.0 ** -53}
-        log(f"sampled error: emulated {e_emu:.3e}, native {e_nat:.3e}")
+        log(f"error: componentwise emulated {e_emu:.3e} native {e_nat:.3e}; relative emulated "
+            f"{r_emu:.3e} native {r_nat:.3e}")
 
     dists = {}
     if not args.no_check:

@@ -453,23 +453,29 @@ def gemm_dev_cfg(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stre
 
 
 def dgemm_emulated_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nmoduli=0,
-                       stream=None):
-    """Emulated DGEMM (INT8 Ozaki II); returns (info, moduli used)."""
+                       stream=None, report=True):
+    """Emulated DGEMM (INT8 Ozaki II); returns (info, moduli used; 0 = native FP64).
+
+    report=False passes no count pointer: the call then returns without
+    synchronizing and the count is None."""
     used = ctypes.c_int(0)
     info = lib().hwyblas_dgemm_emulated_dev(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(a),
                                             lda, _ptr(b), ldb, beta, _ptr(c), ldc, nmoduli,
-                                            stream, ctypes.byref(used))
-    return info, used.value
+                                            stream, ctypes.byref(used) if report else None)
+    return info, (used.value if report else None)
 
 
 def dgemm_emulated_bf16_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, slices=0,
-                            stream=None):
-    """Emulated DGEMM (BF16 slices, Ozaki I); returns (info, slices used)."""
+                            stream=None, report=True):
+    """Emulated DGEMM (BF16 slices, Ozaki I); returns (info, slices used; 0 = native FP64).
+
+    report=False: asynchronous, the count is None."""
     used = ctypes.c_int(0)
     info = lib().hwyblas_dgemm_emulated_bf16_dev(ta.encode(), tb.encode(), m, n, k, alpha,
                                                  _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc,
-                                                 slices, stream, ctypes.byref(used))
-    return info, used.value
+                                                 slices, stream,
+                                                 ctypes.byref(used) if report else None)
+    return info, (used.value if report else None)
 
 
 def gemm_configs():

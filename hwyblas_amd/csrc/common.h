@@ -188,6 +188,33 @@ void GemmDeviceKRange<float>(int krange, bool ta, bool tb, int64_t m, int64_t n,
                              int64_t ldb, float beta, float* c, int64_t ldc,
                              hipStream_t stream);
 
+// As GemmDevice<double>, doing the work only if *gate == 0 when the kernel
+// runs (a device-side decision made earlier on the stream; every workgroup
+// exits at once otherwise).
+template <typename T>
+void GemmDeviceGated(const int* gate, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                     T alpha, const T* a, int64_t lda, const T* b, int64_t ldb, T beta, T* c,
+                     int64_t ldc, hipStream_t stream);
+template <>
+void GemmDeviceGated<double>(const int* gate, bool ta, bool tb, int64_t m, int64_t n,
+                             int64_t k, double alpha, const double* a, int64_t lda,
+                             const double* b, int64_t ldb, double beta, double* c, int64_t ldc,
+                             hipStream_t stream);
+
+namespace ozaki {
+// The emulated DGEMMs (ozaki.hip: INT8 moduli; ozaki_bf16.hip: BF16 slices)
+// for m, n, k > 0 and alpha != 0, transpositions as GemmDevice. count > 0
+// fixes the moduli (slices); 0 lets the device choose them, or the native
+// kernel. Asynchronous unless `used` is non-null: it then receives the
+// count that ran (0: the native kernel) after synchronizing the stream.
+void EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                   const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                   double* c, int64_t ldc, int count, hipStream_t stream, int* used);
+void EmulatedDgemmBf16(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                       const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                       double* c, int64_t ldc, int count, hipStream_t stream, int* used);
+}  // namespace ozaki
+
 // Tile configuration GemmDevice uses for a shape (an index into the table of
 // gemm_configs.h).
 int ChooseGemmConfig(int dtype_bytes, bool ta, bool tb, int64_t m, int64_t n, int64_t k);

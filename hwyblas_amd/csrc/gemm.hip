@@ -13,7 +13,7 @@ namespace hwy_amd {
 void GemmLaunchF64(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
                    const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                    double* c, int64_t ldc, hipStream_t stream, int part = 0,
-                   int krange = 0);
+                   int krange = 0, const int* gate = nullptr);
 void GemmLaunchF32(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
                    const float* a, int64_t lda, const float* b, int64_t ldb, float beta,
                    float* c, int64_t ldc, hipStream_t stream, int part = 0,
@@ -61,6 +61,15 @@ void GemmDevice<float>(bool ta, bool tb, int64_t m, int64_t n, int64_t k, float 
                        float* c, int64_t ldc, hipStream_t stream) {
   const int cfg = ChooseGemmConfig(4, ta, tb, m, n, k);
   GemmLaunchF32(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+
+template <>
+void GemmDeviceGated<double>(const int* gate, bool ta, bool tb, int64_t m, int64_t n,
+                             int64_t k, double alpha, const double* a, int64_t lda,
+                             const double* b, int64_t ldb, double beta, double* c, int64_t ldc,
+                             hipStream_t stream) {
+  const int cfg = ChooseGemmConfig(8, ta, tb, m, n, k);
+  GemmLaunchF64(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, 0, 0, gate);
 }
 
 namespace {

@@ -5,9 +5,11 @@ namespace hwy_amd {
 
 void GemmLaunchF64(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
                    const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
-                   double* c, int64_t ldc, hipStream_t stream, int part, int krange) {
+                   double* c, int64_t ldc, hipStream_t stream, int part, int krange,
+                   const int* gate) {
   gemm_detail::GemmParams<double> prm{a, b, c, m, n, k, lda, ldb, ldc, alpha, beta, 0, 0, part};
   prm.krange = krange;
+  prm.gate = gate;
   gemm_detail::LaunchAny<double>(cfg, ta, tb, prm, stream);
 }
 

@@ -77,6 +77,9 @@ struct GemmParams {
   // meeting [kb, ke) run): 0 none; 1 kb = the tile's first row; 2 ke = its
   // last row + 1; 3 ke = its last column + 1; 4 kb = its first column.
   int krange = 0;
+  // Non-null: the kernel does its work only if *gate == 0 (a decision made
+  // on the device earlier on the stream).
+  const int* gate = nullptr;
 };
 
 constexpr int kXcds = 8;    // MI355X: 8 XCDs, each with its own L2
@@ -302,6 +305,7 @@ template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WAVES_M,
 __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
     __attribute__((amdgpu_waves_per_eu(OCC))) GemmKernel(GemmParams<T> p) {
   __shared__ T lds[TileLds<T, TA, TB, BM, BN, BK>::kElems];
+  if (p.gate && *p.gate != 0) return;  // uniform
   int tm, tn;
   if (p.tri_grid) {
     // Tile t of the triangle, row by row: r (r + 1) / 2 <= t < (r + 1) (r + 2) / 2.

@@ -43,12 +43,6 @@
 #include "gemm_kernel.h"
 
 namespace hwy_amd {
-namespace ozaki {
-int EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
-                  const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
-                  double* c, int64_t ldc, int nmod, hipStream_t stream);
-}  // namespace ozaki
-
 namespace lu {
 namespace {
 
@@ -840,7 +834,8 @@ void Update(int64_t m, int64_t n, int64_t k, const double* a21, const double* a1
             int64_t lda, int update, Work& w, hipStream_t s) {
   if (m <= 0 || n <= 0 || k <= 0) return;
   if (update == 1 && k >= kEmulatedMinK) {
-    ozaki::EmulatedDgemm(false, false, m, n, k, -1.0, a21, lda, a12, lda, 1.0, a22, lda, 0, s);
+    ozaki::EmulatedDgemm(false, false, m, n, k, -1.0, a21, lda, a12, lda, 1.0, a22, lda, 0, s,
+                         nullptr);
     return;
   }
   if (update == 2) {

@@ -19,12 +19,13 @@
 //     an exact wrap-around integer sum (error < N 256 2^-192 of P); its
 //     centred value times P gives C' to about one rounding. Finally
 //     C = alpha 2^-(sa_i + sb_j) C' + beta C with hwyBLAS's Store rules.
-// The number of moduli is given by the caller, or chosen from the operands:
-// a sampled estimate of how much the per-row fixed-point truncation can
-// perturb C relative to |A||B| sets bA, bB, hence N, so that the error stays
-// at the level of native FP64 accumulation (see ChooseModuli). Inputs with
-// NaN or Inf, which the integer scheme cannot carry, fall back to the native
-// FP64 kernel, as does k >= 131072.
+// The number of moduli is given by the caller, or chosen on the device from
+// exact statistics of every pair (i, j) (DecideKernel, ozaki_shared.h): the
+// smallest N whose error bound stays at native FP64's typical level, else
+// the smallest that keeps hwyBLAS's componentwise bound, else the native
+// FP64 kernel runs. Inputs with NaN or Inf, which the integer scheme cannot
+// carry, also run natively, as does k >= 131072. The decision never returns
+// to the host, so a call is asynchronous unless it reports the count.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -57,22 +58,25 @@ struct Crt {
 };
 
 // ---------------------------------------------------------------------------
-// Host-side constants
+// CRT constants for every N, built at compile time
 
-int64_t ModInverse(int64_t a, int64_t m) {  // a^-1 mod m, gcd(a, m) = 1
+constexpr int64_t ModInverse(int64_t a, int64_t m) {  // a^-1 mod m, gcd(a, m) = 1
   int64_t t = 0, nt = 1, r = m, nr = a % m;
   while (nr != 0) {
     const int64_t q = r / nr;
-    int64_t tmp = t - q * nt; t = nt; nt = tmp;
-    tmp = r - q * nr; r = nr; nr = tmp;
+    int64_t tmp = t - q * nt;
+    t = nt;
+    nt = tmp;
+    tmp = r - q * nr;
+    r = nr;
+    nr = tmp;
   }
   return t < 0 ? t + m : t;
 }
 
-Crt MakeCrt(int nmod) {
+constexpr Crt MakeCrt(int nmod) {
   Crt c{};
   c.nmod = nmod;
-  double logp = 0;
   c.P = 1;
   for (int l = 0; l < nmod; ++l) {
     c.p[l] = kModuli[l];
@@ -81,9 +85,7 @@ Crt MakeCrt(int nmod) {
     const uint64_t p32 = (uint64_t(1) << 32) % uint64_t(kModuli[l]);
     c.pow32[l] = double(p32);
     c.pow64[l] = double((p32 * p32) % uint64_t(kModuli[l]));
-    logp += std::log2(double(kModuli[l]));
   }
-  (void)logp;
   for (int l = 0; l < nmod; ++l) {
     const int64_t p = kModuli[l];
     // M_l mod p_l, then y_l = (M_l mod p_l)^-1 mod p_l.
@@ -102,6 +104,16 @@ Crt MakeCrt(int nmod) {
   }
   return c;
 }
+
+struct CrtTable {
+  Crt t[kMaxModuli + 1];  // t[N]: the first N moduli
+};
+constexpr CrtTable MakeCrtTable() {
+  CrtTable T{};
+  for (int nmod = 1; nmod <= kMaxModuli; ++nmod) T.t[nmod] = MakeCrt(nmod);
+  return T;
+}
+__device__ const CrtTable kCrtTable = MakeCrtTable();
 
 double Log2P(int nmod) {
   double s = 0;
@@ -164,7 +176,9 @@ __device__ __forceinline__ int ResidueOf(double v, int p, double inv_p) {
 }
 
 // Slices: Y[l][r][p] (row stride KP, plane stride RP * KP) = symmetric
-// residue mod p_l of round(x(r, p) 2^s_r), zero for r >= R or p >= K. Every
+// residue mod p_l of round(x(r, p) 2^s_r), zero for r >= R or p >= K, with
+// s_r = bits - E_r (bits = bA for op(A), side 0, bB for op(B)); nothing runs
+// when the decision is the native kernel. Every
 // thread produces 8 consecutive bytes of one row per modulus (16 threads
 // cover a row's 128 bytes: whole cache lines).
 // An integer 2^52 <= |v| < 2^96 as exact 32-bit limbs l0 + l1 2^32 + l2 2^64
@@ -250,7 +264,11 @@ __device__ __forceinline__ void SliceEight(const double (&x)[8], int sc, const C
 // and become zero). Block: 16 rows x 128 p.
 __global__ void __launch_bounds__(256)
     SliceRowsKernel(const double* x, int64_t ld, int64_t R, int64_t K, int64_t RP, int64_t KP,
-                    const int* s, Crt crt, int8_t* Y) {
+                    const int* s0, const EmuCtl* ctl, int side, int8_t* Y) {
+  const Decision d = ctl->dec;
+  if (d.count == 0) return;
+  const Crt& crt = kCrtTable.t[d.count];
+  const int bits = side == 0 ? d.ba : d.bb;
   const int tid = threadIdx.x;
   const int64_t r = int64_t(blockIdx.y) * 16 + (tid >> 4);
   const int64_t p0 = int64_t(blockIdx.x) * 128 + (tid & 15) * 8;
@@ -262,14 +280,18 @@ __global__ void __launch_bounds__(256)
     const double t = x[ok ? r * ld + p0 + q : 0];
     v[q] = ok ? t : 0.0;
   }
-  SliceEight(v, r < R ? s[r] : 0, crt, Y + r * KP + p0, RP * KP);
+  SliceEight(v, r < R ? bits + s0[r] : 0, crt, Y + r * KP + p0, RP * KP);
 }
 
 // x(r, p) = x[r + p * ld] (r contiguous): a 32 x 128 tile is transposed
 // through LDS, read coalesced along r.
 __global__ void __launch_bounds__(256)
     SliceColsKernel(const double* x, int64_t ld, int64_t R, int64_t K, int64_t RP, int64_t KP,
-                    const int* s, Crt crt, int8_t* Y) {
+                    const int* s0, const EmuCtl* ctl, int side, int8_t* Y) {
+  const Decision d = ctl->dec;
+  if (d.count == 0) return;
+  const Crt& crt = kCrtTable.t[d.count];
+  const int bits = side == 0 ? d.ba : d.bb;
   constexpr int TR = 32, TP = 128, PER = TR * TP / 256;
   __shared__ double T[TP][TR + 1];
   const int tid = threadIdx.x;
@@ -298,7 +320,7 @@ __global__ void __launch_bounds__(256)
     double v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) v[q] = T[pq + q][rr];
-    SliceEight(v, r < R ? s[r] : 0, crt, Y + r * KP + p0 + pq, RP * KP);
+    SliceEight(v, r < R ? bits + s0[r] : 0, crt, Y + r * KP + p0 + pq, RP * KP);
   }
 }
 
@@ -327,113 +349,13 @@ struct ModGemmArgs {
   int tiles_m, tiles_n;
   int p[kMaxModuli];
   double inv_p[kMaxModuli];
+  EmuCtl* ctl;
+  // The coarse product's epilogue: the exact sums of the rows of op(A)
+  // and the columns of op(B), and the m x n pairs that count.
+  const uint64_t* rs;
+  const uint64_t* cs;
+  int64_t m, n;
 };
-
-template <int BM, int BN, int BK, int WAVES_M, int WAVES_N>
-__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmKernel(ModGemmArgs g) {
-  constexpr int NT = 64 * WAVES_M * WAVES_N;
-  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
-  constexpr int TM = WM / 32, TN = WN / 32;
-  constexpr int CH = BK / 16;            // 16-byte chunks per row
-  constexpr int RPB = 16 / CH;           // rows per 256-byte bank row
-  constexpr int A_CHUNKS = BM * CH / NT; // chunks per thread
-  constexpr int B_CHUNKS = BN * CH / NT;
-  static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "load shape");
-  static_assert(WM % 32 == 0 && WN % 32 == 0 && BK % 32 == 0, "tile shape");
-  __shared__ i32x4 As[2][BM * CH];
-  __shared__ i32x4 Bs[2][BN * CH];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
-  const int l = blockIdx.y;
-  int tm, tn;
-  XcdTile(g.tiles_m, g.tiles_n, &tm, &tn);
-  const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
-  const int8_t* A = g.a + l * g.a_plane + m0 * g.kp;
-  const int8_t* B = g.b + l * g.b_plane + n0 * g.kp;
-
-  i32x4 ra[A_CHUNKS], rb[B_CHUNKS];
-  auto load = [&](int64_t k0) {
-#pragma unroll
-    for (int q = 0; q < A_CHUNKS; ++q) {
-      const int idx = tid + q * NT, row = idx / CH, c = idx % CH;
-      ra[q] = *reinterpret_cast<const i32x4*>(A + row * g.kp + k0 + c * 16);
-    }
-#pragma unroll
-    for (int q = 0; q < B_CHUNKS; ++q) {
-      const int idx = tid + q * NT, row = idx / CH, c = idx % CH;
-      rb[q] = *reinterpret_cast<const i32x4*>(B + row * g.kp + k0 + c * 16);
-    }
-  };
-  auto swz = [](int row, int c) { return row * CH + (c ^ ((row / RPB) % CH)); };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < A_CHUNKS; ++q) {
-      const int idx = tid + q * NT, row = idx / CH, c = idx % CH;
-      As[buf][swz(row, c)] = ra[q];
-    }
-#pragma unroll
-    for (int q = 0; q < B_CHUNKS; ++q) {
-      const int idx = tid + q * NT, row = idx / CH, c = idx % CH;
-      Bs[buf][swz(row, c)] = rb[q];
-    }
-  };
-
-  i32x16 acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = i32x16{};
-
-  const int ktiles = int(g.kp / BK);
-  load(0);
-  store(0);
-  __syncthreads();
-  const int lr = lane & 31, lh = lane >> 5;
-  for (int kt = 0; kt < ktiles; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < ktiles;
-    if (more) load(int64_t(kt + 1) * BK);
-#pragma unroll
-    for (int ks = 0; ks < BK / 32; ++ks) {
-      const int c = ks * 2 + lh;
-      i32x4 af[TM], bf[TN];
-#pragma unroll
-      for (int ti = 0; ti < TM; ++ti) af[ti] = As[cur][swz(wm0 + ti * 32 + lr, c)];
-#pragma unroll
-      for (int tj = 0; tj < TN; ++tj) bf[tj] = Bs[cur][swz(wn0 + tj * 32 + lr, c)];
-#pragma unroll
-      for (int ti = 0; ti < TM; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < TN; ++tj)
-          acc[ti][tj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
-    }
-    if (more) store(cur ^ 1);
-    __syncthreads();
-  }
-
-  // Epilogue: reduce mod p_l into bytes; four consecutive rows of a column
-  // form one 32-bit store (the plane is column-major, rows contiguous).
-  const int p = g.p[l];
-  const double ip = g.inv_p[l];
-  uint8_t* R = g.r + l * g.r_plane;
-#pragma unroll
-  for (int ti = 0; ti < TM; ++ti) {
-#pragma unroll
-    for (int tj = 0; tj < TN; ++tj) {
-      const int64_t col = n0 + wn0 + tj * 32 + lr;
-#pragma unroll
-      for (int grp = 0; grp < 4; ++grp) {
-        const int64_t row = m0 + wm0 + ti * 32 + 8 * grp + 4 * lh;
-        uint32_t word = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          word |= uint32_t(ModU(acc[ti][tj][grp * 4 + q], p, ip)) << (8 * q);
-        *reinterpret_cast<uint32_t*>(R + row + col * g.ldr) = word;
-      }
-    }
-  }
-}
 
 // The same product with the operand slices streamed straight into LDS
 // (global_load_lds_dwordx4: each wave writes 64 consecutive 16-byte chunks,
@@ -443,9 +365,13 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmKernel(ModGemmA
 // chunk that belongs at its LDS position under the swizzle (the inverse of
 // swz). The waves synchronise with s_barrier after waiting (vmcnt) for their
 // own DMA of the slice about to be read; the slot refilled after the barrier
-// is the one every wave finished reading in the previous iteration.
-template <int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES, bool PRIO = false,
-          int LATE = 0>
+// is the one every wave finished reading in the previous iteration. The
+// next slice's A loads go out after the first MFMA step and its B loads
+// after the second, not in one burst with every wave's first LDS reads.
+// EPI 0: the residue planes, reduced mod p_l (blockIdx.y = l; planes beyond
+// the decided count do nothing). EPI 1: the coarse product G of the
+// decision (plane 0), reduced to the maximum ratio and minimum G.
+template <int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES, int EPI>
 __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmDmaKernel(ModGemmArgs g) {
   constexpr int NT = 64 * WAVES_M * WAVES_N;
   constexpr int NW = WAVES_M * WAVES_N;
@@ -464,6 +390,7 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmDmaKernel(ModGe
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
   const int l = blockIdx.y;
+  if (EPI == 0 && l >= g.ctl->dec.count) return;  // uniform
   int tm, tn;
   XcdTile(g.tiles_m, g.tiles_n, &tm, &tn);
   const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
@@ -516,18 +443,12 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmDmaKernel(ModGe
     WaitSlices<PER_SLICE>(later);
     BarrierNoFence();
     const int next = kt + STAGES - 1;
-    if (LATE == 0 && next < ktiles) issue(next % STAGES, int64_t(next) * BK);
     const int cur = kt % STAGES;
-    if (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int ks = 0; ks < BK / 32; ++ks) {
-      // (LATE = 1: the next slice's loads go out after the first step's
-      // MFMAs, not in one burst with every wave's first LDS reads; 2: after
-      // the second; 3: A's after the first, B's after the second.)
       if (next < ktiles) {
-        if ((LATE == 1 && ks == 1) || (LATE == 2 && ks == 2)) issue(next % STAGES, int64_t(next) * BK);
-        if (LATE == 3 && ks == 1) issue_a(next % STAGES, int64_t(next) * BK);
-        if (LATE == 3 && ks == 2) issue_b(next % STAGES, int64_t(next) * BK);
+        if (ks == 1) issue_a(next % STAGES, int64_t(next) * BK);
+        if (ks == 2) issue_b(next % STAGES, int64_t(next) * BK);
       }
       const int c = ks * 2 + lh;
       i32x4 af[TM], bf[TN];
@@ -541,129 +462,43 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmDmaKernel(ModGe
         for (int tj = 0; tj < TN; ++tj)
           acc[ti][tj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
     }
-    if (PRIO) __builtin_amdgcn_s_setprio(0);
   }
 
-  const int p = g.p[l];
-  const double ip = g.inv_p[l];
-  uint8_t* R = g.r + l * g.r_plane;
+  if (EPI == 1) {
+    // max (cs_j + rs_i) 2^-27 / G_ij and min G_ij over the pairs whose row
+    // and column are not zero; one atomic each per wave.
+    uint64_t xmax = 0;
+    int gmin = 0x7FFFFFFF;
 #pragma unroll
-  for (int ti = 0; ti < TM; ++ti) {
+    for (int ti = 0; ti < TM; ++ti)
 #pragma unroll
-    for (int tj = 0; tj < TN; ++tj) {
-      const int64_t col = n0 + wn0 + tj * 32 + lr;
+      for (int tj = 0; tj < TN; ++tj) {
+        const int64_t col = n0 + wn0 + tj * 32 + lr;
+        const uint64_t cj = col < g.n ? g.cs[col] : 0;
 #pragma unroll
-      for (int grp = 0; grp < 4; ++grp) {
-        const int64_t row = m0 + wm0 + ti * 32 + 8 * grp + 4 * lh;
-        uint32_t word = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          word |= uint32_t(ModU(acc[ti][tj][grp * 4 + q], p, ip)) << (8 * q);
-        *reinterpret_cast<uint32_t*>(R + row + col * g.ldr) = word;
+        for (int e = 0; e < 16; ++e) {
+          const int64_t row = m0 + wm0 + ti * 32 + 8 * (e / 4) + 4 * lh + e % 4;
+          const uint64_t ri = row < g.m ? g.rs[row] : 0;
+          if (ri == 0 || cj == 0) continue;
+          const int G = acc[ti][tj][e];
+          const double x = G == 0 ? __longlong_as_double(int64_t(kInfBits))
+                                  : (double(cj) + double(ri)) * 0x1p-27 / double(G);
+          const uint64_t xb = uint64_t(__double_as_longlong(x));
+          xmax = xb > xmax ? xb : xmax;
+          gmin = G < gmin ? G : gmin;
+        }
       }
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint64_t ox = __shfl_xor(xmax, off, 64);
+      const int og = __shfl_xor(gmin, off, 64);
+      xmax = ox > xmax ? ox : xmax;
+      gmin = og < gmin ? og : gmin;
     }
-  }
-}
-
-// As ModGemmDmaKernel, with the operand fragments double-buffered in
-// registers so that the LDS latency and the block barrier are hidden behind
-// MFMAs: the fragments of step ks + 1 are read while step ks multiplies, and
-// before the last step of slice kt the waves wait for slice kt + 1's DMA,
-// meet at the barrier, refill the slot of slice kt - 1 (read by every wave
-// in the previous iteration) and read slice kt + 1's first fragments.
-template <int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES>
-__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) ModGemmPipeKernel(ModGemmArgs g) {
-  constexpr int NT = 64 * WAVES_M * WAVES_N;
-  constexpr int NW = WAVES_M * WAVES_N;
-  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
-  constexpr int TM = WM / 32, TN = WN / 32;
-  constexpr int CH = BK / 16;
-  constexpr int RPB = 16 / CH;
-  constexpr int KS = BK / 32;  // MFMA k steps per slice
-  constexpr int A_DMA = BM * CH / NT;
-  constexpr int B_DMA = BN * CH / NT;
-  constexpr int PER_SLICE = A_DMA + B_DMA;
-  static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "load shape");
-  static_assert(STAGES >= 3 && PER_SLICE * 3 < 64, "pipeline");
-  static_assert(KS % 2 == 0, "each slice starts in fragment buffer 0");
-  __shared__ i32x4 As[STAGES][BM * CH];
-  __shared__ i32x4 Bs[STAGES][BN * CH];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
-  const int l = blockIdx.y;
-  int tm, tn;
-  XcdTile(g.tiles_m, g.tiles_n, &tm, &tn);
-  const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
-  const int8_t* A = g.a + l * g.a_plane + m0 * g.kp;
-  const int8_t* B = g.b + l * g.b_plane + n0 * g.kp;
-
-  int64_t a_off[A_DMA], b_off[B_DMA];
-#pragma unroll
-  for (int q = 0; q < A_DMA; ++q) {
-    const int P = (q * NW + wave) * 64 + lane;
-    const int row = P / CH, c = (P % CH) ^ ((row / RPB) % CH);
-    a_off[q] = int64_t(row) * g.kp + c * 16;
-  }
-#pragma unroll
-  for (int q = 0; q < B_DMA; ++q) {
-    const int P = (q * NW + wave) * 64 + lane;
-    const int row = P / CH, c = (P % CH) ^ ((row / RPB) % CH);
-    b_off[q] = int64_t(row) * g.kp + c * 16;
-  }
-  auto issue = [&](int stage, int64_t k0) {
-#pragma unroll
-    for (int q = 0; q < A_DMA; ++q) DmaToLds(A + a_off[q] + k0, &As[stage][(q * NW + wave) * 64]);
-#pragma unroll
-    for (int q = 0; q < B_DMA; ++q) DmaToLds(B + b_off[q] + k0, &Bs[stage][(q * NW + wave) * 64]);
-  };
-  auto swz = [](int row, int c) { return row * CH + (c ^ ((row / RPB) % CH)); };
-  const int lr = lane & 31, lh = lane >> 5;
-  i32x4 fa[2][TM], fb[2][TN];
-  auto read_frags = [&](int buf, int stage, int ks) {
-    const int c = ks * 2 + lh;
-#pragma unroll
-    for (int ti = 0; ti < TM; ++ti) fa[buf][ti] = As[stage][swz(wm0 + ti * 32 + lr, c)];
-#pragma unroll
-    for (int tj = 0; tj < TN; ++tj) fb[buf][tj] = Bs[stage][swz(wn0 + tj * 32 + lr, c)];
-  };
-
-  i32x16 acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = i32x16{};
-
-  const int ktiles = int(g.kp / BK);
-#pragma unroll
-  for (int s = 0; s < STAGES - 1; ++s)
-    if (s < ktiles) issue(s, int64_t(s) * BK);
-  WaitSlices<PER_SLICE>(min(STAGES - 2, ktiles - 1));
-  BarrierNoFence();
-  read_frags(0, 0, 0);
-
-  for (int kt = 0; kt < ktiles; ++kt) {
-    const int cur = kt % STAGES;
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const int fc = ks & 1;  // KS is even: slice kt always starts in buffer 0
-      if (ks + 1 < KS) {
-        read_frags(fc ^ 1, cur, ks + 1);
-      } else if (kt + 1 < ktiles) {
-        // Slices issued after kt + 1 so far: kt + 2 .. kt + STAGES - 2.
-        WaitSlices<PER_SLICE>(min(STAGES - 3, ktiles - 2 - kt));
-        BarrierNoFence();
-        const int next = kt + STAGES - 1;
-        if (next < ktiles) issue(next % STAGES, int64_t(next) * BK);
-        read_frags(fc ^ 1, (kt + 1) % STAGES, 0);
-      }
-#pragma unroll
-      for (int ti = 0; ti < TM; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < TN; ++tj)
-          acc[ti][tj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[fc][ti], fb[fc][tj], acc[ti][tj],
-                                                               0, 0, 0);
+    if (lane == 0) {
+      if (xmax != 0) atomicMax(reinterpret_cast<unsigned long long*>(&g.ctl->xmax_bits), xmax);
+      if (gmin != 0x7FFFFFFF) atomicMin(&g.ctl->gmin, gmin);
     }
+    return;
   }
 
   const int p = g.p[l];
@@ -726,8 +561,12 @@ __device__ __forceinline__ double CrtValue(double (&S)[kLimbs], double P) {
 
 __global__ void __launch_bounds__(256)
     ReconstructKernel(const uint8_t* R, int64_t r_plane, int64_t ldr, int64_t m, int64_t n,
-                      const int* sa, const int* sb, Crt crt, double alpha, double beta,
-                      double* c, int64_t ldc) {
+                      const int* s0a, const int* s0b, const EmuCtl* ctl, double alpha,
+                      double beta, double* c, int64_t ldc) {
+  const Decision d = ctl->dec;
+  if (d.count == 0) return;
+  const Crt& crt = kCrtTable.t[d.count];
+  const int bits = d.ba + d.bb;
   const int64_t quads = (m + 3) / 4;
   const int64_t total = quads * n;
   for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
@@ -750,12 +589,12 @@ __global__ void __launch_bounds__(256)
         for (int t = 0; t < kLimbs; ++t) S[q][t] = __builtin_fma(cl, crt.frac[l][t], S[q][t]);
       }
     }
-    const int sbj = sb[j];
+    const int sbj = bits + s0b[j];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int64_t i = i0 + q;
       if (i >= m) break;
-      const double x = ldexp(CrtValue(S[q], crt.P), -(sa[i] + sbj));
+      const double x = ldexp(CrtValue(S[q], crt.P), -(s0a[i] + sbj));
       double* cij = c + i + j * ldc;
       double v = x * alpha;
       if (beta != 0.0) v = (beta == 1.0) ? v + *cij : __builtin_fma(*cij, beta, v);
@@ -764,28 +603,9 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-constexpr int kTileM = 256, kTileN = 256, kTileK = 64;
+constexpr int kTileM = 256, kTileN = 256;
 // k of the padded slices: a multiple of every variant's k step.
 constexpr int kPadK = 128;
-
-// INT8 GEMM variant (HWYBLAS_AMD_INT8_KERNEL). All stream the operand slices
-// and give the same bits; they differ in schedule. 9 (default, fastest on
-// MI355X at m = n = k = 16384): DMA into LDS, 8 waves of 128 x 64, k in
-// steps of 128 through two LDS stages, the next slice's A loads issued
-// after the first MFMA step and its B loads after the second (not in one
-// burst with every wave's first LDS reads). 3: the same with all loads
-// issued right after the barrier; 0: k steps of 64 and four stages; 1:
-// slices staged through registers; 2: 4 waves of 128 x 128; 4: three stages
-// of 64; 5: fragments double-buffered in registers; 6: as 3 with the waves'
-// priority raised while they issue MFMAs; 7: loads after the first MFMA
-// step; 8: after the second.
-int ModGemmVariant() {
-  static const int v = [] {
-    const char* e = getenv("HWYBLAS_AMD_INT8_KERNEL");
-    return (e && *e >= '0' && *e <= '9') ? *e - '0' : 9;
-  }();
-  return v;
-}
 
 }  // namespace
 
@@ -811,132 +631,162 @@ void Mark(int phase_boundary, hipStream_t stream) {
 
 void ResetPhases() { t_timer.recorded = false; }
 
-// Chooses the number of moduli from the sampled statistics: the per-row
-// fixed-point truncation perturbs C(i, j) by at most
-//   2^-(bA+1) maxrow|a_i| sum_p |b_pj| + 2^-(bB+1) maxcol|b_j| sum_p |a_ip|,
-// and native FP64 accumulation's typical error is about sqrt(k) u
-// sum_p |a_ip||b_pj|; the smallest N whose bits keep the first within the
-// second on every sampled (i, j) is used.
-int ChooseModuli(const SampleOut& s, int64_t k) {
-  double worst_a, worst_b, worst_ab;
-  WorstRatios(s, k, &worst_a, &worst_b, &worst_ab);
-  const double u = 0x1p-53;
-  const double target = std::sqrt(double(k)) * u;
-  for (int nmod = 8; nmod <= kMaxModuli; ++nmod) {
-    const int bits = TotalBits(nmod, k);
-    const int ba = bits / 2, bb = bits - ba;
-    const double err = std::ldexp(worst_a, -(ba + 1)) + std::ldexp(worst_b, -(bb + 1));
-    if (err <= target) return nmod;
-  }
-  return kMaxModuli;
+size_t StatsBytes(int64_t m, int64_t n) {
+  return AlignUp(sizeof(EmuCtl), 256) + AlignUp(sizeof(uint64_t) * (m + n), 256) * 2 +
+         AlignUp(sizeof(int) * m, 256) + AlignUp(sizeof(int) * n, 256);
 }
 
-int EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
-                  const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
-                  double* c, int64_t ldc, int nmod, hipStream_t stream) {
-  ResetPhases();
-  if (k >= 131072) {
-    GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-    return 0;
-  }
-  // op(A)(i, p) = a[i * a_rs + p * a_cs]; op(B)(p, j) = b[p * b_rs + j * b_cs].
-  const int64_t a_rs = ta ? lda : 1, a_cs = ta ? 1 : lda;
-  const int64_t b_rs = tb ? ldb : 1, b_cs = tb ? 1 : ldb;
-  const int64_t MP = AlignUp(m, kTileM), NP = AlignUp(n, kTileN), KP = AlignUp(k, kPadK);
+EmuStats CarveStats(void* base, int64_t m, int64_t n) {
+  char* p = static_cast<char*>(base);
+  EmuStats st;
+  st.ctl = reinterpret_cast<EmuCtl*>(p);
+  p += AlignUp(sizeof(EmuCtl), 256);
+  st.rowmax = reinterpret_cast<uint64_t*>(p);
+  st.sums = st.rowmax + (m + n);  // contiguous with rowmax: zeroed together
+  p += AlignUp(sizeof(uint64_t) * (m + n), 256) * 2;
+  st.sa = reinterpret_cast<int*>(p);
+  p += AlignUp(sizeof(int) * m, 256);
+  st.sb = reinterpret_cast<int*>(p);
+  return st;
+}
 
-  // Workspace layout.
-  size_t off = 0;
-  auto carve = [&](size_t bytes) {
-    const size_t o = off;
-    off = AlignUp(off + bytes, 256);
-    return o;
+namespace {
+
+int TotalBitsTable(int nmod, int64_t k) { return std::max(TotalBits(nmod, k), 2); }
+
+}  // namespace
+
+void Decide(int kind, int forced, const EmuOperands& op, const EmuStats& st, int scale_bits,
+            uint8_t* coarse_a, uint8_t* coarse_b, int64_t MPc, int64_t NPc, int64_t KPc,
+            hipStream_t stream) {
+  const int64_t m = op.m, n = op.n, k = op.k;
+  // The sums follow the maxima directly: both are zeroed in one pass.
+  const int64_t words = 2 * (m + n);
+  hipLaunchKernelGGL(InitStatsKernel, dim3(unsigned(std::min<int64_t>((words + 255) / 256, 4096))),
+                     dim3(256), 0, stream, st.rowmax, words, st.ctl);
+  const int64_t pchunk = 512;
+  auto rowmax_launch = [&](const double* x, int64_t rs, int64_t cs, int64_t R, uint64_t* out) {
+    const unsigned chunks = unsigned((k + pchunk - 1) / pchunk);
+    if (rs == 1)
+      hipLaunchKernelGGL(RowAbsMaxKernel<true>, dim3(unsigned((R + 255) / 256), chunks),
+                         dim3(256), 0, stream, x, cs, R, k, pchunk, out);
+    else
+      hipLaunchKernelGGL(RowAbsMaxKernel<false>, dim3(unsigned((R + 3) / 4), chunks), dim3(256),
+                         0, stream, x, rs, R, k, pchunk, out);
+    HWY_HIP_CHECK(hipGetLastError());
   };
-  const size_t o_rowmax = carve(sizeof(uint64_t) * (m + n + 2));
-  const size_t o_sa = carve(sizeof(int) * m);
-  const size_t o_sb = carve(sizeof(int) * n);
-  const size_t o_sample = carve(sizeof(SampleOut));
-  const size_t o_a8 = carve(size_t(kMaxModuli) * MP * KP);
-  const size_t o_b8 = carve(size_t(kMaxModuli) * NP * KP);
-  const size_t o_r = carve(size_t(kMaxModuli) * MP * NP);
-  char* ws = static_cast<char*>(GetWorkspace(off, stream));
-  uint64_t* rowmax = reinterpret_cast<uint64_t*>(ws + o_rowmax);
-  uint64_t* colmax = rowmax + m;
-  uint64_t* gmax = colmax + n;  // [0]: A, [1]: B
-  int* sa = reinterpret_cast<int*>(ws + o_sa);
-  int* sb = reinterpret_cast<int*>(ws + o_sb);
-  SampleOut* sample = reinterpret_cast<SampleOut*>(ws + o_sample);
+  rowmax_launch(op.a, op.a_rs, op.a_cs, m, st.rowmax);
+  // op(B)^T(j, p) = b[j * b_cs + p * b_rs]
+  rowmax_launch(op.b, op.b_cs, op.b_rs, n, st.rowmax + m);
+  hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((m + 255) / 256)), dim3(256), 0, stream,
+                     st.rowmax, m, scale_bits, st.sa, &st.ctl->amax_bits);
+  hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream,
+                     st.rowmax + m, n, scale_bits, st.sb, &st.ctl->bmax_bits);
+  HWY_HIP_CHECK(hipGetLastError());
+  if (forced <= 0) {
+    // Coarse planes (value 2^(7 - E) |x|) and exact sums, then their product.
+    const int bias = 7 - scale_bits;
+    auto coarse = [&](const double* x, int64_t rs, int64_t cs, int64_t R, int64_t RP,
+                      const int* s, uint8_t* y, uint64_t* sums) {
+      if (rs == 1)
+        hipLaunchKernelGGL(CoarseColsKernel, dim3(unsigned(KPc / 128), unsigned(RP / 32)),
+                           dim3(256), 0, stream, x, cs, R, k, RP, KPc, s, bias, y, sums);
+      else
+        hipLaunchKernelGGL(CoarseRowsKernel, dim3(unsigned(KPc / 128), unsigned(RP / 16)),
+                           dim3(256), 0, stream, x, rs, R, k, RP, KPc, s, bias, y, sums);
+      HWY_HIP_CHECK(hipGetLastError());
+    };
+    coarse(op.a, op.a_rs, op.a_cs, m, MPc, st.sa, coarse_a, st.sums);
+    coarse(op.b, op.b_cs, op.b_rs, n, NPc, st.sb, coarse_b, st.sums + m);
+    ModGemmArgs g{};
+    g.a = reinterpret_cast<const int8_t*>(coarse_a);
+    g.b = reinterpret_cast<const int8_t*>(coarse_b);
+    g.kp = KPc;
+    g.tiles_m = int(MPc / kTileM);
+    g.tiles_n = int(NPc / kTileN);
+    g.ctl = st.ctl;
+    g.rs = st.sums;
+    g.cs = st.sums + m;
+    g.m = m;
+    g.n = n;
+    hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2, 1>),
+                       dim3(unsigned(g.tiles_m * g.tiles_n), 1u), dim3(512), 0, stream, g);
+    HWY_HIP_CHECK(hipGetLastError());
+  }
+  DecideArgs d{};
+  d.ctl = st.ctl;
+  d.kind = kind;
+  d.forced = forced;
+  d.k = k;
+  for (int c = 0; c <= kMaxCount; ++c) d.bits[c] = c >= 1 ? TotalBitsTable(c, k) : 0;
+  hipLaunchKernelGGL(DecideKernel, dim3(1), dim3(64), 0, stream, d);
+  HWY_HIP_CHECK(hipGetLastError());
+}
+
+int ReportCount(const EmuStats& st, hipStream_t stream) {
+  int count = 0;
+  HWY_HIP_CHECK(hipMemcpyAsync(&count, &st.ctl->dec.count, sizeof(int), hipMemcpyDeviceToHost,
+                               stream));
+  HWY_HIP_CHECK(hipStreamSynchronize(stream));
+  return count;
+}
+
+void EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                   const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                   double* c, int64_t ldc, int nmod, hipStream_t stream, int* used) {
+  ResetPhases();
+  if (k >= 131072) {  // beyond exact INT32 accumulation of the products
+    GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+    if (used) *used = 0;
+    return;
+  }
+  EmuOperands op;
+  op.a = a;
+  op.a_rs = ta ? lda : 1;
+  op.a_cs = ta ? 1 : lda;
+  op.b = b;
+  op.b_rs = tb ? ldb : 1;
+  op.b_cs = tb ? 1 : ldb;
+  op.m = m;
+  op.n = n;
+  op.k = k;
+  const int64_t MP = AlignUp(m, kTileM), NP = AlignUp(n, kTileN), KP = AlignUp(k, kPadK);
+  // Workspace: statistics, then kMaxModuli planes of each kind (the count
+  // is decided on the device).
+  const size_t o_stats = 0;
+  const size_t o_a8 = AlignUp(StatsBytes(m, n), 256);
+  const size_t o_b8 = o_a8 + AlignUp(size_t(kMaxModuli) * MP * KP, 256);
+  const size_t o_r = o_b8 + AlignUp(size_t(kMaxModuli) * NP * KP, 256);
+  const size_t total = o_r + size_t(kMaxModuli) * MP * NP;
+  char* ws = static_cast<char*>(GetWorkspace(total, stream));
+  const EmuStats st = CarveStats(ws + o_stats, m, n);
   int8_t* a8 = reinterpret_cast<int8_t*>(ws + o_a8);
   int8_t* b8 = reinterpret_cast<int8_t*>(ws + o_b8);
   uint8_t* rr = reinterpret_cast<uint8_t*>(ws + o_r);
 
+  // 1. Statistics and the count of moduli (plane 0 holds the coarse
+  //    magnitudes until the slices overwrite it).
   Mark(kPhaseStats, stream);
-  // 1. Row maxima of op(A), column maxima of op(B) (= row maxima of op(B)^T).
-  hipLaunchKernelGGL(FillU64, dim3(unsigned(std::min<int64_t>((m + n + 2 + 255) / 256, 4096))),
-                     dim3(256), 0, stream, rowmax, m + n + 2, uint64_t(0));
-  const int64_t pchunk = 512;
-  auto rowmax_launch = [&](const double* x, int64_t rs, int64_t cs, int64_t R, int64_t K,
-                           uint64_t* out) {
-    const unsigned chunks = unsigned((K + pchunk - 1) / pchunk);
-    if (rs == 1) {
-      hipLaunchKernelGGL(RowAbsMaxKernel<true>, dim3(unsigned((R + 255) / 256), chunks),
-                         dim3(256), 0, stream, x, cs, R, K, pchunk, out);
-    } else {
-      hipLaunchKernelGGL(RowAbsMaxKernel<false>, dim3(unsigned((R + 3) / 4), chunks), dim3(256),
-                         0, stream, x, rs, R, K, pchunk, out);
-    }
-    HWY_HIP_CHECK(hipGetLastError());
-  };
-  rowmax_launch(a, a_rs, a_cs, m, k, rowmax);
-  // op(B)^T(j, p) = b[j * b_cs + p * b_rs]
-  rowmax_launch(b, b_cs, b_rs, n, k, colmax);
+  Decide(0, nmod, op, st, 0, reinterpret_cast<uint8_t*>(a8), reinterpret_cast<uint8_t*>(b8), MP,
+         NP, KP, stream);
 
-  // 2. Moduli: given, or from sampled statistics. Either way the host
-  //    learns whether the operands hold NaN or Inf.
-  hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((m + 255) / 256)), dim3(256), 0, stream,
-                     rowmax, m, 0, sa, gmax);
-  hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream,
-                     colmax, n, 0, sb, gmax + 1);
-  if (nmod <= 0) {
-    hipLaunchKernelGGL(SampleKernel, dim3(kSamples), dim3(256), 0, stream, a, a_rs, a_cs, b,
-                       b_rs, b_cs, m, n, k, sample);
-  }
-  HWY_HIP_CHECK(hipGetLastError());
-  SampleOut host{};
-  uint64_t gm[2];
-  HWY_HIP_CHECK(hipMemcpyAsync(gm, gmax, sizeof(gm), hipMemcpyDeviceToHost, stream));
-  if (nmod <= 0)
-    HWY_HIP_CHECK(hipMemcpyAsync(&host, sample, sizeof(host), hipMemcpyDeviceToHost, stream));
-  HWY_HIP_CHECK(hipStreamSynchronize(stream));
-  if (gm[0] >= kInfBits || gm[1] >= kInfBits) {
-    GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-    return 0;
-  }
-  if (nmod <= 0) nmod = ChooseModuli(host, k);
-  nmod = std::min(std::max(nmod, 2), kMaxModuli);
-  const Crt crt = MakeCrt(nmod);
-  const int bits = std::max(TotalBits(nmod, k), 2);
-  const int ba = bits / 2, bb = bits - ba;
-
-  // 3. Scale exponents with the chosen bits, then the int8 slices.
+  // 2. The int8 slices.
   Mark(kPhaseSlice, stream);
-  hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((m + 255) / 256)), dim3(256), 0, stream,
-                     rowmax, m, ba, sa, gmax);
-  hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream,
-                     colmax, n, bb, sb, gmax + 1);
-  auto slice = [&](const double* x, int64_t rs, int64_t cs, int64_t R, int64_t RP, const int* s,
-                   int8_t* y) {
+  auto slice = [&](const double* x, int64_t rs, int64_t cs, int64_t R, int64_t RP, const int* s0,
+                   int side, int8_t* y) {
     if (rs == 1)
       hipLaunchKernelGGL(SliceColsKernel, dim3(unsigned(KP / 128), unsigned(RP / 32)), dim3(256),
-                         0, stream, x, cs, R, k, RP, KP, s, crt, y);
+                         0, stream, x, cs, R, k, RP, KP, s0, st.ctl, side, y);
     else
       hipLaunchKernelGGL(SliceRowsKernel, dim3(unsigned(KP / 128), unsigned(RP / 16)), dim3(256),
-                         0, stream, x, rs, R, k, RP, KP, s, crt, y);
+                         0, stream, x, rs, R, k, RP, KP, s0, st.ctl, side, y);
     HWY_HIP_CHECK(hipGetLastError());
   };
-  slice(a, a_rs, a_cs, m, MP, sa, a8);
-  slice(b, b_cs, b_rs, n, NP, sb, b8);
+  slice(op.a, op.a_rs, op.a_cs, m, MP, st.sa, 0, a8);
+  slice(op.b, op.b_cs, op.b_rs, n, NP, st.sb, 1, b8);
 
-  // 4. One INT8 GEMM per modulus, all in one launch.
+  // 3. One INT8 GEMM per modulus, all in one launch (planes beyond the
+  //    decided count exit at once).
   ModGemmArgs g{};
   g.a = a8;
   g.b = b8;
@@ -948,66 +798,28 @@ int EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alph
   g.ldr = MP;
   g.tiles_m = int(MP / kTileM);
   g.tiles_n = int(NP / kTileN);
-  for (int l = 0; l < nmod; ++l) {
-    g.p[l] = crt.p[l];
-    g.inv_p[l] = crt.inv_p[l];
+  g.ctl = st.ctl;
+  for (int l = 0; l < kMaxModuli; ++l) {
+    g.p[l] = kModuli[l];
+    g.inv_p[l] = 1.0 / kModuli[l];
   }
   Mark(kPhaseModGemm, stream);
-  const dim3 grid(unsigned(g.tiles_m * g.tiles_n), unsigned(nmod));
-  switch (ModGemmVariant()) {
-    case 0:
-      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, kTileK, 2, 4, 4>), grid, dim3(512), 0,
-                         stream, g);
-      break;
-    case 1:
-      hipLaunchKernelGGL((ModGemmKernel<kTileM, kTileN, kTileK, 2, 4>), grid, dim3(512), 0,
-                         stream, g);
-      break;
-    case 2:
-      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, kTileK, 2, 2, 4>), grid, dim3(256), 0,
-                         stream, g);
-      break;
-
-    case 4:
-      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, kTileK, 2, 4, 3>), grid, dim3(512), 0,
-                         stream, g);
-      break;
-    case 5:
-      hipLaunchKernelGGL((ModGemmPipeKernel<kTileM, kTileN, kTileK, 2, 4, 4>), grid, dim3(512), 0,
-                         stream, g);
-      break;
-    case 6:
-      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2, true>), grid, dim3(512),
-                         0, stream, g);
-      break;
-    case 7:
-      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2, false, 1>), grid,
-                         dim3(512), 0, stream, g);
-      break;
-    case 8:
-      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2, false, 2>), grid,
-                         dim3(512), 0, stream, g);
-      break;
-
-    case 3:
-      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2>), grid, dim3(512), 0,
-                         stream, g);
-      break;
-    default:  // 9
-      hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2, false, 3>), grid,
-                         dim3(512), 0, stream, g);
-  }
+  hipLaunchKernelGGL((ModGemmDmaKernel<kTileM, kTileN, 128, 2, 4, 2, 0>),
+                     dim3(unsigned(g.tiles_m * g.tiles_n), unsigned(kMaxModuli)), dim3(512), 0,
+                     stream, g);
   HWY_HIP_CHECK(hipGetLastError());
 
-  // 5. Reconstruction and the alpha/beta epilogue.
+  // 4. Reconstruction and the alpha/beta epilogue, or the native kernel.
   Mark(kPhaseReconstruct, stream);
-  const int64_t total = ((m + 3) / 4) * n;
-  hipLaunchKernelGGL(ReconstructKernel, dim3(unsigned(std::min<int64_t>((total + 255) / 256, 1 << 20))),
-                     dim3(256), 0, stream, rr, MP * NP, MP, m, n, sa, sb, crt, alpha, beta, c,
-                     ldc);
+  const int64_t quads = ((m + 3) / 4) * n;
+  hipLaunchKernelGGL(ReconstructKernel,
+                     dim3(unsigned(std::min<int64_t>((quads + 255) / 256, 1 << 20))), dim3(256), 0,
+                     stream, rr, MP * NP, MP, m, n, st.sa, st.sb, st.ctl, alpha, beta, c, ldc);
   HWY_HIP_CHECK(hipGetLastError());
+  GemmDeviceGated<double>(&st.ctl->dec.count, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c,
+                          ldc, stream);
   Mark(kNumPhases, stream);
-  return nmod;
+  if (used) *used = ReportCount(st, stream);
 }
 
 }  // namespace ozaki
@@ -1046,9 +858,8 @@ HWYBLAS_AMD_API int hwyblas_dgemm_emulated_dev(char transa, char transb, int m, 
     ScaleDevice<double>(m, n, beta, c, ldc, s);
     return 0;
   }
-  const int used = ozaki::EmulatedDgemm(oa == Op::kTrans, ob == Op::kTrans, m, n, k, alpha, a,
-                                        lda, b, ldb, beta, c, ldc, nmoduli, s);
-  if (used_moduli) *used_moduli = used;
+  ozaki::EmulatedDgemm(oa == Op::kTrans, ob == Op::kTrans, m, n, k, alpha, a, lda, b, ldb, beta,
+                       c, ldc, nmoduli, s, used_moduli);
   return 0;
 }
 

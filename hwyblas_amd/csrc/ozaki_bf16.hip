@@ -22,9 +22,11 @@
 //     (so the result is reproducible bit for bit).
 //  4. C = alpha 2^-(sa_i + sb_j) acc + beta C (hwyBLAS's Store rules), in the
 //     GEMM's epilogue.
-// The slice count is given by the caller or chosen from sampled operand
-// statistics so that the error stays at native FP64 level (ChooseSlices).
-// Operands with NaN or Inf fall back to the native FP64 kernel.
+// The slice count is given by the caller or chosen on the device from exact
+// statistics of every pair (i, j) (DecideKernel, ozaki_shared.h): the
+// smallest count whose error bound stays at native FP64's typical level,
+// else the smallest that keeps hwyBLAS's componentwise bound, else the
+// native FP64 kernel runs, as it does for operands with NaN or Inf.
 //
 // BF16 products are exact only 8 bits at a time, so FP64 accuracy takes
 // about s (s + 1) / 2 = 28 slice products (s = 7); the INT8 scheme II needs
@@ -82,7 +84,9 @@ __device__ __forceinline__ void DigitsEight(const double (&x)[8], int sc, int sl
 // p >= K. x(r, p) = x[r * ld + p] (p contiguous). Block: 16 rows x 128 p.
 __global__ void __launch_bounds__(256)
     DigitRowsKernel(const double* x, int64_t ld, int64_t R, int64_t K, int64_t RP, int64_t KP,
-                    const int* s, int slices, uint16_t* Y) {
+                    const int* s, const EmuCtl* ctl, uint16_t* Y) {
+  const int slices = ctl->dec.count;
+  if (slices == 0) return;
   const int tid = threadIdx.x;
   const int64_t r = int64_t(blockIdx.y) * 16 + (tid >> 4);
   const int64_t p0 = int64_t(blockIdx.x) * 128 + (tid & 15) * 8;
@@ -97,7 +101,9 @@ __global__ void __launch_bounds__(256)
 // transposed through LDS, read coalesced along r.
 __global__ void __launch_bounds__(256)
     DigitColsKernel(const double* x, int64_t ld, int64_t R, int64_t K, int64_t RP, int64_t KP,
-                    const int* s, int slices, uint16_t* Y) {
+                    const int* s, const EmuCtl* ctl, uint16_t* Y) {
+  const int slices = ctl->dec.count;
+  if (slices == 0) return;
   constexpr int TR = 32, TP = 128;
   __shared__ double T[TP][TR + 1];
   const int tid = threadIdx.x;
@@ -123,7 +129,8 @@ struct SliceGemmArgs {
   const uint16_t* a;  // slices x MP x KP BF16 digits, k contiguous
   const uint16_t* b;  // slices x NP x KP
   int64_t a_plane, b_plane, kp;
-  int tiles_m, tiles_n, slices;
+  int tiles_m, tiles_n;
+  const EmuCtl* ctl;  // the decided slice count (0: nothing to do)
   const int* sa;
   const int* sb;
   double alpha, beta;
@@ -185,7 +192,8 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) SliceGemmKernel(SliceG
   // of their own (wave-uniform scalars: no table lookups with a runtime
   // index, whose loads would join the DMA in the vmcnt count).
   const int KT = int(g.kp / kBK);  // k tiles per pair
-  const int slices = g.slices;
+  const int slices = g.ctl->dec.count;
+  if (slices == 0) return;  // uniform: the native kernel runs instead
   const int total = slices * (slices + 1) / 2 * KT;
   int i_t = 0, i_u = slices - 1, i_l = slices + 1, i_kt = 0;  // next tile to fetch
   auto issue = [&](int stage) {
@@ -292,193 +300,91 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N) SliceGemmKernel(SliceG
   }
 }
 
-// Kernel schedule (HWYBLAS_AMD_BF16_KERNEL; all give the same bits): 0
-// (default) 256 x 128 tiles of 8 waves, three LDS stages; 1 the same with
-// two stages; 2 and 3: 128 x 128 tiles of 4 waves, two and three stages.
-int SliceGemmVariant() {
-  static const int v = [] {
-    const char* e = getenv("HWYBLAS_AMD_BF16_KERNEL");
-    return (e && *e >= '0' && *e <= '9') ? *e - '0' : 0;
-  }();
-  return v;
-}
-
 }  // namespace
 
-// Chooses the slice count from the sampled statistics. Cutting row i of
-// op(A) after s digits perturbs it by at most 2^(E_i + 1) 2^-8s / 2 <=
-// 2^(1 - 8s) maxrow|a_i|, and the pairs left out (t + u > s + 1; at most
-// s - 1 per level, each digit product at most 2^14 in units of
-// 2^(E_i + 1 + F_j + 1 - 8(t + u))) add less than
-// 4.02 (s - 1) 2^-8s maxrow|a_i| maxcol|b_j| per term of the sum, so
-//   |C - C_emulated| <~ 2^-8s (2 (wa + wb) + 4.02 (s - 1) wab) S_ij
-// in terms of the worst ratios wa, wb, wab of WorstRatios. The smallest s in
-// [3, 16] that keeps this within native FP64's typical sqrt(k) u S_ij is
-// used.
-int ChooseSlices(const SampleOut& s, int64_t k) {
-  double wa, wb, wab;
-  WorstRatios(s, k, &wa, &wb, &wab);
-  const double target = std::sqrt(double(k)) * 0x1p-53;
-  for (int sl = 3; sl <= kMaxSlices; ++sl) {
-    const double err = std::ldexp(2.0 * (wa + wb) + 4.02 * (sl - 1) * wab, -8 * sl);
-    if (err <= target) return sl;
-  }
-  return kMaxSlices;
-}
-
-int EmulatedDgemmBf16(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
-                      const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
-                      double* c, int64_t ldc, int slices, hipStream_t stream) {
+void EmulatedDgemmBf16(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                       const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                       double* c, int64_t ldc, int slices, hipStream_t stream, int* used) {
   ResetPhases();
   const int64_t MP = AlignUp(m, kSliceTileM), NP = AlignUp(n, kSliceTileN);
   const int64_t KP = AlignUp(k, kPadK);
-  if (int64_t(kSliceTileM) * KP >= (int64_t(1) << 31)) {  // 32-bit DMA offsets
+  // 32-bit DMA offsets; the coarse INT8 product sums exactly for k < 2^17.
+  if (int64_t(kSliceTileM) * KP >= (int64_t(1) << 31) || k >= 131072) {
     GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-    return 0;
+    if (used) *used = 0;
+    return;
   }
-  const int64_t a_rs = ta ? lda : 1, a_cs = ta ? 1 : lda;
-  const int64_t b_rs = tb ? ldb : 1, b_cs = tb ? 1 : ldb;
-
-  size_t off = 0;
-  auto carve = [&](size_t bytes) {
-    const size_t o = off;
-    off = AlignUp(off + bytes, 256);
-    return o;
-  };
-  const size_t o_rowmax = carve(sizeof(uint64_t) * (m + n + 2));
-  const size_t o_sa = carve(sizeof(int) * m);
-  const size_t o_sb = carve(sizeof(int) * n);
-  const size_t o_sample = carve(sizeof(SampleOut));
-  // The digit planes: carved now when the slice count is given, else once
-  // the statistics have chosen it.
-  size_t o_da = 0, o_db = 0;
-  auto carve_digits = [&]() {
-    o_da = carve(sizeof(uint16_t) * size_t(slices) * MP * KP);
-    o_db = carve(sizeof(uint16_t) * size_t(slices) * NP * KP);
-  };
-  if (slices > 0) {
-    slices = std::min(slices, kMaxSlices);
-    carve_digits();
-  }
-
-  char* ws = static_cast<char*>(GetWorkspace(off, stream));
-  uint64_t* rowmax = nullptr;
-  uint64_t* gmax = nullptr;
-  int* sa = nullptr;
-  int* sb = nullptr;
-  // Row maxima of op(A), column maxima of op(B) and the scale exponents
-  // -(E + 1) (ScaleExpKernel with bits = -1: |x 2^-(E+1)| < 1/2).
-  auto statistics = [&]() {
-    rowmax = reinterpret_cast<uint64_t*>(ws + o_rowmax);
-    uint64_t* colmax = rowmax + m;
-    gmax = colmax + n;
-    sa = reinterpret_cast<int*>(ws + o_sa);
-    sb = reinterpret_cast<int*>(ws + o_sb);
-    hipLaunchKernelGGL(FillU64,
-                       dim3(unsigned(std::min<int64_t>((m + n + 2 + 255) / 256, 4096))),
-                       dim3(256), 0, stream, rowmax, m + n + 2, uint64_t(0));
-    const int64_t pchunk = 512;
-    auto rowmax_launch = [&](const double* x, int64_t rs, int64_t cs, int64_t R,
-                             uint64_t* out) {
-      const unsigned chunks = unsigned((k + pchunk - 1) / pchunk);
-      if (rs == 1)
-        hipLaunchKernelGGL(RowAbsMaxKernel<true>, dim3(unsigned((R + 255) / 256), chunks),
-                           dim3(256), 0, stream, x, cs, R, k, pchunk, out);
-      else
-        hipLaunchKernelGGL(RowAbsMaxKernel<false>, dim3(unsigned((R + 3) / 4), chunks),
-                           dim3(256), 0, stream, x, rs, R, k, pchunk, out);
-      HWY_HIP_CHECK(hipGetLastError());
-    };
-    rowmax_launch(a, a_rs, a_cs, m, rowmax);
-    rowmax_launch(b, b_cs, b_rs, n, colmax);
-    hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((m + 255) / 256)), dim3(256), 0, stream,
-                       rowmax, m, -1, sa, gmax);
-    hipLaunchKernelGGL(ScaleExpKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream,
-                       colmax, n, -1, sb, gmax + 1);
-    HWY_HIP_CHECK(hipGetLastError());
-  };
-
-  Mark(kPhaseStats, stream);
-  statistics();
-  SampleOut* sample = reinterpret_cast<SampleOut*>(ws + o_sample);
-  if (slices <= 0)
-    hipLaunchKernelGGL(SampleKernel, dim3(kSamples), dim3(256), 0, stream, a, a_rs, a_cs, b,
-                       b_rs, b_cs, m, n, k, sample);
-  HWY_HIP_CHECK(hipGetLastError());
-  SampleOut host{};
-  uint64_t gm[2];
-  HWY_HIP_CHECK(hipMemcpyAsync(gm, gmax, sizeof(gm), hipMemcpyDeviceToHost, stream));
-  if (slices <= 0)
-    HWY_HIP_CHECK(hipMemcpyAsync(&host, sample, sizeof(host), hipMemcpyDeviceToHost, stream));
-  HWY_HIP_CHECK(hipStreamSynchronize(stream));
-  if (gm[0] >= kInfBits || gm[1] >= kInfBits) {
-    GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-    return 0;
-  }
-  if (slices <= 0) {
-    slices = ChooseSlices(host, k);
-    carve_digits();
-    bool grown = false;
-    ws = static_cast<char*>(GetWorkspace(off, stream, &grown));
-    if (grown) statistics();  // the new buffer holds none of it
-  }
+  EmuOperands op;
+  op.a = a;
+  op.a_rs = ta ? lda : 1;
+  op.a_cs = ta ? 1 : lda;
+  op.b = b;
+  op.b_rs = tb ? ldb : 1;
+  op.b_cs = tb ? 1 : ldb;
+  op.m = m;
+  op.n = n;
+  op.k = k;
+  // The coarse planes of the decision (INT8 tile geometry) live in the
+  // first digit planes until the digits overwrite them.
+  const int64_t MPc = AlignUp(m, 256), NPc = AlignUp(n, 256), KPc = AlignUp(k, 128);
+  const int planes = slices > 0 ? std::min(slices, kMaxSlices) : kMaxSlices;
+  const size_t o_da = AlignUp(StatsBytes(m, n), 256);
+  const size_t da_bytes = std::max(sizeof(uint16_t) * size_t(planes) * MP * KP, size_t(MPc * KPc));
+  const size_t o_db = o_da + AlignUp(da_bytes, 256);
+  const size_t db_bytes = std::max(sizeof(uint16_t) * size_t(planes) * NP * KP, size_t(NPc * KPc));
+  char* ws = static_cast<char*>(GetWorkspace(o_db + db_bytes, stream));
+  const EmuStats st = CarveStats(ws, m, n);
   uint16_t* da = reinterpret_cast<uint16_t*>(ws + o_da);
   uint16_t* db = reinterpret_cast<uint16_t*>(ws + o_db);
 
+  // 1. Statistics, scale exponents -(E + 1) (|x 2^-(E+1)| < 1/2), and the
+  //    slice count.
+  Mark(kPhaseStats, stream);
+  Decide(1, slices, op, st, -1, reinterpret_cast<uint8_t*>(da), reinterpret_cast<uint8_t*>(db),
+         MPc, NPc, KPc, stream);
+
+  // 2. The digits.
   Mark(kPhaseSlice, stream);
   auto digits = [&](const double* x, int64_t rs, int64_t cs, int64_t R, int64_t RP,
                     const int* s, uint16_t* y) {
     if (rs == 1)
       hipLaunchKernelGGL(DigitColsKernel, dim3(unsigned(KP / 128), unsigned(RP / 32)), dim3(256),
-                         0, stream, x, cs, R, k, RP, KP, s, slices, y);
+                         0, stream, x, cs, R, k, RP, KP, s, st.ctl, y);
     else
       hipLaunchKernelGGL(DigitRowsKernel, dim3(unsigned(KP / 128), unsigned(RP / 16)), dim3(256),
-                         0, stream, x, rs, R, k, RP, KP, s, slices, y);
+                         0, stream, x, rs, R, k, RP, KP, s, st.ctl, y);
     HWY_HIP_CHECK(hipGetLastError());
   };
-  digits(a, a_rs, a_cs, m, MP, sa, da);
-  digits(b, b_cs, b_rs, n, NP, sb, db);
+  digits(op.a, op.a_rs, op.a_cs, m, MP, st.sa, da);
+  digits(op.b, op.b_cs, op.b_rs, n, NP, st.sb, db);
 
+  // 3. All digit pairs and the epilogue in one launch, or the native kernel.
   SliceGemmArgs g{};
   g.a = da;
   g.b = db;
   g.a_plane = MP * KP;
   g.b_plane = NP * KP;
   g.kp = KP;
-  g.slices = slices;
-  g.sa = sa;
-  g.sb = sb;
+  g.ctl = st.ctl;
+  g.sa = st.sa;
+  g.sb = st.sb;
   g.alpha = alpha;
   g.beta = beta;
   g.c = c;
   g.ldc = ldc;
   g.m = m;
   g.n = n;
+  g.tiles_m = int(MP / kSliceTileM);
+  g.tiles_n = int(NP / kSliceTileN);
   Mark(kPhaseModGemm, stream);
-  // MP is a multiple of 256 and NP of 128, so every variant's tiles divide them.
-  auto launch = [&](auto kernel, int bm, int bn, int threads) {
-    g.tiles_m = int(MP / bm);
-    g.tiles_n = int(NP / bn);
-    hipLaunchKernelGGL(kernel, dim3(unsigned(g.tiles_m * g.tiles_n)), dim3(threads), 0, stream,
-                       g);
-  };
-  switch (SliceGemmVariant()) {
-    case 1:
-      launch(SliceGemmKernel<256, 128, 4, 2, 2>, 256, 128, 512);
-      break;
-    case 2:
-      launch(SliceGemmKernel<128, 128, 2, 2, 2>, 128, 128, 256);
-      break;
-    case 3:
-      launch(SliceGemmKernel<128, 128, 2, 2, 3>, 128, 128, 256);
-      break;
-    default:
-      launch(SliceGemmKernel<256, 128, 4, 2, 3>, 256, 128, 512);
-  }
+  hipLaunchKernelGGL((SliceGemmKernel<kSliceTileM, kSliceTileN, 4, 2, 3>),
+                     dim3(unsigned(g.tiles_m * g.tiles_n)), dim3(512), 0, stream, g);
   HWY_HIP_CHECK(hipGetLastError());
+  GemmDeviceGated<double>(&st.ctl->dec.count, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c,
+                          ldc, stream);
   Mark(kPhaseReconstruct, stream);  // the epilogue is fused into the GEMM
   Mark(kNumPhases, stream);
-  return slices;
+  if (used) *used = ReportCount(st, stream);
 }
 
 }  // namespace ozaki
@@ -517,9 +423,8 @@ HWYBLAS_AMD_API int hwyblas_dgemm_emulated_bf16_dev(char transa, char transb, in
     ScaleDevice<double>(m, n, beta, c, ldc, s);
     return 0;
   }
-  const int used = ozaki::EmulatedDgemmBf16(oa == Op::kTrans, ob == Op::kTrans, m, n, k, alpha,
-                                            a, lda, b, ldb, beta, c, ldc, slices, s);
-  if (used_slices) *used_slices = used;
+  ozaki::EmulatedDgemmBf16(oa == Op::kTrans, ob == Op::kTrans, m, n, k, alpha, a, lda, b, ldb,
+                           beta, c, ldc, slices, s, used_slices);
   return 0;
 }
 

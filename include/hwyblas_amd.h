@@ -342,13 +342,21 @@ HWYBLAS_AMD_API int hwyblas_strsm_dev(char side, char uplo, char transa,
  * arguments, checks (reported as "DGEMM") and quick returns as
  * hwyblas_dgemm_dev, FP64 in and out. The product is computed by the Ozaki
  * scheme II: power-of-two row/column scaling to integers, symmetric residues
- * modulo `nmoduli` pairwise coprime moduli <= 256 (2..24; 0 = chosen from a
- * sampled estimate of the operands' dynamic range so that the error stays at
- * native FP64 level), exact INT8 products and a Chinese-remainder
- * reconstruction. Operands with NaN or Inf, and k >= 131072, use the native
- * FP64 kernel. The call synchronises `stream` once (to read the operand
- * statistics). *used_moduli (if not NULL) receives the number of moduli, or
- * 0 when the native kernel ran. Returns 0 or the Fortran info. */
+ * modulo `nmoduli` pairwise coprime moduli <= 256, exact INT8 products and a
+ * Chinese-remainder reconstruction.
+ * nmoduli 2..24 (larger values use 24) fixes the count, with no accuracy
+ * guarantee. nmoduli 0 lets the device choose it from exact statistics of
+ * every pair (i, j) (row and column sums, and an INT8 product of coarse
+ * magnitudes that bounds |A||B| from below): the smallest count whose error
+ * bound is within 2 sqrt(k) u of |A||B| (u = 2^-53), else the smallest that
+ * keeps hwyBLAS's componentwise bound
+ * |C - C_exact| <= 4 (k + 2) u (|alpha||A||B| + |beta||C|), else the native
+ * FP64 kernel. Operands with NaN or Inf, and k >= 131072, also use the
+ * native kernel. The decision stays on the device: with used_moduli NULL the
+ * call is asynchronous (it can be captured in a HIP graph once its
+ * workspace has grown to the problem's size); a non-NULL used_moduli makes it
+ * synchronise `stream` at the end and receive the count, or 0 when the
+ * native kernel ran. Returns 0 or the Fortran info. */
 HWYBLAS_AMD_API int hwyblas_dgemm_emulated_dev(char transa, char transb, int m, int n,
                                                int k, double alpha, const double* a,
                                                int lda, const double* b, int ldb,
@@ -360,15 +368,15 @@ HWYBLAS_AMD_API int hwyblas_dgemm_emulated_dev(char transa, char transb, int m, 
  * Ozaki scheme I with BF16 slices. Same arguments, checks and quick returns
  * as hwyblas_dgemm_emulated_dev. Rows of op(A) and columns of op(B) are
  * scaled by powers of two below 1 in magnitude and cut into `slices`
- * (1..16; larger values use 16; 0 = chosen from a sampled estimate of the
- * operands' dynamic range so that the error stays at native FP64 level)
- * 8-bit integer digits, exact in BF16; the s (s + 1) / 2 digit products
- * with t + u <= s + 1 run on the BF16 MFMA with exact FP32 accumulation
- * over k chunks of 256, summed in FP64 in a fixed order (the result is
- * reproducible bit for bit). Operands with NaN or Inf use the native FP64
- * kernel. Synchronises `stream` once. *used_slices (if not NULL) receives
- * the slice count, or 0 when the native kernel ran. Returns 0 or the
- * Fortran info. */
+ * (1..16; larger values use 16) 8-bit integer digits, exact in BF16; the
+ * s (s + 1) / 2 digit products with t + u <= s + 1 run on the BF16 MFMA with
+ * exact FP32 accumulation over k chunks of 1024, summed in FP64 in a fixed
+ * order (the result is reproducible bit for bit). slices 0 chooses the count
+ * on the device by the same rule and statistics as the INT8 method, or runs
+ * the native FP64 kernel; NaN or Inf operands and k >= 131072 run it too.
+ * Asynchronous with used_slices NULL; otherwise synchronises at the end and
+ * *used_slices receives the slice count, or 0 when the native kernel ran.
+ * Returns 0 or the Fortran info. */
 HWYBLAS_AMD_API int hwyblas_dgemm_emulated_bf16_dev(char transa, char transb, int m, int n,
                                                     int k, double alpha, const double* a,
                                                     int lda, const double* b, int ldb,

@@ -75,6 +75,14 @@ def lib():
                                            _d, _p, _i, _i]
         L.orc_bf16_digits.restype = None
         L.orc_bf16_digits.argtypes = [_d, _i, _i, _p]
+        L.orc_emulated_choose.restype = _i
+        L.orc_emulated_choose.argtypes = [_i, _i, _c_char, _c_char, _i, _i, _i, _p, _i, _p, _i,
+                                          _p, _p]
+        L.orc_emulated_bound.restype = None
+        L.orc_emulated_bound.argtypes = [_i, _i, ctypes.c_int64, _d, _i, _p, _p]
+        L.orc_emulated_coarse.restype = None
+        L.orc_emulated_coarse.argtypes = [_c_char, _c_char, _i, _i, _i, _p, _i, _p, _i, _p, _p,
+                                          _p]
         L.orc_fill_pow2.restype = None
         L.orc_fill_pow2.argtypes = [_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                     ctypes.c_uint64, _i, _i]
@@ -151,15 +159,38 @@ def gemm_ref(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
     return exp[:m], bnd[:m]
 
 
+def _threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _in_parallel(count, work, per_item_cost):
+    """Runs work(lo, hi) over [0, count) in slices on a thread pool (the
+    oracle's C functions release the GIL through ctypes) when the job is
+    large, else in one call. Results do not depend on the split."""
+    threads = _threads()
+    if count == 0:
+        return
+    if threads == 1 or count * per_item_cost < 5e7:
+        work(0, count)
+        return
+    from concurrent.futures import ThreadPoolExecutor
+    step = (count + 4 * threads - 1) // (4 * threads)
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda lo: work(lo, min(count, lo + step)), range(0, count, step)))
+
+
 def gemm_ref_entries(ta, tb, k, alpha, a, lda, b, ldb, beta, c, ldc, rows, cols):
     rows = np.ascontiguousarray(rows, dtype=np.int32)
     cols = np.ascontiguousarray(cols, dtype=np.int32)
     exp = np.zeros(len(rows), dtype=np.float64)
     bnd = np.zeros(len(rows), dtype=np.float64)
     fn = getattr(lib(), f"orc_{_pre(a.dtype)}gemm_ref_entries")
-    fn(_b(ta), _b(tb), k, alpha, _ptr(a), lda, _ptr(b), ldb, beta,
-       _ptr(c) if c is not None else None, ldc, _ptr(rows), _ptr(cols), len(rows),
-       _ptr(exp), _ptr(bnd))
+    cptr = _ptr(c) if c is not None else None
+
+    def work(lo, hi):
+        fn(_b(ta), _b(tb), k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, cptr, ldc,
+           _ptr(rows[lo:]), _ptr(cols[lo:]), hi - lo, _ptr(exp[lo:]), _ptr(bnd[lo:]))
+    _in_parallel(len(rows), work, 20 * max(k, 1))
     return exp, bnd
 
 
@@ -167,8 +198,13 @@ def trsm_residual(side, uplo, ta, diag, m, n, alpha, a, lda, b0, ldb, x, ldx, wh
     which = np.ascontiguousarray(which, dtype=np.int32)
     ratio = np.zeros(len(which), dtype=np.float64)
     fn = getattr(lib(), f"orc_{_pre(b0.dtype)}trsm_residual")
-    fn(_b(side), _b(uplo), _b(ta), _b(diag), m, n, alpha, _ptr(a), lda, _ptr(b0), ldb,
-       _ptr(x), ldx, _ptr(which), len(which), _ptr(ratio))
+    ka = m if side in "Ll" else n
+    other = m if side in "Ll" else n
+
+    def work(lo, hi):
+        fn(_b(side), _b(uplo), _b(ta), _b(diag), m, n, alpha, _ptr(a), lda, _ptr(b0), ldb,
+           _ptr(x), ldx, _ptr(which[lo:]), hi - lo, _ptr(ratio[lo:]))
+    _in_parallel(len(which), work, 20 * ka * other)
     return ratio
 
 
@@ -195,6 +231,36 @@ def dgemm_ozaki_bf16(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, slice
     """In-place C update by the CPU restatement of the BF16-slice emulated DGEMM."""
     lib().orc_dgemm_ozaki_bf16(_b(ta), _b(tb), m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb,
                                beta, _ptr(c), ldc, slices)
+
+
+def emulated_choose(kind, ta, tb, m, n, k, a, lda, b, ldb, forced=0):
+    """The count the emulated DGEMM picks (kind "int8" or "bf16"; 0 = native
+    FP64) and its statistics: (count, X, gmin)."""
+    x = ctypes.c_double(0)
+    g = ctypes.c_int(0)
+    c = lib().orc_emulated_choose(0 if kind == "int8" else 1, forced, _b(ta), _b(tb), m, n, k,
+                                  _ptr(a), lda, _ptr(b), ldb, ctypes.byref(x), ctypes.byref(g))
+    return c, x.value, g.value
+
+
+def emulated_bound(kind, count, k, xmax, gmin):
+    """(typical, worst) error bound of `count`, in units of |A||B|(i, j)."""
+    t = ctypes.c_double(0)
+    w = ctypes.c_double(0)
+    lib().orc_emulated_bound(0 if kind == "int8" else 1, count, k, xmax, gmin,
+                             ctypes.byref(t), ctypes.byref(w))
+    return t.value, w.value
+
+
+def emulated_coarse(ta, tb, m, n, k, a, lda, b, ldb):
+    """(lower bound of |A||B| (m x n), upper row sums of |op(A)|, upper column
+    sums of |op(B)|) from the coarse statistics."""
+    lower = np.zeros((m, n), order="F")
+    rs = np.zeros(m)
+    cs = np.zeros(n)
+    lib().orc_emulated_coarse(_b(ta), _b(tb), m, n, k, _ptr(a), lda, _ptr(b), ldb, _ptr(lower),
+                              _ptr(rs), _ptr(cs))
+    return lower, rs, cs
 
 
 def bf16_digits(x, sc, slices):

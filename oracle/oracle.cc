@@ -436,14 +436,18 @@ template <typename T>
 void GemmEntry(const Opd<T>& a, const Opd<T>& b, int i, int j, int k, T alpha,
                T beta, const T* c, ptrdiff_t ldc, double* expected,
                double* bound) {
-  DD sum;
-  double abs_sum = 0;
+  // Four interleaved double-double chains (p mod 4), combined at the end:
+  // the same accuracy as one chain, four times the instruction parallelism.
+  DD part[4];
+  double abs_part[4] = {0, 0, 0, 0};
   for (int p = 0; p < k; ++p) {
     const double x = static_cast<double>(a.at(i, p));
     const double y = static_cast<double>(b.at(p, j));
-    sum = Add(sum, TwoProd(x, y));
-    abs_sum += std::fabs(x) * std::fabs(y);
+    part[p & 3] = Add(part[p & 3], TwoProd(x, y));
+    abs_part[p & 3] += std::fabs(x) * std::fabs(y);
   }
+  const DD sum = Add(Add(part[0], part[1]), Add(part[2], part[3]));
+  const double abs_sum = (abs_part[0] + abs_part[1]) + (abs_part[2] + abs_part[3]);
   DD r = MulD(sum, static_cast<double>(alpha));
   double scale = std::fabs(static_cast<double>(alpha)) * abs_sum;
   if (beta != T(0)) {

@@ -125,6 +125,22 @@ void orc_dgemm_ozaki_bf16(char transa, char transb, int m, int n, int k, double 
                           const double* a, int lda, const double* b, int ldb, double beta,
                           double* c, int ldc, int slices);
 
+/* ---- the count choice of the emulated DGEMMs (oracle/emulated_choice.cc) --
+ * Exact statistics of every pair (i, j): the upper row/column sums, the
+ * coarse INT8 product G that bounds |A||B| from below, X and gmin; the error
+ * bound of a count in units of |A||B|(i, j); the rule that picks the count
+ * (kind 0: INT8 moduli, 1: BF16 slices; 0 returned: native FP64). */
+int orc_emulated_choose(int kind, int forced, char transa, char transb, int m, int n, int k,
+                        const double* a, int lda, const double* b, int ldb, double* xmax,
+                        int* gmin);
+void orc_emulated_bound(int kind, int count, int64_t k, double xmax, int gmin, double* typical,
+                        double* worst);
+/* lower[i + j m] = 2^(E_i + F_j - 14) G_ij; rowsum_up[i] = rs_i 2^(E_i - 40),
+ * colsum_up[j] = cs_j 2^(F_j - 40). */
+void orc_emulated_coarse(char transa, char transb, int m, int n, int k, const double* a,
+                         int lda, const double* b, int ldb, double* lower, double* rowsum_up,
+                         double* colsum_up);
+
 /* ---- LU with partial pivoting and its solve (oracle/lu.cc) ---------------
  * hwyBLAS's dgetrf (blocked NB = 64 over the recursive dgetrf2 with a leaf of
  * 16), dlaswp (1-based k1..k2, incx = +-1), dgetrs and gerfs's componentwise

@@ -71,15 +71,22 @@ def test_auto_slices_native_level_error(H, ta, tb):
                 a = np.asfortranarray(a * (ra[:, None] if ta == "N" else ra[None, :]))
                 b = np.asfortranarray(b * (cb[None, :] if tb == "N" else cb[:, None]))
             got, used = run_bf16(H, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.5, c, ldc, 0)
-            assert 3 <= used <= 16
+            want_count = oracle.emulated_choose("bf16", ta, tb, m, n, k, a, lda, b, ldb)[0]
+            assert used == want_count and 3 <= used <= 16, (used, want_count)
             r = gemm_bound_ratio(got, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.5, c, ldc)
             assert r <= 1.0, (m, n, k, scaled, used, r)
+            want = c.copy(order="F")
+            oracle.dgemm_ozaki_bf16(ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.5, want, ldc, used)
+            assert np.array_equal(got, want)
 
 
 def test_auto_slices_wide_exponent_range(H):
     """Entries u 2^e with e uniform in [-30, 30]: the statistics ask for more
-    slices than uniform operands do, and the error stays within the bound."""
-    m, n, k = 160, 96, 512
+    slices than uniform operands do, and the error stays within the bound.
+    (At k = 512 some pair's large entries do not meet at all, the coarse
+    product cannot certify it and the native kernel runs; k = 2048 leaves
+    every pair dozens of meeting large entries.)"""
+    m, n, k = 160, 96, 2048
     a = oracle.fill_pow2(m, k, 31)
     b = oracle.fill_pow2(k, n, 32)
     c = np.zeros((m, n), order="F")
@@ -87,8 +94,34 @@ def test_auto_slices_wide_exponent_range(H):
                                oracle.fill_uniform(k, n, 2), k, 0.0, c, m, 0)
     got, used = run_bf16(H, "N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m, 0)
     assert used > used_uniform
+    assert used == oracle.emulated_choose("bf16", "N", "N", m, n, k, a, m, b, k)[0]
     r = gemm_bound_ratio(got, "N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m)
     assert r <= 1.0, (used, r)
+
+
+@pytest.mark.parametrize("meets", [False, True])
+def test_auto_row_with_a_dominant_entry(H, meets):
+    """As for the INT8 method: a row with one entry 2^60 above the rest,
+    against a column of op(B) that is zero there (no certified count: the
+    native kernel runs) or not (a count is certified and the bound holds)."""
+    m, n, k = 100, 90, 513
+    a = oracle.fill_uniform(m, k, 41)
+    b = oracle.fill_uniform(k, n, 42)
+    c = oracle.fill_uniform(m, n, 43)
+    a[37, 200] = 2.0 ** 60
+    if meets:
+        b[200, :] = 0.75
+    else:
+        b[200, 41] = 0.0
+    got, used = run_bf16(H, "N", "N", m, n, k, 1.0, a, m, b, k, 0.5, c, m, 0)
+    assert used == oracle.emulated_choose("bf16", "N", "N", m, n, k, a, m, b, k)[0]
+    assert (used > 0) == meets
+    r = gemm_bound_ratio(got, "N", "N", m, n, k, 1.0, a, m, b, k, 0.5, c, m)
+    assert r <= 1.0, (used, r)
+    if not meets:
+        nat = c.copy(order="F")
+        H.gemm("N", "N", m, n, k, 1.0, a, m, b, k, 0.5, nat, m)
+        assert np.array_equal(got, nat)
 
 
 def test_special_values_fall_back_to_native(H):

@@ -784,3 +784,191 @@ def test_entry_reference_equals_the_full_reference(ta, tb, dtype):
                                          m, rows, cols)
         assert np.array_equal(e2, exp[rows, cols])
         assert np.array_equal(b2, bnd[rows, cols])
+
+
+# ---------------------------------------------------------------------------
+# The count choice of the emulated DGEMMs (oracle/emulated_choice.cc)
+#
+# The decision rests on three facts, each pinned here by exact arithmetic:
+# (1) the coarse product bounds |A||B| from below and the exact sums bound
+# the row/column sums from above; (2) the per-entry error of the emulation
+# is within the analytic formula (for INT8 with bA + bB bits:
+# 2^(E_i - bA - 1) sum_p |b_pj| + 2^(F_j - bB - 1) sum_p |a_ip|
+# + k 2^(E_i + F_j - bA - bB - 2)), which adversarial operands attain within
+# 1 %; (3) the decision's bound times the lower bound covers the formula on
+# every pair. Planted mistakes 13 (a dropped sum term) and 14 (coarse
+# magnitudes rounded to nearest) each break one of them.
+
+def _exact_abs_product(a, b):
+    m, k = a.shape
+    n = b.shape[1]
+    fa = [[Fraction(abs(float(a[i, p]))) for p in range(k)] for i in range(m)]
+    fb = [[Fraction(abs(float(b[p, j]))) for j in range(n)] for p in range(k)]
+    return [[sum(fa[i][p] * fb[p][j] for p in range(k)) for j in range(n)] for i in range(m)]
+
+
+def _exact_product(a, b):
+    m, k = a.shape
+    n = b.shape[1]
+    return [[sum(Fraction(float(a[i, p])) * Fraction(float(b[p, j])) for p in range(k))
+             for j in range(n)] for i in range(m)]
+
+
+def _choice_operands():
+    rng = np.random.default_rng(21)
+    m, n, k = 7, 6, 40
+    yield m, n, k, oracle.fill_uniform(m, k, 1), oracle.fill_uniform(k, n, 2)
+    yield m, n, k, oracle.fill_pow2(m, k, 3, -20, 20), oracle.fill_pow2(k, n, 4, -20, 20)
+    a = oracle.fill_uniform(m, k, 5) * (10.0 ** rng.uniform(-8, 8, m))[:, None]
+    b = oracle.fill_uniform(k, n, 6) * (10.0 ** rng.uniform(-8, 8, n))[None, :]
+    a[2, :] = 0.0  # a zero row
+    yield m, n, k, np.asfortranarray(a), np.asfortranarray(b)
+
+
+def test_emulated_coarse_statistics_bound_exactly():
+    for m, n, k, a, b in _choice_operands():
+        lower, rs, cs = oracle.emulated_coarse("N", "N", m, n, k, a, m, b, k)
+        exact = _exact_abs_product(a, b)
+        for i in range(m):
+            for j in range(n):
+                assert Fraction(float(lower[i, j])) <= exact[i][j], (i, j)
+        for i in range(m):
+            s = sum(Fraction(abs(float(x))) for x in a[i])
+            assert Fraction(float(rs[i])) >= s
+            assert float(rs[i]) <= float(s) * (1 + 1e-9) + k * 2.0 ** -39 * float(np.max(np.abs(a[i])))
+        for j in range(n):
+            s = sum(Fraction(abs(float(x))) for x in b[:, j])
+            assert Fraction(float(cs[j])) >= s
+    # Uniform operands: the lower bound is close (7-bit magnitudes).
+    m, n, k, a, b = next(_choice_operands())
+    lower = oracle.emulated_coarse("N", "N", m, n, k, a, m, b, k)[0]
+    exact = np.array([[float(x) for x in row] for row in _exact_abs_product(a, b)])
+    assert np.all(lower >= 0.9 * exact)
+
+
+def test_planted_coarse_rounding_mistake_is_caught():
+    oracle.plant_mistake(14)
+    with pytest.raises(AssertionError):
+        test_emulated_coarse_statistics_bound_exactly()
+
+
+def _int8_formula(a, b, nmod):
+    """The per-entry error formula (exact sums), as floats, m x n."""
+    m, k = a.shape
+    n = b.shape[1]
+    bits = max(oracle.lib().orc_ozaki_total_bits(nmod, k), 2)
+    ba, bb = bits // 2, bits - bits // 2
+    E = [np.frexp(np.max(np.abs(a[i])))[1] for i in range(m)]
+    F = [np.frexp(np.max(np.abs(b[:, j])))[1] for j in range(n)]
+    rows = [float(sum(Fraction(abs(float(x))) for x in a[i])) for i in range(m)]
+    cols = [float(sum(Fraction(abs(float(x))) for x in b[:, j])) for j in range(n)]
+    out = np.zeros((m, n))
+    for i in range(m):
+        for j in range(n):
+            out[i, j] = (2.0 ** (E[i] - ba - 1) * cols[j] + 2.0 ** (F[j] - bb - 1) * rows[i]
+                         + k * 2.0 ** (E[i] + F[j] - bits - 2))
+    return out, ba, bb
+
+
+def _emulated_error(a, b, nmod):
+    m, k = a.shape
+    n = b.shape[1]
+    c = np.zeros((m, n), order="F")
+    oracle.dgemm_ozaki("N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m, nmod)
+    exact = _exact_product(a, b)
+    return np.array([[abs(float(Fraction(float(c[i, j])) - exact[i][j])) for j in range(n)]
+                     for i in range(m)])
+
+
+def _half_integer_operands(k, rough_a):
+    """With 5 moduli at k = 64 (bA = bB = 16): entries whose scaled values sit
+    just below x.5 on one side (every rounding error about -1/2, aligned) and
+    exactly representable values on the other, so the formula's active term
+    is attained."""
+    c = 0.75 + 2.0 ** -17 - 2.0 ** -36  # 0.75 2^16 + 1/2 - 2^-20
+    m, n = 3, 2
+    if rough_a:
+        return np.full((m, k), c, order="F"), np.ones((k, n), order="F")
+    return np.ones((m, k), order="F"), np.full((k, n), c, order="F")
+
+
+def test_emulated_error_formula_holds_and_is_attained():
+    u = 2.0 ** -53
+    for m, n, k, a, b in _choice_operands():
+        for nmod in (3, 5, 8):
+            formula, _, _ = _int8_formula(a, b, nmod)
+            err = _emulated_error(a, b, nmod)
+            absab = np.array([[float(x) for x in row] for row in _exact_abs_product(a, b)])
+            assert np.all(err <= formula + 32 * u * absab), nmod
+    k = 64
+    assert oracle.lib().orc_ozaki_total_bits(5, k) == 32
+    for rough_a in (True, False):
+        a, b = _half_integer_operands(k, rough_a)
+        formula, _, _ = _int8_formula(a, b, 5)
+        err = _emulated_error(a, b, 5)
+        assert np.all(err <= formula)
+        # The rounding side's term: 2^(E - bA - 1) k (rough A: E = 0, B = 1)
+        # or 2^(F - bB - 1) k (rough B: F = 0, A = 1); both 2^-17 k.
+        term = 2.0 ** -17 * k
+        assert np.all(err >= 0.99 * term), (rough_a, err.max(), term)
+
+
+def test_emulated_decision_bound_covers_the_formula():
+    """worst(N) x lower_ij >= the per-entry formula on every pair, for the
+    statistics X, gmin the choice computed."""
+    for m, n, k, a, b in _choice_operands():
+        count, x, gmin = oracle.emulated_choose("int8", "N", "N", m, n, k, a, m, b, k)
+        lower = oracle.emulated_coarse("N", "N", m, n, k, a, m, b, k)[0]
+        for nmod in (3, 8, 14, 20):
+            formula, _, _ = _int8_formula(a, b, nmod)
+            worst = oracle.emulated_bound("int8", nmod, k, x, gmin)[1]
+            if not np.isfinite(worst):  # some pair cannot be certified: native
+                assert count == 0 and not np.all(lower > 0)
+                continue
+            nz = (np.abs(a).sum(axis=1)[:, None] > 0) & (np.abs(b).sum(axis=0)[None, :] > 0)
+            assert np.all(formula[nz] <= worst * lower[nz] * (1 + 1e-12)), nmod
+        # The rule: the smallest count meeting the targets (restated here).
+        u = 2.0 ** -53
+        t_typ, t_bound = 2 * np.sqrt(k) * u, (4 * (k + 2) - 24) * u
+        pick = 0
+        for c in range(2, 25):
+            typ, worst = oracle.emulated_bound("int8", c, k, x, gmin)
+            if typ <= t_typ and worst <= t_bound:
+                pick = c
+                break
+        if pick == 0:
+            for c in range(2, 25):
+                if oracle.emulated_bound("int8", c, k, x, gmin)[1] <= t_bound:
+                    pick = c
+                    break
+        assert count == pick
+
+
+def test_planted_dropped_sum_term_mistake_is_caught():
+    oracle.plant_mistake(13)
+    with pytest.raises(AssertionError):
+        test_emulated_decision_bound_covers_the_formula()
+
+
+def test_emulated_choice_special_cases():
+    m, n, k = 5, 4, 30
+    a = oracle.fill_uniform(m, k, 1)
+    b = oracle.fill_uniform(k, n, 2)
+    # Forced counts are clamped, NaN/Inf give 0.
+    assert oracle.emulated_choose("int8", "N", "N", m, n, k, a, m, b, k, forced=30)[0] == 24
+    assert oracle.emulated_choose("bf16", "N", "N", m, n, k, a, m, b, k, forced=30)[0] == 16
+    a2 = a.copy(order="F")
+    a2[1, 1] = np.inf
+    assert oracle.emulated_choose("int8", "N", "N", m, n, k, a2, m, b, k)[0] == 0
+    assert oracle.emulated_choose("int8", "N", "N", m, n, k, a2, m, b, k, forced=9)[0] == 0
+    # All-zero op(A): every pair is exact, the smallest count.
+    z = np.zeros((m, k), order="F")
+    assert oracle.emulated_choose("int8", "N", "N", m, n, k, z, m, b, k)[0] == 2
+    # Disjoint supports: |A||B| = 0 where the rows and columns are not zero.
+    a3 = a.copy(order="F")
+    b3 = b.copy(order="F")
+    a3[:, 15:] = 0.0
+    b3[:15, :] = 0.0
+    assert oracle.emulated_choose("int8", "N", "N", m, n, k, a3, m, b3, k)[0] == 0
+    # k too small for the guarantee to leave room: native.
+    assert oracle.emulated_choose("int8", "N", "N", m, n, 3, a, m, b, 3)[0] == 0
