@@ -95,6 +95,8 @@ def _declare(L):
                                           _i, _p, _pi])
     fn("hwyblas_dgemm_emulated_bf16_dev", _i, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p,
                                                _i, _i, _p, _pi])
+    fn("hwyblas_dgemm_emulated_noalloc_dev", _i, [_c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d,
+                                                  _p, _i, _i, _p, _pi])
     fn("hwyblas_gemm_plan_create", _p, [_i, _i, _p, _p, _pi, _pi])
     fn("hwyblas_trsm_plan_create", _p, [_i, _i, _p, _pi, _pi])
     fn("hwyblas_plan_launch", None, [_p, _p])
@@ -462,6 +464,19 @@ def dgemm_emulated_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nmo
     info = lib().hwyblas_dgemm_emulated_dev(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(a),
                                             lda, _ptr(b), ldb, beta, _ptr(c), ldc, nmoduli,
                                             stream, ctypes.byref(used) if report else None)
+    return info, (used.value if report else None)
+
+
+def dgemm_emulated_noalloc_dev(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nmoduli=0,
+                               stream=None, report=True):
+    """The INT8 emulated DGEMM on the library's fixed workspace (no
+    allocation); returns (info, moduli used), info -1 when the problem needs
+    more workspace than the library holds."""
+    used = ctypes.c_int(0)
+    info = lib().hwyblas_dgemm_emulated_noalloc_dev(ta.encode(), tb.encode(), m, n, k, alpha,
+                                                    _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c),
+                                                    ldc, nmoduli, stream,
+                                                    ctypes.byref(used) if report else None)
     return info, (used.value if report else None)
 
 

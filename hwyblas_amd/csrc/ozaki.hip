@@ -26,10 +26,13 @@
 // FP64 kernel runs. Inputs with NaN or Inf, which the integer scheme cannot
 // carry, also run natively, as does k >= 131072. The decision never returns
 // to the host, so a call is asynchronous unless it reports the count.
+#include <stdio.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "common.h"
 #include "ozaki_shared.h"
@@ -731,9 +734,33 @@ int ReportCount(const EmuStats& st, hipStream_t stream) {
   return count;
 }
 
-void EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
-                   const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
-                   double* c, int64_t ldc, int nmod, hipStream_t stream, int* used) {
+namespace {
+
+// Workspace of one call: statistics, then kMaxModuli planes of each kind
+// (the count is decided on the device).
+struct EmuLayout {
+  int64_t MP, NP, KP;
+  size_t o_a8, o_b8, o_r, total;
+};
+
+EmuLayout LayoutOf(int64_t m, int64_t n, int64_t k) {
+  EmuLayout w;
+  w.MP = AlignUp(m, kTileM);
+  w.NP = AlignUp(n, kTileN);
+  w.KP = AlignUp(k, kPadK);
+  w.o_a8 = AlignUp(StatsBytes(m, n), 256);
+  w.o_b8 = w.o_a8 + AlignUp(size_t(kMaxModuli) * w.MP * w.KP, 256);
+  w.o_r = w.o_b8 + AlignUp(size_t(kMaxModuli) * w.NP * w.KP, 256);
+  w.total = w.o_r + size_t(kMaxModuli) * w.MP * w.NP;
+  return w;
+}
+
+// The emulated DGEMM on a given workspace (at least LayoutOf(...).total
+// bytes), or on the calling thread's grow-only one when `workspace` is null.
+void EmulatedDgemmOn(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                     const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                     double* c, int64_t ldc, int nmod, hipStream_t stream, int* used,
+                     void* workspace) {
   ResetPhases();
   if (k >= 131072) {  // beyond exact INT32 accumulation of the products
     GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
@@ -750,19 +777,13 @@ void EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alp
   op.m = m;
   op.n = n;
   op.k = k;
-  const int64_t MP = AlignUp(m, kTileM), NP = AlignUp(n, kTileN), KP = AlignUp(k, kPadK);
-  // Workspace: statistics, then kMaxModuli planes of each kind (the count
-  // is decided on the device).
-  const size_t o_stats = 0;
-  const size_t o_a8 = AlignUp(StatsBytes(m, n), 256);
-  const size_t o_b8 = o_a8 + AlignUp(size_t(kMaxModuli) * MP * KP, 256);
-  const size_t o_r = o_b8 + AlignUp(size_t(kMaxModuli) * NP * KP, 256);
-  const size_t total = o_r + size_t(kMaxModuli) * MP * NP;
-  char* ws = static_cast<char*>(GetWorkspace(total, stream));
-  const EmuStats st = CarveStats(ws + o_stats, m, n);
-  int8_t* a8 = reinterpret_cast<int8_t*>(ws + o_a8);
-  int8_t* b8 = reinterpret_cast<int8_t*>(ws + o_b8);
-  uint8_t* rr = reinterpret_cast<uint8_t*>(ws + o_r);
+  const EmuLayout w = LayoutOf(m, n, k);
+  const int64_t MP = w.MP, NP = w.NP, KP = w.KP;
+  char* ws = static_cast<char*>(workspace ? workspace : GetWorkspace(w.total, stream));
+  const EmuStats st = CarveStats(ws, m, n);
+  int8_t* a8 = reinterpret_cast<int8_t*>(ws + w.o_a8);
+  int8_t* b8 = reinterpret_cast<int8_t*>(ws + w.o_b8);
+  uint8_t* rr = reinterpret_cast<uint8_t*>(ws + w.o_r);
 
   // 1. Statistics and the count of moduli (plane 0 holds the coarse
   //    magnitudes until the slices overwrite it).
@@ -822,6 +843,44 @@ void EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alp
   if (used) *used = ReportCount(st, stream);
 }
 
+// The workspace of the entry that allocates nothing: a fixed array in every
+// device's copy of the code object, held from load until the process ends.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// The device address of the current device's `scratch` (each GPU holds its
+// own copy at its own address), looked up once per device. The array's name
+// is a host-side stand-in and is never passed to a kernel or a copy.
+void* ScratchAddress() {
+  constexpr int kDevices = 64;
+  static std::mutex mu;
+  static void* address[kDevices] = {};
+  int dev = 0;
+  HWY_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (address[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      fprintf(stderr, "hwyblas_amd: hipGetSymbolAddress(scratch) failed on device %d: %s\n",
+              dev, hipGetErrorString(e));
+      return nullptr;
+    }
+    address[dev] = p;
+  }
+  return address[dev];
+}
+
+}  // namespace
+
+void EmulatedDgemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                   const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                   double* c, int64_t ldc, int nmod, hipStream_t stream, int* used) {
+  EmulatedDgemmOn(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nmod, stream, used,
+                  nullptr);
+}
+
 }  // namespace ozaki
 }  // namespace hwy_amd
 
@@ -860,6 +919,55 @@ HWYBLAS_AMD_API int hwyblas_dgemm_emulated_dev(char transa, char transb, int m, 
   }
   ozaki::EmulatedDgemm(oa == Op::kTrans, ob == Op::kTrans, m, n, k, alpha, a, lda, b, ldb, beta,
                        c, ldc, nmoduli, s, used_moduli);
+  return 0;
+}
+
+HWYBLAS_AMD_API int hwyblas_dgemm_emulated_noalloc_dev(char transa, char transb, int m, int n,
+                                                       int k, double alpha, const double* a,
+                                                       int lda, const double* b, int ldb,
+                                                       double beta, double* c, int ldc,
+                                                       int nmoduli, void* stream,
+                                                       int* used_moduli) {
+  Op oa, ob;
+  const bool va = DecodeOp(transa, &oa), vb = DecodeOp(transb, &ob);
+  const int rows_a = oa == Op::kNoTrans ? m : k, rows_b = ob == Op::kNoTrans ? k : n;
+  int info = 0;
+  if (!va) info = 1;
+  else if (!vb) info = 2;
+  else if (m < 0) info = 3;
+  else if (n < 0) info = 4;
+  else if (k < 0) info = 5;
+  else if (lda < std::max(1, rows_a)) info = 8;
+  else if (ldb < std::max(1, rows_b)) info = 10;
+  else if (ldc < std::max(1, m)) info = 13;
+  if (info != 0) {
+    Reject("DGEMM", info);
+    return info;
+  }
+  if (used_moduli) *used_moduli = 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (m == 0 || n == 0) return 0;
+  if ((k == 0 || alpha == 0.0) && beta == 1.0) return 0;
+  if (k == 0 || alpha == 0.0) {
+    ScaleDevice<double>(m, n, beta, c, ldc, s);
+    return 0;
+  }
+  const bool ta = oa == Op::kTrans, tb = ob == Op::kTrans;
+  void* ws = nullptr;
+  if (k < 131072) {  // beyond, the native kernel runs and no workspace is used
+    const size_t need = ozaki::LayoutOf(m, n, k).total;
+    if (need > ozaki::kScratchBytes) {
+      fprintf(stderr,
+              "hwyblas_dgemm_emulated_noalloc_dev: m = %d, n = %d, k = %d needs %zu bytes of "
+              "workspace, more than the %zu bytes the library holds; nothing was done\n",
+              m, n, k, need, ozaki::kScratchBytes);
+      return -1;
+    }
+    ws = ozaki::ScratchAddress();
+    if (ws == nullptr) return -2;
+  }
+  ozaki::EmulatedDgemmOn(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nmoduli, s,
+                         used_moduli, ws);
   return 0;
 }
 

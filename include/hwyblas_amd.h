@@ -364,6 +364,30 @@ HWYBLAS_AMD_API int hwyblas_dgemm_emulated_dev(char transa, char transb, int m, 
                                                int nmoduli, void* stream,
                                                int* used_moduli);
 
+/* The INT8 emulated DGEMM that allocates nothing (no hwyBLAS counterpart;
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * checks, quick returns, count rule and results (bit for bit) as
+ * hwyblas_dgemm_emulated_dev, but its workspace is a fixed 64 MiB array
+ * that every GPU the library runs on holds from load until the process ends,
+ * instead of a grow-only per-thread buffer; the first call on a device looks
+ * up that array's address (no allocation). It needs
+ * S + 24 (MP KP + NP KP + MP NP) bytes, MP and NP = m and n rounded up to
+ * 256, KP = k rounded up to 128, S a few (m + n) words of statistics: for
+ * example m = n = 768 with k up to 1280 (61 MB). A problem that needs
+ * more returns -1 with a message on stderr and enqueues nothing (k >= 131072
+ * runs the native kernel and needs none); -2 if the array's address cannot be
+ * obtained. Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (several streams or threads at once) race and must be ordered by
+ * the caller. With used_moduli NULL the call neither allocates nor
+ * synchronizes. Returns 0, the Fortran info, -1 or -2. */
+HWYBLAS_AMD_API int hwyblas_dgemm_emulated_noalloc_dev(char transa, char transb, int m, int n,
+                                                       int k, double alpha, const double* a,
+                                                       int lda, const double* b, int ldb,
+                                                       double beta, double* c, int ldc,
+                                                       int nmoduli, void* stream,
+                                                       int* used_moduli);
+
 /* Emulated DGEMM on the BF16 matrix cores (no hwyBLAS counterpart): the
  * Ozaki scheme I with BF16 slices. Same arguments, checks and quick returns
  * as hwyblas_dgemm_emulated_dev. Rows of op(A) and columns of op(B) are
