@@ -73,6 +73,8 @@ def _declare(L):
     fn("dgetrf_", None, [_pi, _pi, _p, _pi, _p, _pi])
     fn("dgetrs_", None, [ctypes.c_char_p, _pi, _pi, _p, _pi, _p, _p, _pi, _pi])
     fn("dgesv_", None, [_pi, _pi, _p, _pi, _p, _p, _pi, _pi])
+    fn("dgerfs_", None, [ctypes.c_char_p, _pi, _pi, _p, _pi, _p, _pi, _p, _p, _pi, _p, _pi, _p,
+                         _p, _p, _p, _pi])
     for w in ("", "_work"):
         fn(f"LAPACKE_dgetrf{w}", _i, [_i, _i, _i, _p, _i, _p])
         fn(f"LAPACKE_dgetrs{w}", _i, [_i, ctypes.c_char, _i, _i, _p, _i, _p, _p, _i])
@@ -133,6 +135,12 @@ def _declare(L):
     fn("hwyblas_grouped_set_model", None, [_p])
     fn("hwyblas_dgetrf_dev", _i, [_i, _i, _p, _i, _p, _i, _p])
     fn("hwyblas_dgetrs_dev", _i, [ctypes.c_char, _i, _i, _p, _i, _p, _p, _i, _p])
+    fn("hwyblas_set_dgemm_mode", _i, [_i])
+    fn("hwyblas_get_dgemm_mode", _i, [])
+    fn("dgeqrf_", None, [_pi, _pi, _p, _pi, _p, _p, _pi, _pi])
+    fn("hwyblas_dgeqrf_dev", _i, [_i, _i, _p, _i, _p, _p])
+    fn("hwyblas_dgerfs_dev", _i, [ctypes.c_char, _i, _i, _p, _i, _p, _i, _p, _p, _i, _p, _i, _p,
+                                  _p, _p])
     fn("hwyblas_dgesv_ir_dev", _i, [_i, _i, _p, _i, _p, _i, _p, _p, _i, _p, _i, _i, _i, _p, _p,
                                     _p, _p, _p])
     fn("hwyblas_gemm_config_count", _i, [])
@@ -420,6 +428,36 @@ def gesv(n, nrhs, a, lda, ipiv, b, ldb):
     lib().dgesv_(_ip(n), _ip(nrhs), _ptr(a), _ip(lda), _ptr(ipiv), _ptr(b), _ip(ldb),
                  ctypes.byref(info))
     return info.value
+
+
+def geqrf(m, n, a, lda, tau, lwork=None):
+    """Fortran dgeqrf_ (host arrays or DeviceArrays for A and tau); returns
+    (info, work[0])."""
+    info = ctypes.c_int(0)
+    work = np.zeros(1)
+    lw = max(1, n) if lwork is None else lwork
+    lib().dgeqrf_(_ip(m), _ip(n), _ptr(a), _ip(lda), _ptr(tau), _ptr(work), _ip(lw),
+                  ctypes.byref(info))
+    return info.value, float(work[0])
+
+
+def geqrf_dev(a, tau, stream=None):
+    """hwyblas_dgeqrf_dev on a DeviceArray (m = a.ld rows); tau a DeviceArray."""
+    return lib().hwyblas_dgeqrf_dev(a.ld, a.cols, a.ptr, a.ld, tau.ptr, stream)
+
+
+def gerfs(trans, n, nrhs, a, lda, af, ldaf, ipiv, b, ldb, x, ldx):
+    """Fortran dgerfs_ (host arrays or DeviceArrays for the matrices and
+    pivots); X refined in place. Returns (info, ferr, berr)."""
+    info = ctypes.c_int(0)
+    ferr = np.zeros(max(1, nrhs))
+    berr = np.zeros(max(1, nrhs))
+    work = np.zeros(max(1, 3 * n))
+    iwork = np.zeros(max(1, n), dtype=np.int32)
+    lib().dgerfs_(trans.encode(), _ip(n), _ip(nrhs), _ptr(a), _ip(lda), _ptr(af), _ip(ldaf),
+                  _ptr(ipiv), _ptr(b), _ip(ldb), _ptr(x), _ip(ldx), _ptr(ferr), _ptr(berr),
+                  _ptr(work), _ptr(iwork), ctypes.byref(info))
+    return info.value, ferr[:nrhs], berr[:nrhs]
 
 
 def lapacke_lu(routine, layout, *args, work=False):

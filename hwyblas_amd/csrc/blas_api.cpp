@@ -8,14 +8,48 @@
 // when beta != 0, and only rows [0, m) of each column are copied back, so
 // the padding of C between m and ldc is never written. Device operands
 // (hipMalloc or managed memory) are used in place.
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <type_traits>
 
 #include "entries.h"
 
 namespace hwy_amd {
+
+// How DGEMM-shaped work on the drop-in entries runs: 0 native FP64 (the
+// default), 1 the emulated DGEMM on the INT8 matrix cores with the count
+// chosen on the device and hwyBLAS's componentwise bound guaranteed (the
+// native kernel where it cannot be certified), for products large enough
+// to gain (m, n >= 2048 and k >= 1024). The environment variable
+// HWYBLAS_AMD_DGEMM=emulated sets 1 at load; hwyblas_set_dgemm_mode changes
+// it for every thread.
+std::atomic<int>& DgemmModeFlag() {
+  static std::atomic<int> mode{[] {
+    const char* e = getenv("HWYBLAS_AMD_DGEMM");
+    return (e && strcmp(e, "emulated") == 0) ? 1 : 0;
+  }()};
+  return mode;
+}
+
+int DgemmMode() { return DgemmModeFlag().load(std::memory_order_relaxed); }
+
 namespace {
+
+// C = alpha op(A) op(B) + beta C on device operands, by the current mode.
+template <typename T>
+void GemmByMode(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
+                int64_t lda, const T* b, int64_t ldb, T beta, T* c, int64_t ldc, hipStream_t s) {
+  if constexpr (std::is_same_v<T, double>) {
+    if (DgemmMode() == 1 && m >= 2048 && n >= 2048 && k >= 1024) {
+      ozaki::EmulatedDgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, 0, s, nullptr);
+      return;
+    }
+  }
+  GemmDevice<T>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
+}
 
 template <typename T>
 void CopyIn(T* dst, const T* src, int64_t rows, int64_t cols, int64_t ld,
@@ -118,7 +152,7 @@ void GemmHost(const Caller& who, const char* transa, const char* transb,
       db = buf;
       ldb_d = rb;
     }
-    GemmDevice<T>(ta, tb, m, n, k, alpha, da, lda_d, db, ldb_d, beta, dc, ldc_d, s);
+    GemmByMode<T>(ta, tb, m, n, k, alpha, da, lda_d, db, ldb_d, beta, dc, ldc_d, s);
   }
   if (!c_dev) CopyOut(c, ldc, dc, m, n, s);
   HWY_HIP_CHECK(hipStreamSynchronize(s));
@@ -140,8 +174,8 @@ int GemmDev(const Caller& who, char transa, char transb, int m, int n, int k,
     ScaleDevice<T>(m, n, beta, c, ldc, s);
     return 0;
   }
-  GemmDevice<T>(oa == Op::kTrans, ob == Op::kTrans, m, n, k, alpha, a, lda, b,
-                ldb, beta, c, ldc, s);
+  GemmByMode<T>(oa == Op::kTrans, ob == Op::kTrans, m, n, k, alpha, a, lda, b, ldb, beta, c,
+                ldc, s);
   return 0;
 }
 
@@ -233,6 +267,13 @@ HWY_DEFINE_CORES(double)
 using hwy_amd::blasint;
 
 extern "C" {
+
+HWYBLAS_AMD_API int hwyblas_set_dgemm_mode(int mode) {
+  if (mode != 0 && mode != 1) return -1;
+  return hwy_amd::DgemmModeFlag().exchange(mode, std::memory_order_relaxed);
+}
+
+HWYBLAS_AMD_API int hwyblas_get_dgemm_mode(void) { return hwy_amd::DgemmMode(); }
 
 HWYBLAS_AMD_API void sgemm_(const char* transa, const char* transb,
                             const blasint* m, const blasint* n, const blasint* k,

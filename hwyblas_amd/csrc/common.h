@@ -80,6 +80,11 @@ void Reject(const Caller& who, blasint info);
 // Reports a CBLAS layout that is neither row- nor column-major (argument 1).
 void RejectLayout(const char* name, int layout);
 
+// Reports a routine's internal failure `code` (negative, not an argument
+// position) to the installed handler, if any, as (name, code); the routine
+// has already printed what failed.
+void ReportInternal(const char* name, int code);
+
 // Prints the HIP error and aborts: device failures have no BLAS error code,
 // and a routine that cannot run must not return as if it had.
 [[noreturn]] void FatalHip(hipError_t e, const char* what, const char* file,
@@ -200,6 +205,10 @@ void GemmDeviceGated<double>(const int* gate, bool ta, bool tb, int64_t m, int64
                              int64_t k, double alpha, const double* a, int64_t lda,
                              const double* b, int64_t ldb, double beta, double* c, int64_t ldc,
                              hipStream_t stream);
+
+// The DGEMM mode of the drop-in entries (hwyblas_set_dgemm_mode): 0 native
+// FP64, 1 the emulated DGEMM for large products.
+int DgemmMode();
 
 namespace ozaki {
 // The emulated DGEMMs (ozaki.hip: INT8 moduli; ozaki_bf16.hip: BF16 slices)

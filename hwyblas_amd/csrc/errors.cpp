@@ -51,6 +51,10 @@ void RejectLayout(const char* name, int layout) {
   }
 }
 
+void ReportInternal(const char* name, int code) {
+  if (hwyblas_error_handler h = installed_handler.load()) h(name, code);
+}
+
 void FatalHip(hipError_t e, const char* what, const char* file, int line) {
   fprintf(stderr, "hwyblas_amd: HIP error %d (%s) in %s at %s:%d\n",
           static_cast<int>(e), hipGetErrorString(e), what, file, line);

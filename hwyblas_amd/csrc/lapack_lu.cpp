@@ -1,7 +1,8 @@
 // Fortran LAPACK entries of the LU solver: dgetrf_, dgetrs_, dgesv_ (hwyBLAS
 // src/lapack/getrf.cc:112-339 for the routines, :341-367 for xGESV's checks,
-// :500-540 for the entries), over the device path (hwyblas_dgetrf_dev,
-// hwyblas_dgetrs_dev; FP64 trailing updates). Host or device operands: host
+// :500-540 for the entries) and dgerfs_ (src/lapack/gesvx.cc:333-521), over
+// the device path (hwyblas_dgetrf_dev, hwyblas_dgetrs_dev,
+// hwyblas_dgerfs_dev; FP64 trailing updates). Host or device operands: host
 // matrices and pivots are staged in the calling thread's device buffers and
 // copied back. Each call synchronizes.
 #include <algorithm>
@@ -78,7 +79,8 @@ blasint Getrf(blasint m, blasint n, double* a, blasint lda, blasint* ipiv) {
   hipStream_t s = Context().stream;
   Staged A(a, m, n, lda, kSlotA, true, s);
   StagedPivots P(ipiv, std::min(m, n), false, s);
-  const int info = hwyblas_dgetrf_dev(m, n, A.dev, int(A.ld), P.dev, 0, s);
+  // Trailing updates on the emulated DGEMM when the DGEMM mode asks for it.
+  const int info = hwyblas_dgetrf_dev(m, n, A.dev, int(A.ld), P.dev, DgemmMode() == 1 ? 1 : 0, s);
   A.Finish(s);
   P.Finish(s);
   HWY_HIP_CHECK(hipStreamSynchronize(s));
@@ -104,6 +106,34 @@ blasint Getrs(char trans, blasint n, blasint nrhs, const double* a, blasint lda,
   return info;
 }
 
+blasint Gerfs(char trans, blasint n, blasint nrhs, const double* a, blasint lda,
+              const double* af, blasint ldaf, const blasint* ipiv, const double* b, blasint ldb,
+              double* x, blasint ldx, double* ferr, double* berr) {
+  Op op;
+  if (!DecodeOp(trans, &op)) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (lda < std::max(1, n)) return -5;
+  if (ldaf < std::max(1, n)) return -7;
+  if (ldb < std::max(1, n)) return -10;
+  if (ldx < std::max(1, n)) return -12;
+  if (n == 0 || nrhs == 0) {
+    for (blasint j = 0; j < nrhs; ++j) ferr[j] = berr[j] = 0.0;
+    return 0;
+  }
+  hipStream_t s = Context().stream;
+  Staged A(const_cast<double*>(a), n, n, lda, kSlotA, true, s);
+  Staged AF(const_cast<double*>(af), n, n, ldaf, kSlotPackA, true, s);
+  StagedPivots P(const_cast<blasint*>(ipiv), n, true, s);
+  Staged B(const_cast<double*>(b), n, nrhs, ldb, kSlotB, true, s);
+  Staged X(x, n, nrhs, ldx, kSlotC, true, s);
+  const int info = hwyblas_dgerfs_dev(trans, n, nrhs, A.dev, int(A.ld), AF.dev, int(AF.ld),
+                                      P.dev, B.dev, int(B.ld), X.dev, int(X.ld), ferr, berr, s);
+  X.Finish(s);
+  HWY_HIP_CHECK(hipStreamSynchronize(s));
+  return info;
+}
+
 }  // namespace
 }  // namespace hwy_amd
 
@@ -112,7 +142,8 @@ extern "C" {
 HWYBLAS_AMD_API void dgetrf_(const hwyblas_int* m, const hwyblas_int* n, double* a,
                              const hwyblas_int* lda, hwyblas_int* ipiv, hwyblas_int* info) {
   *info = hwy_amd::Getrf(*m, *n, a, *lda, ipiv);
-  if (*info < 0 && *info != -1000) hwy_amd::Reject("DGETRF", -*info);
+  if (*info == -1000) hwy_amd::ReportInternal("DGETRF", -1000);
+  else if (*info < 0) hwy_amd::Reject("DGETRF", -*info);
 }
 
 HWYBLAS_AMD_API void dgetrs_(const char* trans, const hwyblas_int* n, const hwyblas_int* nrhs,
@@ -120,6 +151,19 @@ HWYBLAS_AMD_API void dgetrs_(const char* trans, const hwyblas_int* n, const hwyb
                              double* b, const hwyblas_int* ldb, hwyblas_int* info) {
   *info = hwy_amd::Getrs(*trans, *n, *nrhs, a, *lda, ipiv, b, *ldb);
   if (*info < 0) hwy_amd::Reject("DGETRS", -*info);
+}
+
+HWYBLAS_AMD_API void dgerfs_(const char* trans, const hwyblas_int* n, const hwyblas_int* nrhs,
+                             const double* a, const hwyblas_int* lda, const double* af,
+                             const hwyblas_int* ldaf, const hwyblas_int* ipiv, const double* b,
+                             const hwyblas_int* ldb, double* x, const hwyblas_int* ldx,
+                             double* ferr, double* berr, double* work, hwyblas_int* iwork,
+                             hwyblas_int* info) {
+  (void)work;  // the GPU path keeps its work arrays on the device
+  (void)iwork;
+  *info = hwy_amd::Gerfs(*trans, *n, *nrhs, a, *lda, af, *ldaf, ipiv, b, *ldb, x, *ldx, ferr,
+                         berr);
+  if (*info < 0) hwy_amd::Reject("DGERFS", -*info);
 }
 
 HWYBLAS_AMD_API void dgesv_(const hwyblas_int* n, const hwyblas_int* nrhs, double* a,
@@ -137,6 +181,7 @@ HWYBLAS_AMD_API void dgesv_(const hwyblas_int* n, const hwyblas_int* nrhs, doubl
     return;
   }
   *info = hwy_amd::Getrf(N, N, a, *lda, ipiv);
+  if (*info == -1000) hwy_amd::ReportInternal("DGESV", -1000);
   if (*info == 0) *info = hwy_amd::Getrs('N', N, R, a, *lda, ipiv, b, *ldb);
 }
 

@@ -204,7 +204,7 @@ constexpr int kMaxAffected = 2 * kPanel;
 // entries through LDS, all loads before all stores.
 __global__ void __launch_bounds__(64)
     PanelSwapKernel(double* __restrict__ full, int64_t lda, int ncols, int pc0, int nb,
-                    const int* __restrict__ ipiv, int count, int row0) {
+                    const int* __restrict__ ipiv, int count, int row0, int rows) {
   __shared__ int L[kMaxAffected], S[kMaxAffected];
   __shared__ int naff;
   __shared__ double buf[kMaxAffected][64 + 1];
@@ -218,7 +218,10 @@ __global__ void __launch_bounds__(64)
   int extra_row = -1, extra_src = -1, na = count;
   __syncthreads();
   for (int i = 0; i < count; ++i) {
-    const int q = ipiv[i] - 1 - row0;  // relative to the panel's first row
+    // Relative to the panel's first row; a pivot outside the panel's rows
+    // (not written by a panel kernel that gave up) is no interchange.
+    int q = ipiv[i] - 1 - row0;
+    if (q < i || q >= rows) q = i;
     if (q < count) {
       if (lane == 0) {
         const int tmp = S[i];
@@ -289,6 +292,14 @@ __global__ void CopyBack(const double* __restrict__ in, int n, int ncols, double
   for (int c = 0; c < ncols; ++c) b[i + int64_t(c) * ldb] = in[i + int64_t(c) * n];
 }
 
+
+// ipiv[j] = j + 1: no interchange. Written before the factorization, so
+// that a panel whose kernel gave up (a timed-out barrier) leaves in-range
+// pivots for the row interchanges that follow it.
+__global__ void IdentityPivotsKernel(int* __restrict__ ipiv, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) ipiv[i] = i + 1;
+}
 
 __global__ void FillInt(int* __restrict__ x, int count, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -798,7 +809,7 @@ void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, 
   const int others = w.ncols - nb;
   if (others > 0)
     hipLaunchKernelGGL(PanelSwapKernel, dim3((others + 63) / 64), dim3(64), 0, s, full, lda,
-                       w.ncols, pc0, nb, ipiv, mn, row0);
+                       w.ncols, pc0, nb, ipiv, mn, row0, m);
   // The inverse of the panel's unit lower triangle (final from here on: later
   // interchanges only move rows below it), for the TRSMs with it; the
   // cooperative panel kernel has already written it.
@@ -926,7 +937,16 @@ int Getrf(int m, int n, double* a, int64_t lda, int* ipiv, int update, hipStream
     HWY_HIP_CHECK(hipMalloc(&w.prec, size_t(2) * (g + 1) * kRecord * sizeof(double)));
     HWY_HIP_CHECK(hipMalloc(&w.sync, 512));
     hipLaunchKernelGGL(FillUnsigned, dim3(1), dim3(128), 0, s, w.sync, 128, 0u);
+    // Test hook: HWYBLAS_AMD_LU_FAIL_BARRIER=1 raises the failure flag before
+    // the first panel, so that every bounded wait gives up at its first poll
+    // and every workgroup returns: the path a timed-out barrier takes, without
+    // waiting for one.
+    const char* fail = getenv("HWYBLAS_AMD_LU_FAIL_BARRIER");
+    if (fail && *fail == '1')
+      hipLaunchKernelGGL(FillUnsigned, dim3(1), dim3(64), 0, s, w.sync + 64, 1, 1u);
   }
+  const int kmin = std::min(m, n);
+  hipLaunchKernelGGL(IdentityPivotsKernel, dim3((kmin + 255) / 256), dim3(256), 0, s, ipiv, kmin);
   Recurse(a, lda, m, n, 0, ipiv, update, w, s);
   int info = INT_MAX;
   unsigned failed = 0;
@@ -1365,6 +1385,52 @@ __global__ void __launch_bounds__(256)
   if (ap) ap[int64_t(blockIdx.y) * n + i] = sa;
 }
 
+// The same sums for op(A) = A^T, in ResidualPartialKernel's layout with one
+// chunk: rp(i) = sum_k a(k, i) x(k), wp(i) = sum_k |a(k, i)| |x(k)| for every
+// right-hand side, and ap(i) = sum_k |a(k, i)| (row sums of |A^T|). One wave
+// per column i of A (contiguous), its lanes striding k; the lanes' sums are
+// combined by a fixed butterfly.
+__global__ void __launch_bounds__(256)
+    ResidualTransKernel(const double* __restrict__ a, int64_t lda, int n,
+                        const double* __restrict__ x, int64_t ldx, int nrhs,
+                        double* __restrict__ rp, double* __restrict__ wp,
+                        double* __restrict__ ap) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;  // uniform per wave
+  double r[kMaxRhs], w[kMaxRhs];
+#pragma unroll
+  for (int j = 0; j < kMaxRhs; ++j) r[j] = w[j] = 0.0;
+  double sa = 0.0;
+  const double* ai = a + int64_t(i) * lda;
+  for (int k = lane; k < n; k += 64) {
+    const double v = ai[k];
+    const double av = fabs(v);
+    sa += av;
+#pragma unroll
+    for (int j = 0; j < kMaxRhs; ++j) {
+      const double xk = j < nrhs ? x[k + int64_t(j) * ldx] : 0.0;
+      r[j] = __builtin_fma(v, xk, r[j]);
+      w[j] = __builtin_fma(av, fabs(xk), w[j]);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sa += __shfl_xor(sa, off, 64);
+#pragma unroll
+    for (int j = 0; j < kMaxRhs; ++j) {
+      r[j] += __shfl_xor(r[j], off, 64);
+      w[j] += __shfl_xor(w[j], off, 64);
+    }
+  }
+  if (lane == 0) {
+    for (int j = 0; j < nrhs; ++j) {
+      rp[int64_t(j) * n + i] = r[j];
+      wp[int64_t(j) * n + i] = w[j];
+    }
+    if (ap) ap[i] = sa;
+  }
+}
+
 // R = B - sum rp; D = |B| + sum wp; per workgroup and right-hand side the
 // largest berr term and the largest |R| (columns of `stats`: 2 * nrhs).
 // With w_out, also the weights of the forward error bound,
@@ -1559,8 +1625,9 @@ class NormEstimator {
 // A^T, then times W) and those that want M^T x (times W, then getrs with
 // A) into z, one solve per kind. w: the weights (n x nrhs, leading
 // dimension n); z: n x nrhs scratch.
-void ForwardErrorEstimates(int n, int nrhs, const double* af, int64_t ldaf, const Perm& perm,
-                           const double* w, double* z, hipStream_t s, double* est) {
+void ForwardErrorEstimates(bool trans, int n, int nrhs, const double* af, int64_t ldaf,
+                           const Perm& perm, const double* w, double* z, hipStream_t s,
+                           double* est) {
   std::vector<NormEstimator> ests;
   std::vector<NormEstimator::Request> req(nrhs);
   for (int j = 0; j < nrhs; ++j) {
@@ -1584,15 +1651,16 @@ void ForwardErrorEstimates(int n, int nrhs, const double* af, int64_t ldaf, cons
     HWY_HIP_CHECK(hipMemcpyAsync(z, packed.data(), size_t(n) * (c1 + c2) * 8,
                                  hipMemcpyHostToDevice, s));
     double* z2 = z + size_t(c1) * n;
+    // M = diag(W) op(A)^-T: M x solves with op(A)^T first, M^T x with op(A).
     if (c1 > 0) {
-      GetrsPerm(true, n, c1, af, ldaf, perm, z, n, s);
+      GetrsPerm(!trans, n, c1, af, ldaf, perm, z, n, s);
       hipLaunchKernelGGL(ScaleByWeightsKernel, dim3(rblocks, c1), dim3(256), 0, s, z, n, w,
                          first);
     }
     if (c2 > 0) {
       hipLaunchKernelGGL(ScaleByWeightsKernel, dim3(rblocks, c2), dim3(256), 0, s, z2, n, w,
                          second);
-      GetrsPerm(false, n, c2, af, ldaf, perm, z2, n, s);
+      GetrsPerm(trans, n, c2, af, ldaf, perm, z2, n, s);
     }
     HWY_HIP_CHECK(hipGetLastError());
     HWY_HIP_CHECK(hipMemcpyAsync(packed.data(), z, size_t(n) * (c1 + c2) * 8,
@@ -1605,6 +1673,107 @@ void ForwardErrorEstimates(int n, int nrhs, const double* af, int64_t ldaf, cons
     }
   }
   for (int j = 0; j < nrhs; ++j) est[j] = ests[j].estimate();
+}
+
+// gerfs's refinement (src/lapack/gesvx.cc:333-521) of X (n x nrhs,
+// nrhs <= kMaxRhs) for op(A) X = B with the factors af and their
+// permutation: per column, while berr > eps, berr at most half the last one
+// and fewer than itmax corrections, X += op(A)^-1 (B - op(A) X), all columns
+// of a round in one residual pass over A and one solve; then, with ferr, the
+// forward error bound of the final X. berr, ferr, nbe (the normwise backward
+// error ||B - op(A) X||inf / (||op(A)||inf ||X||inf)) and iterations (the
+// corrections applied) are host arrays; ferr, nbe and iterations may be null.
+void Refine(bool trans, int n, int nrhs, const double* a, int64_t lda, const double* af,
+            int64_t ldaf, const Perm& perm, const double* b, int64_t ldb, double* x,
+            int64_t ldx, int itmax, double* ferr, double* berr, double* nbe, int* iterations,
+            hipStream_t s) {
+  const int nchunks = trans ? 1 : (n + kResChunk - 1) / kResChunk;
+  const int rblocks = (n + 255) / 256;
+  double *rp, *wp, *r, *stats, *rowsum, *rowpart, *wts = nullptr, *z = nullptr;
+  int* active_d;
+  if (ferr) {
+    HWY_HIP_CHECK(hipMalloc(&wts, size_t(n) * nrhs * 8));
+    HWY_HIP_CHECK(hipMalloc(&z, size_t(n) * nrhs * 8));
+  }
+  HWY_HIP_CHECK(hipMalloc(&rp, size_t(nchunks) * kMaxRhs * n * 8));
+  HWY_HIP_CHECK(hipMalloc(&wp, size_t(nchunks) * kMaxRhs * n * 8));
+  HWY_HIP_CHECK(hipMalloc(&r, size_t(n) * nrhs * 8));
+  HWY_HIP_CHECK(hipMalloc(&stats, size_t(rblocks) * 2 * nrhs * 8));
+  HWY_HIP_CHECK(hipMalloc(&rowsum, size_t(n) * 8));
+  HWY_HIP_CHECK(hipMalloc(&rowpart, size_t(nchunks) * n * 8));
+  HWY_HIP_CHECK(hipMalloc(&active_d, kMaxRhs * sizeof(int)));
+  const double eps = std::numeric_limits<double>::epsilon() * 0.5;  // LAPACK's dlamch('E')
+  const double safmin = std::numeric_limits<double>::min();
+  const double safe1 = double(n + 1) * safmin, safe2 = safe1 / eps;
+  std::vector<double> host_stats(size_t(rblocks) * 2 * nrhs);
+  std::vector<double> lstres(nrhs, 3.0), rinf(nrhs, 0.0);
+  std::vector<int> count(nrhs, 1), active(nrhs, 1), done(nrhs, 0);
+  for (bool first = true;; first = false) {
+    if (trans)
+      hipLaunchKernelGGL(ResidualTransKernel, dim3((n + 3) / 4), dim3(256), 0, s, a, lda, n, x,
+                         ldx, nrhs, rp, wp, first ? rowpart : nullptr);
+    else
+      hipLaunchKernelGGL(ResidualPartialKernel, dim3(rblocks, nchunks), dim3(256), 0, s, a,
+                         lda, n, x, ldx, nrhs, rp, wp, first ? rowpart : nullptr);
+    hipLaunchKernelGGL(ResidualFinishKernel, dim3(rblocks), dim3(256), 0, s, rp, wp, nchunks, b,
+                       ldb, n, nrhs, r, int64_t(n), safe1, safe2, stats, wts,
+                       double(n + 1) * eps);
+    HWY_HIP_CHECK(hipGetLastError());
+    HWY_HIP_CHECK(hipMemcpyAsync(host_stats.data(), stats, host_stats.size() * 8,
+                                 hipMemcpyDeviceToHost, s));
+    HWY_HIP_CHECK(hipStreamSynchronize(s));
+    bool any = false;
+    for (int j = 0; j < nrhs; ++j) {
+      double be = 0.0, ri = 0.0;
+      for (int blk = 0; blk < rblocks; ++blk) {
+        be = std::max(be, host_stats[size_t(blk) * 2 * nrhs + j]);
+        ri = std::max(ri, host_stats[size_t(blk) * 2 * nrhs + nrhs + j]);
+      }
+      active[j] = 0;
+      if (done[j]) continue;
+      berr[j] = be;
+      rinf[j] = ri;
+      if (be > eps && 2.0 * be <= lstres[j] && count[j] <= itmax) {
+        active[j] = 1;
+        lstres[j] = be;
+        ++count[j];
+        any = true;
+      } else {
+        done[j] = 1;
+      }
+    }
+    if (!any) break;
+    HWY_HIP_CHECK(hipMemcpyAsync(active_d, active.data(), nrhs * sizeof(int),
+                                 hipMemcpyHostToDevice, s));
+    GetrsPerm(trans, n, nrhs, af, ldaf, perm, r, n, s);
+    hipLaunchKernelGGL(AddMaskedKernel, dim3(rblocks), dim3(256), 0, s, x, ldx, r, int64_t(n), n,
+                       nrhs, active_d);
+    HWY_HIP_CHECK(hipGetLastError());
+  }
+  // The last residual pass was of the final X: its weights bound the error.
+  std::vector<double> est(nrhs, 0.0);
+  if (ferr) ForwardErrorEstimates(trans, n, nrhs, af, ldaf, perm, wts, z, s, est.data());
+  // Normwise backward error of the final X.
+  hipLaunchKernelGGL(RowAbsSumKernel, dim3(rblocks), dim3(256), 0, s, rowpart, nchunks, n,
+                     rowsum);
+  std::vector<double> rows(n), xs(size_t(n) * nrhs);
+  HWY_HIP_CHECK(hipMemcpyAsync(rows.data(), rowsum, size_t(n) * 8, hipMemcpyDeviceToHost, s));
+  HWY_HIP_CHECK(hipMemcpy2DAsync(xs.data(), size_t(n) * 8, x, size_t(ldx) * 8, size_t(n) * 8,
+                                 nrhs, hipMemcpyDeviceToHost, s));
+  HWY_HIP_CHECK(hipStreamSynchronize(s));
+  const double anorm = *std::max_element(rows.begin(), rows.end());
+  for (int j = 0; j < nrhs; ++j) {
+    double xn = 0.0;
+    for (int i = 0; i < n; ++i) xn = std::max(xn, std::fabs(xs[size_t(j) * n + i]));
+    if (nbe) nbe[j] = (anorm > 0 && xn > 0) ? rinf[j] / (anorm * xn) : 0.0;
+    if (ferr) ferr[j] = xn != 0.0 ? est[j] / xn : est[j];
+    if (iterations) iterations[j] = count[j] - 1;
+  }
+  for (void* p : {static_cast<void*>(rp), static_cast<void*>(wp), static_cast<void*>(r),
+                  static_cast<void*>(stats), static_cast<void*>(rowsum),
+                  static_cast<void*>(rowpart), static_cast<void*>(active_d),
+                  static_cast<void*>(wts), static_cast<void*>(z)})
+    if (p) HWY_HIP_CHECK(hipFree(p));
 }
 
 }  // namespace lu
@@ -1696,91 +1865,45 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
                                  hipMemcpyDeviceToDevice, s));
   const lu::Perm perm(n, nrhs, ipiv, s);
   lu::GetrsPerm(false, n, nrhs, af, ldaf, perm, x, ldx, s);
+  lu::Refine(false, n, nrhs, a, lda, af, ldaf, perm, b, ldb, x, ldx, itmax, ferr, berr, nbe,
+             iterations, s);
+  return 0;
+}
 
-  const int nchunks = (n + lu::kResChunk - 1) / lu::kResChunk;
-  const int rblocks = (n + 255) / 256;
-  double *rp, *wp, *r, *stats, *rowsum, *rowpart, *wts = nullptr, *z = nullptr;
-  int* active_d;
-  if (ferr) {
-    HWY_HIP_CHECK(hipMalloc(&wts, size_t(n) * nrhs * 8));
-    HWY_HIP_CHECK(hipMalloc(&z, size_t(n) * nrhs * 8));
+// xGERFS (src/lapack/gesvx.cc:333-521) on device operands: refines X for
+// op(A) X = B with the factors of hwyblas_dgetrf_dev (af, ipiv: device,
+// 1-based), itmax 5, right-hand sides in groups of 16; ferr and berr are
+// host arrays of nrhs. Returns 0 or -i for invalid argument i (reported as
+// DGERFS, positions as xGERFS's). Synchronizes.
+HWYBLAS_AMD_API int hwyblas_dgerfs_dev(char trans, int n, int nrhs, const double* a, int lda,
+                                       const double* af, int ldaf, const int* ipiv,
+                                       const double* b, int ldb, double* x, int ldx,
+                                       double* ferr, double* berr, void* stream) {
+  Op op;
+  int info = 0;
+  if (!DecodeOp(trans, &op)) info = -1;
+  else if (n < 0) info = -2;
+  else if (nrhs < 0) info = -3;
+  else if (lda < std::max(1, n)) info = -5;
+  else if (ldaf < std::max(1, n)) info = -7;
+  else if (ldb < std::max(1, n)) info = -10;
+  else if (ldx < std::max(1, n)) info = -12;
+  if (info != 0) {
+    Reject("DGERFS", -info);
+    return info;
   }
-  HWY_HIP_CHECK(hipMalloc(&rp, size_t(nchunks) * lu::kMaxRhs * n * 8));
-  HWY_HIP_CHECK(hipMalloc(&wp, size_t(nchunks) * lu::kMaxRhs * n * 8));
-  HWY_HIP_CHECK(hipMalloc(&r, size_t(n) * nrhs * 8));
-  HWY_HIP_CHECK(hipMalloc(&stats, size_t(rblocks) * 2 * nrhs * 8));
-  HWY_HIP_CHECK(hipMalloc(&rowsum, size_t(n) * 8));
-  HWY_HIP_CHECK(hipMalloc(&rowpart, size_t(nchunks) * n * 8));
-  HWY_HIP_CHECK(hipMalloc(&active_d, lu::kMaxRhs * sizeof(int)));
-  const double eps = std::numeric_limits<double>::epsilon() * 0.5;  // LAPACK's dlamch('E')
-  const double safmin = std::numeric_limits<double>::min();
-  const double safe1 = double(n + 1) * safmin, safe2 = safe1 / eps;
-  std::vector<double> host_stats(size_t(rblocks) * 2 * nrhs);
-  std::vector<double> lstres(nrhs, 3.0), rinf(nrhs, 0.0);
-  std::vector<int> count(nrhs, 1), active(nrhs, 1), done(nrhs, 0);
-  for (bool first = true;; first = false) {
-    hipLaunchKernelGGL(lu::ResidualPartialKernel, dim3(rblocks, nchunks), dim3(256), 0, s, a,
-                       int64_t(lda), n, x, int64_t(ldx), nrhs, rp, wp,
-                       first ? rowpart : nullptr);
-    hipLaunchKernelGGL(lu::ResidualFinishKernel, dim3(rblocks), dim3(256), 0, s, rp, wp, nchunks,
-                       b, int64_t(ldb), n, nrhs, r, int64_t(n), safe1, safe2, stats, wts,
-                       double(n + 1) * eps);
-    HWY_HIP_CHECK(hipGetLastError());
-    HWY_HIP_CHECK(hipMemcpyAsync(host_stats.data(), stats, host_stats.size() * 8,
-                                 hipMemcpyDeviceToHost, s));
-    HWY_HIP_CHECK(hipStreamSynchronize(s));
-    bool any = false;
-    for (int j = 0; j < nrhs; ++j) {
-      double be = 0.0, ri = 0.0;
-      for (int blk = 0; blk < rblocks; ++blk) {
-        be = std::max(be, host_stats[size_t(blk) * 2 * nrhs + j]);
-        ri = std::max(ri, host_stats[size_t(blk) * 2 * nrhs + nrhs + j]);
-      }
-      active[j] = 0;
-      if (done[j]) continue;
-      berr[j] = be;
-      rinf[j] = ri;
-      if (be > eps && 2.0 * be <= lstres[j] && count[j] <= itmax) {
-        active[j] = 1;
-        lstres[j] = be;
-        ++count[j];
-        any = true;
-      } else {
-        done[j] = 1;
-      }
-    }
-    if (!any) break;
-    HWY_HIP_CHECK(hipMemcpyAsync(active_d, active.data(), nrhs * sizeof(int),
-                                 hipMemcpyHostToDevice, s));
-    lu::GetrsPerm(false, n, nrhs, af, ldaf, perm, r, n, s);
-    hipLaunchKernelGGL(lu::AddMaskedKernel, dim3(rblocks), dim3(256), 0, s, x, int64_t(ldx), r,
-                       int64_t(n), n, nrhs, active_d);
-    HWY_HIP_CHECK(hipGetLastError());
+  if (n == 0 || nrhs == 0) {
+    for (int j = 0; j < nrhs; ++j) ferr[j] = berr[j] = 0.0;
+    return 0;
   }
-  // The last residual pass was of the final X: its weights bound the error.
-  std::vector<double> est(nrhs, 0.0);
-  if (ferr) lu::ForwardErrorEstimates(n, nrhs, af, ldaf, perm, wts, z, s, est.data());
-  // Normwise backward error of the final X.
-  hipLaunchKernelGGL(lu::RowAbsSumKernel, dim3(rblocks), dim3(256), 0, s, rowpart, nchunks, n,
-                     rowsum);
-  std::vector<double> rows(n), xs(size_t(n) * nrhs);
-  HWY_HIP_CHECK(hipMemcpyAsync(rows.data(), rowsum, size_t(n) * 8, hipMemcpyDeviceToHost, s));
-  HWY_HIP_CHECK(hipMemcpy2DAsync(xs.data(), size_t(n) * 8, x, size_t(ldx) * 8, size_t(n) * 8,
-                                 nrhs, hipMemcpyDeviceToHost, s));
-  HWY_HIP_CHECK(hipStreamSynchronize(s));
-  const double anorm = *std::max_element(rows.begin(), rows.end());
-  for (int j = 0; j < nrhs; ++j) {
-    double xn = 0.0;
-    for (int i = 0; i < n; ++i) xn = std::max(xn, std::fabs(xs[size_t(j) * n + i]));
-    if (nbe) nbe[j] = (anorm > 0 && xn > 0) ? rinf[j] / (anorm * xn) : 0.0;
-    if (ferr) ferr[j] = xn != 0.0 ? est[j] / xn : est[j];
-    if (iterations) iterations[j] = count[j] - 1;
+  hipStream_t s = StreamOr(stream);
+  const int group = std::min(nrhs, lu::kMaxRhs);
+  const lu::Perm perm(n, group, ipiv, s);
+  for (int j0 = 0; j0 < nrhs; j0 += lu::kMaxRhs) {
+    const int cols = std::min(lu::kMaxRhs, nrhs - j0);
+    lu::Refine(op == Op::kTrans, n, cols, a, lda, af, ldaf, perm, b + int64_t(j0) * ldb, ldb,
+               x + int64_t(j0) * ldx, ldx, 5, ferr + j0, berr + j0, nullptr, nullptr, s);
   }
-  for (void* p : {static_cast<void*>(rp), static_cast<void*>(wp), static_cast<void*>(r),
-                  static_cast<void*>(stats), static_cast<void*>(rowsum),
-                  static_cast<void*>(rowpart), static_cast<void*>(active_d),
-                  static_cast<void*>(wts), static_cast<void*>(z)})
-    if (p) HWY_HIP_CHECK(hipFree(p));
   return 0;
 }
 

@@ -178,6 +178,19 @@ typedef void (*hwyblas_error_handler)(const char* name, hwyblas_int position);
 HWYBLAS_AMD_API hwyblas_error_handler
 hwyblas_set_error_handler(hwyblas_error_handler handler);
 
+/* The DGEMM mode of the drop-in entries (no hwyBLAS counterpart): 0, the
+ * default, runs dgemm_, cblas_dgemm and hwyblas_dgemm_dev on the native FP64
+ * kernel; 1 runs products with m, n >= 2048 and k >= 1024 on the emulated
+ * DGEMM (INT8 matrix cores, count chosen on the device, hwyBLAS's
+ * componentwise bound 4 (k + 2) u (|alpha||A||B| + |beta||C|) guaranteed, the
+ * native kernel where it cannot be certified; see hwyblas_dgemm_emulated_dev),
+ * and dgetrf_/dgesv_'s trailing updates on it. The environment variable
+ * HWYBLAS_AMD_DGEMM=emulated selects 1 at load. The mode is process-wide.
+ * hwyblas_set_dgemm_mode returns the previous mode, or -1 (and changes
+ * nothing) for a mode other than 0 or 1. */
+HWYBLAS_AMD_API int hwyblas_set_dgemm_mode(int mode);
+HWYBLAS_AMD_API int hwyblas_get_dgemm_mode(void);
+
 /* ------------------------------------------------------------------------ */
 /* CBLAS (include/hwyblas/cblas.h:26-38 enums, :607-622, :858-871)          */
 
@@ -584,6 +597,32 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
                                          double* x, int ldx, int update, int itmax,
                                          double* ferr, double* berr, double* nbe,
                                          int* iterations, void* stream);
+/* Householder QR, A = Q R (hwyBLAS src/lapack/geqrf.cc:66-150; signature as
+ * include/hwyblas/lapack.h:1281-1283): R on and above the diagonal of A,
+ * the reflectors H(i) = I - tau_i v v^T (v(i) = 1 implicit, v(i+1:m) below
+ * the diagonal) and tau (min(m, n) values), with xLARFG's signs. Info
+ * positions: 1 m, 2 n, 4 lda, 7 lwork (lwork >= max(1, n), or -1 for the
+ * workspace query, which returns n * 32 in work[0]); reported as DGEQRF. A
+ * and tau may be host or device memory; work is only written (work[0]).
+ * Each call synchronizes. The factorization runs on the GPU in panels of 64
+ * columns with the trailing updates on the MFMA GEMM. */
+HWYBLAS_AMD_API void dgeqrf_(const hwyblas_int* m, const hwyblas_int* n, double* a,
+                             const hwyblas_int* lda, double* tau, double* work,
+                             const hwyblas_int* lwork, hwyblas_int* info);
+/* The same on device memory (A and tau), asynchronous on `stream` (no
+ * hwyBLAS counterpart as such). Returns 0 or -i (reported as DGEQRF). */
+HWYBLAS_AMD_API int hwyblas_dgeqrf_dev(int m, int n, double* a, int lda, double* tau,
+                                       void* stream);
+
+/* xGERFS on device operands (no hwyBLAS counterpart as such; dgerfs_ above is
+ * the LAPACK form): refines X (n x nrhs) for op(A) X = B (trans 'N' or 'T')
+ * with the factors af and the 1-based pivots ipiv of hwyblas_dgetrf_dev,
+ * all device memory; ferr and berr receive nrhs values in host memory.
+ * Returns 0 or -i (reported as DGERFS, xGERFS's positions). Synchronizes. */
+HWYBLAS_AMD_API int hwyblas_dgerfs_dev(char trans, int n, int nrhs, const double* a, int lda,
+                                       const double* af, int ldaf, const int* ipiv,
+                                       const double* b, int ldb, double* x, int ldx,
+                                       double* ferr, double* berr, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Tile configurations of the plain GEMM kernel (no hwyBLAS counterpart: its
@@ -810,8 +849,12 @@ HWYBLAS_AMD_API hwyblas_int LAPACKE_dposv_work(int matrix_layout, char uplo, hwy
  * updates, the solve hwyblas_dgetrs_dev. Info positions as LAPACK's: dgetrf
  * 1 m, 2 n, 4 lda; dgetrs 1 trans, 2 n, 3 nrhs, 5 lda, 8 ldb; dgesv 1 n,
  * 2 nrhs, 4 lda, 7 ldb. Host or device operands (matrices and pivots);
- * each call synchronizes. The LAPACKE forms follow the rules stated for
- * the Cholesky routines above (host operands). */
+ * each call synchronizes. info = -1000 (dgetrf_, dgesv_) is not an argument
+ * position: a wait inside the cooperative panel kernel exceeded its bound, a
+ * message names it on stderr, the installed error handler (if any) receives
+ * ("DGETRF" or "DGESV", -1000), and A and ipiv hold no valid factorization.
+ * The LAPACKE forms follow the rules stated for the Cholesky routines above
+ * (host operands). */
 HWYBLAS_AMD_API void dgetrf_(const hwyblas_int* m, const hwyblas_int* n, double* a,
                              const hwyblas_int* lda, hwyblas_int* ipiv, hwyblas_int* info);
 HWYBLAS_AMD_API void dgetrs_(const char* trans, const hwyblas_int* n, const hwyblas_int* nrhs,
@@ -820,6 +863,22 @@ HWYBLAS_AMD_API void dgetrs_(const char* trans, const hwyblas_int* n, const hwyb
 HWYBLAS_AMD_API void dgesv_(const hwyblas_int* n, const hwyblas_int* nrhs, double* a,
                             const hwyblas_int* lda, hwyblas_int* ipiv, double* b,
                             const hwyblas_int* ldb, hwyblas_int* info);
+/* xGERFS: iterative refinement of X for op(A) X = B with the LU factors of
+ * dgetrf_ (af, ipiv), with the componentwise backward error (berr) and the
+ * forward error bound (ferr) of every column (hwyBLAS src/lapack/gesvx.cc:
+ * 333-521; signature as include/hwyblas/lapack.h:465-471). Up to 5
+ * corrections per column while berr > eps and halves; residuals in FP64 on
+ * the GPU; ferr by xLACN2's estimator. Info positions as xGERFS's: 1 trans,
+ * 2 n, 3 nrhs, 5 lda, 7 ldaf, 10 ldb, 12 ldx (reported as DGERFS). A, af, B,
+ * X and ipiv may be host or device memory; ferr and berr are host arrays;
+ * work and iwork are not used (the work arrays live on the device). Each
+ * call synchronizes. */
+HWYBLAS_AMD_API void dgerfs_(const char* trans, const hwyblas_int* n, const hwyblas_int* nrhs,
+                             const double* a, const hwyblas_int* lda, const double* af,
+                             const hwyblas_int* ldaf, const hwyblas_int* ipiv, const double* b,
+                             const hwyblas_int* ldb, double* x, const hwyblas_int* ldx,
+                             double* ferr, double* berr, double* work, hwyblas_int* iwork,
+                             hwyblas_int* info);
 HWYBLAS_AMD_API hwyblas_int LAPACKE_dgetrf(int matrix_layout, hwyblas_int m, hwyblas_int n,
                                            double* a, hwyblas_int lda, hwyblas_int* ipiv);
 HWYBLAS_AMD_API hwyblas_int LAPACKE_dgetrf_work(int matrix_layout, hwyblas_int m, hwyblas_int n,

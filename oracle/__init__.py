@@ -75,6 +75,12 @@ def lib():
                                            _d, _p, _i, _i]
         L.orc_bf16_digits.restype = None
         L.orc_bf16_digits.argtypes = [_d, _i, _i, _p]
+        L.orc_dgeqrf.restype = _i
+        L.orc_dgeqrf.argtypes = [_i, _i, _p, _i, _p]
+        L.orc_dgeqr2.restype = None
+        L.orc_dgeqr2.argtypes = [_i, _i, _p, _i, _p]
+        L.orc_dlarft.restype = None
+        L.orc_dlarft.argtypes = [_i, _i, _p, _i, _p, _p, _i]
         L.orc_emulated_choose.restype = _i
         L.orc_emulated_choose.argtypes = [_i, _i, _c_char, _c_char, _i, _i, _i, _p, _i, _p, _i,
                                           _p, _p]
@@ -231,6 +237,32 @@ def dgemm_ozaki_bf16(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, slice
     """In-place C update by the CPU restatement of the BF16-slice emulated DGEMM."""
     lib().orc_dgemm_ozaki_bf16(_b(ta), _b(tb), m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb,
                                beta, _ptr(c), ldc, slices)
+
+
+def geqrf(a, blocked=True):
+    """Householder QR of a copy of `a`: (factored matrix, tau), as xGEQRF
+    (blocked=True) or xGEQR2 stores them."""
+    h = np.array(a, dtype=np.float64, order="F", copy=True)
+    m, n = h.shape
+    tau = np.zeros(max(1, min(m, n)))
+    if blocked:
+        info = lib().orc_dgeqrf(m, n, _ptr(h), max(1, m), _ptr(tau))
+        assert info == 0
+    else:
+        lib().orc_dgeqr2(m, n, _ptr(h), max(1, m), _ptr(tau))
+    return h, tau[:min(m, n)]
+
+
+def larft(v, tau):
+    """The upper triangular T of the block reflector I - V T V^T (forward,
+    columnwise; V unit lower trapezoidal, entries on and above its diagonal
+    not read)."""
+    v = np.asfortranarray(v, dtype=np.float64)
+    n, k = v.shape
+    t = np.zeros((k, k), order="F")
+    lib().orc_dlarft(n, k, _ptr(v), max(1, n), _ptr(np.ascontiguousarray(tau, dtype=np.float64)),
+                     _ptr(t), max(1, k))
+    return t
 
 
 def emulated_choose(kind, ta, tb, m, n, k, a, lda, b, ldb, forced=0):

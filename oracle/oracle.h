@@ -141,6 +141,14 @@ void orc_emulated_coarse(char transa, char transb, int m, int n, int k, const do
                          int lda, const double* b, int ldb, double* lower, double* rowsum_up,
                          double* colsum_up);
 
+/* ---- Householder QR (oracle/qr.cc) ------------------------------------------
+ * xGEQRF (NB 32, NX 128), xGEQR2 and xLARFT ('F', 'C') as hwyBLAS's LAPACK
+ * port computes them: R on and above the diagonal of A, the reflectors'
+ * vectors below it (unit first element implicit) and tau. */
+int orc_dgeqrf(int m, int n, double* a, int lda, double* tau);
+void orc_dgeqr2(int m, int n, double* a, int lda, double* tau);
+void orc_dlarft(int n, int k, const double* v, int ldv, const double* tau, double* t, int ldt);
+
 /* ---- LU with partial pivoting and its solve (oracle/lu.cc) ---------------
  * hwyBLAS's dgetrf (blocked NB = 64 over the recursive dgetrf2 with a leaf of
  * 16), dlaswp (1-based k1..k2, incx = +-1), dgetrs and gerfs's componentwise

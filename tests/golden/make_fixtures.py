@@ -1,7 +1,8 @@
-"""Writes tests/golden/blas_fixtures.npz: small GEMM, TRSM and LU cases
+"""Writes tests/golden/blas_fixtures.npz: small GEMM, TRSM, LU and QR cases
 whose expected outputs come from an independent library, numpy's BLAS
-(OpenBLAS), scipy.linalg.solve_triangular (LAPACK trtrs) and
-scipy.linalg.lu_factor / lu_solve (LAPACK getrf / getrs), in this
+(OpenBLAS), scipy.linalg.solve_triangular (LAPACK trtrs),
+scipy.linalg.lu_factor / lu_solve (LAPACK getrf / getrs) and
+scipy.linalg.qr(mode="raw") (LAPACK geqrf: reflectors and tau), in this
 container.
 
 hwyBLAS stores no golden vectors (its tests regenerate inputs from seeds and
@@ -95,9 +96,28 @@ def main():
             data[key + "_xt"] = scipy.linalg.lu_solve((lu, piv), b, trans=1)
         lcases.append(f"{key},{m},{n}")
     data["lu_cases"] = np.array(lcases)
+    # Householder QR (LAPACK dgeqrf through scipy): the factored matrix (R on
+    # and above the diagonal, reflectors below) and tau; square, tall, wide,
+    # large enough for the blocked path (k > 128), a zero column (a reflector
+    # with tau = 0) and a column already zero below its diagonal.
+    qrng = np.random.default_rng(20261203)
+    qcases = []
+    for idx, (m, n) in enumerate([(1, 1), (5, 3), (3, 5), (40, 40), (200, 150), (150, 200),
+                                  (300, 300), (260, 64), (50, 30), (64, 48)]):
+        a = qrng.standard_normal((m, n))
+        if idx == 8:
+            a[:, 7] = 0.0
+        if idx == 9:
+            a[1:, 0] = 0.0
+        (h, tau), _ = scipy.linalg.qr(a, mode="raw", check_finite=False)
+        key = f"qr{idx}"
+        data[key + "_a"], data[key + "_h"], data[key + "_tau"] = a, np.asfortranarray(h), tau
+        qcases.append(f"{key},{m},{n}")
+    data["qr_cases"] = np.array(qcases)
     data["versions"] = np.array([f"numpy {np.__version__}", f"scipy {scipy.__version__}"])
     np.savez_compressed(OUT, **data)
-    print(f"wrote {OUT}: {len(cases)} gemm, {len(tcases)} trsm, {len(lcases)} lu cases")
+    print(f"wrote {OUT}: {len(cases)} gemm, {len(tcases)} trsm, {len(lcases)} lu, "
+          f"{len(qcases)} qr cases")
 
 
 if __name__ == "__main__":

@@ -506,6 +506,32 @@ def test_fortran_lu_argument_checks():
         assert H.gesv(4, 2, a, 4, piv, b, 3) == -7
     assert cap.calls == [("DGETRF", 1), ("DGETRF", 4), ("DGETRS", 1), ("DGETRS", 8),
                          ("DGESV", 2), ("DGESV", 7)]
+    # dgerfs_: xGERFS's positions (src/lapack/gesvx.cc:359-376), in order.
+    x = np.ones((4, 2), order="F")
+    cases = [(("X", 4, 2, 4, 4, 4, 4), -1), (("N", -1, 2, 4, 4, 4, 4), -2),
+             (("N", 4, -1, 4, 4, 4, 4), -3), (("T", 4, 2, 3, 4, 4, 4), -5),
+             (("N", 4, 2, 4, 3, 4, 4), -7), (("N", 4, 2, 4, 4, 3, 4), -10),
+             (("N", 4, 2, 4, 4, 4, 3), -12)]
+    with H.ErrorCapture() as cap:
+        for (tr, n, nrhs, lda, ldaf, ldb, ldx), want in cases:
+            info, _, _ = H.gerfs(tr, n, nrhs, a, lda, a, ldaf, piv, b, ldb, x, ldx)
+            assert info == want
+        info, ferr, berr = H.gerfs("N", 0, 2, a, 4, a, 4, piv, b, 4, x, 4)
+        assert info == 0 and np.all(ferr == 0) and np.all(berr == 0)
+    assert cap.calls == [("DGERFS", -w) for _, w in cases]
+    # dgeqrf_: xGEQRF's positions (src/lapack/geqrf.cc:82-95) and its query.
+    tau = np.zeros(4)
+    with H.ErrorCapture() as cap:
+        assert H.geqrf(-1, 4, a, 4, tau)[0] == -1
+        assert H.geqrf(4, -1, a, 4, tau)[0] == -2
+        assert H.geqrf(4, 4, a, 3, tau)[0] == -4
+        assert H.geqrf(4, 4, a, 4, tau, lwork=3)[0] == -7
+        assert H.geqrf(4, 4, a, 4, tau, lwork=0)[0] == -7
+        assert H.geqrf(4, 4, a, 4, tau, lwork=-1) == (0, 4 * 32.0)
+        assert H.geqrf(0, 4, a, 1, tau, lwork=0)[0] == -7
+        assert H.geqrf(0, 4, a, 1, tau, lwork=1) == (0, 1.0)
+    assert cap.calls == [("DGEQRF", 1), ("DGEQRF", 2), ("DGEQRF", 4), ("DGEQRF", 7),
+                         ("DGEQRF", 7), ("DGEQRF", 7)]
     H.lib().LAPACKE_set_nancheck(1)
     assert H.lapacke_lu("getrf", 3, 4, 4, a, 4, piv) == -1
     an = a.copy(order="F")
@@ -573,3 +599,17 @@ def test_error_positions_under_concurrent_callers():
     got = collections.Counter(cap.calls)
     assert got == collections.Counter({want: rounds for _, want in cases})
     assert np.all(c == 0)
+
+
+def test_dgemm_mode_switch():
+    """hwyblas_set_dgemm_mode: 0 and 1 accepted (returning the previous mode),
+    anything else refused without a change."""
+    L = H.lib()
+    start = L.hwyblas_get_dgemm_mode()
+    assert start in (0, 1)
+    assert L.hwyblas_set_dgemm_mode(1) == start
+    assert L.hwyblas_get_dgemm_mode() == 1
+    assert L.hwyblas_set_dgemm_mode(7) == -1
+    assert L.hwyblas_get_dgemm_mode() == 1
+    assert L.hwyblas_set_dgemm_mode(0) == 1
+    assert L.hwyblas_set_dgemm_mode(start) == 0

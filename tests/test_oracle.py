@@ -45,12 +45,15 @@ def _f(x):
     return np.asfortranarray(x)
 
 
-def oracle_matches_gemm_fixtures():
+def oracle_matches_gemm_fixtures(square_only=False):
     """Largest bound ratio of the oracle's GEMM and of the double-double
     reference against numpy's results (both must satisfy the bound, so they
-    are within twice the tolerance of each other)."""
+    are within twice the tolerance of each other). square_only: the m = n = k
+    cases alone, where a wrongly transposed operand is still read in bounds."""
     worst = 0.0
     for key, dt, ta, tb, m, n, k, alpha, beta in gemm_fixture_cases():
+        if square_only and not (m == n == k):
+            continue
         a, b, c = _f(FIX[key + "_a"]), _f(FIX[key + "_b"]), _f(FIX[key + "_c"])
         out = FIX[key + "_out"]
         got = c.copy(order="F")
@@ -204,7 +207,8 @@ def test_generator_is_layout_independent():
 @pytest.mark.parametrize("mistake", [1, 2, 3])
 def test_planted_gemm_mistakes_are_caught(mistake):
     oracle.plant_mistake(mistake)
-    assert oracle_matches_gemm_fixtures() > 1.0
+    # A wrong transposition (2) would read past non-square operands.
+    assert oracle_matches_gemm_fixtures(square_only=mistake == 2) > 1.0
 
 
 def test_planted_trsm_mistake_is_caught():
@@ -972,3 +976,73 @@ def test_emulated_choice_special_cases():
     assert oracle.emulated_choose("int8", "N", "N", m, n, k, a3, m, b3, k)[0] == 0
     # k too small for the guarantee to leave room: native.
     assert oracle.emulated_choose("int8", "N", "N", m, n, 3, a, m, b, 3)[0] == 0
+
+
+# ---------------------------------------------------------------------------
+# Householder QR (oracle/qr.cc)
+
+def qr_fixture_cases():
+    for line in FIX["qr_cases"]:
+        key, m, n = str(line).split(",")
+        yield key, int(m), int(n)
+
+
+def explicit_q(h, tau):
+    """Q = H(0) H(1) ... H(k-1) formed densely from the stored reflectors."""
+    m, n = h.shape
+    q = np.eye(m)
+    for i in reversed(range(min(m, n))):
+        v = np.zeros(m)
+        v[i] = 1.0
+        v[i + 1:] = h[i + 1:, i]
+        q -= tau[i] * np.outer(v, v @ q)
+    return q
+
+
+def test_qr_matches_lapack_fixtures():
+    """Against LAPACK dgeqrf (scipy): the same R, reflectors and tau up to
+    rounding (same signs: beta = -sign(alpha) |(alpha, x)|), blocked
+    (k > 128) and unblocked, with a zero column and a column already zero
+    below the diagonal (tau = 0 there)."""
+    eps = np.finfo(float).eps
+    for key, m, n in qr_fixture_cases():
+        a, hs, ts = FIX[key + "_a"], FIX[key + "_h"], FIX[key + "_tau"]
+        h, tau = oracle.geqrf(a)
+        scale = max(1.0, float(np.max(np.abs(hs))))
+        assert np.max(np.abs(h - hs)) <= 64 * max(m, n) * eps * scale, key
+        assert np.max(np.abs(tau - ts), initial=0.0) <= 64 * max(m, n) * eps, key
+    h, tau = oracle.geqrf(FIX["qr9_a"])
+    assert tau[0] == 0.0
+
+
+def test_qr_orthogonality_and_backward_error():
+    eps = np.finfo(float).eps
+    rng = np.random.default_rng(3)
+    for (m, n) in ((300, 300), (400, 170), (170, 400)):
+        a = rng.standard_normal((m, n))
+        h, tau = oracle.geqrf(a)
+        q = explicit_q(h, tau)
+        r = np.triu(h[:min(m, n), :])
+        assert np.max(np.abs(q.T @ q - np.eye(m))) <= 8 * m * eps
+        resid = q[:, :min(m, n)] @ r - a
+        assert np.linalg.norm(resid, 1) <= 8 * m * eps * np.linalg.norm(a, 1)
+        # Blocked and unblocked agree to rounding.
+        h2, tau2 = oracle.geqrf(a, blocked=False)
+        assert np.max(np.abs(h - h2)) <= 64 * m * eps * np.max(np.abs(h))
+
+
+def test_larft_forms_the_block_reflector():
+    """I - V T V^T equals H(0) ... H(k-1) formed one reflector at a time."""
+    rng = np.random.default_rng(4)
+    for (n, k) in ((50, 1), (50, 7), (80, 32), (33, 33)):
+        h, tau = oracle.geqrf(rng.standard_normal((n, k)), blocked=False)
+        t = oracle.larft(h, tau)
+        v = np.tril(h, -1)[:, :k] + np.eye(n, k)
+        assert np.allclose(np.triu(t), t)
+        assert np.max(np.abs((np.eye(n) - v @ t @ v.T) - explicit_q(h, tau))) <= 1e-13
+
+
+def test_planted_qr_sign_mistake_is_caught():
+    oracle.plant_mistake(15)
+    with pytest.raises(AssertionError):
+        test_qr_matches_lapack_fixtures()

@@ -1,0 +1,136 @@
+"""Householder QR on the GPU (dgeqrf_, hwyblas_dgeqrf_dev) through the C-ABI.
+
+Pinned against (1) LAPACK dgeqrf's own output (scipy, the committed
+fixtures) and (2) the oracle's restatement of hwyBLAS's xGEQRF: the same
+reflectors, signs and tau up to rounding (the GPU blocks by 64 columns and
+sums in another order, so the bits differ); (3) the defining properties:
+Q^T Q = I and Q R = A to rounding, R upper triangular. Also: the device
+entry equals the Fortran entry on host memory bit for bit, tau = 0 for a
+column already zero below the diagonal, and operands near underflow take
+xLARFG's rescaling path.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import hwyblas_amd as H
+import oracle
+
+pytestmark = pytest.mark.gpu
+FIX = np.load(os.path.join(os.path.dirname(__file__), "golden", "blas_fixtures.npz"))
+EPS = np.finfo(float).eps
+
+
+def gpu_qr(a):
+    m, n = a.shape
+    h = np.asfortranarray(a.copy())
+    tau = np.zeros(max(1, min(m, n)))
+    info, _ = H.geqrf(m, n, h, m, tau)
+    assert info == 0
+    return h, tau[:min(m, n)]
+
+
+def explicit_q(h, tau):
+    m, n = h.shape
+    q = np.eye(m)
+    for i in reversed(range(min(m, n))):
+        v = np.zeros(m)
+        v[i] = 1.0
+        v[i + 1:] = h[i + 1:, i]
+        q -= tau[i] * np.outer(v, v @ q)
+    return q
+
+
+def close(h, tau, h_ref, tau_ref, m, n, c=64):
+    """Entries within c max(m, n) eps of the reference: R relative to the
+    largest column norm, the reflectors and tau absolutely (|v| <= 1-ish,
+    tau in [1, 2])."""
+    k = min(m, n)
+    scale = max(1.0, float(np.max(np.abs(np.triu(h_ref)))))
+    tol = c * max(m, n) * EPS
+    r_err = np.max(np.abs(np.triu(h) - np.triu(h_ref))) / scale
+    v_err = np.max(np.abs(np.tril(h, -1)[:, :k] - np.tril(h_ref, -1)[:, :k]), initial=0.0)
+    t_err = np.max(np.abs(tau - tau_ref), initial=0.0)
+    return r_err <= tol and v_err <= tol * 16 and t_err <= tol, (r_err, v_err, t_err)
+
+
+def test_qr_matches_lapack_fixtures():
+    for line in FIX["qr_cases"]:
+        key, m, n = str(line).split(",")
+        m, n = int(m), int(n)
+        a = FIX[key + "_a"]
+        h, tau = gpu_qr(a)
+        ok, errs = close(h, tau, FIX[key + "_h"], FIX[key + "_tau"], m, n)
+        assert ok, (key, errs)
+    h, tau = gpu_qr(FIX["qr9_a"])  # column 0 already zero below the diagonal
+    assert tau[0] == 0.0
+
+
+@pytest.mark.parametrize("m,n", [(513, 300), (300, 513), (1000, 1000), (2000, 130), (64, 64),
+                                 (65, 63), (1, 40), (40, 1)])
+def test_qr_against_the_oracle_and_its_properties(m, n):
+    a = oracle.fill_uniform(m, n, m + 7 * n)
+    h, tau = gpu_qr(a)
+    ho, to = oracle.geqrf(a)
+    ok, errs = close(h, tau, ho, to, m, n)
+    assert ok, errs
+    k = min(m, n)
+    if m <= 2000:
+        q = explicit_q(h, tau)
+        r = np.triu(h[:k, :])
+        assert np.max(np.abs(q.T @ q - np.eye(m))) <= 16 * m * EPS
+        assert np.linalg.norm(q[:, :k] @ r - a, 1) <= 16 * m * EPS * np.linalg.norm(a, 1)
+
+
+def test_device_entry_equals_host_entry_and_is_asynchronous():
+    m, n = 700, 400
+    a = oracle.fill_uniform(m, n, 5)
+    h, tau = gpu_qr(a)
+    da = H.DeviceArray.from_host(a)
+    dt = H.DeviceArray(min(m, n), 1)
+    stream = H.lib().hwyblas_amd_stream_create()
+    try:
+        assert H.geqrf_dev(da, dt, stream) == 0
+        H.lib().hwyblas_amd_stream_synchronize(stream)
+    finally:
+        H.lib().hwyblas_amd_stream_destroy(stream)
+    assert np.array_equal(da.to_host(), h)
+    assert np.array_equal(dt.to_host().ravel(), tau)
+    # Device operands through the Fortran entry.
+    db = H.DeviceArray.from_host(a)
+    dt2 = H.DeviceArray(min(m, n), 1)
+    info, _ = H.geqrf(m, n, db, m, dt2)
+    assert info == 0 and np.array_equal(db.to_host(), h)
+
+
+def test_near_underflow_takes_the_rescaling_path():
+    """A scaled by 2^-1000 (columns' norms below safmin = tiny / eps): xLARFG
+    rescales by 1/safmin; the factors equal the oracle's to rounding and
+    are those of the unscaled matrix, scaled."""
+    m, n = 150, 90
+    a0 = oracle.fill_uniform(m, n, 9)
+    a = a0 * 2.0 ** -1000
+    h, tau = gpu_qr(a)
+    ho, to = oracle.geqrf(a)
+    h1, t1 = gpu_qr(a0)
+    assert np.max(np.abs(tau - to)) <= 64 * m * EPS
+    assert np.max(np.abs(tau - t1)) <= 64 * m * EPS
+    assert np.max(np.abs(np.tril(h, -1) - np.tril(h1, -1))) <= 1e-12
+    r_scaled = np.triu(h)[:n] * 2.0 ** 1000
+    assert np.max(np.abs(r_scaled - np.triu(h1)[:n])) <= 1e-12 * np.max(np.abs(np.triu(h1)))
+
+
+def test_padding_rows_untouched():
+    m, n, lda = 130, 70, 137
+    a = np.zeros((lda, n), order="F")
+    a[:m] = oracle.fill_uniform(m, n, 3)
+    a[m:] = 7.25
+    h = a.copy(order="F")
+    tau = np.zeros(n)
+    info, _ = H.geqrf(m, n, h, lda, tau)
+    assert info == 0
+    assert np.all(h[m:] == 7.25)
+    ho, to = oracle.geqrf(a[:m])
+    ok, errs = close(h[:m], tau, ho, to, m, n)
+    assert ok, errs
