@@ -251,3 +251,38 @@ def test_invalid_arguments_reported(H):
     with H.ErrorCapture() as cap:
         info, _ = H.dgemm_emulated_dev("N", "N", 4, 4, 4, 1.0, None, 3, None, 4, 0.0, None, 4)
     assert info == 8 and cap.calls == [("DGEMM", 8)]
+
+
+def test_dgemm_mode_routes_large_products_to_the_emulation(H):
+    """hwyblas_set_dgemm_mode(1): dgemm_ on host operands and
+    hwyblas_dgemm_dev give the emulated DGEMM's bits (count chosen on the
+    device) for m, n >= 2048, k >= 1024, and the native kernel's below."""
+    L = H.lib()
+    prev = L.hwyblas_set_dgemm_mode(1)
+    try:
+        for (m, n, k, emulated) in ((2048, 2304, 1024, True), (2048, 2048, 512, False)):
+            a = oracle.fill_uniform(m, k, 81)
+            b = oracle.fill_uniform(k, n, 82)
+            c = oracle.fill_uniform(m, n, 83)
+            host = c.copy(order="F")
+            H.gemm("N", "N", m, n, k, 1.0, a, m, b, k, 0.5, host, m)
+            da, db, dc = (H.DeviceArray.from_host(x) for x in (a, b, c))
+            assert H.gemm_dev("N", "N", m, n, k, 1.0, da, m, db, k, 0.5, dc, m,
+                              dtype=np.float64) == 0
+            H.synchronize()
+            dev = dc.to_host()
+            ref = H.DeviceArray.from_host(c)
+            if emulated:
+                info, used = H.dgemm_emulated_dev("N", "N", m, n, k, 1.0, da, m, db, k, 0.5,
+                                                  ref, m)
+                assert info == 0 and used == 14
+            else:
+                L.hwyblas_set_dgemm_mode(0)
+                assert H.gemm_dev("N", "N", m, n, k, 1.0, da, m, db, k, 0.5, ref, m,
+                                  dtype=np.float64) == 0
+                L.hwyblas_set_dgemm_mode(1)
+            H.synchronize()
+            want = ref.to_host()
+            assert np.array_equal(host, want) and np.array_equal(dev, want), (m, n, k)
+    finally:
+        L.hwyblas_set_dgemm_mode(prev)
