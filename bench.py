@@ -841,31 +841,175 @@ def level3_leg(H, n=8192, reps=3):
     return out
 
 
-def cpu_baseline(oracle, seconds=12.0):
-    """The oracle's single-threaded restatement of hwyBLAS's blocked DGEMM,
-    DGEMM NN 2048^3 on one pinned core, best of the calls that fit in about
-    `seconds` (at least 2), as bench/blas_bench.cc:109-133 times it."""
-    n = 2048
-    a = oracle.fill_uniform(n, n, 1)
-    b = oracle.fill_uniform(n, n, 2)
-    c = np.zeros((n, n), order="F")
+def cpu_model():
     try:
-        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
-    except Exception:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
         pass
-    oracle.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n)  # warm-up
+    return "unknown"
+
+
+def worker_cores():
+    """The host cores this process may use, at most 16 (one GPU's share)."""
+    return sorted(os.sched_getaffinity(0))[:16]
+
+
+def pin(core):
+    try:
+        os.sched_setaffinity(0, {core})
+    except OSError:
+        pass
+
+
+def best_time(fn, seconds, min_calls=2):
+    fn()  # warm-up
     best, calls = float("inf"), 0
     t_end = time.perf_counter() + seconds
-    while calls < 2 or time.perf_counter() < t_end:
+    while calls < min_calls or time.perf_counter() < t_end:
         t = time.perf_counter()
-        oracle.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n)
+        fn()
         best = min(best, time.perf_counter() - t)
         calls += 1
+    return best, calls
+
+
+def cpu_gemm_leg(oracle, seconds):
+    """The oracle's single-threaded restatement of hwyBLAS's blocked DGEMM,
+    NN 2048^3 on one pinned core, best of the calls that fit in about
+    `seconds` (bench/blas_bench.cc:109-133's method)."""
+    n = 2048
+    a, b = oracle.fill_uniform(n, n, 1), oracle.fill_uniform(n, n, 2)
+    c = np.zeros((n, n), order="F")
+    best, calls = best_time(lambda: oracle.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n),
+                            seconds)
     return {"value": round(2.0 * n ** 3 / best / 1e12, 5), "unit": "TFLOP/s", "cores": 1,
             "kind": "port",
             "sample": f"oracle restatement of hwyBLAS's blocked DGEMM (AVX2 8x6 FMA tile, "
                       f"KC=256, beta on the first K block), NN {n}^3, alpha=1 beta=0, best of "
                       f"{calls} calls, 1 pinned core"}
+
+
+def cpu_trsm_leg(oracle, seconds):
+    """The restatement of hwyBLAS's recursive DTRSM (halving at multiples of
+    32, dot-form base with the reciprocal), L/L/N/N 4096 x 4096, the
+    diagonal n + U[0, 1], one pinned core."""
+    n = 4096
+    t = oracle.fill_uniform(n, n, 3)
+    t[np.arange(n), np.arange(n)] = n + (oracle.fill_uniform(n, 1, 4)[:, 0] + 1) / 2
+    b = oracle.fill_uniform(n, n, 5)
+    x = b.copy(order="F")
+
+    def step():
+        x[:] = b
+        oracle.trsm("L", "L", "N", "N", n, n, 1.0, t, n, x, n)
+    best, calls = best_time(step, seconds)
+    return {"value": round(float(n) ** 3 / best / 1e12, 5), "unit": "TFLOP/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle restatement of hwyBLAS's recursive DTRSM L/L/N/N {n} x {n} "
+                      f"(m^2 n flops), best of {calls} calls (the copy of B included), 1 pinned "
+                      f"core"}
+
+
+def _batched_part(job):
+    """One process's share of a batch: generate its problems (untimed), then
+    time them with the oracle's restatement on the pinned core."""
+    kind, dtype, core, shapes, seed = job
+    import oracle
+    pin(core)
+    ops = []
+    for t, sh in enumerate(shapes):
+        if kind == "gemm":
+            m, n, k, ta, tb = sh
+            ra, ca = (k, m) if ta else (m, k)
+            rb, cb = (n, k) if tb else (k, n)
+            ops.append((m, n, k, ta, tb, oracle.fill_uniform(ra, ca, seed + 3 * t, dtype=dtype),
+                        oracle.fill_uniform(rb, cb, seed + 3 * t + 1, dtype=dtype),
+                        np.zeros((m, n), dtype=dtype, order="F")))
+        else:
+            m, n = sh
+            a = oracle.fill_uniform(m, m, seed + 2 * t)
+            a[np.arange(m), np.arange(m)] = m + (a[np.arange(m), np.arange(m)] + 1) / 2
+            ops.append((m, n, a, oracle.fill_uniform(m, n, seed + 2 * t + 1)))
+    def once():
+        t0 = time.perf_counter()
+        for o in ops:
+            if kind == "gemm":
+                m, n, k, ta, tb, a, b, c = o
+                oracle.gemm("T" if ta else "N", "T" if tb else "N", m, n, k, 1.0, a, a.shape[0],
+                            b, b.shape[0], 0.0, c, m)
+            else:
+                m, n, a, b = o
+                oracle.trsm("L", "L", "N", "N", m, n, 1.0, a, m, b, m)
+        return time.perf_counter() - t0
+    # Best of the repetitions that fit in about 2 s (at least 3).
+    times = [once()]
+    while len(times) < 3 or sum(times) < 2.0:
+        times.append(once())
+    return min(times), len(times)
+
+
+def cpu_batched_leg(kind, dtype, cores, count=10000):
+    """The batched workload on the host: the same seeded shapes as the GPU
+    leg, split round-robin over one process per core (each pinned), all
+    running at once; the batch time is the slowest process's."""
+    import multiprocessing
+    if kind == "gemm":
+        rng = np.random.default_rng(7)
+        m, n, k = (log_uniform_ints(rng, 8, 512, count) for _ in range(3))
+        ta = rng.integers(0, 2, count)
+        tb = rng.integers(0, 2, count)
+        shapes = [(int(m[t]), int(n[t]), int(k[t]), int(ta[t]), int(tb[t])) for t in range(count)]
+        flops = float(np.sum(2.0 * m * n * k))
+    else:
+        rng = np.random.default_rng(11)
+        m = log_uniform_ints(rng, 8, 256, count)
+        n = log_uniform_ints(rng, 1, 64, count)
+        shapes = [(int(m[t]), int(n[t])) for t in range(count)]
+        flops = float(np.sum(m.astype(float) ** 2 * n))
+    jobs = [(kind, dtype, c, shapes[i::len(cores)], 5000 + 100000 * i)
+            for i, c in enumerate(cores)]
+    with multiprocessing.get_context("fork").Pool(len(cores)) as pool:
+        parts = pool.map(_batched_part, jobs)
+    secs = max(t for t, _ in parts)
+    reps = min(r for _, r in parts)
+    unit = "GEMM/s" if kind == "gemm" else "solves/s"
+    name = {"gemm": f"{'D' if np.dtype(dtype) == np.float64 else 'S'}GEMM",
+            "trsm": "DTRSM L/L/N/N"}[kind]
+    return {"value": round(count / secs, 1), "unit": unit, "cores": len(cores), "kind": "port",
+            "tflops": round(flops / secs / 1e12, 5),
+            "sample": f"the {count} seeded uneven {name} problems of the GPU leg, split over "
+                      f"{len(cores)} processes, one pinned core each, by the oracle's "
+                      f"restatement, all at once; each process's best of {reps}+ repetitions, "
+                      f"batch time = the slowest process's ({secs:.4f} s)"}
+
+
+def cpu_baseline_worker():
+    """Runs in a process started before the benchmark touches the GPU: waits
+    for "go" on stdin (sent once the GPU legs are done, so that nothing else
+    runs on the host meanwhile), then times every CPU leg and prints one
+    JSON line."""
+    import oracle
+    oracle.build()
+    if sys.stdin.readline().strip() != "go":
+        return
+    cores = worker_cores()
+    out = {}
+    pin(cores[0])
+    out["dgemm"] = cpu_gemm_leg(oracle, 8.0)
+    out["dtrsm"] = cpu_trsm_leg(oracle, 1.0)
+    try:
+        os.sched_setaffinity(0, set(cores))
+    except OSError:
+        pass
+    out["batched_gemm_f64"] = cpu_batched_leg("gemm", np.float64, cores)
+    out["batched_gemm_f32"] = cpu_batched_leg("gemm", np.float32, cores)
+    out["batched_trsm_f64"] = cpu_batched_leg("trsm", np.float64, cores)
+    out["host"] = {"cpu_model": cpu_model(), "cores_used": len(cores),
+                   "cores_visible": os.cpu_count()}
+    print(json.dumps(out), flush=True)
 
 
 def run_batched(H, oracle, args, stream):
@@ -947,11 +1091,24 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (a kernel of known traffic, "
                          "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
+    ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_baseline_worker()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): the hot path is "
               "single-GPU by design; run without --gpus or with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    # The CPU baseline runs in its own process, started before anything here
+    # touches the GPU; it waits until the GPU legs are done.
+    cpu_proc = None
+    if not args.no_cpu and not (args.lu or args.plain_sweep or args.level3 or args.potrf or
+                                args.zgemm or args.batched_only):
+        import subprocess
+        cpu_proc = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker"],
+                                    stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
 
     import hwyblas_amd as H
     import oracle
@@ -1061,9 +1218,19 @@ def main():
 
     batched = None if args.no_batched else run_batched(H, oracle, args, stream)
 
-    cpu = None if args.no_cpu else cpu_baseline(oracle)
-    if cpu:
-        log(f"cpu baseline {cpu['value'] * 1e3:.2f} GFLOP/s")
+    cpu = None
+    if cpu_proc is not None:
+        cpu_proc.stdin.write("go\n")
+        cpu_proc.stdin.flush()
+        legs = json.loads(cpu_proc.stdout.readline())
+        cpu_proc.wait()
+        cpu = dict(legs["dgemm"])
+        cpu["host"] = legs["host"]
+        cpu["legs"] = {k: v for k, v in legs.items() if k not in ("dgemm", "host")}
+        log(f"cpu baseline {cpu['value'] * 1e3:.2f} GFLOP/s (DGEMM 2048, 1 core); DTRSM "
+            f"{legs['dtrsm']['value'] * 1e3:.2f} GFLOP/s; batched DGEMM "
+            f"{legs['batched_gemm_f64']['value']:.0f} GEMM/s on {legs['host']['cores_used']} "
+            f"cores ({legs['host']['cpu_model']})")
 
     out = {
         "metric": METRIC,
