@@ -1059,6 +1059,8 @@ def main():
     ap.add_argument("--plain-sweep", action="store_true",
                     help="S/DGEMM at 4096-16384 (every transposition, both layouts) and DTRSM "
                          "8192 (every side/uplo/trans) instead of the headline")
+    ap.add_argument("--sweep-sizes", default="4096,8192,16384",
+                    help="comma-separated m = n = k for --plain-sweep's S/DGEMM")
     ap.add_argument("--batch-count", type=int, default=10000)
     ap.add_argument("--lu", action="store_true",
                     help="the mixed-precision LU solve (n = --lu-n, kappa 1e2, 1e6, 1e10) "
@@ -1132,7 +1134,8 @@ def main():
         print(json.dumps({"lu": lu_leg(H, args.lu_n, kappas, stream)}), flush=True)
         return
     if args.plain_sweep:
-        print(json.dumps({"plain_sweep": plain_sweep(H, oracle)}), flush=True)
+        sizes = tuple(int(x) for x in args.sweep_sizes.split(",") if x)
+        print(json.dumps({"plain_sweep": plain_sweep(H, oracle, sizes)}), flush=True)
         return
     if args.level3:
         print(json.dumps({"level3": level3_leg(H)}), flush=True)

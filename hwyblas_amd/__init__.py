@@ -145,6 +145,7 @@ def _declare(L):
                                     _p, _p, _p])
     fn("hwyblas_gemm_config_count", _i, [])
     fn("hwyblas_gemm_config_info", _i, [_i, _pi, _pi, _pi, _pi])
+    fn("hwyblas_gemm_config_stages", _i, [_i])
     fn("hwyblas_gemm_default_config", _i, [_i, _c, _c, _i, _i, _i])
     fn("xerbla_", None, [ctypes.c_char_p, _pi, _sz])
     fn("hwyblas_set_error_handler", _p, [_p])

@@ -34,9 +34,7 @@ int64_t Tiles(int64_t m, int64_t n, int bm, int bn) {
 // to 64 x 64 and then 32 x 32 single-wave tiles; those are latency-bound,
 // and a 32-deep k slice halves their k steps (FP64 512^3: 37.0 -> 32.6 us,
 // 64 x 4096 x 4096: 249 -> 215 us).
-int ChooseGemmConfig(int dtype_bytes, bool ta, bool tb, int64_t m, int64_t n, int64_t k) {
-  (void)ta;
-  (void)tb;
+int ChooseRegularGemmConfig(int dtype_bytes, int64_t m, int64_t n, int64_t k) {
   if (dtype_bytes == 8) {
     if (Tiles(m, n, 128, 128) >= 512) return 8;
   } else {
@@ -45,6 +43,19 @@ int ChooseGemmConfig(int dtype_bytes, bool ta, bool tb, int64_t m, int64_t n, in
   }
   if (Tiles(m, n, 64, 64) >= 256) return 0;
   return k > 16 ? 11 : 7;
+}
+
+// Large products take the DMA-fed kernel (MI355X, m = n = k = 16384, NN):
+// FP64 on 128 x 128 x 16 tiles through a two-slice LDS ring, 86 % of the
+// FP64 MFMA peak against 80 % for the best plain configuration; FP32 on
+// 256 x 256 x 16 tiles of 16 waves, 86 % against 75 %. Calls it cannot take
+// (and its edge strips) go to the plain rule above.
+int ChooseGemmConfig(int dtype_bytes, bool ta, bool tb, int64_t m, int64_t n, int64_t k) {
+  (void)ta;
+  (void)tb;
+  if (dtype_bytes == 8 && Tiles(m, n, 128, 128) >= 512) return 15;
+  if (dtype_bytes == 4 && Tiles(m, n, 256, 256) >= 256) return 16;
+  return ChooseRegularGemmConfig(dtype_bytes, m, n, k);
 }
 
 template <>
@@ -77,7 +88,8 @@ namespace {
 // A triangle of C runs on square tiles, so that only the tiles meeting it
 // are launched (the 128 x 128 x 16 two-waves-per-SIMD tile otherwise).
 int PartConfig(int cfg, int part) {
-  if (part == 0 || kGemmConfigs[cfg].bm == kGemmConfigs[cfg].bn) return cfg;
+  if (part == 0 || (kGemmConfigs[cfg].bm == kGemmConfigs[cfg].bn && !kGemmConfigs[cfg].stages))
+    return cfg;
   return 8;
 }
 
@@ -163,6 +175,11 @@ HWYBLAS_AMD_API int hwyblas_gemm_config_info(int cfg, int* bm, int* bn, int* bk,
   *bk = g.bk;
   *waves = g.waves_m * g.waves_n;
   return 0;
+}
+
+HWYBLAS_AMD_API int hwyblas_gemm_config_stages(int cfg) {
+  if (cfg < 0 || cfg >= kNumGemmConfigs) return -1;
+  return kGemmConfigs[cfg].stages;
 }
 
 HWYBLAS_AMD_API int hwyblas_gemm_default_config(int dtype_bytes, char transa, char transb,

@@ -26,6 +26,7 @@
 #include <type_traits>
 
 #include "common.h"
+#include "dma.h"
 #include "gemm_configs.h"
 
 namespace hwy_amd {
@@ -112,17 +113,59 @@ __device__ __forceinline__ void TileCoords(int tiles_m, int tiles_n, int* tm,
 
 // LDS bytes one macro-tile needs (two buffers of each operand's slice).
 // An operand whose memory is contiguous along k is kept row-major with k
-// fastest ([row][BK + 1], the odd stride spreads the MFMA reads over the
-// banks); one contiguous along rows/columns is kept k-major ([k][BM + PAD]).
+// fastest ([row][K_LD]); one contiguous along rows/columns is kept k-major
+// ([k][MN + PAD]). An MFMA operand read takes 16 consecutive rows/columns
+// (lane % 16) at 4 consecutive k (lane / 16), so the strides are chosen for
+// those 64 addresses to fall in distinct banks (64 banks of 4 bytes):
+//  - FP32, k-contiguous: K_LD = BK + 4 with K_LD / 4 odd, so row r sits
+//    4 (r K_LD / 4 mod 16) banks on and the four k of a row fill the gap;
+//  - FP32, k-major: MN + 16, so each k row starts 16 banks after the last;
+//  - FP64 (8-byte values, a read served a half-wave at a time): BK + 1 and
+//    MN + 4, measured free of conflicts.
 template <typename T, bool TA, bool TB, int BM, int BN, int BK>
 struct TileLds {
-  static constexpr int kPadMN = 4;
-  static constexpr int A_LD = TA ? BK + 1 : BM + kPadMN;
+  static constexpr bool kF32 = sizeof(T) == 4;
+  static constexpr int kKLd = kF32 ? BK + 4 : BK + 1;
+  static constexpr int kPadMN = kF32 ? 16 : 4;
+  static_assert(!kF32 || (kKLd / 4) % 2 == 1, "k-contiguous stride");
+  static constexpr int A_LD = TA ? kKLd : BM + kPadMN;
   static constexpr int A_SZ = TA ? BM * A_LD : BK * A_LD;
-  static constexpr int B_LD = TB ? BN + kPadMN : BK + 1;
+  static constexpr int B_LD = TB ? BN + kPadMN : kKLd;
   static constexpr int B_SZ = TB ? BK * B_LD : BN * B_LD;
   static constexpr int kElems = 2 * (A_SZ + B_SZ);
 };
+
+// The epilogue (hwyBLAS Store semantics) of a wave's TM x TN block of 16 x 16
+// MFMA tiles whose first row and column of C are (r0, c0): r = alpha * acc;
+// beta == 1 adds C; other non-zero beta is fused as fma(C, beta, r); beta == 0
+// never reads C. Entries outside m x n or the requested triangle are skipped.
+template <typename T, int TM, int TN>
+__device__ __forceinline__ void StoreTile(const GemmParams<T>& p,
+                                          const typename Mfma<T>::Acc (&acc)[TM][TN], int64_t r0,
+                                          int64_t c0, int lane) {
+  using M = Mfma<T>;
+  const int li = lane & 15, lk = lane >> 4;
+  const T alpha = p.alpha, beta = p.beta;
+  const bool beta_zero = beta == T(0), beta_one = beta == T(1);
+#pragma unroll
+  for (int ti = 0; ti < TM; ++ti) {
+#pragma unroll
+    for (int tj = 0; tj < TN; ++tj) {
+      const int64_t col = c0 + tj * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = r0 + ti * 16 + M::Row(lk, r);
+        const bool in_part = p.part == 0 || (p.part == 1 ? row >= col : row <= col);
+        if (row < p.m && col < p.n && in_part) {
+          T* cij = p.c + row + col * p.ldc;
+          T v = acc[ti][tj][r] * alpha;
+          if (!beta_zero) v = beta_one ? v + *cij : __builtin_fma(*cij, beta, v);
+          *cij = v;
+        }
+      }
+    }
+  }
+}
 
 // One workgroup (WAVES_M x WAVES_N waves) computes the BM x BN macro-tile of
 // C at (m0, n0) over the whole k range; each wave computes
@@ -275,29 +318,8 @@ __device__ __forceinline__ void MacroTile(const GemmParams<T>& p, int64_t m0, in
     __syncthreads();
   }
 
-  // Epilogue (hwyBLAS Store semantics).
-  const T alpha = p.alpha, beta = p.beta;
-  const bool beta_zero = beta == T(0), beta_one = beta == T(1);
-#pragma unroll
-  for (int ti = 0; ti < TM; ++ti) {
-#pragma unroll
-    for (int tj = 0; tj < TN; ++tj) {
-      const int64_t col = n0 + wn0 + tj * 16 + li;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm0 + ti * 16 + M::Row(lk, r);
-        const bool in_part = p.part == 0 || (p.part == 1 ? row >= col : row <= col);
-        if (row < p.m && col < p.n && in_part) {
-          T* cij = p.c + row + col * p.ldc;
-          T v = acc[ti][tj][r] * alpha;
-          if (!beta_zero) v = beta_one ? v + *cij : __builtin_fma(*cij, beta, v);
-          *cij = v;
-        }
-      }
-    }
-  }
+  StoreTile<T, TM, TN>(p, acc, m0 + wm0, n0 + wn0, lane);
 }
-
 
 // Macro-tile kernel: one workgroup per tile, tiles mapped XCD-aware.
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WAVES_M,
@@ -329,6 +351,146 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
   MacroTile<T, TA, TB, BM, BN, BK, WAVES_M, WAVES_N>(p, m0, n0, lds);
 }
 
+// ---------------------------------------------------------------------------
+// The DMA-fed kernel. One operand's BK-deep slice in LDS as 16-byte chunks:
+// KC (contiguous along k in memory: op(A) with A transposed, op(B) with B
+// not transposed) holds MN rows of BK values, the other layout BK rows of MN
+// values. A global_load_lds_dwordx4 writes 64 consecutive chunks, so the
+// layout is fixed by which global chunk each lane fetches: chunk c of a row
+// is stored at slot c ^ Swz(row), chosen so that the MFMA operand reads
+// (16 consecutive rows or columns, lane % 16, at 4 consecutive k, lane / 16)
+// meet distinct banks:
+//  - KC: a read takes one chunk from each of 16 rows; RPB = 16 / (chunks per
+//    row) rows share a 256-byte bank line, and Swz = (row / RPB) mod chunks
+//    per row spreads the 16 over all 16 chunk slots of the line;
+//  - otherwise: a read takes 16 consecutive values (16 sizeof(T) bytes) from
+//    each of 4 k rows; the rows served together (2 for FP64, whose 8-byte
+//    reads go a half-wave at a time, 4 for FP32) are moved apart by
+//    Swz = (row mod rows together) * sizeof(T) chunks.
+template <typename T, bool KC, int MN, int BK>
+struct DmaOperand {
+  static constexpr int E = 16 / int(sizeof(T));  // values per chunk
+  static constexpr int kRowVals = KC ? BK : MN;
+  static constexpr int kRows = KC ? MN : BK;
+  static constexpr int kCpr = kRowVals / E;  // chunks per row
+  static constexpr int kChunks = kRows * kCpr;
+  static_assert(kRowVals % E == 0, "rows of whole chunks");
+  static_assert(KC ? (kCpr >= 2 && kCpr <= 16 && 16 % kCpr == 0) : kCpr >= 16, "bank layout");
+  static __device__ __forceinline__ int Swz(int row) {
+    if (KC) return (row / (16 / kCpr)) % kCpr;
+    constexpr int together = 256 / (16 * int(sizeof(T)));
+    return (row % together) * int(sizeof(T));
+  }
+  // Index (in values) of op(X)'s element at (mn index x, k index y).
+  static __device__ __forceinline__ int At(int x, int y) {
+    const int row = KC ? x : y, v = KC ? y : x;
+    return row * kRowVals + (((v / E) ^ Swz(row)) * E) + v % E;
+  }
+  // Offset (in values, from the tile's first row/column at k = 0) of the
+  // global chunk that belongs in LDS slot `slot`.
+  static __device__ __forceinline__ int64_t Source(int slot, int64_t ld) {
+    const int row = slot / kCpr, c = (slot % kCpr) ^ Swz(row);
+    return int64_t(row) * ld + c * E;
+  }
+};
+
+typedef int i32x4_gemm __attribute__((ext_vector_type(4)));
+
+// Full BM x BN tiles only (m, n multiples of the tile, k of BK; the launcher
+// guarantees it), the whole k range per workgroup, the epilogue StoreTile.
+// Each iteration: wait for this thread's DMAs of slice kt, barrier (every
+// wave's DMAs of slice kt have landed and every wave is done reading the
+// slot of slice kt - 1), multiply slice kt, refilling that slot with slice
+// kt + STAGES - 1 (A's chunks after the first MFMA step, B's after the
+// second).
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WAVES_M, int WAVES_N,
+          int OCC, int STAGES>
+__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
+    __attribute__((amdgpu_waves_per_eu(OCC))) GemmDmaKernel(GemmParams<T> p) {
+  using M = Mfma<T>;
+  using Acc = typename M::Acc;
+  using OA = DmaOperand<T, TA, BM, BK>;
+  using OB = DmaOperand<T, !TB, BN, BK>;
+  constexpr int NW = WAVES_M * WAVES_N;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int A_DMA = OA::kChunks / (64 * NW), B_DMA = OB::kChunks / (64 * NW);
+  static_assert(A_DMA * 64 * NW == OA::kChunks && B_DMA * 64 * NW == OB::kChunks,
+                "whole DMA instructions per wave");
+  constexpr int PER_SLICE = A_DMA + B_DMA;
+  static_assert(PER_SLICE * 3 < 64 && STAGES >= 2 && STAGES <= 4, "vmcnt table");
+  static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 8 == 0, "tile shape");
+  constexpr int kStage = OA::kChunks + OB::kChunks;  // chunks per stage
+  __shared__ i32x4_gemm lds[STAGES * kStage];
+  if (p.gate && *p.gate != 0) return;  // uniform
+
+  int tm, tn;
+  TileCoords(p.tiles_m, p.tiles_n, &tm, &tn);
+  const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+  // op(A)'s tile: rows m0.. (KC: A's columns m0.., k along them).
+  const T* A = p.a + (TA ? m0 * p.lda : m0);
+  const T* B = p.b + (TB ? n0 : n0 * p.ldb);
+  const int64_t a_kstep = TA ? 1 : p.lda, b_kstep = TB ? p.ldb : 1;
+  int64_t a_src[A_DMA], b_src[B_DMA];
+#pragma unroll
+  for (int q = 0; q < A_DMA; ++q) a_src[q] = OA::Source((q * NW + wave) * 64 + lane, p.lda);
+#pragma unroll
+  for (int q = 0; q < B_DMA; ++q) b_src[q] = OB::Source((q * NW + wave) * 64 + lane, p.ldb);
+  auto issue_a = [&](int stage, int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < A_DMA; ++q)
+      DmaToLds(A + a_src[q] + k0 * a_kstep, &lds[stage * kStage + (q * NW + wave) * 64]);
+  };
+  auto issue_b = [&](int stage, int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < B_DMA; ++q)
+      DmaToLds(B + b_src[q] + k0 * b_kstep,
+               &lds[stage * kStage + OA::kChunks + (q * NW + wave) * 64]);
+  };
+
+  Acc acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = Acc{};
+
+  const int ktiles = int(p.k / BK);
+#pragma unroll
+  for (int st = 0; st < STAGES - 1; ++st) {
+    if (st < ktiles) {
+      issue_a(st, int64_t(st) * BK);
+      issue_b(st, int64_t(st) * BK);
+    }
+  }
+  const int li = lane & 15, lk = lane >> 4;
+  for (int kt = 0; kt < ktiles; ++kt) {
+    WaitSlices<PER_SLICE>(min(STAGES - 2, ktiles - 1 - kt));
+    BarrierNoFence();
+    const int next = kt + STAGES - 1, cur = kt % STAGES;
+    const T* as = reinterpret_cast<const T*>(&lds[cur * kStage]);
+    const T* bs = reinterpret_cast<const T*>(&lds[cur * kStage + OA::kChunks]);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T af[TM], bf[TN];
+#pragma unroll
+      for (int ti = 0; ti < TM; ++ti) af[ti] = as[OA::At(wm0 + ti * 16 + li, kk + lk)];
+#pragma unroll
+      for (int tj = 0; tj < TN; ++tj) bf[tj] = bs[OB::At(wn0 + tj * 16 + li, kk + lk)];
+#pragma unroll
+      for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TN; ++tj) acc[ti][tj] = M::Run(af[ti], bf[tj], acc[ti][tj]);
+      if (next < ktiles) {
+        if (kk == 0) issue_a(next % STAGES, int64_t(next) * BK);
+        if (kk == 4) issue_b(next % STAGES, int64_t(next) * BK);
+      }
+    }
+  }
+  StoreTile<T, TM, TN>(p, acc, m0 + wm0, n0 + wn0, lane);
+}
+
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WGM, int WGN, int OCC>
 void Launch(const GemmParams<T>& prm, hipStream_t stream) {
   GemmParams<T> q = prm;
@@ -345,6 +507,69 @@ void Launch(const GemmParams<T>& prm, hipStream_t stream) {
   HWY_HIP_CHECK(hipGetLastError());
 }
 
+template <typename T, bool TA, bool TB>
+void LaunchConfig(int cfg, const GemmParams<T>& prm, hipStream_t stream);
+
+// A DMA configuration: its kernel on the full tiles when the call allows it
+// (a plain product, k a multiple of BK, 16-byte aligned operands, leading
+// dimensions of whole chunks), the built-in plain configuration on the
+// bottom row strip and the right column strip; otherwise the plain
+// configuration on all of C.
+// Whether a DMA configuration is built for type T: its LDS ring fits the
+// 160 KB of a CU and its accumulators leave room for the rest of the
+// registers at its occupancy (otherwise the configuration index runs the
+// plain rule for that type).
+template <typename T, int BM, int BN, int BK, int WGM, int WGN, int OCC, int STAGES>
+constexpr bool DmaBuilt() {
+  return int64_t(STAGES) * (BM + BN) * BK * int(sizeof(T)) <= 160 * 1024 &&
+         (BM / WGM / 16) * (BN / WGN / 16) * int(sizeof(T)) <= 512 / OCC - 64;
+}
+
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WGM, int WGN, int OCC,
+          int STAGES>
+void LaunchDma(const GemmParams<T>& prm, hipStream_t stream) {
+  constexpr int E = 16 / int(sizeof(T));
+  const int64_t mi = prm.m / BM * BM, ni = prm.n / BN * BN;
+  const bool ok = prm.part == 0 && prm.krange == 0 && mi > 0 && ni > 0 && prm.k > 0 &&
+                  prm.k % BK == 0 && prm.lda % E == 0 && prm.ldb % E == 0 &&
+                  reinterpret_cast<uintptr_t>(prm.a) % 16 == 0 &&
+                  reinterpret_cast<uintptr_t>(prm.b) % 16 == 0 &&
+                  (mi / BM) * (ni / BN) < (int64_t(1) << 31);
+  if (!ok) {
+    LaunchConfig<T, TA, TB>(ChooseRegularGemmConfig(sizeof(T), prm.m, prm.n, prm.k), prm, stream);
+    return;
+  }
+  if constexpr (!DmaBuilt<T, BM, BN, BK, WGM, WGN, OCC, STAGES>()) {
+    LaunchConfig<T, TA, TB>(ChooseRegularGemmConfig(sizeof(T), prm.m, prm.n, prm.k), prm, stream);
+  } else {
+    GemmParams<T> q = prm;
+    q.m = mi;
+    q.n = ni;
+    q.tiles_m = int(mi / BM);
+    q.tiles_n = int(ni / BN);
+    q.tri_grid = 0;
+    hipLaunchKernelGGL((GemmDmaKernel<T, TA, TB, BM, BN, BK, WGM, WGN, OCC, STAGES>),
+                       dim3(unsigned(int64_t(q.tiles_m) * q.tiles_n)), dim3(64 * WGM * WGN), 0,
+                       stream, q);
+    HWY_HIP_CHECK(hipGetLastError());
+    if (mi < prm.m) {  // rows mi .. m - 1, every column
+      GemmParams<T> r = prm;
+      r.m = prm.m - mi;
+      r.a = prm.a + (TA ? mi * prm.lda : mi);
+      r.c = prm.c + mi;
+      LaunchConfig<T, TA, TB>(ChooseRegularGemmConfig(sizeof(T), r.m, r.n, r.k), r, stream);
+    }
+    if (ni < prm.n) {  // rows 0 .. mi - 1, columns ni .. n - 1
+      GemmParams<T> r = prm;
+      r.m = mi;
+      r.n = prm.n - ni;
+      r.b = prm.b + (TB ? ni : ni * prm.ldb);
+      r.c = prm.c + ni * prm.ldc;
+      LaunchConfig<T, TA, TB>(ChooseRegularGemmConfig(sizeof(T), r.m, r.n, r.k), r, stream);
+    }
+  }
+}
+
 // Runs configuration `cfg` (an index into kGemmConfigs).
 template <typename T, bool TA, bool TB>
 void LaunchConfig(int cfg, const GemmParams<T>& prm, hipStream_t stream) {
@@ -354,6 +579,11 @@ void LaunchConfig(int cfg, const GemmParams<T>& prm, hipStream_t stream) {
     Launch<T, TA, TB, BM, BN, BK, WM, WN, OCC>(prm, stream); break;
     HWY_GEMM_CONFIGS(HWY_GEMM_CASE)
 #undef HWY_GEMM_CASE
+#define HWY_GEMM_DMA_CASE(ID, BM, BN, BK, WM, WN, OCC, ST) \
+  case ID:                                                 \
+    LaunchDma<T, TA, TB, BM, BN, BK, WM, WN, OCC, ST>(prm, stream); break;
+    HWY_GEMM_DMA_CONFIGS(HWY_GEMM_DMA_CASE)
+#undef HWY_GEMM_DMA_CASE
     default:
       Launch<T, TA, TB, 64, 64, 16, 2, 2, 1>(prm, stream);
   }

@@ -637,6 +637,13 @@ HWYBLAS_AMD_API int hwyblas_gemm_config_count(void);
  * out of range. */
 HWYBLAS_AMD_API int hwyblas_gemm_config_info(int cfg, int* bm, int* bn, int* bk,
                                              int* waves);
+/* The depth of a configuration's LDS ring when its slices are streamed by
+ * direct global-to-LDS loads (2 or more; such a configuration runs the
+ * full tiles of calls with k a multiple of bk, 16-byte aligned operands and
+ * leading dimensions of whole 16-byte chunks, and hands the edge strips and
+ * other calls to the plain kernel), 0 for a register-staged configuration,
+ * -1 if cfg is out of range. */
+HWYBLAS_AMD_API int hwyblas_gemm_config_stages(int cfg);
 /* The configuration the Fortran/CBLAS/dev entry points use for a shape. */
 HWYBLAS_AMD_API int hwyblas_gemm_default_config(int dtype_bytes, char transa,
                                                 char transb, int m, int n, int k);

@@ -284,6 +284,57 @@ def test_every_tile_config_gives_identical_bits(H, dtype):
             assert np.array_equal(got[m:], c[m:])
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_dma_configs_cover_full_tiles_strips_and_fallbacks(H, dtype):
+    """The DMA-fed configurations take the full tiles of calls with k a
+    multiple of 16, 16-byte aligned operands and leading dimensions of whole
+    16-byte chunks, and hand the bottom and right strips (and every other
+    call) to the plain kernel. Each case equals a plain configuration bit for
+    bit and stays within the GEMM bound; C's padding rows are untouched."""
+    L = H.lib()
+    dma = [c for c in range(L.hwyblas_gemm_config_count()) if L.hwyblas_gemm_config_stages(c) > 0]
+    assert dma and L.hwyblas_gemm_config_stages(0) == 0
+    assert L.hwyblas_gemm_config_stages(L.hwyblas_gemm_config_count()) == -1
+    item = np.dtype(dtype).itemsize
+    cases = [  # (m, n, k, pad, byte offset of A's first element)
+        (300, 520, 48, 0, 0),    # full tiles plus both strips
+        (256, 256, 64, 4, 0),    # whole tiles only, padded leading dimensions
+        (1000, 600, 160, 0, 0),  # several tiles and strips
+        (300, 520, 40, 0, 0),    # k not a multiple of 16: the plain kernel
+        (300, 520, 48, 1, 0),    # odd leading dimensions: the plain kernel
+        (300, 520, 48, 0, item),  # A one element off 16 bytes: the plain kernel
+        (100, 90, 32, 0, 0),     # smaller than one tile: the plain kernel
+    ]
+    for (ta, tb), (m, n, k, pad, off) in itertools.product(TRANS, cases):
+        a, lda, b, ldb, c, ldc = gemm_operands(ta, tb, m, n, k, pad, dtype, m + 3 * n + k)
+        abig = np.zeros(a.size + 4, dtype=dtype)
+        da = H.DeviceArray.from_host(abig.reshape(-1, 1))
+        H.lib().hwyblas_amd_memcpy(da.ptr + off, a.ctypes.data, a.nbytes, 1)
+        H.synchronize()
+        db = H.DeviceArray.from_host(b)
+        ref = None
+        for cfg in [0] + dma:
+            dc = H.DeviceArray.from_host(c)
+            assert H.gemm_dev_cfg(cfg, ta, tb, m, n, k, 0.5, da.ptr + off, lda, db, ldb, -1.0,
+                                  dc, ldc, dtype=dtype) == 0
+            got = dc.to_host()
+            if ref is None:
+                ref = got
+                r = gemm_bound_ratio(got, ta, tb, m, n, k, 0.5, a, lda, b, ldb, -1.0, c, ldc)
+                assert r <= 1.0, (ta, tb, m, n, k, pad, off, r)
+            assert np.array_equal(got, ref), (cfg, ta, tb, m, n, k, pad, off)
+            assert np.array_equal(got[m:], c[m:])
+
+
+def test_default_config_takes_the_dma_kernel_for_large_products(H):
+    L = H.lib()
+    for dtype_bytes in (8, 4):
+        cfg = L.hwyblas_gemm_default_config(dtype_bytes, b"N", b"N", 8192, 8192, 8192)
+        assert L.hwyblas_gemm_config_stages(cfg) > 0
+        small = L.hwyblas_gemm_default_config(dtype_bytes, b"N", b"N", 256, 256, 256)
+        assert L.hwyblas_gemm_config_stages(small) == 0
+
+
 def test_device_pow2_fill_and_scaling_match_oracle(H):
     """The wide-exponent test data (+-u 2^e) and the row/column scaling the
     benchmark uses are bit for bit those the oracle regenerates on the host."""
