@@ -713,6 +713,50 @@ def zgemm_leg(H, n=8192, reps=3):
     return out
 
 
+def host_path_leg(H, sizes=(8192, 16384), reps=3):
+    """dgemm_ on host (pageable numpy) operands, the path Eigen and the LAPACK
+    port use: NN, alpha = 1, beta = 0 (C is not copied in), synchronous,
+    best of `reps` wall-clock calls. Beside it the same call on device
+    operands and the pageable copy rates of one n x n matrix each way."""
+    L = H.lib()
+    out = {}
+    for n in sizes:
+        dA, dB, dC = (H.DeviceArray(n, n) for _ in range(3))
+        dA.fill_uniform(n, n, 1)
+        dB.fill_uniform(n, n, 2)
+        H.synchronize()
+        a, b = dA.to_host(), dB.to_host()
+        c = np.zeros((n, n), order="F")
+
+        def best(fn):
+            fn()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            return min(ts)
+        t_host = best(lambda: H.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n))
+        t_dev = best(lambda: (H.gemm("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n),
+                              H.synchronize()))
+        nbytes = n * n * 8
+        t_h2d = best(lambda: L.hwyblas_amd_memcpy(dC.ptr, a.ctypes.data, nbytes, 1))
+        t_d2h = best(lambda: L.hwyblas_amd_memcpy(c.ctypes.data, dC.ptr, nbytes, 2))
+        flops = 2.0 * n ** 3
+        r = {"n": n, "host_ms": round(t_host * 1e3, 1),
+             "host_tflops": round(flops / t_host / 1e12, 2),
+             "device_ms": round(t_dev * 1e3, 1),
+             "device_tflops": round(flops / t_dev / 1e12, 2),
+             "h2d_gbs": round(nbytes / t_h2d / 1e9, 1), "d2h_gbs": round(nbytes / t_d2h / 1e9, 1),
+             "pcie_bytes": 3 * nbytes}
+        out[str(n)] = r
+        log(f"dgemm_ host operands n={n}: {r['host_ms']} ms ({r['host_tflops']} TFLOP/s) "
+            f"vs device {r['device_ms']} ms; pageable H2D {r['h2d_gbs']} GB/s, "
+            f"D2H {r['d2h_gbs']} GB/s")
+        del dA, dB, dC, a, b, c
+    return out
+
+
 def potrf_leg(H, pn=16384, reps=2):
     """DPOTRF (lower) at pn on A = G G^T / pn + I (SPD), G uniform[-1, 1] seed
     11, formed on the device with the library's DGEMM; the best of `reps`
@@ -1068,6 +1112,9 @@ def main():
     ap.add_argument("--level3", action="store_true",
                     help="DSYMM/DSYRK/DSYR2K/DGEMMTR/DTRMM at n = 8192 beside DGEMM, "
                          "DGEMV and DPOTRF at 16384, ZGEMM/CGEMM and ZTRSM/CTRSM at 8192")
+    ap.add_argument("--host-path", action="store_true",
+                    help="dgemm_ on host (pageable) operands at 8192 and 16384 beside the "
+                         "device-operand call, and the pageable copy rates")
     ap.add_argument("--potrf", action="store_true",
                     help="only DPOTRF (lower) at n = 16384")
     ap.add_argument("--zgemm", action="store_true",
@@ -1107,7 +1154,7 @@ def main():
     # touches the GPU; it waits until the GPU legs are done.
     cpu_proc = None
     if not args.no_cpu and not (args.lu or args.plain_sweep or args.level3 or args.potrf or
-                                args.zgemm or args.batched_only):
+                                args.zgemm or args.batched_only or args.host_path):
         import subprocess
         cpu_proc = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker"],
                                     stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
@@ -1142,6 +1189,9 @@ def main():
         return
     if args.potrf:
         print(json.dumps({"dpotrf": potrf_leg(H)}), flush=True)
+        return
+    if args.host_path:
+        print(json.dumps({"host_path": host_path_leg(H)}), flush=True)
         return
     if args.zgemm:
         print(json.dumps({"zgemm": zgemm_leg(H)}), flush=True)

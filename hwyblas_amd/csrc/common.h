@@ -137,6 +137,11 @@ void* Staging(int slot, size_t bytes, hipStream_t user_stream = nullptr,
 // managed memory); host memory, including pinned memory, is staged.
 bool IsDevicePointer(const void* p);
 
+// hipMemcpy2DAsync with the same arguments, except that a contiguous span
+// (both pitches equal to the width, or a single row) goes as one 1-D copy.
+hipError_t Memcpy2DAsync(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                         size_t height, hipMemcpyKind kind, hipStream_t stream);
+
 // ---------------------------------------------------------------------------
 // Device operands and launchers (gemm.hip, trsm.hip). All take device
 // pointers, enqueue work on `stream` and return without synchronising.

@@ -113,6 +113,14 @@ void* Staging(int slot, size_t bytes, hipStream_t user_stream, bool* grown) {
   return c.buf[slot];
 }
 
+hipError_t Memcpy2DAsync(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                         size_t height, hipMemcpyKind kind, hipStream_t stream) {
+  if (width == 0 || height == 0) return hipSuccess;
+  if (height == 1 || (dpitch == width && spitch == width))
+    return hipMemcpyAsync(dst, src, width * height, kind, stream);
+  return hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, kind, stream);
+}
+
 bool IsDevicePointer(const void* p) {
   if (p == nullptr) return false;
   hipPointerAttribute_t attr;
