@@ -13,7 +13,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <thread>
 #include <type_traits>
+#include <vector>
 
 #include "entries.h"
 
@@ -105,6 +107,108 @@ blasint TrsmInfo(char side, char uplo, char transa, char diag, blasint m,
   return 0;
 }
 
+// Large products on host operands, pipelined over blocks of C's columns:
+// a helper thread copies op(B)'s block (and C's, when beta != 0) in on one
+// stream, the calling thread runs the block's product on its own stream,
+// and a second helper copies the block of C out on a third, through two
+// device buffers per operand, so that the copies of neighbouring blocks
+// overlap the product (copies from pageable memory block the thread that
+// issues them, hence one thread per direction). A goes in first. Events
+// order the streams; host counters tell each thread which events have been
+// recorded. Every block runs the same k order as the whole product, so the
+// bits equal the unblocked call's. False (nothing done) when the product is
+// small or an operand is device memory.
+template <typename T>
+bool GemmHostPipelined(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
+                       int64_t lda, const T* b, int64_t ldb, T beta, T* c, int64_t ldc) {
+  constexpr int64_t kMinBlock = 1024;
+  if (m * n * k < (int64_t(1) << 36) || n < 4 * kMinBlock) return false;
+  if (IsDevicePointer(a) || IsDevicePointer(b) || IsDevicePointer(c)) return false;
+  const int64_t nb = std::max<int64_t>(kMinBlock, ((n + 7) / 8 + 255) / 256 * 256);
+  const int nblk = int((n + nb - 1) / nb);
+  int dev = 0;
+  HWY_HIP_CHECK(hipGetDevice(&dev));
+  ThreadContext& ctx = Context();
+  if (ctx.copy_in == nullptr) {
+    HWY_HIP_CHECK(hipStreamCreateWithFlags(&ctx.copy_in, hipStreamNonBlocking));
+    HWY_HIP_CHECK(hipStreamCreateWithFlags(&ctx.copy_out, hipStreamNonBlocking));
+  }
+  hipStream_t s = ctx.stream, sin = ctx.copy_in, sout = ctx.copy_out;
+  const int64_t ra = ta ? k : m, ca = ta ? m : k;
+  T* da = static_cast<T*>(Staging(kSlotA, size_t(ra) * ca * sizeof(T)));
+  T* bring = static_cast<T*>(Staging(kSlotB, size_t(2) * k * nb * sizeof(T)));
+  T* cring = static_cast<T*>(Staging(kSlotC, size_t(2) * m * nb * sizeof(T)));
+  auto width = [&](int j) { return std::min(nb, n - j * nb); };
+  auto bslot = [&](int j) { return bring + (j % 2) * k * nb; };
+  auto cslot = [&](int j) { return cring + (j % 2) * m * nb; };
+
+  std::vector<hipEvent_t> ev_in(nblk), ev_gemm(nblk), ev_out(nblk);
+  for (int j = 0; j < nblk; ++j) {
+    HWY_HIP_CHECK(hipEventCreateWithFlags(&ev_in[j], hipEventDisableTiming));
+    HWY_HIP_CHECK(hipEventCreateWithFlags(&ev_gemm[j], hipEventDisableTiming));
+    HWY_HIP_CHECK(hipEventCreateWithFlags(&ev_out[j], hipEventDisableTiming));
+  }
+  // Blocks whose copy-in / product / copy-out event has been recorded.
+  std::atomic<int> in_rec{0}, gemm_rec{0}, out_rec{0};
+  auto wait_for = [](const std::atomic<int>& count, int need) {
+    while (count.load(std::memory_order_acquire) < need) std::this_thread::yield();
+  };
+
+  CopyIn(da, a, ra, ca, lda, s);
+  std::thread copier_in([&] {
+    HWY_HIP_CHECK(hipSetDevice(dev));
+    for (int j = 0; j < nblk; ++j) {
+      const int64_t j0 = j * nb, w = width(j);
+      if (j >= 2) {  // the slots of block j - 2 are free once its product (and copy-out) ran
+        wait_for(gemm_rec, j - 1);
+        HWY_HIP_CHECK(hipStreamWaitEvent(sin, ev_gemm[j - 2], 0));
+        if (beta != T(0)) {
+          wait_for(out_rec, j - 1);
+          HWY_HIP_CHECK(hipStreamWaitEvent(sin, ev_out[j - 2], 0));
+        }
+      }
+      if (tb) CopyIn(bslot(j), b + j0, w, k, ldb, sin);  // rows j0.. of B: w x k
+      else CopyIn(bslot(j), b + j0 * ldb, k, w, ldb, sin);
+      if (beta != T(0)) CopyIn(cslot(j), c + j0 * ldc, m, w, ldc, sin);
+      HWY_HIP_CHECK(hipEventRecord(ev_in[j], sin));
+      in_rec.store(j + 1, std::memory_order_release);
+    }
+  });
+  std::thread copier_out([&] {
+    HWY_HIP_CHECK(hipSetDevice(dev));
+    for (int j = 0; j < nblk; ++j) {
+      wait_for(gemm_rec, j + 1);
+      HWY_HIP_CHECK(hipStreamWaitEvent(sout, ev_gemm[j], 0));
+      CopyOut(c + j * nb * ldc, ldc, cslot(j), m, width(j), sout);
+      HWY_HIP_CHECK(hipEventRecord(ev_out[j], sout));
+      out_rec.store(j + 1, std::memory_order_release);
+    }
+  });
+  for (int j = 0; j < nblk; ++j) {
+    const int64_t w = width(j);
+    wait_for(in_rec, j + 1);
+    HWY_HIP_CHECK(hipStreamWaitEvent(s, ev_in[j], 0));
+    if (j >= 2 && beta == T(0)) {  // C's slot is free once block j - 2 went out
+      wait_for(out_rec, j - 1);
+      HWY_HIP_CHECK(hipStreamWaitEvent(s, ev_out[j - 2], 0));
+    }
+    GemmByMode<T>(ta, tb, m, w, k, alpha, da, ra, bslot(j), tb ? w : k, beta, cslot(j), m, s);
+    HWY_HIP_CHECK(hipEventRecord(ev_gemm[j], s));
+    gemm_rec.store(j + 1, std::memory_order_release);
+  }
+  copier_in.join();
+  copier_out.join();
+  HWY_HIP_CHECK(hipStreamSynchronize(sout));
+  HWY_HIP_CHECK(hipStreamSynchronize(sin));
+  HWY_HIP_CHECK(hipStreamSynchronize(s));
+  for (int j = 0; j < nblk; ++j) {
+    HWY_HIP_CHECK(hipEventDestroy(ev_in[j]));
+    HWY_HIP_CHECK(hipEventDestroy(ev_gemm[j]));
+    HWY_HIP_CHECK(hipEventDestroy(ev_out[j]));
+  }
+  return true;
+}
+
 template <typename T>
 void GemmHost(const Caller& who, const char* transa, const char* transb,
               const blasint* mp, const blasint* np, const blasint* kp,
@@ -121,6 +225,10 @@ void GemmHost(const Caller& who, const char* transa, const char* transb,
   if (m == 0 || n == 0) return;
   if ((k == 0 || alpha == T(0)) && beta == T(1)) return;
 
+  if (k != 0 && alpha != T(0) &&
+      GemmHostPipelined<T>(oa == Op::kTrans, ob == Op::kTrans, m, n, k, alpha, a, lda, b, ldb,
+                           beta, c, ldc))
+    return;
   ThreadContext& ctx = Context();
   hipStream_t s = ctx.stream;
   const bool c_dev = IsDevicePointer(c);

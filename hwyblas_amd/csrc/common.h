@@ -120,6 +120,8 @@ enum StagingSlot : int {
 
 struct ThreadContext {
   hipStream_t stream = nullptr;
+  // Streams of the pipelined host-operand GEMM's copies (created on first use).
+  hipStream_t copy_in = nullptr, copy_out = nullptr;
   int device = -1;
   void* buf[kNumSlots] = {};
   size_t cap[kNumSlots] = {};

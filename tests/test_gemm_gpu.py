@@ -326,6 +326,35 @@ def test_dma_configs_cover_full_tiles_strips_and_fallbacks(H, dtype):
             assert np.array_equal(got[m:], c[m:])
 
 
+@pytest.mark.parametrize("ta,tb,beta,pad", [("N", "N", 0.0, 0), ("T", "T", 0.5, 3),
+                                            ("N", "T", 0.5, 0), ("T", "N", 0.0, 2)])
+def test_host_operand_pipeline_equals_device_call(H, ta, tb, beta, pad):
+    """dgemm_ on host operands large enough for the pipelined path (blocks of
+    C's columns copied in, multiplied and copied out on three streams) gives
+    bit for bit the device-operand call's C, and leaves C's padding rows
+    alone."""
+    m, n, k = 4096, 4608, 4096
+    ra, ca = op_shape(ta, m, k)
+    rb, cb = op_shape(tb, k, n)
+    lda, ldb, ldc = ra + pad, rb + pad, m + pad
+    arrs = []
+    for rows, cols, ld, seed in ((ra, ca, lda, 1), (rb, cb, ldb, 2), (m, n, ldc, 3)):
+        d = H.DeviceArray(ld, cols)
+        d.fill_uniform(ld, cols, seed)
+        arrs.append(d)
+    H.synchronize()
+    a, b, c0 = (d.to_host() for d in arrs)
+    c0[m:] = 7.25
+    dc = H.DeviceArray.from_host(c0)
+    H.gemm(ta, tb, m, n, k, 1.0, arrs[0], lda, arrs[1], ldb, beta, dc, ldc)
+    H.synchronize()
+    want = dc.to_host()
+    c = c0.copy(order="F")
+    H.gemm(ta, tb, m, n, k, 1.0, a, lda, b, ldb, beta, c, ldc)
+    assert np.array_equal(c, want)
+    assert np.all(c[m:] == 7.25)
+
+
 def test_default_config_takes_the_dma_kernel_for_large_products(H):
     L = H.lib()
     for dtype_bytes in (8, 4):
