@@ -757,6 +757,48 @@ def host_path_leg(H, sizes=(8192, 16384), reps=3):
     return out
 
 
+def vendor_leg(n=16384, reps=5):
+    """The vendor libraries on the same products, for reference: torch.mm
+    (hipBLASLt / rocBLAS) in FP64 and FP32 and torch._int_mm (hipBLASLt INT8,
+    int32 result) at m = n = k = `n`, HIP-event timed, best of `reps` after a
+    warm-up. Not this library's code path."""
+    import torch
+    dev = "cuda:0"
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    out = {}
+
+    def best(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ts = []
+        for _ in range(reps):
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return min(ts)
+    a8 = torch.randint(-127, 128, (n, n), dtype=torch.int8, device=dev, generator=g)
+    b8 = torch.randint(-127, 128, (n, n), dtype=torch.int8, device=dev, generator=g)
+    ms = best(lambda: torch._int_mm(a8, b8))
+    out["int8"] = {"ms": round(ms, 3), "tops": round(2 * n ** 3 / ms / 1e9, 1)}
+    del a8, b8
+    for name, dt, peak in (("fp64", torch.float64, PEAK_FP64), ("fp32", torch.float32, 157.3)):
+        a = torch.rand(n, n, dtype=dt, device=dev, generator=g)
+        b = torch.rand(n, n, dtype=dt, device=dev, generator=g)
+        ms = best(lambda: torch.mm(a, b))
+        tf = 2 * n ** 3 / ms / 1e9
+        out[name] = {"ms": round(ms, 3), "tflops": round(tf, 1), "frac_peak": round(tf / peak, 3)}
+        del a, b
+    torch.cuda.empty_cache()
+    log(f"vendor n={n}: FP64 {out['fp64']['ms']} ms ({out['fp64']['tflops']} TFLOP/s), FP32 "
+        f"{out['fp32']['ms']} ms ({out['fp32']['tflops']} TFLOP/s), INT8 {out['int8']['ms']} ms "
+        f"({out['int8']['tops']} TOP/s)")
+    return out
+
+
 def potrf_leg(H, pn=16384, reps=2):
     """DPOTRF (lower) at pn on A = G G^T / pn + I (SPD), G uniform[-1, 1] seed
     11, formed on the device with the library's DGEMM; the best of `reps`
@@ -1115,6 +1157,9 @@ def main():
     ap.add_argument("--host-path", action="store_true",
                     help="dgemm_ on host (pageable) operands at 8192 and 16384 beside the "
                          "device-operand call, and the pageable copy rates")
+    ap.add_argument("--vendor", action="store_true",
+                    help="the vendor libraries' FP64/FP32/INT8 GEMM at 16384 through torch, for "
+                         "reference")
     ap.add_argument("--potrf", action="store_true",
                     help="only DPOTRF (lower) at n = 16384")
     ap.add_argument("--zgemm", action="store_true",
@@ -1154,7 +1199,8 @@ def main():
     # touches the GPU; it waits until the GPU legs are done.
     cpu_proc = None
     if not args.no_cpu and not (args.lu or args.plain_sweep or args.level3 or args.potrf or
-                                args.zgemm or args.batched_only or args.host_path):
+                                args.zgemm or args.batched_only or args.host_path or
+                                args.vendor):
         import subprocess
         cpu_proc = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker"],
                                     stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
@@ -1192,6 +1238,9 @@ def main():
         return
     if args.host_path:
         print(json.dumps({"host_path": host_path_leg(H)}), flush=True)
+        return
+    if args.vendor:
+        print(json.dumps({"vendor": vendor_leg()}), flush=True)
         return
     if args.zgemm:
         print(json.dumps({"zgemm": zgemm_leg(H)}), flush=True)
