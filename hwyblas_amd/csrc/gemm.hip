@@ -88,8 +88,7 @@ namespace {
 // A triangle of C runs on square tiles, so that only the tiles meeting it
 // are launched (the 128 x 128 x 16 two-waves-per-SIMD tile otherwise).
 int PartConfig(int cfg, int part) {
-  if (part == 0 || (kGemmConfigs[cfg].bm == kGemmConfigs[cfg].bn && !kGemmConfigs[cfg].stages))
-    return cfg;
+  if (part == 0 || kGemmConfigs[cfg].bm == kGemmConfigs[cfg].bn) return cfg;
   return 8;
 }
 

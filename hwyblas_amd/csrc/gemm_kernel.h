@@ -111,6 +111,47 @@ __device__ __forceinline__ void TileCoords(int tiles_m, int tiles_n, int* tm,
   *tn = in_group / gm;
 }
 
+// The workgroup's output tile (tm, tn): the triangle's tiles row by row when
+// the grid holds only those (tri_grid), else the XCD-aware raster, its bands
+// over tile columns when the work varies by tile row (krange 1, 2).
+template <typename T>
+__device__ __forceinline__ void TileOf(const GemmParams<T>& p, int* tm, int* tn) {
+  if (p.tri_grid) {
+    // Tile t of the triangle, row by row: r (r + 1) / 2 <= t < (r + 1) (r + 2) / 2.
+    const int t = XcdRunIndex();
+    int r = int((sqrtf(8.0f * float(t) + 1.0f) - 1.0f) * 0.5f);
+    while ((r + 1) * (r + 2) / 2 <= t) ++r;
+    while (r * (r + 1) / 2 > t) --r;
+    const int c = t - r * (r + 1) / 2;
+    *tm = p.part == 1 ? r : c;
+    *tn = p.part == 1 ? c : r;
+  } else if (p.krange == 1 || p.krange == 2) {
+    // Work that varies by tile row: the raster bands run over tile columns,
+    // so every XCD's share spans all rows.
+    TileCoords(p.tiles_n, p.tiles_m, tn, tm);
+  } else {
+    TileCoords(p.tiles_m, p.tiles_n, tm, tn);
+  }
+}
+
+// The k slices [kt0, kt1) of BK a tile at (m0, n0) runs: all of k, or those
+// meeting a triangular factor's nonzero k-range (GemmParams::krange).
+template <typename T, int BM, int BN, int BK>
+__device__ __forceinline__ void KSlices(const GemmParams<T>& p, int64_t m0, int64_t n0, int* kt0,
+                                        int* kt1) {
+  *kt0 = 0;
+  *kt1 = int((p.k + BK - 1) / BK);
+  if (p.krange) {
+    int64_t kb = 0, ke = p.k;
+    if (p.krange == 1) kb = m0;
+    else if (p.krange == 2) ke = m0 + BM;
+    else if (p.krange == 3) ke = n0 + BN;
+    else kb = n0;
+    *kt0 = int(min(kb, p.k) / BK);
+    *kt1 = int((min(ke, p.k) + BK - 1) / BK);
+  }
+}
+
 // LDS bytes one macro-tile needs (two buffers of each operand's slice).
 // An operand whose memory is contiguous along k is kept row-major with k
 // fastest ([row][K_LD]); one contiguous along rows/columns is kept k-major
@@ -275,16 +316,8 @@ __device__ __forceinline__ void MacroTile(const GemmParams<T>& p, int64_t m0, in
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = Acc{};
 
-  int kt0 = 0, kt1 = int((p.k + BK - 1) / BK);
-  if (p.krange) {
-    int64_t kb = 0, ke = p.k;
-    if (p.krange == 1) kb = m0;
-    else if (p.krange == 2) ke = m0 + BM;
-    else if (p.krange == 3) ke = n0 + BN;
-    else kb = n0;
-    kt0 = int(min(kb, p.k) / BK);
-    kt1 = int((min(ke, p.k) + BK - 1) / BK);
-  }
+  int kt0, kt1;
+  KSlices<T, BM, BN, BK>(p, m0, n0, &kt0, &kt1);
   load_tile(int64_t(kt0) * BK);
   store_lds(0);
   __syncthreads();
@@ -329,22 +362,7 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
   __shared__ T lds[TileLds<T, TA, TB, BM, BN, BK>::kElems];
   if (p.gate && *p.gate != 0) return;  // uniform
   int tm, tn;
-  if (p.tri_grid) {
-    // Tile t of the triangle, row by row: r (r + 1) / 2 <= t < (r + 1) (r + 2) / 2.
-    const int t = XcdRunIndex();
-    int r = int((sqrtf(8.0f * float(t) + 1.0f) - 1.0f) * 0.5f);
-    while ((r + 1) * (r + 2) / 2 <= t) ++r;
-    while (r * (r + 1) / 2 > t) --r;
-    const int c = t - r * (r + 1) / 2;
-    tm = p.part == 1 ? r : c;
-    tn = p.part == 1 ? c : r;
-  } else if (p.krange == 1 || p.krange == 2) {
-    // Work that varies by tile row: the raster bands run over tile columns,
-    // so every XCD's share spans all rows.
-    TileCoords(p.tiles_n, p.tiles_m, &tn, &tm);
-  } else {
-    TileCoords(p.tiles_m, p.tiles_n, &tm, &tn);
-  }
+  TileOf(p, &tm, &tn);
   const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
   // A tile entirely on the other side of the diagonal (uniform exit).
   if ((p.part == 1 && m0 + BM - 1 < n0) || (p.part == 2 && m0 > n0 + BN - 1)) return;
@@ -425,8 +443,10 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
   if (p.gate && *p.gate != 0) return;  // uniform
 
   int tm, tn;
-  TileCoords(p.tiles_m, p.tiles_n, &tm, &tn);
+  TileOf(p, &tm, &tn);
   const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
+  // A tile entirely on the other side of the diagonal (uniform exit).
+  if ((p.part == 1 && m0 + BM - 1 < n0) || (p.part == 2 && m0 > n0 + BN - 1)) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
   // op(A)'s tile: rows m0.. (KC: A's columns m0.., k along them).
@@ -456,12 +476,16 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = Acc{};
 
-  const int ktiles = int(p.k / BK);
+  // Slices kt0 + i, i < ktiles; ring position i % STAGES.
+  int kt0, kt1;
+  KSlices<T, BM, BN, BK>(p, m0, n0, &kt0, &kt1);
+  const int ktiles = kt1 - kt0;
+  const int64_t kbase = int64_t(kt0) * BK;
 #pragma unroll
   for (int st = 0; st < STAGES - 1; ++st) {
     if (st < ktiles) {
-      issue_a(st, int64_t(st) * BK);
-      issue_b(st, int64_t(st) * BK);
+      issue_a(st, kbase + int64_t(st) * BK);
+      issue_b(st, kbase + int64_t(st) * BK);
     }
   }
   const int li = lane & 15, lk = lane >> 4;
@@ -483,8 +507,8 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
 #pragma unroll
         for (int tj = 0; tj < TN; ++tj) acc[ti][tj] = M::Run(af[ti], bf[tj], acc[ti][tj]);
       if (next < ktiles) {
-        if (kk == 0) issue_a(next % STAGES, int64_t(next) * BK);
-        if (kk == 4) issue_b(next % STAGES, int64_t(next) * BK);
+        if (kk == 0) issue_a(next % STAGES, kbase + int64_t(next) * BK);
+        if (kk == 4) issue_b(next % STAGES, kbase + int64_t(next) * BK);
       }
     }
   }
@@ -511,10 +535,10 @@ template <typename T, bool TA, bool TB>
 void LaunchConfig(int cfg, const GemmParams<T>& prm, hipStream_t stream);
 
 // A DMA configuration: its kernel on the full tiles when the call allows it
-// (a plain product, k a multiple of BK, 16-byte aligned operands, leading
-// dimensions of whole chunks), the built-in plain configuration on the
-// bottom row strip and the right column strip; otherwise the plain
-// configuration on all of C.
+// (k a multiple of BK, 16-byte aligned operands, leading dimensions of whole
+// chunks; a triangle of C or a k-range only without strips), the built-in
+// plain configuration on the bottom row strip and the right column strip;
+// otherwise the plain configuration on all of C.
 // Whether a DMA configuration is built for type T: its LDS ring fits the
 // 160 KB of a CU and its accumulators leave room for the rest of the
 // registers at its occupancy (otherwise the configuration index runs the
@@ -530,8 +554,14 @@ template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WGM, int WGN
 void LaunchDma(const GemmParams<T>& prm, hipStream_t stream) {
   constexpr int E = 16 / int(sizeof(T));
   const int64_t mi = prm.m / BM * BM, ni = prm.n / BN * BN;
-  const bool ok = prm.part == 0 && prm.krange == 0 && mi > 0 && ni > 0 && prm.k > 0 &&
-                  prm.k % BK == 0 && prm.lda % E == 0 && prm.ldb % E == 0 &&
+  // A triangle of C or a k-range that follows the tile position (defined on
+  // the whole C) is taken only when there are no strips; a triangle needs a
+  // square C on square tiles.
+  const bool plain = prm.part == 0 && prm.krange == 0;
+  const bool whole = mi == prm.m && ni == prm.n &&
+                     (prm.part == 0 || (prm.m == prm.n && BM == BN));
+  const bool ok = (plain || whole) && mi > 0 && ni > 0 && prm.k > 0 && prm.k % BK == 0 &&
+                  prm.lda % E == 0 && prm.ldb % E == 0 &&
                   reinterpret_cast<uintptr_t>(prm.a) % 16 == 0 &&
                   reinterpret_cast<uintptr_t>(prm.b) % 16 == 0 &&
                   (mi / BM) * (ni / BN) < (int64_t(1) << 31);
@@ -547,10 +577,11 @@ void LaunchDma(const GemmParams<T>& prm, hipStream_t stream) {
     q.n = ni;
     q.tiles_m = int(mi / BM);
     q.tiles_n = int(ni / BN);
-    q.tri_grid = 0;
+    q.tri_grid = q.part != 0 ? 1 : 0;
+    const int64_t blocks = q.tri_grid ? int64_t(q.tiles_m) * (q.tiles_m + 1) / 2
+                                      : int64_t(q.tiles_m) * q.tiles_n;
     hipLaunchKernelGGL((GemmDmaKernel<T, TA, TB, BM, BN, BK, WGM, WGN, OCC, STAGES>),
-                       dim3(unsigned(int64_t(q.tiles_m) * q.tiles_n)), dim3(64 * WGM * WGN), 0,
-                       stream, q);
+                       dim3(unsigned(blocks)), dim3(64 * WGM * WGN), 0, stream, q);
     HWY_HIP_CHECK(hipGetLastError());
     if (mi < prm.m) {  // rows mi .. m - 1, every column
       GemmParams<T> r = prm;

@@ -206,6 +206,47 @@ def test_device_operands(H):
     assert np.array_equal(db.to_host(), want)
 
 
+@pytest.mark.parametrize("dtype,n", [(np.float64, 3072), (np.float32, 4096)])
+def test_large_triangle_and_k_range_products(H, dtype, n):
+    """Sizes where the default configuration is the DMA-fed kernel: SYRK's
+    triangle (a triangle-of-C launch) equals the full product A A^T of the
+    same library bit for bit and leaves the other triangle alone; TRMM (a
+    k-range launch that skips the factor's zero slices) equals the product
+    with the triangle written out densely."""
+    L = H.lib()
+    assert L.hwyblas_gemm_config_stages(
+        L.hwyblas_gemm_default_config(np.dtype(dtype).itemsize, b"N", b"T", n, n, n)) > 0
+    k = 512
+    a = H.DeviceArray(n, k, dtype)
+    a.fill_uniform(n, k, 31)
+    full = H.DeviceArray(n, n, dtype)
+    H.gemm("N", "T", n, n, k, 1.5, a, n, a, n, 0.0, full, n, dtype=dtype)
+    want = full.to_host()
+    for uplo in "LU":
+        c = H.DeviceArray(n, n, dtype)
+        c.fill_uniform(n, n, 32)
+        c0 = c.to_host()
+        H.syrk(uplo, "N", n, k, 1.5, a, n, 0.0, c, n, dtype=dtype)
+        got = c.to_host()
+        tri = tri_mask(n, uplo == "L")
+        assert np.array_equal(got[tri], want[tri]), uplo
+        assert np.array_equal(got[~tri], c0[~tri]), uplo
+    t = H.DeviceArray(n, n, dtype)
+    t.fill_uniform(n, n, 33)
+    th = t.to_host()
+    b = H.DeviceArray(n, n, dtype)
+    b.fill_uniform(n, n, 34)
+    bh = b.to_host()
+    for uplo, ta in (("L", "N"), ("U", "T")):
+        dense = np.asfortranarray(np.where(tri_mask(n, uplo == "L"), th, 0).astype(dtype))
+        dd = H.DeviceArray.from_host(dense)
+        ref = H.DeviceArray(n, n, dtype)
+        H.gemm(ta, "N", n, n, n, 1.0, dd, n, b, n, 0.0, ref, n, dtype=dtype)
+        bb = H.DeviceArray.from_host(bh)
+        H.trmm("L", uplo, ta, "N", n, n, 1.0, t, n, bb, n, dtype=dtype)
+        assert np.array_equal(bb.to_host(), ref.to_host()), (uplo, ta)
+
+
 def _rowmajor(rows, cols, seed, dtype):
     """A row-major rows x cols matrix (C order) with uniform[-1, 1] entries."""
     return np.ascontiguousarray(rand(rows, cols, seed, dtype))
