@@ -804,7 +804,6 @@ void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, 
     PanelSteps(a, lda, m, nb, row0, ipiv, w, s);
   }
   // The panel's interchanges on every other column of the matrix.
-  // The panel's interchanges on every other column of the matrix.
   const int pc0 = int((a - full) / lda);  // the panel's first column
   const int others = w.ncols - nb;
   if (others > 0)
