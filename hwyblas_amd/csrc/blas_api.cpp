@@ -117,12 +117,18 @@ blasint TrsmInfo(char side, char uplo, char transa, char diag, blasint m,
 // order the streams; host counters tell each thread which events have been
 // recorded. Every block runs the same k order as the whole product, so the
 // bits equal the unblocked call's. False (nothing done) when the product is
-// small or an operand is device memory.
+// small, an operand is device memory, or HWYBLAS_AMD_HOST_PIPELINE=0 was set
+// at the first call (the helper threads are then never created).
 template <typename T>
 bool GemmHostPipelined(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
                        int64_t lda, const T* b, int64_t ldb, T beta, T* c, int64_t ldc) {
   constexpr int64_t kMinBlock = 1024;
   if (m * n * k < (int64_t(1) << 36) || n < 4 * kMinBlock) return false;
+  static const bool enabled = [] {
+    const char* e = getenv("HWYBLAS_AMD_HOST_PIPELINE");
+    return !(e && strcmp(e, "0") == 0);
+  }();
+  if (!enabled) return false;
   if (IsDevicePointer(a) || IsDevicePointer(b) || IsDevicePointer(c)) return false;
   const int64_t nb = std::max<int64_t>(kMinBlock, ((n + 7) / 8 + 255) / 256 * 256);
   const int nblk = int((n + nb - 1) / nb);

@@ -20,8 +20,10 @@
  * through device memory on a per-thread stream; the call returns when the
  * result is back) or device pointers (hipMalloc or managed memory: the work
  * runs in place). They are synchronous, reentrant and callable from many
- * threads. The *_dev entry points take device pointers and a stream and are
- * asynchronous. A HIP failure (out of device memory, failed launch) prints a
+ * threads. Large sgemm_/dgemm_ calls on host operands pipeline their copies
+ * with two helper threads that end before the call returns
+ * (HWYBLAS_AMD_HOST_PIPELINE=0 turns that off). The *_dev entry points take
+ * device pointers and a stream and are asynchronous. A HIP failure (out of device memory, failed launch) prints a
  * message and aborts, as hwyBLAS does when scratch allocation fails
  * (src/common.cc:57-60).
  */
