@@ -23,9 +23,9 @@
  * threads. Large sgemm_/dgemm_ calls on host operands pipeline their copies
  * with two helper threads that end before the call returns
  * (HWYBLAS_AMD_HOST_PIPELINE=0 turns that off). The *_dev entry points take
- * device pointers and a stream and are asynchronous. A HIP failure (out of device memory, failed launch) prints a
- * message and aborts, as hwyBLAS does when scratch allocation fails
- * (src/common.cc:57-60).
+ * device pointers and a stream and are asynchronous. A HIP failure (out of
+ * device memory, failed launch) prints a message and aborts, as hwyBLAS does
+ * when scratch allocation fails (src/common.cc:57-60).
  */
 #ifndef HWYBLAS_AMD_H_
 #define HWYBLAS_AMD_H_
