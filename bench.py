@@ -454,6 +454,9 @@ DISTRIBUTIONS = {
 }
 
 
+_REFERENCE_CACHE = {}
+
+
 def distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, samples, nmod=0, steps=2,
                      method="int8"):
     """Emulated (moduli or BF16 slices from the operands, or `nmod` of them)
@@ -499,15 +502,18 @@ def distribution_leg(H, oracle, dist, n, dA, dB, dC, stream, samples, nmod=0, st
     emu_ms, _ = time_steps(H, stream, emu_step, steps, 1)
     used = [emu_step(report=True)]
     emu_vals = result_entries(H, dC, n, R, Cc, sr, sc)
-    if dist == "b":
-        ha, hb = oracle.fill_pow2(n, n, 11), oracle.fill_pow2(n, n, 12)
-    else:
-        ha, hb = oracle.fill_uniform(n, n, 1), oracle.fill_uniform(n, n, 2)
-        if dist == "c":
-            ha *= rs[:, None]
-            hb *= cs[None, :]
-    exp, bnd = reference_entries(oracle, ha, hb, n, R, Cc, sr, sc)
-    del ha, hb
+    key = (dist, n, samples)
+    if key not in _REFERENCE_CACHE:  # the sweeps reuse one distribution's reference
+        if dist == "b":
+            ha, hb = oracle.fill_pow2(n, n, 11), oracle.fill_pow2(n, n, 12)
+        else:
+            ha, hb = oracle.fill_uniform(n, n, 1), oracle.fill_uniform(n, n, 2)
+            if dist == "c":
+                ha *= rs[:, None]
+                hb *= cs[None, :]
+        _REFERENCE_CACHE[key] = reference_entries(oracle, ha, hb, n, R, Cc, sr, sc)
+        del ha, hb
+    exp, bnd = _REFERENCE_CACHE[key]
     e_nat, r_nat = error_summary(nat_vals, exp, bnd)
     e_emu, r_emu = error_summary(emu_vals, exp, bnd)
     count_key = "slices" if method == "bf16" else "moduli"
