@@ -56,15 +56,15 @@ void GemmByMode(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, cons
 template <typename T>
 void CopyIn(T* dst, const T* src, int64_t rows, int64_t cols, int64_t ld,
             hipStream_t s) {
-  HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(dst, rows * sizeof(T), src, ld * sizeof(T),
-                                 rows * sizeof(T), cols, hipMemcpyHostToDevice, s));
+  HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(dst, rows * sizeof(T), src, ld * sizeof(T), rows * sizeof(T),
+                                       cols, hipMemcpyHostToDevice, s));
 }
 
 template <typename T>
 void CopyOut(T* dst, int64_t ld, const T* src, int64_t rows, int64_t cols,
              hipStream_t s) {
-  HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(dst, ld * sizeof(T), src, rows * sizeof(T),
-                                 rows * sizeof(T), cols, hipMemcpyDeviceToHost, s));
+  HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(dst, ld * sizeof(T), src, rows * sizeof(T), rows * sizeof(T),
+                                       cols, hipMemcpyDeviceToHost, s));
 }
 
 // Returns the Fortran info of the first invalid GEMM argument, or 0

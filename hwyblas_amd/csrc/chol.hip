@@ -183,12 +183,13 @@ struct DevMatrix {
     dev = static_cast<T*>(Staging(slot, size_t(std::max<int64_t>(1, r * c)) * sizeof(T)));
     ld = r;
     HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(dev, size_t(r) * sizeof(T), x, size_t(ldx) * sizeof(T),
-                                   size_t(r) * sizeof(T), c, hipMemcpyHostToDevice, s));
+                                         size_t(r) * sizeof(T), c, hipMemcpyHostToDevice, s));
   }
   void CopyBack(hipStream_t s) {
     if (host)
-      HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(host, size_t(host_ld) * sizeof(T), dev, size_t(ld) * sizeof(T),
-                                     size_t(rows) * sizeof(T), cols, hipMemcpyDeviceToHost, s));
+      HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(host, size_t(host_ld) * sizeof(T), dev,
+                                           size_t(ld) * sizeof(T), size_t(rows) * sizeof(T), cols,
+                                           hipMemcpyDeviceToHost, s));
   }
 };
 

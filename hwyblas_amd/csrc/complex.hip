@@ -227,13 +227,13 @@ bool IsOneC(const R* z) {
 template <typename T>
 void CopyIn2D(T* dst, const T* src, int64_t rows, int64_t cols, int64_t ld, hipStream_t s) {
   HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(dst, rows * sizeof(T), src, ld * sizeof(T), rows * sizeof(T),
-                                 cols, hipMemcpyHostToDevice, s));
+                                       cols, hipMemcpyHostToDevice, s));
 }
 
 template <typename T>
 void CopyOut2D(T* dst, int64_t ld, const T* src, int64_t rows, int64_t cols, hipStream_t s) {
   HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(dst, ld * sizeof(T), src, rows * sizeof(T), rows * sizeof(T),
-                                 cols, hipMemcpyDeviceToHost, s));
+                                       cols, hipMemcpyDeviceToHost, s));
 }
 
 // Fortran entry: host or device operands; synchronizes.

@@ -182,7 +182,7 @@ void GemvHost(const Caller& who, const char* transp, const blasint* mp, const bl
     if (!IsDevicePointer(a)) {
       T* buf = static_cast<T*>(Staging(kSlotA, size_t(m) * n * sizeof(T)));
       HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(buf, size_t(m) * sizeof(T), a, size_t(lda) * sizeof(T),
-                                     size_t(m) * sizeof(T), n, hipMemcpyHostToDevice, s));
+                                           size_t(m) * sizeof(T), n, hipMemcpyHostToDevice, s));
       da = buf;
       lda_d = m;
     }
@@ -344,8 +344,9 @@ void ZgemvHost(const Caller& who, char opc, blasint m, blasint n, const R* alpha
     int64_t lda_d = lda;
     if (!IsDevicePointer(a)) {
       C2* buf = static_cast<C2*>(Staging(kSlotA, size_t(m) * n * sizeof(C2)));
-      HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(buf, size_t(m) * sizeof(C2), da, size_t(lda) * sizeof(C2),
-                                     size_t(m) * sizeof(C2), n, hipMemcpyHostToDevice, s));
+      HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(buf, size_t(m) * sizeof(C2), da,
+                                           size_t(lda) * sizeof(C2), size_t(m) * sizeof(C2), n,
+                                           hipMemcpyHostToDevice, s));
       da = buf;
       lda_d = m;
     }

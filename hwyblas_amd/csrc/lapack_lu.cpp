@@ -17,13 +17,13 @@ namespace {
 void CopyIn2D(void* dst, const void* src, int64_t rows_bytes, int64_t cols, int64_t ld_bytes,
               hipStream_t s) {
   HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(dst, rows_bytes, src, ld_bytes, rows_bytes, cols,
-                                 hipMemcpyHostToDevice, s));
+                                       hipMemcpyHostToDevice, s));
 }
 
 void CopyOut2D(void* dst, int64_t ld_bytes, const void* src, int64_t rows_bytes, int64_t cols,
                hipStream_t s) {
   HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(dst, ld_bytes, src, rows_bytes, rows_bytes, cols,
-                                 hipMemcpyDeviceToHost, s));
+                                       hipMemcpyDeviceToHost, s));
 }
 
 // A host-or-device matrix of doubles (rows x cols, ld); a host one is staged

@@ -21,13 +21,13 @@ namespace {
 template <typename T>
 void CopyIn(T* dst, const T* src, int64_t rows, int64_t cols, int64_t ld, hipStream_t s) {
   HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(dst, rows * sizeof(T), src, ld * sizeof(T), rows * sizeof(T),
-                                 cols, hipMemcpyHostToDevice, s));
+                                       cols, hipMemcpyHostToDevice, s));
 }
 
 template <typename T>
 void CopyOut(T* dst, int64_t ld, const T* src, int64_t rows, int64_t cols, hipStream_t s) {
   HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(dst, ld * sizeof(T), src, rows * sizeof(T), rows * sizeof(T),
-                                 cols, hipMemcpyDeviceToHost, s));
+                                       cols, hipMemcpyDeviceToHost, s));
 }
 
 // A host operand (rows x cols, ld) staged into `slot` at leading dimension
@@ -319,8 +319,9 @@ void TrmmHost(const Caller& who, const char* sidep, const char* uplop, const cha
     const T* tri = ExpandDevice<T>(ra, da, lda_d, lower,
                                    unit ? Expand::kUnitTriangular : Expand::kTriangular, s);
     T* x = static_cast<T*>(Staging(kSlotC, size_t(m) * n * sizeof(T)));
-    HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(x, size_t(m) * sizeof(T), ob.dev, size_t(ob.ld) * sizeof(T),
-                                   size_t(m) * sizeof(T), n, hipMemcpyDeviceToDevice, s));
+    HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(x, size_t(m) * sizeof(T), ob.dev,
+                                         size_t(ob.ld) * sizeof(T), size_t(m) * sizeof(T), n,
+                                         hipMemcpyDeviceToDevice, s));
     // op(A) is upper triangular when (uplo U, N) or (uplo L, T). Its zero
     // k-range is skipped per tile, as hwyBLAS's packing does
     // (src/gemm-inl.h:66-130): left, row i needs k >= i (upper) or k <= i;

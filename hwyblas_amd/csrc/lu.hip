@@ -1758,7 +1758,7 @@ void Refine(bool trans, int n, int nrhs, const double* a, int64_t lda, const dou
   std::vector<double> rows(n), xs(size_t(n) * nrhs);
   HWY_HIP_CHECK(hipMemcpyAsync(rows.data(), rowsum, size_t(n) * 8, hipMemcpyDeviceToHost, s));
   HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(xs.data(), size_t(n) * 8, x, size_t(ldx) * 8, size_t(n) * 8,
-                                 nrhs, hipMemcpyDeviceToHost, s));
+                                       nrhs, hipMemcpyDeviceToHost, s));
   HWY_HIP_CHECK(hipStreamSynchronize(s));
   const double anorm = *std::max_element(rows.begin(), rows.end());
   for (int j = 0; j < nrhs; ++j) {
@@ -1857,11 +1857,11 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
   }
   hipStream_t s = StreamOr(stream);
   HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(af, size_t(ldaf) * 8, a, size_t(lda) * 8, size_t(n) * 8, n,
-                                 hipMemcpyDeviceToDevice, s));
+                                       hipMemcpyDeviceToDevice, s));
   info = lu::Getrf(n, n, af, ldaf, ipiv, update, s);
   if (info != 0) return info;
   HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(x, size_t(ldx) * 8, b, size_t(ldb) * 8, size_t(n) * 8, nrhs,
-                                 hipMemcpyDeviceToDevice, s));
+                                       hipMemcpyDeviceToDevice, s));
   const lu::Perm perm(n, nrhs, ipiv, s);
   lu::GetrsPerm(false, n, nrhs, af, ldaf, perm, x, ldx, s);
   lu::Refine(false, n, nrhs, a, lda, af, ldaf, perm, b, ldb, x, ldx, itmax, ferr, berr, nbe,

@@ -367,13 +367,13 @@ HWYBLAS_AMD_API void dgeqrf_(const hwyblas_int* m, const hwyblas_int* n, double*
   if (host_a) {
     da = static_cast<double*>(Staging(kSlotA, size_t(M) * N * 8));
     HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(da, size_t(M) * 8, a, size_t(*lda) * 8, size_t(M) * 8, N,
-                                   hipMemcpyHostToDevice, s));
+                                         hipMemcpyHostToDevice, s));
   }
   if (host_tau) dtau = static_cast<double*>(Staging(kSlotB, size_t(K) * 8));
   qr::Geqrf(M, N, da, host_a ? M : *lda, dtau, s);
   if (host_a)
     HWY_HIP_CHECK(hwy_amd::Memcpy2DAsync(a, size_t(*lda) * 8, da, size_t(M) * 8, size_t(M) * 8, N,
-                                   hipMemcpyDeviceToHost, s));
+                                         hipMemcpyDeviceToHost, s));
   if (host_tau)
     HWY_HIP_CHECK(hipMemcpyAsync(tau, dtau, size_t(K) * 8, hipMemcpyDeviceToHost, s));
   HWY_HIP_CHECK(hipStreamSynchronize(s));
