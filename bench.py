@@ -763,6 +763,44 @@ def host_path_leg(H, sizes=(8192, 16384), reps=3):
     return out
 
 
+def qr_leg(H, sizes=(8192, 16384), reps=2):
+    """Householder QR (hwyblas_dgeqrf_dev) of a square uniform[-1, 1] matrix on
+    the device, best of `reps` factorizations of fresh copies; flops
+    (4/3) n^3 (LAPACK's count for m = n). Check: ||R||_F = ||A||_F (Q is
+    orthogonal), to 1e-10 relative."""
+    L = H.lib()
+    out = {}
+    for n in sizes:
+        A = H.DeviceArray(n, n)
+        A.fill_uniform(n, n, 41)
+        W = H.DeviceArray(n, n)
+        tau = H.DeviceArray(n, 1)
+        H.synchronize()
+        a_host = A.to_host()
+        fro_a = float(np.linalg.norm(a_host))
+        del a_host
+        times = []
+        for _ in range(reps):
+            L.hwyblas_amd_memcpy(W.ptr, A.ptr, n * n * 8, 3)
+            H.synchronize()
+            t0 = time.perf_counter()
+            info = H.geqrf_dev(W, tau)
+            H.synchronize()
+            times.append(time.perf_counter() - t0)
+            assert info == 0
+        r = np.triu(W.to_host())
+        rel = abs(float(np.linalg.norm(r)) - fro_a) / fro_a
+        t = min(times)
+        tf = 4.0 / 3.0 * n ** 3 / t / 1e12
+        out[str(n)] = {"ms": round(t * 1e3, 1), "tflops": round(tf, 2),
+                       "frac_fp64_peak": round(tf / PEAK_FP64, 3), "norm_check": rel}
+        log(f"dgeqrf n={n}: {t * 1e3:.1f} ms, {tf:.2f} TFLOP/s, | ||R||_F - ||A||_F | / ||A||_F "
+            f"= {rel:.1e}")
+        assert rel < 1e-10
+        del A, W, tau, r
+    return out
+
+
 def vendor_leg(n=16384, reps=5):
     """The vendor libraries on the same products, for reference: torch.mm
     (hipBLASLt / rocBLAS) in FP64 and FP32 and torch._int_mm (hipBLASLt INT8,
@@ -1163,6 +1201,8 @@ def main():
     ap.add_argument("--host-path", action="store_true",
                     help="dgemm_ on host (pageable) operands at 8192 and 16384 beside the "
                          "device-operand call, and the pageable copy rates")
+    ap.add_argument("--qr", action="store_true",
+                    help="Householder QR (dgeqrf) of square matrices at 8192 and 16384")
     ap.add_argument("--vendor", action="store_true",
                     help="the vendor libraries' FP64/FP32/INT8 GEMM at 16384 through torch, for "
                          "reference")
@@ -1196,6 +1236,11 @@ def main():
     if args.cpu_worker:
         cpu_baseline_worker()
         return
+    if args.vendor:
+        # The vendor libraries run through torch's own HIP runtime, which
+        # cannot share a process with this library's: measured first, alone.
+        print(json.dumps({"vendor": vendor_leg()}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): the hot path is "
               "single-GPU by design; run without --gpus or with --gpus 1.", file=sys.stderr)
@@ -1206,7 +1251,7 @@ def main():
     cpu_proc = None
     if not args.no_cpu and not (args.lu or args.plain_sweep or args.level3 or args.potrf or
                                 args.zgemm or args.batched_only or args.host_path or
-                                args.vendor):
+                                args.vendor or args.qr):
         import subprocess
         cpu_proc = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker"],
                                     stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
@@ -1245,8 +1290,8 @@ def main():
     if args.host_path:
         print(json.dumps({"host_path": host_path_leg(H)}), flush=True)
         return
-    if args.vendor:
-        print(json.dumps({"vendor": vendor_leg()}), flush=True)
+    if args.qr:
+        print(json.dumps({"dgeqrf": qr_leg(H)}), flush=True)
         return
     if args.zgemm:
         print(json.dumps({"zgemm": zgemm_leg(H)}), flush=True)

@@ -200,6 +200,20 @@ void GemmDeviceKRange<float>(int krange, bool ta, bool tb, int64_t m, int64_t n,
                              int64_t ldb, float beta, float* c, int64_t ldc,
                              hipStream_t stream);
 
+// Split k for skinny products (few 32 x 32 tiles of C, long k), FP64: the k
+// range in up to SplitKCount(m, n, k) chunks of whole 32-deep slices (as
+// many as `capacity` doubles of `partials` hold m x n each) computed at once
+// (one launch, one grid row per chunk), then C = alpha (the chunks summed in
+// ascending order) + beta C. The sum's rounding differs from a single
+// chain's (it stays within the GEMM bound); deterministic. One chunk runs
+// GemmDevice.
+constexpr int kMaxSplitK = 32;
+int SplitKCount(int64_t m, int64_t n, int64_t k);
+void GemmSplitKDevice(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                      const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                      double* c, int64_t ldc, double* partials, int64_t capacity,
+                      hipStream_t stream);
+
 // As GemmDevice<double>, doing the work only if *gate == 0 when the kernel
 // runs (a device-side decision made earlier on the stream; every workgroup
 // exits at once otherwise).

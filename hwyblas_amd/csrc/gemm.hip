@@ -13,7 +13,8 @@ namespace hwy_amd {
 void GemmLaunchF64(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
                    const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                    double* c, int64_t ldc, hipStream_t stream, int part = 0,
-                   int krange = 0, const int* gate = nullptr);
+                   int krange = 0, const int* gate = nullptr, int64_t ksplit = 0,
+                   int64_t c_split = 0);
 void GemmLaunchF32(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
                    const float* a, int64_t lda, const float* b, int64_t ldb, float beta,
                    float* c, int64_t ldc, hipStream_t stream, int part = 0,
@@ -81,6 +82,59 @@ void GemmDeviceGated<double>(const int* gate, bool ta, bool tb, int64_t m, int64
                              hipStream_t stream) {
   const int cfg = ChooseGemmConfig(8, ta, tb, m, n, k);
   GemmLaunchF64(cfg, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream, 0, 0, gate);
+}
+
+namespace {
+
+// C = alpha sum_s P_s + beta C (hwyBLAS's Store rules on the sum), the
+// partials P_s (m x n, leading dimension m, s < splits, P_s at p + s m n)
+// added in ascending s.
+__global__ void SplitSumKernel(const double* __restrict__ p, int splits, int64_t m, int64_t n,
+                               double alpha, double beta, double* __restrict__ c, int64_t ldc) {
+  const int64_t total = m * n, plane = m * n;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    double acc = p[t];
+    for (int s = 1; s < splits; ++s) acc += p[t + s * plane];
+    const int64_t i = t % m, j = t / m;
+    double* cij = c + i + j * ldc;
+    double v = acc * alpha;
+    if (beta != 0.0) v = beta == 1.0 ? v + *cij : __builtin_fma(*cij, beta, v);
+    *cij = v;
+  }
+}
+
+}  // namespace
+
+int SplitKCount(int64_t m, int64_t n, int64_t k) {
+  // Worth it when the 32 x 32 tiles alone cannot fill the device (about
+  // 4096 single-wave tiles in all) and each split keeps at least 512 of k.
+  const int64_t tiles = Tiles(m, n, 32, 32);
+  if (tiles >= 2048 || k < 1024) return 1;
+  const int64_t want = (4096 + tiles - 1) / tiles;
+  return int(std::max<int64_t>(1, std::min<int64_t>({want, k / 512, kMaxSplitK})));
+}
+
+void GemmSplitKDevice(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                      const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                      double* c, int64_t ldc, double* partials, int64_t capacity,
+                      hipStream_t stream) {
+  const int splits =
+      int(std::min<int64_t>(SplitKCount(m, n, k), m * n > 0 ? capacity / (m * n) : 1));
+  if (splits <= 1) {
+    GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+    return;
+  }
+  // Chunks of whole 32-deep slices; each partial is written with alpha 1, beta 0.
+  const int64_t chunk = ((k + splits - 1) / splits + 31) / 32 * 32;
+  const int used = int((k + chunk - 1) / chunk);
+  const int cfg = ChooseRegularGemmConfig(8, m, n, chunk);
+  GemmLaunchF64(cfg, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.0, partials, m, stream, 0, 0,
+                nullptr, chunk, m * n);
+  const int64_t total = m * n;
+  hipLaunchKernelGGL(SplitSumKernel, dim3(unsigned(std::min<int64_t>((total + 255) / 256, 4096))),
+                     dim3(256), 0, stream, partials, used, m, n, alpha, beta, c, ldc);
+  HWY_HIP_CHECK(hipGetLastError());
 }
 
 namespace {

@@ -81,6 +81,10 @@ struct GemmParams {
   // Non-null: the kernel does its work only if *gate == 0 (a decision made
   // on the device earlier on the stream).
   const int* gate = nullptr;
+  // Split k (register-staged kernel only): grid row y computes the k range
+  // [y ksplit, (y + 1) ksplit) into C + y c_split (alpha, beta and part
+  // apply to each; 0: no split).
+  int64_t ksplit = 0, c_split = 0;
 };
 
 constexpr int kXcds = 8;    // MI355X: 8 XCDs, each with its own L2
@@ -366,6 +370,17 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
   const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
   // A tile entirely on the other side of the diagonal (uniform exit).
   if ((p.part == 1 && m0 + BM - 1 < n0) || (p.part == 2 && m0 > n0 + BN - 1)) return;
+  if (p.ksplit) {
+    GemmParams<T> q = p;
+    const int64_t k0 = int64_t(blockIdx.y) * p.ksplit;
+    q.k = min(p.ksplit, p.k - k0);
+    q.a += TA ? k0 : k0 * p.lda;
+    q.b += TB ? k0 * p.ldb : k0;
+    q.c += int64_t(blockIdx.y) * p.c_split;
+    q.ksplit = 0;
+    MacroTile<T, TA, TB, BM, BN, BK, WAVES_M, WAVES_N>(q, m0, n0, lds);
+    return;
+  }
   MacroTile<T, TA, TB, BM, BN, BK, WAVES_M, WAVES_N>(p, m0, n0, lds);
 }
 
@@ -526,8 +541,9 @@ void Launch(const GemmParams<T>& prm, hipStream_t stream) {
     q.tri_grid = 1;
     blocks = int64_t(q.tiles_m) * (q.tiles_m + 1) / 2;
   }
-  hipLaunchKernelGGL((GemmKernel<T, TA, TB, BM, BN, BK, WGM, WGN, OCC>), dim3(unsigned(blocks)),
-                     dim3(64 * WGM * WGN), 0, stream, q);
+  const unsigned splits = q.ksplit ? unsigned((q.k + q.ksplit - 1) / q.ksplit) : 1u;
+  hipLaunchKernelGGL((GemmKernel<T, TA, TB, BM, BN, BK, WGM, WGN, OCC>),
+                     dim3(unsigned(blocks), splits), dim3(64 * WGM * WGN), 0, stream, q);
   HWY_HIP_CHECK(hipGetLastError());
 }
 

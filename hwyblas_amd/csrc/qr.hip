@@ -10,16 +10,18 @@
 //    factored column by column by workgroups of 256 rows each, two launches
 //    per column: QrReflectKernel (the norm from the previous launch's
 //    per-workgroup partial sums, beta and tau computed redundantly and
-//    identically by every workgroup, its rows of v scaled, and partial dot
-//    products v^T P(:, c') for the panel's later columns) and QrUpdateKernel
-//    (the dots summed in a fixed order, the rank-1 update of its rows, and
-//    the next column's norm partials);
+//    identically by every workgroup, its rows of v scaled, and one partial
+//    dot product v^T P(:, c') per workgroup for the panel's later columns)
+//    and QrUpdateKernel (the dots summed in workgroup order, the rank-1
+//    update of its rows, and the next column's norm partials); each thread
+//    issues all of its row's loads before the first sum;
 //  - the block reflector's T (I - V T V^T = H(j0) ... H(j0+63), xLARFT's
 //    forward columnwise T) from G = V^T V (an MFMA GEMM) and the recurrence
 //    T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i;
 //  - the trailing columns updated as A2 := (I - V T^T V^T) A2 = H^T A2
 //    (xLARFB, side L, trans T) by three MFMA GEMMs: W = V^T A2,
-//    W2 = T^T W, A2 -= V W2.
+//    W2 = T^T W, A2 -= V W2. V^T V and V^T A2 have 64 rows and the panel's
+//    height as k: they split k (GemmSplitKDevice) to fill the device.
 // Norms are computed scale-safely: each workgroup sums (x 2^-e)^2 with e the
 // exponent of its largest |x|, and the sums are combined at the largest e.
 #include <algorithm>
@@ -42,7 +44,7 @@ struct PanelArgs {
   double* tau;     // tau + j0
   double* beta;    // ib betas (the new diagonal), written by workgroup 0
   double* part;    // norm partials: [blocks][2] = {e, sum of (x 2^-e)^2}
-  double* dots;    // dot partials: [blocks * kWaves][kPanel]
+  double* dots;    // dot partials: [blocks][kPanel]
 };
 
 __device__ __forceinline__ double WaveSum(double v) {
@@ -163,11 +165,29 @@ __global__ void __launch_bounds__(kRows) QrReflectKernel(PanelArgs g, int c) {
     v = *pic * scal;
     *pic = v;
   }
-  // Partial dots with the later columns (row c contributes P(c, c') * 1).
-  for (int cc = c + 1; cc < g.ib; ++cc) {
-    const double pv = (i >= c && i < g.mp) ? g.p[i + int64_t(cc) * g.lda] : 0.0;
-    const double d = WaveSum(v * pv);
-    if (lane == 0) g.dots[int64_t(blockIdx.x * kWaves + wave) * kPanel + cc] = d;
+  // Partial dots with the later columns (row c contributes P(c, c') * 1):
+  // every load first, then one sum per column over the wave, then over the
+  // workgroup's waves (in wave order) through LDS.
+  __shared__ double wsum[kWaves][kPanel];
+  const bool mine = i >= c && i < g.mp;
+  double pv[kPanel];
+#pragma unroll
+  for (int cc = 0; cc < kPanel; ++cc)
+    pv[cc] = (mine && cc > c && cc < g.ib) ? g.p[i + int64_t(cc) * g.lda] : 0.0;
+#pragma unroll
+  for (int cc = 0; cc < kPanel; ++cc) {
+    if (cc > c && cc < g.ib) {
+      const double d = WaveSum(v * pv[cc]);
+      if (lane == 0) wsum[wave][cc] = d;
+    }
+  }
+  __syncthreads();
+  const int cc = threadIdx.x;
+  if (cc > c && cc < g.ib) {
+    double d = wsum[0][cc];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) d += wsum[w][cc];
+    g.dots[int64_t(blockIdx.x) * kPanel + cc] = d;
   }
 }
 
@@ -177,22 +197,40 @@ __global__ void __launch_bounds__(kRows) QrReflectKernel(PanelArgs g, int c) {
 __global__ void __launch_bounds__(kRows) QrUpdateKernel(PanelArgs g, int c) {
   __shared__ double w[kPanel];
   const int i = blockIdx.x * kRows + threadIdx.x;
-  const int nparts = gridDim.x * kWaves;
+  const int nparts = gridDim.x;
+  const bool mine = i >= c && i < g.mp;
+  // This row's entries of the later columns, loaded while w is summed.
+  double pv[kPanel];
+#pragma unroll
+  for (int cc = 0; cc < kPanel; ++cc)
+    pv[cc] = (mine && cc > c && cc < g.ib) ? g.p[i + int64_t(cc) * g.lda] : 0.0;
+  const double v = !mine ? 0.0 : (i == c ? 1.0 : g.p[i + int64_t(c) * g.lda]);
   const double tau = g.tau[c];
+  // w(c') = the workgroups' dot partials in workgroup order, loads in
+  // batches of 8.
   for (int cc = c + 1 + threadIdx.x; cc < g.ib; cc += blockDim.x) {
     double s = 0.0;
-    for (int q = 0; q < nparts; ++q) s += g.dots[int64_t(q) * kPanel + cc];
+    int q = 0;
+    for (; q + 8 <= nparts; q += 8) {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[u] = g.dots[int64_t(q + u) * kPanel + cc];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += t[u];
+    }
+    for (; q < nparts; ++q) s += g.dots[int64_t(q) * kPanel + cc];
     w[cc] = s;
   }
   __syncthreads();
   double next = 0.0;  // this row's entry of column c + 1 after the update
-  if (i >= c && i < g.mp) {
-    const double v = i == c ? 1.0 : g.p[i + int64_t(c) * g.lda];
-    for (int cc = c + 1; cc < g.ib; ++cc) {
-      double* pij = g.p + i + int64_t(cc) * g.lda;
-      const double nv = *pij - tau * v * w[cc];
-      *pij = nv;
-      if (cc == c + 1) next = nv;
+  if (mine) {
+#pragma unroll
+    for (int cc = 0; cc < kPanel; ++cc) {
+      if (cc > c && cc < g.ib) {
+        const double nv = pv[cc] - tau * v * w[cc];
+        g.p[i + int64_t(cc) * g.lda] = nv;
+        if (cc == c + 1) next = nv;
+      }
     }
     if (i == c) g.p[i + int64_t(c) * g.lda] = g.beta[c];
   }
@@ -239,14 +277,18 @@ struct Work {
   double* w;      // kPanel x n
   double* w2;     // kPanel x n
   double* part;   // [blocks][2]
-  double* dots;   // [blocks * kWaves][kPanel]
+  double* dots;   // [blocks][kPanel]
   double* beta;   // kPanel
+  double* split;  // split-k partials of V^T V and V^T A2
+  int64_t split_cap;
 };
+
+int64_t SplitCap(int64_t n) { return 8 * int64_t(kPanel) * std::max<int64_t>(n, 4096); }
 
 size_t WorkBytes(int64_t m, int64_t n) {
   const int64_t blocks = (m + kRows - 1) / kRows;
   return 8 * (size_t(m) * kPanel + 2 * kPanel * kPanel + 2 * size_t(kPanel) * n +
-              2 * blocks + size_t(blocks) * kWaves * kPanel + kPanel) + 256;
+              2 * blocks + size_t(blocks) * kPanel + kPanel + size_t(SplitCap(n))) + 256;
 }
 
 Work Carve(void* base, int64_t m, int64_t n) {
@@ -266,8 +308,11 @@ Work Carve(void* base, int64_t m, int64_t n) {
   w.part = p;
   p += 2 * blocks;
   w.dots = p;
-  p += blocks * kWaves * kPanel;
+  p += blocks * kPanel;
   w.beta = p;
+  p += kPanel;
+  w.split = p;
+  w.split_cap = SplitCap(n);
   return w;
 }
 
@@ -295,11 +340,13 @@ void Geqrf(int64_t m, int64_t n, double* a, int64_t lda, double* tau, hipStream_
     const int64_t vt = int64_t(mp) * ib;
     hipLaunchKernelGGL(ExplicitVKernel, dim3(unsigned(std::min<int64_t>((vt + 255) / 256, 2048))),
                        dim3(256), 0, s, g.p, lda, mp, ib, w.v);
-    GemmDevice<double>(true, false, ib, ib, mp, 1.0, w.v, mp, w.v, mp, 0.0, w.gm, ib, s);
+    GemmSplitKDevice(true, false, ib, ib, mp, 1.0, w.v, mp, w.v, mp, 0.0, w.gm, ib, w.split,
+                     w.split_cap, s);
     hipLaunchKernelGGL(LarftKernel, dim3(1), dim3(kPanel), 0, s, w.gm, ib, g.tau, w.t);
     HWY_HIP_CHECK(hipGetLastError());
     double* a2 = a + j0 + (j0 + ib) * lda;
-    GemmDevice<double>(true, false, ib, n2, mp, 1.0, w.v, mp, a2, lda, 0.0, w.w, ib, s);
+    GemmSplitKDevice(true, false, ib, n2, mp, 1.0, w.v, mp, a2, lda, 0.0, w.w, ib, w.split,
+                     w.split_cap, s);
     GemmDevice<double>(true, false, ib, n2, ib, 1.0, w.t, kPanel, w.w, ib, 0.0, w.w2, ib, s);
     GemmDevice<double>(false, false, mp, n2, ib, -1.0, w.v, mp, w.w2, ib, 1.0, a2, lda, s);
   }
