@@ -89,17 +89,23 @@ __device__ void NormPartial(double x, double* part_out) {
   }
 }
 
-// ||x||_2 from the partials of `blocks` workgroups, in a fixed order.
-__device__ double NormOf(const double* part, int blocks) {
+// ||x||_2 from the partials of `blocks` workgroups, computed by one whole
+// wave (every lane gets it): lane l takes partials l, l + 64, ..., then the
+// largest exponent and the scaled sum are reduced over the wave. Every
+// workgroup computes it from the same partials in the same order, so all
+// of them get the same bits.
+__device__ double NormOfWave(const double* part, int blocks) {
+  const int lane = threadIdx.x & 63;
   int emax = -100000;
-  for (int b = 0; b < blocks; ++b) emax = max(emax, int(part[2 * b]));
+  for (int b = lane; b < blocks; b += 64) emax = max(emax, int(part[2 * b]));
+  emax = int(WaveMax(double(emax)));
   if (emax == -100000) return 0.0;
   double t = 0.0;
-  for (int b = 0; b < blocks; ++b) {
+  for (int b = lane; b < blocks; b += 64) {
     const int e = int(part[2 * b]);
-    if (e == -100000) continue;
-    t += ldexp(part[2 * b + 1], 2 * (e - emax));
+    if (e != -100000) t += ldexp(part[2 * b + 1], 2 * (e - emax));
   }
+  t = WaveSum(t);
   return ldexp(sqrt(t), emax);
 }
 
@@ -123,8 +129,10 @@ __global__ void __launch_bounds__(kRows) QrReflectKernel(PanelArgs g, int c) {
   __shared__ double coef[2];  // scal, and the workgroup-0 copy of tau
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = blockIdx.x * kRows + threadIdx.x;
+  double xnorm_w = 0.0;
+  if (threadIdx.x < 64) xnorm_w = NormOfWave(g.part, gridDim.x);
   if (threadIdx.x == 0) {
-    const double xnorm = NormOf(g.part, gridDim.x);
+    const double xnorm = xnorm_w;
     const double alpha = g.p[c + int64_t(c) * g.lda];
     double tau = 0.0, scal = 1.0, beta = alpha;
     if (xnorm != 0.0) {
@@ -250,24 +258,36 @@ __global__ void ExplicitVKernel(const double* p, int64_t lda, int mp, int ib, do
 
 // T (ib x ib, upper, leading dimension kPanel) from G = V^T V and tau:
 // T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i; zeros below.
-// One workgroup of kPanel threads, thread r owning row r.
+// One workgroup of kPanel threads, thread r owning row r of T in LDS (G and
+// tau loaded into LDS first, every load in flight at once).
 __global__ void __launch_bounds__(kPanel) LarftKernel(const double* gm, int ib,
                                                        const double* tau, double* t) {
-  __shared__ double col[kPanel];
+  __shared__ double sg[kPanel][kPanel + 1];  // sg[q][i] = G(q, i)
+  __shared__ double st[kPanel][kPanel + 1];  // st[r][q] = T(r, q)
+  __shared__ double stau[kPanel];
   const int r = threadIdx.x;
-  for (int j = 0; j < kPanel; ++j) t[r + j * kPanel] = 0.0;
+  double gcol[kPanel];
+#pragma unroll
+  for (int i = 0; i < kPanel; ++i) gcol[i] = (r < ib && i < ib) ? gm[r + int64_t(i) * ib] : 0.0;
+#pragma unroll
+  for (int i = 0; i < kPanel; ++i) {
+    sg[r][i] = gcol[i];
+    st[r][i] = 0.0;
+  }
+  stau[r] = r < ib ? tau[r] : 0.0;
   __syncthreads();
   for (int i = 0; i < ib; ++i) {
-    // col = T(0:i, 0:i) G(0:i, i), each row r < i.
-    double s = 0.0;
-    if (r < i)
-      for (int q = r; q < i; ++q) s += t[r + q * kPanel] * gm[q + int64_t(i) * ib];
-    col[r] = s;
-    __syncthreads();
-    if (r < i) t[r + i * kPanel] = -tau[i] * col[r];
-    if (r == i) t[r + i * kPanel] = tau[i];
+    // T(r, i) = -tau_i sum_{q = r}^{i - 1} T(r, q) G(q, i), ascending q.
+    if (r < i) {
+      double s = 0.0;
+      for (int q = r; q < i; ++q) s += st[r][q] * sg[q][i];
+      st[r][i] = -stau[i] * s;
+    }
+    if (r == i) st[r][i] = stau[i];
     __syncthreads();
   }
+#pragma unroll
+  for (int j = 0; j < kPanel; ++j) t[r + j * kPanel] = st[r][j];
 }
 
 struct Work {
