@@ -10,6 +10,8 @@
 // FMAs in ascending q (the dot form of hwyBLAS's base kernel,
 // src/trsm-inl.h:107-174, reciprocal at :69). Only the referenced triangle
 // is read; a unit diagonal is never read.
+#include <stdlib.h>
+
 #include <algorithm>
 
 #include "common.h"
@@ -124,6 +126,238 @@ __global__ void __launch_bounds__(64 * kNarrowWaves)
   }
 }
 
+// The recursion's leaves fused: a diagonal block of order 128 or 256 is
+// solved in one launch that runs the recursion's own steps on kNarrowRhs
+// right-hand sides per workgroup, the block of B kept in LDS: each solve of
+// a 64-row diagonal block as DiagNarrowKernel does it, and each
+// off-diagonal update as the GEMM kernel does it (v_mfma 16x16x4 steps over
+// k ascending from a zero accumulator, the same operand roles, then
+// (-acc) + B), so the bits are the unfused recursion's.
+constexpr int kFusedMax = 256;
+constexpr int kMaxFusedSteps = 8;
+
+struct FusedStep {
+  int update;  // 0: solve rows [r0, r0 + len); 1: rows [r0, r0 + len) -= T(., [k0, k0 + klen)) X
+  int r0, len, k0, klen;
+};
+struct FusedPlan {
+  int nsteps;
+  FusedStep s[kMaxFusedSteps];
+};
+
+// The steps Recurse takes on a diagonal block of order r at solve index r0;
+// `lower`: the solve runs in ascending index order (a lower triangle on the
+// left side, an upper one on the right side).
+constexpr void PlanFused(bool lower, int r0, int r, FusedPlan& plan) {
+  if (r <= kNB) {
+    plan.s[plan.nsteps++] = FusedStep{0, r0, r, 0, 0};
+    return;
+  }
+  const int half = (r / 2) / kNB * kNB;
+  const int r1 = half > kNB ? half : kNB, r2 = r - r1;
+  if (lower) {
+    PlanFused(lower, r0, r1, plan);
+    plan.s[plan.nsteps++] = FusedStep{1, r0 + r1, r2, r0, r1};
+    PlanFused(lower, r0 + r1, r2, plan);
+  } else {
+    PlanFused(lower, r0 + r1, r2, plan);
+    plan.s[plan.nsteps++] = FusedStep{1, r0, r1, r0 + r1, r2};
+    PlanFused(lower, r0, r1, plan);
+  }
+}
+constexpr FusedPlan MakeFusedPlan(bool lower, int r) {
+  FusedPlan plan{};
+  PlanFused(lower, 0, r, plan);
+  return plan;
+}
+// Indexed by 2 (order == 256) + lower.
+__constant__ FusedPlan kFusedPlans[4] = {MakeFusedPlan(false, 128), MakeFusedPlan(true, 128),
+                                         MakeFusedPlan(false, 256), MakeFusedPlan(true, 256)};
+static_assert(MakeFusedPlan(true, 256).nsteps == 7 && MakeFusedPlan(false, 128).nsteps == 3,
+              "fused plans");
+
+template <typename T, bool LEFT>
+__global__ void __launch_bounds__(64 * kNarrowWaves)
+    FusedBlockKernel(trsm_chunk::TrsmDesc<T> d, int R) {
+  using M = gemm_detail::Mfma<T>;
+  using Acc = typename M::Acc;
+  // sb[e][c]: solve index e (a row of B for the left side, a column for the
+  // right side) x this workgroup's right-hand side c.
+  __shared__ T sb[kFusedMax][kNarrowRhs + 1];
+  __shared__ T st[kNB][kNB + 1];
+  __shared__ T sinv[kNB];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int c0 = int(blockIdx.x) * kNarrowRhs;
+  const int nvc = min(kNarrowRhs, (LEFT ? d.n : d.m) - c0);
+  // Element (solve index e, right-hand side c) of B; the thread order of the
+  // copies follows B's contiguous direction.
+  auto at = [&](int e, int c) -> T* {
+    return LEFT ? d.b + e + int64_t(c0 + c) * d.ldb : d.b + (c0 + c) + int64_t(e) * d.ldb;
+  };
+  auto split = [&](int idx, int& e, int& c) {
+    if (LEFT) {
+      e = idx % kFusedMax;
+      c = idx / kFusedMax;
+    } else {
+      c = idx % kNarrowRhs;
+      e = idx / kNarrowRhs;
+    }
+  };
+  constexpr int NT = 64 * kNarrowWaves;
+  {
+    constexpr int kPer = kFusedMax * kNarrowRhs / NT;
+    T v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      int e, c;
+      split(int(threadIdx.x) + u * NT, e, c);
+      v[u] = (e < R && c < nvc) ? *at(e, c) : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      int e, c;
+      split(int(threadIdx.x) + u * NT, e, c);
+      sb[e][c] = v[u];
+    }
+  }
+  __syncthreads();
+  const bool asc = d.lower_solve;
+  const FusedPlan& plan = kFusedPlans[(R == 256 ? 2 : 0) + (asc ? 1 : 0)];
+  for (int si = 0; si < plan.nsteps; ++si) {
+    const FusedStep sp = plan.s[si];
+    if (!sp.update) {
+      // A diagonal block: rows r0 .. r0 + len - 1, solve order e.
+      const int r = sp.len;
+      auto phys = [&](int e) { return sp.r0 + (asc ? e : r - 1 - e); };
+      constexpr int kCols = kNB / kNarrowWaves;
+      T trow[kCols];
+#pragma unroll
+      for (int u = 0; u < kCols; ++u) {
+        const int q = wave + kNarrowWaves * u;
+        trow[u] = (lane < r && q < lane) ? trsm_chunk::SolveElem(d, phys(lane), phys(q)) : T(0);
+      }
+      if (wave == 0) {
+        const T diag =
+            (lane < r && !d.unit) ? trsm_chunk::SolveElem(d, phys(lane), phys(lane)) : T(1);
+        sinv[lane] = T(1) / diag;
+      }
+#pragma unroll
+      for (int u = 0; u < kCols; ++u) st[lane][wave + kNarrowWaves * u] = trow[u];
+      __syncthreads();
+      const T myinv = sinv[lane];
+      const int prow = lane < r ? phys(lane) : 0;
+      T x[kWaveRhs];
+#pragma unroll
+      for (int c = 0; c < kWaveRhs; ++c) x[c] = lane < r ? sb[prow][wave * kWaveRhs + c] : T(0);
+      // An opaque copy of the lane index: the 2 x 64 lane masks below are
+      // then formed where they are used, not hoisted out of the step loop
+      // into (spilled) scalar registers.
+      int ln = lane;
+      asm volatile("" : "+v"(ln));
+      // The lane's row of the triangle read kChunk columns ahead of use, the
+      // compiler fence keeping the later chunks' reads (and registers) out
+      // of the current chunk's steps.
+      constexpr int kChunk = 16;
+      T tc[kChunk], tn[kChunk];
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) tc[j] = st[lane][j];
+#pragma unroll
+      for (int ib = 0; ib < kNB; ib += kChunk) {
+        if (ib + kChunk < kNB) {
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j) tn[j] = st[lane][ib + kChunk + j];
+        }
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int i = ib + j;
+#pragma unroll
+          for (int c = 0; c < kWaveRhs; ++c) {
+            if (ln == i) x[c] = x[c] * myinv;
+            const T xi = Broadcast(x[c], i);
+            if (ln > i) x[c] = __builtin_fma(-tc[j], xi, x[c]);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) tc[j] = tn[j];
+      }
+      if (lane < r) {
+#pragma unroll
+        for (int c = 0; c < kWaveRhs; ++c) sb[prow][wave * kWaveRhs + c] = x[c];
+      }
+    } else {
+      // Solve indices r0 .. r0 + len - 1 less the product of the solved
+      // k0 .. k0 + klen - 1 with the triangle, as the GEMM kernel forms it:
+      // left B(rows) -= T(rows, ks) X(ks, rhs), right B(rhs, cols) -= X(rhs,
+      // ks) T(ks, cols); 16 solve indices per block, block rb on wave
+      // rb % kNarrowWaves, all 16 right-hand sides at once.
+      const int nrb = sp.len / 16;
+      for (int rb = wave; rb < nrb; rb += kNarrowWaves) {
+        const int i0 = sp.r0 + rb * 16;
+        Acc acc = Acc{};
+#pragma unroll 8
+        for (int kk = sp.k0; kk < sp.k0 + sp.klen; kk += 4) {
+          const T t = trsm_chunk::SolveElem(d, i0 + li, kk + lk);
+          const T x = sb[kk + lk][li];
+          acc = LEFT ? M::Run(t, x, acc) : M::Run(x, t, acc);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          // Left: output row i0 + Row(lk, e) (solve index), column li (rhs);
+          // right: output row Row(lk, e) (rhs), column i0 + li (solve index).
+          const int eo = LEFT ? i0 + M::Row(lk, e) : i0 + li;
+          const int co = LEFT ? li : M::Row(lk, e);
+          T v = acc[e] * T(-1);
+          v = v + sb[eo][co];
+          sb[eo][co] = v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll 4
+  for (int idx = int(threadIdx.x); idx < R * kNarrowRhs; idx += NT) {
+    const int e = LEFT ? idx % R : idx / kNarrowRhs, c = LEFT ? idx / R : idx % kNarrowRhs;
+    if (c < nvc) *at(e, c) = sb[e][c];
+  }
+}
+
+// Whether a diagonal block of order r is solved by FusedBlockKernel (off
+// with HWYBLAS_AMD_TRSM_FUSED=0).
+bool FusedBlock(int64_t r) {
+  static const bool enabled = [] {
+    const char* e = getenv("HWYBLAS_AMD_TRSM_FUSED");
+    return !(e && e[0] == '0');
+  }();
+  return enabled && (r == 128 || r == 256);
+}
+
+template <typename T>
+void FusedSolve(bool left, const TriView<T>& t, int r, T* b, int64_t ldb, int64_t other,
+                hipStream_t stream) {
+  trsm_chunk::TrsmDesc<T> d;
+  d.a = t.data;
+  d.rs = t.rs;
+  d.cs = t.cs;
+  d.b = b;
+  d.ldb = ldb;
+  d.m = left ? r : int(other);
+  d.n = left ? int(other) : r;
+  d.left = left;
+  d.lower_solve = left ? t.lower : !t.lower;
+  d.unit = t.unit;
+  d.alpha = T(1);
+  const dim3 grid(unsigned((other + kNarrowRhs - 1) / kNarrowRhs));
+  if (left)
+    hipLaunchKernelGGL((FusedBlockKernel<T, true>), grid, dim3(64 * kNarrowWaves), 0, stream, d,
+                       r);
+  else
+    hipLaunchKernelGGL((FusedBlockKernel<T, false>), grid, dim3(64 * kNarrowWaves), 0, stream, d,
+                       r);
+  HWY_HIP_CHECK(hipGetLastError());
+}
+
 template <typename T>
 void DiagSolve(bool left, const TriView<T>& t, int r, T* b, int64_t ldb,
                int64_t other, hipStream_t stream) {
@@ -166,6 +400,10 @@ void Recurse(bool left, const TriView<T>& t, int64_t r, int64_t other, T* b,
              int64_t ldb, hipStream_t stream) {
   if (r <= kNB) {
     DiagSolve(left, t, int(r), b, ldb, other, stream);
+    return;
+  }
+  if (FusedBlock(r)) {
+    FusedSolve(left, t, int(r), b, ldb, other, stream);
     return;
   }
   const int64_t r1 = std::max<int64_t>(kNB, (r / 2) / kNB * kNB);

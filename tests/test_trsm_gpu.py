@@ -145,3 +145,57 @@ def test_large_device_sampled(H, dtype, side, uplo, ta):
     which = rng.integers(0, n, 16)
     r = trsm_residual_ratio(side, uplo, ta, "N", n, n, 1.0, a, n, b, n, x, which)
     assert r <= 1.0, r
+
+
+FUSED_CASES = [(side, m, n, uplo, ta, diag, dt)
+               for side in "LR"
+               for (m, n) in ((128, 17), (256, 40), (300, 33), (1000, 1), (2048, 300))
+               for uplo, ta, diag in (("L", "N", "N"), ("U", "N", "U"), ("L", "T", "N"),
+                                      ("U", "T", "N"))
+               for dt in ("f8", "f4")]
+
+
+def _fused_shape(side, m, n):
+    # The order of the triangle is the first size; the right side swaps them.
+    return (m, n) if side == "L" else (n, m)
+
+
+def solve_fused_cases(H):
+    """Solves whose recursion reaches diagonal blocks of order 128 and 256
+    (the fused block kernel's cases), both sides."""
+    out = []
+    for i, (side, m, n, uplo, ta, diag, dt) in enumerate(FUSED_CASES):
+        mm, nn = _fused_shape(side, m, n)
+        a, lda, b, ldb = trsm_operands(side, uplo, diag, mm, nn, np.dtype(dt).type, 7000 + 3 * i)
+        out.append(run_trsm(H, side, uplo, ta, diag, mm, nn, 0.75, a, lda, b, ldb))
+    return out
+
+
+def test_fused_block_solve_equals_unfused_recursion(H, tmp_path):
+    """A diagonal block of order 128 or 256 is solved by one launch that
+    runs the recursion's own steps; its bits equal the unfused
+    recursion's (small solves and GEMM updates launched one by one, taken
+    with HWYBLAS_AMD_TRSM_FUSED=0 in a child process)."""
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    out = tmp_path / "unfused.npz"
+    code = ("import sys, numpy as np\n"
+            f"sys.path[:0] = [{root!r}, {here!r}]\n"
+            "import hwyblas_amd, test_trsm_gpu as T\n"
+            f"np.savez({str(out)!r}, *T.solve_fused_cases(hwyblas_amd))\n")
+    env = dict(os.environ, HWYBLAS_AMD_TRSM_FUSED="0")
+    subprocess.run([sys.executable, "-c", code], env=env, cwd=root, check=True, timeout=240)
+    got = solve_fused_cases(H)
+    with np.load(out) as ref:
+        for i, case in enumerate(FUSED_CASES):
+            want = ref[f"arr_{i}"]
+            assert got[i].tobytes() == want.tobytes(), case
+    # And each is a solution within the residual bound.
+    for i, (side, m, n, uplo, ta, diag, dt) in enumerate(FUSED_CASES):
+        mm, nn = _fused_shape(side, m, n)
+        a, lda, b, ldb = trsm_operands(side, uplo, diag, mm, nn, np.dtype(dt).type, 7000 + 3 * i)
+        r = trsm_residual_ratio(side, uplo, ta, diag, mm, nn, 0.75, a, lda, b, ldb, got[i])
+        assert r <= 1.0, (side, m, n, uplo, ta, diag, dt, r)
