@@ -46,6 +46,24 @@ int ChooseRegularGemmConfig(int dtype_bytes, int64_t m, int64_t n, int64_t k) {
   return k > 16 ? 11 : 7;
 }
 
+// FP64: whether the DMA-fed 128 x 128 kernel beats the plain 64 x 64 one,
+// by a per-CU time model of both fitted on MI355X (256 CUs; NN products of
+// 2048^3, 3072^3, 1024 x 4096 x 4096, 2048 x 3072 x 3072, 1536 x 2048 x 2048,
+// 4096 x 1024 x 1024 and k = 128 .. 256 shapes): the busiest CU holds j
+// tiles; a pair of DMA tiles shares a CU in 256.5 ns per unit of k and a
+// single one takes 154 ns, plus 20 us of pipeline fill; plain 64 x 64 tiles
+// take 46 ns each per unit of k, plus 8 us. A tile count just past a
+// multiple of the CU count is where the finer tiles win (3072^3: 576 DMA
+// tiles, 3 on the busiest CU).
+bool DmaF64Faster(int64_t m, int64_t n, int64_t k) {
+  constexpr int64_t kCus = 256;
+  const int64_t j15 = (Tiles(m, n, 128, 128) + kCus - 1) / kCus;
+  const int64_t j0 = (Tiles(m, n, 64, 64) + kCus - 1) / kCus;
+  const double t15 = 20e3 + double(k) * (double(j15 / 2) * 256.5 + double(j15 % 2) * 154.0);
+  const double t0 = 8e3 + double(k) * 46.0 * double(j0);
+  return t15 < t0;
+}
+
 // Large products take the DMA-fed kernel (MI355X, m = n = k = 16384, NN):
 // FP64 on 128 x 128 x 16 tiles through a two-slice LDS ring, 86 % of the
 // FP64 MFMA peak against 80 % for the best plain configuration; FP32 on
@@ -54,7 +72,7 @@ int ChooseRegularGemmConfig(int dtype_bytes, int64_t m, int64_t n, int64_t k) {
 int ChooseGemmConfig(int dtype_bytes, bool ta, bool tb, int64_t m, int64_t n, int64_t k) {
   (void)ta;
   (void)tb;
-  if (dtype_bytes == 8 && Tiles(m, n, 128, 128) >= 512) return 15;
+  if (dtype_bytes == 8 && Tiles(m, n, 128, 128) >= 128) return DmaF64Faster(m, n, k) ? 15 : 0;
   if (dtype_bytes == 4 && Tiles(m, n, 256, 256) >= 256) return 16;
   return ChooseRegularGemmConfig(dtype_bytes, m, n, k);
 }

@@ -362,6 +362,13 @@ def test_default_config_takes_the_dma_kernel_for_large_products(H):
         assert L.hwyblas_gemm_config_stages(cfg) > 0
         small = L.hwyblas_gemm_default_config(dtype_bytes, b"N", b"N", 256, 256, 256)
         assert L.hwyblas_gemm_config_stages(small) == 0
+    # FP64 by the per-CU time model: 2048^3 (256 tiles, one per CU) takes the
+    # DMA kernel; 3072^3 (576 tiles, three on the busiest CU) and a k = 256
+    # update of 2048^2 take the finer plain tiles.
+    stages = lambda *mnk: L.hwyblas_gemm_config_stages(
+        L.hwyblas_gemm_default_config(8, b"N", b"N", *mnk))
+    assert stages(2048, 2048, 2048) > 0 and stages(512, 8192, 4096) > 0
+    assert stages(3072, 3072, 3072) == 0 and stages(2048, 2048, 256) == 0
 
 
 def test_device_pow2_fill_and_scaling_match_oracle(H):
