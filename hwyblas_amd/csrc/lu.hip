@@ -41,6 +41,7 @@
 
 #include "common.h"
 #include "gemm_kernel.h"
+#include "grid_sync.h"
 
 namespace hwy_amd {
 namespace lu {
@@ -392,7 +393,6 @@ constexpr int kPersistRows = 256;               // rows per workgroup
 constexpr int kPersistRpt = kPersistRows / 128;  // rows per thread (two row halves)
 constexpr int kPersistMaxGroups = 128;          // records loaded through LDS
 constexpr int kRecord = kPanel + 2;             // |value|, row, 64 values
-constexpr unsigned kSpinLimit = 1u << 22;       // polls (a few seconds at most)
 
 struct PersistArgs {
   double* p;
@@ -407,35 +407,6 @@ struct PersistArgs {
   double* linv;     // the inverse of the panel's unit lower triangle (kPanel x kPanel)
   unsigned bar_base;
 };
-
-// Arrive, then wait until `target` arrivals in all. Every thread's record
-// stores (agent-scope atomics, written through to the coherent level) have
-// completed (vmcnt) before thread 0 arrives; the records are read after the
-// wait with agent-scope atomic loads. False when the wait timed out (here or
-// in another workgroup).
-__device__ bool GridSync(unsigned* bar, unsigned target, unsigned* failed) {
-  __shared__ int ok;
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    int good = 1;
-    unsigned polls = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-          ++polls > kSpinLimit) {
-        __hip_atomic_exchange(failed, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        good = 0;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    ok = good;
-  }
-  asm volatile("" ::: "memory");
-  __syncthreads();
-  return ok != 0;
-}
 
 // Wavefront reductions of the device library (ockl).
 extern "C" __device__ unsigned long long __ockl_wfred_max_u64(unsigned long long);
@@ -452,13 +423,6 @@ __device__ __forceinline__ void WaveBest(double& v, int& row) {
   const unsigned long long kmax = __ockl_wfred_max_u64(key);
   row = __ockl_wfred_min_i32(key == kmax ? row : INT_MAX);
   v = kmax == 0ull ? -1.0 : __longlong_as_double(static_cast<long long>(kmax - 1ull));
-}
-
-__device__ __forceinline__ void PutRecord(double* x, double v) {
-  __hip_atomic_store(x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double GetRecord(const double* x) {
-  return __hip_atomic_load(const_cast<double*>(x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // x[t] for a run-time t < kGroupCols without indexing registers.
@@ -789,17 +753,28 @@ void Panel(double* a, int64_t lda, int m, int nb, int row0, int* ipiv, Work& w, 
   const int mn = std::min(m, nb);
   double* full = w.base + row0;
   const int groups = (m + kPersistRows - 1) / kPersistRows;
-  const bool persistent = groups <= w.persist_max_groups;
+  bool persistent = groups <= w.persist_max_groups;
   if (persistent) {
     PersistArgs g{a,           lda,    m,     nb,     row0,
                   ipiv,        w.info, sfmin, w.prec, w.sync,
                   w.sync + 64, w.linv + int64_t(row0 / kPanel) * kPanel * kPanel,
                   w.arrivals};
-    w.arrivals += unsigned(mn) * unsigned(groups);  // one arrival per workgroup and column
     void* args[] = {&g};
-    HWY_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(PanelPersistentKernel),
-                                             dim3(groups), dim3(kPersistThreads), args, 0, s));
-  } else {
+    // A cooperative launch the runtime refuses (another process holding the
+    // device's cooperative queue, for one) leaves this panel and the later
+    // ones to the column-step launches.
+    const hipError_t e =
+        hipLaunchCooperativeKernel(reinterpret_cast<const void*>(PanelPersistentKernel),
+                                   dim3(groups), dim3(kPersistThreads), args, 0, s);
+    if (e == hipSuccess) {
+      w.arrivals += unsigned(mn) * unsigned(groups);  // one arrival per workgroup and column
+    } else {
+      (void)hipGetLastError();
+      w.persist_max_groups = 0;
+      persistent = false;
+    }
+  }
+  if (!persistent) {
     // Step j eliminates column j (j >= 0) and pivots column j + 1 (if any).
     PanelSteps(a, lda, m, nb, row0, ipiv, w, s);
   }

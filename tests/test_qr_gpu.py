@@ -134,3 +134,85 @@ def test_padding_rows_untouched():
     ho, to = oracle.geqrf(a[:m])
     ok, errs = close(h[:m], tau, ho, to, m, n)
     assert ok, errs
+
+
+PANEL_CASES = [(300, 200, 1), (1000, 700, 2), (2100, 140, 3), (130, 300, 4), (5000, 64, 5),
+               (4097, 100, 6)]
+
+
+def factor_panel_cases():
+    """QR of the cases above (one with column 0 zero below the diagonal,
+    one scaled near underflow) through the Fortran entry."""
+    out = []
+    for m, n, seed in PANEL_CASES:
+        a = oracle.fill_uniform(m, n, 900 + seed)
+        if seed == 2:
+            a[1:, 0] = 0.0
+        if seed == 3:
+            a *= 2.0 ** -1000
+        out.extend(gpu_qr(a))
+    return out
+
+
+def _child(env_extra, code_tail, tmp_path):
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    code = ("import sys, numpy as np\n"
+            f"sys.path[:0] = [{root!r}, {here!r}]\n"
+            "import hwyblas_amd as H, oracle, test_qr_gpu as T\n" + code_tail)
+    env = dict(os.environ, **env_extra)
+    subprocess.run([sys.executable, "-c", code], env=env, cwd=root, check=True, timeout=240)
+
+
+def test_cooperative_panel_equals_column_launches(tmp_path):
+    """A panel whose workgroups all fit is factored in one cooperative launch
+    with two grid barriers per column; its bits equal the two launches per
+    column it replaces (taken with HWYBLAS_AMD_QR_PANEL=steps in a child
+    process), tau = 0 columns and xLARFG's rescaling included."""
+    out = tmp_path / "steps.npz"
+    _child({"HWYBLAS_AMD_QR_PANEL": "steps"},
+           f"np.savez({str(out)!r}, *T.factor_panel_cases())\n", tmp_path)
+    got = factor_panel_cases()
+    with np.load(out) as ref:
+        for i in range(len(got)):
+            assert got[i].tobytes() == ref[f"arr_{i}"].tobytes(), (PANEL_CASES[i // 2], i % 2)
+    assert got[3][0] == 0.0  # the zero column's tau
+
+
+def test_timed_out_panel_barrier_is_reported(monkeypatch):
+    """The failure path of the cooperative panel, forced from the host
+    (HWYBLAS_AMD_QR_FAIL_BARRIER=1 raises the flag before the first panel):
+    every bounded wait gives up at its first poll and every workgroup
+    returns, the panels' tau are NaN, and dgeqrf_ returns info = -1000,
+    reported to the error handler as ("DGEQRF", -1000). A normal call
+    afterwards factors correctly."""
+    m, n = 500, 70
+    a0 = oracle.fill_uniform(m, n, 3)
+    monkeypatch.setenv("HWYBLAS_AMD_QR_FAIL_BARRIER", "1")
+    a = np.asfortranarray(a0.copy())
+    tau = np.zeros(n)
+    with H.ErrorCapture() as cap:
+        info, _ = H.geqrf(m, n, a, m, tau)
+    assert info == -1000 and cap.calls == [("DGEQRF", -1000)]
+    assert np.all(np.isnan(tau))
+    monkeypatch.delenv("HWYBLAS_AMD_QR_FAIL_BARRIER")
+    h, t = gpu_qr(a0)
+    ho, to = oracle.geqrf(a0)
+    ok, errs = close(h, t, ho, to, m, n)
+    assert ok, errs
+
+
+def test_refused_cooperative_launch_falls_back_to_column_launches(tmp_path):
+    """This process has made cooperative launches; a second process on the
+    same GPU may be refused one (the device's cooperative queue is held).
+    Its factorizations fall back to the column launches and equal this
+    process's bit for bit."""
+    gpu_qr(oracle.fill_uniform(300, 100, 1))  # a cooperative launch here first
+    out = tmp_path / "child.npz"
+    _child({}, f"np.savez({str(out)!r}, *T.factor_panel_cases())\n", tmp_path)
+    got = factor_panel_cases()
+    with np.load(out) as ref:
+        for i in range(len(got)):
+            assert got[i].tobytes() == ref[f"arr_{i}"].tobytes(), (PANEL_CASES[i // 2], i % 2)
