@@ -7,16 +7,17 @@
 // -sign(alpha) |(alpha, x)| (xLARFG, src/lapack/householder.cc:44-104). The
 // blocking is the GPU's own:
 //  - panels of kPanel = 64 columns; a panel (mp x 64, mp = m - j0) is
-//    factored column by column by workgroups of 256 rows each: in one
-//    cooperative launch (QrPanelPersistentKernel, two grid barriers per
-//    column) when all its workgroups fit on the device, else in two
-//    launches per column: QrReflectKernel (the norm from the previous launch's
-//    per-workgroup partial sums, beta and tau computed redundantly and
-//    identically by every workgroup, its rows of v scaled, and one partial
-//    dot product v^T P(:, c') per workgroup for the panel's later columns)
-//    and QrUpdateKernel (the dots summed in workgroup order, the rank-1
-//    update of its rows, and the next column's norm partials); each thread
-//    issues all of its row's loads before the first sum;
+//    factored column by column by workgroups of 256 rows each, with ONE
+//    reduction per column: each workgroup publishes, for column c, the
+//    scale-safe norm partial of x = P(c+1:, c) and the dots of the same
+//    (unscaled, exponent-normalized) x with the later columns; then every
+//    workgroup forms, identically, xLARFG's beta and tau and
+//    w = v^T P = P(c, :) + scale x^T P from them, stores its rows of v,
+//    updates its rows, and publishes column c + 1's partials. The panel is
+//    one cooperative launch (QrPanelPersistentKernel: the rows in
+//    registers, one grid barrier per column) when all its workgroups fit on
+//    the device, else one launch per column (QrColumnKernel), bit for bit
+//    the same;
 //  - the block reflector's T (I - V T V^T = H(j0) ... H(j0+63), xLARFT's
 //    forward columnwise T) from G = V^T V (an MFMA GEMM) and the recurrence
 //    T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i;
@@ -24,8 +25,9 @@
 //    (xLARFB, side L, trans T) by three MFMA GEMMs: W = V^T A2,
 //    W2 = T^T W, A2 -= V W2. V^T V and V^T A2 have 64 rows and the panel's
 //    height as k: they split k (GemmSplitKDevice) to fill the device.
-// Norms are computed scale-safely: each workgroup sums (x 2^-e)^2 with e the
-// exponent of its largest |x|, and the sums are combined at the largest e.
+// Norms are computed scale-safely: each workgroup sums (x 2^-e)^2 and
+// (x 2^-e) P(:, cc) with e the exponent of its largest |x|, and the sums are
+// combined at the largest e.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -44,15 +46,32 @@ constexpr int kQrInternalError = -1000;  // dgeqrf_'s info when a panel barrier 
 constexpr int kRows = 256;   // panel rows per workgroup (one per thread)
 constexpr int kWaves = kRows / 64;
 
-struct PanelArgs {
-  double* p;       // the panel's top-left element, A(j0, j0)
-  int64_t lda;
-  int mp, ib;      // panel rows and columns
-  double* tau;     // tau + j0
-  double* beta;    // ib betas (the new diagonal), written by workgroup 0
-  double* part;    // norm partials: [blocks][2] = {e, sum of (x 2^-e)^2}
-  double* dots;    // dot partials: [blocks][kPanel]
+// One slot of a column's records, written by every workgroup before a
+// barrier and read by every workgroup after it.
+struct Slot {
+  double* part;  // [G][2]: {e, sum of (x 2^-e)^2}, e the exponent of the workgroup's max |x|
+  double* dots;  // [G][kPanel]: sum over the workgroup's rows of (x 2^-e) P(i, cc), cc > k
+  double* row;   // [kPanel]: row k of the panel, P(k, cc) for cc >= k
 };
+
+struct PanelArgs {
+  double* p;     // the panel's top-left element, A(j0, j0)
+  int64_t lda;
+  int mp, ib;    // panel rows and columns
+  int groups;    // G: workgroups per column step (mp / kRows, rounded up)
+  double* tau;   // tau + j0
+  double* rec;   // two record slots, by column parity
+};
+
+// Doubles of one record slot for G workgroups.
+__host__ __device__ constexpr int64_t SlotDoubles(int64_t groups) {
+  return groups * (2 + kPanel) + kPanel;
+}
+
+__device__ __forceinline__ Slot SlotOf(const PanelArgs& g, int k) {
+  double* base = g.rec + (k & 1) * SlotDoubles(g.groups);
+  return Slot{base, base + 2 * int64_t(g.groups), base + int64_t(g.groups) * (2 + kPanel)};
+}
 
 __device__ __forceinline__ double WaveSum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -62,58 +81,6 @@ __device__ __forceinline__ double WaveSum(double v) {
 __device__ __forceinline__ double WaveMax(double v) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   return v;
-}
-
-// This workgroup's norm partial of x (its threads' values, zero outside the
-// column's range): {e, sum (x 2^-e)^2} with e the exponent of max |x| (the
-// sum is 0 and e = -100000 when all are zero).
-__device__ void NormPartial(double x, double* part_out) {
-  __shared__ double red[kWaves];
-  __shared__ double mx_s;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double m = WaveMax(fabs(x));
-  if (lane == 0) red[wave] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double mm = 0.0;
-    for (int w = 0; w < kWaves; ++w) mm = fmax(mm, red[w]);
-    mx_s = mm;
-  }
-  __syncthreads();
-  const double mx = mx_s;
-  int e = -100000;
-  if (mx > 0.0) frexp(mx, &e);
-  const double y = mx > 0.0 ? ldexp(x, -e) : 0.0;
-  double s = WaveSum(y * y);
-  __syncthreads();
-  if (lane == 0) red[wave] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < kWaves; ++w) t += red[w];
-    PutRecord(part_out, double(e));
-    PutRecord(part_out + 1, t);
-  }
-}
-
-// ||x||_2 from the partials of `blocks` workgroups, computed by one whole
-// wave (every lane gets it): lane l takes partials l, l + 64, ..., then the
-// largest exponent and the scaled sum are reduced over the wave. Every
-// workgroup computes it from the same partials in the same order, so all
-// of them get the same bits.
-__device__ double NormOfWave(const double* part, int blocks) {
-  const int lane = threadIdx.x & 63;
-  int emax = -100000;
-  for (int b = lane; b < blocks; b += 64) emax = max(emax, int(GetRecord(part + 2 * b)));
-  emax = int(WaveMax(double(emax)));
-  if (emax == -100000) return 0.0;
-  double t = 0.0;
-  for (int b = lane; b < blocks; b += 64) {
-    const int e = int(GetRecord(part + 2 * b));
-    if (e != -100000) t += ldexp(GetRecord(part + 2 * b + 1), 2 * (e - emax));
-  }
-  t = WaveSum(t);
-  return ldexp(sqrt(t), emax);
 }
 
 __device__ double Lapy2(double x, double y) {
@@ -153,130 +120,267 @@ __device__ void Larfg(double alpha, double xnorm, double& tau, double& scal, dou
   beta = b * down;
 }
 
-// Norm partials of panel column 0 (rows 1..mp-1).
-__global__ void __launch_bounds__(kRows) QrNormKernel(PanelArgs g) {
-  const int i = blockIdx.x * kRows + threadIdx.x;
-  const double x = (i > 0 && i < g.mp) ? g.p[i] : 0.0;
-  NormPartial(x, g.part + 2 * blockIdx.x);
+// The wave's sums of 64 values per lane, scattered: on return lane l holds
+// the sum over the wave of value l. x[0..31] on entry are a lane's values
+// after the first halving step (offset 32) has been taken by the caller;
+// each later step keeps the half of the values whose index bit matches the
+// lane's bit, adds the partner's copy of that half and sends the other
+// half: 31 shuffles for the rest instead of 6 per value.
+template <int O>
+__device__ __forceinline__ void ReduceScatterStep(double (&x)[32], int lane) {
+  const bool upper = (lane & O) != 0;
+#pragma unroll
+  for (int j = 0; j < O; ++j) {
+    const double keep = upper ? x[j + O] : x[j];
+    const double send = upper ? x[j] : x[j + O];
+    x[j] = keep + __shfl_xor(send, O, 64);
+  }
+  if constexpr (O > 1) ReduceScatterStep<O / 2>(x, lane);
 }
 
-// Column c: the reflector (xLARFG) and the partial dots for columns c' > c.
-__global__ void __launch_bounds__(kRows) QrReflectKernel(PanelArgs g, int c) {
-  __shared__ double coef[2];  // scal, and the workgroup-0 copy of tau
+__device__ __forceinline__ double WaveReduceScatter32(double (&x)[32]) {
+  ReduceScatterStep<16>(x, threadIdx.x & 63);
+  return x[0];
+}
+
+// Column k's records from this workgroup: with x = P(i, k) on its rows
+// i > k and e the exponent of its largest |x| (-100000 when all are zero),
+// the norm partial {e, sum (x 2^-e)^2} and, for each later column cc,
+// sum (x 2^-e) P(i, cc); the owner of row k publishes that row. `xk` is this
+// thread's P(i, k) and `pv` its row of the panel. The 64 per-column sums of
+// a wave (slot k holding the norm's) are one reduce-scatter; the waves are
+// added in wave order through LDS.
+__device__ void ColumnPartials(const PanelArgs& g, int k, int ib, double xk,
+                               const double (&pv)[kPanel], Slot out) {
+  __shared__ double red[kWaves];
+  __shared__ double mx_s;
+  __shared__ double wsum[kWaves][kPanel];  // [w][k]: the norm sum, [w][cc > k]: the dots
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = blockIdx.x * kRows + threadIdx.x;
-  double xnorm_w = 0.0;
-  if (threadIdx.x < 64) xnorm_w = NormOfWave(g.part, gridDim.x);
+  const bool below = i > k && i < g.mp;
+  const double x = below ? xk : 0.0;
+  const double m = WaveMax(fabs(x));
+  if (lane == 0) red[wave] = m;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double tau, scal, beta;
-    Larfg(g.p[c + int64_t(c) * g.lda], xnorm_w, tau, scal, beta);
-    coef[0] = scal;
-    coef[1] = tau;
-    if (blockIdx.x == 0) {
-      g.tau[c] = tau;
-      g.beta[c] = beta;
+    double mm = 0.0;
+    for (int w = 0; w < kWaves; ++w) mm = fmax(mm, red[w]);
+    mx_s = mm;
+  }
+  __syncthreads();
+  const double mx = mx_s;
+  int e = -100000;
+  if (mx > 0.0) frexp(mx, &e);
+  const double y = mx > 0.0 ? ldexp(x, -e) : 0.0;
+  // This lane's 64 values (y^2 in slot k, y P(i, cc) for the later columns,
+  // zeros elsewhere), folded at once into the first halving step.
+  auto value = [&](int cc) {
+    return cc == k ? y * y : ((cc > k && cc < ib && below) ? y * pv[cc] : 0.0);
+  };
+  double h[32];
+  const bool upper = (lane & 32) != 0;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    const double lo = value(j), hi = value(j + 32);
+    h[j] = (upper ? hi : lo) + __shfl_xor(upper ? lo : hi, 32, 64);
+  }
+  wsum[wave][lane] = WaveReduceScatter32(h);
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t >= k && t < ib) {
+    double d = wsum[0][t];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) d += wsum[w][t];
+    if (t == k) {
+      PutRecord(out.part + 2 * blockIdx.x, double(e));
+      PutRecord(out.part + 2 * blockIdx.x + 1, d);
+    } else {
+      PutRecord(out.dots + int64_t(blockIdx.x) * kPanel + t, d);
+    }
+  }
+  if (i == k) {
+#pragma unroll
+    for (int cc = 0; cc < kPanel; ++cc)
+      if (cc >= k && cc < ib) PutRecord(out.row + cc, pv[cc]);
+  }
+}
+
+constexpr int kMaxLdsGroups = 1024;  // workgroup norm partials kept in LDS
+constexpr int kDotSplit = 4;         // threads per column in the sums of the dot partials
+constexpr int kDotBatch = 8;         // dot partials in flight per thread
+
+// Column c from its records. Every workgroup forms, identically: ||x|| from
+// the norm partials (the largest exponent emax, then the sums rescaled to
+// it, one lane per partial, reduced over a wave), alpha = P(c, c) from the
+// row record, xLARFG's tau, beta and scale, and for each later column cc
+// w(cc) = v^T P(:, cc) = P(c, cc) + scale 2^emax sum_q 2^(e_q - emax) d_q(cc)
+// (the dots of the unscaled x, so that the norm and the dots need one
+// reduction between them; kDotSplit threads per column each sum every
+// kDotSplit-th partial, the sums then added in order). Then its rows:
+// v = scale x stored below the diagonal (beta on it), P(i, cc) -= tau v(i)
+// w(cc), and column c + 1's records. Every record load of the step is
+// issued at its start (one memory round trip) when G <= kDotSplit *
+// kDotBatch. `cur` is this thread's P(i, c) on entry, P(i, c + 1) on return.
+__device__ void ColumnStep(const PanelArgs& g, int c, int ib, double (&pv)[kPanel], double& cur) {
+  __shared__ double coef[4];  // tau, scale, scale 2^emax, beta
+  __shared__ double w[kPanel];
+  __shared__ double wpart[kDotSplit][kPanel];
+  __shared__ int se[kMaxLdsGroups];
+  __shared__ double ss[kMaxLdsGroups];
+  __shared__ int emax_s;
+  const Slot in = SlotOf(g, c);
+  const int G = g.groups;
+  const int t = threadIdx.x, lane = t & 63;
+  // Every load first: the norm partials (to LDS), this thread's row-record
+  // entry, and its first kDotBatch dot partials.
+  const double rowv = (t > c && t < ib) ? GetRecord(in.row + t) : 0.0;
+  const double alpha = t == 0 ? GetRecord(in.row + c) : 0.0;
+  const int cci = t / kDotSplit, part = t % kDotSplit;
+  const int cc = c + 1 + cci;
+  const bool summer = cc < ib;
+  double dv[kDotBatch];
+#pragma unroll
+  for (int u = 0; u < kDotBatch; ++u) {
+    const int q = part + kDotSplit * u;
+    dv[u] = (summer && q < G) ? GetRecord(in.dots + int64_t(q) * kPanel + cc) : 0.0;
+  }
+  for (int q = t; q < G && q < kMaxLdsGroups; q += kRows) {
+    se[q] = int(GetRecord(in.part + 2 * q));
+    ss[q] = GetRecord(in.part + 2 * q + 1);
+  }
+  __syncthreads();
+  auto exp_of = [&](int q) { return q < kMaxLdsGroups ? se[q] : int(GetRecord(in.part + 2 * q)); };
+  if (t < 64) {
+    int emax = -100000;
+    for (int q = lane; q < G; q += 64) emax = max(emax, exp_of(q));
+    emax = int(WaveMax(double(emax)));
+    double sum = 0.0;
+    if (emax != -100000) {
+      for (int q = lane; q < G; q += 64) {
+        const int e = exp_of(q);
+        const double sq = q < kMaxLdsGroups ? ss[q] : GetRecord(in.part + 2 * q + 1);
+        if (e != -100000) sum += ldexp(sq, 2 * (e - emax));
+      }
+    }
+    sum = WaveSum(sum);
+    if (t == 0) {
+      const double xnorm = emax == -100000 ? 0.0 : ldexp(sqrt(sum), emax);
+      double tau, scal, beta;
+      Larfg(alpha, xnorm, tau, scal, beta);
+      coef[0] = tau;
+      coef[1] = scal;
+      coef[2] = ldexp(scal, emax);
+      coef[3] = beta;
+      emax_s = emax;
+      if (blockIdx.x == 0) g.tau[c] = tau;
     }
   }
   __syncthreads();
-  const double scal = coef[0];
-  // v(i) for this thread's row: 1 on the diagonal row, the scaled x below.
+  const double tau = coef[0], scal = coef[1], f = coef[2];
+  const int emax = emax_s;
+  if (summer) {
+    double d = 0.0;
+#pragma unroll
+    for (int u = 0; u < kDotBatch; ++u) {
+      const int q = part + kDotSplit * u;
+      if (q < G) {
+        const int e = exp_of(q);
+        if (e != -100000) d += ldexp(dv[u], e - emax);
+      }
+    }
+    for (int q0 = part + kDotSplit * kDotBatch; q0 < G; q0 += kDotSplit * kDotBatch) {
+#pragma unroll
+      for (int u = 0; u < kDotBatch; ++u) {
+        const int q = q0 + kDotSplit * u;
+        dv[u] = q < G ? GetRecord(in.dots + int64_t(q) * kPanel + cc) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kDotBatch; ++u) {
+        const int q = q0 + kDotSplit * u;
+        if (q < G) {
+          const int e = exp_of(q);
+          if (e != -100000) d += ldexp(dv[u], e - emax);
+        }
+      }
+    }
+    wpart[part][cc] = d;
+  }
+  __syncthreads();
+  if (t > c && t < ib) {
+    double d = wpart[0][t];
+#pragma unroll
+    for (int p = 1; p < kDotSplit; ++p) d += wpart[p][t];
+    w[t] = rowv + f * d;
+  }
+  __syncthreads();
+  const int i = blockIdx.x * kRows + threadIdx.x;
+  const bool inr = i < g.mp;
   double v = 0.0;
   if (i == c) {
     v = 1.0;
-  } else if (i > c && i < g.mp) {
-    double* pic = g.p + i + int64_t(c) * g.lda;
-    v = *pic * scal;
-    *pic = v;
+    g.p[i + int64_t(c) * g.lda] = coef[3];
+  } else if (i > c && inr) {
+    v = cur * scal;
+    g.p[i + int64_t(c) * g.lda] = v;
   }
-  // Partial dots with the later columns (row c contributes P(c, c') * 1):
-  // every load first, then one sum per column over the wave, then over the
-  // workgroup's waves (in wave order) through LDS.
-  __shared__ double wsum[kWaves][kPanel];
-  const bool mine = i >= c && i < g.mp;
-  double pv[kPanel];
+  double next = 0.0;
+  if (i >= c && inr) {
 #pragma unroll
-  for (int cc = 0; cc < kPanel; ++cc)
-    pv[cc] = (mine && cc > c && cc < g.ib) ? g.p[i + int64_t(cc) * g.lda] : 0.0;
-#pragma unroll
-  for (int cc = 0; cc < kPanel; ++cc) {
-    if (cc > c && cc < g.ib) {
-      const double d = WaveSum(v * pv[cc]);
-      if (lane == 0) wsum[wave][cc] = d;
-    }
-  }
-  __syncthreads();
-  const int cc = threadIdx.x;
-  if (cc > c && cc < g.ib) {
-    double d = wsum[0][cc];
-#pragma unroll
-    for (int w = 1; w < kWaves; ++w) d += wsum[w][cc];
-    g.dots[int64_t(blockIdx.x) * kPanel + cc] = d;
-  }
-}
-
-// Column c: w(c') = sum of the dot partials, P(i, c') -= tau v(i) w(c') for
-// this workgroup's rows i >= c, the diagonal set to beta, and the norm
-// partials of column c + 1 (rows c + 2 ..).
-__global__ void __launch_bounds__(kRows) QrUpdateKernel(PanelArgs g, int c) {
-  __shared__ double w[kPanel];
-  const int i = blockIdx.x * kRows + threadIdx.x;
-  const int nparts = gridDim.x;
-  const bool mine = i >= c && i < g.mp;
-  // This row's entries of the later columns, loaded while w is summed.
-  double pv[kPanel];
-#pragma unroll
-  for (int cc = 0; cc < kPanel; ++cc)
-    pv[cc] = (mine && cc > c && cc < g.ib) ? g.p[i + int64_t(cc) * g.lda] : 0.0;
-  const double v = !mine ? 0.0 : (i == c ? 1.0 : g.p[i + int64_t(c) * g.lda]);
-  const double tau = g.tau[c];
-  // w(c') = the workgroups' dot partials in workgroup order, loads in
-  // batches of 8.
-  for (int cc = c + 1 + threadIdx.x; cc < g.ib; cc += blockDim.x) {
-    double s = 0.0;
-    int q = 0;
-    for (; q + 8 <= nparts; q += 8) {
-      double t[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) t[u] = g.dots[int64_t(q + u) * kPanel + cc];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) s += t[u];
-    }
-    for (; q < nparts; ++q) s += g.dots[int64_t(q) * kPanel + cc];
-    w[cc] = s;
-  }
-  __syncthreads();
-  double next = 0.0;  // this row's entry of column c + 1 after the update
-  if (mine) {
-#pragma unroll
-    for (int cc = 0; cc < kPanel; ++cc) {
-      if (cc > c && cc < g.ib) {
-        const double nv = pv[cc] - tau * v * w[cc];
-        g.p[i + int64_t(cc) * g.lda] = nv;
-        if (cc == c + 1) next = nv;
+    for (int k = 0; k < kPanel; ++k) {
+      if (k > c && k < ib) {
+        const double nv = pv[k] - tau * v * w[k];
+        pv[k] = nv;
+        if (k == c + 1) next = nv;
       }
     }
-    if (i == c) g.p[i + int64_t(c) * g.lda] = g.beta[c];
   }
-  if (c + 1 < g.ib) NormPartial(i > c + 1 && i < g.mp ? next : 0.0, g.part + 2 * blockIdx.x);
+  if (c + 1 < ib) ColumnPartials(g, c + 1, ib, next, pv, SlotOf(g, c + 1));
+  cur = next;
+}
+
+// This thread's row of the panel (zeros outside it).
+__device__ __forceinline__ void LoadRow(const PanelArgs& g, double (&pv)[kPanel]) {
+  const int i = blockIdx.x * kRows + threadIdx.x;
+#pragma unroll
+  for (int cc = 0; cc < kPanel; ++cc)
+    pv[cc] = (i < g.mp && cc < g.ib) ? g.p[i + int64_t(cc) * g.lda] : 0.0;
+}
+
+// Column 0's records (the path of one launch per column).
+__global__ void __launch_bounds__(kRows) QrFirstColumnKernel(PanelArgs g) {
+  double pv[kPanel];
+  LoadRow(g, pv);
+  ColumnPartials(g, 0, g.ib, pv[0], pv, SlotOf(g, 0));
+}
+
+// Column c (the path of one launch per column): the rows loaded, the step,
+// the updated entries stored.
+__global__ void __launch_bounds__(kRows) QrColumnKernel(PanelArgs g, int c) {
+  double pv[kPanel];
+  LoadRow(g, pv);
+  double cur = 0.0;
+#pragma unroll
+  for (int cc = 0; cc < kPanel; ++cc)
+    if (cc == c) cur = pv[cc];
+  ColumnStep(g, c, g.ib, pv, cur);
+  const int i = blockIdx.x * kRows + threadIdx.x;
+  if (i >= c && i < g.mp) {
+#pragma unroll
+    for (int cc = 0; cc < kPanel; ++cc)
+      if (cc > c && cc < g.ib) g.p[i + int64_t(cc) * g.lda] = pv[cc];
+  }
 }
 
 // The whole panel in one cooperative launch: each thread keeps its row of
-// the panel in registers, and the two kernel boundaries of a column step
-// become two grid-wide barriers (GridSync). Per column c, as QrReflectKernel
-// and QrUpdateKernel with the same operations in the same order (so the
-// bits are theirs): after barrier A every workgroup forms the norm from the
-// partials and alpha from the record published by the owner of row c, and
-// xLARFG's tau, beta and scale; it scales its rows of v (stored to the
-// panel), and publishes its partial dots with the later columns; after
-// barrier B it sums the partials in workgroup order, updates its rows, and
-// publishes the next column's norm partial (and, on row c + 1, alpha).
-// Records are reused every column: a record is rewritten only after the
-// barrier that follows its last read. R's entries (rows < ib, right of the
-// diagonal) are stored at the end. After a timed-out wait every workgroup
-// returns and workgroup 0 sets the panel's tau to NaN.
+// the panel in registers and the kernel boundary between column steps
+// becomes one grid-wide barrier (GridSync); the steps are those of
+// QrColumnKernel, so the bits are the same. Records alternate between two
+// slots: a slot is rewritten only after the barrier that follows its last
+// read. R's entries (rows < ib, right of the diagonal) are stored at the
+// end. After a timed-out wait every workgroup returns and workgroup 0 sets
+// the panel's tau to NaN.
 struct PersistArgs {
   PanelArgs g;
-  double* alpha;    // the record of P(c, c) before column c's reflector
   unsigned* bar;    // arrivals (bar_base at launch: the count runs on over panels)
   unsigned* failed;
   unsigned bar_base;
@@ -284,108 +388,27 @@ struct PersistArgs {
 
 __global__ void __launch_bounds__(kRows) QrPanelPersistentKernel(PersistArgs a) {
   const PanelArgs& g = a.g;
-  __shared__ double coef[3];  // scal, tau, beta
-  __shared__ double wsum[kWaves][kPanel];
-  __shared__ double w[kPanel];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int i = blockIdx.x * kRows + threadIdx.x;
-  const int G = gridDim.x;
-  const bool in = i < g.mp;
   double pv[kPanel];
-#pragma unroll
-  for (int cc = 0; cc < kPanel; ++cc)
-    pv[cc] = (in && cc < g.ib) ? g.p[i + int64_t(cc) * g.lda] : 0.0;
-  NormPartial(i > 0 && in ? pv[0] : 0.0, g.part + 2 * blockIdx.x);
-  if (i == 0) PutRecord(a.alpha, pv[0]);
-  double cur = pv[0];  // this row's entry of column c
+  LoadRow(g, pv);
+  ColumnPartials(g, 0, g.ib, pv[0], pv, SlotOf(g, 0));
+  double cur = pv[0];
   unsigned arrivals = a.bar_base;
   for (int c = 0; c < g.ib; ++c) {
-    // An opaque copy of the column count: the per-column tests below are
-    // then formed where they are used, not hoisted out of the column loop
-    // into (spilled) scalar registers.
+    // An opaque copy of the column count: the per-column tests are then
+    // formed where they are used, not hoisted out of the loop into
+    // (spilled) scalar registers.
     int ib = g.ib;
     asm volatile("" : "+s"(ib));
-    arrivals += unsigned(G);
+    arrivals += unsigned(g.groups);
     if (!GridSync(a.bar, arrivals, a.failed)) break;
-    double xnorm_w = 0.0;
-    if (threadIdx.x < 64) xnorm_w = NormOfWave(g.part, G);
-    if (threadIdx.x == 0) {
-      double tau, scal, beta;
-      Larfg(GetRecord(a.alpha), xnorm_w, tau, scal, beta);
-      coef[0] = scal;
-      coef[1] = tau;
-      coef[2] = beta;
-      if (blockIdx.x == 0) {
-        g.tau[c] = tau;
-        g.beta[c] = beta;
-      }
-    }
-    __syncthreads();
-    const double scal = coef[0], tau = coef[1];
-    double v = 0.0;
-    if (i == c) {
-      v = 1.0;
-      g.p[i + int64_t(c) * g.lda] = coef[2];
-    } else if (i > c && in) {
-      v = cur * scal;
-      g.p[i + int64_t(c) * g.lda] = v;
-    }
-    const bool mine = i >= c && in;
-#pragma unroll
-    for (int cc = 0; cc < kPanel; ++cc) {
-      if (cc > c && cc < ib) {
-        const double d = WaveSum(v * (mine ? pv[cc] : 0.0));
-        if (lane == 0) wsum[wave][cc] = d;
-      }
-    }
-    __syncthreads();
-    {
-      const int cc = threadIdx.x;
-      if (cc > c && cc < ib) {
-        double d = wsum[0][cc];
-#pragma unroll
-        for (int q = 1; q < kWaves; ++q) d += wsum[q][cc];
-        PutRecord(g.dots + int64_t(blockIdx.x) * kPanel + cc, d);
-      }
-    }
-    arrivals += unsigned(G);
-    if (!GridSync(a.bar, arrivals, a.failed)) break;
-    for (int cc = c + 1 + int(threadIdx.x); cc < ib; cc += kRows) {
-      double sum = 0.0;
-      int q = 0;
-      for (; q + 8 <= G; q += 8) {
-        double t[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) t[u] = GetRecord(g.dots + int64_t(q + u) * kPanel + cc);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) sum += t[u];
-      }
-      for (; q < G; ++q) sum += GetRecord(g.dots + int64_t(q) * kPanel + cc);
-      w[cc] = sum;
-    }
-    __syncthreads();
-    double next = 0.0;
-    if (mine) {
-#pragma unroll
-      for (int cc = 0; cc < kPanel; ++cc) {
-        if (cc > c && cc < ib) {
-          const double nv = pv[cc] - tau * v * w[cc];
-          pv[cc] = nv;
-          if (cc == c + 1) next = nv;
-        }
-      }
-    }
-    if (c + 1 < ib) {
-      NormPartial(i > c + 1 && in ? next : 0.0, g.part + 2 * blockIdx.x);
-      if (i == c + 1) PutRecord(a.alpha, next);
-    }
-    cur = next;
+    ColumnStep(g, c, ib, pv, cur);
   }
   if (__hip_atomic_load(a.failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
     if (blockIdx.x == 0 && int(threadIdx.x) < g.ib) g.tau[threadIdx.x] = __builtin_nan("");
     return;
   }
   // R right of the diagonal: rows i < ib, final since column i's update.
+  const int i = blockIdx.x * kRows + threadIdx.x;
   int ib = g.ib;
   asm volatile("" : "+s"(ib));
   if (i < ib) {
@@ -441,16 +464,13 @@ __global__ void __launch_bounds__(kPanel) LarftKernel(const double* gm, int ib,
 }
 
 struct Work {
-  double* alpha;  // the persistent panel's alpha record (8 doubles)
   unsigned* sync; // [0]: barrier arrivals, [32]: failed (64 words)
   double* v;      // m x kPanel
   double* t;      // kPanel x kPanel
   double* gm;     // kPanel x kPanel
   double* w;      // kPanel x n
   double* w2;     // kPanel x n
-  double* part;   // [blocks][2]
-  double* dots;   // [blocks][kPanel]
-  double* beta;   // kPanel
+  double* rec;    // the panel's two record slots
   double* split;  // split-k partials of V^T V and V^T A2
   int64_t split_cap;
 };
@@ -459,16 +479,14 @@ int64_t SplitCap(int64_t n) { return 8 * int64_t(kPanel) * std::max<int64_t>(n, 
 
 size_t WorkBytes(int64_t m, int64_t n) {
   const int64_t blocks = (m + kRows - 1) / kRows;
-  return 8 * (8 + 32 + size_t(m) * kPanel + 2 * kPanel * kPanel + 2 * size_t(kPanel) * n +
-              2 * blocks + size_t(blocks) * kPanel + kPanel + size_t(SplitCap(n))) + 256;
+  return 8 * (32 + size_t(m) * kPanel + 2 * kPanel * kPanel + 2 * size_t(kPanel) * n +
+              2 * size_t(SlotDoubles(blocks)) + size_t(SplitCap(n))) + 256;
 }
 
 Work Carve(void* base, int64_t m, int64_t n) {
   const int64_t blocks = (m + kRows - 1) / kRows;
   double* p = static_cast<double*>(base);
   Work w;
-  w.alpha = p;
-  p += 8;
   w.sync = reinterpret_cast<unsigned*>(p);
   p += 32;
   w.v = p;
@@ -481,12 +499,8 @@ Work Carve(void* base, int64_t m, int64_t n) {
   p += int64_t(kPanel) * n;
   w.w2 = p;
   p += int64_t(kPanel) * n;
-  w.part = p;
-  p += 2 * blocks;
-  w.dots = p;
-  p += blocks * kPanel;
-  w.beta = p;
-  p += kPanel;
+  w.rec = p;
+  p += 2 * SlotDoubles(blocks);
   w.split = p;
   w.split_cap = SplitCap(n);
   return w;
@@ -541,10 +555,10 @@ void Geqrf(int64_t m, int64_t n, double* a, int64_t lda, double* tau, hipStream_
     const int ib = int(std::min<int64_t>(kPanel, k - j0));
     const int mp = int(m - j0);
     const int blocks = (mp + kRows - 1) / kRows;
-    PanelArgs g{a + j0 + j0 * lda, lda, mp, ib, tau + j0, w.beta, w.part, w.dots};
+    PanelArgs g{a + j0 + j0 * lda, lda, mp, ib, blocks, tau + j0, w.rec};
     bool coop = false;
     if (blocks <= cap) {
-      PersistArgs pa{g, w.alpha, w.sync, w.sync + 32, arrivals};
+      PersistArgs pa{g, w.sync, w.sync + 32, arrivals};
       void* args[] = {&pa};
       // A cooperative launch the runtime refuses (another process holding
       // the device's cooperative queue, for one) leaves this panel and the
@@ -554,18 +568,16 @@ void Geqrf(int64_t m, int64_t n, double* a, int64_t lda, double* tau, hipStream_
           0, s);
       if (e == hipSuccess) {
         coop = true;
-        arrivals += 2u * unsigned(ib) * unsigned(blocks);
+        arrivals += unsigned(ib) * unsigned(blocks);
       } else {
         (void)hipGetLastError();
         cap = 0;
       }
     }
     if (!coop) {
-      hipLaunchKernelGGL(QrNormKernel, dim3(blocks), dim3(kRows), 0, s, g);
-      for (int c = 0; c < ib; ++c) {
-        hipLaunchKernelGGL(QrReflectKernel, dim3(blocks), dim3(kRows), 0, s, g, c);
-        hipLaunchKernelGGL(QrUpdateKernel, dim3(blocks), dim3(kRows), 0, s, g, c);
-      }
+      hipLaunchKernelGGL(QrFirstColumnKernel, dim3(blocks), dim3(kRows), 0, s, g);
+      for (int c = 0; c < ib; ++c)
+        hipLaunchKernelGGL(QrColumnKernel, dim3(blocks), dim3(kRows), 0, s, g, c);
     }
     HWY_HIP_CHECK(hipGetLastError());
     const int64_t n2 = n - j0 - ib;
