@@ -18,13 +18,17 @@
 //    registers, one grid barrier per column) when all its workgroups fit on
 //    the device, else one launch per column (QrColumnKernel), bit for bit
 //    the same;
-//  - the block reflector's T (I - V T V^T = H(j0) ... H(j0+63), xLARFT's
-//    forward columnwise T) from G = V^T V (an MFMA GEMM) and the recurrence
+//  - a panel's T (I - V T V^T = H(j0) ... H(j0+63), xLARFT's forward
+//    columnwise T) from G = V^T V (an MFMA GEMM) and the recurrence
 //    T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i;
+//  - blocks of two panels (128 columns): the first panel's reflectors are
+//    applied to the second panel's columns, the second panel factored, and
+//    the block's T = [T1, -T1 (V1^T V2) T2; 0, T2];
 //  - the trailing columns updated as A2 := (I - V T^T V^T) A2 = H^T A2
 //    (xLARFB, side L, trans T) by three MFMA GEMMs: W = V^T A2,
-//    W2 = T^T W, A2 -= V W2. V^T V and V^T A2 have 64 rows and the panel's
-//    height as k: they split k (GemmSplitKDevice) to fill the device.
+//    W2 = T^T W, A2 -= V W2, with k = 128 for a whole block. V^T V and
+//    V^T A2 have 64 or 128 rows and the panel's height as k: they split k
+//    (GemmSplitKDevice) to fill the device.
 // Norms are computed scale-safely: each workgroup sums (x 2^-e)^2 and
 // (x 2^-e) P(:, cc) with e the exponent of its largest |x|, and the sums are
 // combined at the largest e.
@@ -418,23 +422,26 @@ __global__ void __launch_bounds__(kRows) QrPanelPersistentKernel(PersistArgs a) 
   }
 }
 
-// V (mp x ib, leading dimension mp): the panel's reflectors with the unit
-// diagonal and zeros above written out.
-__global__ void ExplicitVKernel(const double* p, int64_t lda, int mp, int ib, double* v) {
-  const int64_t total = int64_t(mp) * ib;
+// Columns of V (rows x ib, leading dimension ldv): a panel's reflectors
+// with the unit diagonal and zeros above written out, the panel's first row
+// at row d of V (rows above it zero).
+__global__ void ExplicitVKernel(const double* p, int64_t lda, int rows, int ib, int d, double* v,
+                                int64_t ldv) {
+  const int64_t total = int64_t(rows) * ib;
   for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
        t += int64_t(gridDim.x) * blockDim.x) {
-    const int i = int(t % mp), j = int(t / mp);
-    v[t] = i == j ? 1.0 : (i > j ? p[i + int64_t(j) * lda] : 0.0);
+    const int i = int(t % rows), j = int(t / rows), r = i - d;
+    v[i + int64_t(j) * ldv] = r == j ? 1.0 : (r > j ? p[r + int64_t(j) * lda] : 0.0);
   }
 }
 
-// T (ib x ib, upper, leading dimension kPanel) from G = V^T V and tau:
+// T (ib x ib, upper, leading dimension ldt) from G = V^T V and tau:
 // T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i; zeros below.
 // One workgroup of kPanel threads, thread r owning row r of T in LDS (G and
 // tau loaded into LDS first, every load in flight at once).
 __global__ void __launch_bounds__(kPanel) LarftKernel(const double* gm, int ib,
-                                                       const double* tau, double* t) {
+                                                       const double* tau, double* t,
+                                                       int ldt) {
   __shared__ double sg[kPanel][kPanel + 1];  // sg[q][i] = G(q, i)
   __shared__ double st[kPanel][kPanel + 1];  // st[r][q] = T(r, q)
   __shared__ double stau[kPanel];
@@ -460,26 +467,34 @@ __global__ void __launch_bounds__(kPanel) LarftKernel(const double* gm, int ib,
     __syncthreads();
   }
 #pragma unroll
-  for (int j = 0; j < kPanel; ++j) t[r + j * kPanel] = st[r][j];
+  for (int j = 0; j < kPanel; ++j) t[r + j * ldt] = st[r][j];
 }
+
+__global__ void ZeroKernel(double* p, int64_t n) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += int64_t(gridDim.x) * blockDim.x)
+    p[t] = 0.0;
+}
+
+constexpr int kBlock = 2 * kPanel;  // columns per block reflector (two panels)
 
 struct Work {
   unsigned* sync; // [0]: barrier arrivals, [32]: failed (64 words)
-  double* v;      // m x kPanel
-  double* t;      // kPanel x kPanel
-  double* gm;     // kPanel x kPanel
-  double* w;      // kPanel x n
-  double* w2;     // kPanel x n
+  double* v;      // m x kBlock
+  double* t;      // kBlock x kBlock
+  double* gm;     // 4 kPanel x kPanel: G11, G22, G12, and a product
+  double* w;      // kBlock x n
+  double* w2;     // kBlock x n
   double* rec;    // the panel's two record slots
   double* split;  // split-k partials of V^T V and V^T A2
   int64_t split_cap;
 };
 
-int64_t SplitCap(int64_t n) { return 8 * int64_t(kPanel) * std::max<int64_t>(n, 4096); }
+int64_t SplitCap(int64_t n) { return 8 * int64_t(kBlock) * std::max<int64_t>(n, 4096); }
 
 size_t WorkBytes(int64_t m, int64_t n) {
   const int64_t blocks = (m + kRows - 1) / kRows;
-  return 8 * (32 + size_t(m) * kPanel + 2 * kPanel * kPanel + 2 * size_t(kPanel) * n +
+  return 8 * (32 + size_t(m) * kBlock + 2 * kBlock * kBlock + 2 * size_t(kBlock) * n +
               2 * size_t(SlotDoubles(blocks)) + size_t(SplitCap(n))) + 256;
 }
 
@@ -490,15 +505,15 @@ Work Carve(void* base, int64_t m, int64_t n) {
   w.sync = reinterpret_cast<unsigned*>(p);
   p += 32;
   w.v = p;
-  p += m * kPanel;
+  p += m * kBlock;
   w.t = p;
-  p += kPanel * kPanel;
+  p += kBlock * kBlock;
   w.gm = p;
-  p += kPanel * kPanel;
+  p += kBlock * kBlock;
   w.w = p;
-  p += int64_t(kPanel) * n;
+  p += int64_t(kBlock) * n;
   w.w2 = p;
-  p += int64_t(kPanel) * n;
+  p += int64_t(kBlock) * n;
   w.rec = p;
   p += 2 * SlotDoubles(blocks);
   w.split = p;
@@ -551,8 +566,9 @@ void Geqrf(int64_t m, int64_t n, double* a, int64_t lda, double* tau, hipStream_
     if (fail && *fail == '1')
       hipLaunchKernelGGL(FillWords, dim3(1), dim3(64), 0, s, w.sync + 32, 1, 1u);
   }
-  for (int64_t j0 = 0; j0 < k; j0 += kPanel) {
-    const int ib = int(std::min<int64_t>(kPanel, k - j0));
+  // Panel (column j0, ib columns, mp = m - j0 rows): one cooperative launch,
+  // or the column launches.
+  auto factor_panel = [&](int64_t j0, int ib) {
     const int mp = int(m - j0);
     const int blocks = (mp + kRows - 1) / kRows;
     PanelArgs g{a + j0 + j0 * lda, lda, mp, ib, blocks, tau + j0, w.rec};
@@ -580,21 +596,70 @@ void Geqrf(int64_t m, int64_t n, double* a, int64_t lda, double* tau, hipStream_
         hipLaunchKernelGGL(QrColumnKernel, dim3(blocks), dim3(kRows), 0, s, g, c);
     }
     HWY_HIP_CHECK(hipGetLastError());
-    const int64_t n2 = n - j0 - ib;
-    if (n2 <= 0) continue;
-    // The block reflector, then H^T applied to the trailing columns.
+  };
+  // Columns of the explicit V (leading dimension mp) from the panel at
+  // column j0 + d (its first row at row d of V).
+  auto explicit_v = [&](int64_t j0, int d, int ib, int mp, double* v) {
     const int64_t vt = int64_t(mp) * ib;
     hipLaunchKernelGGL(ExplicitVKernel, dim3(unsigned(std::min<int64_t>((vt + 255) / 256, 2048))),
-                       dim3(256), 0, s, g.p, lda, mp, ib, w.v);
-    GemmSplitKDevice(true, false, ib, ib, mp, 1.0, w.v, mp, w.v, mp, 0.0, w.gm, ib, w.split,
+                       dim3(256), 0, s, a + (j0 + d) + (j0 + d) * lda, lda, mp, ib, d, v,
+                       int64_t(mp));
+  };
+  double* g11 = w.gm;
+  double* g22 = w.gm + kPanel * kPanel;
+  double* g12 = w.gm + 2 * kPanel * kPanel;
+  double* tmp = w.gm + 3 * kPanel * kPanel;
+  // Blocks of two panels: the first panel's reflectors are applied to the
+  // second panel's columns only; the block's T = [T1, -T1 (V1^T V2) T2;
+  // 0, T2] (xLARFT's T of the product H1 H2); then H^T = I - V T^T V^T is
+  // applied to the trailing columns with k = 128, half the passes over them
+  // of 64-column blocks.
+  for (int64_t j0 = 0; j0 < k; j0 += kBlock) {
+    const int kb = int(std::min<int64_t>(kBlock, k - j0));
+    const int ib1 = std::min(kPanel, kb), ib2 = kb - ib1;
+    const int mp = int(m - j0);
+    const int64_t n2 = n - j0 - kb;
+    double* v = w.v;
+    double* t = w.t;
+    factor_panel(j0, ib1);
+    if (ib2 == 0 && n2 <= 0) continue;
+    hipLaunchKernelGGL(ZeroKernel, dim3(16), dim3(256), 0, s, t, int64_t(kBlock) * kBlock);
+    explicit_v(j0, 0, ib1, mp, v);
+    GemmSplitKDevice(true, false, ib1, ib1, mp, 1.0, v, mp, v, mp, 0.0, g11, ib1, w.split,
                      w.split_cap, s);
-    hipLaunchKernelGGL(LarftKernel, dim3(1), dim3(kPanel), 0, s, w.gm, ib, g.tau, w.t);
+    hipLaunchKernelGGL(LarftKernel, dim3(1), dim3(kPanel), 0, s, g11, ib1, tau + j0, t, kBlock);
     HWY_HIP_CHECK(hipGetLastError());
-    double* a2 = a + j0 + (j0 + ib) * lda;
-    GemmSplitKDevice(true, false, ib, n2, mp, 1.0, w.v, mp, a2, lda, 0.0, w.w, ib, w.split,
+    if (ib2 > 0) {
+      // H1^T on the second panel's columns, then that panel.
+      double* a2 = a + j0 + (j0 + ib1) * lda;
+      GemmSplitKDevice(true, false, ib1, ib2, mp, 1.0, v, mp, a2, lda, 0.0, w.w, ib1, w.split,
+                       w.split_cap, s);
+      GemmDevice<double>(true, false, ib1, ib2, ib1, 1.0, t, kBlock, w.w, ib1, 0.0, w.w2, ib1,
+                         s);
+      GemmDevice<double>(false, false, mp, ib2, ib1, -1.0, v, mp, w.w2, ib1, 1.0, a2, lda, s);
+      factor_panel(j0 + ib1, ib2);
+      if (n2 <= 0) continue;
+      double* v2 = v + int64_t(mp) * ib1;
+      explicit_v(j0, ib1, ib2, mp, v2);
+      const int mq = mp - ib1;  // V2's rows below the zeros
+      GemmSplitKDevice(true, false, ib2, ib2, mq, 1.0, v2 + ib1, mp, v2 + ib1, mp, 0.0, g22, ib2,
+                       w.split, w.split_cap, s);
+      double* t2 = t + ib1 + int64_t(ib1) * kBlock;
+      hipLaunchKernelGGL(LarftKernel, dim3(1), dim3(kPanel), 0, s, g22, ib2, tau + j0 + ib1, t2,
+                         kBlock);
+      HWY_HIP_CHECK(hipGetLastError());
+      GemmSplitKDevice(true, false, ib1, ib2, mq, 1.0, v + ib1, mp, v2 + ib1, mp, 0.0, g12, ib1,
+                       w.split, w.split_cap, s);
+      GemmDevice<double>(false, false, ib1, ib2, ib2, 1.0, g12, ib1, t2, kBlock, 0.0, tmp, ib1, s);
+      GemmDevice<double>(false, false, ib1, ib2, ib1, -1.0, t, kBlock, tmp, ib1, 0.0,
+                         t + int64_t(ib1) * kBlock, kBlock, s);
+    }
+    // H^T applied to the trailing columns.
+    double* a3 = a + j0 + (j0 + kb) * lda;
+    GemmSplitKDevice(true, false, kb, n2, mp, 1.0, v, mp, a3, lda, 0.0, w.w, kb, w.split,
                      w.split_cap, s);
-    GemmDevice<double>(true, false, ib, n2, ib, 1.0, w.t, kPanel, w.w, ib, 0.0, w.w2, ib, s);
-    GemmDevice<double>(false, false, mp, n2, ib, -1.0, w.v, mp, w.w2, ib, 1.0, a2, lda, s);
+    GemmDevice<double>(true, false, kb, n2, kb, 1.0, t, kBlock, w.w, kb, 0.0, w.w2, kb, s);
+    GemmDevice<double>(false, false, mp, n2, kb, -1.0, v, mp, w.w2, kb, 1.0, a3, lda, s);
   }
   if (failed) {
     *failed = 0;
