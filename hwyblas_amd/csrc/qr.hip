@@ -457,11 +457,20 @@ __global__ void __launch_bounds__(kPanel) LarftKernel(const double* gm, int ib,
   stau[r] = r < ib ? tau[r] : 0.0;
   __syncthreads();
   for (int i = 0; i < ib; ++i) {
-    // T(r, i) = -tau_i sum_{q = r}^{i - 1} T(r, q) G(q, i), ascending q.
+    // T(r, i) = -tau_i sum_{q = r}^{i - 1} T(r, q) G(q, i): four running
+    // sums (q = r + 4u + p), added pairwise at the end, so that the
+    // thread's chain of dependent adds is a quarter as long.
     if (r < i) {
-      double s = 0.0;
-      for (int q = r; q < i; ++q) s += st[r][q] * sg[q][i];
-      st[r][i] = -stau[i] * s;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int q = r;
+      for (; q + 4 <= i; q += 4) {
+        s0 += st[r][q] * sg[q][i];
+        s1 += st[r][q + 1] * sg[q + 1][i];
+        s2 += st[r][q + 2] * sg[q + 2][i];
+        s3 += st[r][q + 3] * sg[q + 3][i];
+      }
+      for (; q < i; ++q) s0 += st[r][q] * sg[q][i];
+      st[r][i] = -stau[i] * ((s0 + s1) + (s2 + s3));
     }
     if (r == i) st[r][i] = stau[i];
     __syncthreads();
