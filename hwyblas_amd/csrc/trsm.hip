@@ -176,8 +176,9 @@ __constant__ FusedPlan kFusedPlans[4] = {MakeFusedPlan(false, 128), MakeFusedPla
 static_assert(MakeFusedPlan(true, 256).nsteps == 7 && MakeFusedPlan(false, 128).nsteps == 3,
               "fused plans");
 
+// At most 128 VGPRs (four waves per SIMD): two workgroups per CU.
 template <typename T, bool LEFT>
-__global__ void __launch_bounds__(64 * kNarrowWaves)
+__global__ void __launch_bounds__(64 * kNarrowWaves, 4)
     FusedBlockKernel(trsm_chunk::TrsmDesc<T> d, int R) {
   using M = gemm_detail::Mfma<T>;
   using Acc = typename M::Acc;
@@ -292,15 +293,35 @@ __global__ void __launch_bounds__(64 * kNarrowWaves)
       // left B(rows) -= T(rows, ks) X(ks, rhs), right B(rhs, cols) -= X(rhs,
       // ks) T(ks, cols); 16 solve indices per block, block rb on wave
       // rb % kNarrowWaves, all 16 right-hand sides at once.
-      const int nrb = sp.len / 16;
+      // When a wave's triangle loads are coalesced (its 16 rows contiguous),
+      // all of them are issued before the first product (one memory round
+      // trip per update); strided ones go in batches of 8, which measured
+      // faster for them. The products are the same either way.
+      constexpr int kMaxSteps = kFusedMax / 2 / 4;  // the longest update has k = 128
+      const int nrb = sp.len / 16, nsteps = sp.klen / 4;
+      const bool coalesced = (LEFT ? d.rs : d.cs) == 1;
       for (int rb = wave; rb < nrb; rb += kNarrowWaves) {
         const int i0 = sp.r0 + rb * 16;
         Acc acc = Acc{};
+        if (coalesced) {
+          T ta[kMaxSteps];
+#pragma unroll
+          for (int u = 0; u < kMaxSteps; ++u)
+            ta[u] = u < nsteps ? trsm_chunk::SolveElem(d, i0 + li, sp.k0 + 4 * u + lk) : T(0);
+#pragma unroll
+          for (int u = 0; u < kMaxSteps; ++u) {
+            if (u < nsteps) {
+              const T x = sb[sp.k0 + 4 * u + lk][li];
+              acc = LEFT ? M::Run(ta[u], x, acc) : M::Run(x, ta[u], acc);
+            }
+          }
+        } else {
 #pragma unroll 8
-        for (int kk = sp.k0; kk < sp.k0 + sp.klen; kk += 4) {
-          const T t = trsm_chunk::SolveElem(d, i0 + li, kk + lk);
-          const T x = sb[kk + lk][li];
-          acc = LEFT ? M::Run(t, x, acc) : M::Run(x, t, acc);
+          for (int kk = sp.k0; kk < sp.k0 + sp.klen; kk += 4) {
+            const T t = trsm_chunk::SolveElem(d, i0 + li, kk + lk);
+            const T x = sb[kk + lk][li];
+            acc = LEFT ? M::Run(t, x, acc) : M::Run(x, t, acc);
+          }
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
