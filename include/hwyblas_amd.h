@@ -607,12 +607,20 @@ HWYBLAS_AMD_API int hwyblas_dgesv_ir_dev(int n, int nrhs, const double* a, int l
  * workspace query, which returns n * 32 in work[0]); reported as DGEQRF. A
  * and tau may be host or device memory; work is only written (work[0]).
  * Each call synchronizes. The factorization runs on the GPU in panels of 64
- * columns with the trailing updates on the MFMA GEMM. */
+ * columns (one cooperative launch per panel when its workgroups all fit on
+ * the device; HWYBLAS_AMD_QR_PANEL=steps selects one launch per column, with
+ * the same bits), block reflectors of 128 columns, and the trailing updates
+ * on the MFMA GEMM. info = -1000 is not an argument position: a wait inside
+ * the cooperative panel kernel exceeded its bound (not expected), a message
+ * names it on stderr, the installed error handler (if any) receives
+ * ("DGEQRF", -1000), the affected tau are NaN and A holds no valid
+ * factorization. */
 HWYBLAS_AMD_API void dgeqrf_(const hwyblas_int* m, const hwyblas_int* n, double* a,
                              const hwyblas_int* lda, double* tau, double* work,
                              const hwyblas_int* lwork, hwyblas_int* info);
 /* The same on device memory (A and tau), asynchronous on `stream` (no
- * hwyBLAS counterpart as such). Returns 0 or -i (reported as DGEQRF). */
+ * hwyBLAS counterpart as such). Returns 0 or -i (reported as DGEQRF); a
+ * timed-out panel barrier (not expected) shows as NaN in tau. */
 HWYBLAS_AMD_API int hwyblas_dgeqrf_dev(int m, int n, double* a, int lda, double* tau,
                                        void* stream);
 
