@@ -21,14 +21,14 @@
 //  - a panel's T (I - V T V^T = H(j0) ... H(j0+63), xLARFT's forward
 //    columnwise T) from G = V^T V (an MFMA GEMM) and the recurrence
 //    T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i;
-//  - blocks of two panels (128 columns): the first panel's reflectors are
-//    applied to the second panel's columns, the second panel factored, and
-//    the block's T = [T1, -T1 (V1^T V2) T2; 0, T2];
+//  - blocks of four panels (256 columns): each panel's columns first get
+//    the block's earlier reflectors, then the panel is factored and the
+//    block's T grows as T = [T_old, -T_old (V_old^T V_p) T_p; 0, T_p];
 //  - the trailing columns updated as A2 := (I - V T^T V^T) A2 = H^T A2
 //    (xLARFB, side L, trans T) by three MFMA GEMMs: W = V^T A2,
-//    W2 = T^T W, A2 -= V W2, with k = 128 for a whole block. V^T V and
-//    V^T A2 have 64 or 128 rows and the panel's height as k: they split k
-//    (GemmSplitKDevice) to fill the device.
+//    W2 = T^T W, A2 -= V W2, with k = 256 for a whole block. The products
+//    with V^T have at most 256 rows and the panel's height as k: they
+//    split k (GemmSplitKDevice) to fill the device.
 // Norms are computed scale-safely: each workgroup sums (x 2^-e)^2 and
 // (x 2^-e) P(:, cc) with e the exponent of its largest |x|, and the sums are
 // combined at the largest e.
@@ -485,13 +485,13 @@ __global__ void ZeroKernel(double* p, int64_t n) {
     p[t] = 0.0;
 }
 
-constexpr int kBlock = 2 * kPanel;  // columns per block reflector (two panels)
+constexpr int kBlock = 4 * kPanel;  // columns per block reflector (four panels)
 
 struct Work {
   unsigned* sync; // [0]: barrier arrivals, [32]: failed (64 words)
   double* v;      // m x kBlock
   double* t;      // kBlock x kBlock
-  double* gm;     // 4 kPanel x kPanel: G11, G22, G12, and a product
+  double* gm;     // kBlock x kBlock: V_p^T V_p, V_old^T V_p and a product
   double* w;      // kBlock x n
   double* w2;     // kBlock x n
   double* rec;    // the panel's two record slots
@@ -614,55 +614,56 @@ void Geqrf(int64_t m, int64_t n, double* a, int64_t lda, double* tau, hipStream_
                        dim3(256), 0, s, a + (j0 + d) + (j0 + d) * lda, lda, mp, ib, d, v,
                        int64_t(mp));
   };
-  double* g11 = w.gm;
-  double* g22 = w.gm + kPanel * kPanel;
-  double* g12 = w.gm + 2 * kPanel * kPanel;
-  double* tmp = w.gm + 3 * kPanel * kPanel;
-  // Blocks of two panels: the first panel's reflectors are applied to the
-  // second panel's columns only; the block's T = [T1, -T1 (V1^T V2) T2;
-  // 0, T2] (xLARFT's T of the product H1 H2); then H^T = I - V T^T V^T is
-  // applied to the trailing columns with k = 128, half the passes over them
-  // of 64-column blocks.
+  double* gpp = w.gm;                          // V_p^T V_p (kPanel x kPanel)
+  double* gop = w.gm + kPanel * kPanel;         // V_old^T V_p (up to kBlock x kPanel)
+  double* tmp = gop + int64_t(kBlock) * kPanel;  // (V_old^T V_p) T_p
+  // Blocks of kBlock columns (kBlock / kPanel panels). Panel p's columns
+  // first get the block's earlier reflectors, H_old^T = I - V_old T_old^T
+  // V_old^T; then the panel is factored and T grows as xLARFT's T of the
+  // product: T = [T_old, -T_old (V_old^T V_p) T_p; 0, T_p]. The trailing
+  // columns get H^T = I - V T^T V^T of the whole block, k = kBlock: a
+  // pass over them per kBlock columns instead of per panel.
   for (int64_t j0 = 0; j0 < k; j0 += kBlock) {
     const int kb = int(std::min<int64_t>(kBlock, k - j0));
-    const int ib1 = std::min(kPanel, kb), ib2 = kb - ib1;
     const int mp = int(m - j0);
     const int64_t n2 = n - j0 - kb;
     double* v = w.v;
     double* t = w.t;
-    factor_panel(j0, ib1);
-    if (ib2 == 0 && n2 <= 0) continue;
+    if (kb <= kPanel && n2 <= 0) {  // one panel and nothing right of it
+      factor_panel(j0, kb);
+      continue;
+    }
     hipLaunchKernelGGL(ZeroKernel, dim3(16), dim3(256), 0, s, t, int64_t(kBlock) * kBlock);
-    explicit_v(j0, 0, ib1, mp, v);
-    GemmSplitKDevice(true, false, ib1, ib1, mp, 1.0, v, mp, v, mp, 0.0, g11, ib1, w.split,
-                     w.split_cap, s);
-    hipLaunchKernelGGL(LarftKernel, dim3(1), dim3(kPanel), 0, s, g11, ib1, tau + j0, t, kBlock);
-    HWY_HIP_CHECK(hipGetLastError());
-    if (ib2 > 0) {
-      // H1^T on the second panel's columns, then that panel.
-      double* a2 = a + j0 + (j0 + ib1) * lda;
-      GemmSplitKDevice(true, false, ib1, ib2, mp, 1.0, v, mp, a2, lda, 0.0, w.w, ib1, w.split,
-                       w.split_cap, s);
-      GemmDevice<double>(true, false, ib1, ib2, ib1, 1.0, t, kBlock, w.w, ib1, 0.0, w.w2, ib1,
-                         s);
-      GemmDevice<double>(false, false, mp, ib2, ib1, -1.0, v, mp, w.w2, ib1, 1.0, a2, lda, s);
-      factor_panel(j0 + ib1, ib2);
-      if (n2 <= 0) continue;
-      double* v2 = v + int64_t(mp) * ib1;
-      explicit_v(j0, ib1, ib2, mp, v2);
-      const int mq = mp - ib1;  // V2's rows below the zeros
-      GemmSplitKDevice(true, false, ib2, ib2, mq, 1.0, v2 + ib1, mp, v2 + ib1, mp, 0.0, g22, ib2,
+    for (int p0 = 0; p0 < kb; p0 += kPanel) {
+      const int ib = std::min(kPanel, kb - p0);
+      if (p0 > 0) {
+        // H_old^T on this panel's columns.
+        double* ap = a + j0 + (j0 + p0) * lda;
+        GemmSplitKDevice(true, false, p0, ib, mp, 1.0, v, mp, ap, lda, 0.0, w.w, p0, w.split,
+                         w.split_cap, s);
+        GemmDevice<double>(true, false, p0, ib, p0, 1.0, t, kBlock, w.w, p0, 0.0, w.w2, p0, s);
+        GemmDevice<double>(false, false, mp, ib, p0, -1.0, v, mp, w.w2, p0, 1.0, ap, lda, s);
+      }
+      factor_panel(j0 + p0, ib);
+      if (p0 + ib >= kb && n2 <= 0) break;  // no later panel or column needs V and T
+      double* vp = v + int64_t(mp) * p0;
+      explicit_v(j0, p0, ib, mp, vp);
+      const int mq = mp - p0;  // V_p's rows below its zeros
+      GemmSplitKDevice(true, false, ib, ib, mq, 1.0, vp + p0, mp, vp + p0, mp, 0.0, gpp, ib,
                        w.split, w.split_cap, s);
-      double* t2 = t + ib1 + int64_t(ib1) * kBlock;
-      hipLaunchKernelGGL(LarftKernel, dim3(1), dim3(kPanel), 0, s, g22, ib2, tau + j0 + ib1, t2,
+      double* tp = t + p0 + int64_t(p0) * kBlock;
+      hipLaunchKernelGGL(LarftKernel, dim3(1), dim3(kPanel), 0, s, gpp, ib, tau + j0 + p0, tp,
                          kBlock);
       HWY_HIP_CHECK(hipGetLastError());
-      GemmSplitKDevice(true, false, ib1, ib2, mq, 1.0, v + ib1, mp, v2 + ib1, mp, 0.0, g12, ib1,
-                       w.split, w.split_cap, s);
-      GemmDevice<double>(false, false, ib1, ib2, ib2, 1.0, g12, ib1, t2, kBlock, 0.0, tmp, ib1, s);
-      GemmDevice<double>(false, false, ib1, ib2, ib1, -1.0, t, kBlock, tmp, ib1, 0.0,
-                         t + int64_t(ib1) * kBlock, kBlock, s);
+      if (p0 > 0) {
+        GemmSplitKDevice(true, false, p0, ib, mq, 1.0, v + p0, mp, vp + p0, mp, 0.0, gop, p0,
+                         w.split, w.split_cap, s);
+        GemmDevice<double>(false, false, p0, ib, ib, 1.0, gop, p0, tp, kBlock, 0.0, tmp, p0, s);
+        GemmDevice<double>(false, false, p0, ib, p0, -1.0, t, kBlock, tmp, p0, 0.0,
+                           t + int64_t(p0) * kBlock, kBlock, s);
+      }
     }
+    if (n2 <= 0) continue;
     // H^T applied to the trailing columns.
     double* a3 = a + j0 + (j0 + kb) * lda;
     GemmSplitKDevice(true, false, kb, n2, mp, 1.0, v, mp, a3, lda, 0.0, w.w, kb, w.split,
