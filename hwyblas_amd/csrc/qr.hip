@@ -489,7 +489,7 @@ constexpr int kBlock = 4 * kPanel;  // columns per block reflector (four panels)
 
 struct Work {
   unsigned* sync; // [0]: barrier arrivals, [32]: failed (64 words)
-  double* v;      // m x kBlock
+  double* v;      // m x VCols(n)
   double* t;      // kBlock x kBlock
   double* gm;     // kBlock x kBlock: V_p^T V_p, V_old^T V_p and a product
   double* w;      // kBlock x n
@@ -501,9 +501,12 @@ struct Work {
 
 int64_t SplitCap(int64_t n) { return 8 * int64_t(kBlock) * std::max<int64_t>(n, 4096); }
 
+// Columns of the explicit V: a block's width at most.
+int64_t VCols(int64_t n) { return std::min<int64_t>(kBlock, std::max<int64_t>(n, 1)); }
+
 size_t WorkBytes(int64_t m, int64_t n) {
   const int64_t blocks = (m + kRows - 1) / kRows;
-  return 8 * (32 + size_t(m) * kBlock + 2 * kBlock * kBlock + 2 * size_t(kBlock) * n +
+  return 8 * (32 + size_t(m) * VCols(n) + 2 * kBlock * kBlock + 2 * size_t(kBlock) * n +
               2 * size_t(SlotDoubles(blocks)) + size_t(SplitCap(n))) + 256;
 }
 
@@ -514,7 +517,7 @@ Work Carve(void* base, int64_t m, int64_t n) {
   w.sync = reinterpret_cast<unsigned*>(p);
   p += 32;
   w.v = p;
-  p += m * kBlock;
+  p += m * VCols(n);
   w.t = p;
   p += kBlock * kBlock;
   w.gm = p;
