@@ -216,3 +216,24 @@ def test_refused_cooperative_launch_falls_back_to_column_launches(tmp_path):
     with np.load(out) as ref:
         for i in range(len(got)):
             assert got[i].tobytes() == ref[f"arr_{i}"].tobytes(), (PANEL_CASES[i // 2], i % 2)
+
+
+def test_tall_matrix_mixes_column_launches_and_cooperative_panels(tmp_path):
+    """m = 65600: the first panel's 257 workgroups of 256 rows exceed what
+    can be resident at once on a 256-CU device, so it takes the column
+    launches, and the later, shorter panels the cooperative launch. R equals
+    numpy's (LAPACK dgeqrf, the same signs) to rounding, and the whole
+    factorization equals the column launches' bit for bit."""
+    m, n = 65600, 200
+    a = oracle.fill_uniform(m, n, 12)
+    h, tau = gpu_qr(a)
+    r_np = np.linalg.qr(a, mode="r")
+    scale = float(np.max(np.abs(r_np)))
+    assert np.max(np.abs(np.triu(h[:n]) - r_np)) <= 64 * m * EPS * scale
+    out = tmp_path / "tall.npz"
+    _child({"HWYBLAS_AMD_QR_PANEL": "steps"},
+           f"m, n = {m}, {n}\n"
+           "h, tau = T.gpu_qr(oracle.fill_uniform(m, n, 12))\n"
+           f"np.savez({str(out)!r}, h=h, tau=tau)\n", tmp_path)
+    with np.load(out) as ref:
+        assert h.tobytes() == ref["h"].tobytes() and tau.tobytes() == ref["tau"].tobytes()
