@@ -435,48 +435,75 @@ __global__ void ExplicitVKernel(const double* p, int64_t lda, int rows, int ib, 
   }
 }
 
-// T (ib x ib, upper, leading dimension ldt) from G = V^T V and tau:
-// T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i; zeros below.
-// One workgroup of kPanel threads, thread r owning row r of T in LDS (G and
-// tau loaded into LDS first, every load in flight at once).
-__global__ void __launch_bounds__(kPanel) LarftKernel(const double* gm, int ib,
-                                                       const double* tau, double* t,
-                                                       int ldt) {
+// T (ib x ib, upper, leading dimension ldt) of the block reflector
+// I - V T V^T = H(0) ... H(ib-1) from G = V^T V and tau (xLARFT, forward,
+// columnwise; zeros below the diagonal and outside ib), by blocks: the four
+// 16 x 16 diagonal blocks by the column recurrence T(0:i, i) = -tau_i
+// T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i (sixteen threads each, one row per
+// thread), then merged pairwise as the T of a product of two block
+// reflectors, T = [T1, -T1 G12 T2; 0, T2], first into 32 x 32 blocks and
+// then into the whole. G, T and the products' scratch live in LDS.
+constexpr int kLarftThreads = 256;
+constexpr int kLarftBlock = 16;
+
+// out(r0.., c0..) = -T(r0.., r0..) (G(r0.., c0..) T(c0.., c0..)), blocks of
+// order h at (r0, c0) for `pairs` pairs side by side (pair q at r0 + 2hq).
+__device__ void LarftMerge(double (*sg)[kPanel + 1], double (*st)[kPanel + 1],
+                           double (*sw)[kPanel + 1], int h, int pairs) {
+  const int tid = threadIdx.x, outs = pairs * h * h;
+  for (int o = tid; o < outs; o += kLarftThreads) {
+    const int q = o / (h * h), e = o % (h * h), r = e % h, c = e / h;
+    const int r0 = 2 * h * q, c0 = r0 + h;
+    double s = 0.0;
+    for (int u = 0; u < h; ++u) s += sg[r0 + r][c0 + u] * st[c0 + u][c0 + c];
+    sw[r0 + r][c0 + c] = s;
+  }
+  __syncthreads();
+  for (int o = tid; o < outs; o += kLarftThreads) {
+    const int q = o / (h * h), e = o % (h * h), r = e % h, c = e / h;
+    const int r0 = 2 * h * q, c0 = r0 + h;
+    double s = 0.0;
+    for (int u = r; u < h; ++u) s += st[r0 + r][r0 + u] * sw[r0 + u][c0 + c];
+    st[r0 + r][c0 + c] = -s;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kLarftThreads) LarftKernel(const double* gm, int ib,
+                                                             const double* tau, double* t,
+                                                             int ldt) {
   __shared__ double sg[kPanel][kPanel + 1];  // sg[q][i] = G(q, i)
   __shared__ double st[kPanel][kPanel + 1];  // st[r][q] = T(r, q)
+  __shared__ double sw[kPanel][kPanel + 1];  // products
   __shared__ double stau[kPanel];
-  const int r = threadIdx.x;
-  double gcol[kPanel];
-#pragma unroll
-  for (int i = 0; i < kPanel; ++i) gcol[i] = (r < ib && i < ib) ? gm[r + int64_t(i) * ib] : 0.0;
-#pragma unroll
-  for (int i = 0; i < kPanel; ++i) {
-    sg[r][i] = gcol[i];
-    st[r][i] = 0.0;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < kPanel * kPanel; idx += kLarftThreads) {
+    const int r = idx % kPanel, c = idx / kPanel;
+    sg[r][c] = (r < ib && c < ib) ? gm[r + int64_t(c) * ib] : 0.0;
+    st[r][c] = 0.0;
   }
-  stau[r] = r < ib ? tau[r] : 0.0;
+  if (tid < kPanel) stau[tid] = tid < ib ? tau[tid] : 0.0;
   __syncthreads();
-  for (int i = 0; i < ib; ++i) {
-    // T(r, i) = -tau_i sum_{q = r}^{i - 1} T(r, q) G(q, i): four running
-    // sums (q = r + 4u + p), added pairwise at the end, so that the
-    // thread's chain of dependent adds is a quarter as long.
-    if (r < i) {
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      int q = r;
-      for (; q + 4 <= i; q += 4) {
-        s0 += st[r][q] * sg[q][i];
-        s1 += st[r][q + 1] * sg[q + 1][i];
-        s2 += st[r][q + 2] * sg[q + 2][i];
-        s3 += st[r][q + 3] * sg[q + 3][i];
+  // The diagonal blocks: thread R < 64 owns row R of its block b = R / 16.
+  const int b = tid / kLarftBlock, r = tid % kLarftBlock, R = tid;
+  for (int i = 0; i < kLarftBlock; ++i) {
+    if (tid < kPanel) {
+      const int I = b * kLarftBlock + i;
+      if (r < i) {
+        double s = 0.0;
+        for (int q = r; q < i; ++q) s += st[R][b * kLarftBlock + q] * sg[b * kLarftBlock + q][I];
+        st[R][I] = -stau[I] * s;
       }
-      for (; q < i; ++q) s0 += st[r][q] * sg[q][i];
-      st[r][i] = -stau[i] * ((s0 + s1) + (s2 + s3));
+      if (r == i) st[R][I] = stau[I];
     }
-    if (r == i) st[r][i] = stau[i];
     __syncthreads();
   }
-#pragma unroll
-  for (int j = 0; j < kPanel; ++j) t[r + j * ldt] = st[r][j];
+  LarftMerge(sg, st, sw, kLarftBlock, 2);      // 16 -> 32
+  LarftMerge(sg, st, sw, 2 * kLarftBlock, 1);  // 32 -> 64
+  for (int idx = tid; idx < kPanel * kPanel; idx += kLarftThreads) {
+    const int rr = idx % kPanel, c = idx / kPanel;
+    t[rr + int64_t(c) * ldt] = st[rr][c];
+  }
 }
 
 __global__ void ZeroKernel(double* p, int64_t n) {
@@ -655,8 +682,8 @@ void Geqrf(int64_t m, int64_t n, double* a, int64_t lda, double* tau, hipStream_
       GemmSplitKDevice(true, false, ib, ib, mq, 1.0, vp + p0, mp, vp + p0, mp, 0.0, gpp, ib,
                        w.split, w.split_cap, s);
       double* tp = t + p0 + int64_t(p0) * kBlock;
-      hipLaunchKernelGGL(LarftKernel, dim3(1), dim3(kPanel), 0, s, gpp, ib, tau + j0 + p0, tp,
-                         kBlock);
+      hipLaunchKernelGGL(LarftKernel, dim3(1), dim3(kLarftThreads), 0, s, gpp, ib,
+                         tau + j0 + p0, tp, kBlock);
       HWY_HIP_CHECK(hipGetLastError());
       if (p0 > 0) {
         GemmSplitKDevice(true, false, p0, ib, mq, 1.0, v + p0, mp, vp + p0, mp, 0.0, gop, p0,
