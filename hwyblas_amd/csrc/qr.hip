@@ -77,15 +77,75 @@ __device__ __forceinline__ Slot SlotOf(const PanelArgs& g, int k) {
   return Slot{base, base + 2 * int64_t(g.groups), base + int64_t(g.groups) * (2 + kPanel)};
 }
 
-__device__ __forceinline__ double WaveSum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// Cross-lane exchanges without the LDS crossbar (gfx950). Lanes are paired
+// across halves of groups of 2O lanes: O = 32 and 16 by v_permlane32_swap /
+// v_permlane16_swap, O = 8, 4, 2, 1 within rows of 16 by DPP mirrors (lane
+// 15 - l, 7 - l within 8, 3 - l within 4, and l ^ 1), so lane l's partner
+// always has the other value of lane bit O.
+__device__ __forceinline__ unsigned Lo32(double v) {
+  return unsigned(__double_as_longlong(v));
+}
+__device__ __forceinline__ unsigned Hi32(double v) {
+  return unsigned(__double_as_longlong(v) >> 32);
+}
+__device__ __forceinline__ double Join(unsigned lo, unsigned hi) {
+  return __longlong_as_double(
+      static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Lanes below O in their group of 2O (lane bit O clear) get a + their
+// partner's a, the others b + their partner's b: the swap moves the upper
+// lanes' a and the lower lanes' b to the partner, so the two swapped
+// registers hold exactly the pair to add.
+template <int O>
+__device__ __forceinline__ double SwapSum(double a, double b) {
+  static_assert(O == 32 || O == 16, "swap width");
+  if constexpr (O == 32) {
+    const auto lo = __builtin_amdgcn_permlane32_swap(Lo32(a), Lo32(b), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(Hi32(a), Hi32(b), false, false);
+    return Join(lo[0], hi[0]) + Join(lo[1], hi[1]);
+  } else {
+    const auto lo = __builtin_amdgcn_permlane16_swap(Lo32(a), Lo32(b), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(Hi32(a), Hi32(b), false, false);
+    return Join(lo[0], hi[0]) + Join(lo[1], hi[1]);
+  }
+}
+
+// The partner's v for O = 8, 4, 2, 1 (DPP row_mirror, row_half_mirror,
+// quad_perm [3, 2, 1, 0], quad_perm [1, 0, 3, 2]).
+template <int O>
+__device__ __forceinline__ double Partner(double v) {
+  constexpr int kCtrl = O == 8 ? 0x140 : O == 4 ? 0x141 : O == 2 ? 0x1B : 0xB1;
+  static_assert(O == 8 || O == 4 || O == 2 || O == 1, "DPP width");
+  const int lo = __builtin_amdgcn_update_dpp(0, int(Lo32(v)), kCtrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(Hi32(v)), kCtrl, 0xf, 0xf, false);
+  return Join(unsigned(lo), unsigned(hi));
+}
+
+// Sum or maximum over the wave, in every lane.
+template <bool kMax>
+__device__ __forceinline__ double WaveAll(double v) {
+  auto op = [](double x, double y) { return kMax ? fmax(x, y) : x + y; };
+  if constexpr (kMax) {
+    const auto lo = __builtin_amdgcn_permlane32_swap(Lo32(v), Lo32(v), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(Hi32(v), Hi32(v), false, false);
+    v = op(Join(lo[0], hi[0]), Join(lo[1], hi[1]));
+    const auto lo16 = __builtin_amdgcn_permlane16_swap(Lo32(v), Lo32(v), false, false);
+    const auto hi16 = __builtin_amdgcn_permlane16_swap(Hi32(v), Hi32(v), false, false);
+    v = op(Join(lo16[0], hi16[0]), Join(lo16[1], hi16[1]));
+  } else {
+    v = SwapSum<32>(v, v);
+    v = SwapSum<16>(v, v);
+  }
+  v = op(v, Partner<8>(v));
+  v = op(v, Partner<4>(v));
+  v = op(v, Partner<2>(v));
+  v = op(v, Partner<1>(v));
   return v;
 }
 
-__device__ __forceinline__ double WaveMax(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
+__device__ __forceinline__ double WaveSum(double v) { return WaveAll<false>(v); }
+__device__ __forceinline__ double WaveMax(double v) { return WaveAll<true>(v); }
 
 __device__ double Lapy2(double x, double y) {
   if (isnan(x)) return x;
@@ -128,16 +188,21 @@ __device__ void Larfg(double alpha, double xnorm, double& tau, double& scal, dou
 // the sum over the wave of value l. x[0..31] on entry are a lane's values
 // after the first halving step (offset 32) has been taken by the caller;
 // each later step keeps the half of the values whose index bit matches the
-// lane's bit, adds the partner's copy of that half and sends the other
-// half: 31 shuffles for the rest instead of 6 per value.
+// lane's bit and adds the partner's copy of that half (63 exchanges in all
+// instead of 6 per value).
 template <int O>
 __device__ __forceinline__ void ReduceScatterStep(double (&x)[32], int lane) {
-  const bool upper = (lane & O) != 0;
+  if constexpr (O == 16) {
 #pragma unroll
-  for (int j = 0; j < O; ++j) {
-    const double keep = upper ? x[j + O] : x[j];
-    const double send = upper ? x[j] : x[j + O];
-    x[j] = keep + __shfl_xor(send, O, 64);
+    for (int j = 0; j < O; ++j) x[j] = SwapSum<16>(x[j], x[j + O]);
+  } else {
+    const bool upper = (lane & O) != 0;
+#pragma unroll
+    for (int j = 0; j < O; ++j) {
+      const double keep = upper ? x[j + O] : x[j];
+      const double send = upper ? x[j] : x[j + O];
+      x[j] = keep + Partner<O>(send);
+    }
   }
   if constexpr (O > 1) ReduceScatterStep<O / 2>(x, lane);
 }
@@ -159,6 +224,7 @@ __device__ void ColumnPartials(const PanelArgs& g, int k, int ib, double xk,
   __shared__ double red[kWaves];
   __shared__ double mx_s;
   __shared__ double wsum[kWaves][kPanel];  // [w][k]: the norm sum, [w][cc > k]: the dots
+  __shared__ double srow[kPanel];          // row k, staged by its owner for a wide publish
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = blockIdx.x * kRows + threadIdx.x;
   const bool below = i > k && i < g.mp;
@@ -182,13 +248,13 @@ __device__ void ColumnPartials(const PanelArgs& g, int k, int ib, double xk,
     return cc == k ? y * y : ((cc > k && cc < ib && below) ? y * pv[cc] : 0.0);
   };
   double h[32];
-  const bool upper = (lane & 32) != 0;
 #pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    const double lo = value(j), hi = value(j + 32);
-    h[j] = (upper ? hi : lo) + __shfl_xor(upper ? lo : hi, 32, 64);
-  }
+  for (int j = 0; j < 32; ++j) h[j] = SwapSum<32>(value(j), value(j + 32));
   wsum[wave][lane] = WaveReduceScatter32(h);
+  if (i == k) {
+#pragma unroll
+    for (int cc = 0; cc < kPanel; ++cc) srow[cc] = pv[cc];
+  }
   __syncthreads();
   const int t = threadIdx.x;
   if (t >= k && t < ib) {
@@ -201,11 +267,8 @@ __device__ void ColumnPartials(const PanelArgs& g, int k, int ib, double xk,
     } else {
       PutRecord(out.dots + int64_t(blockIdx.x) * kPanel + t, d);
     }
-  }
-  if (i == k) {
-#pragma unroll
-    for (int cc = 0; cc < kPanel; ++cc)
-      if (cc >= k && cc < ib) PutRecord(out.row + cc, pv[cc]);
+    // The owner of row k is thread k of workgroup 0.
+    if (blockIdx.x == 0) PutRecord(out.row + t, srow[t]);
   }
 }
 
@@ -315,6 +378,8 @@ __device__ void ColumnStep(const PanelArgs& g, int c, int ib, double (&pv)[kPane
 #pragma unroll
     for (int p = 1; p < kDotSplit; ++p) d += wpart[p][t];
     w[t] = rowv + f * d;
+  } else if (t < kPanel) {
+    w[t] = 0.0;
   }
   __syncthreads();
   const int i = blockIdx.x * kRows + threadIdx.x;
@@ -327,15 +392,16 @@ __device__ void ColumnStep(const PanelArgs& g, int c, int ib, double (&pv)[kPane
     v = cur * scal;
     g.p[i + int64_t(c) * g.lda] = v;
   }
+  // Every entry of the row, without per-column branches (w is zero outside
+  // (c, ib), so the others keep their values; they are not read again),
+  // so that the reads of w are issued together.
   double next = 0.0;
   if (i >= c && inr) {
 #pragma unroll
     for (int k = 0; k < kPanel; ++k) {
-      if (k > c && k < ib) {
-        const double nv = pv[k] - tau * v * w[k];
-        pv[k] = nv;
-        if (k == c + 1) next = nv;
-      }
+      const double nv = pv[k] - tau * v * w[k];
+      pv[k] = nv;
+      next = k == c + 1 ? nv : next;
     }
   }
   if (c + 1 < ib) ColumnPartials(g, c + 1, ib, next, pv, SlotOf(g, c + 1));
