@@ -17,10 +17,12 @@
 //    one cooperative launch (QrPanelPersistentKernel: the rows in
 //    registers, one grid barrier per column) when all its workgroups fit on
 //    the device, else one launch per column (QrColumnKernel), bit for bit
-//    the same;
+//    the same. Cross-lane sums run on gfx950's lane swaps and DPP, not on
+//    the LDS crossbar (one wave per SIMD here: every latency is exposed);
 //  - a panel's T (I - V T V^T = H(j0) ... H(j0+63), xLARFT's forward
-//    columnwise T) from G = V^T V (an MFMA GEMM) and the recurrence
-//    T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i;
+//    columnwise T) from G = V^T V (an MFMA GEMM): the recurrence
+//    T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), T(i, i) = tau_i on 16 x 16
+//    diagonal blocks, merged pairwise as below;
 //  - blocks of four panels (256 columns): each panel's columns first get
 //    the block's earlier reflectors, then the panel is factored and the
 //    block's T grows as T = [T_old, -T_old (V_old^T V_p) T_p; 0, T_p];
