@@ -184,16 +184,25 @@ struct TileLds {
 // MFMA tiles whose first row and column of C are (r0, c0): r = alpha * acc;
 // beta == 1 adds C; other non-zero beta is fused as fma(C, beta, r); beta == 0
 // never reads C. Entries outside m x n or the requested triangle are skipped.
-template <typename T, int TM, int TN>
-__device__ __forceinline__ void StoreTile(const GemmParams<T>& p,
-                                          const typename Mfma<T>::Acc (&acc)[TM][TN], int64_t r0,
-                                          int64_t c0, int lane) {
+// One row of MFMA tiles at a time, all of its C loads issued before the first
+// is used (a skipped entry loads C[0], which exists whenever there is a tile,
+// and ignores it): a row costs one memory round trip instead of one per value.
+// The compiler cannot move a load of C above a store to C by itself (it cannot
+// prove the addresses differ), and a per-value branch on the beta form leaves
+// it waiting for every store before the next value: the body is straight-line,
+// beta == 0 in a body of its own, the two forms with C as a select.
+template <bool READ_C, typename T, int TM, int TN>
+__device__ __forceinline__ void StoreTileForm(const GemmParams<T>& p,
+                                              const typename Mfma<T>::Acc (&acc)[TM][TN],
+                                              int64_t r0, int64_t c0, int lane) {
   using M = Mfma<T>;
   const int li = lane & 15, lk = lane >> 4;
   const T alpha = p.alpha, beta = p.beta;
-  const bool beta_zero = beta == T(0), beta_one = beta == T(1);
+  const bool beta_one = beta == T(1);
 #pragma unroll
   for (int ti = 0; ti < TM; ++ti) {
+    bool ok[TN][4];
+    T cv[TN][4];
 #pragma unroll
     for (int tj = 0; tj < TN; ++tj) {
       const int64_t col = c0 + tj * 16 + li;
@@ -201,15 +210,33 @@ __device__ __forceinline__ void StoreTile(const GemmParams<T>& p,
       for (int r = 0; r < 4; ++r) {
         const int64_t row = r0 + ti * 16 + M::Row(lk, r);
         const bool in_part = p.part == 0 || (p.part == 1 ? row >= col : row <= col);
-        if (row < p.m && col < p.n && in_part) {
-          T* cij = p.c + row + col * p.ldc;
-          T v = acc[ti][tj][r] * alpha;
-          if (!beta_zero) v = beta_one ? v + *cij : __builtin_fma(*cij, beta, v);
-          *cij = v;
+        ok[tj][r] = row < p.m && col < p.n && in_part;
+        if (READ_C) cv[tj][r] = p.c[ok[tj][r] ? row + col * p.ldc : 0];
+      }
+    }
+#pragma unroll
+    for (int tj = 0; tj < TN; ++tj) {
+      const int64_t col = c0 + tj * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = r0 + ti * 16 + M::Row(lk, r);
+        T v = acc[ti][tj][r] * alpha;
+        if (READ_C) {
+          const T sum = v + cv[tj][r], fused = __builtin_fma(cv[tj][r], beta, v);
+          v = beta_one ? sum : fused;
         }
+        if (ok[tj][r]) p.c[row + col * p.ldc] = v;
       }
     }
   }
+}
+
+template <typename T, int TM, int TN>
+__device__ __forceinline__ void StoreTile(const GemmParams<T>& p,
+                                          const typename Mfma<T>::Acc (&acc)[TM][TN], int64_t r0,
+                                          int64_t c0, int lane) {
+  if (p.beta == T(0)) StoreTileForm<false>(p, acc, r0, c0, lane);
+  else StoreTileForm<true>(p, acc, r0, c0, lane);
 }
 
 // One workgroup (WAVES_M x WAVES_N waves) computes the BM x BN macro-tile of
