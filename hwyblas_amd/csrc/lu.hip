@@ -446,13 +446,11 @@ __global__ void __launch_bounds__(kPersistThreads) PanelPersistentKernel(Persist
   const int G = gridDim.x, gid = blockIdx.x;
   const int m = g.m, nb = g.nb, mn = min(m, nb);
   const int64_t lda = g.lda;
-  __shared__ double su[kPanel];                   // the pivot row
   __shared__ double sl[kPersistRows];             // multipliers of the current column
   __shared__ double srec[kPersistMaxGroups + 1][kRecord];
   __shared__ double spub[2][kPanel];             // the candidate row, row jn
   __shared__ double hv[2];
   __shared__ int hrow[2];
-  __shared__ int s_piv;
 
   int rows[kPersistRpt];
   double v[kPersistRpt][kGroupCols];
@@ -466,10 +464,14 @@ __global__ void __launch_bounds__(kPersistThreads) PanelPersistentKernel(Persist
     }
   }
   const int lbase = half * (kPersistRows / 2);  // this wave's rows in sl
+  // The pivot row of the last step: record srec[pivot_rec] (its values from
+  // srec[pivot_rec][2]); it stays in LDS until the next step's records land.
+  int pivot_rec = 0;
 
   for (int j = -1; j < mn; ++j) {
     if (j >= 0) {
       // Eliminate column j with the pivot row (row j, in place).
+      const double* su = &srec[pivot_rec][2];
       const double piv = su[j];
       if (cg == j / kGroupCols) {
 #pragma unroll
@@ -581,33 +583,28 @@ __global__ void __launch_bounds__(kPersistThreads) PanelPersistentKernel(Persist
       }
     }
     __syncthreads();
-    if (wave == 0) {
-      // G <= 128 candidates, two per lane.
-      double gv = -1.0;
-      int grow = INT_MAX;
-      for (int b = lane; b < G; b += 64) {
-        const double cv = srec[b][0];
-        const int crow = int(srec[b][1]);
-        if (Better(cv, crow, gv, grow)) {
-          gv = cv;
-          grow = crow;
-        }
-      }
-      WaveBest(gv, grow);
-      if (lane == 0) {
-        // xIAMAX picks the first row when it holds a NaN or the column is zero.
-        if (isnan(srec[G][2 + jn]) || !(gv > 0.0)) {
-          grow = jn;
-          if (gv == 0.0 && gid == 0) atomicMin(g.info, g.row0 + jn + 1);
-        }
-        if (gid == 0) g.ipiv[jn] = g.row0 + grow + 1;  // 1-based row of the whole matrix
-        s_piv = grow;
+    // Every wave reduces the G <= 128 candidates (two per lane) to the same
+    // pivot, so that no wave waits for another's.
+    double gv = -1.0;
+    int grow = INT_MAX;
+    for (int b = lane; b < G; b += 64) {
+      const double cv = srec[b][0];
+      const int crow = int(srec[b][1]);
+      if (Better(cv, crow, gv, grow)) {
+        gv = cv;
+        grow = crow;
       }
     }
-    __syncthreads();
-    const int pr = s_piv;
-    const double* prow = (pr == jn) ? &srec[G][2] : &srec[pr / kPersistRows][2];
-    if (threadIdx.x < kPanel) su[threadIdx.x] = prow[threadIdx.x];
+    WaveBest(gv, grow);
+    // xIAMAX picks the first row when it holds a NaN or the column is zero.
+    if (isnan(srec[G][2 + jn]) || !(gv > 0.0)) {
+      grow = jn;
+      if (gv == 0.0 && gid == 0 && threadIdx.x == 0) atomicMin(g.info, g.row0 + jn + 1);
+    }
+    if (gid == 0 && threadIdx.x == 0) g.ipiv[jn] = g.row0 + grow + 1;  // 1-based row
+    const int pr = grow;
+    pivot_rec = (pr == jn) ? G : pr / kPersistRows;
+    const double* prow = &srec[pivot_rec][2];
     if (pr != jn) {
 #pragma unroll
       for (int q = 0; q < kPersistRpt; ++q) {
@@ -620,7 +617,6 @@ __global__ void __launch_bounds__(kPersistThreads) PanelPersistentKernel(Persist
         }
       }
     }
-    __syncthreads();
   }
 
 #pragma unroll
