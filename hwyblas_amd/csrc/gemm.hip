@@ -133,20 +133,40 @@ int SplitKCount(int64_t m, int64_t n, int64_t k) {
   return int(std::max<int64_t>(1, std::min<int64_t>({want, k / 512, kMaxSplitK})));
 }
 
+namespace {
+
+// Products whose 128 x 128 tiles cover C exactly but give at most one per
+// CU: the DMA-fed kernel over k ranges of at least 1024, as many ranges as
+// keep every CU at two tiles or fewer (one tile alone runs a CU at about 60 %
+// of the rate of two; a third would queue behind them).
+int DmaSplitKCount(int64_t m, int64_t n, int64_t k) {
+  if (m % 128 != 0 || n % 128 != 0 || k % 16 != 0 || k < 2048) return 1;
+  const int64_t tiles = Tiles(m, n, 128, 128);
+  if (tiles > 256) return 1;
+  const int64_t want = 512 / tiles;
+  return int(std::max<int64_t>(1, std::min<int64_t>({want, k / 1024, kMaxSplitK})));
+}
+
+}  // namespace
+
 void GemmSplitKDevice(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
                       const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                       double* c, int64_t ldc, double* partials, int64_t capacity,
                       hipStream_t stream) {
-  const int splits =
-      int(std::min<int64_t>(SplitKCount(m, n, k), m * n > 0 ? capacity / (m * n) : 1));
+  const int64_t fit = m * n > 0 ? capacity / (m * n) : 1;
+  const int dma_splits = int(std::min<int64_t>(DmaSplitKCount(m, n, k), fit));
+  const int splits = dma_splits > 1 ? dma_splits
+                                    : int(std::min<int64_t>(SplitKCount(m, n, k), fit));
   if (splits <= 1) {
     GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
     return;
   }
-  // Chunks of whole 32-deep slices; each partial is written with alpha 1, beta 0.
-  const int64_t chunk = ((k + splits - 1) / splits + 31) / 32 * 32;
+  // Chunks of whole 32-deep slices (16-deep for the DMA kernel); each partial
+  // is written with alpha 1, beta 0.
+  const int64_t step = dma_splits > 1 ? 16 : 32;
+  const int64_t chunk = ((k + splits - 1) / splits + step - 1) / step * step;
   const int used = int((k + chunk - 1) / chunk);
-  const int cfg = ChooseRegularGemmConfig(8, m, n, chunk);
+  const int cfg = dma_splits > 1 ? 15 : ChooseRegularGemmConfig(8, m, n, chunk);
   GemmLaunchF64(cfg, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.0, partials, m, stream, 0, 0,
                 nullptr, chunk, m * n);
   const int64_t total = m * n;

@@ -466,7 +466,7 @@ typedef int i32x4_gemm __attribute__((ext_vector_type(4)));
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WAVES_M, int WAVES_N,
           int OCC, int STAGES>
 __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
-    __attribute__((amdgpu_waves_per_eu(OCC))) GemmDmaKernel(GemmParams<T> p) {
+    __attribute__((amdgpu_waves_per_eu(OCC))) GemmDmaKernel(GemmParams<T> prm) {
   using M = Mfma<T>;
   using Acc = typename M::Acc;
   using OA = DmaOperand<T, TA, BM, BK>;
@@ -482,7 +482,17 @@ __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
   static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 8 == 0, "tile shape");
   constexpr int kStage = OA::kChunks + OB::kChunks;  // chunks per stage
   __shared__ i32x4_gemm lds[STAGES * kStage];
-  if (p.gate && *p.gate != 0) return;  // uniform
+  if (prm.gate && *prm.gate != 0) return;  // uniform
+  // Split k: grid row y computes the k range [y ksplit, (y + 1) ksplit) into
+  // C + y c_split (the launcher guarantees whole slices in every range).
+  GemmParams<T> p = prm;
+  if (p.ksplit) {
+    const int64_t k0 = int64_t(blockIdx.y) * p.ksplit;
+    p.k = min(p.ksplit, p.k - k0);
+    p.a += TA ? k0 : k0 * p.lda;
+    p.b += TB ? k0 * p.ldb : k0;
+    p.c += int64_t(blockIdx.y) * p.c_split;
+  }
 
   int tm, tn;
   TileOf(p, &tm, &tn);
@@ -603,7 +613,11 @@ void LaunchDma(const GemmParams<T>& prm, hipStream_t stream) {
   const bool plain = prm.part == 0 && prm.krange == 0;
   const bool whole = mi == prm.m && ni == prm.n &&
                      (prm.part == 0 || (prm.m == prm.n && BM == BN));
-  const bool ok = (plain || whole) && mi > 0 && ni > 0 && prm.k > 0 && prm.k % BK == 0 &&
+  // Split k (GemmParams::ksplit) only without strips, in whole slices.
+  const bool split_ok =
+      prm.ksplit == 0 || (plain && mi == prm.m && ni == prm.n && prm.ksplit % BK == 0);
+  const bool ok = (plain || whole) && split_ok && mi > 0 && ni > 0 && prm.k > 0 &&
+                  prm.k % BK == 0 &&
                   prm.lda % E == 0 && prm.ldb % E == 0 &&
                   reinterpret_cast<uintptr_t>(prm.a) % 16 == 0 &&
                   reinterpret_cast<uintptr_t>(prm.b) % 16 == 0 &&
@@ -623,8 +637,9 @@ void LaunchDma(const GemmParams<T>& prm, hipStream_t stream) {
     q.tri_grid = q.part != 0 ? 1 : 0;
     const int64_t blocks = q.tri_grid ? int64_t(q.tiles_m) * (q.tiles_m + 1) / 2
                                       : int64_t(q.tiles_m) * q.tiles_n;
+    const unsigned splits = q.ksplit ? unsigned((q.k + q.ksplit - 1) / q.ksplit) : 1u;
     hipLaunchKernelGGL((GemmDmaKernel<T, TA, TB, BM, BN, BK, WGM, WGN, OCC, STAGES>),
-                       dim3(unsigned(blocks)), dim3(64 * WGM * WGN), 0, stream, q);
+                       dim3(unsigned(blocks), splits), dim3(64 * WGM * WGN), 0, stream, q);
     HWY_HIP_CHECK(hipGetLastError());
     if (mi < prm.m) {  // rows mi .. m - 1, every column
       GemmParams<T> r = prm;

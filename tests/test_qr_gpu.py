@@ -237,3 +237,18 @@ def test_tall_matrix_mixes_column_launches_and_cooperative_panels(tmp_path):
            f"np.savez({str(out)!r}, h=h, tau=tau)\n", tmp_path)
     with np.load(out) as ref:
         assert h.tobytes() == ref["h"].tobytes() and tau.tobytes() == ref["tau"].tobytes()
+
+
+@pytest.mark.parametrize("m,n", [(4096, 2048), (2560, 2560)])
+def test_wide_blocks_take_the_split_dma_products(m, n):
+    """Blocks whose V^T A2 product (256 rows, trailing columns a multiple
+    of 128, k = the block's height >= 2048) has at most 256 tiles of
+    128 x 128 run it on the DMA-fed kernel split over k, the partials added
+    in order. R equals numpy's (LAPACK dgeqrf, the same signs) to rounding,
+    and the reflectors' taus lie in [1, 2] (a nonzero column)."""
+    a = oracle.fill_uniform(m, n, 31)
+    h, tau = gpu_qr(a)
+    r_np = np.linalg.qr(a, mode="r")
+    scale = float(np.max(np.abs(r_np)))
+    assert np.max(np.abs(np.triu(h[:n]) - r_np)) <= 64 * m * EPS * scale
+    assert np.all((tau[:-1] >= 1.0) & (tau[:-1] <= 2.0))
