@@ -41,14 +41,23 @@ int DgemmMode() { return DgemmModeFlag().load(std::memory_order_relaxed); }
 namespace {
 
 // C = alpha op(A) op(B) + beta C on device operands, by the current mode.
+// `split`: whether a skinny product may split k (the host pipeline's column
+// blocks may not: their bits must be the whole product's).
 template <typename T>
 void GemmByMode(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
-                int64_t lda, const T* b, int64_t ldb, T beta, T* c, int64_t ldc, hipStream_t s) {
+                int64_t lda, const T* b, int64_t ldb, T beta, T* c, int64_t ldc, hipStream_t s,
+                bool split = true) {
   if constexpr (std::is_same_v<T, double>) {
     if (DgemmMode() == 1 && m >= 2048 && n >= 2048 && k >= 1024) {
       ozaki::EmulatedDgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, 0, s, nullptr);
       return;
     }
+  }
+  // Skinny products with a long k (too few tiles of C to fill the device):
+  // k split in ranges computed at once, the partials summed in order.
+  if (split && GemmSplitPays<T>(m, n, k)) {
+    GemmSplitKPublic<T>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
+    return;
   }
   GemmDevice<T>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
 }
@@ -124,6 +133,8 @@ bool GemmHostPipelined(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alph
                        int64_t lda, const T* b, int64_t ldb, T beta, T* c, int64_t ldc) {
   constexpr int64_t kMinBlock = 1024;
   if (m * n * k < (int64_t(1) << 36) || n < 4 * kMinBlock) return false;
+  // A product the device call splits over k: its column blocks would not.
+  if (GemmSplitPays<T>(m, n, k)) return false;
   static const bool enabled = [] {
     const char* e = getenv("HWYBLAS_AMD_HOST_PIPELINE");
     return !(e && strcmp(e, "0") == 0);
@@ -198,7 +209,8 @@ bool GemmHostPipelined(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alph
       wait_for(out_rec, j - 1);
       HWY_HIP_CHECK(hipStreamWaitEvent(s, ev_out[j - 2], 0));
     }
-    GemmByMode<T>(ta, tb, m, w, k, alpha, da, ra, bslot(j), tb ? w : k, beta, cslot(j), m, s);
+    GemmByMode<T>(ta, tb, m, w, k, alpha, da, ra, bslot(j), tb ? w : k, beta, cslot(j), m, s,
+                  false);
     HWY_HIP_CHECK(hipEventRecord(ev_gemm[j], s));
     gemm_rec.store(j + 1, std::memory_order_release);
   }

@@ -104,7 +104,8 @@ void ReportInternal(const char* name, int code);
 // kSlotD: a dense copy of a structured operand (SYMM, TRMM). kSlotPackA,
 // kSlotPackB, kSlotProd: the complex GEMM's real packs and real product.
 // kSlotEmulated: the emulated DGEMM's statistics, slices and residues.
-// kSlotTune: the autotuner's operand memory.
+// kSlotTune: the autotuner's operand memory. kSlotSplit: the split-k
+// partials of the GEMM entries (SplitPartials).
 enum StagingSlot : int {
   kSlotA = 0,
   kSlotB = 1,
@@ -115,7 +116,8 @@ enum StagingSlot : int {
   kSlotProd = 6,
   kSlotEmulated = 7,
   kSlotTune = 8,
-  kNumSlots = 9
+  kSlotSplit = 9,
+  kNumSlots = 10
 };
 
 struct ThreadContext {
@@ -125,6 +127,9 @@ struct ThreadContext {
   int device = -1;
   void* buf[kNumSlots] = {};
   size_t cap[kNumSlots] = {};
+  // The last work that used kSlotSplit: its event and stream.
+  hipEvent_t split_done = nullptr;
+  hipStream_t split_stream = nullptr;
 };
 
 ThreadContext& Context();
@@ -134,6 +139,12 @@ ThreadContext& Context();
 // (optional) tells whether the buffer was reallocated (its contents lost).
 void* Staging(int slot, size_t bytes, hipStream_t user_stream = nullptr,
               bool* grown = nullptr);
+// The calling thread's split-k partials (kSlotSplit) for work about to be
+// queued on `stream`: `stream` first waits for the last work that used them
+// on another stream, so calls of one thread on different streams never
+// share them at once. SplitPartialsDone(stream) marks the end of that work.
+void* SplitPartials(size_t bytes, hipStream_t stream);
+void SplitPartialsDone(hipStream_t stream);
 
 // Whether p can be dereferenced by kernels on the current device (device or
 // managed memory); host memory, including pinned memory, is staged.
@@ -209,6 +220,14 @@ void GemmDeviceKRange<float>(int krange, bool ta, bool tb, int64_t m, int64_t n,
 // GemmDevice.
 constexpr int kMaxSplitK = 32;
 int SplitKCount(int64_t m, int64_t n, int64_t k);
+// The GEMM entries' split k (FP64 and FP32): whether (m, n, k) is split, and
+// the product on the thread's split-k partials.
+template <typename T>
+bool GemmSplitPays(int64_t m, int64_t n, int64_t k);
+template <typename T>
+void GemmSplitKPublic(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
+                      int64_t lda, const T* b, int64_t ldb, T beta, T* c, int64_t ldc,
+                      hipStream_t stream);
 void GemmSplitKDevice(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
                       const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                       double* c, int64_t ldc, double* partials, int64_t capacity,

@@ -18,7 +18,7 @@ void GemmLaunchF64(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, d
 void GemmLaunchF32(int cfg, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
                    const float* a, int64_t lda, const float* b, int64_t ldb, float beta,
                    float* c, int64_t ldc, hipStream_t stream, int part = 0,
-                   int krange = 0);
+                   int krange = 0, int64_t ksplit = 0, int64_t c_split = 0);
 
 namespace {
 
@@ -107,17 +107,18 @@ namespace {
 // C = alpha sum_s P_s + beta C (hwyBLAS's Store rules on the sum), the
 // partials P_s (m x n, leading dimension m, s < splits, P_s at p + s m n)
 // added in ascending s.
-__global__ void SplitSumKernel(const double* __restrict__ p, int splits, int64_t m, int64_t n,
-                               double alpha, double beta, double* __restrict__ c, int64_t ldc) {
+template <typename T>
+__global__ void SplitSumKernel(const T* __restrict__ p, int splits, int64_t m, int64_t n,
+                               T alpha, T beta, T* __restrict__ c, int64_t ldc) {
   const int64_t total = m * n, plane = m * n;
   for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
        t += int64_t(gridDim.x) * blockDim.x) {
-    double acc = p[t];
+    T acc = p[t];
     for (int s = 1; s < splits; ++s) acc += p[t + s * plane];
     const int64_t i = t % m, j = t / m;
-    double* cij = c + i + j * ldc;
-    double v = acc * alpha;
-    if (beta != 0.0) v = beta == 1.0 ? v + *cij : __builtin_fma(*cij, beta, v);
+    T* cij = c + i + j * ldc;
+    T v = acc * alpha;
+    if (beta != T(0)) v = beta == T(1) ? v + *cij : __builtin_fma(*cij, beta, v);
     *cij = v;
   }
 }
@@ -147,33 +148,85 @@ int DmaSplitKCount(int64_t m, int64_t n, int64_t k) {
   return int(std::max<int64_t>(1, std::min<int64_t>({want, k / 1024, kMaxSplitK})));
 }
 
+// The k ranges of a split product (1: no split). `entry`: for the GEMM entry
+// points, which split only where the 32 x 32 tiles alone cannot fill the
+// device (SplitKCount), so that products of common sizes keep one k chain.
+template <typename T>
+int GemmSplitCount(int64_t m, int64_t n, int64_t k, int64_t capacity, bool* dma,
+                   bool entry = false) {
+  *dma = false;
+  if (entry && SplitKCount(m, n, k) <= 1) return 1;
+  const int64_t fit = m * n > 0 ? capacity / (m * n) : 1;
+  const int dma_splits =
+      sizeof(T) == 8 ? int(std::min<int64_t>(DmaSplitKCount(m, n, k), fit)) : 1;
+  *dma = dma_splits > 1;
+  return *dma ? dma_splits : int(std::min<int64_t>(SplitKCount(m, n, k), fit));
+}
+
+template <typename T>
+void GemmSplitK(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
+                int64_t lda, const T* b, int64_t ldb, T beta, T* c, int64_t ldc, T* partials,
+                int64_t capacity, hipStream_t stream, bool entry = false) {
+  bool dma = false;
+  const int splits = GemmSplitCount<T>(m, n, k, capacity, &dma, entry);
+  if (splits <= 1) {
+    GemmDevice<T>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+    return;
+  }
+  // Chunks of whole 32-deep slices (16-deep for the DMA kernel); each partial
+  // is written with alpha 1, beta 0.
+  const int64_t step = dma ? 16 : 32;
+  const int64_t chunk = ((k + splits - 1) / splits + step - 1) / step * step;
+  const int used = int((k + chunk - 1) / chunk);
+  const int cfg = dma ? 15 : ChooseRegularGemmConfig(int(sizeof(T)), m, n, chunk);
+  if constexpr (sizeof(T) == 8)
+    GemmLaunchF64(cfg, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.0, partials, m, stream, 0, 0,
+                  nullptr, chunk, m * n);
+  else
+    GemmLaunchF32(cfg, ta, tb, m, n, k, 1.0f, a, lda, b, ldb, 0.0f, partials, m, stream, 0, 0,
+                  chunk, m * n);
+  const int64_t total = m * n;
+  hipLaunchKernelGGL(SplitSumKernel<T>, dim3(unsigned(std::min<int64_t>((total + 255) / 256, 4096))),
+                     dim3(256), 0, stream, partials, used, m, n, alpha, beta, c, ldc);
+  HWY_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace
 
 void GemmSplitKDevice(bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
                       const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
                       double* c, int64_t ldc, double* partials, int64_t capacity,
                       hipStream_t stream) {
-  const int64_t fit = m * n > 0 ? capacity / (m * n) : 1;
-  const int dma_splits = int(std::min<int64_t>(DmaSplitKCount(m, n, k), fit));
-  const int splits = dma_splits > 1 ? dma_splits
-                                    : int(std::min<int64_t>(SplitKCount(m, n, k), fit));
-  if (splits <= 1) {
-    GemmDevice<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
-    return;
-  }
-  // Chunks of whole 32-deep slices (16-deep for the DMA kernel); each partial
-  // is written with alpha 1, beta 0.
-  const int64_t step = dma_splits > 1 ? 16 : 32;
-  const int64_t chunk = ((k + splits - 1) / splits + step - 1) / step * step;
-  const int used = int((k + chunk - 1) / chunk);
-  const int cfg = dma_splits > 1 ? 15 : ChooseRegularGemmConfig(8, m, n, chunk);
-  GemmLaunchF64(cfg, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.0, partials, m, stream, 0, 0,
-                nullptr, chunk, m * n);
-  const int64_t total = m * n;
-  hipLaunchKernelGGL(SplitSumKernel, dim3(unsigned(std::min<int64_t>((total + 255) / 256, 4096))),
-                     dim3(256), 0, stream, partials, used, m, n, alpha, beta, c, ldc);
-  HWY_HIP_CHECK(hipGetLastError());
+  GemmSplitK<double>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, partials, capacity,
+                     stream);
 }
+
+template <typename T>
+bool GemmSplitPays(int64_t m, int64_t n, int64_t k) {
+  bool dma = false;
+  return GemmSplitCount<T>(m, n, k, int64_t(kMaxSplitK) * m * n, &dma, true) > 1;
+}
+template bool GemmSplitPays<double>(int64_t, int64_t, int64_t);
+template bool GemmSplitPays<float>(int64_t, int64_t, int64_t);
+
+template <typename T>
+void GemmSplitKPublic(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
+                      int64_t lda, const T* b, int64_t ldb, T beta, T* c, int64_t ldc,
+                      hipStream_t stream) {
+  bool dma = false;
+  const int splits = GemmSplitCount<T>(m, n, k, int64_t(kMaxSplitK) * m * n, &dma, true);
+  const size_t bytes = size_t(splits) * size_t(m) * size_t(n) * sizeof(T);
+  T* partials = static_cast<T*>(SplitPartials(bytes, stream));
+  GemmSplitK<T>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, partials,
+                int64_t(splits) * m * n, stream, true);
+  SplitPartialsDone(stream);
+}
+template void GemmSplitKPublic<double>(bool, bool, int64_t, int64_t, int64_t, double,
+                                       const double*, int64_t, const double*, int64_t, double,
+                                       double*, int64_t, hipStream_t);
+template void GemmSplitKPublic<float>(bool, bool, int64_t, int64_t, int64_t, float, const float*,
+                                      int64_t, const float*, int64_t, float, float*, int64_t,
+                                      hipStream_t);
 
 namespace {
 

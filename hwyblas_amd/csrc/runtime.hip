@@ -113,6 +113,24 @@ void* Staging(int slot, size_t bytes, hipStream_t user_stream, bool* grown) {
   return c.buf[slot];
 }
 
+void* SplitPartials(size_t bytes, hipStream_t stream) {
+  ThreadContext& c = Context();
+  if (c.split_done != nullptr) {
+    // Growing frees the old buffer, which the last user may still read.
+    if (c.cap[kSlotSplit] < bytes) HWY_HIP_CHECK(hipEventSynchronize(c.split_done));
+    else if (c.split_stream != stream) HWY_HIP_CHECK(hipStreamWaitEvent(stream, c.split_done, 0));
+  }
+  return Staging(kSlotSplit, bytes, stream);
+}
+
+void SplitPartialsDone(hipStream_t stream) {
+  ThreadContext& c = Context();
+  if (c.split_done == nullptr)
+    HWY_HIP_CHECK(hipEventCreateWithFlags(&c.split_done, hipEventDisableTiming));
+  HWY_HIP_CHECK(hipEventRecord(c.split_done, stream));
+  c.split_stream = stream;
+}
+
 hipError_t Memcpy2DAsync(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
                          size_t height, hipMemcpyKind kind, hipStream_t stream) {
   if (width == 0 || height == 0) return hipSuccess;

@@ -389,3 +389,57 @@ def test_device_pow2_fill_and_scaling_match_oracle(H):
     H.lib().hwyblas_amd_scale_rows_cols(d.ptr, m, n, m + 1, drs.ptr, dcs.ptr, None)
     H.synchronize()
     assert np.array_equal(d.to_host()[:m], (host * rs[:, None]) * cs[None, :])
+
+
+# 32 ranges of the 32 x 32 tiles; 4 of the DMA-fed 128 x 128 tiles; 9 ranges,
+# the last one shorter; 5 ranges of one row.
+SPLIT_SHAPES = [(64, 64, 32768), (128, 256, 4096), (100, 37, 5000), (1, 300, 3000)]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_skinny_long_k_products_split_k(H, dtype):
+    """Products with too few tiles of C to fill the device and k >= 1024 run
+    split over k (the ranges at once, the partials added in order, then
+    hwyBLAS's Store rules): within the componentwise GEMM bound, for every
+    transposition and beta form, padding rows untouched; the host entry, the
+    device entry and a second call give the same bits."""
+    for (m, n, k), (ta, tb), beta in zip(SPLIT_SHAPES, TRANS, (0.0, 1.0, -0.5, 2.0)):
+        a, lda, b, ldb, c, ldc = gemm_operands(ta, tb, m, n, k, 3, dtype, m + n + k)
+        got = run_gemm(H, ta, tb, m, n, k, 0.75, a, lda, b, ldb, beta, c, ldc)
+        r = gemm_bound_ratio(got, ta, tb, m, n, k, 0.75, a, lda, b, ldb, beta, c, ldc)
+        assert r <= 1.0, (m, n, k, ta, tb, beta, r)
+        assert np.array_equal(got[m:], c[m:])
+        assert np.array_equal(run_gemm(H, ta, tb, m, n, k, 0.75, a, lda, b, ldb, beta, c, ldc),
+                              got)
+        da, db, dc = (H.DeviceArray.from_host(x) for x in (a, b, c))
+        assert H.gemm_dev(ta, tb, m, n, k, 0.75, da, lda, db, ldb, beta, dc, ldc,
+                          dtype=dtype) == 0
+        H.synchronize()
+        assert np.array_equal(dc.to_host(), got)
+
+
+def test_split_k_partials_shared_across_streams(H):
+    """One thread's device calls on two streams share its split-k partials:
+    the second stream waits for the first call's use of them, so both
+    results equal the synchronous calls'."""
+    m, n, k = 64, 96, 20000
+    L = H.lib()
+    s1, s2 = L.hwyblas_amd_stream_create(), L.hwyblas_amd_stream_create()
+    try:
+        want, dcs = [], []
+        for seed in (1, 2):
+            a, lda, b, ldb, c, ldc = gemm_operands("N", "T", m, n, k, 0, np.float64, seed)
+            want.append(run_gemm(H, "N", "T", m, n, k, 1.0, a, lda, b, ldb, 0.0, c, ldc))
+            dcs.append((H.DeviceArray.from_host(a), H.DeviceArray.from_host(b),
+                        H.DeviceArray.from_host(c), lda, ldb, ldc))
+        H.synchronize()
+        for (da, db, dc, lda, ldb, ldc), s in zip(dcs, (s1, s2)):
+            assert H.gemm_dev("N", "T", m, n, k, 1.0, da, lda, db, ldb, 0.0, dc, ldc,
+                              stream=s) == 0
+        L.hwyblas_amd_stream_synchronize(s1)
+        L.hwyblas_amd_stream_synchronize(s2)
+        for (_, _, dc, *_), w in zip(dcs, want):
+            assert np.array_equal(dc.to_host(), w)
+    finally:
+        L.hwyblas_amd_stream_destroy(s1)
+        L.hwyblas_amd_stream_destroy(s2)
