@@ -218,7 +218,7 @@ void GemmDeviceKRange<float>(int krange, bool ta, bool tb, int64_t m, int64_t n,
 // ascending order) + beta C. The sum's rounding differs from a single
 // chain's (it stays within the GEMM bound); deterministic. One chunk runs
 // GemmDevice.
-constexpr int kMaxSplitK = 32;
+constexpr int kMaxSplitK = 64;
 int SplitKCount(int64_t m, int64_t n, int64_t k);
 // The GEMM entries' split k (FP64 and FP32): whether (m, n, k) is split, and
 // the product on the thread's split-k partials.

@@ -127,11 +127,11 @@ __global__ void SplitSumKernel(const T* __restrict__ p, int splits, int64_t m, i
 
 int SplitKCount(int64_t m, int64_t n, int64_t k) {
   // Worth it when the 32 x 32 tiles alone cannot fill the device (about
-  // 4096 single-wave tiles in all) and each split keeps at least 512 of k.
+  // 4096 single-wave tiles in all) and each split keeps at least 256 of k.
   const int64_t tiles = Tiles(m, n, 32, 32);
   if (tiles >= 2048 || k < 1024) return 1;
   const int64_t want = (4096 + tiles - 1) / tiles;
-  return int(std::max<int64_t>(1, std::min<int64_t>({want, k / 512, kMaxSplitK})));
+  return int(std::max<int64_t>(1, std::min<int64_t>({want, k / 256, kMaxSplitK})));
 }
 
 namespace {
