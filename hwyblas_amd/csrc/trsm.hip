@@ -405,13 +405,19 @@ void DiagSolve(bool left, const TriView<T>& t, int r, T* b, int64_t ldb,
 }
 
 // B(m x n, ldb) -= op1 op2, with op1/op2 general strided operands (one of
-// rs, cs is 1) and B column-major.
+// rs, cs is 1) and B column-major. Skinny updates with a long k (few
+// right-hand sides against a large triangle) split k like the GEMM entries.
 template <typename T>
 void SubtractProduct(int64_t m, int64_t n, int64_t k, const T* d1, int64_t rs1,
                      int64_t cs1, const T* d2, int64_t rs2, int64_t cs2, T* b,
                      int64_t ldb, hipStream_t stream) {
   if (m == 0 || n == 0 || k == 0) return;
   const bool t1 = rs1 != 1, t2 = rs2 != 1;
+  if (GemmSplitPays<T>(m, n, k)) {
+    GemmSplitKPublic<T>(t1, t2, m, n, k, T(-1), d1, t1 ? rs1 : cs1, d2, t2 ? rs2 : cs2, T(1),
+                        b, ldb, stream);
+    return;
+  }
   GemmDevice<T>(t1, t2, m, n, k, T(-1), d1, t1 ? rs1 : cs1, d2, t2 ? rs2 : cs2,
                 T(1), b, ldb, stream);
 }

@@ -199,3 +199,18 @@ def test_fused_block_solve_equals_unfused_recursion(H, tmp_path):
         a, lda, b, ldb = trsm_operands(side, uplo, diag, mm, nn, np.dtype(dt).type, 7000 + 3 * i)
         r = trsm_residual_ratio(side, uplo, ta, diag, mm, nn, 0.75, a, lda, b, ldb, got[i])
         assert r <= 1.0, (side, m, n, uplo, ta, diag, dt, r)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("side,uplo,ta", [("L", "L", "N"), ("L", "U", "T"), ("R", "U", "N"),
+                                          ("R", "L", "T")])
+def test_few_right_hand_sides_split_the_updates(H, dtype, side, uplo, ta):
+    """A 4096 x 4096 triangle against 8 right-hand sides: the recursion's
+    updates (8 columns or rows of B, k up to 2048) split k; the residual
+    stays within the bound and B's padding is untouched."""
+    mm, nn = (4096, 8) if side == "L" else (8, 4096)
+    a, lda, b, ldb = trsm_operands(side, uplo, "N", mm, nn, dtype, 77)
+    x = run_trsm(H, side, uplo, ta, "N", mm, nn, 1.0, a, lda, b, ldb)
+    r = trsm_residual_ratio(side, uplo, ta, "N", mm, nn, 1.0, a, lda, b, ldb, x)
+    assert r <= 1.0, r
+    assert np.array_equal(x[mm:, :], b[mm:, :])
